@@ -1,0 +1,72 @@
+/*
+ * wukong_engine.h -- C ABI of libwkg_engine.so, the host engine above
+ * libwkg_hip.so.
+ *
+ * The engine keeps the reference GPU engine's request model and dispatch:
+ * a query is cmd_chains, 4 x int64 per pattern (start, predicate, direction,
+ * end) (core/query.hpp:68); variables are -1, -2, ... and variable -k is
+ * column k-1 (core/query.hpp:151-154); a variable is known once its column
+ * exists (core/query.hpp:141-149); step 0 starts from an index when
+ * cmd_chains[0] is a predicate/type id (core/query.hpp:120-139); every other
+ * step dispatches on (type(start), type(end)) exactly as
+ * GPU_Engine::execute_one_step does (core/gpu_engine.hpp:395-467).
+ * Intermediate tables stay on the device between steps; the last step's
+ * table is copied to the host (core/rcache.hpp:389-412).
+ *
+ * Status codes are those of wukong_gpu.h.
+ */
+#ifndef WUKONG_ENGINE_H
+#define WUKONG_ENGINE_H
+
+#include <stdint.h>
+
+#include "wukong_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct wke_engine wke_engine;
+
+/* Result of one query: row-major int32 table (malloc'd; free with wke_free). */
+typedef struct wke_result {
+  int32_t *table;
+  uint64_t rows;
+  int32_t cols;
+  int32_t status;     /* WKG_OK or the error of the failing step */
+  uint64_t expanded;  /* work count summed over the steps */
+  double latency_s;   /* submit -> host table */
+} wke_result;
+
+int wke_create(wkg_ctx *ctx, wke_engine **out);
+int wke_destroy(wke_engine *e);
+const char *wke_last_error(const wke_engine *e);
+void wke_free(void *p);
+
+/* Plain path: one query, one step at a time (GPU_Engine::execute_request,
+ * core/gpu_engine.hpp:471-526, collapsed to one server). */
+int wke_run_query(wke_engine *e, const int64_t *cmd, int ncmd, wke_result *res);
+
+/* Plain path over many queries, one after another (the reference runs one
+ * query on the GPU at a time, core/gpu.hpp:53-61). */
+int wke_run_plain(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, wke_result *res);
+
+/* Batched path: all queries advance together, one fused launch pair per
+ * round covering the current step of every unfinished query. */
+int wke_run_batched(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, wke_result *res);
+
+/* Served path: an open-loop arrival stream. Query q arrives at arrival_s[q]
+ * (seconds from the start, ascending). Arrived queries are admitted while
+ * their estimated device tables fit table_budget_bytes; admitted queries
+ * advance together one step per round (continuous batching: a query joins at
+ * the next round after it is admitted and leaves when its last step is done).
+ * Heavy queries (flag heavy[q] != 0, may be NULL) run on a second stream in
+ * their own rounds, so light rounds overlap them on the device.
+ * latency_s in each result is completion - arrival. */
+int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, const double *arrival_s,
+              const uint8_t *heavy, uint64_t table_budget_bytes, wke_result *res, double *wall_s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

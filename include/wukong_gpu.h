@@ -1,0 +1,169 @@
+/*
+ * wukong_gpu.h -- C ABI of libwkg_hip.so, the MI355X (gfx950) graph-exploration
+ * engine for one server's gstore shard.
+ *
+ * This is the drop-in boundary below the reference's GPU engine. The
+ * reference's host side (GPU_Engine, core/gpu_engine.hpp:171-467) drives
+ * RCache (core/rcache.hpp:114-962), which launches the kernels of
+ * core/gpu_hash.cu through C++ free functions (core/gpu_hash.h:18-146). Here
+ * one C call replaces one RCache handler: key-list generation, the hash probe,
+ * the edge-list fetch, the prefix sum and the result-table update all run
+ * behind it, and the result stays on the device as a table handle until the
+ * caller copies it out.
+ *
+ * Conventions
+ *  - Every entry point returns 0 (WKG_OK) or a negative WKG_E* status; none
+ *    aborts. wkg_last_error(ctx) describes the most recent failure.
+ *  - A result table is row-major int32, rows x cols, like
+ *    request_or_reply::result_table (core/query.hpp:69). Row counts are 64-bit.
+ *  - Column numbers are 0-based; the reference's variable -k is column k-1
+ *    (var2column, core/query.hpp:151-154).
+ *  - `stream` is a hipStream_t or NULL (the context's own stream). Calls that
+ *    return a row count wait for the count on that stream; the table contents
+ *    are ordered on the stream.
+ *  - A context is used by one host thread at a time (the reference runs one
+ *    GPU engine thread per GPU, core/wukong.cpp:320-325).
+ */
+#ifndef WUKONG_GPU_H
+#define WUKONG_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WKG_OK 0
+#define WKG_EINVAL (-1)    /* bad argument or unsupported pattern */
+#define WKG_ENOMEM (-2)    /* HBM budget or device allocation exhausted */
+#define WKG_EDEVICE (-3)   /* HIP runtime error */
+#define WKG_ERANGE (-4)    /* a size exceeds a format limit */
+#define WKG_ESTATE (-5)    /* call out of order (e.g. no graph loaded) */
+
+typedef struct wkg_ctx wkg_ctx;
+typedef struct wkg_table wkg_table;
+
+/* pred_meta_t (core/rdf_meta.hpp:31-54), 64 bytes per predicate/type id. */
+typedef struct wkg_pred_meta_abi {
+  uint64_t partition_sz, main_hdr_start, main_hdr_end;
+  uint64_t indrct_hdr_start, indrct_hdr_end, num_buckets;
+  uint64_t edge_start, edge_end;
+} wkg_pred_meta_abi;
+
+/* ---- context ------------------------------------------------------------- */
+
+/* Opens `device` and reserves nothing yet. table_budget_bytes caps the device
+ * memory held by result tables and step scratch (0: 75% of free HBM after the
+ * graph is loaded). Replaces the RCache constructor's fixed 20M-element
+ * buffers (core/rcache.hpp:145-165). */
+int wkg_init(int device, uint64_t table_budget_bytes, wkg_ctx **out);
+int wkg_destroy(wkg_ctx *ctx);
+const char *wkg_last_error(const wkg_ctx *ctx);
+
+/* ---- graph residency (gstore segment cache) -------------------------------- */
+
+#define WKG_LOAD_SORTED_LISTS 1u  /* caller guarantees every edge list ascends */
+
+/* Uploads a gstore image: num_slots 16-byte {key, ptr} slots (2 u64 each),
+ * num_edges u32 edges, num_metas pred_meta_t records. With
+ * graph_budget_bytes == 0 or >= the image size, the whole image becomes
+ * resident (the default on 288 GB of HBM3E). A smaller budget keeps the image
+ * in pinned host memory and loads per-predicate segments into fixed-size
+ * frames on demand, evicting least recently used predicates not needed by the
+ * current step (ShardManager, core/shard_manager.hpp:44-557). The host arrays
+ * must stay valid until wkg_destroy in the framed case. Without
+ * WKG_LOAD_SORTED_LISTS the lists are checked once on the device; unsorted
+ * lists make the known_to_known/known_to_const filters use a linear scan. */
+int wkg_load_graph(wkg_ctx *ctx, const uint64_t *slots, uint64_t num_slots,
+                   const uint32_t *edges, uint64_t num_edges,
+                   const wkg_pred_meta_abi *metas, int num_metas,
+                   uint64_t graph_budget_bytes, uint32_t flags);
+
+/* Makes predicate/type `pid` resident (ShardManager::load_predicate,
+ * core/shard_manager.hpp:473-555; check_pred_exist :351-359). A no-op when the
+ * whole image is resident. */
+int wkg_ensure_predicate(wkg_ctx *ctx, int pid);
+
+/* Graph residency statistics: bytes resident, frame loads, evictions. */
+int wkg_graph_stats(const wkg_ctx *ctx, uint64_t *resident_bytes,
+                    uint64_t *segment_loads, uint64_t *segment_evictions);
+
+/* ---- tables ------------------------------------------------------------------ */
+
+/* Host rows -> device table (the first-step upload the reference requires
+ * before a GPU handler, core/rcache.hpp:252-262). */
+int wkg_table_from_host(wkg_ctx *ctx, const int32_t *host, uint64_t rows, int cols,
+                        void *stream, wkg_table **out);
+/* Device table -> host (the last-step D2H copy, core/rcache.hpp:401-412).
+ * `host` must hold rows*cols int32. */
+int wkg_table_to_host(wkg_ctx *ctx, const wkg_table *t, int32_t *host, void *stream);
+int wkg_table_shape(const wkg_table *t, uint64_t *rows, int *cols);
+/* Device address of the table's first cell (row-major int32). */
+const int32_t *wkg_table_device_ptr(const wkg_table *t);
+int wkg_table_free(wkg_ctx *ctx, wkg_table *t);
+
+/* ---- pattern steps (one RCache handler each) -------------------------------------
+ * `expanded` (may be NULL) receives the step's work count: edges emitted for
+ * index/const/known_to_unknown, edges examined by the reference's early-exit
+ * linear scan for known_to_known/known_to_const. */
+
+/* index_to_unknown: rows = the index list of key (0, idx_id, dir).
+ * RCache::index_to_unknown, core/rcache.hpp:811-962. */
+int wkg_step_i2u(wkg_ctx *ctx, int idx_id, int dir, void *stream, wkg_table **out,
+                 uint64_t *expanded);
+
+/* const_to_unknown: rows = the edge list of key (vid, pid, dir). The reference
+ * runs this on the host (GPU_Engine::const_to_unknown,
+ * core/gpu_engine.hpp:171-189) and cannot continue on the GPU from it
+ * (core/rcache.hpp:252-262); here it is a device probe. */
+int wkg_step_c2u(wkg_ctx *ctx, int64_t vid, int pid, int dir, void *stream,
+                 wkg_table **out, uint64_t *expanded);
+
+/* known_to_unknown: for each row in order, one output row per edge of key
+ * (row[start_col], pid, dir), in edge-list order, with the neighbour appended
+ * as the last column. RCache::known_to_unknown, core/rcache.hpp:232-414. */
+int wkg_step_k2u(wkg_ctx *ctx, const wkg_table *in, int start_col, int pid, int dir,
+                 void *stream, wkg_table **out, uint64_t *expanded);
+
+/* known_to_known: keep the rows whose row[end_col] is in the edge list of
+ * (row[start_col], pid, dir). RCache::known_to_known, core/rcache.hpp:417-612. */
+int wkg_step_k2k(wkg_ctx *ctx, const wkg_table *in, int start_col, int pid, int dir,
+                 int end_col, void *stream, wkg_table **out, uint64_t *expanded);
+
+/* known_to_const: keep the rows whose edge list of (row[start_col], pid, dir)
+ * holds const_end. RCache::known_to_const, core/rcache.hpp:614-809. */
+int wkg_step_k2c(wkg_ctx *ctx, const wkg_table *in, int start_col, int pid, int dir,
+                 int64_t const_end, void *stream, wkg_table **out, uint64_t *expanded);
+
+/* ---- batched steps ------------------------------------------------------------------
+ * Many independent (table, pattern) steps in one fused launch sequence: one
+ * probe/scan pass over the rows of all inputs and one expand/compact pass over
+ * all outputs. Each descriptor is one reference handler call. The reference
+ * runs one query at a time on the GPU (GPU::query_id CAS, core/gpu.hpp:53-61);
+ * this is the primitive under batched and served execution. */
+#define WKG_STEP_I2U 0
+#define WKG_STEP_C2U 1
+#define WKG_STEP_K2U 2
+#define WKG_STEP_K2K 3
+#define WKG_STEP_K2C 4
+
+typedef struct wkg_step_desc {
+  int32_t kind;       /* WKG_STEP_* */
+  int32_t pid;        /* predicate (or index id for I2U) */
+  int32_t dir;        /* 0 = IN, 1 = OUT */
+  int32_t start_col;  /* K2*: column of the start variable */
+  int32_t end_col;    /* K2K: column of the end variable */
+  int32_t pad;
+  int64_t vid;        /* C2U: start constant; K2C: end constant */
+  const wkg_table *in; /* K2*: input table */
+} wkg_step_desc;
+
+/* Runs n steps; out[i] receives a new table, expanded[i] (may be NULL) its
+ * work count. Inputs are not modified or freed. */
+int wkg_step_batch(wkg_ctx *ctx, int n, const wkg_step_desc *steps, void *stream,
+                   wkg_table **out, uint64_t *expanded);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

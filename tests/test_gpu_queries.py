@@ -1,0 +1,105 @@
+"""Whole queries through the C++ engine on the GPU: plain, batched and served
+results against the CPU oracle and the committed golden fixtures."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.conftest import VB, random_queries
+from wukonggpu_amd import Device, GPUEngine, GStore, Triples
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def eng_small(dev_small):
+    e = GPUEngine(dev_small)
+    yield e
+    e.close()
+
+
+def _check(gs, q, res):
+    want, wex, _ = oracle.run_query(gs, q)
+    assert res.status == 0, q
+    assert res.table.shape[0] == want.shape[0], q
+    if want.size:
+        assert np.array_equal(res.table, want), q
+    assert res.expanded == wex, q
+
+
+def test_known_answer_tiny_on_gpu():
+    with open(os.path.join(GOLDEN, "tiny_kat.json")) as f:
+        kat = json.load(f)
+    t = np.array(kat["triples"], dtype=np.uint32)
+    gs = GStore.build(Triples.from_arrays(t[:, 0], t[:, 1], t[:, 2]))
+    dev = Device(0)
+    dev.load(gs)
+    eng = GPUEngine(dev)
+    for q in kat["queries"]:
+        r = eng.run_query(q["cmd"])
+        want = np.array(q["rows"], dtype=np.int32)
+        assert r.table.shape[0] == len(q["rows"]), q["name"]
+        if want.size:
+            assert np.array_equal(r.table, want), q["name"]
+        assert r.expanded == q["expanded"], q["name"]
+    eng.close()
+    dev.close()
+
+
+def test_golden_random_fixture_on_gpu():
+    z = np.load(os.path.join(GOLDEN, "random_small.npz"))
+    gs = GStore.build(Triples.from_arrays(z["s"], z["p"], z["o"]))
+    dev = Device(0)
+    dev.load(gs)
+    eng = GPUEngine(dev)
+    qo, ro = z["q_off"], z["r_off"]
+    queries = [z["queries"][qo[i]:qo[i + 1]] for i in range(len(qo) - 1)]
+    for mode in ("plain", "batched"):
+        res = eng.run_plain(queries) if mode == "plain" else eng.run_batched(queries)
+        for i, r in enumerate(res):
+            want = z["rows"][ro[i]:ro[i + 1]].reshape(-1, z["cols"][i])
+            assert r.status == 0 and r.table.shape[0] == want.shape[0], (mode, i)
+            if want.size:
+                assert np.array_equal(r.table, want), (mode, i)
+            assert r.expanded == z["expanded"][i], (mode, i)
+    eng.close()
+    dev.close()
+
+
+def test_random_queries_plain(eng_small, graph_small):
+    _, gs = graph_small
+    rng = np.random.default_rng(12)
+    for q in random_queries(rng, 300, 4, [6, 7], 120, max_steps=5):
+        _check(gs, q, eng_small.run_query(q))
+
+
+def test_random_queries_batched(eng_small, graph_small):
+    _, gs = graph_small
+    rng = np.random.default_rng(13)
+    qs = random_queries(rng, 300, 4, [6, 7], 200, max_steps=5)
+    for q, r in zip(qs, eng_small.run_batched(qs)):
+        _check(gs, q, r)
+
+
+def test_served_alone_equals_in_mix(eng_small, graph_small):
+    _, gs = graph_small
+    rng = np.random.default_rng(14)
+    qs = random_queries(rng, 300, 4, [6, 7], 150, max_steps=5)
+    arrivals = np.cumsum(rng.exponential(0.0005, size=len(qs)))
+    res, wall = eng_small.serve(qs, arrivals)
+    for q, r in zip(qs, res):
+        alone = eng_small.run_query(q)
+        assert r.status == 0 and np.array_equal(r.table, alone.table), q
+        assert r.expanded == alone.expanded
+        _check(gs, q, r)
+    assert wall > 0
+
+
+def test_engine_rejects_bad_plans(eng_small):
+    from wukonggpu_amd import DeviceError
+    for q in ([VB, 2, 1, VB + 1], [-1, 2, 1, -2], [2, 0, 1, -1, -1, 2, 1, -3], [2, 5, 1, -1]):
+        with pytest.raises(DeviceError):
+            eng_small.run_query(q)

@@ -1,0 +1,475 @@
+// libwkg_hip.so: device context, HBM pool, gstore residency and the C ABI of
+// the pattern-step engine (include/wukong_gpu.h).
+//
+// The reference's RCache (core/rcache.hpp:114-221) allocates one fixed set of
+// 20M-element buffers and one device gcache, and every handler synchronises
+// two or three times. Here tables and step scratch come from a stream-aware
+// caching pool under an explicit HBM budget, sizes are 64-bit, and a step
+// synchronises once (to learn its output size).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/wukong_gpu.h"
+#include "step_kernels.h"
+#include "wkg_internal.h"
+
+using namespace wkg;
+
+// ---------------------------------------------------------------------------
+// error helpers
+
+int wkg_ctx::fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                    \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return (ctx)->fail(WKG_EDEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                         __FILE__, __LINE__);                                                \
+  } while (0)
+
+#define LAUNCHCHK(ctx)                                                                          \
+  do {                                                                                          \
+    hipError_t e_ = hipGetLastError();                                                          \
+    if (e_ != hipSuccess)                                                                       \
+      return (ctx)->fail(WKG_EDEVICE, "kernel launch failed: %s (%s:%d)", hipGetErrorString(e_), \
+                         __FILE__, __LINE__);                                                   \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// pool
+
+namespace {
+size_t round_size(size_t n) {
+  if (n <= 4096) return 4096;
+  if (n < (size_t(2) << 20)) {
+    size_t p = 4096;
+    while (p < n) p <<= 1;
+    return p;
+  }
+  const size_t g = size_t(2) << 20;
+  return (n + g - 1) / g * g;
+}
+}  // namespace
+
+void *Pool::alloc(size_t bytes, hipStream_t s, std::string &err) {
+  std::lock_guard<std::mutex> lk(mu);
+  const size_t sz = round_size(bytes ? bytes : 1);
+  for (auto it = cached.lower_bound(sz); it != cached.end() && it->first <= sz + sz / 4 + (size_t(2) << 20);
+       ++it) {
+    if (it->second.stream != s) continue;
+    void *p = it->second.ptr;
+    live[p] = it->first;
+    live_bytes += it->first;
+    cached_bytes -= it->first;
+    cached.erase(it);
+    return p;
+  }
+  if (live_bytes + sz > budget) {
+    err = "HBM table budget exhausted: need " + std::to_string(sz) + " bytes, " +
+          std::to_string(budget - std::min<uint64_t>(budget, live_bytes)) + " of " + std::to_string(budget) +
+          " left";
+    return nullptr;
+  }
+  if (live_bytes + cached_bytes + sz > budget) trim_locked();
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, sz);
+  if (e != hipSuccess) {
+    trim_locked();
+    e = hipMalloc(&p, sz);
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    err = std::string("hipMalloc failed: ") + hipGetErrorString(e);
+    return nullptr;
+  }
+  live[p] = sz;
+  live_bytes += sz;
+  return p;
+}
+
+void Pool::release(void *p, hipStream_t s) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = live.find(p);
+  if (it == live.end()) return;
+  const size_t sz = it->second;
+  live.erase(it);
+  live_bytes -= sz;
+  cached.emplace(sz, Cached{p, s});
+  cached_bytes += sz;
+}
+
+void Pool::trim_locked() {
+  if (cached.empty()) return;
+  (void)hipDeviceSynchronize();
+  for (auto &kv : cached) (void)hipFree(kv.second.ptr);
+  cached.clear();
+  cached_bytes = 0;
+}
+
+void Pool::trim() {
+  std::lock_guard<std::mutex> lk(mu);
+  trim_locked();
+}
+
+Pool::~Pool() {
+  for (auto &kv : cached) (void)hipFree(kv.second.ptr);
+  for (auto &kv : live) (void)hipFree(kv.first);
+}
+
+// ---------------------------------------------------------------------------
+// context internals
+
+GraphView wkg_ctx::view() const {
+  GraphView g;
+  g.slots = d_slots;
+  g.edges = d_edges;
+  g.meta = d_meta;
+  g.vframes = d_vframes;
+  g.eframes = d_eframes;
+  g.vshift = vshift;
+  g.eshift = eshift;
+  g.num_metas = num_metas;
+  return g;
+}
+
+hipStream_t wkg_ctx::pick(void *s) const { return s ? (hipStream_t)s : stream; }
+
+int wkg_ctx::new_table(uint64_t rows, int cols, hipStream_t s, wkg_table **out) {
+  wkg_table *t = new (std::nothrow) wkg_table;
+  if (!t) return fail(WKG_ENOMEM, "out of host memory");
+  t->rows = rows;
+  t->cols = cols;
+  t->stream = s;
+  t->bytes = rows * (uint64_t)cols * sizeof(int32_t);
+  t->d = nullptr;
+  if (t->bytes) {
+    t->d = (int32_t *)pool->alloc(t->bytes, s, err);
+    if (!t->d) {
+      delete t;
+      return WKG_ENOMEM;
+    }
+  }
+  *out = t;
+  return WKG_OK;
+}
+
+void wkg_ctx::drop_table(wkg_table *t) {
+  if (!t) return;
+  pool->release(t->d, t->stream);
+  delete t;
+}
+
+int wkg_ctx::read_counters(const uint64_t *d, int n, hipStream_t s, uint64_t *host) {
+  HIPCHK(this, hipMemcpyAsync(h_rb, d, n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(this, hipStreamSynchronize(s));
+  memcpy(host, h_rb, n * sizeof(uint64_t));
+  return WKG_OK;
+}
+
+int wkg_ctx::check_pattern(int pid, int dir) {
+  if (!loaded) return fail(WKG_ESTATE, "no graph loaded");
+  if (pid < 1 || (uint32_t)pid >= num_metas) return fail(WKG_EINVAL, "predicate id %d outside [1, %u)", pid, num_metas);
+  if (dir != 0 && dir != 1) return fail(WKG_EINVAL, "direction %d is not IN(0) or OUT(1)", dir);
+  return WKG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// step implementations (plain path)
+
+int wkg_ctx::step_single_list(uint64_t key, int pid, hipStream_t s, wkg_table **out, uint64_t *expanded) {
+  int rc = ensure_pred(pid, s);
+  if (rc) return rc;
+  uint64_t *d_rb = (uint64_t *)pool->alloc(2 * sizeof(uint64_t), s, err);
+  if (!d_rb) return WKG_ENOMEM;
+  GraphView g = view();
+  if (framed) k_probe_one<true><<<1, kWave, 0, s>>>(g, key, (uint32_t)pid, d_rb);
+  else k_probe_one<false><<<1, kWave, 0, s>>>(g, key, (uint32_t)pid, d_rb);
+  LAUNCHCHK(this);
+  uint64_t hv[2];
+  rc = read_counters(d_rb, 2, s, hv);
+  pool->release(d_rb, s);
+  if (rc) return rc;
+  wkg_table *t;
+  if ((rc = new_table(hv[0], 1, s, &t))) return rc;
+  if (hv[0]) {
+    const uint64_t blocks = std::min<uint64_t>((hv[0] + kBlock - 1) / kBlock, 4096);
+    if (framed) k_copy_list<true><<<blocks, kBlock, 0, s>>>(g, (uint32_t)pid, hv[1], hv[0], t->d);
+    else k_copy_list<false><<<blocks, kBlock, 0, s>>>(g, (uint32_t)pid, hv[1], hv[0], t->d);
+    LAUNCHCHK(this);
+  }
+  if (expanded) *expanded = hv[0];
+  *out = t;
+  return WKG_OK;
+}
+
+static void launch_probe_scan(bool framed, bool sorted, const GraphView &g, const StepParams &p, uint32_t *ticket,
+                              hipStream_t s) {
+  const unsigned grid = (unsigned)p.ntiles;
+  if (framed) {
+    if (sorted) k_probe_scan<true, true><<<grid, kBlock, 0, s>>>(g, p, ticket);
+    else k_probe_scan<false, true><<<grid, kBlock, 0, s>>>(g, p, ticket);
+  } else {
+    if (sorted) k_probe_scan<true, false><<<grid, kBlock, 0, s>>>(g, p, ticket);
+    else k_probe_scan<false, false><<<grid, kBlock, 0, s>>>(g, p, ticket);
+  }
+}
+
+int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval,
+                        hipStream_t s, wkg_table **out, uint64_t *expanded) {
+  int rc = check_pattern(pid, dir);
+  if (rc) return rc;
+  if (!in) return fail(WKG_EINVAL, "null input table");
+  const uint64_t R = in->rows;
+  const int c = in->cols;
+  if (c < 1 || sc < 0 || sc >= c) return fail(WKG_EINVAL, "start column %d outside the %d-column table", sc, c);
+  if (mode == kModeK2K && (ec < 0 || ec >= c))
+    return fail(WKG_EINVAL, "end column %d outside the %d-column table", ec, c);
+  const int out_cols = mode == kModeK2U ? c + 1 : c;
+  if (out_cols > kBlock) return fail(WKG_ERANGE, "tables are limited to %d columns", kBlock);
+  if ((rc = ensure_pred(pid, s))) return rc;
+  if (R == 0) {  // empty input gives an empty result (core/rcache.hpp:242-250)
+    if (expanded) *expanded = 0;
+    return new_table(0, out_cols, s, out);
+  }
+  const uint64_t ntiles = (R + kTileRows - 1) / kTileRows;
+  // scratch: out_rows, examined, ticket, pad | status[ntiles] | rowinfo[R] | excl[R]
+  const uint64_t n_misc = 4;
+  const uint64_t scratch_words = n_misc + ntiles + (mode == kModeK2U ? 2 * R : 0);
+  uint64_t *scr = (uint64_t *)pool->alloc(scratch_words * 8, s, err);
+  if (!scr) return WKG_ENOMEM;
+  uint64_t *counters = scr, *status = scr + n_misc;
+  uint32_t *ticket = (uint32_t *)(scr + 2);
+  uint2 *rowinfo = (uint2 *)(status + ntiles);
+  uint64_t *excl = (uint64_t *)(rowinfo + R);
+
+  wkg_table *t = nullptr;
+  if (mode != kModeK2U) {
+    if ((rc = new_table(R, c, s, &t))) {
+      pool->release(scr, s);
+      return rc;
+    }
+  }
+  {
+    const uint64_t n = n_misc + ntiles;
+    k_clear<<<(unsigned)std::min<uint64_t>((n + kBlock - 1) / kBlock, 1024), kBlock, 0, s>>>(scr, n);
+  }
+  StepParams p{};
+  p.T = in->d;
+  p.R = R;
+  p.c = (uint32_t)c;
+  p.sc = (uint32_t)sc;
+  p.ec = (uint32_t)(ec < 0 ? 0 : ec);
+  p.pid = (uint32_t)pid;
+  p.dir = (uint32_t)dir;
+  p.target = (uint32_t)cval;
+  p.target_ok = (cval >= 0 && cval <= 0xffffffffLL) ? 1u : 0u;
+  p.mode = mode;
+  p.key = 0;
+  p.rowinfo = rowinfo;
+  p.excl = excl;
+  p.O = t ? t->d : nullptr;
+  p.out_rows = counters;
+  p.examined = counters + 1;
+  p.status = status;
+  p.ntiles = ntiles;
+  const GraphView g = view();
+  launch_probe_scan(framed, sorted || mode == kModeK2U, g, p, ticket, s);
+  hipError_t le = hipGetLastError();
+  uint64_t cnt[2] = {0, 0};
+  if (le == hipSuccess) rc = read_counters(counters, 2, s, cnt);
+  if (le != hipSuccess || rc) {
+    pool->release(scr, s);
+    drop_table(t);
+    return le != hipSuccess ? fail(WKG_EDEVICE, "probe launch failed: %s", hipGetErrorString(le)) : rc;
+  }
+  if (mode != kModeK2U) {  // filters: compacted in place
+    pool->release(scr, s);
+    t->rows = cnt[0];
+    t->bytes = t->rows * (uint64_t)c * 4;
+    if (expanded) *expanded = cnt[1];
+    *out = t;
+    return WKG_OK;
+  }
+  const uint64_t E = cnt[0];
+  if (expanded) *expanded = E;
+  if ((rc = new_table(E, out_cols, s, &t))) {
+    pool->release(scr, s);
+    return rc;
+  }
+  if (E) {
+    const uint64_t nx = (R + E + kExpandNV - 1) / kExpandNV;
+    uint64_t *part = (uint64_t *)pool->alloc((nx + 1) * 8, s, err);
+    if (!part) {
+      pool->release(scr, s);
+      drop_table(t);
+      return WKG_ENOMEM;
+    }
+    k_partition<<<(unsigned)((nx + 1 + kBlock - 1) / kBlock), kBlock, 0, s>>>(excl, R, E, nx, part);
+    ExpandParams x{};
+    x.pid = (uint32_t)pid;
+    x.T = in->d;
+    x.R = R;
+    x.c = (uint32_t)c;
+    x.rowinfo = rowinfo;
+    x.excl = excl;
+    x.part = part;
+    x.E = E;
+    x.O = t->d;
+    if (framed) k_expand<true><<<(unsigned)nx, kBlock, 0, s>>>(g, x);
+    else k_expand<false><<<(unsigned)nx, kBlock, 0, s>>>(g, x);
+    hipError_t e2 = hipGetLastError();
+    pool->release(part, s);
+    if (e2 != hipSuccess) {
+      pool->release(scr, s);
+      drop_table(t);
+      return fail(WKG_EDEVICE, "expand launch failed: %s", hipGetErrorString(e2));
+    }
+  }
+  pool->release(scr, s);
+  *out = t;
+  return WKG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+int wkg_init(int device, uint64_t table_budget_bytes, wkg_ctx **out) {
+  if (!out) return WKG_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return WKG_EDEVICE;
+  if (device < 0 || device >= n) return WKG_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return WKG_EDEVICE;
+  wkg_ctx *c = new (std::nothrow) wkg_ctx;
+  if (!c) return WKG_ENOMEM;
+  c->device = device;
+  c->budget_request = table_budget_bytes;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_rb, 4096, hipHostMallocDefault) != hipSuccess) {
+    delete c;
+    return WKG_EDEVICE;
+  }
+  size_t fr = 0, tot = 0;
+  (void)hipMemGetInfo(&fr, &tot);
+  c->pool = new Pool(table_budget_bytes ? table_budget_bytes : (uint64_t)(fr * 0.75));
+  *out = c;
+  return WKG_OK;
+}
+
+int wkg_destroy(wkg_ctx *ctx) {
+  if (!ctx) return WKG_EINVAL;
+  (void)hipSetDevice(ctx->device);
+  (void)hipDeviceSynchronize();
+  ctx->free_graph();
+  delete ctx->pool;
+  if (ctx->h_rb) (void)hipHostFree(ctx->h_rb);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return WKG_OK;
+}
+
+const char *wkg_last_error(const wkg_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int wkg_table_from_host(wkg_ctx *ctx, const int32_t *host, uint64_t rows, int cols, void *stream,
+                        wkg_table **out) {
+  if (!ctx || !out || cols < 0 || (rows && cols && !host)) return WKG_EINVAL;
+  hipStream_t s = ctx->pick(stream);
+  wkg_table *t;
+  int rc = ctx->new_table(rows, cols, s, &t);
+  if (rc) return rc;
+  if (t->bytes) {
+    hipError_t e = hipMemcpyAsync(t->d, host, t->bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      ctx->drop_table(t);
+      return ctx->fail(WKG_EDEVICE, "table upload failed: %s", hipGetErrorString(e));
+    }
+  }
+  *out = t;
+  return WKG_OK;
+}
+
+int wkg_table_to_host(wkg_ctx *ctx, const wkg_table *t, int32_t *host, void *stream) {
+  if (!ctx || !t) return WKG_EINVAL;
+  if (!t->bytes) return WKG_OK;
+  if (!host) return ctx->fail(WKG_EINVAL, "null host buffer");
+  hipStream_t s = ctx->pick(stream);
+  HIPCHK(ctx, hipMemcpyAsync(host, t->d, t->bytes, hipMemcpyDeviceToHost, s));
+  HIPCHK(ctx, hipStreamSynchronize(s));
+  return WKG_OK;
+}
+
+int wkg_table_shape(const wkg_table *t, uint64_t *rows, int *cols) {
+  if (!t) return WKG_EINVAL;
+  if (rows) *rows = t->rows;
+  if (cols) *cols = t->cols;
+  return WKG_OK;
+}
+
+const int32_t *wkg_table_device_ptr(const wkg_table *t) { return t ? t->d : nullptr; }
+
+int wkg_table_free(wkg_ctx *ctx, wkg_table *t) {
+  if (!ctx) return WKG_EINVAL;
+  ctx->drop_table(t);
+  return WKG_OK;
+}
+
+int wkg_step_i2u(wkg_ctx *ctx, int idx_id, int dir, void *stream, wkg_table **out, uint64_t *expanded) {
+  if (!ctx || !out) return WKG_EINVAL;
+  int rc = ctx->check_pattern(idx_id, dir);
+  if (rc) return rc;
+  if (!is_tpid(idx_id)) return ctx->fail(WKG_EINVAL, "index id %d is not a predicate/type id", idx_id);
+  return ctx->step_single_list(make_key(0, (uint64_t)idx_id, (uint64_t)dir), idx_id, ctx->pick(stream), out,
+                               expanded);
+}
+
+int wkg_step_c2u(wkg_ctx *ctx, int64_t vid, int pid, int dir, void *stream, wkg_table **out,
+                 uint64_t *expanded) {
+  if (!ctx || !out) return WKG_EINVAL;
+  int rc = ctx->check_pattern(pid, dir);
+  if (rc) return rc;
+  if (vid < 0 || (uint64_t)vid > kMaxVid) return ctx->fail(WKG_EINVAL, "constant %lld outside the vertex id range", (long long)vid);
+  const uint64_t key = make_key((uint64_t)vid, (uint64_t)pid, (uint64_t)dir);
+  return ctx->step_single_list(key, pid, ctx->pick(stream), out, expanded);
+}
+
+int wkg_step_k2u(wkg_ctx *ctx, const wkg_table *in, int start_col, int pid, int dir, void *stream,
+                 wkg_table **out, uint64_t *expanded) {
+  if (!ctx || !out) return WKG_EINVAL;
+  return ctx->step_known(kModeK2U, in, start_col, pid, dir, -1, 0, ctx->pick(stream), out, expanded);
+}
+
+int wkg_step_k2k(wkg_ctx *ctx, const wkg_table *in, int start_col, int pid, int dir, int end_col,
+                 void *stream, wkg_table **out, uint64_t *expanded) {
+  if (!ctx || !out) return WKG_EINVAL;
+  return ctx->step_known(kModeK2K, in, start_col, pid, dir, end_col, 0, ctx->pick(stream), out, expanded);
+}
+
+int wkg_step_k2c(wkg_ctx *ctx, const wkg_table *in, int start_col, int pid, int dir, int64_t const_end,
+                 void *stream, wkg_table **out, uint64_t *expanded) {
+  if (!ctx || !out) return WKG_EINVAL;
+  return ctx->step_known(kModeK2C, in, start_col, pid, dir, -1, const_end, ctx->pick(stream), out, expanded);
+}
+
+}  // extern "C"

@@ -1,0 +1,102 @@
+// Host-side state of libwkg_hip.so (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <list>
+#include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/wukong_gpu.h"
+#include "device.h"
+
+// Stream-aware caching allocator for tables and step scratch under an HBM
+// budget. A freed block is reused only by later work on the stream that
+// freed it (stream order makes that safe without a synchronisation); blocks
+// cached for other streams are returned to HIP when the budget needs them.
+struct Pool {
+  struct Cached {
+    void *ptr;
+    hipStream_t stream;
+  };
+  explicit Pool(uint64_t b) : budget(b) {}
+  ~Pool();
+  void *alloc(size_t bytes, hipStream_t s, std::string &err);
+  void release(void *p, hipStream_t s);
+  void trim();
+  uint64_t budget;
+  uint64_t live_bytes = 0, cached_bytes = 0, peak_bytes = 0;
+
+ private:
+  void trim_locked();
+  std::mutex mu;
+  std::multimap<size_t, Cached> cached;
+  std::unordered_map<void *, size_t> live;
+};
+
+struct wkg_table {
+  int32_t *d;
+  uint64_t rows;
+  int cols;
+  uint64_t bytes;
+  hipStream_t stream;  // stream that produced it; its memory returns there
+};
+
+// Framed residency (graph larger than its HBM budget): per-predicate
+// segments in fixed power-of-two frames with least-recently-used eviction
+// (ShardManager, core/shard_manager.hpp:44-557).
+struct Residency {
+  uint64_t vframe_buckets = 0, eframe_edges = 0;   // frame geometry
+  uint64_t n_vframes = 0, n_eframes = 0;           // physical frames in the pools
+  std::vector<uint64_t> vfree, efree;              // free physical frames
+  std::vector<uint64_t> h_vframes, h_eframes;      // logical -> physical (or ~0)
+  std::vector<uint64_t> vlen, elen;                // frames per predicate
+  std::vector<uint64_t> used_v, used_e;            // loaded frames per predicate
+  std::list<int> lru;                              // resident predicates, most recent first
+  std::vector<std::list<int>::iterator> lru_pos;
+  std::vector<char> in_lru;
+  const uint64_t *h_slots = nullptr;               // caller's image (host)
+  const uint32_t *h_edges = nullptr;
+  hipStream_t copy_stream = nullptr;
+};
+
+struct wkg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Pool *pool = nullptr;
+  uint64_t budget_request = 0;
+  uint64_t *h_rb = nullptr;  // pinned read-back words
+  std::string err;
+
+  // graph
+  bool loaded = false, framed = false, sorted = false;
+  uint4 *d_slots = nullptr;
+  uint32_t *d_edges = nullptr;
+  wkg::DevMeta *d_meta = nullptr;
+  uint64_t *d_vframes = nullptr, *d_eframes = nullptr;
+  uint32_t vshift = 0, eshift = 0;
+  uint32_t num_metas = 0;
+  uint64_t num_slots = 0, num_edges = 0, num_buckets_total = 0, num_main = 0;
+  std::vector<wkg_pred_meta_abi> metas;
+  std::vector<wkg::DevMeta> h_meta;
+  Residency *res = nullptr;
+  uint64_t resident_bytes = 0, stat_loads = 0, stat_evictions = 0;
+
+  int fail(int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
+  wkg::GraphView view() const;
+  hipStream_t pick(void *s) const;
+  int new_table(uint64_t rows, int cols, hipStream_t s, wkg_table **out);
+  void drop_table(wkg_table *t);
+  int read_counters(const uint64_t *d, int n, hipStream_t s, uint64_t *host);
+  int check_pattern(int pid, int dir);
+  int step_single_list(uint64_t key, int pid, hipStream_t s, wkg_table **out, uint64_t *expanded);
+  int step_known(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval, hipStream_t s,
+                 wkg_table **out, uint64_t *expanded);
+  // residency
+  int ensure_pred(int pid, hipStream_t s, const std::vector<int> *keep = nullptr);
+  void free_graph();
+};

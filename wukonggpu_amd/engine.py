@@ -1,0 +1,225 @@
+"""Device context, result tables and the query engine (Python side).
+
+Device / Table wrap libwkg_hip.so (include/wukong_gpu.h): one call per
+RCache handler. GPUEngine wraps libwkg_engine.so (include/wukong_engine.h),
+the C++ engine that keeps GPU_Engine::execute_one_step's dispatch
+(core/gpu_engine.hpp:395-467).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+
+class DeviceError(RuntimeError):
+    def __init__(self, rc, msg):
+        super().__init__(f"status {rc}: {msg}")
+        self.rc = rc
+
+
+def _i64p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+class Table:
+    """A device-resident row-major int32 result table."""
+
+    def __init__(self, dev, handle):
+        self.dev = dev
+        self._h = C.c_void_p(handle)
+
+    @property
+    def shape(self):
+        r, c = C.c_uint64(), C.c_int()
+        N.hip_lib().wkg_table_shape(self._h, C.byref(r), C.byref(c))
+        return int(r.value), int(c.value)
+
+    def to_host(self):
+        rows, cols = self.shape
+        out = np.empty((rows, cols), dtype=np.int32)
+        if rows and cols:
+            self.dev._check(N.hip_lib().wkg_table_to_host(self.dev._h, self._h,
+                                                          out.ctypes.data_as(C.POINTER(C.c_int32)), None))
+        return out
+
+    def free(self):
+        if self._h and self.dev._h:
+            N.hip_lib().wkg_table_free(self.dev._h, self._h)
+        self._h = None
+
+    __del__ = free
+
+
+class Device:
+    """A wkg_ctx: one GPU, one loaded gstore image, a table HBM budget."""
+
+    def __init__(self, device=0, table_budget_bytes=0):
+        L = N.hip_lib()
+        h = C.c_void_p()
+        rc = L.wkg_init(device, table_budget_bytes, C.byref(h))
+        if rc != 0:
+            raise DeviceError(rc, "wkg_init failed (no usable GPU?)")
+        self._h = h
+        self._image = None
+
+    def _check(self, rc):
+        if rc != 0:
+            raise DeviceError(rc, N.hip_lib().wkg_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            N.hip_lib().wkg_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def load(self, gstore, graph_budget_bytes=0, sorted_lists=True):
+        """Uploads a GStore image (whole-image residency unless the budget is smaller)."""
+        L = N.hip_lib()
+        self._image = gstore  # the framed mode reads the host image later
+        G = N.gstore_lib()
+        self._check(L.wkg_load_graph(self._h, G.wks_slots(gstore._h), gstore.num_slots,
+                                     G.wks_edges(gstore._h), gstore.num_edges,
+                                     gstore.metas_ptr(), gstore.num_metas, graph_budget_bytes,
+                                     1 if sorted_lists else 0))
+
+    def stats(self):
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self._check(N.hip_lib().wkg_graph_stats(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"resident_bytes": a.value, "segment_loads": b.value, "segment_evictions": c.value}
+
+    def table(self, rows):
+        """Host rows (2-D int array) -> device Table."""
+        a = np.ascontiguousarray(rows, dtype=np.int32)
+        if a.ndim != 2:
+            raise ValueError("table must be 2-D")
+        h = C.c_void_p()
+        self._check(N.hip_lib().wkg_table_from_host(self._h, a.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                     a.shape[0], a.shape[1], None, C.byref(h)))
+        return Table(self, h.value)
+
+    def _step(self, fn, *args):
+        h = C.c_void_p()
+        ex = C.c_uint64()
+        self._check(fn(self._h, *args, None, C.byref(h), C.byref(ex)))
+        return Table(self, h.value), int(ex.value)
+
+    # one call per RCache handler; each returns (Table, expanded)
+    def i2u(self, idx, d):
+        return self._step(N.hip_lib().wkg_step_i2u, idx, d)
+
+    def c2u(self, vid, pid, d):
+        return self._step(N.hip_lib().wkg_step_c2u, vid, pid, d)
+
+    def k2u(self, t, start_col, pid, d):
+        return self._step(N.hip_lib().wkg_step_k2u, t._h, start_col, pid, d)
+
+    def k2k(self, t, start_col, pid, d, end_col):
+        return self._step(N.hip_lib().wkg_step_k2k, t._h, start_col, pid, d, end_col)
+
+    def k2c(self, t, start_col, pid, d, const):
+        return self._step(N.hip_lib().wkg_step_k2c, t._h, start_col, pid, d, const)
+
+    def step_batch(self, steps):
+        """steps: list of dicts {kind, pid, dir, start_col, end_col, vid, input(Table)}."""
+        n = len(steps)
+        arr = (N.StepDesc * n)()
+        for i, s in enumerate(steps):
+            arr[i].kind = s["kind"]
+            arr[i].pid = s["pid"]
+            arr[i].dir = s["dir"]
+            arr[i].start_col = s.get("start_col", 0)
+            arr[i].end_col = s.get("end_col", 0)
+            arr[i].vid = s.get("vid", 0)
+            t = s.get("input")
+            arr[i].input = t._h.value if t is not None else None
+        outs = (C.c_void_p * n)()
+        ex = (C.c_uint64 * n)()
+        self._check(N.hip_lib().wkg_step_batch(self._h, n, arr, None, outs, ex))
+        return [Table(self, outs[i]) for i in range(n)], [int(x) for x in ex]
+
+
+def pack_queries(queries):
+    """List of cmd_chains (sequences of int) -> (flat int64 array, int64 offsets)."""
+    off = np.zeros(len(queries) + 1, dtype=np.int64)
+    for i, q in enumerate(queries):
+        off[i + 1] = off[i] + len(q)
+    flat = np.zeros(int(off[-1]), dtype=np.int64)
+    for i, q in enumerate(queries):
+        flat[off[i]:off[i + 1]] = q
+    return flat, off
+
+
+class QueryResult:
+    __slots__ = ("table", "status", "expanded", "latency_s")
+
+    def __init__(self, table, status, expanded, latency_s):
+        self.table, self.status, self.expanded, self.latency_s = table, status, expanded, latency_s
+
+
+class GPUEngine:
+    """The C++ engine: plain, batched and served query execution."""
+
+    def __init__(self, dev):
+        L = N.engine_lib()
+        h = C.c_void_p()
+        rc = L.wke_create(dev._h, C.byref(h))
+        if rc != 0:
+            raise DeviceError(rc, "wke_create failed")
+        self.dev = dev
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            N.engine_lib().wke_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def _collect(self, res, n):
+        L = N.engine_lib()
+        out = []
+        for i in range(n):
+            r = res[i]
+            if r.table and r.rows and r.cols:
+                t = np.ctypeslib.as_array(r.table, shape=(r.rows * r.cols,)).reshape(r.rows, r.cols).copy()
+                L.wke_free(C.cast(r.table, C.c_void_p))
+            else:
+                t = np.zeros((r.rows, max(r.cols, 0)), dtype=np.int32)
+            out.append(QueryResult(t, r.status, r.expanded, r.latency_s))
+        return out
+
+    def run_query(self, cmd):
+        L = N.engine_lib()
+        a = np.ascontiguousarray(cmd, dtype=np.int64)
+        res = N.Result()
+        rc = L.wke_run_query(self._h, _i64p(a), len(a), C.byref(res))
+        if rc != 0:
+            raise DeviceError(rc, L.wke_last_error(self._h).decode())
+        return self._collect([res], 1)[0]
+
+    def _many(self, fn, queries):
+        flat, off = pack_queries(queries)
+        n = len(queries)
+        res = (N.Result * max(n, 1))()
+        fn(self._h, _i64p(flat), _i64p(off), n, res)
+        return self._collect(res, n)
+
+    def run_plain(self, queries):
+        return self._many(N.engine_lib().wke_run_plain, queries)
+
+    def run_batched(self, queries):
+        return self._many(N.engine_lib().wke_run_batched, queries)
+
+    def serve(self, queries, arrival_s, heavy=None, table_budget_bytes=0):
+        L = N.engine_lib()
+        flat, off = pack_queries(queries)
+        n = len(queries)
+        arr = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        hv = np.ascontiguousarray(heavy if heavy is not None else np.zeros(n), dtype=np.uint8)
+        res = (N.Result * max(n, 1))()
+        wall = C.c_double()
+        L.wke_serve(self._h, _i64p(flat), _i64p(off), n, arr.ctypes.data_as(C.POINTER(C.c_double)),
+                    hv.ctypes.data_as(C.POINTER(C.c_uint8)), table_budget_bytes, res, C.byref(wall))
+        return self._collect(res, n), wall.value
