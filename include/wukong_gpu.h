@@ -163,6 +163,21 @@ typedef struct wkg_step_desc {
 int wkg_step_batch(wkg_ctx *ctx, int n, const wkg_step_desc *steps, void *stream,
                    wkg_table **out, uint64_t *expanded);
 
+/* ---- measurement ---------------------------------------------------------------------
+ * The reference has printf timers around asynchronous launches only
+ * (core/rcache.hpp:300-385, BREAK_DOWN). Here each kernel launch can be
+ * bracketed by HIP events on its own stream. Kernel names: "probe",
+ * "probe_filter", "scan_tiles", "finalize", "compact", "partition", "expand",
+ * "probe_one", "copy_list", "batch_probe", "batch_scan", "batch_finalize",
+ * "batch_partition", "batch_expand", "copy_calibrate".
+ * alg_bytes sums each launch's algorithmic byte count. */
+int wkg_profile_enable(wkg_ctx *ctx, int on);
+int wkg_profile_read(wkg_ctx *ctx, const char *kernel, uint64_t *launches, double *total_ms,
+                     double *alg_bytes);
+/* Device-to-device copy bandwidth of a plain 16-byte-per-lane copy kernel
+ * over `bytes` (read + written bytes per second). */
+int wkg_bandwidth_probe(wkg_ctx *ctx, uint64_t bytes, int iters, double *copy_gbps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,6 +114,9 @@ def hip_lib():
         _sig(L, "wkg_step_k2k", C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, P(vp), P(u64))
         _sig(L, "wkg_step_k2c", C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, i64, vp, P(vp), P(u64))
         _sig(L, "wkg_step_batch", C.c_int, vp, C.c_int, P(StepDesc), vp, P(vp), P(u64))
+        _sig(L, "wkg_profile_enable", C.c_int, vp, C.c_int)
+        _sig(L, "wkg_profile_read", C.c_int, vp, C.c_char_p, P(u64), P(dbl), P(dbl))
+        _sig(L, "wkg_bandwidth_probe", C.c_int, vp, u64, C.c_int, P(dbl))
         _hip = L
     return _hip
 

@@ -1,10 +1,10 @@
 // Fused batched steps: many independent pattern steps (different tables,
-// predicates, directions and step kinds) run as one probe/scan launch and
-// one expand launch. Workgroups take tiles from a shared ticket counter over
-// the concatenated tiles of all steps; the look-back scan is segmented per
-// step. This is the primitive under batched and served execution; the
-// reference serialises queries on the GPU (core/gpu.hpp:53-61) and launches
-// five kernels plus two or three synchronisations per step
+// predicates, directions and step kinds) run as one probe launch, one tile
+// scan launch, one finalize launch and one partition + expand pair. Each
+// workgroup finds its step by binary search over the steps' first tiles.
+// This is the primitive under batched and served execution; the reference
+// serialises queries on the GPU (core/gpu.hpp:53-61) and launches five
+// kernels plus two or three synchronisations per step
 // (core/rcache.hpp:232-414).
 #include <hip/hip_runtime.h>
 
@@ -20,13 +20,13 @@ using namespace wkg;
 namespace {
 
 struct BatchItem {
-  StepParams p;       // probe/scan view (pointers already offset to this step)
-  ExpandParams x;     // expand view (filled after the totals are known)
-  uint64_t pad[2];
+  StepParams p;    // probe / scan / finalize view (pointers offset to this step)
+  ExpandParams x;  // expand view (filled after the totals are known)
 };
 
+// last q in [0, n] with base[q] <= t (base ascending, base[n] = total)
 __device__ __forceinline__ uint32_t find_item(const uint64_t *base, uint32_t n, uint64_t t) {
-  uint32_t lo = 0, hi = n;  // last q with base[q] <= t
+  uint32_t lo = 0, hi = n;
   while (lo < hi) {
     const uint32_t mid = (lo + hi + 1) >> 1;
     if (base[mid] <= t) lo = mid; else hi = mid - 1;
@@ -35,19 +35,26 @@ __device__ __forceinline__ uint32_t find_item(const uint64_t *base, uint32_t n, 
 }
 
 template <bool SORTED, bool FRAMED>
-__global__ __launch_bounds__(kBlock) void k_batch_probe_scan(GraphView g, const BatchItem *items,
-                                                             const uint64_t *tile_base, uint32_t n,
-                                                             uint32_t *ticket) {
-  __shared__ uint64_t s_t;
+__global__ __launch_bounds__(kBlock) void k_batch_probe(GraphView g, const BatchItem *items,
+                                                        const uint64_t *tile_base, uint32_t n) {
   __shared__ uint32_t s_q;
-  if (threadIdx.x == 0) {
-    const uint64_t t = atomicAdd(ticket, 1u);
-    s_t = t;
-    s_q = find_item(tile_base, n, t);
-  }
+  if (threadIdx.x == 0) s_q = find_item(tile_base, n, blockIdx.x);
   __syncthreads();
   const uint32_t q = s_q;
-  probe_scan_tile<SORTED, FRAMED>(g, items[q].p, s_t - tile_base[q]);
+  probe_tile<SORTED, FRAMED>(g, items[q].p, blockIdx.x - tile_base[q]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_scan(const BatchItem *items) {
+  scan_step_tiles(items[blockIdx.x].p);
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_finalize(const BatchItem *items, const uint64_t *tile_base,
+                                                           uint32_t n) {
+  __shared__ uint32_t s_q;
+  if (threadIdx.x == 0) s_q = find_item(tile_base, n, blockIdx.x);
+  __syncthreads();
+  const uint32_t q = s_q;
+  finalize_tile(items[q].p, blockIdx.x - tile_base[q]);
 }
 
 __global__ void k_batch_partition(const BatchItem *items, const uint64_t *part_base, uint32_t n, uint64_t *part) {
@@ -55,7 +62,7 @@ __global__ void k_batch_partition(const BatchItem *items, const uint64_t *part_b
   if (k >= part_base[n]) return;
   const uint32_t q = find_item(part_base, n, k);
   const ExpandParams &x = items[q].x;
-  part[k] = merge_split(x.excl, x.R, x.E, k - part_base[q]);
+  part[k] = merge_split(x.cexcl, x.R, x.E, k - part_base[q]);
 }
 
 template <bool FRAMED>
@@ -108,6 +115,7 @@ extern "C" int wkg_step_batch(wkg_ctx *ctx, int n, const wkg_step_desc *steps, v
         outc[i] = d.kind == WKG_STEP_K2U ? c + 1 : c;
         if (outc[i] > kBlock) return ctx->fail(WKG_ERANGE, "step %d: tables are limited to %d columns", i, kBlock);
         R[i] = d.in->rows;
+        if (R[i] >= (1ull << 32)) return ctx->fail(WKG_ERANGE, "step %d: tables are limited to 2^32 - 1 rows", i);
         break;
       }
       default:
@@ -121,18 +129,22 @@ extern "C" int wkg_step_batch(wkg_ctx *ctx, int n, const wkg_step_desc *steps, v
     if (int rc = ctx->ensure_pred(p, s, &pids)) return rc;
 
   const uint64_t NR = rowb[n], NT = tileb[n];
-  // scratch: ticket | totals[2n] | tile_base[n+1] | status[NT] | rowinfo[NR] | excl[NR] | items
+  // scratch: totals[4n] | tile_base[n+1] | tile_sum, tile_nz, tile_ex [NT each] |
+  //          hits[NT * kTileRows] (16 B) | cinfo[NR] (8 B) | cexcl[NR] | items
   const uint64_t w_items = (sizeof(BatchItem) * n + 7) / 8;
-  const uint64_t words = 2 + 2 * (uint64_t)n + (n + 1) + NT + 2 * NR + w_items;
+  const uint64_t hit_words = 2 * NT * kTileRows;
+  const uint64_t words = 4 * (uint64_t)n + (n + 1) + 3 * NT + hit_words + 2 * NR + w_items;
   uint64_t *scr = (uint64_t *)ctx->pool->alloc(words * 8, s, ctx->err);
   if (!scr) return WKG_ENOMEM;
-  uint32_t *ticket = (uint32_t *)scr;
-  uint64_t *totals = scr + 2;
-  uint64_t *d_tileb = totals + 2 * n;
-  uint64_t *status = d_tileb + (n + 1);
-  uint2 *rowinfo = (uint2 *)(status + NT);
-  uint64_t *excl = (uint64_t *)(rowinfo + NR);
-  BatchItem *d_items = (BatchItem *)(excl + NR);
+  uint64_t *totals = scr;
+  uint64_t *d_tileb = totals + 4 * n;
+  uint64_t *tsum = d_tileb + (n + 1);
+  uint64_t *tnz = tsum + NT;
+  uint64_t *tex = tnz + NT;
+  uint4 *hits = (uint4 *)(tex + NT);
+  uint2 *cinfo = (uint2 *)((uint64_t *)hits + hit_words);
+  uint64_t *cexcl = (uint64_t *)(cinfo + NR);
+  BatchItem *d_items = (BatchItem *)(cexcl + NR);
   auto bail = [&](int rc) {
     ctx->pool->release(scr, s);
     for (int i = 0; i < n; i++) {
@@ -146,15 +158,18 @@ extern "C" int wkg_step_batch(wkg_ctx *ctx, int n, const wkg_step_desc *steps, v
   for (int i = 0; i < n; i++) {
     const wkg_step_desc &d = steps[i];
     StepParams &p = items[i].p;
+    p = StepParams{};
     p.pid = (uint32_t)d.pid;
     p.dir = (uint32_t)d.dir;
     p.mode = mode[i];
     p.R = R[i];
-    p.rowinfo = rowinfo + rowb[i];
-    p.excl = excl + rowb[i];
-    p.out_rows = totals + 2 * i;
-    p.examined = totals + 2 * i + 1;
-    p.status = status + tileb[i];
+    p.hits = hits + tileb[i] * kTileRows;
+    p.cinfo = cinfo + rowb[i];
+    p.cexcl = cexcl + rowb[i];
+    p.tile_sum = tsum + tileb[i];
+    p.tile_nz = tnz + tileb[i];
+    p.tile_ex = tex + tileb[i];
+    p.totals = totals + 4 * i;
     p.ntiles = tileb[i + 1] - tileb[i];
     if (d.kind == WKG_STEP_I2U || d.kind == WKG_STEP_C2U) {
       p.key = d.kind == WKG_STEP_I2U ? make_key(0, (uint64_t)d.pid, (uint64_t)d.dir)
@@ -178,25 +193,31 @@ extern "C" int wkg_step_batch(wkg_ctx *ctx, int n, const wkg_step_desc *steps, v
   hipError_t e = hipMemcpyAsync(d_tileb, tileb.data(), (n + 1) * 8, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(d_items, items.data(), sizeof(BatchItem) * n, hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return bail(ctx->fail(WKG_EDEVICE, "descriptor upload failed: %s", hipGetErrorString(e)));
-  {
-    k_clear<<<1, kBlock, 0, s>>>(scr, 2 + 2 * (uint64_t)n);
-    if (NT) k_clear<<<(unsigned)std::min<uint64_t>((NT + kBlock - 1) / kBlock, 1024), kBlock, 0, s>>>(status, NT);
-  }
   const GraphView g = ctx->view();
   if (NT) {
+    double pb = 0;
+    for (int i = 0; i < n; i++) pb += (double)R[i] * (4.0 + (mode[i] == kModeK2K ? 4.0 : 0.0) + 128.0);
+    ProfScope ps(ctx, s, "batch_probe", pb);
     if (ctx->framed) {
-      if (ctx->sorted) k_batch_probe_scan<true, true><<<(unsigned)NT, kBlock, 0, s>>>(g, d_items, d_tileb, n, ticket);
-      else k_batch_probe_scan<false, true><<<(unsigned)NT, kBlock, 0, s>>>(g, d_items, d_tileb, n, ticket);
+      if (ctx->sorted) k_batch_probe<true, true><<<(unsigned)NT, kBlock, 0, s>>>(g, d_items, d_tileb, n);
+      else k_batch_probe<false, true><<<(unsigned)NT, kBlock, 0, s>>>(g, d_items, d_tileb, n);
     } else {
-      if (ctx->sorted) k_batch_probe_scan<true, false><<<(unsigned)NT, kBlock, 0, s>>>(g, d_items, d_tileb, n, ticket);
-      else k_batch_probe_scan<false, false><<<(unsigned)NT, kBlock, 0, s>>>(g, d_items, d_tileb, n, ticket);
+      if (ctx->sorted) k_batch_probe<true, false><<<(unsigned)NT, kBlock, 0, s>>>(g, d_items, d_tileb, n);
+      else k_batch_probe<false, false><<<(unsigned)NT, kBlock, 0, s>>>(g, d_items, d_tileb, n);
     }
+  }
+  {
+    ProfScope ps(ctx, s, "batch_scan", 24.0 * (double)NT);
+    k_batch_scan<<<(unsigned)n, kBlock, 0, s>>>(d_items);
+  }
+  if (NT) {
+    ProfScope ps(ctx, s, "batch_finalize", 0.0);
+    k_batch_finalize<<<(unsigned)NT, kBlock, 0, s>>>(d_items, d_tileb, n);
   }
   if ((e = hipGetLastError()) != hipSuccess)
     return bail(ctx->fail(WKG_EDEVICE, "batched probe launch failed: %s", hipGetErrorString(e)));
-  std::vector<uint64_t> tot(2 * (size_t)n);
+  std::vector<uint64_t> tot(4 * (size_t)n);
   {
-    // totals can exceed the pinned read-back page; copy through pageable memory
     e = hipMemcpyAsync(tot.data(), totals, tot.size() * 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return bail(ctx->fail(WKG_EDEVICE, "totals read-back failed: %s", hipGetErrorString(e)));
@@ -204,30 +225,34 @@ extern "C" int wkg_step_batch(wkg_ctx *ctx, int n, const wkg_step_desc *steps, v
 
   // ---- outputs and the expand pass --------------------------------------------------
   std::vector<uint64_t> partb(n + 1, 0), xtileb(n + 1, 0);
+  double fin_bytes = 0;
   for (int i = 0; i < n; i++) {
-    const uint64_t E = tot[2 * i];
+    const uint64_t E = tot[4 * i], nz = tot[4 * i + 2];
     uint64_t nx = 0;
     if (mode[i] == kModeK2U) {
+      fin_bytes += 32.0 * nz;
       if (int rc = ctx->new_table(E, outc[i], s, &out[i])) return bail(rc);
-      if (E) nx = (R[i] + E + kExpandNV - 1) / kExpandNV;
+      if (E) nx = (nz + E + kExpandNV - 1) / kExpandNV;
       ExpandParams &x = items[i].x;
       x.pid = items[i].p.pid;
       x.T = items[i].p.T;
-      x.R = R[i];
       x.c = items[i].p.c;
-      x.rowinfo = items[i].p.rowinfo;
-      x.excl = items[i].p.excl;
+      x.R = nz;
+      x.cinfo = items[i].p.cinfo;
+      x.cexcl = items[i].p.cexcl;
       x.E = E;
       x.O = out[i]->d;
       if (expanded) expanded[i] = E;
     } else {
+      fin_bytes += 16.0 * nz + 8.0 * nz * outc[i];
       out[i]->rows = E;
       out[i]->bytes = E * (uint64_t)outc[i] * 4;
-      if (expanded) expanded[i] = tot[2 * i + 1];
+      if (expanded) expanded[i] = tot[4 * i + 1];
     }
     partb[i + 1] = partb[i] + (nx ? nx + 1 : 0);
     xtileb[i + 1] = xtileb[i] + nx;
   }
+  ctx->prof_add_bytes("batch_finalize", fin_bytes);
   const uint64_t NX = xtileb[n], NP = partb[n];
   if (NX) {
     uint64_t *xs = (uint64_t *)ctx->pool->alloc((NP + 2 * (n + 1)) * 8, s, ctx->err);
@@ -238,7 +263,17 @@ extern "C" int wkg_step_batch(wkg_ctx *ctx, int n, const wkg_step_desc *steps, v
     if (e == hipSuccess) e = hipMemcpyAsync(d_partb, partb.data(), (n + 1) * 8, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(d_xtileb, xtileb.data(), (n + 1) * 8, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) {
-      k_batch_partition<<<(unsigned)((NP + kBlock - 1) / kBlock), kBlock, 0, s>>>(d_items, d_partb, n, d_part);
+      {
+        ProfScope ps(ctx, s, "batch_partition", 8.0 * (double)NP);
+        k_batch_partition<<<(unsigned)((NP + kBlock - 1) / kBlock), kBlock, 0, s>>>(d_items, d_partb, n, d_part);
+      }
+      double xb = 0;
+      for (int i = 0; i < n; i++) {
+        const ExpandParams &x = items[i].x;
+        if (mode[i] == kModeK2U && x.E)
+          xb += 4.0 * x.E + 4.0 * x.E * (x.c + 1) + 4.0 * x.R * x.c + 16.0 * x.R;
+      }
+      ProfScope ps(ctx, s, "batch_expand", xb);
       if (ctx->framed) k_batch_expand<true><<<(unsigned)NX, kBlock, 0, s>>>(g, d_items, d_xtileb, n);
       else k_batch_expand<false><<<(unsigned)NX, kBlock, 0, s>>>(g, d_items, d_xtileb, n);
       e = hipGetLastError();

@@ -23,9 +23,9 @@
 namespace wkg {
 
 constexpr int kWave = 64;
-constexpr int kBlock = 256;         // threads per workgroup for every step kernel
-constexpr int kTileRows = kBlock;   // probe/scan tile: one row per thread
-constexpr int kExpandVT = 8;        // merged items per thread in the expand
+constexpr int kBlock = 256;            // threads per workgroup for every step kernel
+constexpr int kTileRows = 8 * kBlock;  // probe tile: 8 rows per thread
+constexpr int kExpandVT = 8;           // merged items per thread in the expand
 constexpr int kExpandNV = kBlock * kExpandVT;
 constexpr uint32_t kMaxHops = 1u << 20;  // chain-walk guard (images are validated)
 
@@ -77,68 +77,62 @@ __device__ __forceinline__ uint32_t edge_at(const GraphView &g, uint64_t edge_st
   return g.edges[g.eframes[eframe_base + (local >> g.eshift)] + (local & mask)];
 }
 
-// Cluster-chaining probe, 8 lanes per key: each lane of a group loads one
-// 16-byte slot of the bucket (one 128-byte coalesced read per bucket), a
-// wave ballot finds the matching slot, and lane 7's link moves the group to
-// the next bucket (get_vertex_local, core/gstore.hpp:389-409; device form
-// core/gpu_hash.cu:181-234). Every lane of the wave must call this; all 8
-// lanes of a group carry the same key. key == 0 means "no key" (a miss).
-// Returns, in every lane of the group: deg (0 on a miss) and the list's
-// offset relative to its predicate's edge_start.
-template <int U, bool FRAMED>
-__device__ __forceinline__ void group_probe(const GraphView &g, const uint64_t (&key)[U],
-                                            const uint32_t (&pid)[U], uint32_t (&deg)[U],
-                                            uint32_t (&off)[U], uint32_t *hops_out = nullptr) {
+// Cluster-chaining probe of 64 consecutive rows, one per lane, all under one
+// predicate (get_vertex_local, core/gstore.hpp:389-409; device form
+// core/gpu_hash.cu:181-234). Each lane hashes its own key once. A run of
+// equal keys in consecutive lanes is probed once, by its first lane. The
+// probes then run 8 lanes per key: group g walks the keys of lanes g*8..g*8+7
+// in turn, each lane of the group loading one 16-byte slot of the bucket (one
+// coalesced 128-byte read), a ballot finding the matching slot and lane 7's
+// link moving to the next bucket; the 8 walks of a group are interleaved so 8
+// bucket reads are in flight per lane. Every lane of the wave must call this.
+// key == 0 means "no key". Returns the lane's degree (0 on a miss) and list
+// offset relative to the predicate's edge_start.
+template <bool FRAMED>
+__device__ __forceinline__ void wave_probe(const GraphView &g, const DevMeta &m, uint64_t key, uint32_t &deg_out,
+                                           uint32_t &off_out) {
   const uint32_t lane = lane_id(), l8 = lane & 7, gbase = lane & ~7u;
-  uint64_t bkt[U];
-  bool act[U];
-  uint64_t istart[U], psz[U], vfb[U], estart[U];
-  uint32_t hops = 0;
+  const bool valid = key != 0 && m.psz != 0;
+  const uint64_t prev = shfl64(key, (lane - 1) & 63);
+  const bool lead = valid && (lane == 0 || key != prev);
+  const uint64_t leaders = wave_ballot(lead);
+  uint64_t b0 = 0;
+  if (lead) {
+    const uint64_t local = fastmod(mix64(key), m.psz, m.magic);
+    b0 = FRAMED ? bucket_phys<true>(g, m, local) : m.main_start + local;
+  }
+  uint64_t keyu[8], bkt[8];
+  bool act[8];
+  uint32_t deg[8], off[8];
 #pragma unroll
-  for (int u = 0; u < U; u++) {
+  for (int u = 0; u < 8; u++) {
+    keyu[u] = shfl64(key, gbase + u);
+    bkt[u] = shfl64(b0, gbase + u);
+    act[u] = (leaders >> (gbase + u)) & 1;
     deg[u] = 0;
     off[u] = 0;
-    act[u] = key[u] != 0 && pid[u] < g.num_metas;
-    bkt[u] = 0;
-    if (act[u]) {
-      const DevMeta &m = g.meta[pid[u]];
-      psz[u] = m.psz;
-      istart[u] = m.indrct_start;
-      vfb[u] = m.vframe_base;
-      estart[u] = m.edge_start;
-      if (psz[u] == 0) {
-        act[u] = false;
-      } else {
-        uint64_t local = fastmod(mix64(key[u]), m.psz, m.magic);
-        bkt[u] = FRAMED ? bucket_phys<true>(g, m, local) : m.main_start + local;
-      }
-    }
   }
-  while (wave_ballot([&] {
-    bool a = false;
+  uint32_t hops = 0;
+  while (wave_ballot(act[0] | act[1] | act[2] | act[3] | act[4] | act[5] | act[6] | act[7])) {
+    uint4 s[8];
 #pragma unroll
-    for (int u = 0; u < U; u++) a |= act[u];
-    return a;
-  }())) {
-    uint4 s[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
+    for (int u = 0; u < 8; u++) {
       s[u] = make_uint4(0, 0, 0, 0);
       if (act[u]) s[u] = g.slots[bkt[u] * kAssoc + l8];
     }
 #pragma unroll
-    for (int u = 0; u < U; u++) {
+    for (int u = 0; u < 8; u++) {
       const uint64_t k = ((uint64_t)s[u].y << 32) | s[u].x;
-      const bool hit = act[u] && l8 < 7 && k == key[u];
+      const bool hit = act[u] && l8 < 7 && k == keyu[u];
       const uint32_t gb = (uint32_t)(wave_ballot(hit) >> gbase) & 0xffu;
       const uint32_t src = gbase + (gb ? (uint32_t)__builtin_ctz(gb) : 7u);
       const uint32_t plo = shfl32(s[u].z, src), phi = shfl32(s[u].w, src);
       const uint32_t klo = shfl32(s[u].x, src), khi = shfl32(s[u].y, src);
       if (act[u]) {
-        const uint64_t v = ((uint64_t)phi << 32) | plo;
         if (gb) {
+          const uint64_t v = ((uint64_t)phi << 32) | plo;
           deg[u] = (uint32_t)ptr_size(v);
-          off[u] = (uint32_t)(ptr_off(v) - estart[u]);
+          off[u] = (uint32_t)(ptr_off(v) - m.edge_start);
           act[u] = false;
         } else {
           const uint64_t link = ((uint64_t)khi << 32) | klo;
@@ -146,19 +140,26 @@ __device__ __forceinline__ void group_probe(const GraphView &g, const uint64_t (
             act[u] = false;
           } else {
             const uint64_t nb = key_vid(link);
-            if (FRAMED) {
-              const DevMeta &m = g.meta[pid[u]];
-              bkt[u] = bucket_phys<true>(g, m, nb - istart[u] + psz[u]);
-            } else {
-              bkt[u] = nb;
-            }
+            bkt[u] = FRAMED ? bucket_phys<true>(g, m, nb - m.indrct_start + m.psz) : nb;
           }
         }
       }
     }
   }
-  (void)vfb;
-  if (hops_out) *hops_out = hops;
+  uint32_t d = 0, o = 0;
+#pragma unroll
+  for (int u = 0; u < 8; u++)
+    if (l8 == (uint32_t)u) {
+      d = deg[u];
+      o = off[u];
+    }
+  // a follower takes its run leader's result
+  const uint64_t upto = leaders & ((2ull << lane) - 1);
+  const uint32_t src = upto ? 63u - (uint32_t)__builtin_clzll(upto) : lane;
+  d = shfl32(d, src);
+  o = shfl32(o, src);
+  deg_out = valid ? d : 0;
+  off_out = valid ? o : 0;
 }
 
 // Inclusive wave scan of u64 values.
@@ -189,33 +190,6 @@ __device__ __forceinline__ uint64_t block_excl_scan(uint64_t v, uint64_t *tmp, u
   __syncthreads();
   total = tot;
   return pre + inc - v;
-}
-
-// Decoupled look-back (single pass) over tiles handed out in launch order
-// by an atomic ticket, so a tile only ever waits on tiles that are already
-// running. status[t] = value << 2 | flag, flag 1 = aggregate, 2 = inclusive.
-// Called by one thread; returns the tile's exclusive prefix.
-__device__ __forceinline__ uint64_t lookback(uint64_t *status, uint64_t tile, uint64_t agg) {
-  if (tile == 0) {
-    __hip_atomic_store(&status[0], (agg << 2) | 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    return 0;
-  }
-  __hip_atomic_store(&status[tile], (agg << 2) | 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  uint64_t excl = 0;
-  uint64_t j = tile - 1;
-  while (true) {
-    const uint64_t s = __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t f = (uint32_t)(s & 3u);
-    if (f == 0) {
-      __builtin_amdgcn_s_sleep(1);
-      continue;
-    }
-    excl += s >> 2;
-    if (f == 2) break;
-    j--;
-  }
-  __hip_atomic_store(&status[tile], ((excl + agg) << 2) | 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  return excl;
 }
 
 }  // namespace wkg

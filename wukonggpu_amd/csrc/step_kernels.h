@@ -1,18 +1,25 @@
-// Pattern-step kernels: fused probe + scan (+ filter compaction), merge-path
-// partition, load-balanced expand, single-key probe and list copy.
+// Pattern-step kernels: probe, tile scan, finalize (compaction of the rows
+// that produce output), merge-path partition, load-balanced expand,
+// single-key probe and list copy.
 //
 // One known_to_unknown step (RCache::known_to_unknown, core/rcache.hpp:
 // 232-414) runs as
-//   k_clear -> k_probe_scan (K2U) -> [total read back] -> k_partition -> k_expand
+//   k_probe -> k_scan_tiles -> k_finalize -> [totals read back] -> k_partition -> k_expand
 // replacing the reference's keygen, probe, edge-list fetch, Thrust scan and
 // one-thread-per-row update (core/gpu_hash.cu:145-316, 527-610, 685-694).
 // known_to_known / known_to_const (core/gpu_hash.cu:320-466, 613-682) run as
-//   k_clear -> k_probe_scan (K2K|K2C)
-// which probes, tests membership, scans the flags and compacts the kept rows
-// in one pass. index_to_unknown / const_to_unknown are k_probe_one + k_copy.
+//   k_probe -> k_scan_tiles -> k_finalize (stable compaction of kept rows)
+// index_to_unknown / const_to_unknown are k_probe_one + k_copy_list.
 //
-// The per-tile bodies (probe_scan_tile, expand_tile) are shared with the
-// batched kernels, which run many independent steps in one launch.
+// The probe writes, per tile, the list of rows that have edges (or that a
+// filter keeps) and the tile's sums; the tile scan turns the sums into
+// prefixes; the finalize pass moves each tile's list to its global place.
+// Everything after the probe touches only rows that produce output. (This is
+// reduce-then-scan: a single-pass decoupled look-back serialises on
+// device-scope loads that cross the XCDs' separate L2s.)
+//
+// The per-tile bodies are shared with the batched kernels, which run many
+// independent steps in one launch.
 #pragma once
 
 #include "device.h"
@@ -22,29 +29,29 @@ namespace {
 
 enum StepMode : int { kModeK2U = 0, kModeK2K = 1, kModeK2C = 2 };
 
-__global__ void k_clear(uint64_t *a, uint64_t n) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    a[i] = 0;
-}
+constexpr int kRowsPerThread = kTileRows / kBlock;
+constexpr int kRounds = kTileRows / kBlock;  // probe rounds: one row per lane per round
 
-// One step's view for a probe/scan tile.
+// One step's view for the probe / scan / finalize passes.
 struct StepParams {
-  const int32_t *T;   // input table (null for a single-key step)
-  uint64_t R;         // input rows (1 for a single-key step)
-  uint32_t c;         // input columns
-  uint32_t sc, ec;    // start column; K2K end column
+  const int32_t *T;    // input table (null for a single-key step)
+  uint64_t R;          // input rows (1 for a single-key step); < 2^32
+  uint32_t c;          // input columns
+  uint32_t sc, ec;     // start column; K2K end column
   uint32_t pid, dir;
-  uint32_t target;    // K2C constant
-  uint32_t target_ok; // K2C constant fits u32 (otherwise nothing matches)
-  int mode;           // kMode*
-  uint64_t key;       // != 0: single-key step (index / const start), R == 1
-  uint2 *rowinfo;     // K2U: (offset in predicate segment, degree) per row
-  uint64_t *excl;     // K2U: exclusive output-row offset per row
-  int32_t *O;         // filters: compacted output, capacity R x c
-  uint64_t *out_rows; // total output rows (written by the step's last tile)
-  uint64_t *examined; // filters: edges examined (atomic add)
-  uint64_t *status;   // look-back words of the step's tiles
-  uint64_t ntiles;    // tiles of this step
+  uint32_t target;     // K2C constant
+  uint32_t target_ok;  // K2C constant fits u32 (otherwise nothing matches)
+  int mode;            // kMode*
+  uint64_t key;        // != 0: single-key step (index / const start), R == 1
+  uint4 *hits;         // per tile, kTileRows entries: (row, offset, degree, 0) of rows with output
+  uint2 *cinfo;        // K2U, compacted rows: (row, offset)
+  uint64_t *cexcl;     // K2U, compacted rows: exclusive output-row offset
+  int32_t *O;          // filters: compacted output, capacity R x c
+  uint64_t *tile_sum;  // per tile: output rows; after the tile scan, the tile's prefix
+  uint64_t *tile_nz;   // per tile: rows with output; after the tile scan, the prefix
+  uint64_t *tile_ex;   // per tile: edges examined (filters)
+  uint64_t *totals;    // [0] output rows, [1] edges examined, [2] rows with output
+  uint64_t ntiles;     // tiles of this step
 };
 
 // Membership of `target` in a group's edge list (known_to_known/const test,
@@ -109,137 +116,211 @@ __device__ __forceinline__ void group_member(const GraphView &g, uint64_t estart
   examined = found ? idx + 1 : deg;
 }
 
-// One tile (kTileRows rows) of a step. 32 groups of 8 lanes probe 8 rows each
-// (8 bucket reads in flight per lane), then the per-row counts (degree, or
-// kept flag) are scanned in the block and across the step's tiles with
-// decoupled look-back. `lt` is the tile's index within its step.
+// Probe pass over one tile (kTileRows rows; `lt` is the tile's index within
+// its step). Each round, the 4 waves probe 64 consecutive rows each. Writes
+// the tile's list of rows with output and its sums.
 template <bool SORTED, bool FRAMED>
-__device__ __forceinline__ void probe_scan_tile(const GraphView &g, const StepParams &p, uint64_t lt) {
-  __shared__ uint32_t s_cnt[kTileRows];
-  __shared__ uint32_t s_aux[kTileRows];  // K2U: segment offset; filters: examined
-  __shared__ uint32_t s_sel[kTileRows];  // filters: kept row ids in order
+__device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams &p, uint64_t lt) {
+  __shared__ uint32_t s_cnt[kTileRows];  // degree, or kept flag
+  __shared__ uint32_t s_aux[kTileRows];  // segment offset, or edges examined
   __shared__ uint64_t s_tmp[kBlock / kWave];
-  __shared__ uint64_t s_excl;
-  constexpr int U = kTileRows / (kBlock / 8);  // rows per group
-  const uint32_t tid = threadIdx.x, grp = tid >> 3, l8 = tid & 7;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), gbase = lane & ~7u, l8 = lane & 7;
+  const uint32_t wave = tid / kWave;
   const uint64_t base = lt * kTileRows;
+  const DevMeta m = g.meta[p.pid < g.num_metas ? p.pid : 0];
 
-  uint64_t key[U];
-  uint32_t pid[U], deg[U], off[U];
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    const uint64_t row = base + grp + (uint64_t)u * (kBlock / 8);
-    key[u] = 0;
-    pid[u] = p.pid;
+  for (int rd = 0; rd < kRounds; rd++) {
+    const uint32_t local = rd * kBlock + wave * kWave + lane;
+    const uint64_t row = base + local;
+    uint64_t key = 0;
+    int32_t end_cell = 0;
     if (row < p.R) {
       if (p.key) {
-        key[u] = p.key;
+        key = p.key;
       } else {
         const int32_t cell = p.T[row * p.c + p.sc];
-        if (cell >= 0) key[u] = make_key((uint32_t)cell, p.pid, p.dir);
+        if (cell >= 0) key = make_key((uint32_t)cell, p.pid, p.dir);
+        if (p.mode == kModeK2K) end_cell = p.T[row * p.c + p.ec];
       }
     }
-  }
-  group_probe<U, FRAMED>(g, key, pid, deg, off);
-
-  if (p.mode == kModeK2U) {
+    uint32_t deg, off;
+    wave_probe<FRAMED>(g, m, key, deg, off);
+    if (p.mode == kModeK2U) {
+      s_cnt[local] = deg;
+      s_aux[local] = off;
+    } else {
+      // the 8 lanes of a group test the group's 8 rows one after another
+      uint32_t my_found = 0, my_ex = 0;
 #pragma unroll
-    for (int u = 0; u < U; u++)
-      if (l8 == 0) {
-        s_cnt[grp + u * (kBlock / 8)] = deg[u];
-        s_aux[grp + u * (kBlock / 8)] = off[u];
+      for (int u = 0; u < 8; u++) {
+        const uint32_t src = gbase + u;
+        const uint32_t du = shfl32(deg, src), ou = shfl32(off, src);
+        uint32_t target = p.target;
+        bool valid = p.target_ok != 0;
+        if (p.mode == kModeK2K) {
+          const int32_t ec = (int32_t)shfl32((uint32_t)end_cell, src);
+          target = (uint32_t)ec;
+          valid = ec >= 0;
+        }
+        bool found;
+        uint32_t examined;
+        group_member<SORTED, FRAMED>(g, m.edge_start, m.eframe_base, ou, du, target, valid, found, examined);
+        if (l8 == (uint32_t)u) {
+          my_found = found ? 1u : 0u;
+          my_ex = examined;
+        }
       }
-  } else {
-    const DevMeta m = g.meta[p.pid < g.num_metas ? p.pid : 0];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const uint64_t row = base + grp + (uint64_t)u * (kBlock / 8);
-      uint32_t target = p.target;
-      bool valid = p.target_ok != 0;
-      if (p.mode == kModeK2K) {
-        int32_t cell = 0;
-        if (row < p.R) cell = p.T[row * p.c + p.ec];
-        target = (uint32_t)cell;
-        valid = cell >= 0;
-      }
-      bool found;
-      uint32_t examined;
-      group_member<SORTED, FRAMED>(g, m.edge_start, m.eframe_base, off[u], deg[u], target, valid && row < p.R,
-                                   found, examined);
-      if (l8 == 0) {
-        s_cnt[grp + u * (kBlock / 8)] = found ? 1u : 0u;
-        s_aux[grp + u * (kBlock / 8)] = examined;
-      }
+      s_cnt[local] = my_found;
+      s_aux[local] = my_ex;
     }
   }
   __syncthreads();
-
-  const uint32_t cnt = s_cnt[tid];
-  uint64_t total;
-  const uint64_t lex = block_excl_scan(cnt, s_tmp, total);
+  // thread t owns the tile's rows 8t .. 8t+7
+  uint64_t sum = 0, ex = 0;
+  uint32_t nz = 0;
+#pragma unroll
+  for (int k = 0; k < kRowsPerThread; k++) {
+    const uint32_t c = s_cnt[tid * kRowsPerThread + k];
+    sum += c;
+    nz += c ? 1u : 0u;
+    ex += s_aux[tid * kRowsPerThread + k];
+  }
+  uint64_t nz_total, sum_total, ex_total = 0;
+  uint32_t pos = (uint32_t)block_excl_scan(nz, s_tmp, nz_total);
+  (void)block_excl_scan(sum, s_tmp, sum_total);
+  if (p.mode != kModeK2U) (void)block_excl_scan(ex, s_tmp, ex_total);
+  uint4 *hits = p.hits + lt * kTileRows;
+#pragma unroll
+  for (int k = 0; k < kRowsPerThread; k++) {
+    const uint32_t local = tid * kRowsPerThread + k;
+    const uint32_t c = s_cnt[local];
+    if (c) hits[pos++] = make_uint4((uint32_t)(base + local), p.mode == kModeK2U ? s_aux[local] : 0u, c, 0u);
+  }
   if (tid == 0) {
-    s_excl = lookback(p.status, lt, total);
-    if (lt == p.ntiles - 1) *p.out_rows = s_excl + total;
-  }
-  __syncthreads();
-  const uint64_t row = base + tid;
-  if (p.mode == kModeK2U) {
-    if (row < p.R) {
-      p.rowinfo[row] = make_uint2(s_aux[tid], cnt);
-      p.excl[row] = s_excl + lex;
-    }
-  } else {
-    uint64_t ex_total;
-    (void)block_excl_scan(s_aux[tid], s_tmp, ex_total);
-    if (tid == 0 && ex_total) atomicAdd((unsigned long long *)p.examined, (unsigned long long)ex_total);
-    if (cnt) s_sel[lex] = tid;
-    __syncthreads();
-    // kept rows of this tile are contiguous in O from row s_excl on
-    const uint32_t c = p.c, nsel = (uint32_t)total;
-    const uint32_t rpp = kBlock / c;
-    if (tid < rpp * c) {
-      const uint32_t col = tid % c;
-      for (uint32_t r = tid / c; r < nsel; r += rpp) {
-        const uint64_t src = base + s_sel[r];
-        p.O[(s_excl + r) * c + col] = p.T[src * c + col];
-      }
-    }
+    p.tile_sum[lt] = sum_total;
+    p.tile_nz[lt] = nz_total;
+    if (p.mode != kModeK2U) p.tile_ex[lt] = ex_total;
   }
 }
 
 template <bool SORTED, bool FRAMED>
-__global__ __launch_bounds__(kBlock) void k_probe_scan(GraphView g, StepParams p, uint32_t *ticket) {
-  __shared__ uint64_t s_tile;
-  if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
-  __syncthreads();
-  probe_scan_tile<SORTED, FRAMED>(g, p, s_tile);
+__global__ __launch_bounds__(kBlock) void k_probe(GraphView g, StepParams p) {
+  probe_tile<SORTED, FRAMED>(g, p, blockIdx.x);
 }
 
-// Merge-path partition of the (row starts, output rows) sequence: part[k] =
-// number of rows whose start item lies before diagonal k * kExpandNV. Row i's
-// start item sits at i + excl[i].
-__device__ __forceinline__ uint64_t merge_split(const uint64_t *excl, uint64_t R, uint64_t E, uint64_t k) {
+// In-place exclusive scan of a[0, n) by one block; returns the total.
+__device__ __forceinline__ uint64_t block_scan_array(uint64_t *a, uint64_t n, uint64_t *s_tmp) {
+  const uint64_t per = (n + kBlock - 1) / kBlock;
+  const uint64_t lo = threadIdx.x * per < n ? threadIdx.x * per : n;
+  const uint64_t hi = lo + per < n ? lo + per : n;
+  uint64_t s = 0;
+  uint64_t i = lo;
+  for (; i + 8 <= hi; i += 8) {  // 8 independent loads in flight
+    uint64_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = a[i + k];
+#pragma unroll
+    for (int k = 0; k < 8; k++) s += v[k];
+  }
+  for (; i < hi; i++) s += a[i];
+  uint64_t total;
+  uint64_t run = block_excl_scan(s, s_tmp, total);
+  for (i = lo; i < hi; i++) {
+    const uint64_t v = a[i];
+    a[i] = run;
+    run += v;
+  }
+  return total;
+}
+
+// Tile scan of one step (one block): tile_sum and tile_nz become prefixes;
+// writes the step's totals.
+__device__ __forceinline__ void scan_step_tiles(const StepParams &p) {
+  __shared__ uint64_t s_tmp[kBlock / kWave];
+  const uint64_t rows = block_scan_array(p.tile_sum, p.ntiles, s_tmp);
+  const uint64_t nz = block_scan_array(p.tile_nz, p.ntiles, s_tmp);
+  uint64_t ex = 0;
+  if (p.mode != kModeK2U) {
+    uint64_t e = 0;
+    for (uint64_t i = threadIdx.x; i < p.ntiles; i += kBlock) e += p.tile_ex[i];
+    (void)block_excl_scan(e, s_tmp, ex);
+  }
+  if (threadIdx.x == 0) {
+    p.totals[0] = rows;
+    p.totals[1] = ex;
+    p.totals[2] = nz;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(StepParams p) { scan_step_tiles(p); }
+
+// Finalize pass over one tile: K2U moves the tile's rows with edges to the
+// compacted row list with their exclusive output offsets; the filters copy
+// the kept rows, in order, to the compacted output.
+__device__ __forceinline__ void finalize_tile(const StepParams &p, uint64_t lt) {
+  __shared__ uint64_t s_tmp[kBlock / kWave];
+  __shared__ uint32_t s_sel[kBlock];
+  const uint32_t tid = threadIdx.x;
+  const uint64_t nz0 = p.tile_nz[lt];
+  const uint64_t nz1 = lt + 1 < p.ntiles ? p.tile_nz[lt + 1] : p.totals[2];
+  const uint32_t nnz = (uint32_t)(nz1 - nz0);
+  const uint64_t out0 = p.tile_sum[lt];
+  const uint4 *hits = p.hits + lt * kTileRows;
+  uint64_t carry = 0;
+  for (uint32_t k0 = 0; k0 < nnz; k0 += kBlock) {
+    const uint32_t j = k0 + tid;
+    const uint4 h = j < nnz ? hits[j] : make_uint4(0, 0, 0, 0);
+    if (p.mode == kModeK2U) {
+      uint64_t tot;
+      const uint64_t ex = block_excl_scan(h.z, s_tmp, tot);
+      if (j < nnz) {
+        p.cinfo[nz0 + j] = make_uint2(h.x, h.y);
+        p.cexcl[nz0 + j] = out0 + carry + ex;
+      }
+      carry += tot;
+    } else {
+      if (j < nnz) s_sel[tid] = h.x;
+      __syncthreads();
+      // kept rows k0.. of this tile go to output rows out0 + k0 ..
+      const uint32_t c = p.c, nsel = nnz - k0 < (uint32_t)kBlock ? nnz - k0 : (uint32_t)kBlock;
+      const uint32_t rpp = kBlock / c;
+      if (tid < rpp * c) {
+        const uint32_t col = tid % c;
+        for (uint32_t r = tid / c; r < nsel; r += rpp)
+          p.O[(out0 + k0 + r) * c + col] = p.T[(uint64_t)s_sel[r] * c + col];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_finalize(StepParams p) { finalize_tile(p, blockIdx.x); }
+
+// Merge-path partition of the (row starts, output rows) sequence over the
+// compacted rows: part[k] = number of rows whose start item lies before
+// diagonal k * kExpandNV. Row i's start item sits at i + cexcl[i].
+__device__ __forceinline__ uint64_t merge_split(const uint64_t *cexcl, uint64_t R, uint64_t E, uint64_t k) {
   const uint64_t diag = k * (uint64_t)kExpandNV < R + E ? k * (uint64_t)kExpandNV : R + E;
-  uint64_t lo = 0, hi = R;  // first i with i + excl[i] >= diag (R if none)
+  uint64_t lo = 0, hi = R;  // first i with i + cexcl[i] >= diag (R if none)
   while (lo < hi) {
     const uint64_t mid = (lo + hi) >> 1;
-    if (mid + excl[mid] < diag) lo = mid + 1; else hi = mid;
+    if (mid + cexcl[mid] < diag) lo = mid + 1; else hi = mid;
   }
   return lo;
 }
 
-__global__ void k_partition(const uint64_t *excl, uint64_t R, uint64_t E, uint64_t ntiles, uint64_t *part) {
+__global__ void k_partition(const uint64_t *cexcl, uint64_t R, uint64_t E, uint64_t ntiles, uint64_t *part) {
   const uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (k <= ntiles) part[k] = merge_split(excl, R, E, k);
+  if (k <= ntiles) part[k] = merge_split(cexcl, R, E, k);
 }
 
 struct ExpandParams {
   uint32_t pid;
-  const int32_t *T;
-  uint64_t R;
-  uint32_t c;
-  const uint2 *rowinfo;
-  const uint64_t *excl;
+  const int32_t *T;      // input table
+  uint32_t c;            // input columns
+  uint64_t R;            // compacted rows (rows with at least one edge)
+  const uint2 *cinfo;    // compacted rows: (input row, segment offset)
+  const uint64_t *cexcl; // compacted rows: exclusive output-row offset
   const uint64_t *part;  // ntiles + 1 split points
   uint64_t E;
   int32_t *O;  // E x (c+1)
@@ -247,15 +328,16 @@ struct ExpandParams {
 
 // Load-balanced expand (known_to_unknown result update): a tile owns
 // kExpandNV items of the merged sequence of row starts and output rows, so a
-// hub row's outputs spread over many workgroups and runs of empty rows cost
-// one item each. Output row o of input row i is (row i, edge k) with
-// k = o - excl[i]; rows come out in input order and edges in list order
-// (core/engine.hpp:142-150). Output cells are written with consecutive lanes
-// on consecutive ints. `tile` is the tile index within the step.
+// hub row's outputs spread over many workgroups. Output row o of compacted
+// row i is (input row, edge k) with k = o - cexcl[i]; rows come out in input
+// order and edges in list order (core/engine.hpp:142-150). Output cells are
+// written with consecutive lanes on consecutive ints. `tile` is the tile
+// index within the step.
 template <bool FRAMED>
 __device__ __forceinline__ void expand_tile(const GraphView &g, const ExpandParams &a, uint64_t tile) {
   __shared__ int32_t s_ex[kExpandNV + 1];
   __shared__ uint32_t s_off[kExpandNV + 1];
+  __shared__ uint32_t s_row[kExpandNV + 1];
   __shared__ uint16_t s_orow[kExpandNV];
   const uint32_t tid = threadIdx.x;
   const uint64_t d0 = tile * (uint64_t)kExpandNV;
@@ -263,15 +345,18 @@ __device__ __forceinline__ void expand_tile(const GraphView &g, const ExpandPara
   const uint64_t d1 = d0 + kExpandNV < dend ? d0 + kExpandNV : dend;
   const uint64_t i0 = a.part[tile], i1 = a.part[tile + 1];
   const uint64_t j0 = d0 - i0, j1 = d1 - i1;
-  const uint32_t nr = (uint32_t)(i1 - i0 + 1);  // local row r <-> input row i0 - 1 + r
+  const uint32_t nr = (uint32_t)(i1 - i0 + 1);  // local row r <-> compacted row i0 - 1 + r
   for (uint32_t r = tid; r < nr; r += kBlock) {
     if (r == 0 && i0 == 0) {
       s_ex[0] = 0;
       s_off[0] = 0;
+      s_row[0] = 0;
     } else {
       const uint64_t i = i0 + r - 1;
-      s_ex[r] = (int32_t)((int64_t)a.excl[i] - (int64_t)j0);
-      s_off[r] = a.rowinfo[i].x;
+      const uint2 ci = a.cinfo[i];
+      s_ex[r] = (int32_t)((int64_t)a.cexcl[i] - (int64_t)j0);
+      s_row[r] = ci.x;
+      s_off[r] = ci.y;
     }
   }
   __syncthreads();
@@ -305,10 +390,9 @@ __device__ __forceinline__ void expand_tile(const GraphView &g, const ExpandPara
     const DevMeta m = g.meta[a.pid];
     for (uint32_t r = tid / nc; r < nout; r += rpp) {
       const uint32_t lr = s_orow[r];
-      const uint64_t i = i0 + lr - 1;
       int32_t v;
       if (col < c) {
-        v = a.T[i * c + col];
+        v = a.T[(uint64_t)s_row[lr] * c + col];
       } else {
         const uint32_t k = (uint32_t)((int32_t)r - s_ex[lr]);
         v = (int32_t)edge_at<FRAMED>(g, m.edge_start, m.eframe_base, (uint64_t)s_off[lr] + k);
@@ -328,12 +412,12 @@ __global__ __launch_bounds__(kBlock) void k_expand(GraphView g, ExpandParams a) 
 // out[1] = offset in the predicate segment. One wave.
 template <bool FRAMED>
 __global__ __launch_bounds__(kWave) void k_probe_one(GraphView g, uint64_t key, uint32_t pid, uint64_t *out) {
-  uint64_t k[1] = {threadIdx.x < 8 ? key : 0};
-  uint32_t p[1] = {pid}, deg[1], off[1];
-  group_probe<1, FRAMED>(g, k, p, deg, off);
+  const DevMeta m = g.meta[pid];
+  uint32_t deg, off;
+  wave_probe<FRAMED>(g, m, threadIdx.x == 0 ? key : 0, deg, off);
   if (threadIdx.x == 0) {
-    out[0] = deg[0];
-    out[1] = off[0];
+    out[0] = deg;
+    out[1] = off;
   }
 }
 

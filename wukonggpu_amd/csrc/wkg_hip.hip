@@ -201,8 +201,11 @@ int wkg_ctx::step_single_list(uint64_t key, int pid, hipStream_t s, wkg_table **
   uint64_t *d_rb = (uint64_t *)pool->alloc(2 * sizeof(uint64_t), s, err);
   if (!d_rb) return WKG_ENOMEM;
   GraphView g = view();
-  if (framed) k_probe_one<true><<<1, kWave, 0, s>>>(g, key, (uint32_t)pid, d_rb);
-  else k_probe_one<false><<<1, kWave, 0, s>>>(g, key, (uint32_t)pid, d_rb);
+  {
+    ProfScope ps(this, s, "probe_one", 128.0 + 16.0);
+    if (framed) k_probe_one<true><<<1, kWave, 0, s>>>(g, key, (uint32_t)pid, d_rb);
+    else k_probe_one<false><<<1, kWave, 0, s>>>(g, key, (uint32_t)pid, d_rb);
+  }
   LAUNCHCHK(this);
   uint64_t hv[2];
   rc = read_counters(d_rb, 2, s, hv);
@@ -212,8 +215,11 @@ int wkg_ctx::step_single_list(uint64_t key, int pid, hipStream_t s, wkg_table **
   if ((rc = new_table(hv[0], 1, s, &t))) return rc;
   if (hv[0]) {
     const uint64_t blocks = std::min<uint64_t>((hv[0] + kBlock - 1) / kBlock, 4096);
-    if (framed) k_copy_list<true><<<blocks, kBlock, 0, s>>>(g, (uint32_t)pid, hv[1], hv[0], t->d);
-    else k_copy_list<false><<<blocks, kBlock, 0, s>>>(g, (uint32_t)pid, hv[1], hv[0], t->d);
+    {
+      ProfScope ps(this, s, "copy_list", 8.0 * (double)hv[0]);
+      if (framed) k_copy_list<true><<<blocks, kBlock, 0, s>>>(g, (uint32_t)pid, hv[1], hv[0], t->d);
+      else k_copy_list<false><<<blocks, kBlock, 0, s>>>(g, (uint32_t)pid, hv[1], hv[0], t->d);
+    }
     LAUNCHCHK(this);
   }
   if (expanded) *expanded = hv[0];
@@ -221,15 +227,14 @@ int wkg_ctx::step_single_list(uint64_t key, int pid, hipStream_t s, wkg_table **
   return WKG_OK;
 }
 
-static void launch_probe_scan(bool framed, bool sorted, const GraphView &g, const StepParams &p, uint32_t *ticket,
-                              hipStream_t s) {
+static void launch_probe(bool framed, bool sorted, const GraphView &g, const StepParams &p, hipStream_t s) {
   const unsigned grid = (unsigned)p.ntiles;
   if (framed) {
-    if (sorted) k_probe_scan<true, true><<<grid, kBlock, 0, s>>>(g, p, ticket);
-    else k_probe_scan<false, true><<<grid, kBlock, 0, s>>>(g, p, ticket);
+    if (sorted) k_probe<true, true><<<grid, kBlock, 0, s>>>(g, p);
+    else k_probe<false, true><<<grid, kBlock, 0, s>>>(g, p);
   } else {
-    if (sorted) k_probe_scan<true, false><<<grid, kBlock, 0, s>>>(g, p, ticket);
-    else k_probe_scan<false, false><<<grid, kBlock, 0, s>>>(g, p, ticket);
+    if (sorted) k_probe<true, false><<<grid, kBlock, 0, s>>>(g, p);
+    else k_probe<false, false><<<grid, kBlock, 0, s>>>(g, p);
   }
 }
 
@@ -245,21 +250,23 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
     return fail(WKG_EINVAL, "end column %d outside the %d-column table", ec, c);
   const int out_cols = mode == kModeK2U ? c + 1 : c;
   if (out_cols > kBlock) return fail(WKG_ERANGE, "tables are limited to %d columns", kBlock);
+  if (R >= (1ull << 32)) return fail(WKG_ERANGE, "tables are limited to 2^32 - 1 rows per step");
   if ((rc = ensure_pred(pid, s))) return rc;
   if (R == 0) {  // empty input gives an empty result (core/rcache.hpp:242-250)
     if (expanded) *expanded = 0;
     return new_table(0, out_cols, s, out);
   }
   const uint64_t ntiles = (R + kTileRows - 1) / kTileRows;
-  // scratch: out_rows, examined, ticket, pad | status[ntiles] | rowinfo[R] | excl[R]
-  const uint64_t n_misc = 4;
-  const uint64_t scratch_words = n_misc + ntiles + (mode == kModeK2U ? 2 * R : 0);
+  // scratch: totals[4] | tile_sum | tile_nz | tile_ex | hits[ntiles * kTileRows] (16 B) |
+  //          K2U: cinfo[R] (8 B) | cexcl[R]
+  const uint64_t hit_words = 2 * ntiles * kTileRows;
+  const uint64_t scratch_words = 4 + 3 * ntiles + hit_words + (mode == kModeK2U ? 2 * R : 0);
   uint64_t *scr = (uint64_t *)pool->alloc(scratch_words * 8, s, err);
   if (!scr) return WKG_ENOMEM;
-  uint64_t *counters = scr, *status = scr + n_misc;
-  uint32_t *ticket = (uint32_t *)(scr + 2);
-  uint2 *rowinfo = (uint2 *)(status + ntiles);
-  uint64_t *excl = (uint64_t *)(rowinfo + R);
+  uint64_t *totals = scr, *tile_sum = scr + 4, *tile_nz = tile_sum + ntiles, *tile_ex = tile_nz + ntiles;
+  uint4 *hits = (uint4 *)(tile_ex + ntiles);
+  uint2 *cinfo = (uint2 *)((uint64_t *)hits + hit_words);
+  uint64_t *cexcl = (uint64_t *)(cinfo + R);
 
   wkg_table *t = nullptr;
   if (mode != kModeK2U) {
@@ -267,10 +274,6 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
       pool->release(scr, s);
       return rc;
     }
-  }
-  {
-    const uint64_t n = n_misc + ntiles;
-    k_clear<<<(unsigned)std::min<uint64_t>((n + kBlock - 1) / kBlock, 1024), kBlock, 0, s>>>(scr, n);
   }
   StepParams p{};
   p.T = in->d;
@@ -284,24 +287,41 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
   p.target_ok = (cval >= 0 && cval <= 0xffffffffLL) ? 1u : 0u;
   p.mode = mode;
   p.key = 0;
-  p.rowinfo = rowinfo;
-  p.excl = excl;
+  p.hits = hits;
+  p.cinfo = cinfo;
+  p.cexcl = cexcl;
   p.O = t ? t->d : nullptr;
-  p.out_rows = counters;
-  p.examined = counters + 1;
-  p.status = status;
+  p.tile_sum = tile_sum;
+  p.tile_nz = tile_nz;
+  p.tile_ex = tile_ex;
+  p.totals = totals;
   p.ntiles = ntiles;
   const GraphView g = view();
-  launch_probe_scan(framed, sorted || mode == kModeK2U, g, p, ticket, s);
+  {
+    // start cell (+ end cell for K2K) and one 128-byte bucket per row
+    const double b = (double)R * (4.0 + (mode == kModeK2K ? 4.0 : 0.0) + 128.0);
+    ProfScope ps(this, s, mode == kModeK2U ? "probe" : "probe_filter", b);
+    launch_probe(framed, sorted || mode == kModeK2U, g, p, s);
+  }
+  {
+    ProfScope ps(this, s, "scan_tiles", 24.0 * (double)ntiles);
+    k_scan_tiles<<<1, kBlock, 0, s>>>(p);
+  }
+  {
+    ProfScope ps(this, s, mode == kModeK2U ? "finalize" : "compact", 0.0);
+    k_finalize<<<(unsigned)ntiles, kBlock, 0, s>>>(p);
+  }
   hipError_t le = hipGetLastError();
-  uint64_t cnt[2] = {0, 0};
-  if (le == hipSuccess) rc = read_counters(counters, 2, s, cnt);
+  uint64_t cnt[3] = {0, 0, 0};
+  if (le == hipSuccess) rc = read_counters(totals, 3, s, cnt);
   if (le != hipSuccess || rc) {
     pool->release(scr, s);
     drop_table(t);
     return le != hipSuccess ? fail(WKG_EDEVICE, "probe launch failed: %s", hipGetErrorString(le)) : rc;
   }
-  if (mode != kModeK2U) {  // filters: compacted in place
+  const uint64_t nz = cnt[2];
+  if (mode != kModeK2U) {  // filters: kept rows compacted in place
+    prof_add_bytes("compact", 16.0 * nz + 8.0 * nz * c);
     pool->release(scr, s);
     t->rows = cnt[0];
     t->bytes = t->rows * (uint64_t)c * 4;
@@ -309,6 +329,7 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
     *out = t;
     return WKG_OK;
   }
+  prof_add_bytes("finalize", 32.0 * nz);
   const uint64_t E = cnt[0];
   if (expanded) *expanded = E;
   if ((rc = new_table(E, out_cols, s, &t))) {
@@ -316,26 +337,35 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
     return rc;
   }
   if (E) {
-    const uint64_t nx = (R + E + kExpandNV - 1) / kExpandNV;
+    const uint64_t nx = (nz + E + kExpandNV - 1) / kExpandNV;
     uint64_t *part = (uint64_t *)pool->alloc((nx + 1) * 8, s, err);
     if (!part) {
       pool->release(scr, s);
       drop_table(t);
       return WKG_ENOMEM;
     }
-    k_partition<<<(unsigned)((nx + 1 + kBlock - 1) / kBlock), kBlock, 0, s>>>(excl, R, E, nx, part);
+    {
+      ProfScope ps(this, s, "partition", 8.0 * (double)(nx + 1));
+      k_partition<<<(unsigned)((nx + 1 + kBlock - 1) / kBlock), kBlock, 0, s>>>(cexcl, nz, E, nx, part);
+    }
     ExpandParams x{};
     x.pid = (uint32_t)pid;
     x.T = in->d;
-    x.R = R;
     x.c = (uint32_t)c;
-    x.rowinfo = rowinfo;
-    x.excl = excl;
+    x.R = nz;
+    x.cinfo = cinfo;
+    x.cexcl = cexcl;
     x.part = part;
     x.E = E;
     x.O = t->d;
-    if (framed) k_expand<true><<<(unsigned)nx, kBlock, 0, s>>>(g, x);
-    else k_expand<false><<<(unsigned)nx, kBlock, 0, s>>>(g, x);
+    {
+      // edges gathered (4E) + output rows written (4E(c+1)) + the input rows
+      // with edges and their offsets read once (4 nz c + 16 nz)
+      const double b = 4.0 * E + 4.0 * E * (c + 1) + 4.0 * nz * c + 16.0 * nz;
+      ProfScope ps(this, s, "expand", b);
+      if (framed) k_expand<true><<<(unsigned)nx, kBlock, 0, s>>>(g, x);
+      else k_expand<false><<<(unsigned)nx, kBlock, 0, s>>>(g, x);
+    }
     hipError_t e2 = hipGetLastError();
     pool->release(part, s);
     if (e2 != hipSuccess) {
@@ -350,9 +380,116 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
 }
 
 // ---------------------------------------------------------------------------
+// profiling and the copy-bandwidth probe
+
+hipEvent_t wkg_ctx::prof_event() {
+  {
+    std::lock_guard<std::mutex> lk(prof_mu);
+    if (!prof_free.empty()) {
+      hipEvent_t e = prof_free.back();
+      prof_free.pop_back();
+      return e;
+    }
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void wkg_ctx::prof_add_bytes(const char *name, double bytes) {
+  if (!profiling) return;
+  std::lock_guard<std::mutex> lk(prof_mu);
+  prof_stats[name].bytes += bytes;
+}
+
+void wkg_ctx::prof_collect() {
+  std::lock_guard<std::mutex> lk(prof_mu);
+  for (ProfRec &r : prof_pending) {
+    float ms = 0;
+    if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      KStat &k = prof_stats[r.name];
+      k.launches++;
+      k.ms += ms;
+      k.bytes += r.bytes;
+    }
+    prof_free.push_back(r.a);
+    prof_free.push_back(r.b);
+  }
+  prof_pending.clear();
+}
+
+namespace {
+// Plain 16-byte-per-lane copy: the reference point for achievable HBM
+// bandwidth and for scaling FETCH_SIZE / WRITE_SIZE counters.
+__global__ __launch_bounds__(256) void k_copy_calibrate(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                         uint64_t n) {
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256ull) dst[i] = src[i];
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // C ABI
 
 extern "C" {
+
+int wkg_profile_enable(wkg_ctx *ctx, int on) {
+  if (!ctx) return WKG_EINVAL;
+  ctx->prof_collect();
+  if (on) {
+    std::lock_guard<std::mutex> lk(ctx->prof_mu);
+    ctx->prof_stats.clear();
+  }
+  ctx->profiling = on != 0;
+  return WKG_OK;
+}
+
+int wkg_profile_read(wkg_ctx *ctx, const char *kernel, uint64_t *launches, double *total_ms, double *alg_bytes) {
+  if (!ctx || !kernel) return WKG_EINVAL;
+  ctx->prof_collect();
+  std::lock_guard<std::mutex> lk(ctx->prof_mu);
+  auto it = ctx->prof_stats.find(kernel);
+  const wkg_ctx::KStat k = it == ctx->prof_stats.end() ? wkg_ctx::KStat{} : it->second;
+  if (launches) *launches = k.launches;
+  if (total_ms) *total_ms = k.ms;
+  if (alg_bytes) *alg_bytes = k.bytes;
+  return WKG_OK;
+}
+
+int wkg_bandwidth_probe(wkg_ctx *ctx, uint64_t bytes, int iters, double *copy_gbps) {
+  if (!ctx || !copy_gbps || bytes < 16 || iters < 1) return WKG_EINVAL;
+  const uint64_t n = bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess) {
+    (void)hipGetLastError();
+    if (a) (void)hipFree(a);
+    return ctx->fail(WKG_ENOMEM, "cannot allocate %llu bytes for the bandwidth probe", (unsigned long long)bytes);
+  }
+  hipStream_t s = ctx->stream;
+  (void)hipMemsetAsync(a, 1, n * 16, s);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  int dev_cus = 256;
+  (void)hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+  const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, (uint64_t)dev_cus * 16);
+  k_copy_calibrate<<<grid, 256, 0, s>>>((const uint4 *)a, (uint4 *)b, n);  // warm up
+  (void)hipEventRecord(e0, s);
+  for (int i = 0; i < iters; i++) {
+    ProfScope ps(ctx, s, "copy_calibrate", 2.0 * n * 16);
+    k_copy_calibrate<<<grid, 256, 0, s>>>((const uint4 *)a, (uint4 *)b, n);
+  }
+  (void)hipEventRecord(e1, s);
+  hipError_t e = hipEventSynchronize(e1);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) return ctx->fail(WKG_EDEVICE, "bandwidth probe failed: %s", hipGetErrorString(e));
+  *copy_gbps = 2.0 * n * 16 * iters / (ms * 1e-3) / 1e9;
+  return WKG_OK;
+}
 
 int wkg_init(int device, uint64_t table_budget_bytes, wkg_ctx **out) {
   if (!out) return WKG_EINVAL;

@@ -86,6 +86,25 @@ struct wkg_ctx {
   Residency *res = nullptr;
   uint64_t resident_bytes = 0, stat_loads = 0, stat_evictions = 0;
 
+  // per-kernel timing with HIP events on the launch stream
+  struct ProfRec {
+    const char *name;
+    hipEvent_t a, b;
+    double bytes;
+  };
+  struct KStat {
+    uint64_t launches = 0;
+    double ms = 0, bytes = 0;
+  };
+  bool profiling = false;
+  std::mutex prof_mu;
+  std::vector<ProfRec> prof_pending;
+  std::vector<hipEvent_t> prof_free;
+  std::map<std::string, KStat> prof_stats;
+  hipEvent_t prof_event();
+  void prof_collect();
+  void prof_add_bytes(const char *name, double bytes);  // bytes known only after a launch
+
   int fail(int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
   wkg::GraphView view() const;
   hipStream_t pick(void *s) const;
@@ -99,4 +118,28 @@ struct wkg_ctx {
   // residency
   int ensure_pred(int pid, hipStream_t s, const std::vector<int> *keep = nullptr);
   void free_graph();
+};
+
+// Records HIP events around the launches in its scope when profiling is on.
+// `bytes` is the algorithmic byte count of the launch (for GB/s figures).
+struct ProfScope {
+  wkg_ctx *c;
+  hipStream_t s;
+  const char *name;
+  double bytes;
+  hipEvent_t a = nullptr;
+  ProfScope(wkg_ctx *ctx, hipStream_t st, const char *n, double b) : c(ctx), s(st), name(n), bytes(b) {
+    if (c->profiling) {
+      a = c->prof_event();
+      if (a) (void)hipEventRecord(a, s);
+    }
+  }
+  ~ProfScope() {
+    if (!a) return;
+    hipEvent_t b = c->prof_event();
+    if (!b) return;
+    (void)hipEventRecord(b, s);
+    std::lock_guard<std::mutex> lk(c->prof_mu);
+    c->prof_pending.push_back({name, a, b, bytes});
+  }
 };

@@ -84,6 +84,22 @@ class Device:
                                      gstore.metas_ptr(), gstore.num_metas, graph_budget_bytes,
                                      1 if sorted_lists else 0))
 
+    def profile(self, on=True):
+        """Turns per-kernel HIP-event timing on (clearing totals) or off."""
+        self._check(N.hip_lib().wkg_profile_enable(self._h, 1 if on else 0))
+
+    def profile_read(self, kernel):
+        """(launches, total_ms, algorithmic_bytes) for a kernel name."""
+        n, ms, b = C.c_uint64(), C.c_double(), C.c_double()
+        self._check(N.hip_lib().wkg_profile_read(self._h, kernel.encode(), C.byref(n), C.byref(ms), C.byref(b)))
+        return int(n.value), float(ms.value), float(b.value)
+
+    def copy_bandwidth(self, nbytes=1 << 30, iters=20):
+        """GB/s (read + write) of a plain device copy kernel."""
+        g = C.c_double()
+        self._check(N.hip_lib().wkg_bandwidth_probe(self._h, nbytes, iters, C.byref(g)))
+        return float(g.value)
+
     def stats(self):
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
         self._check(N.hip_lib().wkg_graph_stats(self._h, C.byref(a), C.byref(b), C.byref(c)))
