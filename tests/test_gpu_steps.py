@@ -149,6 +149,47 @@ def test_hub_rows_span_tiles(dev_hubs, graph_hubs):
             assert ex2 == wex2
 
 
+def test_runs_of_equal_keys(dev_small, graph_small):
+    # long runs of one start value (probed once per wave), runs crossing wave
+    # and tile boundaries, misses and negative cells inside runs
+    _, gs = graph_small
+    rng = np.random.default_rng(21)
+    vals = VB + rng.integers(0, 300, size=400)
+    lens = rng.integers(1, 300, size=400)
+    col = np.repeat(vals, lens).astype(np.int32)
+    col[rng.random(len(col)) < 0.01] = -1
+    col[rng.random(len(col)) < 0.01] = VB + 5000
+    tab = np.stack([col, np.roll(col, 7)], axis=1)
+    dt = dev_small.table(tab)
+    for pid, d in ((2, 1), (3, 0), (1, 1)):
+        want, wex = oracle.run_step(gs, [VB, 2, 1, -1, -2, pid, d, -3], 1, tab)
+        t, ex = dev_small.k2u(dt, 1, pid, d)
+        _eq(t.to_host(), want, ("runs k2u", pid, d))
+        assert ex == wex
+        want, wex = oracle.run_step(gs, [VB, 2, 1, -1, -1, pid, d, -2], 1, tab)
+        t, ex = dev_small.k2k(dt, 0, pid, d, 1)
+        _eq(t.to_host(), want, ("runs k2k", pid, d))
+        assert ex == wex
+
+
+def test_many_tiles_two_level_scan(dev_small, graph_small):
+    # more than 2048 probe tiles of 2048 rows: the tile scan runs in two levels
+    _, gs = graph_small
+    rng = np.random.default_rng(22)
+    n = 2049 * 2048 + 77
+    tab = (VB + rng.integers(0, 300, size=(n, 1))).astype(np.int32)
+    dt = dev_small.table(tab)
+    want, wex = oracle.run_step(gs, [VB, 2, 1, -1, -1, 2, 1, -2], 1, tab)
+    t, ex = dev_small.k2u(dt, 0, 2, 1)
+    _eq(t.to_host(), want, "two-level k2u")
+    assert ex == wex
+    const = int(tab[5, 0])
+    want, wex = oracle.run_step(gs, [VB, 2, 1, -1, -1, 3, 0, const], 1, tab)
+    t, ex = dev_small.k2c(dt, 0, 3, 0, const)
+    _eq(t.to_host(), want, "two-level k2c")
+    assert ex == wex
+
+
 def test_type_index_large(dev_hubs, graph_hubs):
     _, gs = graph_hubs
     tid = 2 + 3  # first type id after 3 predicates

@@ -77,40 +77,62 @@ __device__ __forceinline__ uint32_t edge_at(const GraphView &g, uint64_t edge_st
   return g.edges[g.eframes[eframe_base + (local >> g.eshift)] + (local & mask)];
 }
 
+// Per-wave LDS scratch of wave_probe.
+struct WaveProbeLds {
+  uint4 kb[kWave];   // run leaders' (key, first bucket)
+  uint2 res[kWave];  // run leaders' (degree, offset)
+};
+
 // Cluster-chaining probe of 64 consecutive rows, one per lane, all under one
 // predicate (get_vertex_local, core/gstore.hpp:389-409; device form
-// core/gpu_hash.cu:181-234). Each lane hashes its own key once. A run of
-// equal keys in consecutive lanes is probed once, by its first lane. The
-// probes then run 8 lanes per key: group g walks the keys of lanes g*8..g*8+7
-// in turn, each lane of the group loading one 16-byte slot of the bucket (one
-// coalesced 128-byte read), a ballot finding the matching slot and lane 7's
-// link moving to the next bucket; the 8 walks of a group are interleaved so 8
-// bucket reads are in flight per lane. Every lane of the wave must call this.
-// key == 0 means "no key". Returns the lane's degree (0 on a miss) and list
-// offset relative to the predicate's edge_start.
+// core/gpu_hash.cu:181-234).
+//  1. A run of equal keys in consecutive lanes is probed once: its first lane
+//     (the run leader) hashes the key and parks (key, bucket) in LDS at its
+//     rank among the leaders.
+//  2. The leaders are walked 8 lanes per key: group g takes leaders g, g+8,
+//     g+16, ...; each lane of the group loads one 16-byte slot of the bucket
+//     (one coalesced 128-byte read), a ballot finds the matching slot and
+//     lane 7's link moves to the next bucket. A group's keys are walked
+//     together, so up to 8 bucket reads are in flight per lane, and a wave
+//     with L leaders does ceil(L/8) key slots instead of 8.
+//  3. Results go back through LDS; every lane reads its run leader's.
+// Every lane of the wave must call this with the wave's own `w`. key == 0
+// means "no key". Returns the lane's degree (0 on a miss) and list offset
+// relative to the predicate's edge_start.
 template <bool FRAMED>
-__device__ __forceinline__ void wave_probe(const GraphView &g, const DevMeta &m, uint64_t key, uint32_t &deg_out,
-                                           uint32_t &off_out) {
-  const uint32_t lane = lane_id(), l8 = lane & 7, gbase = lane & ~7u;
+__device__ __forceinline__ void wave_probe(const GraphView &g, const DevMeta &m, uint64_t key, WaveProbeLds &w,
+                                           uint32_t &deg_out, uint32_t &off_out) {
+  const uint32_t lane = lane_id(), l8 = lane & 7, grp = lane >> 3, gbase = lane & ~7u;
   const bool valid = key != 0 && m.psz != 0;
   const uint64_t prev = shfl64(key, (lane - 1) & 63);
   const bool lead = valid && (lane == 0 || key != prev);
   const uint64_t leaders = wave_ballot(lead);
-  uint64_t b0 = 0;
+  const uint32_t L = (uint32_t)__builtin_popcountll(leaders);
   if (lead) {
+    const uint32_t j = (uint32_t)__builtin_popcountll(leaders & ((1ull << lane) - 1));
     const uint64_t local = fastmod(mix64(key), m.psz, m.magic);
-    b0 = FRAMED ? bucket_phys<true>(g, m, local) : m.main_start + local;
+    const uint64_t b = FRAMED ? bucket_phys<true>(g, m, local) : m.main_start + local;
+    w.kb[j] = make_uint4((uint32_t)key, (uint32_t)(key >> 32), (uint32_t)b, (uint32_t)(b >> 32));
   }
+  __builtin_amdgcn_wave_barrier();
+  const uint32_t umax = (L + 7) >> 3;
   uint64_t keyu[8], bkt[8];
   bool act[8];
   uint32_t deg[8], off[8];
 #pragma unroll
   for (int u = 0; u < 8; u++) {
-    keyu[u] = shfl64(key, gbase + u);
-    bkt[u] = shfl64(b0, gbase + u);
-    act[u] = (leaders >> (gbase + u)) & 1;
+    act[u] = false;
+    keyu[u] = 0;
+    bkt[u] = 0;
     deg[u] = 0;
     off[u] = 0;
+    const uint32_t idx = grp + 8u * u;
+    if ((uint32_t)u < umax && idx < L) {
+      const uint4 v = w.kb[idx];
+      keyu[u] = ((uint64_t)v.y << 32) | v.x;
+      bkt[u] = ((uint64_t)v.w << 32) | v.z;
+      act[u] = true;
+    }
   }
   uint32_t hops = 0;
   while (wave_ballot(act[0] | act[1] | act[2] | act[3] | act[4] | act[5] | act[6] | act[7])) {
@@ -122,6 +144,7 @@ __device__ __forceinline__ void wave_probe(const GraphView &g, const DevMeta &m,
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) {
+      if ((uint32_t)u >= umax) break;  // wave-uniform
       const uint64_t k = ((uint64_t)s[u].y << 32) | s[u].x;
       const bool hit = act[u] && l8 < 7 && k == keyu[u];
       const uint32_t gb = (uint32_t)(wave_ballot(hit) >> gbase) & 0xffu;
@@ -146,6 +169,7 @@ __device__ __forceinline__ void wave_probe(const GraphView &g, const DevMeta &m,
       }
     }
   }
+  // lane g*8+u publishes the result of leader g + 8u
   uint32_t d = 0, o = 0;
 #pragma unroll
   for (int u = 0; u < 8; u++)
@@ -153,13 +177,17 @@ __device__ __forceinline__ void wave_probe(const GraphView &g, const DevMeta &m,
       d = deg[u];
       o = off[u];
     }
-  // a follower takes its run leader's result
-  const uint64_t upto = leaders & ((2ull << lane) - 1);
-  const uint32_t src = upto ? 63u - (uint32_t)__builtin_clzll(upto) : lane;
-  d = shfl32(d, src);
-  o = shfl32(o, src);
-  deg_out = valid ? d : 0;
-  off_out = valid ? o : 0;
+  if (grp + 8u * l8 < L) w.res[grp + 8u * l8] = make_uint2(d, o);
+  __builtin_amdgcn_wave_barrier();
+  deg_out = 0;
+  off_out = 0;
+  if (valid) {
+    const uint32_t jl = (uint32_t)__builtin_popcountll(leaders & ((2ull << lane) - 1)) - 1;
+    const uint2 r = w.res[jl];
+    deg_out = r.x;
+    off_out = r.y;
+  }
+  __builtin_amdgcn_wave_barrier();  // w is reused by the wave's next call
 }
 
 // Inclusive wave scan of u64 values.

@@ -52,7 +52,12 @@ struct StepParams {
   uint64_t *tile_ex;   // per tile: edges examined (filters)
   uint64_t *totals;    // [0] output rows, [1] edges examined, [2] rows with output
   uint64_t ntiles;     // tiles of this step
+  // two-level tile scan (null when one block scans all tiles): per chunk of
+  // kScanChunk tiles, the chunk's prefix of output rows / rows with output
+  uint64_t *chunk_sum, *chunk_nz, *chunk_ex;
 };
+
+constexpr int kScanChunk = 8 * kBlock;  // tiles per block in the two-level tile scan
 
 // Membership of `target` in a group's edge list (known_to_known/const test,
 // core/engine.hpp:179-184,211-216). SORTED lists use an 8-lane 9-ary search:
@@ -124,27 +129,32 @@ __device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams 
   __shared__ uint32_t s_cnt[kTileRows];  // degree, or kept flag
   __shared__ uint32_t s_aux[kTileRows];  // segment offset, or edges examined
   __shared__ uint64_t s_tmp[kBlock / kWave];
+  __shared__ WaveProbeLds s_wp[kBlock / kWave];
   const uint32_t tid = threadIdx.x, lane = lane_id(), gbase = lane & ~7u, l8 = lane & 7;
   const uint32_t wave = tid / kWave;
   const uint64_t base = lt * kTileRows;
   const DevMeta m = g.meta[p.pid < g.num_metas ? p.pid : 0];
 
+  // all rounds' table reads are issued up front
+  int32_t cells[kRounds], ends[kRounds];
+#pragma unroll
+  for (int rd = 0; rd < kRounds; rd++) {
+    const uint64_t row = base + rd * kBlock + wave * kWave + lane;
+    const bool in = row < p.R && !p.key;
+    cells[rd] = in ? p.T[row * p.c + p.sc] : -1;
+    ends[rd] = in && p.mode == kModeK2K ? p.T[row * p.c + p.ec] : 0;
+  }
   for (int rd = 0; rd < kRounds; rd++) {
     const uint32_t local = rd * kBlock + wave * kWave + lane;
     const uint64_t row = base + local;
     uint64_t key = 0;
-    int32_t end_cell = 0;
+    const int32_t end_cell = ends[rd];
     if (row < p.R) {
-      if (p.key) {
-        key = p.key;
-      } else {
-        const int32_t cell = p.T[row * p.c + p.sc];
-        if (cell >= 0) key = make_key((uint32_t)cell, p.pid, p.dir);
-        if (p.mode == kModeK2K) end_cell = p.T[row * p.c + p.ec];
-      }
+      if (p.key) key = p.key;
+      else if (cells[rd] >= 0) key = make_key((uint32_t)cells[rd], p.pid, p.dir);
     }
     uint32_t deg, off;
-    wave_probe<FRAMED>(g, m, key, deg, off);
+    wave_probe<FRAMED>(g, m, key, s_wp[wave], deg, off);
     if (p.mode == kModeK2U) {
       s_cnt[local] = deg;
       s_aux[local] = off;
@@ -254,6 +264,66 @@ __device__ __forceinline__ void scan_step_tiles(const StepParams &p) {
 
 __global__ __launch_bounds__(kBlock) void k_scan_tiles(StepParams p) { scan_step_tiles(p); }
 
+// Two-level tile scan, level 1: block b scans tiles [b*kScanChunk, ...) in
+// place (chunk-local prefixes) and stores the chunk's sums. With a single
+// chunk it writes the step's totals and zero chunk prefixes directly.
+__global__ __launch_bounds__(kBlock) void k_scan_chunks(StepParams p) {
+  __shared__ uint64_t s_tmp[kBlock / kWave];
+  constexpr int PT = kScanChunk / kBlock;
+  const uint64_t lo = blockIdx.x * (uint64_t)kScanChunk + threadIdx.x * PT;
+  uint64_t vs[PT], vn[PT], s = 0, nzs = 0, e = 0;
+#pragma unroll
+  for (int k = 0; k < PT; k++) {
+    const bool in = lo + k < p.ntiles;
+    vs[k] = in ? p.tile_sum[lo + k] : 0;
+    vn[k] = in ? p.tile_nz[lo + k] : 0;
+    if (p.mode != kModeK2U && in) e += p.tile_ex[lo + k];
+    s += vs[k];
+    nzs += vn[k];
+  }
+  uint64_t ts, tn, te = 0;
+  uint64_t rs = block_excl_scan(s, s_tmp, ts);
+  uint64_t rn = block_excl_scan(nzs, s_tmp, tn);
+  if (p.mode != kModeK2U) (void)block_excl_scan(e, s_tmp, te);
+#pragma unroll
+  for (int k = 0; k < PT; k++)
+    if (lo + k < p.ntiles) {
+      p.tile_sum[lo + k] = rs;
+      p.tile_nz[lo + k] = rn;
+      rs += vs[k];
+      rn += vn[k];
+    }
+  if (threadIdx.x == 0) {
+    if (gridDim.x == 1) {
+      p.totals[0] = ts;
+      p.totals[1] = te;
+      p.totals[2] = tn;
+      p.chunk_sum[0] = 0;
+      p.chunk_nz[0] = 0;
+    } else {
+      p.chunk_sum[blockIdx.x] = ts;
+      p.chunk_nz[blockIdx.x] = tn;
+      p.chunk_ex[blockIdx.x] = te;
+    }
+  }
+}
+
+// Level 2 (one block): chunk sums become chunk prefixes; writes the totals.
+__global__ __launch_bounds__(kBlock) void k_scan_top(StepParams p, uint64_t nchunks) {
+  __shared__ uint64_t s_tmp[kBlock / kWave];
+  const uint64_t rows = block_scan_array(p.chunk_sum, nchunks, s_tmp);
+  const uint64_t nz = block_scan_array(p.chunk_nz, nchunks, s_tmp);
+  uint64_t e = 0, ex = 0;
+  if (p.mode != kModeK2U)
+    for (uint64_t i = threadIdx.x; i < nchunks; i += kBlock) e += p.chunk_ex[i];
+  (void)block_excl_scan(e, s_tmp, ex);
+  if (threadIdx.x == 0) {
+    p.totals[0] = rows;
+    p.totals[1] = ex;
+    p.totals[2] = nz;
+  }
+}
+
 // Finalize pass over one tile: K2U moves the tile's rows with edges to the
 // compacted row list with their exclusive output offsets; the filters copy
 // the kept rows, in order, to the compacted output.
@@ -261,10 +331,13 @@ __device__ __forceinline__ void finalize_tile(const StepParams &p, uint64_t lt) 
   __shared__ uint64_t s_tmp[kBlock / kWave];
   __shared__ uint32_t s_sel[kBlock];
   const uint32_t tid = threadIdx.x;
-  const uint64_t nz0 = p.tile_nz[lt];
-  const uint64_t nz1 = lt + 1 < p.ntiles ? p.tile_nz[lt + 1] : p.totals[2];
-  const uint32_t nnz = (uint32_t)(nz1 - nz0);
-  const uint64_t out0 = p.tile_sum[lt];
+  auto nz_prefix = [&](uint64_t t) -> uint64_t {
+    if (t >= p.ntiles) return p.totals[2];
+    return p.tile_nz[t] + (p.chunk_nz ? p.chunk_nz[t / kScanChunk] : 0);
+  };
+  const uint64_t nz0 = nz_prefix(lt);
+  const uint32_t nnz = (uint32_t)(nz_prefix(lt + 1) - nz0);
+  const uint64_t out0 = p.tile_sum[lt] + (p.chunk_sum ? p.chunk_sum[lt / kScanChunk] : 0);
   const uint4 *hits = p.hits + lt * kTileRows;
   uint64_t carry = 0;
   for (uint32_t k0 = 0; k0 < nnz; k0 += kBlock) {
@@ -330,15 +403,18 @@ struct ExpandParams {
 // kExpandNV items of the merged sequence of row starts and output rows, so a
 // hub row's outputs spread over many workgroups. Output row o of compacted
 // row i is (input row, edge k) with k = o - cexcl[i]; rows come out in input
-// order and edges in list order (core/engine.hpp:142-150). Output cells are
-// written with consecutive lanes on consecutive ints. `tile` is the tile
-// index within the step.
+// order and edges in list order (core/engine.hpp:142-150).
+// Phase 1 stages the tile's rows in LDS; phase 2 walks the merge path and
+// issues each thread's (up to kExpandVT) edge loads together, parking the
+// edges in LDS; phase 3 writes the tile's contiguous output region with
+// aligned 16-byte stores. `tile` is the tile index within the step.
 template <bool FRAMED>
 __device__ __forceinline__ void expand_tile(const GraphView &g, const ExpandParams &a, uint64_t tile) {
   __shared__ int32_t s_ex[kExpandNV + 1];
   __shared__ uint32_t s_off[kExpandNV + 1];
   __shared__ uint32_t s_row[kExpandNV + 1];
   __shared__ uint16_t s_orow[kExpandNV];
+  __shared__ uint32_t s_edge[kExpandNV];
   const uint32_t tid = threadIdx.x;
   const uint64_t d0 = tile * (uint64_t)kExpandNV;
   const uint64_t dend = a.R + a.E;
@@ -363,6 +439,7 @@ __device__ __forceinline__ void expand_tile(const GraphView &g, const ExpandPara
   // local merged position of local row r's start item: (r - 1) + s_ex[r]
   const int32_t span = (int32_t)(d1 - d0);
   const int32_t x0 = (int32_t)(tid * kExpandVT);
+  const DevMeta m = g.meta[a.pid];
   if (x0 < span) {
     uint32_t lo = 1, hi = nr;  // rows r >= 1 whose start lies before x0
     while (lo < hi) {
@@ -370,35 +447,65 @@ __device__ __forceinline__ void expand_tile(const GraphView &g, const ExpandPara
       if ((int32_t)(mid - 1) + s_ex[mid] < x0) lo = mid + 1; else hi = mid;
     }
     uint32_t cur = lo - 1;
+    int32_t oidx[kExpandVT];
+    uint32_t epos[kExpandVT];
 #pragma unroll
     for (int v = 0; v < kExpandVT; v++) {
       const int32_t x = x0 + v;
-      if (x >= span) break;
-      if (cur + 1 < nr && (int32_t)cur + s_ex[cur + 1] == x) {
-        cur++;
-      } else {
-        s_orow[x - (int32_t)cur] = (uint16_t)cur;
+      oidx[v] = -1;
+      epos[v] = 0;
+      if (x < span) {
+        if (cur + 1 < nr && (int32_t)cur + s_ex[cur + 1] == x) {
+          cur++;
+        } else {
+          const int32_t o = x - (int32_t)cur;
+          oidx[v] = o;
+          epos[v] = s_off[cur] + (uint32_t)(o - s_ex[cur]);
+          s_orow[o] = (uint16_t)cur;
+        }
       }
     }
+    uint32_t ev[kExpandVT];
+#pragma unroll
+    for (int v = 0; v < kExpandVT; v++)
+      ev[v] = oidx[v] >= 0 ? edge_at<FRAMED>(g, m.edge_start, m.eframe_base, epos[v]) : 0u;
+#pragma unroll
+    for (int v = 0; v < kExpandVT; v++)
+      if (oidx[v] >= 0) s_edge[oidx[v]] = ev[v];
   }
   __syncthreads();
+  // the tile's output is the contiguous int range [j0 * nc, j1 * nc)
   const uint32_t nout = (uint32_t)(j1 - j0);
   const uint32_t c = a.c, nc = c + 1;
-  const uint32_t rpp = kBlock / nc;
-  if (tid < rpp * nc) {
-    const uint32_t col = tid % nc;
-    const DevMeta m = g.meta[a.pid];
-    for (uint32_t r = tid / nc; r < nout; r += rpp) {
-      const uint32_t lr = s_orow[r];
-      int32_t v;
-      if (col < c) {
-        v = a.T[(uint64_t)s_row[lr] * c + col];
-      } else {
-        const uint32_t k = (uint32_t)((int32_t)r - s_ex[lr]);
-        v = (int32_t)edge_at<FRAMED>(g, m.edge_start, m.eframe_base, (uint64_t)s_off[lr] + k);
+  const uint32_t nint = nout * nc;
+  const uint64_t g0 = j0 * nc;
+  auto cell = [&](uint32_t r, uint32_t col) -> int32_t {
+    return col < c ? a.T[(uint64_t)s_row[s_orow[r]] * c + col] : (int32_t)s_edge[r];
+  };
+  // streaming (nontemporal) stores: the table is far larger than the caches
+  typedef int32_t v4i __attribute__((ext_vector_type(4)));
+  const uint32_t head = (uint32_t)((4 - (g0 & 3)) & 3) < nint ? (uint32_t)((4 - (g0 & 3)) & 3) : nint;
+  if (tid < head) __builtin_nontemporal_store(cell(tid / nc, tid % nc), &a.O[g0 + tid]);
+  const uint32_t nquad = (nint - head) / 4;
+  v4i *oq = (v4i *)(a.O + g0 + head);
+  for (uint32_t q = tid; q < nquad; q += kBlock) {
+    const uint32_t e = head + 4 * q;
+    uint32_t r = e / nc, col = e - r * nc;
+    v4i v;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      v[k] = cell(r, col);
+      if (++col == nc) {
+        col = 0;
+        r++;
       }
-      a.O[(j0 + r) * nc + col] = v;
     }
+    __builtin_nontemporal_store(v, &oq[q]);
+  }
+  const uint32_t tail0 = head + 4 * nquad;
+  if (tid < nint - tail0) {
+    const uint32_t e = tail0 + tid;
+    __builtin_nontemporal_store(cell(e / nc, e % nc), &a.O[g0 + e]);
   }
   __syncthreads();  // LDS is reused by the caller's next tile
 }
@@ -412,9 +519,10 @@ __global__ __launch_bounds__(kBlock) void k_expand(GraphView g, ExpandParams a) 
 // out[1] = offset in the predicate segment. One wave.
 template <bool FRAMED>
 __global__ __launch_bounds__(kWave) void k_probe_one(GraphView g, uint64_t key, uint32_t pid, uint64_t *out) {
+  __shared__ WaveProbeLds s_wp;
   const DevMeta m = g.meta[pid];
   uint32_t deg, off;
-  wave_probe<FRAMED>(g, m, threadIdx.x == 0 ? key : 0, deg, off);
+  wave_probe<FRAMED>(g, m, threadIdx.x == 0 ? key : 0, s_wp, deg, off);
   if (threadIdx.x == 0) {
     out[0] = deg;
     out[1] = off;

@@ -260,11 +260,13 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
   // scratch: totals[4] | tile_sum | tile_nz | tile_ex | hits[ntiles * kTileRows] (16 B) |
   //          K2U: cinfo[R] (8 B) | cexcl[R]
   const uint64_t hit_words = 2 * ntiles * kTileRows;
-  const uint64_t scratch_words = 4 + 3 * ntiles + hit_words + (mode == kModeK2U ? 2 * R : 0);
+  const uint64_t nchunks = (ntiles + kScanChunk - 1) / kScanChunk;
+  const uint64_t scratch_words = 4 + 3 * ntiles + 3 * nchunks + hit_words + (mode == kModeK2U ? 2 * R : 0);
   uint64_t *scr = (uint64_t *)pool->alloc(scratch_words * 8, s, err);
   if (!scr) return WKG_ENOMEM;
   uint64_t *totals = scr, *tile_sum = scr + 4, *tile_nz = tile_sum + ntiles, *tile_ex = tile_nz + ntiles;
-  uint4 *hits = (uint4 *)(tile_ex + ntiles);
+  uint64_t *chunks = tile_ex + ntiles;
+  uint4 *hits = (uint4 *)(chunks + 3 * nchunks);
   uint2 *cinfo = (uint2 *)((uint64_t *)hits + hit_words);
   uint64_t *cexcl = (uint64_t *)(cinfo + R);
 
@@ -296,6 +298,9 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
   p.tile_ex = tile_ex;
   p.totals = totals;
   p.ntiles = ntiles;
+  p.chunk_sum = chunks;
+  p.chunk_nz = chunks + nchunks;
+  p.chunk_ex = chunks + 2 * nchunks;
   const GraphView g = view();
   {
     // start cell (+ end cell for K2K) and one 128-byte bucket per row
@@ -304,8 +309,9 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
     launch_probe(framed, sorted || mode == kModeK2U, g, p, s);
   }
   {
-    ProfScope ps(this, s, "scan_tiles", 24.0 * (double)ntiles);
-    k_scan_tiles<<<1, kBlock, 0, s>>>(p);
+    ProfScope ps(this, s, "scan_tiles", 48.0 * (double)ntiles);
+    k_scan_chunks<<<(unsigned)nchunks, kBlock, 0, s>>>(p);
+    if (nchunks > 1) k_scan_top<<<1, kBlock, 0, s>>>(p, nchunks);
   }
   {
     ProfScope ps(this, s, mode == kModeK2U ? "finalize" : "compact", 0.0);
