@@ -97,6 +97,14 @@ def main():
 
     n_exp, ms_exp, b_exp = kernels["expand"]
     achieved = b_exp / (ms_exp * 1e-3) / 1e9 if ms_exp > 0 else 0.0
+    # HBM bytes per expand launch from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE summary of this workload (scaled by a copy kernel of known size)
+    traffic, traffic_src = None, None
+    tp = os.path.join(ROOT, "profiles", "expand_heavy_query.traffic.json")
+    if os.path.exists(tp):
+        with open(tp) as f:
+            t = json.load(f)
+        traffic, traffic_src = t["hbm_bytes_per_launch"], t["source"]
     gpu_kernel_ms = sum(v[1] for v in kernels.values()) / args.steps
 
     if args.heavy_only:
@@ -166,10 +174,10 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": "expand (k_expand)", "launches_per_query": n_exp / args.steps,
-            "alg_bytes_per_query": b_exp / args.steps, "kernel_ms_per_query": ms_exp / args.steps,
-            "copy_kernel_gbs": copy_gbs,
+            "alg_bytes_per_launch": b_exp / max(n_exp, 1), "avg_launch_ms": ms_exp / max(n_exp, 1),
+            "traffic_source": traffic_src, "copy_kernel_gbs": copy_gbs,
         },
         "cpu_baseline": cpu,
         "batched": {
