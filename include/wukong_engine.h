@@ -51,20 +51,40 @@ int wke_run_query(wke_engine *e, const int64_t *cmd, int ncmd, wke_result *res);
  * query on the GPU at a time, core/gpu.hpp:53-61). */
 int wke_run_plain(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, wke_result *res);
 
-/* Batched path: all queries advance together, one fused launch pair per
- * round covering the current step of every unfinished query. */
+/* Batched path: the queries are compiled into device plans and run by the
+ * device-resident executor (wkg_batch_*, include/wukong_gpu.h): every query
+ * advances through all its steps on the device inside one captured launch
+ * sequence, max_queries at a time. A query that does not fit the executor's
+ * arena is rerun on the plain path, so every result matches wke_run_query. */
 int wke_run_batched(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, wke_result *res);
 
+/* Step-synchronous batched path: all queries advance one step per round, the
+ * current steps of all unfinished queries fused into one wkg_step_batch. */
+int wke_run_rounds(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, wke_result *res);
+
+/* Executor shape used by wke_run_batched and wke_serve (defaults: 1024
+ * queries, 8 patterns, 16 GiB arena per executor). Drops existing executors. */
+int wke_configure(wke_engine *e, int max_queries, int max_patterns, uint64_t arena_bytes);
+
+/* Runs the autotuner of the batched executor on these queries (reps timed
+ * runs per candidate); the chosen shape stays in effect for the engine. */
+int wke_autotune(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, int reps,
+                 wkg_batch_tuning *best, double *best_ms);
+
 /* Served path: an open-loop arrival stream. Query q arrives at arrival_s[q]
- * (seconds from the start, ascending). Arrived queries are admitted while
- * their estimated device tables fit table_budget_bytes; admitted queries
- * advance together one step per round (continuous batching: a query joins at
- * the next round after it is admitted and leaves when its last step is done).
- * Heavy queries (flag heavy[q] != 0, may be NULL) run on a second stream in
- * their own rounds, so light rounds overlap them on the device.
- * latency_s in each result is completion - arrival. */
+ * (seconds from the start, ascending). Arrived queries wait in a light or a
+ * heavy queue (heavy[q] != 0; heavy may be NULL). Light queries run on
+ * `light_lanes` device executors, heavy queries on one more, each with its
+ * own stream and arena; the arenas together are the table budget
+ * (table_budget_bytes; half to the heavy lane, the rest split over the light
+ * lanes). Whenever a lane is free, every query waiting in its queue (up to
+ * the executor's capacity) is admitted as one device batch, so work that
+ * arrives while other batches run overlaps them on the device. A query that
+ * does not fit its lane's arena beside the others is retried alone, then on
+ * the plain path. latency_s in each result is completion - arrival. */
 int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, const double *arrival_s,
-              const uint8_t *heavy, uint64_t table_budget_bytes, wke_result *res, double *wall_s);
+              const uint8_t *heavy, uint64_t table_budget_bytes, int light_lanes, wke_result *res,
+              double *wall_s);
 
 #ifdef __cplusplus
 }

@@ -163,6 +163,72 @@ typedef struct wkg_step_desc {
 int wkg_step_batch(wkg_ctx *ctx, int n, const wkg_step_desc *steps, void *stream,
                    wkg_table **out, uint64_t *expanded);
 
+/* ---- device-resident batched execution -----------------------------------------------
+ * Whole queries, not single steps: every query of a batch advances through
+ * all of its pattern steps on the device, with no host round trip between
+ * steps. A query is a compiled plan, one wkg_plan_step per pattern (the step
+ * GPU_Engine::execute_one_step would dispatch, core/gpu_engine.hpp:395-467).
+ * Per step level, a single-block plan kernel sizes every query's work and
+ * carves its tables and scratch out of an HBM arena; persistent kernels then
+ * take the probe, finalize and expand tiles of all queries from one shared
+ * work queue (an atomic tile ticket over the concatenated tile space). The
+ * whole sequence -- plan upload, every level, the gather of all result tables
+ * and the read-back of the result records -- is captured once in a hipGraph
+ * and replayed per batch. Results are the same, cell for cell, as the plain
+ * per-step calls give. A query whose tables do not fit the arena ends with
+ * status WKG_ENOMEM and can be rerun on the plain path. */
+typedef struct wkg_batch wkg_batch;
+
+typedef struct wkg_plan_step {
+  int32_t kind;      /* WKG_STEP_* (I2U / C2U only as a query's first step) */
+  int32_t pid, dir;  /* predicate (index id for I2U), 0 = IN / 1 = OUT */
+  int32_t start_col, end_col;
+  int32_t pad;
+  int64_t vid;       /* C2U start constant / K2C end constant */
+} wkg_plan_step;
+
+/* Launch and tile shapes (0 = default). The autotuner below picks them. */
+typedef struct wkg_batch_tuning {
+  int32_t probe_wg_per_cu;   /* persistent probe / finalize grid, workgroups per CU */
+  int32_t expand_wg_per_cu;  /* persistent expand grid */
+  int32_t aux_wg_per_cu;     /* tile scan, merge-path partition, gather */
+  int32_t probe_tile_rows;   /* rows per probe tile: 1024 or 2048 */
+  int32_t expand_tile_items; /* merged items per expand tile: 1024 or 2048 */
+  int32_t pad;
+} wkg_batch_tuning;
+
+typedef struct wkg_batch_result {
+  uint64_t offset;   /* first cell of the query's table in the returned cells */
+  uint64_t rows;
+  int32_t cols;
+  int32_t status;    /* WKG_OK, or WKG_ENOMEM when the arena was full */
+  uint64_t expanded; /* work count summed over the steps */
+} wkg_batch_result;
+
+/* max_queries and max_patterns bound a batch; arena_bytes of HBM (taken from
+ * the table budget) hold all intermediate tables, step scratch and the
+ * gathered results. stream may be NULL (a stream of the batch's own). */
+int wkg_batch_create(wkg_ctx *ctx, int max_queries, int max_patterns, uint64_t arena_bytes,
+                     void *stream, wkg_batch **out);
+int wkg_batch_destroy(wkg_batch *b);
+int wkg_batch_set_tuning(wkg_batch *b, const wkg_batch_tuning *t);
+int wkg_batch_get_tuning(const wkg_batch *b, wkg_batch_tuning *t);
+/* Starts nq <= max_queries queries and returns at once; query q's steps are
+ * steps[step_off[q] .. step_off[q+1]). The arrays are copied. */
+int wkg_batch_launch(wkg_batch *b, int nq, const wkg_plan_step *steps, const int32_t *step_off);
+/* 1 while the launched batch runs, 0 once it is done. */
+int wkg_batch_busy(wkg_batch *b);
+/* Waits for the launched batch. res[nq] gets the records; *cells (malloc'd,
+ * release with free()) holds all result tables back to back, *ncells cells. */
+int wkg_batch_collect(wkg_batch *b, wkg_batch_result *res, int32_t **cells, uint64_t *ncells);
+/* launch + collect */
+int wkg_batch_run(wkg_batch *b, int nq, const wkg_plan_step *steps, const int32_t *step_off,
+                  wkg_batch_result *res, int32_t **cells, uint64_t *ncells);
+/* Autotuner: times `reps` runs of the given batch under each candidate
+ * launch/tile shape, keeps the fastest (median) and returns it. */
+int wkg_batch_autotune(wkg_batch *b, int nq, const wkg_plan_step *steps, const int32_t *step_off,
+                       int reps, wkg_batch_tuning *best, double *best_ms);
+
 /* ---- measurement ---------------------------------------------------------------------
  * The reference has printf timers around asynchronous launches only
  * (core/rcache.hpp:300-385, BREAK_DOWN). Here each kernel launch can be

@@ -1,0 +1,202 @@
+"""Device-resident batched executor (wkg_batch_*) and the served lanes.
+
+Every query's table and work count must equal the plain per-step path's and
+the CPU oracle's, whatever the launch/tile shape, the executor's capacity or
+arena, and however often the captured launch sequence is replayed."""
+import numpy as np
+import pytest
+
+import oracle
+from tests.conftest import VB, random_queries
+from wukonggpu_amd import (STEP_C2U, STEP_I2U, STEP_K2C, STEP_K2K, STEP_K2U, Batch, DeviceError, GPUEngine)
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [dict(probe_tile_rows=tr, expand_tile_items=ni, probe_wg_per_cu=pw, expand_wg_per_cu=xw, aux_wg_per_cu=aw)
+          for tr, ni, pw, xw, aw in ((1024, 1024, 1, 1, 1), (1024, 2048, 2, 4, 2), (2048, 1024, 4, 2, 1),
+                                     (2048, 2048, 2, 2, 1))]
+
+
+@pytest.fixture(scope="module")
+def eng_small(dev_small):
+    e = GPUEngine(dev_small)
+    yield e
+    e.close()
+
+
+def _same(gs, q, r):
+    want, wex, _ = oracle.run_query(gs, q)
+    assert r.status == 0, (q, r.status)
+    assert r.table.shape[0] == want.shape[0], q
+    if want.size:
+        assert np.array_equal(r.table, want), q
+    assert r.expanded == wex, q
+
+
+def test_executor_random_queries_match_oracle(eng_small, graph_small):
+    _, gs = graph_small
+    rng = np.random.default_rng(21)
+    qs = random_queries(rng, 300, 4, [6, 7], 300, max_steps=6)
+    eng_small.configure(max_queries=64, max_patterns=8, arena_bytes=1 << 30)  # several launches
+    print("executor: 300 random queries, 64 per launch", flush=True)
+    for q, r in zip(qs, eng_small.run_batched(qs)):
+        _same(gs, q, r)
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"tr{s['probe_tile_rows']}-nv{s['expand_tile_items']}")
+def test_executor_shapes_match_plain(dev_small, graph_small, shape):
+    _, gs = graph_small
+    rng = np.random.default_rng(22)
+    qs = random_queries(rng, 300, 4, [6, 7], 120, max_steps=5)
+    e = GPUEngine(dev_small)
+    b = Batch(dev_small, 128, 6, 1 << 30)
+    b.set_tuning(**shape)
+    assert b.tuning() == shape
+    plans = [_plan(q) for q in qs]
+    print(f"executor shape {shape}", flush=True)
+    for q, (st, tab, ex) in zip(qs, b.run(plans)):
+        alone = e.run_query(q)
+        assert st == 0, q
+        assert tab.shape == alone.table.shape and np.array_equal(tab, alone.table), q
+        assert ex == alone.expanded, q
+    b.close()
+    e.close()
+
+
+def test_executor_hub_rows_and_big_index(graph_hubs):
+    from wukonggpu_amd import Device
+    _, gs = graph_hubs
+    dev = Device(0)
+    dev.load(gs)
+    e = GPUEngine(dev)
+    e.configure(max_queries=32, max_patterns=6, arena_bytes=1 << 30)
+    qs = [[2, 1, 0, -1, -1, 2, 1, -2],                       # 3000-member type index, fan-out
+          [2, 1, 0, -1, -1, 2, 1, -2, -2, 3, 0, -3],          # through the hubs
+          [VB + 0, 2, 1, -1, -1, 3, 0, -2],                    # hub constant start
+          [2, 1, 0, -1, -1, 2, 1, -2, -2, 2, 0, -1],          # closing known_to_known
+          [3, 0, 1, -1, -1, 2, 1, -2, -1, 3, 1, VB + 5]]      # predicate index + known_to_const
+    rng = np.random.default_rng(23)
+    qs += random_queries(rng, 6000, 3, [2, 3], 60, max_steps=4)
+    print("executor on the hub graph", flush=True)
+    for q, r in zip(qs, e.run_batched(qs)):
+        _same(gs, q, r)
+    e.close()
+    dev.close()
+
+
+def test_executor_arena_exhaustion_falls_back(dev_small, graph_small):
+    _, gs = graph_small
+    rng = np.random.default_rng(24)
+    qs = random_queries(rng, 300, 4, [6, 7], 100, max_steps=5)
+    e = GPUEngine(dev_small)
+    e.configure(max_queries=100, max_patterns=6, arena_bytes=1 << 20)  # 1 MiB: most queries do not fit
+    b = Batch(dev_small, 100, 6, 1 << 20)
+    recs = b.run([_plan(q) for q in qs])
+    assert any(st == -2 for st, _, _ in recs), "expected some queries to exceed a 1 MiB arena"
+    assert any(st == 0 for st, _, _ in recs)
+    b.close()
+    print("executor with a 1 MiB arena (fallback to the plain path)", flush=True)
+    for q, r in zip(qs, e.run_batched(qs)):
+        _same(gs, q, r)
+    e.close()
+
+
+def test_batch_replays_with_eager_work_between(dev_small, graph_small, eng_small):
+    """The captured sequence replayed several times, with plain-path work and
+    synchronisations between replays, and a smaller second batch."""
+    _, gs = graph_small
+    cmds = [[6, 1, 0, -1, -1, 2, 1, -2],
+            [VB + 5, 3, 1, -1, -1, 2, 0, -2, -2, 2, 1, -1],
+            [2, 0, 1, -1, -1, 3, 1, VB + 7],
+            [7, 1, 0, -1, -1, 3, 1, -2, -2, 4, 0, -3, -1, 2, 1, -3]]
+    plans = [_plan(c) for c in cmds]
+    want = [eng_small.run_query(c) for c in cmds]
+    b = Batch(dev_small, 8, 4, 64 << 20)
+    for rep in range(4):
+        n = len(cmds) if rep % 2 == 0 else 2
+        print(f"replay {rep}: {n} queries", flush=True)
+        got = b.run(plans[:n])
+        for i in range(n):
+            st, tab, ex = got[i]
+            assert st == 0 and np.array_equal(tab, want[i].table) and ex == want[i].expanded, (rep, i)
+        eng_small.run_query(cmds[rep % len(cmds)])  # eager work on another stream between replays
+    b.close()
+
+
+def test_batch_rejects_bad_plans(dev_small):
+    b = Batch(dev_small, 4, 3, 16 << 20)
+    bad = [
+        [[dict(kind=STEP_K2U, pid=2, dir=1, start_col=0)]],                        # no start
+        [[dict(kind=STEP_I2U, pid=6, dir=0), dict(kind=STEP_K2U, pid=2, dir=1, start_col=1)]],  # column 1 of 1
+        [[dict(kind=STEP_I2U, pid=6, dir=0), dict(kind=STEP_I2U, pid=6, dir=0)]],  # index start at step 1
+        [[dict(kind=STEP_I2U, pid=6, dir=0)] + [dict(kind=STEP_K2U, pid=2, dir=1, start_col=0)] * 3],  # 4 > 3
+        [[dict(kind=STEP_C2U, pid=2, dir=1, vid=-5)]],                             # constant out of range
+        [[dict(kind=STEP_I2U, pid=6, dir=2)]],                                     # direction
+        [[dict(kind=STEP_I2U, pid=6, dir=0)]] * 5,                                 # 5 > 4 queries
+    ]
+    for plans in bad:
+        with pytest.raises(DeviceError):
+            b.run(plans)
+    ok = b.run([[dict(kind=STEP_I2U, pid=6, dir=0)]])  # still usable
+    assert ok[0][0] == 0
+    b.close()
+
+
+def test_served_lanes_alone_equals_in_mix(eng_small, graph_small):
+    _, gs = graph_small
+    rng = np.random.default_rng(25)
+    qs = random_queries(rng, 300, 4, [6, 7], 200, max_steps=6)
+    heavy = np.array([len(q) >= 20 for q in qs], dtype=np.uint8)
+    arrivals = np.cumsum(rng.exponential(0.0002, size=len(qs)))
+    eng_small.configure(max_queries=256, max_patterns=8, arena_bytes=1 << 30)
+    print("served: 200 arrivals, 2 light lanes + heavy lane", flush=True)
+    res, wall = eng_small.serve(qs, arrivals, heavy=heavy, table_budget_bytes=2 << 30, light_lanes=2)
+    assert wall > 0
+    for q, r in zip(qs, res):
+        alone = eng_small.run_query(q)
+        assert r.status == 0 and r.table.shape == alone.table.shape and np.array_equal(r.table, alone.table), q
+        assert r.expanded == alone.expanded, q
+        assert r.latency_s > 0
+    # a budget too small for most batches still answers every query
+    res2, _ = eng_small.serve(qs[:60], arrivals[:60] * 0, heavy=heavy[:60], table_budget_bytes=4 << 20,
+                              light_lanes=2)
+    for q, r in zip(qs[:60], res2):
+        _same(gs, q, r)
+
+
+def test_autotune_keeps_answers(dev_small, graph_small):
+    _, gs = graph_small
+    rng = np.random.default_rng(26)
+    qs = random_queries(rng, 300, 4, [6, 7], 64, max_steps=5)
+    e = GPUEngine(dev_small)
+    e.configure(max_queries=64, max_patterns=6, arena_bytes=256 << 20)
+    print("autotune over 64 queries", flush=True)
+    best, ms = e.autotune(qs, reps=1)
+    assert best["probe_tile_rows"] in (1024, 2048) and best["expand_tile_items"] in (1024, 2048)
+    assert best["probe_wg_per_cu"] >= 1 and ms > 0
+    for q, r in zip(qs, e.run_batched(qs)):
+        _same(gs, q, r)
+    e.close()
+
+
+# ---- cmd_chains -> plan (GPU_Engine::execute_one_step's dispatch, for the direct Batch tests)
+
+def _plan(cmd):
+    steps, cols = [], 0
+    for i in range(0, len(cmd), 4):
+        s, p, d, e = (int(x) for x in cmd[i:i + 4])
+        known = lambda v: v < 0 and -v <= cols  # noqa: E731
+        if i == 0 and 1 < s < (1 << 17):
+            steps.append(dict(kind=STEP_I2U, pid=s, dir=d))
+            cols = 1
+        elif s >= 0:
+            steps.append(dict(kind=STEP_C2U, pid=p, dir=d, vid=s))
+            cols = 1
+        elif e >= 0:
+            steps.append(dict(kind=STEP_K2C, pid=p, dir=d, start_col=-s - 1, vid=e))
+        elif known(e):
+            steps.append(dict(kind=STEP_K2K, pid=p, dir=d, start_col=-s - 1, end_col=-e - 1))
+        else:
+            steps.append(dict(kind=STEP_K2U, pid=p, dir=d, start_col=-s - 1))
+            cols += 1
+    return steps

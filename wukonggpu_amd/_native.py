@@ -39,6 +39,23 @@ class StepDesc(C.Structure):
                 ("pad", i32), ("vid", i64), ("input", vp)]
 
 
+class PlanStep(C.Structure):
+    _fields_ = [("kind", i32), ("pid", i32), ("dir", i32), ("start_col", i32), ("end_col", i32),
+                ("pad", i32), ("vid", i64)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("probe_wg_per_cu", i32), ("expand_wg_per_cu", i32), ("aux_wg_per_cu", i32),
+                ("probe_tile_rows", i32), ("expand_tile_items", i32), ("pad", i32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "pad"}
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("offset", u64), ("rows", u64), ("cols", i32), ("status", i32), ("expanded", u64)]
+
+
 class Result(C.Structure):
     _fields_ = [("table", P(i32)), ("rows", u64), ("cols", i32), ("status", i32),
                 ("expanded", u64), ("latency_s", dbl)]
@@ -117,6 +134,15 @@ def hip_lib():
         _sig(L, "wkg_profile_enable", C.c_int, vp, C.c_int)
         _sig(L, "wkg_profile_read", C.c_int, vp, C.c_char_p, P(u64), P(dbl), P(dbl))
         _sig(L, "wkg_bandwidth_probe", C.c_int, vp, u64, C.c_int, P(dbl))
+        _sig(L, "wkg_batch_create", C.c_int, vp, C.c_int, C.c_int, u64, vp, P(vp))
+        _sig(L, "wkg_batch_destroy", C.c_int, vp)
+        _sig(L, "wkg_batch_set_tuning", C.c_int, vp, P(BatchTuning))
+        _sig(L, "wkg_batch_get_tuning", C.c_int, vp, P(BatchTuning))
+        _sig(L, "wkg_batch_launch", C.c_int, vp, C.c_int, P(PlanStep), P(i32))
+        _sig(L, "wkg_batch_busy", C.c_int, vp)
+        _sig(L, "wkg_batch_collect", C.c_int, vp, P(BatchResult), P(P(i32)), P(u64))
+        _sig(L, "wkg_batch_run", C.c_int, vp, C.c_int, P(PlanStep), P(i32), P(BatchResult), P(P(i32)), P(u64))
+        _sig(L, "wkg_batch_autotune", C.c_int, vp, C.c_int, P(PlanStep), P(i32), C.c_int, P(BatchTuning), P(dbl))
         _hip = L
     return _hip
 
@@ -133,6 +159,10 @@ def engine_lib():
         _sig(L, "wke_run_query", C.c_int, vp, P(i64), C.c_int, P(Result))
         _sig(L, "wke_run_plain", C.c_int, vp, P(i64), P(i64), C.c_int, P(Result))
         _sig(L, "wke_run_batched", C.c_int, vp, P(i64), P(i64), C.c_int, P(Result))
-        _sig(L, "wke_serve", C.c_int, vp, P(i64), P(i64), C.c_int, P(dbl), P(C.c_uint8), u64, P(Result), P(dbl))
+        _sig(L, "wke_run_rounds", C.c_int, vp, P(i64), P(i64), C.c_int, P(Result))
+        _sig(L, "wke_configure", C.c_int, vp, C.c_int, C.c_int, u64)
+        _sig(L, "wke_autotune", C.c_int, vp, P(i64), P(i64), C.c_int, C.c_int, P(BatchTuning), P(dbl))
+        _sig(L, "wke_serve", C.c_int, vp, P(i64), P(i64), C.c_int, P(dbl), P(C.c_uint8), u64, C.c_int,
+             P(Result), P(dbl))
         _engine = L
     return _engine

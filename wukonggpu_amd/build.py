@@ -42,11 +42,24 @@ def build_gstore():
     return out
 
 
+HIP_SOURCES = ("wkg_hip.hip", "graph.hip", "batch.hip", "exec.hip")
+
+
 def build_hip():
+    """Compiles each HIP source to an object in parallel, then links."""
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(LIB, exist_ok=True)
+    obj_dir = os.path.join(LIB, "obj")
+    os.makedirs(obj_dir, exist_ok=True)
     out = os.path.join(LIB, "libwkg_hip.so")
-    srcs = [os.path.join(CSRC, f) for f in ("wkg_hip.hip", "graph.hip", "batch.hip")]
-    _run([HIPCC, *HIP_FLAGS, f"-I{INC}", *srcs, "-o", out])
+    cflags = [f for f in HIP_FLAGS if f != "-shared"]
+    jobs = []
+    for f in HIP_SOURCES:
+        obj = os.path.join(obj_dir, f.replace(".hip", ".o"))
+        jobs.append(([HIPCC, *cflags, f"-I{INC}", "-c", os.path.join(CSRC, f), "-o", obj], obj))
+    with ThreadPoolExecutor(max_workers=min(len(jobs), 4)) as ex:
+        list(ex.map(lambda j: _run(j[0]), jobs))
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *[o for _, o in jobs], "-o", out])
     return out
 
 

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_probe(GraphView g, const Batch
   if (threadIdx.x == 0) s_q = find_item(tile_base, n, blockIdx.x);
   __syncthreads();
   const uint32_t q = s_q;
-  probe_tile<SORTED, FRAMED>(g, items[q].p, blockIdx.x - tile_base[q]);
+  probe_tile<kTileRows, SORTED, FRAMED>(g, items[q].p, blockIdx.x - tile_base[q]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_batch_scan(const BatchItem *items) {
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_finalize(const BatchItem *item
   if (threadIdx.x == 0) s_q = find_item(tile_base, n, blockIdx.x);
   __syncthreads();
   const uint32_t q = s_q;
-  finalize_tile(items[q].p, blockIdx.x - tile_base[q]);
+  finalize_tile<kTileRows>(items[q].p, blockIdx.x - tile_base[q]);
 }
 
 __global__ void k_batch_partition(const BatchItem *items, const uint64_t *part_base, uint32_t n, uint64_t *part) {
@@ -62,7 +62,7 @@ __global__ void k_batch_partition(const BatchItem *items, const uint64_t *part_b
   if (k >= part_base[n]) return;
   const uint32_t q = find_item(part_base, n, k);
   const ExpandParams &x = items[q].x;
-  part[k] = merge_split(x.cexcl, x.R, x.E, k - part_base[q]);
+  part[k] = merge_split<kExpandNV>(x.cexcl, x.R, x.E, k - part_base[q]);
 }
 
 template <bool FRAMED>
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_expand(GraphView g, const Batc
   if (threadIdx.x == 0) s_q = find_item(xtile_base, n, blockIdx.x);
   __syncthreads();
   const uint32_t q = s_q;
-  expand_tile<FRAMED>(g, items[q].x, blockIdx.x - xtile_base[q]);
+  expand_tile<kExpandVT, FRAMED>(g, items[q].x, blockIdx.x - xtile_base[q]);
 }
 
 }  // namespace

@@ -40,20 +40,145 @@ struct Request {
   bool done = false;
 
   bool is_finished() const { return step * 4 >= ncmd; }
-  bool start_from_index() const { return is_tpid(cmd[0]); }
-  VarType variable_type(int64_t v) const {
+  static int64_t var2column(int64_t v) { return -v - 1; }
+};
+
+// execute_one_step's dispatch (core/gpu_engine.hpp:395-467) for pattern
+// `step` of cmd when col_num columns are bound: the step kind, its columns
+// and the column count after it.
+int dispatch(const int64_t *cmd, int step, int col_num, wkg_step_desc *d, int *new_cols, std::string &err) {
+  memset(d, 0, sizeof *d);
+  auto fail = [&](const char *m) {
+    err = m;
+    return WKG_EINVAL;
+  };
+  auto var_type = [&](int64_t v) {
     if (v >= 0) return kConst;
     if (-v > col_num) return kUnknown;
     return kKnown;
+  };
+  const int64_t *c = cmd + 4 * step;
+  const int64_t start = c[0], pid = c[1], dir = c[2], end = c[3];
+  if (dir != 0 && dir != 1) return fail("direction must be IN(0) or OUT(1)");
+  if (step == 0 && is_tpid(cmd[0])) {
+    if (!(pid == 0 || pid == 1)) return fail("index start needs __PREDICATE__ or rdf:type");
+    if (!(col_num == 0 && Request::var2column(end) == 0)) return fail("wrong query plan");
+    d->kind = WKG_STEP_I2U;
+    d->pid = (int32_t)start;
+    d->dir = (int32_t)dir;
+    *new_cols = 1;
+    return WKG_OK;
   }
-  static int64_t var2column(int64_t v) { return -v - 1; }
-};
+  if (pid < 0) return fail("predicate variables are not supported");
+  if (pid >= (1 << 17)) return fail("predicate id out of range");
+  const VarType ts = var_type(start), te = var_type(end);
+  d->pid = (int32_t)pid;
+  d->dir = (int32_t)dir;
+  if (ts == kConst && te == kUnknown) {
+    if (!(col_num == 0 && Request::var2column(end) == 0)) return fail("wrong query plan");
+    d->kind = WKG_STEP_C2U;
+    d->vid = start;
+    *new_cols = 1;
+  } else if (ts == kKnown && te == kConst) {
+    d->kind = WKG_STEP_K2C;
+    d->start_col = (int32_t)Request::var2column(start);
+    d->vid = end;
+    *new_cols = col_num;
+  } else if (ts == kKnown && te == kKnown) {
+    d->kind = WKG_STEP_K2K;
+    d->start_col = (int32_t)Request::var2column(start);
+    d->end_col = (int32_t)Request::var2column(end);
+    *new_cols = col_num;
+  } else if (ts == kKnown && te == kUnknown) {
+    if (col_num != Request::var2column(end)) return fail("wrong query plan");
+    d->kind = WKG_STEP_K2U;
+    d->start_col = (int32_t)Request::var2column(start);
+    *new_cols = col_num + 1;
+  } else {
+    return fail("unsupported triple pattern");
+  }
+  return WKG_OK;
+}
+
+// A whole query as a device plan: the dispatch of every step, with the
+// column count carried from step to step.
+int compile_query(const int64_t *cmd, int ncmd, std::vector<wkg_plan_step> &out, std::string &err) {
+  out.clear();
+  if (!cmd || ncmd <= 0 || ncmd % 4) {
+    err = "a query is 4 x int64 per pattern";
+    return WKG_EINVAL;
+  }
+  int cols = 0;
+  for (int step = 0; step * 4 < ncmd; step++) {
+    wkg_step_desc d;
+    int nc;
+    if (int rc = dispatch(cmd, step, cols, &d, &nc, err)) return rc;
+    wkg_plan_step p{};
+    p.kind = d.kind;
+    p.pid = d.pid;
+    p.dir = d.dir;
+    p.start_col = d.start_col;
+    p.end_col = d.end_col;
+    p.vid = d.vid;
+    out.push_back(p);
+    cols = nc;
+  }
+  return WKG_OK;
+}
+
+// Copies batch record r (cells of the whole batch) into a wke_result.
+void fill_result(const wkg_batch_result &r, const int32_t *cells, wke_result *out) {
+  out->status = r.status;
+  out->expanded = r.expanded;
+  out->rows = r.status == WKG_OK ? r.rows : 0;
+  out->cols = r.cols;
+  out->table = nullptr;
+  const uint64_t n = out->rows * (uint64_t)(r.cols > 0 ? r.cols : 0);
+  if (r.status == WKG_OK && n) {
+    out->table = (int32_t *)malloc(n * 4);
+    if (!out->table) {
+      out->status = WKG_ENOMEM;
+      out->rows = 0;
+    } else {
+      memcpy(out->table, cells + r.offset, n * 4);
+    }
+  }
+}
 
 }  // namespace
 
 struct wke_engine {
   wkg_ctx *ctx = nullptr;
   std::string err;
+  // device-resident executor shape (wke_configure) and tuning (wke_autotune)
+  int cfg_queries = 1024, cfg_patterns = 8;
+  uint64_t cfg_arena = 16ull << 30;
+  bool tuned = false;
+  wkg_batch_tuning tuning{};
+  wkg_batch *exec = nullptr;
+
+  ~wke_engine() { drop_exec(); }
+  void drop_exec() {
+    if (exec) wkg_batch_destroy(exec);
+    exec = nullptr;
+  }
+  // A new executor with the engine's shape and tuning (arena_bytes 0: the default).
+  int new_exec(uint64_t arena_bytes, wkg_batch **out) {
+    int rc = wkg_batch_create(ctx, cfg_queries, cfg_patterns, arena_bytes ? arena_bytes : cfg_arena, nullptr, out);
+    if (rc) return fail_ctx(rc);
+    if (tuned && (rc = wkg_batch_set_tuning(*out, &tuning))) {
+      wkg_batch_destroy(*out);
+      *out = nullptr;
+      return fail_ctx(rc);
+    }
+    return WKG_OK;
+  }
+  int get_exec(wkg_batch **out) {
+    if (!exec)
+      if (int rc = new_exec(0, &exec)) return rc;
+    *out = exec;
+    return WKG_OK;
+  }
   int fail(int code, const std::string &m) {
     err = m;
     return code;
@@ -63,51 +188,11 @@ struct wke_engine {
     return code;
   }
 
-  // The step descriptor of r's current step (execute_one_step's dispatch).
+  // The step descriptor of r's current step.
   int plan(const Request &r, wkg_step_desc *d, int *new_cols) {
-    memset(d, 0, sizeof *d);
-    const int64_t *c = r.cmd + 4 * r.step;
-    const int64_t start = c[0], pid = c[1], dir = c[2], end = c[3];
-    if (dir != 0 && dir != 1) return fail(WKG_EINVAL, "direction must be IN(0) or OUT(1)");
-    if (r.step == 0 && r.start_from_index()) {
-      if (!(pid == 0 || pid == 1)) return fail(WKG_EINVAL, "index start needs __PREDICATE__ or rdf:type");
-      if (!(r.col_num == 0 && Request::var2column(end) == 0)) return fail(WKG_EINVAL, "wrong query plan");
-      d->kind = WKG_STEP_I2U;
-      d->pid = (int32_t)start;
-      d->dir = (int32_t)dir;
-      *new_cols = 1;
-      return WKG_OK;
-    }
-    if (pid < 0) return fail(WKG_EINVAL, "predicate variables are not supported");
-    if (pid >= (1 << 17)) return fail(WKG_EINVAL, "predicate id out of range");
-    const VarType ts = r.variable_type(start), te = r.variable_type(end);
-    d->pid = (int32_t)pid;
-    d->dir = (int32_t)dir;
+    int rc = dispatch(r.cmd, r.step, r.col_num, d, new_cols, err);
     d->in = r.t;
-    if (ts == kConst && te == kUnknown) {
-      if (!(r.col_num == 0 && Request::var2column(end) == 0)) return fail(WKG_EINVAL, "wrong query plan");
-      d->kind = WKG_STEP_C2U;
-      d->vid = start;
-      *new_cols = 1;
-    } else if (ts == kKnown && te == kConst) {
-      d->kind = WKG_STEP_K2C;
-      d->start_col = (int32_t)Request::var2column(start);
-      d->vid = end;
-      *new_cols = r.col_num;
-    } else if (ts == kKnown && te == kKnown) {
-      d->kind = WKG_STEP_K2K;
-      d->start_col = (int32_t)Request::var2column(start);
-      d->end_col = (int32_t)Request::var2column(end);
-      *new_cols = r.col_num;
-    } else if (ts == kKnown && te == kUnknown) {
-      if (r.col_num != Request::var2column(end)) return fail(WKG_EINVAL, "wrong query plan");
-      d->kind = WKG_STEP_K2U;
-      d->start_col = (int32_t)Request::var2column(start);
-      *new_cols = r.col_num + 1;
-    } else {
-      return fail(WKG_EINVAL, "unsupported triple pattern");
-    }
-    return WKG_OK;
+    return rc;
   }
 
   int run_one_step(Request &r, void *stream) {
@@ -264,7 +349,7 @@ int batched_round(wke_engine *e, std::vector<Request *> &act, void *stream) {
 
 }  // namespace
 
-int wke_run_batched(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, wke_result *res) {
+int wke_run_rounds(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, wke_result *res) {
   if (!e || !cmd || !cmd_off || !res || nq < 0) return WKG_EINVAL;
   const auto t0 = Clock::now();
   std::vector<Request> reqs(nq);
@@ -295,60 +380,259 @@ int wke_run_batched(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, i
   return first;
 }
 
-int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, const double *arrival_s,
-              const uint8_t *heavy, uint64_t table_budget_bytes, wke_result *res, double *wall_s) {
-  if (!e || !cmd || !cmd_off || !arrival_s || !res || nq < 0) return WKG_EINVAL;
-  (void)heavy;
-  (void)table_budget_bytes;
-  std::vector<Request> reqs(nq);
+int wke_configure(wke_engine *e, int max_queries, int max_patterns, uint64_t arena_bytes) {
+  if (!e || max_queries < 1 || max_patterns < 1 || max_patterns > 64 || arena_bytes < (1u << 20)) return WKG_EINVAL;
+  e->drop_exec();
+  e->cfg_queries = max_queries;
+  e->cfg_patterns = max_patterns;
+  e->cfg_arena = arena_bytes;
+  return WKG_OK;
+}
+
+namespace {
+
+struct Compiled {
+  std::vector<wkg_plan_step> steps;
+  std::vector<int32_t> off;
+  std::vector<int> who;  // query index of each compiled plan
+};
+
+// Appends query q's plan to c.
+void add_plan(Compiled &c, const std::vector<wkg_plan_step> &plan, int q) {
+  if (c.off.empty()) c.off.push_back(0);
+  c.steps.insert(c.steps.end(), plan.begin(), plan.end());
+  c.off.push_back((int32_t)c.steps.size());
+  c.who.push_back(q);
+}
+
+}  // namespace
+
+int wke_run_batched(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, wke_result *res) {
+  if (!e || !cmd || !cmd_off || !res || nq < 0) return WKG_EINVAL;
+  const auto t0 = Clock::now();
+  auto since = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
+  int first = WKG_OK;
+  auto note = [&](int rc) {
+    if (rc && !first) first = rc;
+  };
+  std::vector<int> plain;  // queries for the plain path
+  std::vector<std::vector<wkg_plan_step>> plans(nq);
   for (int q = 0; q < nq; q++) {
     memset(&res[q], 0, sizeof res[q]);
-    reqs[q].cmd = cmd + cmd_off[q];
-    reqs[q].ncmd = (int)(cmd_off[q + 1] - cmd_off[q]);
-    if (reqs[q].ncmd <= 0 || reqs[q].ncmd % 4) reqs[q].status = WKG_EINVAL;
+    const int n = (int)(cmd_off[q + 1] - cmd_off[q]);
+    std::string why;
+    if (int rc = compile_query(cmd + cmd_off[q], n, plans[q], why)) {
+      res[q].status = rc;
+      e->err = why;
+      note(rc);
+    } else if ((int)plans[q].size() > e->cfg_patterns) {
+      plain.push_back(q);
+    }
   }
+  wkg_batch *b = nullptr;
+  if (int rc = e->get_exec(&b)) return rc;
+  int q = 0;
+  std::vector<wkg_batch_result> rec((size_t)e->cfg_queries);
+  while (q < nq) {
+    Compiled c;
+    for (; q < nq && (int)c.who.size() < e->cfg_queries; q++)
+      if (res[q].status == WKG_OK && !plans[q].empty() && (int)plans[q].size() <= e->cfg_patterns)
+        add_plan(c, plans[q], q);
+    if (c.who.empty()) break;
+    int32_t *cells = nullptr;
+    int rc = wkg_batch_run(b, (int)c.who.size(), c.steps.data(), c.off.data(), rec.data(), &cells, nullptr);
+    if (rc) {
+      free(cells);
+      return e->fail_ctx(rc);
+    }
+    const double lat = since();
+    for (size_t i = 0; i < c.who.size(); i++) {
+      const int qi = c.who[i];
+      if (rec[i].status == WKG_ENOMEM) {
+        plain.push_back(qi);
+        continue;
+      }
+      fill_result(rec[i], cells, &res[qi]);
+      res[qi].latency_s = lat;
+      note(res[qi].status);
+    }
+    free(cells);
+  }
+  for (int qi : plain) {  // beyond the executor's shape or its arena
+    int rc = wke_run_query(e, cmd + cmd_off[qi], (int)(cmd_off[qi + 1] - cmd_off[qi]), &res[qi]);
+    res[qi].latency_s = since();
+    note(rc);
+  }
+  return first;
+}
+
+int wke_autotune(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, int reps,
+                 wkg_batch_tuning *best, double *best_ms) {
+  if (!e || !cmd || !cmd_off || nq < 1 || reps < 1) return WKG_EINVAL;
+  Compiled c;
+  std::vector<wkg_plan_step> plan;
+  for (int q = 0; q < nq && (int)c.who.size() < e->cfg_queries; q++) {
+    std::string why;
+    if (compile_query(cmd + cmd_off[q], (int)(cmd_off[q + 1] - cmd_off[q]), plan, why) == WKG_OK &&
+        (int)plan.size() <= e->cfg_patterns)
+      add_plan(c, plan, q);
+  }
+  if (c.who.empty()) return e->fail(WKG_EINVAL, "no query of the sample compiles to a device plan");
+  wkg_batch *b = nullptr;
+  if (int rc = e->get_exec(&b)) return rc;
+  wkg_batch_tuning t{};
+  double ms = 0;
+  if (int rc = wkg_batch_autotune(b, (int)c.who.size(), c.steps.data(), c.off.data(), reps, &t, &ms))
+    return e->fail_ctx(rc);
+  e->tuning = t;
+  e->tuned = true;
+  if (best) *best = t;
+  if (best_ms) *best_ms = ms;
+  return WKG_OK;
+}
+
+namespace {
+
+// One device executor of the served path with the queries it runs.
+struct Lane {
+  wkg_batch *b = nullptr;
+  bool heavy = false;
+  bool busy = false;
+  std::vector<int> run;
+};
+
+}  // namespace
+
+int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, const double *arrival_s,
+              const uint8_t *heavy, uint64_t table_budget_bytes, int light_lanes, wke_result *res,
+              double *wall_s) {
+  if (!e || !cmd || !cmd_off || !arrival_s || !res || nq < 0 || light_lanes < 1 || light_lanes > 16)
+    return WKG_EINVAL;
+  bool any_heavy = false;
+  for (int q = 0; heavy && q < nq; q++) any_heavy |= heavy[q] != 0;
+  if (table_budget_bytes == 0) table_budget_bytes = 64ull << 30;
+  const uint64_t heavy_arena = any_heavy ? table_budget_bytes / 2 : 0;
+  const uint64_t light_arena = (table_budget_bytes - heavy_arena) / (uint64_t)light_lanes;
+  std::vector<Lane> lanes;
+  auto release = [&] {
+    for (Lane &l : lanes) {
+      if (l.busy) {
+        std::vector<wkg_batch_result> sink(l.run.size());
+        wkg_batch_collect(l.b, sink.data(), nullptr, nullptr);
+      }
+      wkg_batch_destroy(l.b);
+    }
+  };
+  for (int i = 0; i < light_lanes + (any_heavy ? 1 : 0); i++) {
+    Lane l;
+    l.heavy = i == light_lanes;
+    if (int rc = e->new_exec(l.heavy ? heavy_arena : light_arena, &l.b)) {
+      release();
+      return rc;
+    }
+    lanes.push_back(l);
+  }
+  std::vector<std::vector<wkg_plan_step>> plans(nq);
+  std::vector<char> solo(nq, 0);
+  std::deque<int> wait[2];  // light, heavy
   const auto t0 = Clock::now();
   auto now = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
   int next = 0, completed = 0, first = WKG_OK;
-  std::vector<Request *> act;
-  std::vector<int> act_q;
+  auto done = [&](int q, int rc) {
+    res[q].latency_s = now() - arrival_s[q];
+    if (rc && !first) first = rc;
+    completed++;
+  };
+  auto run_plain = [&](int q) {
+    int rc = wke_run_query(e, cmd + cmd_off[q], (int)(cmd_off[q + 1] - cmd_off[q]), &res[q]);
+    done(q, rc);
+  };
+  std::vector<wkg_batch_result> rec((size_t)e->cfg_queries);
   while (completed < nq) {
-    // admit every arrival so far (continuous batching: they join this round)
+    bool progressed = false;
+    // admit arrivals
     const double t = now();
     while (next < nq && arrival_s[next] <= t) {
-      act.push_back(&reqs[next]);
-      act_q.push_back(next);
-      next++;
-    }
-    if (act.empty()) {
-      const double wait = arrival_s[next] - now();
-      if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
-      continue;
-    }
-    std::vector<Request *> run;
-    for (Request *r : act)
-      if (r->status == WKG_OK && !r->is_finished()) run.push_back(r);
-    if (!run.empty()) batched_round(e, run, nullptr);
-    // retire finished requests
-    std::vector<Request *> keep;
-    std::vector<int> keep_q;
-    for (size_t i = 0; i < act.size(); i++) {
-      Request &r = *act[i];
-      if (r.status != WKG_OK || r.is_finished()) {
-        if (r.status != WKG_OK) e->drop(r);
-        e->finish(r, &res[act_q[i]], nullptr);
-        res[act_q[i]].latency_s = now() - arrival_s[act_q[i]];
-        if (res[act_q[i]].status && !first) first = res[act_q[i]].status;
-        completed++;
+      const int q = next++;
+      memset(&res[q], 0, sizeof res[q]);
+      std::string why;
+      int rc = compile_query(cmd + cmd_off[q], (int)(cmd_off[q + 1] - cmd_off[q]), plans[q], why);
+      if (rc) {
+        res[q].status = rc;
+        e->err = why;
+        done(q, rc);
+      } else if ((int)plans[q].size() > e->cfg_patterns) {
+        run_plain(q);
       } else {
-        keep.push_back(act[i]);
-        keep_q.push_back(act_q[i]);
+        wait[any_heavy && heavy[q] ? 1 : 0].push_back(q);
       }
+      progressed = true;
     }
-    act.swap(keep);
-    act_q.swap(keep_q);
+    // retire finished batches
+    for (Lane &l : lanes) {
+      if (!l.busy || wkg_batch_busy(l.b)) continue;
+      int32_t *cells = nullptr;
+      int rc = wkg_batch_collect(l.b, rec.data(), &cells, nullptr);
+      l.busy = false;
+      progressed = true;
+      std::deque<int> &w = wait[l.heavy ? 1 : 0];
+      for (size_t i = 0; i < l.run.size(); i++) {
+        const int q = l.run[i];
+        if (rc) {
+          res[q].status = rc;
+          done(q, rc);
+        } else if (rec[i].status == WKG_ENOMEM && l.run.size() > 1) {
+          solo[q] = 1;  // retried alone
+          w.push_front(q);
+        } else if (rec[i].status == WKG_ENOMEM) {
+          run_plain(q);
+        } else {
+          fill_result(rec[i], cells, &res[q]);
+          done(q, res[q].status);
+        }
+      }
+      free(cells);
+      l.run.clear();
+    }
+    // start every idle lane on what waits in its queue
+    for (Lane &l : lanes) {
+      std::deque<int> &w = wait[l.heavy ? 1 : 0];
+      if (l.busy || w.empty()) continue;
+      Compiled c;
+      if (solo[w.front()]) {
+        add_plan(c, plans[w.front()], w.front());
+        w.pop_front();
+      } else {
+        while (!w.empty() && !solo[w.front()] && (int)c.who.size() < e->cfg_queries) {
+          add_plan(c, plans[w.front()], w.front());
+          w.pop_front();
+        }
+      }
+      int rc = wkg_batch_launch(l.b, (int)c.who.size(), c.steps.data(), c.off.data());
+      if (rc) {
+        e->err = wkg_last_error(e->ctx);
+        for (int q : c.who) {
+          res[q].status = rc;
+          done(q, rc);
+        }
+      } else {
+        l.busy = true;
+        l.run = c.who;
+      }
+      progressed = true;
+    }
+    if (progressed) continue;
+    bool any_busy = false;
+    for (Lane &l : lanes) any_busy |= l.busy;
+    if (!any_busy && next < nq) {
+      const double dt = arrival_s[next] - now();
+      if (dt > 0) std::this_thread::sleep_for(std::chrono::duration<double>(dt));
+    } else {
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
   }
   if (wall_s) *wall_s = now();
+  release();
   return first;
 }
 

@@ -1,0 +1,957 @@
+// Device-resident batched execution: whole queries advance through all of
+// their pattern steps on the device, with no host round trip between steps.
+//
+// A batch is nq queries, each a compiled plan of pattern steps (the step
+// GPU_Engine::execute_one_step dispatches, core/gpu_engine.hpp:395-467). Step
+// level l (the l-th pattern of every query that has one) runs as
+//
+//   k_level_plan   1 block: per query, size the step (rows, tiles, scratch,
+//                  filter output), carve it out of the HBM arena, write the
+//                  step's descriptor and the tile prefix; reset the tickets
+//   k_x_probe      persistent: tiles of all queries from one atomic ticket
+//   k_x_scan       persistent over queries: tile sums -> prefixes, totals
+//   k_x_finalize   persistent: compaction tiles from a second ticket
+//   k_expand_plan  1 block: per query, read the totals, allocate the
+//                  known_to_unknown output, commit the query's new table
+//   k_x_partition  persistent: merge-path split points of all expands
+//   k_x_expand     persistent: expand tiles from a third ticket
+//
+// and after the last level k_result_plan + k_x_gather pack every result
+// table into one contiguous region whose records are copied to the host.
+// The per-tile bodies are the plain path's (step_kernels.h), so each query's
+// table is the one the per-step calls produce, cell for cell.
+//
+// Memory: one arena per batch object. Tables grow up from its bottom and
+// stay until the batch ends (a table is the next level's input); each
+// level's scratch is carved down from its top and dropped at the next level.
+// A query whose needs do not fit ends with WKG_ENOMEM; which queries fit is
+// decided in query order, so the outcome is deterministic.
+//
+// The launch sequence for a given level count is captured once as a hipGraph
+// (no memset nodes: counters are reset by the plan kernels) and replayed.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <vector>
+
+#include "../../include/wukong_gpu.h"
+#include "step_kernels.h"
+#include "wkg_internal.h"
+
+using namespace wkg;
+
+namespace {
+
+constexpr uint64_t kAlign = 256;
+__host__ __device__ __forceinline__ uint64_t al(uint64_t b) { return (b + kAlign - 1) & ~(kAlign - 1); }
+
+constexpr int32_t kInactive = 1;          // QState::status of an unused slot
+constexpr uint64_t kGatherUnit = 16384;   // bytes per gather work unit
+constexpr uint64_t kRowLimit = 1ull << 32;
+
+struct QState {
+  const int32_t *T;   // current table (null before the first step)
+  uint64_t rows;
+  uint64_t expanded;
+  uint32_t cols;
+  int32_t status;     // WKG_OK, a WKG_E* code, or kInactive
+  uint32_t nsteps;
+  uint32_t pad;
+};
+
+struct ExecState {
+  uint64_t nq;
+  uint64_t table_top;    // arena bytes used by tables, from the bottom
+  uint64_t scratch_low;  // bottom of this level's scratch (scratch is [scratch_low, cap))
+  uint64_t nt, np, nx, nu;
+  uint64_t gather_base, gather_bytes;
+  uint64_t ticket[4];    // work-queue heads: probe, finalize, expand, spare
+  uint64_t pad[3];
+};
+
+struct Item {
+  StepParams p;
+  ExpandParams x;
+};
+
+struct ExecArgs {
+  GraphView g;
+  ExecState *st;
+  wkg_batch_result *res;  // [Q], right after st (one read-back copy)
+  QState *qs;             // [Q]
+  Item *items;            // [Q]
+  uint64_t *tile_base;    // [Q+1] probe/finalize tiles, later gather units
+  uint64_t *part_base;    // [Q+1]
+  uint64_t *xtile_base;   // [Q+1]
+  const uint64_t *up_nq;  // uploaded: query count
+  const uint32_t *up_nsteps;       // uploaded: [Q]
+  const wkg_plan_step *up_steps;   // uploaded: [Q * P], query q's level l at q * P + l
+  char *arena;
+  uint64_t cap;
+  uint32_t Q, P;
+};
+
+// Last q in [0, n] with base[q] <= t, for base ascending with base[n] > t.
+// Wave 0 runs a 64-ary search (each lane tests one pivot; the lanes that pass
+// form a prefix) and the block gets the answer through LDS.
+__device__ __forceinline__ uint32_t block_find(const uint64_t *base, uint32_t n, uint64_t t, uint32_t *s_q) {
+  if (threadIdx.x < kWave) {
+    const uint32_t lane = threadIdx.x;
+    uint32_t lo = 0, hi = n;  // base[lo] <= t < base[hi]
+    while (hi - lo > 1) {
+      const uint32_t span = hi - lo;
+      const uint32_t pos = lo + (uint32_t)(((uint64_t)span * (lane + 1)) / (kWave + 1));  // in [lo, hi)
+      const bool le = base[pos] <= t;
+      const uint32_t k = (uint32_t)__builtin_popcountll(wave_ballot(le));
+      const uint32_t plo = shfl32(pos, k ? k - 1 : 0), phi = shfl32(pos, k < kWave ? k : 0);
+      if (k) lo = plo;
+      if (k < kWave) hi = phi;
+    }
+    if (lane == 0) *s_q = lo;
+  }
+  __syncthreads();
+  return *s_q;
+}
+
+// Thread-level binary search with the same contract.
+__device__ __forceinline__ uint32_t thread_find(const uint64_t *base, uint32_t n, uint64_t t) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (base[mid] <= t) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Queries [q0, q1) of this thread in a single-block plan kernel.
+__device__ __forceinline__ void my_queries(uint32_t nq, uint32_t &q0, uint32_t &q1) {
+  const uint32_t per = (nq + kBlock - 1) / kBlock;
+  q0 = min(nq, threadIdx.x * per);
+  q1 = min(nq, q0 + per);
+}
+
+__device__ __forceinline__ bool level_active(const QState &s, uint32_t level) {
+  return s.status == WKG_OK && level < s.nsteps;
+}
+
+__global__ __launch_bounds__(kBlock) void k_x_begin(ExecArgs a) {
+  const uint64_t nq = *a.up_nq;
+  for (uint32_t q = threadIdx.x; q < a.Q; q += kBlock) {
+    QState s{};
+    s.T = nullptr;
+    s.status = q < nq ? WKG_OK : kInactive;
+    s.nsteps = q < nq ? a.up_nsteps[q] : 0;
+    a.qs[q] = s;
+  }
+  if (threadIdx.x == 0) {
+    a.st->nq = nq;
+    a.st->table_top = 0;
+    a.st->scratch_low = a.cap;
+    a.st->nt = a.st->np = a.st->nx = a.st->nu = 0;
+    a.st->gather_base = a.st->gather_bytes = 0;
+  }
+}
+
+// Step sizing shared by the two passes of k_level_plan.
+struct LevelNeed {
+  bool active;
+  uint64_t R, nt, scratch, table;
+};
+
+template <int TR>
+__device__ __forceinline__ LevelNeed level_need(const ExecArgs &a, uint32_t q, uint32_t level) {
+  LevelNeed n{false, 0, 0, 0, 0};
+  const QState &s = a.qs[q];
+  if (!level_active(s, level)) return n;
+  const wkg_plan_step &d = a.up_steps[(uint64_t)q * a.P + level];
+  const bool key = d.kind == WKG_STEP_I2U || d.kind == WKG_STEP_C2U;
+  n.active = true;
+  n.R = key ? 1 : s.rows;
+  n.nt = (n.R + TR - 1) / TR;
+  // totals[4] | tile_sum | tile_nz | tile_ex | hits (TR per tile, 16 B) | K2U: cinfo, cexcl
+  n.scratch = al(32 + 24 * n.nt) + n.nt * TR * 16 + (d.kind == WKG_STEP_K2U || key ? 2 * al(n.R * 8) : 0);
+  n.table = (d.kind == WKG_STEP_K2K || d.kind == WKG_STEP_K2C) ? al(n.R * s.cols * 4) : 0;
+  return n;
+}
+
+template <int TR>
+__global__ __launch_bounds__(kBlock) void k_level_plan(ExecArgs a, uint32_t level) {
+  __shared__ uint64_t s_tmp[kBlock / kWave];
+  __shared__ unsigned long long s_ut, s_us;
+  ExecState *st = a.st;
+  const uint32_t nq = (uint32_t)st->nq;
+  const uint64_t top = st->table_top;
+  const uint64_t room = a.cap - top;
+  uint32_t q0, q1;
+  my_queries(nq, q0, q1);
+  if (threadIdx.x == 0) {  // ordered before the atomics by the scans' barriers
+    s_ut = 0;
+    s_us = 0;
+  }
+  uint64_t ss = 0, ts = 0;
+  for (uint32_t q = q0; q < q1; q++) {
+    const LevelNeed n = level_need<TR>(a, q, level);
+    ss += n.scratch;
+    ts += n.table;
+  }
+  uint64_t tot;
+  uint64_t os = block_excl_scan(ss, s_tmp, tot);
+  uint64_t ot = block_excl_scan(ts, s_tmp, tot);
+  // pass 2: fit (in query order), tiles
+  uint64_t nts = 0;
+  {
+    uint64_t os2 = os, ot2 = ot;
+    for (uint32_t q = q0; q < q1; q++) {
+      const LevelNeed n = level_need<TR>(a, q, level);
+      const bool fits = ot2 + n.table + os2 + n.scratch <= room;
+      if (n.active && fits) nts += n.nt;
+      os2 += n.scratch;
+      ot2 += n.table;
+    }
+  }
+  uint64_t ntile = block_excl_scan(nts, s_tmp, tot);
+  const uint64_t NT = tot;
+  uint64_t used_t = 0, used_s = 0;
+  for (uint32_t q = q0; q < q1; q++) {
+    const LevelNeed n = level_need<TR>(a, q, level);
+    a.tile_base[q] = ntile;
+    if (n.active) {
+      if (ot + n.table + os + n.scratch > room) {
+        a.qs[q].status = WKG_ENOMEM;
+      } else {
+        const QState &s = a.qs[q];
+        const wkg_plan_step &d = a.up_steps[(uint64_t)q * a.P + level];
+        StepParams p{};
+        p.pid = (uint32_t)d.pid;
+        p.dir = (uint32_t)d.dir;
+        p.R = n.R;
+        p.ntiles = n.nt;
+        if (d.kind == WKG_STEP_I2U || d.kind == WKG_STEP_C2U) {
+          p.key = d.kind == WKG_STEP_I2U ? make_key(0, (uint64_t)d.pid, (uint64_t)d.dir)
+                                         : make_key((uint64_t)d.vid, (uint64_t)d.pid, (uint64_t)d.dir);
+          p.mode = kModeK2U;
+        } else {
+          p.T = s.T;
+          p.c = s.cols;
+          p.sc = (uint32_t)d.start_col;
+          p.mode = d.kind == WKG_STEP_K2U ? kModeK2U : d.kind == WKG_STEP_K2K ? kModeK2K : kModeK2C;
+          p.ec = d.kind == WKG_STEP_K2K ? (uint32_t)d.end_col : 0u;
+          p.target = (uint32_t)d.vid;
+          p.target_ok = (d.kind == WKG_STEP_K2C && d.vid >= 0 && d.vid <= 0xffffffffLL) ? 1u : 0u;
+        }
+        char *sb = a.arena + a.cap - (os + n.scratch);  // [cap - os - scratch, cap - os)
+        p.totals = (uint64_t *)sb;
+        p.tile_sum = p.totals + 4;
+        p.tile_nz = p.tile_sum + n.nt;
+        p.tile_ex = p.tile_nz + n.nt;
+        sb += al(32 + 24 * n.nt);
+        p.hits = (uint4 *)sb;
+        sb += n.nt * TR * 16;
+        if (p.mode == kModeK2U) {
+          p.cinfo = (uint2 *)sb;
+          p.cexcl = (uint64_t *)(sb + al(n.R * 8));
+        } else {
+          p.O = (int32_t *)(a.arena + top + ot);
+        }
+        a.items[q].p = p;
+        ntile += n.nt;
+        used_t = ot + n.table;
+        used_s = os + n.scratch;
+      }
+    }
+    os += n.scratch;
+    ot += n.table;
+  }
+  // the fitting queries form a prefix (their cumulative needs are the
+  // smallest), so the highest ends in use are the last fitting query's
+  if (used_t) atomicMax(&s_ut, (unsigned long long)used_t);
+  if (used_s) atomicMax(&s_us, (unsigned long long)used_s);
+  __syncthreads();
+  if (threadIdx.x == kBlock - 1) {
+    a.tile_base[nq] = NT;
+    st->nt = NT;
+    st->table_top = top + s_ut;
+    st->scratch_low = a.cap - s_us;
+    st->ticket[0] = 0;
+    st->ticket[1] = 0;
+  }
+}
+
+// Probe tiles of all queries from the shared ticket; the next ticket is taken
+// while the current tile is worked on.
+template <int TR, bool SORTED, bool FRAMED>
+__global__ __launch_bounds__(kBlock) void k_x_probe(ExecArgs a) {
+  __shared__ uint64_t s_t;
+  __shared__ uint32_t s_q;
+  const uint64_t NT = a.st->nt;
+  const uint32_t nq = (uint32_t)a.st->nq;
+  unsigned long long *ticket = (unsigned long long *)&a.st->ticket[0];
+  uint64_t nxt = 0;
+  if (threadIdx.x == 0) nxt = atomicAdd(ticket, 1ull);
+  while (true) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      s_t = nxt;
+      if (nxt < NT) nxt = atomicAdd(ticket, 1ull);
+    }
+    __syncthreads();
+    const uint64_t t = s_t;
+    if (t >= NT) break;
+    const uint32_t q = block_find(a.tile_base, nq, t, &s_q);
+    probe_tile<TR, SORTED, FRAMED>(a.g, a.items[q].p, t - a.tile_base[q]);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_x_scan(ExecArgs a, uint32_t level) {
+  const uint32_t nq = (uint32_t)a.st->nq;
+  for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+    __syncthreads();
+    if (level_active(a.qs[q], level)) scan_step_tiles(a.items[q].p);
+  }
+}
+
+template <int TR>
+__global__ __launch_bounds__(kBlock) void k_x_finalize(ExecArgs a) {
+  __shared__ uint64_t s_t;
+  __shared__ uint32_t s_q;
+  const uint64_t NT = a.st->nt;
+  const uint32_t nq = (uint32_t)a.st->nq;
+  unsigned long long *ticket = (unsigned long long *)&a.st->ticket[1];
+  uint64_t nxt = 0;
+  if (threadIdx.x == 0) nxt = atomicAdd(ticket, 1ull);
+  while (true) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      s_t = nxt;
+      if (nxt < NT) nxt = atomicAdd(ticket, 1ull);
+    }
+    __syncthreads();
+    const uint64_t t = s_t;
+    if (t >= NT) break;
+    const uint32_t q = block_find(a.tile_base, nq, t, &s_q);
+    finalize_tile<TR>(a.items[q].p, t - a.tile_base[q]);
+  }
+}
+
+struct ExpandNeed {
+  bool active, k2u;
+  uint64_t table, scratch, nx;
+};
+
+template <int NV>
+__device__ __forceinline__ ExpandNeed expand_need(const ExecArgs &a, uint32_t q, uint32_t level) {
+  ExpandNeed n{false, false, 0, 0, 0};
+  if (!level_active(a.qs[q], level)) return n;
+  const StepParams &p = a.items[q].p;
+  n.active = true;
+  n.k2u = p.mode == kModeK2U;
+  if (n.k2u) {
+    const uint64_t E = p.totals[0], nz = p.totals[2];
+    if (E >= kRowLimit) return n;  // refused below
+    n.table = al(E * (p.c + 1) * 4);
+    n.nx = E ? (nz + E + NV - 1) / NV : 0;
+    n.scratch = n.nx ? al((n.nx + 1) * 8) : 0;
+  }
+  return n;
+}
+
+template <int NV>
+__global__ __launch_bounds__(kBlock) void k_expand_plan(ExecArgs a, uint32_t level) {
+  __shared__ uint64_t s_tmp[kBlock / kWave];
+  __shared__ unsigned long long s_used;
+  ExecState *st = a.st;
+  const uint32_t nq = (uint32_t)st->nq;
+  const uint64_t top = st->table_top, low = st->scratch_low;
+  const uint64_t room = low - top;
+  uint32_t q0, q1;
+  my_queries(nq, q0, q1);
+  if (threadIdx.x == 0) s_used = 0;
+  uint64_t ts = 0, ss = 0;
+  for (uint32_t q = q0; q < q1; q++) {
+    const ExpandNeed n = expand_need<NV>(a, q, level);
+    ts += n.table;
+    ss += n.scratch;
+  }
+  uint64_t tot;
+  uint64_t ot = block_excl_scan(ts, s_tmp, tot);
+  uint64_t os = block_excl_scan(ss, s_tmp, tot);
+  uint64_t nps = 0, nxs = 0;
+  {
+    uint64_t ot2 = ot, os2 = os;
+    for (uint32_t q = q0; q < q1; q++) {
+      const ExpandNeed n = expand_need<NV>(a, q, level);
+      if (n.active && ot2 + n.table + os2 + n.scratch <= room && n.nx) {
+        nps += n.nx + 1;
+        nxs += n.nx;
+      }
+      ot2 += n.table;
+      os2 += n.scratch;
+    }
+  }
+  uint64_t pbase = block_excl_scan(nps, s_tmp, tot);
+  const uint64_t NP = tot;
+  uint64_t xbase = block_excl_scan(nxs, s_tmp, tot);
+  const uint64_t NX = tot;
+  uint64_t used_t = 0;
+  for (uint32_t q = q0; q < q1; q++) {
+    const ExpandNeed n = expand_need<NV>(a, q, level);
+    a.part_base[q] = pbase;
+    a.xtile_base[q] = xbase;
+    if (n.active) {
+      QState &s = a.qs[q];
+      const StepParams &p = a.items[q].p;
+      if (n.k2u && p.totals[0] >= kRowLimit) {
+        s.status = WKG_ERANGE;
+      } else if (ot + n.table + os + n.scratch > room) {
+        s.status = WKG_ENOMEM;
+      } else if (n.k2u) {
+        ExpandParams x{};
+        x.pid = p.pid;
+        x.T = p.T;
+        x.c = p.c;
+        x.R = p.totals[2];
+        x.cinfo = p.cinfo;
+        x.cexcl = p.cexcl;
+        x.part = (const uint64_t *)(a.arena + low - (os + n.scratch));
+        x.E = p.totals[0];
+        x.O = (int32_t *)(a.arena + top + ot);
+        a.items[q].x = x;
+        s.T = x.O;
+        s.rows = x.E;
+        s.cols = p.c + 1;
+        s.expanded += x.E;
+        if (n.nx) {
+          pbase += n.nx + 1;
+          xbase += n.nx;
+        }
+        used_t = ot + n.table;
+      } else {  // filters: the kept rows were compacted into p.O
+        s.T = p.O;
+        s.rows = p.totals[0];
+        s.expanded += p.totals[1];
+      }
+    }
+    ot += n.table;
+    os += n.scratch;
+  }
+  if (used_t) atomicMax(&s_used, (unsigned long long)used_t);
+  __syncthreads();
+  if (threadIdx.x == kBlock - 1) {
+    a.part_base[nq] = NP;
+    a.xtile_base[nq] = NX;
+    st->np = NP;
+    st->nx = NX;
+    st->table_top = top + s_used;
+    st->ticket[2] = 0;
+  }
+}
+
+template <int NV>
+__global__ __launch_bounds__(kBlock) void k_x_partition(ExecArgs a) {
+  const uint64_t NP = a.st->np;
+  const uint32_t nq = (uint32_t)a.st->nq;
+  for (uint64_t k = blockIdx.x * (uint64_t)kBlock + threadIdx.x; k < NP; k += (uint64_t)gridDim.x * kBlock) {
+    const uint32_t q = thread_find(a.part_base, nq, k);
+    const ExpandParams &x = a.items[q].x;
+    const uint64_t j = k - a.part_base[q];
+    ((uint64_t *)x.part)[j] = merge_split<NV>(x.cexcl, x.R, x.E, j);
+  }
+}
+
+template <int VT, bool FRAMED>
+__global__ __launch_bounds__(kBlock) void k_x_expand(ExecArgs a) {
+  __shared__ uint64_t s_t;
+  __shared__ uint32_t s_q;
+  const uint64_t NX = a.st->nx;
+  const uint32_t nq = (uint32_t)a.st->nq;
+  unsigned long long *ticket = (unsigned long long *)&a.st->ticket[2];
+  uint64_t nxt = 0;
+  if (threadIdx.x == 0) nxt = atomicAdd(ticket, 1ull);
+  while (true) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      s_t = nxt;
+      if (nxt < NX) nxt = atomicAdd(ticket, 1ull);
+    }
+    __syncthreads();
+    const uint64_t t = s_t;
+    if (t >= NX) break;
+    const uint32_t q = block_find(a.xtile_base, nq, t, &s_q);
+    expand_tile<VT, FRAMED>(a.g, a.items[q].x, t - a.xtile_base[q]);
+  }
+}
+
+// Result records and the packed result region (above the tables; the last
+// level's scratch is free by now). Gather work units go to tile_base.
+__global__ __launch_bounds__(kBlock) void k_result_plan(ExecArgs a) {
+  __shared__ uint64_t s_tmp[kBlock / kWave];
+  __shared__ unsigned long long s_used;
+  ExecState *st = a.st;
+  const uint32_t nq = (uint32_t)st->nq;
+  const uint64_t top = st->table_top, room = a.cap - top;
+  uint32_t q0, q1;
+  my_queries(nq, q0, q1);
+  if (threadIdx.x == 0) s_used = 0;
+  auto bytes = [&](uint32_t q) -> uint64_t {
+    const QState &s = a.qs[q];
+    return s.status == WKG_OK ? al(s.rows * s.cols * 4) : 0;
+  };
+  uint64_t bs = 0;
+  for (uint32_t q = q0; q < q1; q++) bs += bytes(q);
+  uint64_t tot;
+  uint64_t ob = block_excl_scan(bs, s_tmp, tot);
+  uint64_t us = 0;
+  {
+    uint64_t ob2 = ob;
+    for (uint32_t q = q0; q < q1; q++) {
+      const uint64_t b = bytes(q);
+      if (ob2 + b <= room) us += (b + kGatherUnit - 1) / kGatherUnit;
+      ob2 += b;
+    }
+  }
+  uint64_t ubase = block_excl_scan(us, s_tmp, tot);
+  const uint64_t NU = tot;
+  uint64_t used = 0;
+  for (uint32_t q = q0; q < q1; q++) {
+    const uint64_t b = bytes(q);
+    QState &s = a.qs[q];
+    a.tile_base[q] = ubase;
+    wkg_batch_result r{};
+    if (s.status == WKG_OK && ob + b > room) s.status = WKG_ENOMEM;
+    r.status = s.status;
+    r.expanded = s.expanded;
+    r.cols = (int32_t)s.cols;
+    if (s.status == WKG_OK) {
+      r.offset = ob / 4;
+      r.rows = s.rows;
+      ubase += (b + kGatherUnit - 1) / kGatherUnit;
+      used = ob + b;
+    }
+    a.res[q] = r;
+    ob += b;
+  }
+  if (used) atomicMax(&s_used, (unsigned long long)used);
+  __syncthreads();
+  if (threadIdx.x == kBlock - 1) {
+    a.tile_base[nq] = NU;
+    st->nu = NU;
+    st->gather_base = top;
+    st->gather_bytes = s_used;
+  }
+}
+
+// Copies each result table (256-byte aligned in the arena, so 16-byte chunks
+// stay inside its allocation) to its place in the result region.
+__global__ __launch_bounds__(kBlock) void k_x_gather(ExecArgs a) {
+  __shared__ uint32_t s_q;
+  const uint64_t NU = a.st->nu;
+  const uint32_t nq = (uint32_t)a.st->nq;
+  char *dst0 = a.arena + a.st->gather_base;
+  for (uint64_t u = blockIdx.x; u < NU; u += gridDim.x) {
+    __syncthreads();
+    const uint32_t q = block_find(a.tile_base, nq, u, &s_q);
+    const QState &s = a.qs[q];
+    const uint64_t bytes = s.rows * s.cols * 4;
+    const uint64_t b0 = (u - a.tile_base[q]) * kGatherUnit;
+    if (b0 >= bytes) continue;  // (units are sized from the same bytes; kept for safety)
+    const uint64_t b1 = b0 + kGatherUnit < bytes ? b0 + kGatherUnit : bytes;
+    const uint4 *src = (const uint4 *)((const char *)s.T + b0);
+    uint4 *dst = (uint4 *)(dst0 + a.res[q].offset * 4 + b0);
+    const uint64_t n16 = (b1 - b0 + 15) / 16;
+    for (uint64_t i = threadIdx.x; i < n16; i += kBlock) dst[i] = src[i];
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+
+struct wkg_batch {
+  wkg_ctx *ctx = nullptr;
+  int Q = 0, P = 0;
+  uint64_t cap = 0;
+  hipStream_t s = nullptr;
+  bool own_stream = false;
+  int ncu = 0;
+  char *arena = nullptr;
+  char *dctl = nullptr;          // device control block
+  size_t ctl_bytes = 0, down_bytes = 0, up_bytes = 0;
+  char *d_up = nullptr;          // device copy of the upload block
+  char *h_up = nullptr;          // pinned: nq | nsteps[Q] | steps[Q*P]
+  char *h_down = nullptr;        // pinned: ExecState | results[Q]
+  char *h_cells = nullptr;       // pinned staging of the packed tables
+  size_t h_cells_cap = 0;
+  ExecArgs args{};
+  wkg_batch_tuning tune{};
+  std::map<int, hipGraphExec_t> graphs;  // by level count
+  bool launched = false;
+  int launched_nq = 0;
+  std::vector<int> pids;
+
+  void clear_graphs() {
+    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+  }
+  int enqueue(int levels);
+  int graph_for(int levels, hipGraphExec_t *out);
+};
+
+namespace {
+
+wkg_batch_tuning normalized(const wkg_batch_tuning &t) {
+  wkg_batch_tuning r = t;
+  if (r.probe_wg_per_cu <= 0) r.probe_wg_per_cu = 2;
+  if (r.expand_wg_per_cu <= 0) r.expand_wg_per_cu = 2;
+  if (r.aux_wg_per_cu <= 0) r.aux_wg_per_cu = 1;
+  if (r.probe_tile_rows != 1024) r.probe_tile_rows = 2048;
+  if (r.expand_tile_items != 1024) r.expand_tile_items = 2048;
+  r.pad = 0;
+  return r;
+}
+
+template <int TR, int VT, bool SORTED, bool FRAMED>
+void enqueue_levels(const ExecArgs &a, hipStream_t s, int levels, unsigned gp, unsigned gx, unsigned ga) {
+  constexpr int NV = kBlock * VT;
+  for (int l = 0; l < levels; l++) {
+    k_level_plan<TR><<<1, kBlock, 0, s>>>(a, (uint32_t)l);
+    k_x_probe<TR, SORTED, FRAMED><<<gp, kBlock, 0, s>>>(a);
+    k_x_scan<<<ga, kBlock, 0, s>>>(a, (uint32_t)l);
+    k_x_finalize<TR><<<gp, kBlock, 0, s>>>(a);
+    k_expand_plan<NV><<<1, kBlock, 0, s>>>(a, (uint32_t)l);
+    k_x_partition<NV><<<ga, kBlock, 0, s>>>(a);
+    k_x_expand<VT, FRAMED><<<gx, kBlock, 0, s>>>(a);
+  }
+}
+
+template <int TR, int VT>
+void enqueue_shape(const ExecArgs &a, hipStream_t s, int levels, bool sorted, bool framed, unsigned gp, unsigned gx,
+                   unsigned ga) {
+  if (framed) {
+    if (sorted) enqueue_levels<TR, VT, true, true>(a, s, levels, gp, gx, ga);
+    else enqueue_levels<TR, VT, false, true>(a, s, levels, gp, gx, ga);
+  } else {
+    if (sorted) enqueue_levels<TR, VT, true, false>(a, s, levels, gp, gx, ga);
+    else enqueue_levels<TR, VT, false, false>(a, s, levels, gp, gx, ga);
+  }
+}
+
+}  // namespace
+
+int wkg_batch::enqueue(int levels) {
+  const wkg_batch_tuning t = normalized(tune);
+  const unsigned gp = (unsigned)(ncu * t.probe_wg_per_cu), gx = (unsigned)(ncu * t.expand_wg_per_cu),
+                 ga = (unsigned)(ncu * t.aux_wg_per_cu);
+  hipError_t e = hipMemcpyAsync(d_up, h_up, up_bytes, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return ctx->fail(WKG_EDEVICE, "plan upload failed: %s", hipGetErrorString(e));
+  k_x_begin<<<1, kBlock, 0, s>>>(args);
+  const bool sorted = ctx->sorted, framed = ctx->framed;
+  if (t.probe_tile_rows == 1024) {
+    if (t.expand_tile_items == 1024) enqueue_shape<1024, 4>(args, s, levels, sorted, framed, gp, gx, ga);
+    else enqueue_shape<1024, 8>(args, s, levels, sorted, framed, gp, gx, ga);
+  } else {
+    if (t.expand_tile_items == 1024) enqueue_shape<2048, 4>(args, s, levels, sorted, framed, gp, gx, ga);
+    else enqueue_shape<2048, 8>(args, s, levels, sorted, framed, gp, gx, ga);
+  }
+  k_result_plan<<<1, kBlock, 0, s>>>(args);
+  k_x_gather<<<ga, kBlock, 0, s>>>(args);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(h_down, dctl, down_bytes, hipMemcpyDeviceToHost, s);
+  if (e != hipSuccess) return ctx->fail(WKG_EDEVICE, "batch launch failed: %s", hipGetErrorString(e));
+  return WKG_OK;
+}
+
+int wkg_batch::graph_for(int levels, hipGraphExec_t *out) {
+  auto it = graphs.find(levels);
+  if (it != graphs.end()) {
+    *out = it->second;
+    return WKG_OK;
+  }
+  hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return ctx->fail(WKG_EDEVICE, "stream capture failed: %s", hipGetErrorString(e));
+  int rc = enqueue(levels);
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(s, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return ctx->fail(WKG_EDEVICE, "stream capture failed: %s", hipGetErrorString(e));
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return ctx->fail(WKG_EDEVICE, "graph instantiation failed: %s", hipGetErrorString(e));
+  graphs[levels] = x;
+  *out = x;
+  return WKG_OK;
+}
+
+extern "C" {
+
+int wkg_batch_create(wkg_ctx *ctx, int max_queries, int max_patterns, uint64_t arena_bytes, void *stream,
+                     wkg_batch **out) {
+  if (!ctx || !out || max_queries < 1 || max_patterns < 1 || max_patterns > 64 || arena_bytes < (1u << 20))
+    return WKG_EINVAL;
+  if (max_queries > (1 << 20)) return ctx->fail(WKG_ERANGE, "a batch holds at most 2^20 queries");
+  if (!ctx->loaded) return ctx->fail(WKG_ESTATE, "no graph loaded");
+  *out = nullptr;
+  wkg_batch *b = new wkg_batch;
+  b->ctx = ctx;
+  b->Q = max_queries;
+  b->P = max_patterns;
+  b->cap = al(arena_bytes) - kAlign;
+  hipError_t e = hipSuccess;
+  if (stream) {
+    b->s = (hipStream_t)stream;
+  } else {
+    e = hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking);
+    b->own_stream = e == hipSuccess;
+  }
+  int dev_cu = 0;
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
+  b->ncu = dev_cu > 0 ? dev_cu : 256;
+  const uint64_t Q = (uint64_t)max_queries, P = (uint64_t)max_patterns;
+  // control block: ExecState | results[Q] | qs[Q] | items[Q] | 3 x base[Q+1] | upload
+  const uint64_t o_res = al(sizeof(ExecState));
+  const uint64_t o_qs = al(o_res + Q * sizeof(wkg_batch_result));
+  const uint64_t o_items = al(o_qs + Q * sizeof(QState));
+  const uint64_t o_tb = al(o_items + Q * sizeof(Item));
+  const uint64_t o_pb = al(o_tb + (Q + 1) * 8);
+  const uint64_t o_xb = al(o_pb + (Q + 1) * 8);
+  const uint64_t o_up = al(o_xb + (Q + 1) * 8);
+  b->up_bytes = al(8 + Q * 4) + Q * P * sizeof(wkg_plan_step);
+  b->ctl_bytes = o_up + b->up_bytes;
+  b->down_bytes = o_res + Q * sizeof(wkg_batch_result);
+  auto bail = [&](int rc) {
+    wkg_batch_destroy(b);
+    return rc;
+  };
+  if (e != hipSuccess) return bail(ctx->fail(WKG_EDEVICE, "batch stream setup failed: %s", hipGetErrorString(e)));
+  b->dctl = (char *)ctx->pool->alloc(b->ctl_bytes, b->s, ctx->err);
+  if (!b->dctl) return bail(WKG_ENOMEM);
+  b->arena = (char *)ctx->pool->alloc(b->cap, b->s, ctx->err);
+  if (!b->arena) return bail(WKG_ENOMEM);
+  e = hipHostMalloc((void **)&b->h_up, b->up_bytes, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_down, b->down_bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    b->h_up = b->h_down = nullptr;
+    return bail(ctx->fail(WKG_ENOMEM, "pinned host buffers: %s", hipGetErrorString(e)));
+  }
+  memset(b->h_up, 0, b->up_bytes);
+  b->d_up = b->dctl + o_up;
+  ExecArgs &a = b->args;
+  a.g = ctx->view();
+  a.st = (ExecState *)b->dctl;
+  a.res = (wkg_batch_result *)(b->dctl + o_res);
+  a.qs = (QState *)(b->dctl + o_qs);
+  a.items = (Item *)(b->dctl + o_items);
+  a.tile_base = (uint64_t *)(b->dctl + o_tb);
+  a.part_base = (uint64_t *)(b->dctl + o_pb);
+  a.xtile_base = (uint64_t *)(b->dctl + o_xb);
+  a.up_nq = (const uint64_t *)b->d_up;
+  a.up_nsteps = (const uint32_t *)(b->d_up + 8);
+  a.up_steps = (const wkg_plan_step *)(b->d_up + al(8 + Q * 4));
+  a.arena = b->arena;
+  a.cap = b->cap;
+  a.Q = (uint32_t)Q;
+  a.P = (uint32_t)P;
+  *out = b;
+  return WKG_OK;
+}
+
+int wkg_batch_destroy(wkg_batch *b) {
+  if (!b) return WKG_OK;
+  if (b->s) (void)hipStreamSynchronize(b->s);
+  b->clear_graphs();
+  if (b->arena) b->ctx->pool->release(b->arena, b->s);
+  if (b->dctl) b->ctx->pool->release(b->dctl, b->s);
+  if (b->h_up) (void)hipHostFree(b->h_up);
+  if (b->h_down) (void)hipHostFree(b->h_down);
+  if (b->h_cells) (void)hipHostFree(b->h_cells);
+  if (b->own_stream) {
+    // pooled blocks tagged with this stream go back to HIP before it is destroyed
+    b->ctx->pool->trim();
+    (void)hipStreamDestroy(b->s);
+  }
+  delete b;
+  return WKG_OK;
+}
+
+int wkg_batch_set_tuning(wkg_batch *b, const wkg_batch_tuning *t) {
+  if (!b || !t) return WKG_EINVAL;
+  if (b->launched) return b->ctx->fail(WKG_ESTATE, "a batch is in flight");
+  const wkg_batch_tuning n = normalized(*t);
+  if (memcmp(&n, &b->tune, sizeof n) != 0) {
+    b->clear_graphs();
+    b->tune = n;
+  }
+  return WKG_OK;
+}
+
+int wkg_batch_get_tuning(const wkg_batch *b, wkg_batch_tuning *t) {
+  if (!b || !t) return WKG_EINVAL;
+  *t = normalized(b->tune);
+  return WKG_OK;
+}
+
+int wkg_batch_launch(wkg_batch *b, int nq, const wkg_plan_step *steps, const int32_t *step_off) {
+  if (!b) return WKG_EINVAL;
+  wkg_ctx *ctx = b->ctx;
+  if (b->launched) return ctx->fail(WKG_ESTATE, "a batch is already in flight");
+  if (nq < 0 || nq > b->Q || (nq && (!steps || !step_off)))
+    return ctx->fail(WKG_EINVAL, "batch of %d queries (at most %d)", nq, b->Q);
+  if (step_off && step_off[0] != 0) return ctx->fail(WKG_EINVAL, "step_off[0] must be 0");
+  if (!ctx->loaded) return ctx->fail(WKG_ESTATE, "no graph loaded");
+  {  // the captured sequences hold the graph view; a reloaded graph invalidates them
+    const GraphView v = ctx->view(), &o = b->args.g;
+    if (v.slots != o.slots || v.edges != o.edges || v.meta != o.meta || v.vframes != o.vframes ||
+        v.eframes != o.eframes || v.vshift != o.vshift || v.eshift != o.eshift || v.num_metas != o.num_metas) {
+      b->clear_graphs();
+      b->args.g = v;
+    }
+  }
+  // validate every plan (column bookkeeping as the steps will run) and stage it
+  uint32_t *h_nsteps = (uint32_t *)(b->h_up + 8);
+  wkg_plan_step *h_steps = (wkg_plan_step *)(b->h_up + al(8 + (uint64_t)b->Q * 4));
+  int levels = 0;
+  b->pids.clear();
+  for (int q = 0; q < nq; q++) {
+    const int n = step_off[q + 1] - step_off[q];
+    if (n < 1 || n > b->P) return ctx->fail(WKG_EINVAL, "query %d has %d steps (1..%d)", q, n, b->P);
+    levels = std::max(levels, n);
+    int cols = 0;
+    for (int l = 0; l < n; l++) {
+      const wkg_plan_step &d = steps[step_off[q] + l];
+      if (int rc = ctx->check_pattern(d.pid, d.dir)) return ctx->fail(rc, "query %d step %d: %s", q, l, ctx->err.c_str());
+      switch (d.kind) {
+        case WKG_STEP_I2U:
+        case WKG_STEP_C2U:
+          if (l != 0) return ctx->fail(WKG_EINVAL, "query %d step %d: index/constant start after step 0", q, l);
+          if (d.kind == WKG_STEP_I2U && !is_tpid(d.pid))
+            return ctx->fail(WKG_EINVAL, "query %d: index id %d is not a predicate/type id", q, d.pid);
+          if (d.kind == WKG_STEP_C2U && (d.vid < 0 || (uint64_t)d.vid > kMaxVid))
+            return ctx->fail(WKG_EINVAL, "query %d: constant out of range", q);
+          cols = 1;
+          break;
+        case WKG_STEP_K2U:
+        case WKG_STEP_K2K:
+        case WKG_STEP_K2C:
+          if (l == 0) return ctx->fail(WKG_EINVAL, "query %d: step 0 needs an index or constant start", q);
+          if (d.start_col < 0 || d.start_col >= cols)
+            return ctx->fail(WKG_EINVAL, "query %d step %d: start column %d outside %d columns", q, l, d.start_col,
+                             cols);
+          if (d.kind == WKG_STEP_K2K && (d.end_col < 0 || d.end_col >= cols))
+            return ctx->fail(WKG_EINVAL, "query %d step %d: end column %d outside %d columns", q, l, d.end_col, cols);
+          if (d.kind == WKG_STEP_K2U) cols++;
+          if (cols > kBlock) return ctx->fail(WKG_ERANGE, "query %d: tables are limited to %d columns", q, kBlock);
+          break;
+        default:
+          return ctx->fail(WKG_EINVAL, "query %d step %d: unknown kind %d", q, l, d.kind);
+      }
+      if (std::find(b->pids.begin(), b->pids.end(), d.pid) == b->pids.end()) b->pids.push_back(d.pid);
+    }
+  }
+  if (ctx->framed)  // every predicate of the batch must be resident for the whole run
+    for (int p : b->pids)
+      if (int rc = ctx->ensure_pred(p, b->s, &b->pids)) return rc;
+  *(uint64_t *)b->h_up = (uint64_t)nq;
+  for (int q = 0; q < b->Q; q++) h_nsteps[q] = q < nq ? (uint32_t)(step_off[q + 1] - step_off[q]) : 0u;
+  for (int q = 0; q < nq; q++)
+    memcpy(&h_steps[(uint64_t)q * b->P], &steps[step_off[q]], sizeof(wkg_plan_step) * (step_off[q + 1] - step_off[q]));
+  if (levels == 0) levels = 1;
+  hipGraphExec_t x;
+  if (int rc = b->graph_for(levels, &x)) return rc;
+  ProfScope ps(ctx, b->s, "batch_graph", 0.0);
+  hipError_t e = hipGraphLaunch(x, b->s);
+  if (e != hipSuccess) return ctx->fail(WKG_EDEVICE, "graph launch failed: %s", hipGetErrorString(e));
+  b->launched = true;
+  b->launched_nq = nq;
+  return WKG_OK;
+}
+
+int wkg_batch_busy(wkg_batch *b) {
+  if (!b || !b->launched) return 0;
+  return hipStreamQuery(b->s) == hipErrorNotReady ? 1 : 0;
+}
+
+int wkg_batch_collect(wkg_batch *b, wkg_batch_result *res, int32_t **cells, uint64_t *ncells) {
+  if (!b || (!res && b->launched_nq)) return WKG_EINVAL;
+  wkg_ctx *ctx = b->ctx;
+  if (!b->launched) return ctx->fail(WKG_ESTATE, "no batch in flight");
+  if (cells) *cells = nullptr;
+  if (ncells) *ncells = 0;
+  b->launched = false;
+  hipError_t e = hipStreamSynchronize(b->s);
+  if (e != hipSuccess) return ctx->fail(WKG_EDEVICE, "batch failed: %s", hipGetErrorString(e));
+  const ExecState *st = (const ExecState *)b->h_down;
+  const wkg_batch_result *r = (const wkg_batch_result *)(b->h_down + al(sizeof(ExecState)));
+  if (b->launched_nq) memcpy(res, r, sizeof(wkg_batch_result) * b->launched_nq);
+  const uint64_t bytes = st->gather_bytes;
+  if (ncells) *ncells = bytes / 4;
+  if (!cells || bytes == 0) return WKG_OK;
+  if (bytes > b->h_cells_cap) {
+    if (b->h_cells) (void)hipHostFree(b->h_cells);
+    b->h_cells = nullptr;
+    b->h_cells_cap = 0;
+    const size_t want = std::max<size_t>(bytes + bytes / 4, 1 << 22);
+    if ((e = hipHostMalloc((void **)&b->h_cells, want, hipHostMallocDefault)) != hipSuccess) {
+      b->h_cells = nullptr;
+      return ctx->fail(WKG_ENOMEM, "pinned result staging of %zu bytes: %s", want, hipGetErrorString(e));
+    }
+    b->h_cells_cap = want;
+  }
+  e = hipMemcpyAsync(b->h_cells, b->arena + st->gather_base, bytes, hipMemcpyDeviceToHost, b->s);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->s);
+  if (e != hipSuccess) return ctx->fail(WKG_EDEVICE, "result read-back failed: %s", hipGetErrorString(e));
+  *cells = (int32_t *)malloc(bytes);
+  if (!*cells) return ctx->fail(WKG_ENOMEM, "host result buffer of %llu bytes", (unsigned long long)bytes);
+  memcpy(*cells, b->h_cells, bytes);
+  return WKG_OK;
+}
+
+int wkg_batch_run(wkg_batch *b, int nq, const wkg_plan_step *steps, const int32_t *step_off, wkg_batch_result *res,
+                  int32_t **cells, uint64_t *ncells) {
+  if (int rc = wkg_batch_launch(b, nq, steps, step_off)) return rc;
+  return wkg_batch_collect(b, res, cells, ncells);
+}
+
+int wkg_batch_autotune(wkg_batch *b, int nq, const wkg_plan_step *steps, const int32_t *step_off, int reps,
+                       wkg_batch_tuning *best, double *best_ms) {
+  if (!b || reps < 1) return WKG_EINVAL;
+  std::vector<wkg_batch_result> res((size_t)std::max(nq, 1));
+  std::vector<wkg_batch_tuning> cand;
+  for (int tr : {1024, 2048})
+    for (int xi : {1024, 2048})
+      for (int pw : {1, 2, 4})
+        for (int xw : {1, 2, 4})
+          for (int aw : {1, 2}) cand.push_back(wkg_batch_tuning{pw, xw, aw, tr, xi, 0});
+  double best_t = 1e30;
+  wkg_batch_tuning best_c = normalized(b->tune);
+  for (const wkg_batch_tuning &c : cand) {
+    if (int rc = wkg_batch_set_tuning(b, &c)) return rc;
+    std::vector<double> ts;
+    for (int i = 0; i <= reps; i++) {  // the first run captures the graph
+      const auto t0 = std::chrono::steady_clock::now();
+      int32_t *cells = nullptr;
+      int rc = wkg_batch_run(b, nq, steps, step_off, res.data(), &cells, nullptr);
+      free(cells);
+      if (rc) return rc;
+      if (i) ts.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
+    std::sort(ts.begin(), ts.end());
+    const double med = ts[ts.size() / 2];
+    if (med < best_t) {
+      best_t = med;
+      best_c = normalized(c);
+    }
+  }
+  if (int rc = wkg_batch_set_tuning(b, &best_c)) return rc;
+  if (best) *best = best_c;
+  if (best_ms) *best_ms = best_t;
+  return WKG_OK;
+}
+
+}  // extern "C"

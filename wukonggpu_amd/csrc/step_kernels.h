@@ -29,8 +29,6 @@ namespace {
 
 enum StepMode : int { kModeK2U = 0, kModeK2K = 1, kModeK2C = 2 };
 
-constexpr int kRowsPerThread = kTileRows / kBlock;
-constexpr int kRounds = kTileRows / kBlock;  // probe rounds: one row per lane per round
 
 // One step's view for the probe / scan / finalize passes.
 struct StepParams {
@@ -43,7 +41,7 @@ struct StepParams {
   uint32_t target_ok;  // K2C constant fits u32 (otherwise nothing matches)
   int mode;            // kMode*
   uint64_t key;        // != 0: single-key step (index / const start), R == 1
-  uint4 *hits;         // per tile, kTileRows entries: (row, offset, degree, 0) of rows with output
+  uint4 *hits;         // per tile, one entry per tile row: (row, offset, degree, 0) of rows with output
   uint2 *cinfo;        // K2U, compacted rows: (row, offset)
   uint64_t *cexcl;     // K2U, compacted rows: exclusive output-row offset
   int32_t *O;          // filters: compacted output, capacity R x c
@@ -121,18 +119,21 @@ __device__ __forceinline__ void group_member(const GraphView &g, uint64_t estart
   examined = found ? idx + 1 : deg;
 }
 
-// Probe pass over one tile (kTileRows rows; `lt` is the tile's index within
-// its step). Each round, the 4 waves probe 64 consecutive rows each. Writes
-// the tile's list of rows with output and its sums.
-template <bool SORTED, bool FRAMED>
+// Probe pass over one tile (TR rows; `lt` is the tile's index within its
+// step). Each round, the 4 waves probe 64 consecutive rows each. Writes the
+// tile's list of rows with output and its sums.
+template <int TR, bool SORTED, bool FRAMED>
 __device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams &p, uint64_t lt) {
-  __shared__ uint32_t s_cnt[kTileRows];  // degree, or kept flag
-  __shared__ uint32_t s_aux[kTileRows];  // segment offset, or edges examined
+  constexpr int kRounds = TR / kBlock;  // one row per lane per round
+  constexpr int kRowsPerThread = TR / kBlock;
+  static_assert(TR % kBlock == 0, "tile rows must be a multiple of the block");
+  __shared__ uint32_t s_cnt[TR];  // degree, or kept flag
+  __shared__ uint32_t s_aux[TR];  // segment offset, or edges examined
   __shared__ uint64_t s_tmp[kBlock / kWave];
   __shared__ WaveProbeLds s_wp[kBlock / kWave];
   const uint32_t tid = threadIdx.x, lane = lane_id(), gbase = lane & ~7u, l8 = lane & 7;
   const uint32_t wave = tid / kWave;
-  const uint64_t base = lt * kTileRows;
+  const uint64_t base = lt * TR;
   const DevMeta m = g.meta[p.pid < g.num_metas ? p.pid : 0];
 
   // all rounds' table reads are issued up front
@@ -199,7 +200,7 @@ __device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams 
   uint32_t pos = (uint32_t)block_excl_scan(nz, s_tmp, nz_total);
   (void)block_excl_scan(sum, s_tmp, sum_total);
   if (p.mode != kModeK2U) (void)block_excl_scan(ex, s_tmp, ex_total);
-  uint4 *hits = p.hits + lt * kTileRows;
+  uint4 *hits = p.hits + lt * TR;
 #pragma unroll
   for (int k = 0; k < kRowsPerThread; k++) {
     const uint32_t local = tid * kRowsPerThread + k;
@@ -215,7 +216,7 @@ __device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams 
 
 template <bool SORTED, bool FRAMED>
 __global__ __launch_bounds__(kBlock) void k_probe(GraphView g, StepParams p) {
-  probe_tile<SORTED, FRAMED>(g, p, blockIdx.x);
+  probe_tile<kTileRows, SORTED, FRAMED>(g, p, blockIdx.x);
 }
 
 // In-place exclusive scan of a[0, n) by one block; returns the total.
@@ -324,9 +325,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_top(StepParams p, uint64_t nchu
   }
 }
 
-// Finalize pass over one tile: K2U moves the tile's rows with edges to the
-// compacted row list with their exclusive output offsets; the filters copy
-// the kept rows, in order, to the compacted output.
+// Finalize pass over one tile of TR rows: K2U moves the tile's rows with
+// edges to the compacted row list with their exclusive output offsets; the
+// filters copy the kept rows, in order, to the compacted output.
+template <int TR>
 __device__ __forceinline__ void finalize_tile(const StepParams &p, uint64_t lt) {
   __shared__ uint64_t s_tmp[kBlock / kWave];
   __shared__ uint32_t s_sel[kBlock];
@@ -338,7 +340,7 @@ __device__ __forceinline__ void finalize_tile(const StepParams &p, uint64_t lt) 
   const uint64_t nz0 = nz_prefix(lt);
   const uint32_t nnz = (uint32_t)(nz_prefix(lt + 1) - nz0);
   const uint64_t out0 = p.tile_sum[lt] + (p.chunk_sum ? p.chunk_sum[lt / kScanChunk] : 0);
-  const uint4 *hits = p.hits + lt * kTileRows;
+  const uint4 *hits = p.hits + lt * TR;
   uint64_t carry = 0;
   for (uint32_t k0 = 0; k0 < nnz; k0 += kBlock) {
     const uint32_t j = k0 + tid;
@@ -367,13 +369,14 @@ __device__ __forceinline__ void finalize_tile(const StepParams &p, uint64_t lt) 
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_finalize(StepParams p) { finalize_tile(p, blockIdx.x); }
+__global__ __launch_bounds__(kBlock) void k_finalize(StepParams p) { finalize_tile<kTileRows>(p, blockIdx.x); }
 
 // Merge-path partition of the (row starts, output rows) sequence over the
 // compacted rows: part[k] = number of rows whose start item lies before
-// diagonal k * kExpandNV. Row i's start item sits at i + cexcl[i].
+// diagonal k * NV. Row i's start item sits at i + cexcl[i].
+template <int NV>
 __device__ __forceinline__ uint64_t merge_split(const uint64_t *cexcl, uint64_t R, uint64_t E, uint64_t k) {
-  const uint64_t diag = k * (uint64_t)kExpandNV < R + E ? k * (uint64_t)kExpandNV : R + E;
+  const uint64_t diag = k * (uint64_t)NV < R + E ? k * (uint64_t)NV : R + E;
   uint64_t lo = 0, hi = R;  // first i with i + cexcl[i] >= diag (R if none)
   while (lo < hi) {
     const uint64_t mid = (lo + hi) >> 1;
@@ -384,7 +387,7 @@ __device__ __forceinline__ uint64_t merge_split(const uint64_t *cexcl, uint64_t 
 
 __global__ void k_partition(const uint64_t *cexcl, uint64_t R, uint64_t E, uint64_t ntiles, uint64_t *part) {
   const uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (k <= ntiles) part[k] = merge_split(cexcl, R, E, k);
+  if (k <= ntiles) part[k] = merge_split<kExpandNV>(cexcl, R, E, k);
 }
 
 struct ExpandParams {
@@ -400,25 +403,27 @@ struct ExpandParams {
 };
 
 // Load-balanced expand (known_to_unknown result update): a tile owns
-// kExpandNV items of the merged sequence of row starts and output rows, so a
+// NV = kBlock * VT items of the merged sequence of row starts and output rows, so a
 // hub row's outputs spread over many workgroups. Output row o of compacted
 // row i is (input row, edge k) with k = o - cexcl[i]; rows come out in input
 // order and edges in list order (core/engine.hpp:142-150).
 // Phase 1 stages the tile's rows in LDS; phase 2 walks the merge path and
-// issues each thread's (up to kExpandVT) edge loads together, parking the
+// issues each thread's (up to VT) edge loads together, parking the
 // edges in LDS; phase 3 writes the tile's contiguous output region with
 // aligned 16-byte stores. `tile` is the tile index within the step.
-template <bool FRAMED>
+template <int VT, bool FRAMED>
 __device__ __forceinline__ void expand_tile(const GraphView &g, const ExpandParams &a, uint64_t tile) {
-  __shared__ int32_t s_ex[kExpandNV + 1];
-  __shared__ uint32_t s_off[kExpandNV + 1];
-  __shared__ uint32_t s_row[kExpandNV + 1];
-  __shared__ uint16_t s_orow[kExpandNV];
-  __shared__ uint32_t s_edge[kExpandNV];
+  constexpr int NV = kBlock * VT;
+  static_assert(NV < 65536, "local output rows are kept in 16 bits");
+  __shared__ int32_t s_ex[NV + 1];
+  __shared__ uint32_t s_off[NV + 1];
+  __shared__ uint32_t s_row[NV + 1];
+  __shared__ uint16_t s_orow[NV];
+  __shared__ uint32_t s_edge[NV];
   const uint32_t tid = threadIdx.x;
-  const uint64_t d0 = tile * (uint64_t)kExpandNV;
+  const uint64_t d0 = tile * (uint64_t)NV;
   const uint64_t dend = a.R + a.E;
-  const uint64_t d1 = d0 + kExpandNV < dend ? d0 + kExpandNV : dend;
+  const uint64_t d1 = d0 + NV < dend ? d0 + NV : dend;
   const uint64_t i0 = a.part[tile], i1 = a.part[tile + 1];
   const uint64_t j0 = d0 - i0, j1 = d1 - i1;
   const uint32_t nr = (uint32_t)(i1 - i0 + 1);  // local row r <-> compacted row i0 - 1 + r
@@ -438,7 +443,7 @@ __device__ __forceinline__ void expand_tile(const GraphView &g, const ExpandPara
   __syncthreads();
   // local merged position of local row r's start item: (r - 1) + s_ex[r]
   const int32_t span = (int32_t)(d1 - d0);
-  const int32_t x0 = (int32_t)(tid * kExpandVT);
+  const int32_t x0 = (int32_t)(tid * VT);
   const DevMeta m = g.meta[a.pid];
   if (x0 < span) {
     uint32_t lo = 1, hi = nr;  // rows r >= 1 whose start lies before x0
@@ -447,10 +452,10 @@ __device__ __forceinline__ void expand_tile(const GraphView &g, const ExpandPara
       if ((int32_t)(mid - 1) + s_ex[mid] < x0) lo = mid + 1; else hi = mid;
     }
     uint32_t cur = lo - 1;
-    int32_t oidx[kExpandVT];
-    uint32_t epos[kExpandVT];
+    int32_t oidx[VT];
+    uint32_t epos[VT];
 #pragma unroll
-    for (int v = 0; v < kExpandVT; v++) {
+    for (int v = 0; v < VT; v++) {
       const int32_t x = x0 + v;
       oidx[v] = -1;
       epos[v] = 0;
@@ -465,12 +470,12 @@ __device__ __forceinline__ void expand_tile(const GraphView &g, const ExpandPara
         }
       }
     }
-    uint32_t ev[kExpandVT];
+    uint32_t ev[VT];
 #pragma unroll
-    for (int v = 0; v < kExpandVT; v++)
+    for (int v = 0; v < VT; v++)
       ev[v] = oidx[v] >= 0 ? edge_at<FRAMED>(g, m.edge_start, m.eframe_base, epos[v]) : 0u;
 #pragma unroll
-    for (int v = 0; v < kExpandVT; v++)
+    for (int v = 0; v < VT; v++)
       if (oidx[v] >= 0) s_edge[oidx[v]] = ev[v];
   }
   __syncthreads();
@@ -512,7 +517,7 @@ __device__ __forceinline__ void expand_tile(const GraphView &g, const ExpandPara
 
 template <bool FRAMED>
 __global__ __launch_bounds__(kBlock) void k_expand(GraphView g, ExpandParams a) {
-  expand_tile<FRAMED>(g, a, blockIdx.x);
+  expand_tile<kExpandVT, FRAMED>(g, a, blockIdx.x);
 }
 
 // Single-key probe for index_to_unknown / const_to_unknown: out[0] = degree,

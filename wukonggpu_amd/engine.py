@@ -226,9 +226,31 @@ class GPUEngine:
         return self._many(N.engine_lib().wke_run_plain, queries)
 
     def run_batched(self, queries):
+        """Device-resident executor: whole queries on the device, one captured launch sequence."""
         return self._many(N.engine_lib().wke_run_batched, queries)
 
-    def serve(self, queries, arrival_s, heavy=None, table_budget_bytes=0):
+    def run_rounds(self, queries):
+        """Step-synchronous batching: one fused launch set per step round."""
+        return self._many(N.engine_lib().wke_run_rounds, queries)
+
+    def configure(self, max_queries=1024, max_patterns=8, arena_bytes=16 << 30):
+        L = N.engine_lib()
+        rc = L.wke_configure(self._h, max_queries, max_patterns, arena_bytes)
+        if rc != 0:
+            raise DeviceError(rc, "wke_configure failed")
+
+    def autotune(self, queries, reps=3):
+        """Autotunes the executor's launch and tile shapes on a sample batch; returns (shape, ms)."""
+        L = N.engine_lib()
+        flat, off = pack_queries(queries)
+        t = N.BatchTuning()
+        ms = C.c_double()
+        rc = L.wke_autotune(self._h, _i64p(flat), _i64p(off), len(queries), reps, C.byref(t), C.byref(ms))
+        if rc != 0:
+            raise DeviceError(rc, L.wke_last_error(self._h).decode())
+        return t.as_dict(), float(ms.value)
+
+    def serve(self, queries, arrival_s, heavy=None, table_budget_bytes=0, light_lanes=2):
         L = N.engine_lib()
         flat, off = pack_queries(queries)
         n = len(queries)
@@ -237,5 +259,69 @@ class GPUEngine:
         res = (N.Result * max(n, 1))()
         wall = C.c_double()
         L.wke_serve(self._h, _i64p(flat), _i64p(off), n, arr.ctypes.data_as(C.POINTER(C.c_double)),
-                    hv.ctypes.data_as(C.POINTER(C.c_uint8)), table_budget_bytes, res, C.byref(wall))
+                    hv.ctypes.data_as(C.POINTER(C.c_uint8)), table_budget_bytes, light_lanes, res,
+                    C.byref(wall))
         return self._collect(res, n), wall.value
+
+
+class Batch:
+    """Direct handle on one device-resident executor (wkg_batch_*)."""
+
+    def __init__(self, dev, max_queries, max_patterns, arena_bytes):
+        self.dev = dev
+        h = C.c_void_p()
+        dev._check(N.hip_lib().wkg_batch_create(dev._h, max_queries, max_patterns, arena_bytes, None,
+                                                 C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self.dev._h:
+            N.hip_lib().wkg_batch_destroy(self._h)
+        self._h = None
+
+    __del__ = close
+
+    @staticmethod
+    def _plans(plans):
+        steps = [s for p in plans for s in p]
+        arr = (N.PlanStep * max(len(steps), 1))()
+        for i, s in enumerate(steps):
+            arr[i].kind, arr[i].pid, arr[i].dir = s["kind"], s["pid"], s["dir"]
+            arr[i].start_col, arr[i].end_col = s.get("start_col", 0), s.get("end_col", 0)
+            arr[i].vid = s.get("vid", 0)
+        off = (C.c_int32 * (len(plans) + 1))()
+        for i, p in enumerate(plans):
+            off[i + 1] = off[i] + len(p)
+        return arr, off
+
+    def set_tuning(self, **kw):
+        t = N.BatchTuning(**kw)
+        self.dev._check(N.hip_lib().wkg_batch_set_tuning(self._h, C.byref(t)))
+
+    def tuning(self):
+        t = N.BatchTuning()
+        self.dev._check(N.hip_lib().wkg_batch_get_tuning(self._h, C.byref(t)))
+        return t.as_dict()
+
+    def run(self, plans):
+        """plans: list of lists of step dicts {kind, pid, dir, start_col, end_col, vid}.
+        Returns a list of (status, table ndarray, expanded)."""
+        arr, off = self._plans(plans)
+        n = len(plans)
+        res = (N.BatchResult * max(n, 1))()
+        cells = C.POINTER(C.c_int32)()
+        ncells = C.c_uint64()
+        self.dev._check(N.hip_lib().wkg_batch_run(self._h, n, arr, off, res, C.byref(cells), C.byref(ncells)))
+        try:
+            flat = (np.ctypeslib.as_array(cells, shape=(ncells.value,)).copy() if ncells.value
+                    else np.zeros(0, np.int32))
+        finally:
+            if cells:
+                N.engine_lib().wke_free(C.cast(cells, C.c_void_p))
+        out = []
+        for i in range(n):
+            r = res[i]
+            t = flat[r.offset:r.offset + r.rows * r.cols].reshape(r.rows, max(r.cols, 0)) if r.status == 0 \
+                else None
+            out.append((int(r.status), t, int(r.expanded)))
+        return out
