@@ -7,12 +7,18 @@ predicates, Zipf 1.0, a 500k-member type index), run on the plain GPU path
 (one pattern step at a time through the C++ engine, tables kept on the
 device, final table copied to the host). `value` = expanded edges/s.
 
-Beside it, on the same graph: the batched mix (512 selective 3-4 pattern
-queries from type indexes of log-uniform size) on the plain path and on the
-batched path, with identical answers required; the roofline of the dominant
-kernel (the expand of the hub fan-out step) from HIP events over the timed
-region; and the CPU oracle (a restatement of the reference's engine.hpp
-handlers) timed on the host as the reported baseline.
+Beside it, on the same graph, with identical answers required everywhere:
+  - the roofline of the dominant kernel (the expand of the hub fan-out step)
+    from HIP events over the timed region;
+  - the heavy query through the device-resident executor;
+  - the batched mix (512 selective 3-4 pattern queries from type indexes of
+    log-uniform size): plain path, step-synchronous rounds, and the
+    device-resident executor under the shape its autotuner picks;
+  - the served form: open-loop Poisson arrivals of a light/heavy mix at a
+    sweep of offered rates, p50/p99 latency and achieved queries/s, each
+    answer compared with the same query run alone;
+  - the CPU oracle (a restatement of the reference's engine.hpp handlers)
+    timed on the host as the reported baseline.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 Progress goes to stderr; one JSON line goes to stdout.
@@ -50,6 +56,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--served-n", type=int, default=2000, help="queries per served rate (0: skip)")
+    ap.add_argument("--served-rates", default="1000,2000,4000", help="offered Poisson rates, queries/s")
     ap.add_argument("--heavy-only", action="store_true",
                     help="run only the heavy query and the copy calibration (for profiler passes)")
     args = ap.parse_args()
@@ -115,24 +123,84 @@ def main():
         dev.close()
         return
 
-    # ---- batched mix: plain vs batched, identical answers -------------------------------
+    # ---- the heavy query through the device-resident executor ---------------------------
+    eng.configure(max_queries=1024, max_patterns=8, arena_bytes=32 << 30)
+    heavy_tuning, _ = eng.autotune([heavy], reps=2, heavy=True)  # shape of the heavy serving lane
+    eng_h = GPUEngine(dev)
+    eng_h.configure(max_queries=64, max_patterns=8, arena_bytes=32 << 30)
+    eng_h.autotune([heavy], reps=2)
+    hx = eng_h.run_batched([heavy])[0]
+    hx_lat = []
+    for _ in range(max(3, args.steps // 2)):
+        q0 = time.perf_counter()
+        hx = eng_h.run_batched([heavy])[0]
+        hx_lat.append(time.perf_counter() - q0)
+    eng_h.close()
+    heavy_exec_same = bool(hx.status == 0 and np.array_equal(hx.table, r.table) and hx.expanded == r.expanded)
+    log(f"heavy query via the executor (shape {heavy_tuning}): {pct(hx_lat, 0.5) * 1e3:.2f} ms "
+        f"(identical={heavy_exec_same})")
+
+    # ---- batched mix: plain vs step rounds vs the autotuned executor ---------------------
     mix = w.queries["batched"]
-    eng.run_batched(mix[:64])  # warm
+    t_tune = time.perf_counter()
+    tuning, tune_ms = eng.autotune(mix, reps=3)
+    t_tune = time.perf_counter() - t_tune
+    log(f"autotuned executor shape {tuning}: {tune_ms:.2f} ms per {len(mix)}-query batch ({t_tune:.1f} s)")
+    eng.run_rounds(mix[:64])  # warm
     tp = time.perf_counter()
     plain = eng.run_plain(mix)
     t_plain = time.perf_counter() - tp
-    tb = time.perf_counter()
-    batched = eng.run_batched(mix)
-    t_batched = time.perf_counter() - tb
-    same = all(a.status == 0 and b.status == 0 and a.table.shape == b.table.shape and np.array_equal(a.table, b.table)
-               and a.expanded == b.expanded for a, b in zip(plain, batched))
+    tr_ = time.perf_counter()
+    rounds = eng.run_rounds(mix)
+    t_rounds = time.perf_counter() - tr_
+    t_exec = []
+    for _ in range(5):
+        tb = time.perf_counter()
+        batched = eng.run_batched(mix)
+        t_exec.append(time.perf_counter() - tb)
+    t_batched = sorted(t_exec)[len(t_exec) // 2]
+
+    def same(xs, ys):
+        return all(a.status == 0 and b.status == 0 and a.table.shape == b.table.shape and
+                   np.array_equal(a.table, b.table) and a.expanded == b.expanded for a, b in zip(xs, ys))
+    same_exec, same_rounds = same(plain, batched), same(plain, rounds)
     mix_edges = sum(x.expanded for x in batched)
-    log(f"batched mix: plain {len(mix) / t_plain:.0f} q/s, batched {len(mix) / t_batched:.0f} q/s, "
-        f"identical={same}")
+    log(f"batched mix: plain {len(mix) / t_plain:.0f} q/s, step rounds {len(mix) / t_rounds:.0f} q/s, "
+        f"executor {len(mix) / t_batched:.0f} q/s (x{t_plain / t_batched:.1f}); identical={same_exec and same_rounds}")
+
+    # ---- served: open-loop Poisson arrivals, light + heavy lanes ----------------------------
+    served = []
+    served_same = None
+    if args.served_n > 0:
+        P = w.info["predicates"]
+        type_ids = list(range(2 + P + 1, 2 + P + w.info["types"]))
+        sq, sflags = workloads.served_mix(P, type_ids, w.info["t0"], args.served_n, args.seed + 11)
+        alone = eng.run_plain(sq)
+        served_same = True
+        rng = np.random.default_rng(args.seed + 12)
+        for rate in [float(x) for x in args.served_rates.split(",") if x]:
+            arr = np.cumsum(rng.exponential(1.0 / rate, size=len(sq)))
+            res, wall = eng.serve(sq, arr, heavy=sflags, table_budget_bytes=64 << 30, light_lanes=2)
+            ok = same(alone, res)
+            served_same = served_same and ok
+            lat_l = [x.latency_s for x, f in zip(res, sflags) if not f]
+            lat_h = [x.latency_s for x, f in zip(res, sflags) if f]
+            rec = {"offered_qps": rate, "queries": len(sq), "heavy": int(sflags.sum()),
+                   "achieved_qps": len(sq) / wall, "p50_ms": pct([x.latency_s for x in res], 0.5) * 1e3,
+                   "p99_ms": pct([x.latency_s for x in res], 0.99) * 1e3,
+                   "light_p50_ms": pct(lat_l, 0.5) * 1e3, "light_p99_ms": pct(lat_l, 0.99) * 1e3,
+                   "heavy_p50_ms": pct(lat_h, 0.5) * 1e3, "heavy_p99_ms": pct(lat_h, 0.99) * 1e3,
+                   "identical_to_alone": bool(ok)}
+            served.append(rec)
+            log(f"served @ {rate:.0f} q/s offered: {rec['achieved_qps']:.0f} q/s, p50 {rec['p50_ms']:.2f} ms, "
+                f"p99 {rec['p99_ms']:.2f} ms (light p99 {rec['light_p99_ms']:.2f}, heavy p99 "
+                f"{rec['heavy_p99_ms']:.2f}), identical={ok}")
 
     # ---- CPU baseline (the oracle), parity of the heavy result --------------------------
     cpu = None
-    parity = {"heavy_rows_equal": None, "mix_plain_equals_batched": bool(same)}
+    parity = {"heavy_rows_equal": None, "heavy_executor_equals_plain": heavy_exec_same,
+              "mix_plain_equals_executor": bool(same_exec), "mix_plain_equals_rounds": bool(same_rounds),
+              "served_equals_alone": served_same}
     if not args.no_cpu:
         import oracle
         want, wex, sec = oracle.run_query(w.gstore, heavy)
@@ -183,14 +251,18 @@ def main():
         "batched": {
             "workload": f"{len(mix)} selective 3-4 pattern queries from type indexes (sizes log-uniform 1..1e5)",
             "plain_queries_per_s": len(mix) / t_plain,
+            "rounds_queries_per_s": len(mix) / t_rounds,
             "batched_queries_per_s": len(mix) / t_batched,
             "speedup": t_plain / t_batched,
+            "executor": "device-resident: whole queries in one captured hipGraph, persistent tile-ticket kernels",
+            "tuning": tuning, "autotune_ms_per_batch": tune_ms, "autotune_s": t_tune,
             "plain_p50_ms": pct([x.latency_s for x in plain], 0.5) * 1e3,
             "plain_p99_ms": pct([x.latency_s for x in plain], 0.99) * 1e3,
-            "batched_p50_ms": pct([x.latency_s for x in batched], 0.5) * 1e3,
-            "batched_p99_ms": pct([x.latency_s for x in batched], 0.99) * 1e3,
+            "batched_ms_per_batch": t_batched * 1e3,
             "expanded_edges": int(mix_edges),
         },
+        "heavy_via_executor_ms": pct(hx_lat, 0.5) * 1e3, "heavy_tuning": heavy_tuning,
+        "served": served,
         "parity": parity,
     }
     print(json.dumps(line), flush=True)

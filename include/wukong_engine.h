@@ -67,8 +67,10 @@ int wke_run_rounds(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, in
 int wke_configure(wke_engine *e, int max_queries, int max_patterns, uint64_t arena_bytes);
 
 /* Runs the autotuner of the batched executor on these queries (reps timed
- * runs per candidate); the chosen shape stays in effect for the engine. */
-int wke_autotune(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, int reps,
+ * runs per candidate). The chosen shape stays in effect for the engine's
+ * executors: with heavy == 0 for wke_run_batched and the light serving lanes,
+ * with heavy != 0 for the heavy serving lane. */
+int wke_autotune(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, int reps, int heavy,
                  wkg_batch_tuning *best, double *best_ms);
 
 /* Served path: an open-loop arrival stream. Query q arrives at arrival_s[q]
@@ -77,11 +79,13 @@ int wke_autotune(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int 
  * `light_lanes` device executors, heavy queries on one more, each with its
  * own stream and arena; the arenas together are the table budget
  * (table_budget_bytes; half to the heavy lane, the rest split over the light
- * lanes). Whenever a lane is free, every query waiting in its queue (up to
- * the executor's capacity) is admitted as one device batch, so work that
- * arrives while other batches run overlaps them on the device. A query that
- * does not fit its lane's arena beside the others is retried alone, then on
- * the plain path. latency_s in each result is completion - arrival. */
+ * lanes). Whenever a lane is free, the queries waiting in its queue are
+ * admitted as one device batch, as many as its arena holds at the bytes per
+ * query the lane has seen (its arena high-water mark over its batch size; a
+ * heavy lane starts with one query), so work that arrives while other
+ * batches run overlaps them on the device. A query that does not fit its
+ * lane's arena beside the others is retried alone, then on the plain path.
+ * latency_s in each result is completion - arrival. */
 int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, const double *arrival_s,
               const uint8_t *heavy, uint64_t table_budget_bytes, int light_lanes, wke_result *res,
               double *wall_s);

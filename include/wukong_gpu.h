@@ -172,8 +172,8 @@ int wkg_step_batch(wkg_ctx *ctx, int n, const wkg_step_desc *steps, void *stream
  * carves its tables and scratch out of an HBM arena; persistent kernels then
  * take the probe, finalize and expand tiles of all queries from one shared
  * work queue (an atomic tile ticket over the concatenated tile space). The
- * whole sequence -- plan upload, every level, the gather of all result tables
- * and the read-back of the result records -- is captured once in a hipGraph
+ * whole sequence -- plan upload, every level, the retirement of finished
+ * queries' tables and the read-back of the result records -- is captured once in a hipGraph
  * and replayed per batch. Results are the same, cell for cell, as the plain
  * per-step calls give. A query whose tables do not fit the arena ends with
  * status WKG_ENOMEM and can be rerun on the plain path. */
@@ -206,8 +206,11 @@ typedef struct wkg_batch_result {
 } wkg_batch_result;
 
 /* max_queries and max_patterns bound a batch; arena_bytes of HBM (taken from
- * the table budget) hold all intermediate tables, step scratch and the
- * gathered results. stream may be NULL (a stream of the batch's own). */
+ * the table budget) hold the intermediate tables and step scratch. Level l's
+ * tables go to one of two stacks by the parity of l, so only a level's
+ * inputs, outputs and scratch are live at once; a query's last table is
+ * retired to the host as soon as its last step ends. stream may be NULL (a
+ * stream of the batch's own). */
 int wkg_batch_create(wkg_ctx *ctx, int max_queries, int max_patterns, uint64_t arena_bytes,
                      void *stream, wkg_batch **out);
 int wkg_batch_destroy(wkg_batch *b);
@@ -218,12 +221,21 @@ int wkg_batch_get_tuning(const wkg_batch *b, wkg_batch_tuning *t);
 int wkg_batch_launch(wkg_batch *b, int nq, const wkg_plan_step *steps, const int32_t *step_off);
 /* 1 while the launched batch runs, 0 once it is done. */
 int wkg_batch_busy(wkg_batch *b);
-/* Waits for the launched batch. res[nq] gets the records; *cells (malloc'd,
- * release with free()) holds all result tables back to back, *ncells cells. */
-int wkg_batch_collect(wkg_batch *b, wkg_batch_result *res, int32_t **cells, uint64_t *ncells);
+/* Waits for the launched batch. res[nq] gets the records; *cells points at
+ * the batch's result buffer (pinned host memory the retire kernels write
+ * directly), which holds every result table at its record's offset and stays
+ * valid until the next launch on this batch. A query whose table did not fit
+ * the buffer ends with WKG_ENOMEM; the buffer grows for the next launch. */
+int wkg_batch_collect(wkg_batch *b, wkg_batch_result *res, const int32_t **cells, uint64_t *ncells);
 /* launch + collect */
 int wkg_batch_run(wkg_batch *b, int nq, const wkg_plan_step *steps, const int32_t *step_off,
-                  wkg_batch_result *res, int32_t **cells, uint64_t *ncells);
+                  wkg_batch_result *res, const int32_t **cells, uint64_t *ncells);
+/* Sets the result buffer to `bytes` (default 64 MiB; it also grows by itself
+ * after a batch whose results overflowed it). */
+int wkg_batch_reserve_results(wkg_batch *b, uint64_t bytes);
+/* Of the last collected batch: the arena's high-water mark and the result
+ * bytes (for admission against a memory budget). */
+int wkg_batch_stats(const wkg_batch *b, uint64_t *arena_peak, uint64_t *result_bytes);
 /* Autotuner: times `reps` runs of the given batch under each candidate
  * launch/tile shape, keeps the fastest (median) and returns it. */
 int wkg_batch_autotune(wkg_batch *b, int nq, const wkg_plan_step *steps, const int32_t *step_off,

@@ -123,6 +123,33 @@ def test_batch_replays_with_eager_work_between(dev_small, graph_small, eng_small
     b.close()
 
 
+def test_result_buffer_overflow_then_growth(dev_small, eng_small):
+    """Results beyond the host result buffer end in ENOMEM (in query order);
+    the buffer grows for the next launch, which then answers every query."""
+    rng = np.random.default_rng(27)
+    qs = random_queries(rng, 300, 4, [6, 7], 80, max_steps=4)
+    want = [eng_small.run_query(q) for q in qs]
+    total = sum(w.table.nbytes for w in want)
+    assert total > 16384
+    b = Batch(dev_small, 80, 6, 256 << 20)
+    b.reserve_results(8192)
+    print("result buffer of 8 KiB", flush=True)
+    got = b.run([_plan(q) for q in qs])
+    st = [g[0] for g in got]
+    assert -2 in st and 0 in st
+    first_fail = st.index(-2)
+    for i, (s_, tab, ex) in enumerate(got):
+        if s_ == 0:
+            assert np.array_equal(tab, want[i].table) and ex == want[i].expanded, i
+    peak, rbytes = b.stats()
+    assert peak > 0 and rbytes <= 8192
+    got = b.run([_plan(q) for q in qs])  # grown
+    for i, (s_, tab, ex) in enumerate(got):
+        assert s_ == 0 and np.array_equal(tab, want[i].table) and ex == want[i].expanded, i
+    assert first_fail >= 0
+    b.close()
+
+
 def test_batch_rejects_bad_plans(dev_small):
     b = Batch(dev_small, 4, 3, 16 << 20)
     bad = [

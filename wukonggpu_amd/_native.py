@@ -142,6 +142,8 @@ def hip_lib():
         _sig(L, "wkg_batch_busy", C.c_int, vp)
         _sig(L, "wkg_batch_collect", C.c_int, vp, P(BatchResult), P(P(i32)), P(u64))
         _sig(L, "wkg_batch_run", C.c_int, vp, C.c_int, P(PlanStep), P(i32), P(BatchResult), P(P(i32)), P(u64))
+        _sig(L, "wkg_batch_stats", C.c_int, vp, P(u64), P(u64))
+        _sig(L, "wkg_batch_reserve_results", C.c_int, vp, u64)
         _sig(L, "wkg_batch_autotune", C.c_int, vp, C.c_int, P(PlanStep), P(i32), C.c_int, P(BatchTuning), P(dbl))
         _hip = L
     return _hip
@@ -161,7 +163,7 @@ def engine_lib():
         _sig(L, "wke_run_batched", C.c_int, vp, P(i64), P(i64), C.c_int, P(Result))
         _sig(L, "wke_run_rounds", C.c_int, vp, P(i64), P(i64), C.c_int, P(Result))
         _sig(L, "wke_configure", C.c_int, vp, C.c_int, C.c_int, u64)
-        _sig(L, "wke_autotune", C.c_int, vp, P(i64), P(i64), C.c_int, C.c_int, P(BatchTuning), P(dbl))
+        _sig(L, "wke_autotune", C.c_int, vp, P(i64), P(i64), C.c_int, C.c_int, C.c_int, P(BatchTuning), P(dbl))
         _sig(L, "wke_serve", C.c_int, vp, P(i64), P(i64), C.c_int, P(dbl), P(C.c_uint8), u64, C.c_int,
              P(Result), P(dbl))
         _engine = L

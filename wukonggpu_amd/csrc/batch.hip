@@ -35,7 +35,7 @@ __device__ __forceinline__ uint32_t find_item(const uint64_t *base, uint32_t n, 
 }
 
 template <bool SORTED, bool FRAMED>
-__global__ __launch_bounds__(kBlock) void k_batch_probe(GraphView g, const BatchItem *items,
+__global__ __launch_bounds__(kBlock, 3) void k_batch_probe(GraphView g, const BatchItem *items,
                                                         const uint64_t *tile_base, uint32_t n) {
   __shared__ uint32_t s_q;
   if (threadIdx.x == 0) s_q = find_item(tile_base, n, blockIdx.x);
@@ -48,13 +48,13 @@ __global__ __launch_bounds__(kBlock) void k_batch_scan(const BatchItem *items) {
   scan_step_tiles(items[blockIdx.x].p);
 }
 
+// One wave per tile.
 __global__ __launch_bounds__(kBlock) void k_batch_finalize(const BatchItem *items, const uint64_t *tile_base,
                                                            uint32_t n) {
-  __shared__ uint32_t s_q;
-  if (threadIdx.x == 0) s_q = find_item(tile_base, n, blockIdx.x);
-  __syncthreads();
-  const uint32_t q = s_q;
-  finalize_tile<kTileRows>(items[q].p, blockIdx.x - tile_base[q]);
+  const uint64_t t = blockIdx.x * (uint64_t)(kBlock / kWave) + threadIdx.x / kWave;
+  if (t >= tile_base[n]) return;
+  const uint32_t q = find_item(tile_base, n, t);
+  finalize_tile<kTileRows>(items[q].p, t - tile_base[q]);
 }
 
 __global__ void k_batch_partition(const BatchItem *items, const uint64_t *part_base, uint32_t n, uint64_t *part) {
@@ -212,7 +212,7 @@ extern "C" int wkg_step_batch(wkg_ctx *ctx, int n, const wkg_step_desc *steps, v
   }
   if (NT) {
     ProfScope ps(ctx, s, "batch_finalize", 0.0);
-    k_batch_finalize<<<(unsigned)NT, kBlock, 0, s>>>(d_items, d_tileb, n);
+    k_batch_finalize<<<(unsigned)((NT + kBlock / kWave - 1) / (kBlock / kWave)), kBlock, 0, s>>>(d_items, d_tileb, n);
   }
   if ((e = hipGetLastError()) != hipSuccess)
     return bail(ctx->fail(WKG_EDEVICE, "batched probe launch failed: %s", hipGetErrorString(e)));

@@ -86,6 +86,9 @@ struct WaveProbeLds {
 // Cluster-chaining probe of 64 consecutive rows, one per lane, all under one
 // predicate (get_vertex_local, core/gstore.hpp:389-409; device form
 // core/gpu_hash.cu:181-234).
+//  0. The caller's carry holds the key of the row just before lane 0 (the
+//     previous call's lane 63) and its result: a run of equal keys that
+//     continues from the previous call is not probed again.
 //  1. A run of equal keys in consecutive lanes is probed once: its first lane
 //     (the run leader) hashes the key and parks (key, bucket) in LDS at its
 //     rank among the leaders.
@@ -98,14 +101,21 @@ struct WaveProbeLds {
 //  3. Results go back through LDS; every lane reads its run leader's.
 // Every lane of the wave must call this with the wave's own `w`. key == 0
 // means "no key". Returns the lane's degree (0 on a miss) and list offset
-// relative to the predicate's edge_start.
+// relative to the predicate's edge_start, and moves the carry to lane 63.
+// (The carry starts as key 0: valid keys are never 0.)
+struct ProbeCarry {
+  uint64_t key = 0;
+  uint32_t deg = 0, off = 0;
+};
+
 template <bool FRAMED>
 __device__ __forceinline__ void wave_probe(const GraphView &g, const DevMeta &m, uint64_t key, WaveProbeLds &w,
-                                           uint32_t &deg_out, uint32_t &off_out) {
+                                           uint32_t &deg_out, uint32_t &off_out, ProbeCarry &carry) {
   const uint32_t lane = lane_id(), l8 = lane & 7, grp = lane >> 3, gbase = lane & ~7u;
   const bool valid = key != 0 && m.psz != 0;
-  const uint64_t prev = shfl64(key, (lane - 1) & 63);
-  const bool lead = valid && (lane == 0 || key != prev);
+  const uint64_t before = shfl64(key, (lane - 1) & 63);
+  const uint64_t prev = lane == 0 ? carry.key : before;
+  const bool lead = valid && key != prev;
   const uint64_t leaders = wave_ballot(lead);
   const uint32_t L = (uint32_t)__builtin_popcountll(leaders);
   if (lead) {
@@ -182,12 +192,21 @@ __device__ __forceinline__ void wave_probe(const GraphView &g, const DevMeta &m,
   deg_out = 0;
   off_out = 0;
   if (valid) {
-    const uint32_t jl = (uint32_t)__builtin_popcountll(leaders & ((2ull << lane) - 1)) - 1;
-    const uint2 r = w.res[jl];
-    deg_out = r.x;
-    off_out = r.y;
+    // leaders at or below this lane; none: the lane continues the carried run
+    const uint32_t nl = (uint32_t)__builtin_popcountll(leaders & ((2ull << lane) - 1));
+    if (nl) {
+      const uint2 r = w.res[nl - 1];
+      deg_out = r.x;
+      off_out = r.y;
+    } else {
+      deg_out = carry.deg;
+      off_out = carry.off;
+    }
   }
   __builtin_amdgcn_wave_barrier();  // w is reused by the wave's next call
+  carry.key = shfl64(key, kWave - 1);
+  carry.deg = shfl32(deg_out, kWave - 1);
+  carry.off = shfl32(off_out, kWave - 1);
 }
 
 // Inclusive wave scan of u64 values.

@@ -153,8 +153,8 @@ struct wke_engine {
   // device-resident executor shape (wke_configure) and tuning (wke_autotune)
   int cfg_queries = 1024, cfg_patterns = 8;
   uint64_t cfg_arena = 16ull << 30;
-  bool tuned = false;
-  wkg_batch_tuning tuning{};
+  bool tuned = false, tuned_heavy = false;
+  wkg_batch_tuning tuning{}, tuning_heavy{};  // light / default executors, heavy lane
   wkg_batch *exec = nullptr;
 
   ~wke_engine() { drop_exec(); }
@@ -162,11 +162,13 @@ struct wke_engine {
     if (exec) wkg_batch_destroy(exec);
     exec = nullptr;
   }
-  // A new executor with the engine's shape and tuning (arena_bytes 0: the default).
-  int new_exec(uint64_t arena_bytes, wkg_batch **out) {
+  // A new executor with the engine's shape and the tuning of its kind
+  // (arena_bytes 0: the default).
+  int new_exec(uint64_t arena_bytes, bool heavy, wkg_batch **out) {
     int rc = wkg_batch_create(ctx, cfg_queries, cfg_patterns, arena_bytes ? arena_bytes : cfg_arena, nullptr, out);
     if (rc) return fail_ctx(rc);
-    if (tuned && (rc = wkg_batch_set_tuning(*out, &tuning))) {
+    const bool have = heavy ? tuned_heavy : tuned;
+    if (have && (rc = wkg_batch_set_tuning(*out, heavy ? &tuning_heavy : &tuning))) {
       wkg_batch_destroy(*out);
       *out = nullptr;
       return fail_ctx(rc);
@@ -175,7 +177,7 @@ struct wke_engine {
   }
   int get_exec(wkg_batch **out) {
     if (!exec)
-      if (int rc = new_exec(0, &exec)) return rc;
+      if (int rc = new_exec(0, false, &exec)) return rc;
     *out = exec;
     return WKG_OK;
   }
@@ -439,12 +441,9 @@ int wke_run_batched(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, i
       if (res[q].status == WKG_OK && !plans[q].empty() && (int)plans[q].size() <= e->cfg_patterns)
         add_plan(c, plans[q], q);
     if (c.who.empty()) break;
-    int32_t *cells = nullptr;
+    const int32_t *cells = nullptr;
     int rc = wkg_batch_run(b, (int)c.who.size(), c.steps.data(), c.off.data(), rec.data(), &cells, nullptr);
-    if (rc) {
-      free(cells);
-      return e->fail_ctx(rc);
-    }
+    if (rc) return e->fail_ctx(rc);
     const double lat = since();
     for (size_t i = 0; i < c.who.size(); i++) {
       const int qi = c.who[i];
@@ -456,7 +455,6 @@ int wke_run_batched(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, i
       res[qi].latency_s = lat;
       note(res[qi].status);
     }
-    free(cells);
   }
   for (int qi : plain) {  // beyond the executor's shape or its arena
     int rc = wke_run_query(e, cmd + cmd_off[qi], (int)(cmd_off[qi + 1] - cmd_off[qi]), &res[qi]);
@@ -466,7 +464,7 @@ int wke_run_batched(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, i
   return first;
 }
 
-int wke_autotune(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, int reps,
+int wke_autotune(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, int reps, int heavy,
                  wkg_batch_tuning *best, double *best_ms) {
   if (!e || !cmd || !cmd_off || nq < 1 || reps < 1) return WKG_EINVAL;
   Compiled c;
@@ -479,13 +477,19 @@ int wke_autotune(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int 
   }
   if (c.who.empty()) return e->fail(WKG_EINVAL, "no query of the sample compiles to a device plan");
   wkg_batch *b = nullptr;
-  if (int rc = e->get_exec(&b)) return rc;
+  if (heavy) {  // a separate executor, so the default one keeps its captured sequences
+    if (int rc = e->new_exec(0, true, &b)) return rc;
+  } else if (int rc = e->get_exec(&b)) {
+    return rc;
+  }
   wkg_batch_tuning t{};
   double ms = 0;
-  if (int rc = wkg_batch_autotune(b, (int)c.who.size(), c.steps.data(), c.off.data(), reps, &t, &ms))
-    return e->fail_ctx(rc);
-  e->tuning = t;
-  e->tuned = true;
+  int rc = wkg_batch_autotune(b, (int)c.who.size(), c.steps.data(), c.off.data(), reps, &t, &ms);
+  if (rc) e->fail_ctx(rc);
+  if (heavy) wkg_batch_destroy(b);
+  if (rc) return rc;
+  (heavy ? e->tuning_heavy : e->tuning) = t;
+  (heavy ? e->tuned_heavy : e->tuned) = true;
   if (best) *best = t;
   if (best_ms) *best_ms = ms;
   return WKG_OK;
@@ -493,11 +497,15 @@ int wke_autotune(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int 
 
 namespace {
 
-// One device executor of the served path with the queries it runs.
+// One device executor of the served path with the queries it runs. `est`
+// is the arena bytes per query seen so far (its high-water mark over the
+// batch size); a batch admits at most arena / est queries.
 struct Lane {
   wkg_batch *b = nullptr;
   bool heavy = false;
   bool busy = false;
+  uint64_t arena = 0;
+  double est = 0;
   std::vector<int> run;
 };
 
@@ -526,7 +534,8 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
   for (int i = 0; i < light_lanes + (any_heavy ? 1 : 0); i++) {
     Lane l;
     l.heavy = i == light_lanes;
-    if (int rc = e->new_exec(l.heavy ? heavy_arena : light_arena, &l.b)) {
+    l.arena = l.heavy ? heavy_arena : light_arena;
+    if (int rc = e->new_exec(l.arena, l.heavy, &l.b)) {
       release();
       return rc;
     }
@@ -571,10 +580,13 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
     // retire finished batches
     for (Lane &l : lanes) {
       if (!l.busy || wkg_batch_busy(l.b)) continue;
-      int32_t *cells = nullptr;
+      const int32_t *cells = nullptr;
       int rc = wkg_batch_collect(l.b, rec.data(), &cells, nullptr);
       l.busy = false;
       progressed = true;
+      uint64_t peak = 0;
+      if (rc == WKG_OK && wkg_batch_stats(l.b, &peak, nullptr) == WKG_OK && !l.run.empty())
+        l.est = std::max(l.est, (double)peak / (double)l.run.size());
       std::deque<int> &w = wait[l.heavy ? 1 : 0];
       for (size_t i = 0; i < l.run.size(); i++) {
         const int q = l.run[i];
@@ -591,7 +603,6 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
           done(q, res[q].status);
         }
       }
-      free(cells);
       l.run.clear();
     }
     // start every idle lane on what waits in its queue
@@ -599,11 +610,16 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
       std::deque<int> &w = wait[l.heavy ? 1 : 0];
       if (l.busy || w.empty()) continue;
       Compiled c;
+      // admission: what the lane's arena holds at the bytes per query seen so
+      // far (a heavy lane starts with one query to learn it)
+      int cap_n = e->cfg_queries;
+      if (l.est > 0) cap_n = (int)std::min<double>(cap_n, std::max(1.0, 0.9 * (double)l.arena / l.est));
+      else if (l.heavy) cap_n = 1;
       if (solo[w.front()]) {
         add_plan(c, plans[w.front()], w.front());
         w.pop_front();
       } else {
-        while (!w.empty() && !solo[w.front()] && (int)c.who.size() < e->cfg_queries) {
+        while (!w.empty() && !solo[w.front()] && (int)c.who.size() < cap_n) {
           add_plan(c, plans[w.front()], w.front());
           w.pop_front();
         }

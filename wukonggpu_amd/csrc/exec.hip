@@ -5,30 +5,34 @@
 // GPU_Engine::execute_one_step dispatches, core/gpu_engine.hpp:395-467). Step
 // level l (the l-th pattern of every query that has one) runs as
 //
-//   k_level_plan   1 block: per query, size the step (rows, tiles, scratch,
-//                  filter output), carve it out of the HBM arena, write the
-//                  step's descriptor and the tile prefix; reset the tickets
-//   k_x_probe      persistent: tiles of all queries from one atomic ticket
-//   k_x_scan       persistent over queries: tile sums -> prefixes, totals
-//   k_x_finalize   persistent: compaction tiles from a second ticket
-//   k_expand_plan  1 block: per query, read the totals, allocate the
-//                  known_to_unknown output, commit the query's new table
-//   k_x_partition  persistent: merge-path split points of all expands
-//   k_x_expand     persistent: expand tiles from a third ticket
+//   k_level_plan     1 block: per query, size the step (rows, tiles, scan
+//                    chunks, scratch, filter output), carve it out of the HBM
+//                    arena, write the step's descriptor and the tile prefix
+//   k_x_probe        persistent: probe tiles of all queries from the work queue
+//   k_x_scan_chunks  persistent: tile sums -> chunk-local prefixes
+//   k_x_scan_top     persistent over queries of more than one chunk
+//   k_x_finalize     persistent: compaction tiles, one wave per tile
+//   k_expand_plan    1 block: per query, read the totals, allocate the
+//                    known_to_unknown output, commit the query's new table,
+//                    give finishing queries their place in the host buffer
+//   k_x_partition    persistent: merge-path split points of all expands
+//   k_x_expand       persistent: expand tiles from a second work queue
+//   k_x_retire       persistent: finished queries' tables -> host buffer
 //
-// and after the last level k_result_plan + k_x_gather pack every result
-// table into one contiguous region whose records are copied to the host.
-// The per-tile bodies are the plain path's (step_kernels.h), so each query's
+// and k_x_records writes the per-query records that are copied back. The
+// per-tile bodies are the plain path's (step_kernels.h), so each query's
 // table is the one the per-step calls produce, cell for cell.
 //
-// Memory: one arena per batch object. Tables grow up from its bottom and
-// stay until the batch ends (a table is the next level's input); each
-// level's scratch is carved down from its top and dropped at the next level.
-// A query whose needs do not fit ends with WKG_ENOMEM; which queries fit is
-// decided in query order, so the outcome is deterministic.
+// Memory: one arena per batch object. Level l's output tables go to stack
+// (l & 1) (stack 0 grows up from the bottom, stack 1 down from the top),
+// whose previous contents -- level l-2's tables -- are dead by then; the
+// level's scratch is carved from the other end of the free window between
+// the stacks. So only one level's inputs, outputs and scratch are live at a
+// time. A query whose needs do not fit ends with WKG_ENOMEM; which queries
+// fit is decided in query order, so the outcome is deterministic.
 //
 // The launch sequence for a given level count is captured once as a hipGraph
-// (no memset nodes: counters are reset by the plan kernels) and replayed.
+// (no memset nodes: the plan kernels reset the counters) and replayed.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -48,28 +52,37 @@ namespace {
 constexpr uint64_t kAlign = 256;
 __host__ __device__ __forceinline__ uint64_t al(uint64_t b) { return (b + kAlign - 1) & ~(kAlign - 1); }
 
-constexpr int32_t kInactive = 1;          // QState::status of an unused slot
-constexpr uint64_t kGatherUnit = 16384;   // bytes per gather work unit
+constexpr int32_t kInactive = 1;         // QState::status of an unused slot
+constexpr uint64_t kCopyUnit = 16384;    // bytes per retire work unit
 constexpr uint64_t kRowLimit = 1ull << 32;
+constexpr uint32_t kSubQueues = 8;       // one per XCD (blocks are dealt to the XCDs round-robin)
 
 struct QState {
   const int32_t *T;   // current table (null before the first step)
   uint64_t rows;
   uint64_t expanded;
+  uint64_t out_off;   // first cell in the host result buffer, once retired
   uint32_t cols;
   int32_t status;     // WKG_OK, a WKG_E* code, or kInactive
   uint32_t nsteps;
-  uint32_t pad;
+  uint32_t nchunks;   // tile-scan chunks of the current step
 };
 
 struct ExecState {
   uint64_t nq;
-  uint64_t table_top;    // arena bytes used by tables, from the bottom
-  uint64_t scratch_low;  // bottom of this level's scratch (scratch is [scratch_low, cap))
-  uint64_t nt, np, nx, nu;
-  uint64_t gather_base, gather_bytes;
-  uint64_t ticket[4];    // work-queue heads: probe, finalize, expand, spare
-  uint64_t pad[3];
+  uint64_t lo_top;     // table stack 0 (even levels' outputs): [0, lo_top)
+  uint64_t hi_low;     // table stack 1 (odd levels' outputs): [hi_low, cap)
+  uint64_t w0, w1;     // this level's free window
+  uint64_t out_used;   // this level's outputs, bytes from the window's output end
+  uint64_t scr_used;   // this level's scratch, bytes from the other end
+  uint64_t nt, nc, np, nx, nu;  // probe tiles, scan chunks, partition entries, expand tiles, retire units
+  uint64_t res_bytes;  // host result buffer bytes used
+  uint64_t res_over;   // result bytes refused for lack of host buffer
+  uint64_t peak;       // arena high-water mark
+  uint64_t pad[2];
+  // work queues of the probe (0) and expand (1) passes: 8 sub-queue heads
+  // each, one 64-byte line apart so their atomics do not serialise on one line
+  uint64_t queue[2][kSubQueues][8];
 };
 
 struct Item {
@@ -83,16 +96,34 @@ struct ExecArgs {
   wkg_batch_result *res;  // [Q], right after st (one read-back copy)
   QState *qs;             // [Q]
   Item *items;            // [Q]
-  uint64_t *tile_base;    // [Q+1] probe/finalize tiles, later gather units
+  uint64_t *tile_base;    // [Q+1] probe / finalize tiles
+  uint64_t *chunk_base;   // [Q+1] tile-scan chunks
   uint64_t *part_base;    // [Q+1]
   uint64_t *xtile_base;   // [Q+1]
+  uint64_t *unit_base;    // [Q+1] retire copy units
   const uint64_t *up_nq;  // uploaded: query count
   const uint32_t *up_nsteps;       // uploaded: [Q]
   const wkg_plan_step *up_steps;   // uploaded: [Q * P], query q's level l at q * P + l
   char *arena;
   uint64_t cap;
+  char *hres;             // host result buffer (device-visible pinned memory)
+  uint64_t hcap;
   uint32_t Q, P;
 };
+
+// Arena placement. Level l writes its output tables into stack (l & 1) --
+// stack 0 grows up from 0, stack 1 down from cap -- after dropping that
+// stack's old contents (level l-2's tables: dead, since their queries either
+// moved on or were retired to the host). Scratch is carved from the other
+// end of the window between the stacks, towards the outputs.
+__device__ __forceinline__ char *out_at(const ExecArgs &a, uint32_t level, uint64_t w0, uint64_t w1, uint64_t off,
+                                        uint64_t bytes) {
+  return (level & 1) ? a.arena + w1 - off - bytes : a.arena + w0 + off;
+}
+__device__ __forceinline__ char *scr_at(const ExecArgs &a, uint32_t level, uint64_t w0, uint64_t w1, uint64_t off,
+                                        uint64_t bytes) {
+  return (level & 1) ? a.arena + w0 + off : a.arena + w1 - off - bytes;
+}
 
 // Last q in [0, n] with base[q] <= t, for base ascending with base[n] > t.
 // Wave 0 runs a 64-ary search (each lane tests one pivot; the lanes that pass
@@ -116,6 +147,22 @@ __device__ __forceinline__ uint32_t block_find(const uint64_t *base, uint32_t n,
   return *s_q;
 }
 
+// The same search done by one wave for itself (the result is wave-uniform).
+__device__ __forceinline__ uint32_t wave_find(const uint64_t *base, uint32_t n, uint64_t t) {
+  const uint32_t lane = lane_id();
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t span = hi - lo;
+    const uint32_t pos = lo + (uint32_t)(((uint64_t)span * (lane + 1)) / (kWave + 1));
+    const bool le = base[pos] <= t;
+    const uint32_t k = (uint32_t)__builtin_popcountll(wave_ballot(le));
+    const uint32_t plo = shfl32(pos, k ? k - 1 : 0), phi = shfl32(pos, k < kWave ? k : 0);
+    if (k) lo = plo;
+    if (k < kWave) hi = phi;
+  }
+  return lo;
+}
+
 // Thread-level binary search with the same contract.
 __device__ __forceinline__ uint32_t thread_find(const uint64_t *base, uint32_t n, uint64_t t) {
   uint32_t lo = 0, hi = n;
@@ -137,6 +184,72 @@ __device__ __forceinline__ bool level_active(const QState &s, uint32_t level) {
   return s.status == WKG_OK && level < s.nsteps;
 }
 
+// The shared work queue over a pass's flattened tile space (all queries'
+// tiles back to back). It is split into kSubQueues interleaved sub-queues --
+// sub-queue k hands out chunks k, k + 8, k + 16, ... of K consecutive tiles
+// -- and block b draws from sub-queue b % 8, so the heads' atomics spread
+// over eight addresses and each draw covers K tiles. K gives each block
+// about four draws. Thread 0 draws the next chunk while the current one is
+// worked on. The block's queue position and the query of its current tile
+// (consecutive tiles mostly belong to one query) live in LDS, so the tile
+// bodies keep their registers.
+struct QueueLds {
+  uint64_t cur, end, nxt, t;
+  uint64_t qlo, qhi;  // tile range of query q
+  uint32_t q, sub;
+  uint32_t need;      // tile t is outside [qlo, qhi)
+};
+
+struct BlockQueue {
+  QueueLds &s;
+  unsigned long long *head;
+  uint64_t n, K;
+  __device__ BlockQueue(QueueLds &lds, uint64_t (*heads)[8], uint64_t total) : s(lds), n(total) {
+    const uint32_t sub = blockIdx.x % kSubQueues;
+    head = (unsigned long long *)heads[sub];
+    const uint64_t per = (uint64_t)gridDim.x * 4;
+    K = total > per ? total / per : 1;
+    if (threadIdx.x == 0) {
+      s.cur = s.end = 0;
+      s.sub = sub;
+      s.qlo = 1;
+      s.qhi = 0;
+      s.nxt = atomicAdd(head, 1ull);
+    }
+  }
+  // Next tile and its query; false once the sub-queue is drained.
+  __device__ bool next(const uint64_t *base, uint32_t nq, uint64_t &t, uint32_t &q, uint64_t &lt) {
+    __syncthreads();  // the previous tile's LDS use is over
+    if (threadIdx.x == 0) {
+      if (s.cur >= s.end) {
+        const uint64_t c = s.sub + (uint64_t)kSubQueues * s.nxt;
+        s.cur = c * K;
+        s.end = s.cur + K < n ? s.cur + K : n;
+        if (s.cur < n) s.nxt = atomicAdd(head, 1ull);
+      }
+      s.t = s.cur < s.end ? s.cur++ : n;
+      s.need = s.t < n && (s.t < s.qlo || s.t >= s.qhi);
+    }
+    __syncthreads();
+    t = s.t;
+    if (t >= n) return false;
+    if (s.need) {  // block-uniform: wave 0 finds the tile's query
+      if (threadIdx.x < kWave) {
+        const uint32_t qq = wave_find(base, nq, t);
+        if (threadIdx.x == 0) {
+          s.q = qq;
+          s.qlo = base[qq];
+          s.qhi = base[qq + 1];
+        }
+      }
+      __syncthreads();
+    }
+    q = s.q;
+    lt = t - s.qlo;
+    return true;
+  }
+};
+
 __global__ __launch_bounds__(kBlock) void k_x_begin(ExecArgs a) {
   const uint64_t nq = *a.up_nq;
   for (uint32_t q = threadIdx.x; q < a.Q; q += kBlock) {
@@ -147,23 +260,26 @@ __global__ __launch_bounds__(kBlock) void k_x_begin(ExecArgs a) {
     a.qs[q] = s;
   }
   if (threadIdx.x == 0) {
-    a.st->nq = nq;
-    a.st->table_top = 0;
-    a.st->scratch_low = a.cap;
-    a.st->nt = a.st->np = a.st->nx = a.st->nu = 0;
-    a.st->gather_base = a.st->gather_bytes = 0;
+    ExecState &st = *a.st;
+    st.nq = nq;
+    st.lo_top = 0;
+    st.hi_low = a.cap;
+    st.w0 = st.w1 = st.out_used = st.scr_used = 0;
+    st.nt = st.nc = st.np = st.nx = st.nu = 0;
+    st.res_bytes = st.res_over = st.peak = 0;
   }
 }
 
-// Step sizing shared by the two passes of k_level_plan.
+// ---- level plan ------------------------------------------------------------
+
 struct LevelNeed {
   bool active;
-  uint64_t R, nt, scratch, table;
+  uint64_t R, nt, nc, scratch, table;
 };
 
 template <int TR>
 __device__ __forceinline__ LevelNeed level_need(const ExecArgs &a, uint32_t q, uint32_t level) {
-  LevelNeed n{false, 0, 0, 0, 0};
+  LevelNeed n{false, 0, 0, 0, 0, 0};
   const QState &s = a.qs[q];
   if (!level_active(s, level)) return n;
   const wkg_plan_step &d = a.up_steps[(uint64_t)q * a.P + level];
@@ -171,8 +287,9 @@ __device__ __forceinline__ LevelNeed level_need(const ExecArgs &a, uint32_t q, u
   n.active = true;
   n.R = key ? 1 : s.rows;
   n.nt = (n.R + TR - 1) / TR;
-  // totals[4] | tile_sum | tile_nz | tile_ex | hits (TR per tile, 16 B) | K2U: cinfo, cexcl
-  n.scratch = al(32 + 24 * n.nt) + n.nt * TR * 16 + (d.kind == WKG_STEP_K2U || key ? 2 * al(n.R * 8) : 0);
+  n.nc = n.nt > kScanChunk ? (n.nt + kScanChunk - 1) / kScanChunk : 1;  // an empty step still writes totals
+  // totals[4] | tile_sum | tile_nz | tile_ex | chunk sums x 3 | hits (TR per tile, 16 B) | K2U: cinfo, cexcl
+  n.scratch = al(32 + 24 * n.nt + 24 * n.nc) + n.nt * TR * 16 + (d.kind == WKG_STEP_K2U || key ? 2 * al(n.R * 8) : 0);
   n.table = (d.kind == WKG_STEP_K2K || d.kind == WKG_STEP_K2C) ? al(n.R * s.cols * 4) : 0;
   return n;
 }
@@ -183,8 +300,8 @@ __global__ __launch_bounds__(kBlock) void k_level_plan(ExecArgs a, uint32_t leve
   __shared__ unsigned long long s_ut, s_us;
   ExecState *st = a.st;
   const uint32_t nq = (uint32_t)st->nq;
-  const uint64_t top = st->table_top;
-  const uint64_t room = a.cap - top;
+  const uint64_t w0 = (level & 1) ? st->lo_top : 0, w1 = (level & 1) ? a.cap : st->hi_low;
+  const uint64_t room = w1 - w0;
   uint32_t q0, q1;
   my_queries(nq, q0, q1);
   if (threadIdx.x == 0) {  // ordered before the atomics by the scans' barriers
@@ -200,29 +317,34 @@ __global__ __launch_bounds__(kBlock) void k_level_plan(ExecArgs a, uint32_t leve
   uint64_t tot;
   uint64_t os = block_excl_scan(ss, s_tmp, tot);
   uint64_t ot = block_excl_scan(ts, s_tmp, tot);
-  // pass 2: fit (in query order), tiles
-  uint64_t nts = 0;
+  // fitting queries (in query order) and their tiles and scan chunks
+  uint64_t nts = 0, ncs = 0;
   {
     uint64_t os2 = os, ot2 = ot;
     for (uint32_t q = q0; q < q1; q++) {
       const LevelNeed n = level_need<TR>(a, q, level);
-      const bool fits = ot2 + n.table + os2 + n.scratch <= room;
-      if (n.active && fits) nts += n.nt;
+      if (n.active && ot2 + n.table + os2 + n.scratch <= room) {
+        nts += n.nt;
+        ncs += n.nc;
+      }
       os2 += n.scratch;
       ot2 += n.table;
     }
   }
-  uint64_t ntile = block_excl_scan(nts, s_tmp, tot);
+  uint64_t tbase = block_excl_scan(nts, s_tmp, tot);
   const uint64_t NT = tot;
+  uint64_t cbase = block_excl_scan(ncs, s_tmp, tot);
+  const uint64_t NC = tot;
   uint64_t used_t = 0, used_s = 0;
   for (uint32_t q = q0; q < q1; q++) {
     const LevelNeed n = level_need<TR>(a, q, level);
-    a.tile_base[q] = ntile;
+    a.tile_base[q] = tbase;
+    a.chunk_base[q] = cbase;
     if (n.active) {
+      QState &s = a.qs[q];
       if (ot + n.table + os + n.scratch > room) {
-        a.qs[q].status = WKG_ENOMEM;
+        s.status = WKG_ENOMEM;
       } else {
-        const QState &s = a.qs[q];
         const wkg_plan_step &d = a.up_steps[(uint64_t)q * a.P + level];
         StepParams p{};
         p.pid = (uint32_t)d.pid;
@@ -242,22 +364,27 @@ __global__ __launch_bounds__(kBlock) void k_level_plan(ExecArgs a, uint32_t leve
           p.target = (uint32_t)d.vid;
           p.target_ok = (d.kind == WKG_STEP_K2C && d.vid >= 0 && d.vid <= 0xffffffffLL) ? 1u : 0u;
         }
-        char *sb = a.arena + a.cap - (os + n.scratch);  // [cap - os - scratch, cap - os)
+        char *sb = scr_at(a, level, w0, w1, os, n.scratch);
         p.totals = (uint64_t *)sb;
         p.tile_sum = p.totals + 4;
         p.tile_nz = p.tile_sum + n.nt;
         p.tile_ex = p.tile_nz + n.nt;
-        sb += al(32 + 24 * n.nt);
+        p.chunk_sum = p.tile_ex + n.nt;
+        p.chunk_nz = p.chunk_sum + n.nc;
+        p.chunk_ex = p.chunk_nz + n.nc;
+        sb += al(32 + 24 * n.nt + 24 * n.nc);
         p.hits = (uint4 *)sb;
         sb += n.nt * TR * 16;
         if (p.mode == kModeK2U) {
           p.cinfo = (uint2 *)sb;
           p.cexcl = (uint64_t *)(sb + al(n.R * 8));
         } else {
-          p.O = (int32_t *)(a.arena + top + ot);
+          p.O = (int32_t *)out_at(a, level, w0, w1, ot, n.table);
         }
         a.items[q].p = p;
-        ntile += n.nt;
+        s.nchunks = (uint32_t)n.nc;
+        tbase += n.nt;
+        cbase += n.nc;
         used_t = ot + n.table;
         used_s = os + n.scratch;
       }
@@ -272,88 +399,99 @@ __global__ __launch_bounds__(kBlock) void k_level_plan(ExecArgs a, uint32_t leve
   __syncthreads();
   if (threadIdx.x == kBlock - 1) {
     a.tile_base[nq] = NT;
+    a.chunk_base[nq] = NC;
+    st->w0 = w0;
+    st->w1 = w1;
+    st->out_used = s_ut;
+    st->scr_used = s_us;
     st->nt = NT;
-    st->table_top = top + s_ut;
-    st->scratch_low = a.cap - s_us;
-    st->ticket[0] = 0;
-    st->ticket[1] = 0;
+    st->nc = NC;
   }
+  if (threadIdx.x < kSubQueues) st->queue[0][threadIdx.x][0] = 0;
 }
 
-// Probe tiles of all queries from the shared ticket; the next ticket is taken
-// while the current tile is worked on.
+// ---- probe, tile scan, finalize ----------------------------------------------
+
 template <int TR, bool SORTED, bool FRAMED>
-__global__ __launch_bounds__(kBlock) void k_x_probe(ExecArgs a) {
-  __shared__ uint64_t s_t;
-  __shared__ uint32_t s_q;
-  const uint64_t NT = a.st->nt;
+__global__ __launch_bounds__(kBlock, 3) void k_x_probe(ExecArgs a) {
+  __shared__ QueueLds s_wq;
+  BlockQueue wq(s_wq, a.st->queue[0], a.st->nt);
   const uint32_t nq = (uint32_t)a.st->nq;
-  unsigned long long *ticket = (unsigned long long *)&a.st->ticket[0];
-  uint64_t nxt = 0;
-  if (threadIdx.x == 0) nxt = atomicAdd(ticket, 1ull);
-  while (true) {
+  uint64_t t, lt;
+  uint32_t q;
+  while (wq.next(a.tile_base, nq, t, q, lt)) probe_tile<TR, SORTED, FRAMED>(a.g, a.items[q].p, lt);
+}
+
+__global__ __launch_bounds__(kBlock) void k_x_scan_chunks(ExecArgs a) {
+  __shared__ uint32_t s_q;
+  const uint64_t NC = a.st->nc;
+  const uint32_t nq = (uint32_t)a.st->nq;
+  for (uint64_t u = blockIdx.x; u < NC; u += gridDim.x) {
     __syncthreads();
-    if (threadIdx.x == 0) {
-      s_t = nxt;
-      if (nxt < NT) nxt = atomicAdd(ticket, 1ull);
-    }
-    __syncthreads();
-    const uint64_t t = s_t;
-    if (t >= NT) break;
-    const uint32_t q = block_find(a.tile_base, nq, t, &s_q);
-    probe_tile<TR, SORTED, FRAMED>(a.g, a.items[q].p, t - a.tile_base[q]);
+    const uint32_t q = block_find(a.chunk_base, nq, u, &s_q);
+    scan_chunk(a.items[q].p, u - a.chunk_base[q], a.qs[q].nchunks);
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_x_scan(ExecArgs a, uint32_t level) {
+__global__ __launch_bounds__(kBlock) void k_x_scan_top(ExecArgs a, uint32_t level) {
   const uint32_t nq = (uint32_t)a.st->nq;
   for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
-    __syncthreads();
-    if (level_active(a.qs[q], level)) scan_step_tiles(a.items[q].p);
+    const QState &s = a.qs[q];
+    if (level_active(s, level) && s.nchunks > 1) {
+      __syncthreads();
+      scan_top(a.items[q].p, s.nchunks);
+    }
   }
 }
 
+// One wave per tile, waves interleaved over the tile space (the tiles are
+// short and alike, so a static deal balances).
 template <int TR>
 __global__ __launch_bounds__(kBlock) void k_x_finalize(ExecArgs a) {
-  __shared__ uint64_t s_t;
-  __shared__ uint32_t s_q;
-  const uint64_t NT = a.st->nt;
   const uint32_t nq = (uint32_t)a.st->nq;
-  unsigned long long *ticket = (unsigned long long *)&a.st->ticket[1];
-  uint64_t nxt = 0;
-  if (threadIdx.x == 0) nxt = atomicAdd(ticket, 1ull);
-  while (true) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      s_t = nxt;
-      if (nxt < NT) nxt = atomicAdd(ticket, 1ull);
+  const uint64_t NT = a.st->nt, W = (uint64_t)gridDim.x * (kBlock / kWave);
+  uint32_t q = 0;
+  uint64_t lo = 1, hi = 0;
+  for (uint64_t t = blockIdx.x * (uint64_t)(kBlock / kWave) + threadIdx.x / kWave; t < NT; t += W) {
+    if (t < lo || t >= hi) {  // wave-uniform
+      q = wave_find(a.tile_base, nq, t);
+      lo = a.tile_base[q];
+      hi = a.tile_base[q + 1];
     }
-    __syncthreads();
-    const uint64_t t = s_t;
-    if (t >= NT) break;
-    const uint32_t q = block_find(a.tile_base, nq, t, &s_q);
-    finalize_tile<TR>(a.items[q].p, t - a.tile_base[q]);
+    finalize_tile<TR>(a.items[q].p, t - lo);
   }
 }
 
+// ---- expand plan: outputs, merge-path splits, retirement of finished queries ----
+
 struct ExpandNeed {
-  bool active, k2u;
-  uint64_t table, scratch, nx;
+  bool active, k2u, fin;  // fin: the query's last step
+  bool bad;               // output beyond the row limit
+  uint64_t table, scratch, nx, rows, cols;
 };
 
 template <int NV>
 __device__ __forceinline__ ExpandNeed expand_need(const ExecArgs &a, uint32_t q, uint32_t level) {
-  ExpandNeed n{false, false, 0, 0, 0};
-  if (!level_active(a.qs[q], level)) return n;
+  ExpandNeed n{false, false, false, false, 0, 0, 0, 0, 0};
+  const QState &s = a.qs[q];
+  if (!level_active(s, level)) return n;
   const StepParams &p = a.items[q].p;
   n.active = true;
+  n.fin = level + 1 == s.nsteps;
   n.k2u = p.mode == kModeK2U;
+  n.rows = p.totals[0];
   if (n.k2u) {
-    const uint64_t E = p.totals[0], nz = p.totals[2];
-    if (E >= kRowLimit) return n;  // refused below
-    n.table = al(E * (p.c + 1) * 4);
-    n.nx = E ? (nz + E + NV - 1) / NV : 0;
+    n.cols = p.c + 1;
+    if (n.rows >= kRowLimit) {
+      n.bad = true;
+      return n;
+    }
+    const uint64_t nz = p.totals[2];
+    n.table = al(n.rows * n.cols * 4);
+    n.nx = n.rows ? (nz + n.rows + NV - 1) / NV : 0;
     n.scratch = n.nx ? al((n.nx + 1) * 8) : 0;
+  } else {
+    n.cols = p.c;
   }
   return n;
 }
@@ -361,14 +499,17 @@ __device__ __forceinline__ ExpandNeed expand_need(const ExecArgs &a, uint32_t q,
 template <int NV>
 __global__ __launch_bounds__(kBlock) void k_expand_plan(ExecArgs a, uint32_t level) {
   __shared__ uint64_t s_tmp[kBlock / kWave];
-  __shared__ unsigned long long s_used;
+  __shared__ unsigned long long s_ut, s_us, s_rb, s_over;
   ExecState *st = a.st;
   const uint32_t nq = (uint32_t)st->nq;
-  const uint64_t top = st->table_top, low = st->scratch_low;
-  const uint64_t room = low - top;
+  const uint64_t w0 = st->w0, w1 = st->w1, ou = st->out_used, su = st->scr_used;
+  const uint64_t room = (w1 - w0) - ou - su;
+  const uint64_t rb0 = st->res_bytes;
   uint32_t q0, q1;
   my_queries(nq, q0, q1);
-  if (threadIdx.x == 0) s_used = 0;
+  if (threadIdx.x == 0) {
+    s_ut = s_us = s_rb = s_over = 0;
+  }
   uint64_t ts = 0, ss = 0;
   for (uint32_t q = q0; q < q1; q++) {
     const ExpandNeed n = expand_need<NV>(a, q, level);
@@ -378,14 +519,18 @@ __global__ __launch_bounds__(kBlock) void k_expand_plan(ExecArgs a, uint32_t lev
   uint64_t tot;
   uint64_t ot = block_excl_scan(ts, s_tmp, tot);
   uint64_t os = block_excl_scan(ss, s_tmp, tot);
-  uint64_t nps = 0, nxs = 0;
+  // pass 2: arena fit; partition entries, expand tiles, result bytes of finishing queries
+  uint64_t nps = 0, nxs = 0, rbs = 0;
   {
     uint64_t ot2 = ot, os2 = os;
     for (uint32_t q = q0; q < q1; q++) {
       const ExpandNeed n = expand_need<NV>(a, q, level);
-      if (n.active && ot2 + n.table + os2 + n.scratch <= room && n.nx) {
-        nps += n.nx + 1;
-        nxs += n.nx;
+      if (n.active && !n.bad && ot2 + n.table + os2 + n.scratch <= room) {
+        if (n.nx) {
+          nps += n.nx + 1;
+          nxs += n.nx;
+        }
+        if (n.fin) rbs += al(n.rows * n.cols * 4);
       }
       ot2 += n.table;
       os2 += n.scratch;
@@ -395,58 +540,102 @@ __global__ __launch_bounds__(kBlock) void k_expand_plan(ExecArgs a, uint32_t lev
   const uint64_t NP = tot;
   uint64_t xbase = block_excl_scan(nxs, s_tmp, tot);
   const uint64_t NX = tot;
-  uint64_t used_t = 0;
+  uint64_t orb = block_excl_scan(rbs, s_tmp, tot);
+  // pass 3: host-buffer fit (again a prefix) and copy units
+  uint64_t nus = 0;
+  {
+    uint64_t ot2 = ot, os2 = os, orb2 = orb;
+    for (uint32_t q = q0; q < q1; q++) {
+      const ExpandNeed n = expand_need<NV>(a, q, level);
+      if (n.active && !n.bad && ot2 + n.table + os2 + n.scratch <= room && n.fin) {
+        const uint64_t b = al(n.rows * n.cols * 4);
+        if (rb0 + orb2 + b <= a.hcap) nus += (b + kCopyUnit - 1) / kCopyUnit;
+        orb2 += b;
+      }
+      ot2 += n.table;
+      os2 += n.scratch;
+    }
+  }
+  uint64_t ubase = block_excl_scan(nus, s_tmp, tot);
+  const uint64_t NU = tot;
+  uint64_t used_t = 0, used_s = 0, used_r = 0, over = 0;
   for (uint32_t q = q0; q < q1; q++) {
     const ExpandNeed n = expand_need<NV>(a, q, level);
     a.part_base[q] = pbase;
     a.xtile_base[q] = xbase;
+    a.unit_base[q] = ubase;
     if (n.active) {
       QState &s = a.qs[q];
       const StepParams &p = a.items[q].p;
-      if (n.k2u && p.totals[0] >= kRowLimit) {
+      if (n.bad) {
         s.status = WKG_ERANGE;
       } else if (ot + n.table + os + n.scratch > room) {
         s.status = WKG_ENOMEM;
-      } else if (n.k2u) {
-        ExpandParams x{};
-        x.pid = p.pid;
-        x.T = p.T;
-        x.c = p.c;
-        x.R = p.totals[2];
-        x.cinfo = p.cinfo;
-        x.cexcl = p.cexcl;
-        x.part = (const uint64_t *)(a.arena + low - (os + n.scratch));
-        x.E = p.totals[0];
-        x.O = (int32_t *)(a.arena + top + ot);
-        a.items[q].x = x;
-        s.T = x.O;
-        s.rows = x.E;
-        s.cols = p.c + 1;
-        s.expanded += x.E;
-        if (n.nx) {
-          pbase += n.nx + 1;
-          xbase += n.nx;
+      } else {
+        if (n.k2u) {
+          ExpandParams x{};
+          x.pid = p.pid;
+          x.T = p.T;
+          x.c = p.c;
+          x.R = p.totals[2];
+          x.cinfo = p.cinfo;
+          x.cexcl = p.cexcl;
+          x.part = (const uint64_t *)scr_at(a, level, w0, w1, su + os, n.scratch);
+          x.E = n.rows;
+          x.O = (int32_t *)out_at(a, level, w0, w1, ou + ot, n.table);
+          a.items[q].x = x;
+          s.T = x.O;
+          s.expanded += x.E;
+          if (n.nx) {
+            pbase += n.nx + 1;
+            xbase += n.nx;
+          }
+          used_t = ot + n.table;
+          used_s = os + n.scratch;
+        } else {  // filters: the kept rows were compacted into p.O
+          s.T = p.O;
+          s.expanded += p.totals[1];
         }
-        used_t = ot + n.table;
-      } else {  // filters: the kept rows were compacted into p.O
-        s.T = p.O;
-        s.rows = p.totals[0];
-        s.expanded += p.totals[1];
+        s.rows = n.rows;
+        s.cols = (uint32_t)n.cols;
+        if (n.fin) {
+          const uint64_t b = al(n.rows * n.cols * 4);
+          if (rb0 + orb + b <= a.hcap) {
+            s.out_off = (rb0 + orb) / 4;
+            ubase += (b + kCopyUnit - 1) / kCopyUnit;
+            used_r = orb + b;
+          } else {
+            s.status = WKG_ENOMEM;
+            over += b;
+          }
+          orb += b;
+        }
       }
     }
     ot += n.table;
     os += n.scratch;
   }
-  if (used_t) atomicMax(&s_used, (unsigned long long)used_t);
+  if (used_t) atomicMax(&s_ut, (unsigned long long)used_t);
+  if (used_s) atomicMax(&s_us, (unsigned long long)used_s);
+  if (used_r) atomicMax(&s_rb, (unsigned long long)used_r);
+  if (over) atomicAdd(&s_over, (unsigned long long)over);
   __syncthreads();
   if (threadIdx.x == kBlock - 1) {
     a.part_base[nq] = NP;
     a.xtile_base[nq] = NX;
+    a.unit_base[nq] = NU;
     st->np = NP;
     st->nx = NX;
-    st->table_top = top + s_used;
-    st->ticket[2] = 0;
+    st->nu = NU;
+    const uint64_t out = ou + s_ut, scr = su + s_us;
+    if (level & 1) st->hi_low = w1 - out;
+    else st->lo_top = w0 + out;
+    const uint64_t in_use = a.cap - (w1 - w0) + out + scr;
+    if (in_use > st->peak) st->peak = in_use;
+    st->res_bytes = rb0 + s_rb;
+    st->res_over += s_over;
   }
+  if (threadIdx.x < kSubQueues) st->queue[1][threadIdx.x][0] = 0;
 }
 
 template <int NV>
@@ -463,105 +652,49 @@ __global__ __launch_bounds__(kBlock) void k_x_partition(ExecArgs a) {
 
 template <int VT, bool FRAMED>
 __global__ __launch_bounds__(kBlock) void k_x_expand(ExecArgs a) {
-  __shared__ uint64_t s_t;
-  __shared__ uint32_t s_q;
-  const uint64_t NX = a.st->nx;
+  __shared__ QueueLds s_wq;
+  BlockQueue wq(s_wq, a.st->queue[1], a.st->nx);
   const uint32_t nq = (uint32_t)a.st->nq;
-  unsigned long long *ticket = (unsigned long long *)&a.st->ticket[2];
-  uint64_t nxt = 0;
-  if (threadIdx.x == 0) nxt = atomicAdd(ticket, 1ull);
-  while (true) {
+  uint64_t t, lt;
+  uint32_t q;
+  while (wq.next(a.xtile_base, nq, t, q, lt)) expand_tile<VT, FRAMED>(a.g, a.items[q].x, lt);
+}
+
+// Copies the tables of queries that finished at this level into the host
+// result buffer (device-visible pinned memory). Tables and result slots are
+// 256-byte aligned and padded, so 16-byte chunks stay inside both.
+__global__ __launch_bounds__(kBlock) void k_x_retire(ExecArgs a) {
+  __shared__ uint32_t s_q;
+  const uint32_t nq = (uint32_t)a.st->nq;
+  const uint64_t NU = a.st->nu;
+  for (uint64_t u = blockIdx.x; u < NU; u += gridDim.x) {
     __syncthreads();
-    if (threadIdx.x == 0) {
-      s_t = nxt;
-      if (nxt < NX) nxt = atomicAdd(ticket, 1ull);
-    }
-    __syncthreads();
-    const uint64_t t = s_t;
-    if (t >= NX) break;
-    const uint32_t q = block_find(a.xtile_base, nq, t, &s_q);
-    expand_tile<VT, FRAMED>(a.g, a.items[q].x, t - a.xtile_base[q]);
+    const uint32_t q = block_find(a.unit_base, nq, u, &s_q);
+    const QState &s = a.qs[q];
+    const uint64_t bytes = s.rows * s.cols * 4;
+    const uint64_t b0 = (u - a.unit_base[q]) * kCopyUnit;
+    if (b0 >= bytes) continue;  // (units are sized from the same bytes)
+    const uint64_t b1 = b0 + kCopyUnit < bytes ? b0 + kCopyUnit : bytes;
+    const uint4 *src = (const uint4 *)((const char *)s.T + b0);
+    uint4 *dst = (uint4 *)(a.hres + s.out_off * 4 + b0);
+    const uint64_t n16 = (b1 - b0 + 15) / 16;
+    for (uint64_t i = threadIdx.x; i < n16; i += kBlock) dst[i] = src[i];
   }
 }
 
-// Result records and the packed result region (above the tables; the last
-// level's scratch is free by now). Gather work units go to tile_base.
-__global__ __launch_bounds__(kBlock) void k_result_plan(ExecArgs a) {
-  __shared__ uint64_t s_tmp[kBlock / kWave];
-  __shared__ unsigned long long s_used;
-  ExecState *st = a.st;
-  const uint32_t nq = (uint32_t)st->nq;
-  const uint64_t top = st->table_top, room = a.cap - top;
-  uint32_t q0, q1;
-  my_queries(nq, q0, q1);
-  if (threadIdx.x == 0) s_used = 0;
-  auto bytes = [&](uint32_t q) -> uint64_t {
+__global__ __launch_bounds__(kBlock) void k_x_records(ExecArgs a) {
+  const uint32_t nq = (uint32_t)a.st->nq;
+  for (uint32_t q = threadIdx.x; q < nq; q += kBlock) {
     const QState &s = a.qs[q];
-    return s.status == WKG_OK ? al(s.rows * s.cols * 4) : 0;
-  };
-  uint64_t bs = 0;
-  for (uint32_t q = q0; q < q1; q++) bs += bytes(q);
-  uint64_t tot;
-  uint64_t ob = block_excl_scan(bs, s_tmp, tot);
-  uint64_t us = 0;
-  {
-    uint64_t ob2 = ob;
-    for (uint32_t q = q0; q < q1; q++) {
-      const uint64_t b = bytes(q);
-      if (ob2 + b <= room) us += (b + kGatherUnit - 1) / kGatherUnit;
-      ob2 += b;
-    }
-  }
-  uint64_t ubase = block_excl_scan(us, s_tmp, tot);
-  const uint64_t NU = tot;
-  uint64_t used = 0;
-  for (uint32_t q = q0; q < q1; q++) {
-    const uint64_t b = bytes(q);
-    QState &s = a.qs[q];
-    a.tile_base[q] = ubase;
     wkg_batch_result r{};
-    if (s.status == WKG_OK && ob + b > room) s.status = WKG_ENOMEM;
     r.status = s.status;
     r.expanded = s.expanded;
     r.cols = (int32_t)s.cols;
     if (s.status == WKG_OK) {
-      r.offset = ob / 4;
+      r.offset = s.out_off;
       r.rows = s.rows;
-      ubase += (b + kGatherUnit - 1) / kGatherUnit;
-      used = ob + b;
     }
     a.res[q] = r;
-    ob += b;
-  }
-  if (used) atomicMax(&s_used, (unsigned long long)used);
-  __syncthreads();
-  if (threadIdx.x == kBlock - 1) {
-    a.tile_base[nq] = NU;
-    st->nu = NU;
-    st->gather_base = top;
-    st->gather_bytes = s_used;
-  }
-}
-
-// Copies each result table (256-byte aligned in the arena, so 16-byte chunks
-// stay inside its allocation) to its place in the result region.
-__global__ __launch_bounds__(kBlock) void k_x_gather(ExecArgs a) {
-  __shared__ uint32_t s_q;
-  const uint64_t NU = a.st->nu;
-  const uint32_t nq = (uint32_t)a.st->nq;
-  char *dst0 = a.arena + a.st->gather_base;
-  for (uint64_t u = blockIdx.x; u < NU; u += gridDim.x) {
-    __syncthreads();
-    const uint32_t q = block_find(a.tile_base, nq, u, &s_q);
-    const QState &s = a.qs[q];
-    const uint64_t bytes = s.rows * s.cols * 4;
-    const uint64_t b0 = (u - a.tile_base[q]) * kGatherUnit;
-    if (b0 >= bytes) continue;  // (units are sized from the same bytes; kept for safety)
-    const uint64_t b1 = b0 + kGatherUnit < bytes ? b0 + kGatherUnit : bytes;
-    const uint4 *src = (const uint4 *)((const char *)s.T + b0);
-    uint4 *dst = (uint4 *)(dst0 + a.res[q].offset * 4 + b0);
-    const uint64_t n16 = (b1 - b0 + 15) / 16;
-    for (uint64_t i = threadIdx.x; i < n16; i += kBlock) dst[i] = src[i];
   }
 }
 
@@ -583,8 +716,10 @@ struct wkg_batch {
   char *d_up = nullptr;          // device copy of the upload block
   char *h_up = nullptr;          // pinned: nq | nsteps[Q] | steps[Q*P]
   char *h_down = nullptr;        // pinned: ExecState | results[Q]
-  char *h_cells = nullptr;       // pinned staging of the packed tables
-  size_t h_cells_cap = 0;
+  char *h_res = nullptr;         // pinned, device-visible: result tables
+  uint64_t h_res_cap = 0;
+  uint64_t grow_to = 0;          // result buffer size wanted at the next launch
+  uint64_t last_peak = 0, last_res_bytes = 0;
   ExecArgs args{};
   wkg_batch_tuning tune{};
   std::map<int, hipGraphExec_t> graphs;  // by level count
@@ -596,11 +731,14 @@ struct wkg_batch {
     for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
     graphs.clear();
   }
+  int set_result_buffer(uint64_t bytes);
   int enqueue(int levels);
   int graph_for(int levels, hipGraphExec_t *out);
 };
 
 namespace {
+
+constexpr uint64_t kDefaultResultBuffer = 64ull << 20;
 
 wkg_batch_tuning normalized(const wkg_batch_tuning &t) {
   wkg_batch_tuning r = t;
@@ -614,50 +752,77 @@ wkg_batch_tuning normalized(const wkg_batch_tuning &t) {
 }
 
 template <int TR, int VT, bool SORTED, bool FRAMED>
-void enqueue_levels(const ExecArgs &a, hipStream_t s, int levels, unsigned gp, unsigned gx, unsigned ga) {
+void enqueue_levels(const ExecArgs &a, hipStream_t s, int levels, unsigned gp, unsigned gx, unsigned ga,
+                    unsigned gf) {
   constexpr int NV = kBlock * VT;
   for (int l = 0; l < levels; l++) {
     k_level_plan<TR><<<1, kBlock, 0, s>>>(a, (uint32_t)l);
     k_x_probe<TR, SORTED, FRAMED><<<gp, kBlock, 0, s>>>(a);
-    k_x_scan<<<ga, kBlock, 0, s>>>(a, (uint32_t)l);
-    k_x_finalize<TR><<<gp, kBlock, 0, s>>>(a);
+    k_x_scan_chunks<<<ga, kBlock, 0, s>>>(a);
+    k_x_scan_top<<<ga, kBlock, 0, s>>>(a, (uint32_t)l);
+    k_x_finalize<TR><<<gf, kBlock, 0, s>>>(a);
     k_expand_plan<NV><<<1, kBlock, 0, s>>>(a, (uint32_t)l);
     k_x_partition<NV><<<ga, kBlock, 0, s>>>(a);
     k_x_expand<VT, FRAMED><<<gx, kBlock, 0, s>>>(a);
+    k_x_retire<<<ga, kBlock, 0, s>>>(a);
   }
 }
 
 template <int TR, int VT>
 void enqueue_shape(const ExecArgs &a, hipStream_t s, int levels, bool sorted, bool framed, unsigned gp, unsigned gx,
-                   unsigned ga) {
+                   unsigned ga, unsigned gf) {
   if (framed) {
-    if (sorted) enqueue_levels<TR, VT, true, true>(a, s, levels, gp, gx, ga);
-    else enqueue_levels<TR, VT, false, true>(a, s, levels, gp, gx, ga);
+    if (sorted) enqueue_levels<TR, VT, true, true>(a, s, levels, gp, gx, ga, gf);
+    else enqueue_levels<TR, VT, false, true>(a, s, levels, gp, gx, ga, gf);
   } else {
-    if (sorted) enqueue_levels<TR, VT, true, false>(a, s, levels, gp, gx, ga);
-    else enqueue_levels<TR, VT, false, false>(a, s, levels, gp, gx, ga);
+    if (sorted) enqueue_levels<TR, VT, true, false>(a, s, levels, gp, gx, ga, gf);
+    else enqueue_levels<TR, VT, false, false>(a, s, levels, gp, gx, ga, gf);
   }
 }
 
 }  // namespace
 
+// (Re)allocates the host result buffer; the captured sequences hold its
+// address, so they are dropped.
+int wkg_batch::set_result_buffer(uint64_t bytes) {
+  bytes = al(std::max<uint64_t>(bytes, 4096));
+  if (h_res) (void)hipHostFree(h_res);
+  h_res = nullptr;
+  h_res_cap = 0;
+  args.hres = nullptr;
+  args.hcap = 0;
+  clear_graphs();
+  hipError_t e = hipHostMalloc((void **)&h_res, bytes, hipHostMallocMapped);
+  void *d = nullptr;
+  if (e == hipSuccess) e = hipHostGetDevicePointer(&d, h_res, 0);
+  if (e != hipSuccess) {
+    if (h_res) (void)hipHostFree(h_res);
+    h_res = nullptr;
+    return ctx->fail(WKG_ENOMEM, "pinned result buffer of %llu bytes: %s", (unsigned long long)bytes,
+                     hipGetErrorString(e));
+  }
+  h_res_cap = bytes;
+  args.hres = (char *)d;
+  args.hcap = bytes;
+  return WKG_OK;
+}
+
 int wkg_batch::enqueue(int levels) {
   const wkg_batch_tuning t = normalized(tune);
   const unsigned gp = (unsigned)(ncu * t.probe_wg_per_cu), gx = (unsigned)(ncu * t.expand_wg_per_cu),
-                 ga = (unsigned)(ncu * t.aux_wg_per_cu);
+                 ga = (unsigned)(ncu * t.aux_wg_per_cu), gf = (unsigned)(ncu * 8);
   hipError_t e = hipMemcpyAsync(d_up, h_up, up_bytes, hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return ctx->fail(WKG_EDEVICE, "plan upload failed: %s", hipGetErrorString(e));
   k_x_begin<<<1, kBlock, 0, s>>>(args);
   const bool sorted = ctx->sorted, framed = ctx->framed;
   if (t.probe_tile_rows == 1024) {
-    if (t.expand_tile_items == 1024) enqueue_shape<1024, 4>(args, s, levels, sorted, framed, gp, gx, ga);
-    else enqueue_shape<1024, 8>(args, s, levels, sorted, framed, gp, gx, ga);
+    if (t.expand_tile_items == 1024) enqueue_shape<1024, 4>(args, s, levels, sorted, framed, gp, gx, ga, gf);
+    else enqueue_shape<1024, 8>(args, s, levels, sorted, framed, gp, gx, ga, gf);
   } else {
-    if (t.expand_tile_items == 1024) enqueue_shape<2048, 4>(args, s, levels, sorted, framed, gp, gx, ga);
-    else enqueue_shape<2048, 8>(args, s, levels, sorted, framed, gp, gx, ga);
+    if (t.expand_tile_items == 1024) enqueue_shape<2048, 4>(args, s, levels, sorted, framed, gp, gx, ga, gf);
+    else enqueue_shape<2048, 8>(args, s, levels, sorted, framed, gp, gx, ga, gf);
   }
-  k_result_plan<<<1, kBlock, 0, s>>>(args);
-  k_x_gather<<<ga, kBlock, 0, s>>>(args);
+  k_x_records<<<1, kBlock, 0, s>>>(args);
   e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(h_down, dctl, down_bytes, hipMemcpyDeviceToHost, s);
   if (e != hipSuccess) return ctx->fail(WKG_EDEVICE, "batch launch failed: %s", hipGetErrorString(e));
@@ -714,14 +879,13 @@ int wkg_batch_create(wkg_ctx *ctx, int max_queries, int max_patterns, uint64_t a
   if (e == hipSuccess) e = hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
   b->ncu = dev_cu > 0 ? dev_cu : 256;
   const uint64_t Q = (uint64_t)max_queries, P = (uint64_t)max_patterns;
-  // control block: ExecState | results[Q] | qs[Q] | items[Q] | 3 x base[Q+1] | upload
+  // control block: ExecState | results[Q] | qs[Q] | items[Q] | 5 x base[Q+1] | upload
   const uint64_t o_res = al(sizeof(ExecState));
   const uint64_t o_qs = al(o_res + Q * sizeof(wkg_batch_result));
   const uint64_t o_items = al(o_qs + Q * sizeof(QState));
-  const uint64_t o_tb = al(o_items + Q * sizeof(Item));
-  const uint64_t o_pb = al(o_tb + (Q + 1) * 8);
-  const uint64_t o_xb = al(o_pb + (Q + 1) * 8);
-  const uint64_t o_up = al(o_xb + (Q + 1) * 8);
+  const uint64_t o_base = al(o_items + Q * sizeof(Item));
+  const uint64_t base_bytes = al((Q + 1) * 8);
+  const uint64_t o_up = o_base + 5 * base_bytes;
   b->up_bytes = al(8 + Q * 4) + Q * P * sizeof(wkg_plan_step);
   b->ctl_bytes = o_up + b->up_bytes;
   b->down_bytes = o_res + Q * sizeof(wkg_batch_result);
@@ -748,9 +912,11 @@ int wkg_batch_create(wkg_ctx *ctx, int max_queries, int max_patterns, uint64_t a
   a.res = (wkg_batch_result *)(b->dctl + o_res);
   a.qs = (QState *)(b->dctl + o_qs);
   a.items = (Item *)(b->dctl + o_items);
-  a.tile_base = (uint64_t *)(b->dctl + o_tb);
-  a.part_base = (uint64_t *)(b->dctl + o_pb);
-  a.xtile_base = (uint64_t *)(b->dctl + o_xb);
+  a.tile_base = (uint64_t *)(b->dctl + o_base);
+  a.chunk_base = (uint64_t *)(b->dctl + o_base + base_bytes);
+  a.part_base = (uint64_t *)(b->dctl + o_base + 2 * base_bytes);
+  a.xtile_base = (uint64_t *)(b->dctl + o_base + 3 * base_bytes);
+  a.unit_base = (uint64_t *)(b->dctl + o_base + 4 * base_bytes);
   a.up_nq = (const uint64_t *)b->d_up;
   a.up_nsteps = (const uint32_t *)(b->d_up + 8);
   a.up_steps = (const wkg_plan_step *)(b->d_up + al(8 + Q * 4));
@@ -758,6 +924,7 @@ int wkg_batch_create(wkg_ctx *ctx, int max_queries, int max_patterns, uint64_t a
   a.cap = b->cap;
   a.Q = (uint32_t)Q;
   a.P = (uint32_t)P;
+  if (int rc = b->set_result_buffer(kDefaultResultBuffer)) return bail(rc);
   *out = b;
   return WKG_OK;
 }
@@ -770,7 +937,7 @@ int wkg_batch_destroy(wkg_batch *b) {
   if (b->dctl) b->ctx->pool->release(b->dctl, b->s);
   if (b->h_up) (void)hipHostFree(b->h_up);
   if (b->h_down) (void)hipHostFree(b->h_down);
-  if (b->h_cells) (void)hipHostFree(b->h_cells);
+  if (b->h_res) (void)hipHostFree(b->h_res);
   if (b->own_stream) {
     // pooled blocks tagged with this stream go back to HIP before it is destroyed
     b->ctx->pool->trim();
@@ -794,6 +961,20 @@ int wkg_batch_set_tuning(wkg_batch *b, const wkg_batch_tuning *t) {
 int wkg_batch_get_tuning(const wkg_batch *b, wkg_batch_tuning *t) {
   if (!b || !t) return WKG_EINVAL;
   *t = normalized(b->tune);
+  return WKG_OK;
+}
+
+int wkg_batch_reserve_results(wkg_batch *b, uint64_t bytes) {
+  if (!b) return WKG_EINVAL;
+  if (b->launched) return b->ctx->fail(WKG_ESTATE, "a batch is in flight");
+  b->grow_to = 0;
+  return b->set_result_buffer(bytes);
+}
+
+int wkg_batch_stats(const wkg_batch *b, uint64_t *arena_peak, uint64_t *result_bytes) {
+  if (!b) return WKG_EINVAL;
+  if (arena_peak) *arena_peak = b->last_peak;
+  if (result_bytes) *result_bytes = b->last_res_bytes;
   return WKG_OK;
 }
 
@@ -862,6 +1043,9 @@ int wkg_batch_launch(wkg_batch *b, int nq, const wkg_plan_step *steps, const int
   for (int q = 0; q < nq; q++)
     memcpy(&h_steps[(uint64_t)q * b->P], &steps[step_off[q]], sizeof(wkg_plan_step) * (step_off[q + 1] - step_off[q]));
   if (levels == 0) levels = 1;
+  if (b->grow_to > b->h_res_cap)
+    if (int rc = b->set_result_buffer(b->grow_to)) return rc;
+  b->grow_to = 0;
   hipGraphExec_t x;
   if (int rc = b->graph_for(levels, &x)) return rc;
   ProfScope ps(ctx, b->s, "batch_graph", 0.0);
@@ -877,7 +1061,7 @@ int wkg_batch_busy(wkg_batch *b) {
   return hipStreamQuery(b->s) == hipErrorNotReady ? 1 : 0;
 }
 
-int wkg_batch_collect(wkg_batch *b, wkg_batch_result *res, int32_t **cells, uint64_t *ncells) {
+int wkg_batch_collect(wkg_batch *b, wkg_batch_result *res, const int32_t **cells, uint64_t *ncells) {
   if (!b || (!res && b->launched_nq)) return WKG_EINVAL;
   wkg_ctx *ctx = b->ctx;
   if (!b->launched) return ctx->fail(WKG_ESTATE, "no batch in flight");
@@ -889,31 +1073,18 @@ int wkg_batch_collect(wkg_batch *b, wkg_batch_result *res, int32_t **cells, uint
   const ExecState *st = (const ExecState *)b->h_down;
   const wkg_batch_result *r = (const wkg_batch_result *)(b->h_down + al(sizeof(ExecState)));
   if (b->launched_nq) memcpy(res, r, sizeof(wkg_batch_result) * b->launched_nq);
-  const uint64_t bytes = st->gather_bytes;
-  if (ncells) *ncells = bytes / 4;
-  if (!cells || bytes == 0) return WKG_OK;
-  if (bytes > b->h_cells_cap) {
-    if (b->h_cells) (void)hipHostFree(b->h_cells);
-    b->h_cells = nullptr;
-    b->h_cells_cap = 0;
-    const size_t want = std::max<size_t>(bytes + bytes / 4, 1 << 22);
-    if ((e = hipHostMalloc((void **)&b->h_cells, want, hipHostMallocDefault)) != hipSuccess) {
-      b->h_cells = nullptr;
-      return ctx->fail(WKG_ENOMEM, "pinned result staging of %zu bytes: %s", want, hipGetErrorString(e));
-    }
-    b->h_cells_cap = want;
-  }
-  e = hipMemcpyAsync(b->h_cells, b->arena + st->gather_base, bytes, hipMemcpyDeviceToHost, b->s);
-  if (e == hipSuccess) e = hipStreamSynchronize(b->s);
-  if (e != hipSuccess) return ctx->fail(WKG_EDEVICE, "result read-back failed: %s", hipGetErrorString(e));
-  *cells = (int32_t *)malloc(bytes);
-  if (!*cells) return ctx->fail(WKG_ENOMEM, "host result buffer of %llu bytes", (unsigned long long)bytes);
-  memcpy(*cells, b->h_cells, bytes);
+  b->last_peak = st->peak;
+  b->last_res_bytes = st->res_bytes;
+  if (cells) *cells = (const int32_t *)b->h_res;
+  if (ncells) *ncells = st->res_bytes / 4;
+  // some results did not fit: grow the buffer at the next launch (this
+  // batch's results stay readable in the current one until then)
+  if (st->res_over) b->grow_to = std::max(b->grow_to, (st->res_bytes + st->res_over) * 3 / 2);
   return WKG_OK;
 }
 
 int wkg_batch_run(wkg_batch *b, int nq, const wkg_plan_step *steps, const int32_t *step_off, wkg_batch_result *res,
-                  int32_t **cells, uint64_t *ncells) {
+                  const int32_t **cells, uint64_t *ncells) {
   if (int rc = wkg_batch_launch(b, nq, steps, step_off)) return rc;
   return wkg_batch_collect(b, res, cells, ncells);
 }
@@ -935,9 +1106,7 @@ int wkg_batch_autotune(wkg_batch *b, int nq, const wkg_plan_step *steps, const i
     std::vector<double> ts;
     for (int i = 0; i <= reps; i++) {  // the first run captures the graph
       const auto t0 = std::chrono::steady_clock::now();
-      int32_t *cells = nullptr;
-      int rc = wkg_batch_run(b, nq, steps, step_off, res.data(), &cells, nullptr);
-      free(cells);
+      int rc = wkg_batch_run(b, nq, steps, step_off, res.data(), nullptr, nullptr);
       if (rc) return rc;
       if (i) ts.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
     }

@@ -120,8 +120,9 @@ __device__ __forceinline__ void group_member(const GraphView &g, uint64_t estart
 }
 
 // Probe pass over one tile (TR rows; `lt` is the tile's index within its
-// step). Each round, the 4 waves probe 64 consecutive rows each. Writes the
-// tile's list of rows with output and its sums.
+// step). Wave w owns the tile's rows [w * TR/4, (w+1) * TR/4) and probes
+// them 64 consecutive rows per round, carrying the last key across rounds.
+// Writes the tile's list of rows with output and its sums.
 template <int TR, bool SORTED, bool FRAMED>
 __device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams &p, uint64_t lt) {
   constexpr int kRounds = TR / kBlock;  // one row per lane per round
@@ -137,16 +138,18 @@ __device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams 
   const DevMeta m = g.meta[p.pid < g.num_metas ? p.pid : 0];
 
   // all rounds' table reads are issued up front
+  constexpr uint32_t kWaveRows = TR / (kBlock / kWave);
   int32_t cells[kRounds], ends[kRounds];
 #pragma unroll
   for (int rd = 0; rd < kRounds; rd++) {
-    const uint64_t row = base + rd * kBlock + wave * kWave + lane;
+    const uint64_t row = base + wave * kWaveRows + rd * kWave + lane;
     const bool in = row < p.R && !p.key;
     cells[rd] = in ? p.T[row * p.c + p.sc] : -1;
     ends[rd] = in && p.mode == kModeK2K ? p.T[row * p.c + p.ec] : 0;
   }
+  ProbeCarry carry;
   for (int rd = 0; rd < kRounds; rd++) {
-    const uint32_t local = rd * kBlock + wave * kWave + lane;
+    const uint32_t local = wave * kWaveRows + rd * kWave + lane;
     const uint64_t row = base + local;
     uint64_t key = 0;
     const int32_t end_cell = ends[rd];
@@ -155,7 +158,7 @@ __device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams 
       else if (cells[rd] >= 0) key = make_key((uint32_t)cells[rd], p.pid, p.dir);
     }
     uint32_t deg, off;
-    wave_probe<FRAMED>(g, m, key, s_wp[wave], deg, off);
+    wave_probe<FRAMED>(g, m, key, s_wp[wave], deg, off, carry);
     if (p.mode == kModeK2U) {
       s_cnt[local] = deg;
       s_aux[local] = off;
@@ -214,8 +217,9 @@ __device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams 
   }
 }
 
+// At most 168 VGPRs, so three waves per SIMD hide the probe's load chains.
 template <bool SORTED, bool FRAMED>
-__global__ __launch_bounds__(kBlock) void k_probe(GraphView g, StepParams p) {
+__global__ __launch_bounds__(kBlock, 3) void k_probe(GraphView g, StepParams p) {
   probe_tile<kTileRows, SORTED, FRAMED>(g, p, blockIdx.x);
 }
 
@@ -265,13 +269,13 @@ __device__ __forceinline__ void scan_step_tiles(const StepParams &p) {
 
 __global__ __launch_bounds__(kBlock) void k_scan_tiles(StepParams p) { scan_step_tiles(p); }
 
-// Two-level tile scan, level 1: block b scans tiles [b*kScanChunk, ...) in
-// place (chunk-local prefixes) and stores the chunk's sums. With a single
-// chunk it writes the step's totals and zero chunk prefixes directly.
-__global__ __launch_bounds__(kBlock) void k_scan_chunks(StepParams p) {
+// Two-level tile scan, level 1: chunk k (kScanChunk tiles) is scanned in
+// place (chunk-local prefixes) and its sums stored. With a single chunk the
+// step's totals and zero chunk prefixes are written directly.
+__device__ __forceinline__ void scan_chunk(const StepParams &p, uint64_t chunk, uint64_t nchunks) {
   __shared__ uint64_t s_tmp[kBlock / kWave];
   constexpr int PT = kScanChunk / kBlock;
-  const uint64_t lo = blockIdx.x * (uint64_t)kScanChunk + threadIdx.x * PT;
+  const uint64_t lo = chunk * (uint64_t)kScanChunk + threadIdx.x * PT;
   uint64_t vs[PT], vn[PT], s = 0, nzs = 0, e = 0;
 #pragma unroll
   for (int k = 0; k < PT; k++) {
@@ -295,22 +299,24 @@ __global__ __launch_bounds__(kBlock) void k_scan_chunks(StepParams p) {
       rn += vn[k];
     }
   if (threadIdx.x == 0) {
-    if (gridDim.x == 1) {
+    if (nchunks == 1) {
       p.totals[0] = ts;
       p.totals[1] = te;
       p.totals[2] = tn;
       p.chunk_sum[0] = 0;
       p.chunk_nz[0] = 0;
     } else {
-      p.chunk_sum[blockIdx.x] = ts;
-      p.chunk_nz[blockIdx.x] = tn;
-      p.chunk_ex[blockIdx.x] = te;
+      p.chunk_sum[chunk] = ts;
+      p.chunk_nz[chunk] = tn;
+      p.chunk_ex[chunk] = te;
     }
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_scan_chunks(StepParams p) { scan_chunk(p, blockIdx.x, gridDim.x); }
+
 // Level 2 (one block): chunk sums become chunk prefixes; writes the totals.
-__global__ __launch_bounds__(kBlock) void k_scan_top(StepParams p, uint64_t nchunks) {
+__device__ __forceinline__ void scan_top(const StepParams &p, uint64_t nchunks) {
   __shared__ uint64_t s_tmp[kBlock / kWave];
   const uint64_t rows = block_scan_array(p.chunk_sum, nchunks, s_tmp);
   const uint64_t nz = block_scan_array(p.chunk_nz, nchunks, s_tmp);
@@ -325,51 +331,65 @@ __global__ __launch_bounds__(kBlock) void k_scan_top(StepParams p, uint64_t nchu
   }
 }
 
-// Finalize pass over one tile of TR rows: K2U moves the tile's rows with
-// edges to the compacted row list with their exclusive output offsets; the
-// filters copy the kept rows, in order, to the compacted output.
+__global__ __launch_bounds__(kBlock) void k_scan_top(StepParams p, uint64_t nchunks) { scan_top(p, nchunks); }
+
+// Finalize pass over one tile of TR rows, by one wave: K2U moves the tile's
+// rows with edges to the compacted row list with their exclusive output
+// offsets; the filters copy the kept rows, in order, to the compacted output.
+// Waves work on their tiles independently (no block barrier), so tiles with
+// few rows cost one short chain of loads each.
 template <int TR>
 __device__ __forceinline__ void finalize_tile(const StepParams &p, uint64_t lt) {
-  __shared__ uint64_t s_tmp[kBlock / kWave];
-  __shared__ uint32_t s_sel[kBlock];
-  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = lane_id();
   auto nz_prefix = [&](uint64_t t) -> uint64_t {
     if (t >= p.ntiles) return p.totals[2];
     return p.tile_nz[t] + (p.chunk_nz ? p.chunk_nz[t / kScanChunk] : 0);
   };
   const uint64_t nz0 = nz_prefix(lt);
   const uint32_t nnz = (uint32_t)(nz_prefix(lt + 1) - nz0);
+  if (nnz == 0) return;
   const uint64_t out0 = p.tile_sum[lt] + (p.chunk_sum ? p.chunk_sum[lt / kScanChunk] : 0);
   const uint4 *hits = p.hits + lt * TR;
-  uint64_t carry = 0;
-  for (uint32_t k0 = 0; k0 < nnz; k0 += kBlock) {
-    const uint32_t j = k0 + tid;
-    const uint4 h = j < nnz ? hits[j] : make_uint4(0, 0, 0, 0);
-    if (p.mode == kModeK2U) {
-      uint64_t tot;
-      const uint64_t ex = block_excl_scan(h.z, s_tmp, tot);
+  if (p.mode == kModeK2U) {
+    uint64_t carry = 0;
+    for (uint32_t k0 = 0; k0 < nnz; k0 += kWave) {
+      const uint32_t j = k0 + lane;
+      const uint4 h = j < nnz ? hits[j] : make_uint4(0, 0, 0, 0);
+      const uint64_t inc = wave_incl_scan(h.z);
       if (j < nnz) {
         p.cinfo[nz0 + j] = make_uint2(h.x, h.y);
-        p.cexcl[nz0 + j] = out0 + carry + ex;
+        p.cexcl[nz0 + j] = out0 + carry + inc - h.z;
       }
-      carry += tot;
-    } else {
-      if (j < nnz) s_sel[tid] = h.x;
-      __syncthreads();
-      // kept rows k0.. of this tile go to output rows out0 + k0 ..
-      const uint32_t c = p.c, nsel = nnz - k0 < (uint32_t)kBlock ? nnz - k0 : (uint32_t)kBlock;
-      const uint32_t rpp = kBlock / c;
-      if (tid < rpp * c) {
-        const uint32_t col = tid % c;
-        for (uint32_t r = tid / c; r < nsel; r += rpp)
-          p.O[(out0 + k0 + r) * c + col] = p.T[(uint64_t)s_sel[r] * c + col];
+      carry += shfl64(inc, kWave - 1);
+    }
+  } else {
+    // kept rows k0.. of this tile go to output rows out0 + k0 ..
+    const uint32_t c = p.c;
+    for (uint32_t k0 = 0; k0 < nnz; k0 += kWave) {
+      const uint32_t nsel = nnz - k0 < (uint32_t)kWave ? nnz - k0 : (uint32_t)kWave;
+      const uint32_t myrow = lane < nsel ? hits[k0 + lane].x : 0u;
+      if (c <= (uint32_t)kWave) {  // 64 / c rows per pass, one column per lane
+        const uint32_t rpp = kWave / c, col = lane % c, r0 = lane / c;
+        for (uint32_t it = 0; it * rpp < nsel; it++) {
+          const uint32_t r = it * rpp + r0;
+          const uint32_t src = shfl32(myrow, r < (uint32_t)kWave ? r : 0u);
+          if (r0 < rpp && r < nsel) p.O[(out0 + k0 + r) * c + col] = p.T[(uint64_t)src * c + col];
+        }
+      } else {
+        for (uint32_t r = 0; r < nsel; r++) {
+          const uint32_t src = shfl32(myrow, r);
+          for (uint32_t col = lane; col < c; col += kWave) p.O[(out0 + k0 + r) * c + col] = p.T[(uint64_t)src * c + col];
+        }
       }
-      __syncthreads();
     }
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_finalize(StepParams p) { finalize_tile<kTileRows>(p, blockIdx.x); }
+// One wave per tile: tile blockIdx.x * 4 + wave.
+__global__ __launch_bounds__(kBlock) void k_finalize(StepParams p) {
+  const uint64_t t = blockIdx.x * (uint64_t)(kBlock / kWave) + threadIdx.x / kWave;
+  if (t < p.ntiles) finalize_tile<kTileRows>(p, t);
+}
 
 // Merge-path partition of the (row starts, output rows) sequence over the
 // compacted rows: part[k] = number of rows whose start item lies before
@@ -527,7 +547,8 @@ __global__ __launch_bounds__(kWave) void k_probe_one(GraphView g, uint64_t key, 
   __shared__ WaveProbeLds s_wp;
   const DevMeta m = g.meta[pid];
   uint32_t deg, off;
-  wave_probe<FRAMED>(g, m, threadIdx.x == 0 ? key : 0, s_wp, deg, off);
+  ProbeCarry carry;
+  wave_probe<FRAMED>(g, m, threadIdx.x == 0 ? key : 0, s_wp, deg, off, carry);
   if (threadIdx.x == 0) {
     out[0] = deg;
     out[1] = off;

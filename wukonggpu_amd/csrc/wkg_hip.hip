@@ -315,7 +315,7 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
   }
   {
     ProfScope ps(this, s, mode == kModeK2U ? "finalize" : "compact", 0.0);
-    k_finalize<<<(unsigned)ntiles, kBlock, 0, s>>>(p);
+    k_finalize<<<(unsigned)((ntiles + kBlock / kWave - 1) / (kBlock / kWave)), kBlock, 0, s>>>(p);
   }
   hipError_t le = hipGetLastError();
   uint64_t cnt[3] = {0, 0, 0};

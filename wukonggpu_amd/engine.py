@@ -239,13 +239,15 @@ class GPUEngine:
         if rc != 0:
             raise DeviceError(rc, "wke_configure failed")
 
-    def autotune(self, queries, reps=3):
-        """Autotunes the executor's launch and tile shapes on a sample batch; returns (shape, ms)."""
+    def autotune(self, queries, reps=3, heavy=False):
+        """Autotunes the executor's launch and tile shapes on a sample batch (for
+        the heavy serving lane with heavy=True); returns (shape, ms)."""
         L = N.engine_lib()
         flat, off = pack_queries(queries)
         t = N.BatchTuning()
         ms = C.c_double()
-        rc = L.wke_autotune(self._h, _i64p(flat), _i64p(off), len(queries), reps, C.byref(t), C.byref(ms))
+        rc = L.wke_autotune(self._h, _i64p(flat), _i64p(off), len(queries), reps, 1 if heavy else 0,
+                            C.byref(t), C.byref(ms))
         if rc != 0:
             raise DeviceError(rc, L.wke_last_error(self._h).decode())
         return t.as_dict(), float(ms.value)
@@ -298,6 +300,15 @@ class Batch:
         t = N.BatchTuning(**kw)
         self.dev._check(N.hip_lib().wkg_batch_set_tuning(self._h, C.byref(t)))
 
+    def reserve_results(self, nbytes):
+        self.dev._check(N.hip_lib().wkg_batch_reserve_results(self._h, nbytes))
+
+    def stats(self):
+        """(arena high-water mark, result bytes) of the last batch."""
+        p, r = C.c_uint64(), C.c_uint64()
+        self.dev._check(N.hip_lib().wkg_batch_stats(self._h, C.byref(p), C.byref(r)))
+        return int(p.value), int(r.value)
+
     def tuning(self):
         t = N.BatchTuning()
         self.dev._check(N.hip_lib().wkg_batch_get_tuning(self._h, C.byref(t)))
@@ -312,12 +323,9 @@ class Batch:
         cells = C.POINTER(C.c_int32)()
         ncells = C.c_uint64()
         self.dev._check(N.hip_lib().wkg_batch_run(self._h, n, arr, off, res, C.byref(cells), C.byref(ncells)))
-        try:
-            flat = (np.ctypeslib.as_array(cells, shape=(ncells.value,)).copy() if ncells.value
-                    else np.zeros(0, np.int32))
-        finally:
-            if cells:
-                N.engine_lib().wke_free(C.cast(cells, C.c_void_p))
+        # the cells live in the batch's pinned result buffer until the next launch
+        flat = (np.ctypeslib.as_array(cells, shape=(ncells.value,)).copy() if ncells.value
+                else np.zeros(0, np.int32))
         out = []
         for i in range(n):
             r = res[i]
