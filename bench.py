@@ -82,7 +82,9 @@ def main():
     log(f"graph resident in HBM in {time.time() - t1:.1f}s")
     eng = GPUEngine(dev)
     copy_gbs = dev.copy_bandwidth(1 << 30, 20)
-    log(f"copy kernel bandwidth: {copy_gbs:.0f} GB/s")
+    lines_ps = dev.random_access_rate(4 << 30, 10)
+    log(f"copy kernel bandwidth: {copy_gbs:.0f} GB/s; random 128-byte line reads: {lines_ps / 1e9:.2f} G/s "
+        f"({lines_ps * 128 / 1e9:.0f} GB/s)")
 
     heavy = w.queries["heavy"]
     for _ in range(args.warmup):
@@ -97,6 +99,8 @@ def main():
     elapsed = time.perf_counter() - t_start
     kernels = {k: dev.profile_read(k) for k in ("probe", "probe_filter", "scan_tiles", "finalize", "compact",
                                                 "partition", "expand", "probe_one", "copy_list")}
+    probe_keys = dev.profile_read("probe_keys")[2]  # counts kept in the bytes field
+    probe_rows = dev.profile_read("probe_rows")[2]
     dev.profile(False)
     edges = r.expanded
     value = edges * args.steps / elapsed
@@ -114,6 +118,19 @@ def main():
             t = json.load(f)
         traffic, traffic_src = t["hbm_bytes_per_launch"], t["source"]
     gpu_kernel_ms = sum(v[1] for v in kernels.values()) / args.steps
+    # the probe against the random-access roofline: keys probed (each at least
+    # one random 128-byte bucket line) per second of known_to_unknown probe time
+    ms_pr = kernels["probe"][1]
+    probe_rate = probe_keys / (ms_pr * 1e-3) if ms_pr > 0 else 0.0
+    probe_roofline = {
+        "bound": "random-access", "achieved": probe_rate, "peak": lines_ps, "unit": "128-byte lines/s",
+        "frac": probe_rate / lines_ps if lines_ps else None, "kernel": "probe (k_probe, known_to_unknown steps)",
+        "keys_probed_per_query": probe_keys / args.steps, "rows_per_query": probe_rows / args.steps,
+        "rows_per_s": probe_rows / (ms_pr * 1e-3) if ms_pr > 0 else 0.0,
+        "note": "runs of equal keys are probed once, so rows/s exceeds the line roofline",
+    }
+    log(f"probe: {probe_rows / args.steps / 1e6:.1f} M rows and {probe_keys / args.steps / 1e6:.2f} M keys probed "
+        f"per query; {probe_rate / 1e9:.2f} G keys/s = {probe_rate / lines_ps:.2f} of the random-line roofline")
 
     if args.heavy_only:
         print(json.dumps({"metric": METRIC, "value": value, "unit": "expanded edges/s", "n_gpus": 1,
@@ -247,6 +264,7 @@ def main():
             "alg_bytes_per_launch": b_exp / max(n_exp, 1), "avg_launch_ms": ms_exp / max(n_exp, 1),
             "traffic_source": traffic_src, "copy_kernel_gbs": copy_gbs,
         },
+        "probe_roofline": probe_roofline,
         "cpu_baseline": cpu,
         "batched": {
             "workload": f"{len(mix)} selective 3-4 pattern queries from type indexes (sizes log-uniform 1..1e5)",

@@ -255,6 +255,10 @@ int wkg_profile_read(wkg_ctx *ctx, const char *kernel, uint64_t *launches, doubl
 /* Device-to-device copy bandwidth of a plain 16-byte-per-lane copy kernel
  * over `bytes` (read + written bytes per second). */
 int wkg_bandwidth_probe(wkg_ctx *ctx, uint64_t bytes, int iters, double *copy_gbps);
+/* Random 128-byte line reads over a bytes-sized buffer (8 lanes per line, 8
+ * lines in flight per lane group, as the hash probe reads buckets): the
+ * random-access roofline of the probe, in lines per second. */
+int wkg_random_access_probe(wkg_ctx *ctx, uint64_t bytes, int iters, double *lines_per_s);
 
 #ifdef __cplusplus
 }

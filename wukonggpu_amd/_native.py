@@ -134,6 +134,7 @@ def hip_lib():
         _sig(L, "wkg_profile_enable", C.c_int, vp, C.c_int)
         _sig(L, "wkg_profile_read", C.c_int, vp, C.c_char_p, P(u64), P(dbl), P(dbl))
         _sig(L, "wkg_bandwidth_probe", C.c_int, vp, u64, C.c_int, P(dbl))
+        _sig(L, "wkg_random_access_probe", C.c_int, vp, u64, C.c_int, P(dbl))
         _sig(L, "wkg_batch_create", C.c_int, vp, C.c_int, C.c_int, u64, vp, P(vp))
         _sig(L, "wkg_batch_destroy", C.c_int, vp)
         _sig(L, "wkg_batch_set_tuning", C.c_int, vp, P(BatchTuning))

@@ -100,17 +100,18 @@ struct WaveProbeLds {
 //     with L leaders does ceil(L/8) key slots instead of 8.
 //  3. Results go back through LDS; every lane reads its run leader's.
 // Every lane of the wave must call this with the wave's own `w`. key == 0
-// means "no key". Returns the lane's degree (0 on a miss) and list offset
-// relative to the predicate's edge_start, and moves the carry to lane 63.
-// (The carry starts as key 0: valid keys are never 0.)
+// means "no key". Gives the lane's degree (0 on a miss) and list offset
+// relative to the predicate's edge_start, moves the carry to lane 63 (it
+// starts as key 0: valid keys are never 0) and returns the number of keys
+// the wave probed (its run leaders).
 struct ProbeCarry {
   uint64_t key = 0;
   uint32_t deg = 0, off = 0;
 };
 
 template <bool FRAMED>
-__device__ __forceinline__ void wave_probe(const GraphView &g, const DevMeta &m, uint64_t key, WaveProbeLds &w,
-                                           uint32_t &deg_out, uint32_t &off_out, ProbeCarry &carry) {
+__device__ __forceinline__ uint32_t wave_probe(const GraphView &g, const DevMeta &m, uint64_t key, WaveProbeLds &w,
+                                               uint32_t &deg_out, uint32_t &off_out, ProbeCarry &carry) {
   const uint32_t lane = lane_id(), l8 = lane & 7, grp = lane >> 3, gbase = lane & ~7u;
   const bool valid = key != 0 && m.psz != 0;
   const uint64_t before = shfl64(key, (lane - 1) & 63);
@@ -207,6 +208,7 @@ __device__ __forceinline__ void wave_probe(const GraphView &g, const DevMeta &m,
   carry.key = shfl64(key, kWave - 1);
   carry.deg = shfl32(deg_out, kWave - 1);
   carry.off = shfl32(off_out, kWave - 1);
+  return L;
 }
 
 // Inclusive wave scan of u64 values.

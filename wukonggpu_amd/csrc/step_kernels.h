@@ -47,8 +47,9 @@ struct StepParams {
   int32_t *O;          // filters: compacted output, capacity R x c
   uint64_t *tile_sum;  // per tile: output rows; after the tile scan, the tile's prefix
   uint64_t *tile_nz;   // per tile: rows with output; after the tile scan, the prefix
-  uint64_t *tile_ex;   // per tile: edges examined (filters)
-  uint64_t *totals;    // [0] output rows, [1] edges examined, [2] rows with output
+  uint64_t *tile_ex;   // per tile: edges examined (filters) / keys probed (K2U)
+  uint64_t *totals;    // [0] output rows, [1] edges examined (filters) or keys probed (K2U),
+                       // [2] rows with output
   uint64_t ntiles;     // tiles of this step
   // two-level tile scan (null when one block scans all tiles): per chunk of
   // kScanChunk tiles, the chunk's prefix of output rows / rows with output
@@ -131,6 +132,7 @@ __device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams 
   __shared__ uint32_t s_cnt[TR];  // degree, or kept flag
   __shared__ uint32_t s_aux[TR];  // segment offset, or edges examined
   __shared__ uint64_t s_tmp[kBlock / kWave];
+  __shared__ uint32_t s_probed[kBlock / kWave];
   __shared__ WaveProbeLds s_wp[kBlock / kWave];
   const uint32_t tid = threadIdx.x, lane = lane_id(), gbase = lane & ~7u, l8 = lane & 7;
   const uint32_t wave = tid / kWave;
@@ -148,6 +150,7 @@ __device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams 
     ends[rd] = in && p.mode == kModeK2K ? p.T[row * p.c + p.ec] : 0;
   }
   ProbeCarry carry;
+  uint32_t probed = 0;
   for (int rd = 0; rd < kRounds; rd++) {
     const uint32_t local = wave * kWaveRows + rd * kWave + lane;
     const uint64_t row = base + local;
@@ -158,7 +161,7 @@ __device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams 
       else if (cells[rd] >= 0) key = make_key((uint32_t)cells[rd], p.pid, p.dir);
     }
     uint32_t deg, off;
-    wave_probe<FRAMED>(g, m, key, s_wp[wave], deg, off, carry);
+    probed += wave_probe<FRAMED>(g, m, key, s_wp[wave], deg, off, carry);
     if (p.mode == kModeK2U) {
       s_cnt[local] = deg;
       s_aux[local] = off;
@@ -188,6 +191,7 @@ __device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams 
       s_aux[local] = my_ex;
     }
   }
+  if (lane == 0) s_probed[wave] = probed;
   __syncthreads();
   // thread t owns the tile's rows 8t .. 8t+7
   uint64_t sum = 0, ex = 0;
@@ -213,7 +217,10 @@ __device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams 
   if (tid == 0) {
     p.tile_sum[lt] = sum_total;
     p.tile_nz[lt] = nz_total;
-    if (p.mode != kModeK2U) p.tile_ex[lt] = ex_total;
+    uint32_t keys = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / kWave; w++) keys += s_probed[w];
+    p.tile_ex[lt] = p.mode == kModeK2U ? (uint64_t)keys : ex_total;
   }
 }
 
@@ -254,12 +261,9 @@ __device__ __forceinline__ void scan_step_tiles(const StepParams &p) {
   __shared__ uint64_t s_tmp[kBlock / kWave];
   const uint64_t rows = block_scan_array(p.tile_sum, p.ntiles, s_tmp);
   const uint64_t nz = block_scan_array(p.tile_nz, p.ntiles, s_tmp);
-  uint64_t ex = 0;
-  if (p.mode != kModeK2U) {
-    uint64_t e = 0;
-    for (uint64_t i = threadIdx.x; i < p.ntiles; i += kBlock) e += p.tile_ex[i];
-    (void)block_excl_scan(e, s_tmp, ex);
-  }
+  uint64_t ex = 0, e = 0;
+  for (uint64_t i = threadIdx.x; i < p.ntiles; i += kBlock) e += p.tile_ex[i];
+  (void)block_excl_scan(e, s_tmp, ex);
   if (threadIdx.x == 0) {
     p.totals[0] = rows;
     p.totals[1] = ex;
@@ -282,14 +286,14 @@ __device__ __forceinline__ void scan_chunk(const StepParams &p, uint64_t chunk, 
     const bool in = lo + k < p.ntiles;
     vs[k] = in ? p.tile_sum[lo + k] : 0;
     vn[k] = in ? p.tile_nz[lo + k] : 0;
-    if (p.mode != kModeK2U && in) e += p.tile_ex[lo + k];
+    if (in) e += p.tile_ex[lo + k];
     s += vs[k];
     nzs += vn[k];
   }
-  uint64_t ts, tn, te = 0;
+  uint64_t ts, tn, te;
   uint64_t rs = block_excl_scan(s, s_tmp, ts);
   uint64_t rn = block_excl_scan(nzs, s_tmp, tn);
-  if (p.mode != kModeK2U) (void)block_excl_scan(e, s_tmp, te);
+  (void)block_excl_scan(e, s_tmp, te);
 #pragma unroll
   for (int k = 0; k < PT; k++)
     if (lo + k < p.ntiles) {
@@ -321,8 +325,7 @@ __device__ __forceinline__ void scan_top(const StepParams &p, uint64_t nchunks) 
   const uint64_t rows = block_scan_array(p.chunk_sum, nchunks, s_tmp);
   const uint64_t nz = block_scan_array(p.chunk_nz, nchunks, s_tmp);
   uint64_t e = 0, ex = 0;
-  if (p.mode != kModeK2U)
-    for (uint64_t i = threadIdx.x; i < nchunks; i += kBlock) e += p.chunk_ex[i];
+  for (uint64_t i = threadIdx.x; i < nchunks; i += kBlock) e += p.chunk_ex[i];
   (void)block_excl_scan(e, s_tmp, ex);
   if (threadIdx.x == 0) {
     p.totals[0] = rows;

@@ -303,8 +303,10 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
   p.chunk_ex = chunks + 2 * nchunks;
   const GraphView g = view();
   {
-    // start cell (+ end cell for K2K) and one 128-byte bucket per row
-    const double b = (double)R * (4.0 + (mode == kModeK2K ? 4.0 : 0.0) + 128.0);
+    // start cell (+ end cell for K2K); K2U adds one 128-byte bucket per key
+    // probed and a 16-byte hit per row with edges once the counts are known;
+    // the filters are charged one bucket per row
+    const double b = (double)R * (4.0 + (mode == kModeK2K ? 4.0 : 0.0) + (mode == kModeK2U ? 0.0 : 128.0));
     ProfScope ps(this, s, mode == kModeK2U ? "probe" : "probe_filter", b);
     launch_probe(framed, sorted || mode == kModeK2U, g, p, s);
   }
@@ -336,6 +338,9 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
     return WKG_OK;
   }
   prof_add_bytes("finalize", 32.0 * nz);
+  prof_add_bytes("probe", 128.0 * (double)cnt[1] + 16.0 * (double)nz);
+  prof_add_bytes("probe_keys", (double)cnt[1]);  // counts, not bytes: keys probed (run leaders)
+  prof_add_bytes("probe_rows", (double)R);        // and rows looked up
   const uint64_t E = cnt[0];
   if (expanded) *expanded = E;
   if ((rc = new_table(E, out_cols, s, &t))) {
@@ -431,6 +436,27 @@ __global__ __launch_bounds__(256) void k_copy_calibrate(const uint4 *__restrict_
                                                          uint64_t n) {
   for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256ull) dst[i] = src[i];
 }
+// Random 128-byte line reads, 8 lanes per line (the way the probe reads a
+// bucket) and 8 independent lines in flight per 8-lane group: the
+// random-access roofline for the hash probe. buf holds line_mask + 1 lines;
+// out holds one word per thread (keeps the loads alive).
+__global__ __launch_bounds__(256) void k_random_lines(const uint4 *__restrict__ buf, uint64_t line_mask,
+                                                      uint32_t rounds, uint64_t seed, uint32_t *out) {
+  const uint32_t l8 = threadIdx.x & 7;
+  const uint64_t grp = (blockIdx.x * 256ull + threadIdx.x) >> 3;
+  uint32_t acc = 0;
+  for (uint32_t r = 0; r < rounds; r++) {
+    uint4 v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const uint64_t line = mix64(seed + (grp * rounds + r) * 8 + k) & line_mask;
+      v[k] = buf[line * 8 + l8];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc ^= v[k].x + v[k].w;
+  }
+  out[blockIdx.x * 256ull + threadIdx.x] = acc;
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -494,6 +520,46 @@ int wkg_bandwidth_probe(wkg_ctx *ctx, uint64_t bytes, int iters, double *copy_gb
   (void)hipFree(b);
   if (e != hipSuccess) return ctx->fail(WKG_EDEVICE, "bandwidth probe failed: %s", hipGetErrorString(e));
   *copy_gbps = 2.0 * n * 16 * iters / (ms * 1e-3) / 1e9;
+  return WKG_OK;
+}
+
+int wkg_random_access_probe(wkg_ctx *ctx, uint64_t bytes, int iters, double *lines_per_s) {
+  if (!ctx || !lines_per_s || bytes < (1u << 20) || iters < 1) return WKG_EINVAL;
+  uint64_t nlines = 1;
+  while (nlines * 2 * 128 <= bytes) nlines *= 2;
+  int dev_cus = 256;
+  (void)hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+  const unsigned grid = (unsigned)dev_cus * 8;
+  const uint32_t rounds = 16;
+  void *buf = nullptr, *out = nullptr;
+  if (hipMalloc(&buf, nlines * 128) != hipSuccess || hipMalloc(&out, (uint64_t)grid * 256 * 4) != hipSuccess) {
+    (void)hipGetLastError();
+    if (buf) (void)hipFree(buf);
+    return ctx->fail(WKG_ENOMEM, "cannot allocate %llu bytes for the random-access probe",
+                     (unsigned long long)(nlines * 128));
+  }
+  hipStream_t s = ctx->stream;
+  (void)hipMemsetAsync(buf, 1, nlines * 128, s);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  k_random_lines<<<grid, 256, 0, s>>>((const uint4 *)buf, nlines - 1, rounds, 1, (uint32_t *)out);  // warm up
+  (void)hipEventRecord(e0, s);
+  for (int i = 0; i < iters; i++) {
+    ProfScope ps(ctx, s, "random_lines", (double)grid * 32 * rounds * 8 * 128);
+    k_random_lines<<<grid, 256, 0, s>>>((const uint4 *)buf, nlines - 1, rounds, 1000003ull * (i + 2),
+                                       (uint32_t *)out);
+  }
+  (void)hipEventRecord(e1, s);
+  hipError_t e = hipEventSynchronize(e1);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(buf);
+  (void)hipFree(out);
+  if (e != hipSuccess) return ctx->fail(WKG_EDEVICE, "random-access probe failed: %s", hipGetErrorString(e));
+  *lines_per_s = (double)grid * 32 * rounds * 8 * iters / (ms * 1e-3);
   return WKG_OK;
 }
 

@@ -100,6 +100,12 @@ class Device:
         self._check(N.hip_lib().wkg_bandwidth_probe(self._h, nbytes, iters, C.byref(g)))
         return float(g.value)
 
+    def random_access_rate(self, nbytes=4 << 30, iters=10):
+        """Random 128-byte line reads per second (the hash probe's roofline)."""
+        g = C.c_double()
+        self._check(N.hip_lib().wkg_random_access_probe(self._h, nbytes, iters, C.byref(g)))
+        return float(g.value)
+
     def stats(self):
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
         self._check(N.hip_lib().wkg_graph_stats(self._h, C.byref(a), C.byref(b), C.byref(c)))
