@@ -58,6 +58,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--served-n", type=int, default=2000, help="queries per served rate (0: skip)")
     ap.add_argument("--served-rates", default="1000,2000,4000", help="offered Poisson rates, queries/s")
+    ap.add_argument("--config", default="2", choices=["1", "2", "4", "5"],
+                    help="2: heavy query + batched mix + served sweep on the 100M-triple graph (default); "
+                         "1: plain single query on the 10M-triple graph; 4: served sweep on the 1B-triple "
+                         "graph; 5: the cyclic query on the 2B-triple graph in HBM-budgeted chunks")
     ap.add_argument("--heavy-only", action="store_true",
                     help="run only the heavy query and the copy calibration (for profiler passes)")
     args = ap.parse_args()
@@ -71,6 +75,10 @@ def main():
     import numpy as np
 
     from wukonggpu_amd import Device, GPUEngine, workloads
+
+    if args.config != "2":
+        {"1": run_config1, "4": run_config4, "5": run_config5}[args.config](args)
+        return
 
     t0 = time.time()
     w = workloads.config2_3(seed=args.seed)
@@ -284,6 +292,212 @@ def main():
         "parity": parity,
     }
     print(json.dumps(line), flush=True)
+    eng.close()
+    dev.close()
+
+
+def _same(xs, ys):
+    import numpy as np
+    return all(a.status == 0 and b.status == 0 and a.table.shape == b.table.shape and
+               np.array_equal(a.table, b.table) and a.expanded == b.expanded for a, b in zip(xs, ys))
+
+
+def _cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
+
+
+def _line(metric, value, unit, steps, warmup, ms, config, extra, higher=True, dtype="u32"):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": ms, "higher_is_better": higher, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+           "data": "synthetic (seeded generator; graph resident in HBM)", "config": config}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def run_config1(args):
+    """Plain single query on the 10M-triple graph: the 2-hop chain from the
+    constant of p1-degree ~1000, on the GPU and on the CPU oracle."""
+    import numpy as np
+    import oracle
+    from wukonggpu_amd import Device, GPUEngine, workloads
+    w = workloads.config1(seed=1)
+    q = w.queries[0]
+    dev = Device(0)
+    dev.load(w.gstore)
+    eng = GPUEngine(dev)
+    for _ in range(max(args.warmup, 1)):
+        r = eng.run_query(q)
+    lat = []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        r = eng.run_query(q)
+        lat.append(time.perf_counter() - t)
+    el = sum(lat)
+    want, wex, sec = oracle.run_query(w.gstore, q)
+    reps = max(1, int(2.0 / max(sec, 1e-4)))
+    t = time.perf_counter()
+    for _ in range(reps):
+        oracle.run_query(w.gstore, q)
+    cpu_s = (time.perf_counter() - t) / reps
+    same = bool(want.shape == r.table.shape and np.array_equal(want, r.table) and wex == r.expanded)
+    log(f"config 1: {r.table.shape[0]} rows, {r.expanded} expanded edges, GPU {el / args.steps * 1e3:.3f} ms/query, "
+        f"CPU {cpu_s * 1e3:.3f} ms/query, identical={same}")
+    _line("expanded edges/s + queries/s; p50/p99 latency, batched vs plain on 1 MI355X",
+          r.expanded * args.steps / el, "expanded edges/s", args.steps, args.warmup, el / args.steps * 1e3,
+          {"workload": "plain 2-hop chain from one constant (p1 degree ~1k) on a 10M-triple graph: 1M entities, "
+                       "16 predicates, Zipf 1.1 capped at 10k", **{k: v for k, v in w.info.items()},
+           "query": q, "result_rows": int(r.table.shape[0]), "p50_ms": pct(lat, 0.5) * 1e3,
+           "p99_ms": pct(lat, 0.99) * 1e3, "queries_per_s": args.steps / el},
+          {"cpu_baseline": {"value": wex / cpu_s, "unit": "expanded edges/s", "cores": 1, "kind": "port",
+                            "sample": f"the full query, {reps} runs on one thread", **_cpu_info()},
+           "parity": {"rows_equal_cpu": same}})
+    eng.close()
+    dev.close()
+
+
+def run_config4(args):
+    """Served: open-loop Poisson arrivals over the 1B-triple graph, 95% light
+    (2-3 patterns from small type indexes) and 5% heavy (the 6-pattern heavy
+    shape from a 500k-member type, >= 10M-row tables), 64 GB table budget."""
+    import numpy as np
+    import oracle
+    from wukonggpu_amd import Device, GPUEngine, workloads
+    t0 = time.time()
+    w = workloads.config4(seed=4)
+    log(f"graph: {w.info['triples']} triples, {w.info['keys']} keys, image {w.info['image_bytes'] / 1e9:.2f} GB "
+        f"built in {time.time() - t0:.1f}s")
+    dev = Device(0)
+    dev.load(w.gstore)
+    eng = GPUEngine(dev)
+    P = w.info["predicates"]
+    sq, flags = workloads.served_mix(P, w.info["light_types"], w.info["t0"], args.served_n, args.seed + 41)
+    heavy_idx = [i for i in range(len(sq)) if flags[i]]
+    eng.configure(max_queries=1024, max_patterns=8, arena_bytes=16 << 30)
+    light_idx = [i for i in range(len(sq)) if not flags[i]][:512]
+    light = [sq[i] for i in light_idx]
+    tuning, tune_ms = eng.autotune(light, reps=2)
+    htuning, _ = eng.autotune([sq[i] for i in heavy_idx[:2]], reps=1, heavy=True)
+    log(f"autotuned: light {tuning}, heavy {htuning}")
+    tp = time.perf_counter()
+    alone = eng.run_plain(sq)
+    t_plain = time.perf_counter() - tp
+    hrows = [int(alone[i].expanded) for i in heavy_idx]
+    log(f"{len(sq)} queries alone on the plain path: {len(sq) / t_plain:.0f} q/s; heavy queries expand "
+        f"{min(hrows) if hrows else 0}..{max(hrows) if hrows else 0} edges")
+    tb = time.perf_counter()
+    light_b = eng.run_batched(light)
+    t_light = time.perf_counter() - tb
+    served, all_same = [], True
+    rng = np.random.default_rng(args.seed + 42)
+    for rate in [float(x) for x in args.served_rates.split(",") if x]:
+        arr = np.cumsum(rng.exponential(1.0 / rate, size=len(sq)))
+        res, wall = eng.serve(sq, arr, heavy=flags, table_budget_bytes=64 << 30, light_lanes=2)
+        ok = _same(alone, res)
+        all_same = all_same and ok
+        lat_l = [x.latency_s for x, f in zip(res, flags) if not f]
+        lat_h = [x.latency_s for x, f in zip(res, flags) if f]
+        rec = {"offered_qps": rate, "achieved_qps": len(sq) / wall,
+               "p50_ms": pct([x.latency_s for x in res], 0.5) * 1e3,
+               "p99_ms": pct([x.latency_s for x in res], 0.99) * 1e3,
+               "light_p50_ms": pct(lat_l, 0.5) * 1e3, "light_p99_ms": pct(lat_l, 0.99) * 1e3,
+               "heavy_p50_ms": pct(lat_h, 0.5) * 1e3, "heavy_p99_ms": pct(lat_h, 0.99) * 1e3,
+               "identical_to_alone": bool(ok)}
+        served.append(rec)
+        log(f"served @ {rate:.0f} q/s offered: {rec['achieved_qps']:.0f} q/s, p50 {rec['p50_ms']:.2f} ms, "
+            f"p99 {rec['p99_ms']:.2f} ms (light p99 {rec['light_p99_ms']:.2f}, heavy p99 "
+            f"{rec['heavy_p99_ms']:.2f}), identical={ok}")
+    # CPU oracle on a bounded sample: the light queries, and one heavy query
+    samp = light[:256] + [sq[i] for i in heavy_idx[:1]]
+    t = time.perf_counter()
+    cpu_ok, cpu_ex = True, 0
+    ref = {tuple(q): a for q, a in zip(sq, alone)}
+    for q in samp:
+        want, wex, _ = oracle.run_query(w.gstore, q)
+        a = ref[tuple(q)]
+        cpu_ok = cpu_ok and want.shape == a.table.shape and np.array_equal(want, a.table) and wex == a.expanded
+        cpu_ex += wex
+    cpu_s = time.perf_counter() - t
+    best = max(served, key=lambda r: r["achieved_qps"]) if served else None
+    log(f"cpu oracle: {len(samp)} queries in {cpu_s:.1f} s ({len(samp) / cpu_s:.0f} q/s), parity={cpu_ok}")
+    _line("expanded edges/s + queries/s; p50/p99 latency, batched vs plain on 1 MI355X",
+          best["achieved_qps"] if best else 0.0, "queries/s", len(sq), 0,
+          1e3 / best["achieved_qps"] if best else 0.0,
+          {"workload": "served open-loop Poisson mix (95% light 2-3 patterns, 5% heavy 6 patterns) on a "
+                       "1B-triple graph: 100M entities, 256 predicates, Zipf 1.0; 64 GB table budget",
+           **{k: v for k, v in w.info.items() if k != "light_types"},
+           "queries_per_rate": len(sq), "heavy_queries": len(heavy_idx)},
+          {"served": served, "plain_queries_per_s": len(sq) / t_plain,
+           "light_batched_queries_per_s": len(light) / t_light, "tuning": tuning, "heavy_tuning": htuning,
+           "parity": {"served_equals_alone": all_same,
+                      "light_batched_equals_plain": _same([alone[i] for i in light_idx], light_b),
+                      "cpu_sample_equal": bool(cpu_ok)},
+           "cpu_baseline": {"value": len(samp) / cpu_s, "unit": "queries/s", "cores": 1, "kind": "port",
+                            "sample": f"{len(samp)} of the served queries (256 light, 1 heavy) on one thread",
+                            **_cpu_info()}})
+    eng.close()
+    dev.close()
+
+
+def run_config5(args):
+    """The cyclic 6-pattern query on the 2B-triple graph: plain path with a
+    4 GB output-table limit (the ~1B-row hub fan-out is produced and consumed
+    in row chunks, depth-first), the device-resident executor on a batch of
+    the same cycle from both start types, and the CPU oracle on the sample."""
+    import numpy as np
+    import oracle
+    from wukonggpu_amd import Device, GPUEngine, workloads
+    t0 = time.time()
+    w = workloads.config5(seed=5)
+    log(f"graph: {w.info['triples']} triples, {w.info['keys']} keys, image {w.info['image_bytes'] / 1e9:.2f} GB "
+        f"built in {time.time() - t0:.1f}s; hub {w.info['hub']} degree {w.info['hub_degree']}")
+    dev = Device(0, table_budget_bytes=96 << 30)
+    dev.load(w.gstore)
+    eng = GPUEngine(dev)
+    full, sample = w.queries["full"], w.queries["sample"]
+    dev.set_table_limit(4 << 30)
+    r = eng.run_query(full)  # warm
+    lat = []
+    for _ in range(max(1, min(args.steps, 3))):
+        t = time.perf_counter()
+        r = eng.run_query(full)
+        lat.append(time.perf_counter() - t)
+    el = sum(lat) / len(lat)
+    splits = eng.splits()
+    log(f"full query, plain chunked: {r.table.shape[0]} rows, {r.expanded} expanded edges, {el * 1e3:.1f} ms "
+        f"({r.expanded / el / 1e9:.2f} G edges/s), {splits} row-half splits so far")
+    dev.set_table_limit(0)
+    eng.configure(max_queries=16, max_patterns=8, arena_bytes=64 << 30)
+    bx = eng.run_batched([full, sample])
+    tb = time.perf_counter()
+    bx = eng.run_batched([full, sample])
+    t_b = time.perf_counter() - tb
+    rs = eng.run_query(sample)
+    same_b = _same([r, rs], bx)
+    log(f"executor batch [full, sample]: {t_b * 1e3:.1f} ms, identical to plain={same_b}")
+    want, wex, sec = oracle.run_query(w.gstore, sample)
+    cpu_ok = bool(want.shape == rs.table.shape and np.array_equal(want, rs.table) and wex == rs.expanded)
+    log(f"cpu oracle on the sample: {wex} expanded edges in {sec:.1f} s, parity={cpu_ok}")
+    _line("expanded edges/s + queries/s; p50/p99 latency, batched vs plain on 1 MI355X",
+          r.expanded / el, "expanded edges/s", len(lat), 1, el * 1e3,
+          {"workload": "6-pattern cyclic query (triangle closed by known_to_known, known_to_const filter) on a "
+                       "2B-triple graph: 200M entities, 512 predicates, Zipf 0.9, 2M-edge hubs; 4 GB output-table "
+                       "limit (HBM-budgeted chunks)", **w.info, "query": full,
+           "result_rows": int(r.table.shape[0]), "expanded_edges_per_query": int(r.expanded),
+           "row_half_splits": splits},
+          {"batched": {"queries": 2, "ms": t_b * 1e3, "identical_to_plain": same_b},
+           "parity": {"sample_equal_cpu": cpu_ok, "executor_equals_plain": same_b},
+           "cpu_baseline": {"value": wex / sec, "unit": "expanded edges/s", "cores": 1, "kind": "port",
+                            "sample": f"the same cycle from the {w.info['t1_members']}-member type t1 "
+                                      f"({wex} expanded edges) on one thread", **_cpu_info()}})
     eng.close()
     dev.close()
 

@@ -44,8 +44,14 @@ const char *wke_last_error(const wke_engine *e);
 void wke_free(void *p);
 
 /* Plain path: one query, one step at a time (GPU_Engine::execute_request,
- * core/gpu_engine.hpp:471-526, collapsed to one server). */
+ * core/gpu_engine.hpp:471-526, collapsed to one server). A step whose tables
+ * do not fit the context's table budget runs on the row halves of its input,
+ * each carried depth-first through the rest of the query (HBM-budgeted
+ * chunks); the result is the same table an unchunked run gives. */
 int wke_run_query(wke_engine *e, const int64_t *cmd, int ncmd, wke_result *res);
+
+/* Row-half splits made so far by the plain path's chunking. */
+int wke_stats(const wke_engine *e, uint64_t *splits);
 
 /* Plain path over many queries, one after another (the reference runs one
  * query on the GPU at a time, core/gpu.hpp:53-61). */

@@ -100,6 +100,15 @@ int wkg_table_to_host(wkg_ctx *ctx, const wkg_table *t, int32_t *host, void *str
 int wkg_table_shape(const wkg_table *t, uint64_t *rows, int *cols);
 /* Device address of the table's first cell (row-major int32). */
 const int32_t *wkg_table_device_ptr(const wkg_table *t);
+/* A known_to_unknown step of two or more input rows whose output table would
+ * exceed max_table_bytes fails with WKG_ENOMEM before allocating it (0: no
+ * limit, the default). The
+ * engine then runs the step on row halves of its input, so a huge table is
+ * produced and consumed in chunks of at most this size. */
+int wkg_set_table_limit(wkg_ctx *ctx, uint64_t max_table_bytes);
+/* A view of rows [row0, row0 + nrows) of t, sharing its memory (for running
+ * a step over a table in row chunks). Free the view before t. */
+int wkg_table_slice(wkg_ctx *ctx, const wkg_table *t, uint64_t row0, uint64_t nrows, wkg_table **out);
 int wkg_table_free(wkg_ctx *ctx, wkg_table *t);
 
 /* ---- pattern steps (one RCache handler each) -------------------------------------

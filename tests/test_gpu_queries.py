@@ -103,3 +103,37 @@ def test_engine_rejects_bad_plans(eng_small):
     for q in ([VB, 2, 1, VB + 1], [-1, 2, 1, -2], [2, 0, 1, -1, -1, 2, 1, -3], [2, 5, 1, -1]):
         with pytest.raises(DeviceError):
             eng_small.run_query(q)
+
+
+def test_chunked_plain_path_matches_oracle(graph_hubs):
+    """Outputs above a table limit are produced in row chunks, depth-first,
+    and a table budget below a step's needs splits that step's input: every
+    answer and work count is the unchunked one (the oracle's)."""
+    _, gs = graph_hubs
+    dev = Device(0, table_budget_bytes=4 << 20)
+    dev.load(gs)
+    eng = GPUEngine(dev)
+    qs = [[2, 1, 0, -1, -1, 2, 1, -2],                       # 3000-member type index, fan-out
+          [2, 1, 0, -1, -1, 2, 1, -2, -2, 3, 0, -3],          # through the hubs
+          [2, 1, 0, -1, -1, 2, 1, -2, -2, 2, 0, -1],          # closing known_to_known
+          [3, 0, 1, -1, -1, 2, 1, -2, -1, 3, 1, VB + 5],      # predicate index + known_to_const
+          [2, 1, 0, -1, -1, 2, 1, -2, -2, 3, 0, -3, -3, 2, 1, -4]]
+    rng = np.random.default_rng(31)
+    qs += random_queries(rng, 6000, 3, [2, 3], 40, max_steps=4)
+    dev.set_table_limit(64 << 10)
+    print("plain path with a 64 KiB output table limit under a 4 MiB budget", flush=True)
+    for q in qs:
+        _check(gs, q, eng.run_query(q))
+    assert eng.splits() > 0
+    dev.set_table_limit(0)
+    print("plain path under the 4 MiB budget alone", flush=True)
+    for q in qs:
+        _check(gs, q, eng.run_query(q))
+    # the executor's arena is too small for most of them: they fall back to the chunked plain path
+    eng.configure(max_queries=64, max_patterns=6, arena_bytes=1 << 20)
+    dev.set_table_limit(64 << 10)
+    print("executor with a 1 MiB arena", flush=True)
+    for q, r in zip(qs, eng.run_batched(qs)):
+        _check(gs, q, r)
+    eng.close()
+    dev.close()

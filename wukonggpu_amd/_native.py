@@ -125,6 +125,8 @@ def hip_lib():
         _sig(L, "wkg_table_shape", C.c_int, vp, P(u64), P(C.c_int))
         _sig(L, "wkg_table_device_ptr", vp, vp)
         _sig(L, "wkg_table_free", C.c_int, vp, vp)
+        _sig(L, "wkg_table_slice", C.c_int, vp, vp, u64, u64, P(vp))
+        _sig(L, "wkg_set_table_limit", C.c_int, vp, u64)
         _sig(L, "wkg_step_i2u", C.c_int, vp, C.c_int, C.c_int, vp, P(vp), P(u64))
         _sig(L, "wkg_step_c2u", C.c_int, vp, i64, C.c_int, C.c_int, vp, P(vp), P(u64))
         _sig(L, "wkg_step_k2u", C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, vp, P(vp), P(u64))
@@ -164,6 +166,7 @@ def engine_lib():
         _sig(L, "wke_run_batched", C.c_int, vp, P(i64), P(i64), C.c_int, P(Result))
         _sig(L, "wke_run_rounds", C.c_int, vp, P(i64), P(i64), C.c_int, P(Result))
         _sig(L, "wke_configure", C.c_int, vp, C.c_int, C.c_int, u64)
+        _sig(L, "wke_stats", C.c_int, vp, P(u64))
         _sig(L, "wke_autotune", C.c_int, vp, P(i64), P(i64), C.c_int, C.c_int, C.c_int, P(BatchTuning), P(dbl))
         _sig(L, "wke_serve", C.c_int, vp, P(i64), P(i64), C.c_int, P(dbl), P(C.c_uint8), u64, C.c_int,
              P(Result), P(dbl))

@@ -258,6 +258,80 @@ struct wke_engine {
     if (r.t) wkg_table_free(ctx, r.t);
     r.t = nullptr;
   }
+
+  // Host rows of a query's result, appended chunk by chunk.
+  struct HostRows {
+    int32_t *data = nullptr;
+    uint64_t rows = 0, cap_cells = 0;
+    int cols = 0;
+    ~HostRows() { free(data); }
+  };
+
+  uint64_t splits = 0;  // row-half splits made by run_rest (wke_stats)
+
+  // Runs r's remaining steps. A step whose tables do not fit the table
+  // budget (WKG_ENOMEM) is retried on the two row halves of its input, each
+  // carried depth-first through the rest of the query, so only one chunk's
+  // intermediate tables are live at a time. Final rows are appended in input
+  // order -- the order an unchunked run gives, since every step keeps its
+  // input rows' order -- and work counts add up. Takes ownership of r.t.
+  int run_rest(Request &r, HostRows &out, uint64_t &expanded, int depth, void *stream) {
+    while (!r.is_finished()) {
+      const int rc = run_one_step(r, stream);
+      if (rc == WKG_OK) continue;
+      uint64_t R = 0;
+      int c = 0;
+      if (r.t) wkg_table_shape(r.t, &R, &c);
+      if (rc != WKG_ENOMEM || !r.t || R < 2 || depth >= 48) {
+        drop(r);
+        return rc;
+      }
+      splits++;
+      expanded += r.expanded;
+      const uint64_t half = R / 2;
+      for (int h = 0; h < 2; h++) {
+        Request sub = r;
+        sub.expanded = 0;
+        sub.t = nullptr;
+        if (int src = wkg_table_slice(ctx, r.t, h ? half : 0, h ? R - half : half, &sub.t)) {
+          drop(r);
+          return fail_ctx(src);
+        }
+        if (int sr = run_rest(sub, out, expanded, depth + 1, stream)) {
+          drop(r);
+          return sr;
+        }
+      }
+      drop(r);  // the views of both halves are gone
+      return WKG_OK;
+    }
+    expanded += r.expanded;
+    uint64_t rows = 0;
+    int cols = r.col_num;
+    if (r.t) wkg_table_shape(r.t, &rows, &cols);
+    out.cols = cols;
+    const uint64_t n = rows * (uint64_t)cols;
+    if (n) {
+      const uint64_t need = out.rows * (uint64_t)cols + n;
+      if (need > out.cap_cells) {
+        const uint64_t cap = std::max<uint64_t>(need, out.cap_cells + out.cap_cells / 2);
+        int32_t *d = (int32_t *)realloc(out.data, cap * 4);
+        if (!d) {
+          drop(r);
+          return fail(WKG_ENOMEM, "host result buffer");
+        }
+        out.data = d;
+        out.cap_cells = cap;
+      }
+      if (int rc = wkg_table_to_host(ctx, r.t, out.data + out.rows * (uint64_t)cols, stream)) {
+        drop(r);
+        return fail_ctx(rc);
+      }
+      out.rows += rows;
+    }
+    drop(r);
+    return WKG_OK;
+  }
 };
 
 extern "C" {
@@ -285,17 +359,25 @@ int wke_run_query(wke_engine *e, const int64_t *cmd, int ncmd, wke_result *res) 
   Request r;
   r.cmd = cmd;
   r.ncmd = ncmd;
-  while (!r.is_finished()) {
-    int rc = e->run_one_step(r, nullptr);
-    if (rc) {
-      e->drop(r);
-      res->status = rc;
-      return rc;
-    }
+  wke_engine::HostRows out;
+  uint64_t ex = 0;
+  int rc = e->run_rest(r, out, ex, 0, nullptr);
+  res->status = rc;
+  if (rc == WKG_OK) {
+    res->rows = out.rows;
+    res->cols = out.cols;
+    res->expanded = ex;
+    res->table = out.data;  // handed to the caller (wke_free)
+    out.data = nullptr;
   }
-  int rc = e->finish(r, res, nullptr);
   res->latency_s = std::chrono::duration<double>(Clock::now() - t0).count();
   return rc;
+}
+
+int wke_stats(const wke_engine *e, uint64_t *splits) {
+  if (!e) return WKG_EINVAL;
+  if (splits) *splits = e->splits;
+  return WKG_OK;
 }
 
 int wke_run_plain(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, wke_result *res) {

@@ -72,8 +72,9 @@ size_t round_size(size_t n) {
 void *Pool::alloc(size_t bytes, hipStream_t s, std::string &err) {
   std::lock_guard<std::mutex> lk(mu);
   const size_t sz = round_size(bytes ? bytes : 1);
-  for (auto it = cached.lower_bound(sz); it != cached.end() && it->first <= sz + sz / 4 + (size_t(2) << 20);
-       ++it) {
+  // reuse a cached block at most a quarter larger: a small request must not
+  // pin a big block, which counts in full against the budget
+  for (auto it = cached.lower_bound(sz); it != cached.end() && it->first <= sz + sz / 4; ++it) {
     if (it->second.stream != s) continue;
     void *p = it->second.ptr;
     live[p] = it->first;
@@ -174,7 +175,7 @@ int wkg_ctx::new_table(uint64_t rows, int cols, hipStream_t s, wkg_table **out) 
 
 void wkg_ctx::drop_table(wkg_table *t) {
   if (!t) return;
-  pool->release(t->d, t->stream);
+  if (t->owned) pool->release(t->d, t->stream);
   delete t;
 }
 
@@ -336,6 +337,10 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
     if (expanded) *expanded = cnt[1];
     *out = t;
     return WKG_OK;
+  }
+  if (table_limit && R >= 2 && cnt[0] * (uint64_t)out_cols * 4 > table_limit) {  // (one row cannot be split)
+    pool->release(scr, s);
+    return fail(WKG_ENOMEM, "a %llu-row output is above the table limit", (unsigned long long)cnt[0]);
   }
   prof_add_bytes("finalize", 32.0 * nz);
   prof_add_bytes("probe", 128.0 * (double)cnt[1] + 16.0 * (double)nz);
@@ -637,6 +642,29 @@ int wkg_table_shape(const wkg_table *t, uint64_t *rows, int *cols) {
 }
 
 const int32_t *wkg_table_device_ptr(const wkg_table *t) { return t ? t->d : nullptr; }
+
+int wkg_set_table_limit(wkg_ctx *ctx, uint64_t max_table_bytes) {
+  if (!ctx) return WKG_EINVAL;
+  ctx->table_limit = max_table_bytes;
+  return WKG_OK;
+}
+
+int wkg_table_slice(wkg_ctx *ctx, const wkg_table *t, uint64_t row0, uint64_t nrows, wkg_table **out) {
+  if (!ctx || !t || !out) return WKG_EINVAL;
+  if (row0 > t->rows || nrows > t->rows - row0)
+    return ctx->fail(WKG_EINVAL, "rows [%llu, %llu) outside a %llu-row table", (unsigned long long)row0,
+                     (unsigned long long)(row0 + nrows), (unsigned long long)t->rows);
+  wkg_table *v = new (std::nothrow) wkg_table;
+  if (!v) return ctx->fail(WKG_ENOMEM, "out of host memory");
+  v->d = t->d ? t->d + row0 * (uint64_t)t->cols : nullptr;
+  v->rows = nrows;
+  v->cols = t->cols;
+  v->bytes = nrows * (uint64_t)t->cols * sizeof(int32_t);
+  v->stream = t->stream;
+  v->owned = false;
+  *out = v;
+  return WKG_OK;
+}
 
 int wkg_table_free(wkg_ctx *ctx, wkg_table *t) {
   if (!ctx) return WKG_EINVAL;

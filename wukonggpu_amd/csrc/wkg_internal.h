@@ -44,6 +44,7 @@ struct wkg_table {
   int cols;
   uint64_t bytes;
   hipStream_t stream;  // stream that produced it; its memory returns there
+  bool owned = true;   // false: a row slice of another table (wkg_table_slice)
 };
 
 // Framed residency (graph larger than its HBM budget): per-predicate
@@ -69,6 +70,7 @@ struct wkg_ctx {
   hipStream_t stream = nullptr;
   Pool *pool = nullptr;
   uint64_t budget_request = 0;
+  uint64_t table_limit = 0;  // largest step output table in bytes (0: no limit)
   uint64_t *h_rb = nullptr;  // pinned read-back words
   std::string err;
 

@@ -100,6 +100,10 @@ class Device:
         self._check(N.hip_lib().wkg_bandwidth_probe(self._h, nbytes, iters, C.byref(g)))
         return float(g.value)
 
+    def set_table_limit(self, nbytes):
+        """Largest step output table; bigger ones are produced in row chunks (0: none)."""
+        self._check(N.hip_lib().wkg_set_table_limit(self._h, nbytes))
+
     def random_access_rate(self, nbytes=4 << 30, iters=10):
         """Random 128-byte line reads per second (the hash probe's roofline)."""
         g = C.c_double()
@@ -238,6 +242,12 @@ class GPUEngine:
     def run_rounds(self, queries):
         """Step-synchronous batching: one fused launch set per step round."""
         return self._many(N.engine_lib().wke_run_rounds, queries)
+
+    def splits(self):
+        """Row-half splits made so far by the plain path's HBM-budgeted chunking."""
+        n = C.c_uint64()
+        N.engine_lib().wke_stats(self._h, C.byref(n))
+        return int(n.value)
 
     def configure(self, max_queries=1024, max_patterns=8, arena_bytes=16 << 30):
         L = N.engine_lib()

@@ -153,6 +153,14 @@ class GStore:
     def image_nbytes(self):
         return self.num_slots * 16 + self.num_edges * 4
 
+    def degree(self, vid, pid, d):
+        """Length of the edge list of key (vid, pid, d) (0 when absent), without a copy."""
+        L = N.gstore_lib()
+        ptr = C.POINTER(C.c_uint32)()
+        n = C.c_uint64()
+        _check(L.wks_get_edges(self._h, vid, pid, d, C.byref(ptr), C.byref(n)), "get_edges")
+        return int(n.value)
+
     def get_edges(self, vid, pid, d):
         """Edge list of key (vid, pid, d) as a numpy copy (empty when absent)."""
         L = N.gstore_lib()

@@ -129,3 +129,71 @@ def served_mix(P, type_ids, t0, n, seed, heavy_frac=0.05):
             qs.append(q[:4 * int(rng.integers(2, 4))])
             flags.append(0)
     return qs, np.array(flags, dtype=np.uint8)
+
+
+def config4(seed=4, n_light_types=1023):
+    """Served graph: 100M entities, 256 predicates, ~1B triples (Zipf 1.0),
+    type t0 with 500k members for the heavy queries and n_light_types types
+    with log-uniform sizes in [1, 1e4] for the light ones. The triples are
+    released once the image is built."""
+    N, P = 100_000_000, 256
+    rng = np.random.default_rng(seed)
+    extra = np.exp(rng.uniform(0.0, np.log(1e4), n_light_types)).astype(np.int64)
+    sizes = [500_000] + [int(x) for x in extra]
+    tr = Triples.generate(seed=seed, num_entities=N, num_preds=P, num_triples=1_000_000_000, zipf_s=1.0,
+                          type_sizes=sizes)
+    n_tr = len(tr)
+    gs = GStore.build(tr)
+    tr.close()
+    t0 = 2 + P
+    info = {"entities": N, "predicates": P, "triples": n_tr, "types": len(sizes), "t0": t0,
+            "t0_members": sizes[0], "light_types": list(range(t0 + 1, t0 + len(sizes))), "keys": gs.num_keys,
+            "edges": gs.num_edges, "image_bytes": gs.image_nbytes()}
+    return Workload("config4", None, gs, {}, info)
+
+
+def cyclic_query(t, pa, ph, pc, pq, const):
+    """Config-5 six-pattern cycle from type index t:
+        ?a rdf:type t .   index_to_unknown
+        ?a pa ?b .        known_to_unknown  OUT
+        ?c ph ?b .        known_to_unknown  IN   (?c: ph's subjects, the hub among them)
+        ?c ph ?d .        known_to_unknown  OUT  (hub fan-out: the ~1B-row table)
+        ?d pc ?b .        known_to_known         (closes the triangle b-c-d)
+        ?c pq const .     known_to_const
+    """
+    return [t, 1, IN, -1,
+            -1, pa, OUT, -2,
+            -2, ph, IN, -3,
+            -3, ph, OUT, -4,
+            -4, pc, OUT, -2,
+            -3, pq, OUT, const]
+
+
+def config5(seed=5, t0_members=2_000_000, sample_members=20_000):
+    """Large graph: 200M entities, 512 predicates, ~2B triples, Zipf 0.9, and
+    hubs: the rank-1 subject of each of the first 8 predicates has 2M
+    edges. Types: t0 (t0_members) for the full query and t1
+    (sample_members) for a parity sample small enough for the oracle's
+    per-row work. The cyclic query goes through hub predicate ph = 2; the
+    known_to_const constant is one of the hub's own ph objects."""
+    N, P = 200_000_000, 512
+    sizes = [t0_members, sample_members]
+    tr = Triples.generate(seed=seed, num_entities=N, num_preds=P, num_triples=2_000_000_000, zipf_s=0.9,
+                          type_sizes=sizes, num_hubs=8, hub_degree=2_000_000)
+    n_tr = len(tr)
+    gs = GStore.build(tr)
+    tr.close()
+    t0, t1 = 2 + P, 2 + P + 1
+    ph, pa, pc, pq = 2, 100, 200, 2
+    # the hub of ph: the subject with the longest ph OUT list; the constant is its first object
+    hub, best = None, -1
+    for s in gs.get_edges(0, ph, IN).tolist():
+        d = gs.degree(s, ph, OUT)
+        if d > best:
+            hub, best = s, d
+    const = int(gs.get_edges(hub, ph, OUT)[0])
+    queries = {"full": cyclic_query(t0, pa, ph, pc, pq, const), "sample": cyclic_query(t1, pa, ph, pc, pq, const)}
+    info = {"entities": N, "predicates": P, "triples": n_tr, "t0": t0, "t1": t1, "t0_members": t0_members,
+            "t1_members": sample_members, "ph": ph, "pa": pa, "pc": pc, "pq": pq, "hub": hub, "hub_degree": best,
+            "const": const, "keys": gs.num_keys, "edges": gs.num_edges, "image_bytes": gs.image_nbytes()}
+    return Workload("config5", None, gs, queries, info)
