@@ -64,6 +64,12 @@ int wke_run_plain(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int
  * arena is rerun on the plain path, so every result matches wke_run_query. */
 int wke_run_batched(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, wke_result *res);
 
+/* wke_run_batched with every result table in one buffer: *cells (malloc'd,
+ * release with wke_free) holds *ncells cells; query q's table starts at
+ * cell_off[q] (res[q].table stays NULL). */
+int wke_run_batched_packed(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, wke_result *res,
+                           uint64_t *cell_off, int32_t **cells, uint64_t *ncells);
+
 /* Step-synchronous batched path: all queries advance one step per round, the
  * current steps of all unfinished queries fused into one wkg_step_batch. */
 int wke_run_rounds(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, wke_result *res);

@@ -227,3 +227,20 @@ def _plan(cmd):
             steps.append(dict(kind=STEP_K2U, pid=p, dir=d, start_col=-s - 1))
             cols += 1
     return steps
+
+
+def test_packed_results_match(eng_small, graph_small):
+    """wke_run_batched_packed: one buffer for all tables, same answers."""
+    _, gs = graph_small
+    rng = np.random.default_rng(28)
+    qs = random_queries(rng, 300, 4, [6, 7], 150, max_steps=5)
+    eng_small.configure(max_queries=64, max_patterns=8, arena_bytes=1 << 30)
+    packed, wall = eng_small.run_batched_packed(qs)
+    assert wall > 0
+    for q, r in zip(qs, packed):
+        _same(gs, q, r)
+    # with a 1 MiB arena most fall back to the plain path and are appended
+    eng_small.configure(max_queries=64, max_patterns=8, arena_bytes=1 << 20)
+    packed, _ = eng_small.run_batched_packed(qs)
+    for q, r in zip(qs, packed):
+        _same(gs, q, r)

@@ -165,6 +165,8 @@ def engine_lib():
         _sig(L, "wke_run_plain", C.c_int, vp, P(i64), P(i64), C.c_int, P(Result))
         _sig(L, "wke_run_batched", C.c_int, vp, P(i64), P(i64), C.c_int, P(Result))
         _sig(L, "wke_run_rounds", C.c_int, vp, P(i64), P(i64), C.c_int, P(Result))
+        _sig(L, "wke_run_batched_packed", C.c_int, vp, P(i64), P(i64), C.c_int, P(Result), P(u64),
+             P(P(i32)), P(u64))
         _sig(L, "wke_configure", C.c_int, vp, C.c_int, C.c_int, u64)
         _sig(L, "wke_stats", C.c_int, vp, P(u64))
         _sig(L, "wke_autotune", C.c_int, vp, P(i64), P(i64), C.c_int, C.c_int, C.c_int, P(BatchTuning), P(dbl))

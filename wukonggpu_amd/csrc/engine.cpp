@@ -491,8 +491,27 @@ void add_plan(Compiled &c, const std::vector<wkg_plan_step> &plan, int q) {
 
 }  // namespace
 
-int wke_run_batched(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, wke_result *res) {
-  if (!e || !cmd || !cmd_off || !res || nq < 0) return WKG_EINVAL;
+namespace {
+
+// Result sink of the batched path: per-query malloc'd tables (wke_run_batched)
+// or one packed buffer (wke_run_batched_packed).
+struct Sink {
+  bool packed = false;
+  uint64_t *cell_off = nullptr;  // packed: first cell of each query's table
+  int32_t *cells = nullptr;
+  uint64_t ncells = 0, cap = 0;
+  bool reserve(uint64_t n) {
+    if (ncells + n <= cap) return true;
+    const uint64_t c = std::max<uint64_t>(ncells + n, cap + cap / 2);
+    int32_t *d = (int32_t *)realloc(cells, c * 4);
+    if (!d) return false;
+    cells = d;
+    cap = c;
+    return true;
+  }
+};
+
+int run_batched_into(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, wke_result *res, Sink &sk) {
   const auto t0 = Clock::now();
   auto since = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
   int first = WKG_OK;
@@ -503,6 +522,7 @@ int wke_run_batched(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, i
   std::vector<std::vector<wkg_plan_step>> plans(nq);
   for (int q = 0; q < nq; q++) {
     memset(&res[q], 0, sizeof res[q]);
+    if (sk.packed) sk.cell_off[q] = 0;
     const int n = (int)(cmd_off[q + 1] - cmd_off[q]);
     std::string why;
     if (int rc = compile_query(cmd + cmd_off[q], n, plans[q], why)) {
@@ -524,16 +544,32 @@ int wke_run_batched(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, i
         add_plan(c, plans[q], q);
     if (c.who.empty()) break;
     const int32_t *cells = nullptr;
-    int rc = wkg_batch_run(b, (int)c.who.size(), c.steps.data(), c.off.data(), rec.data(), &cells, nullptr);
+    uint64_t ncells = 0;
+    int rc = wkg_batch_run(b, (int)c.who.size(), c.steps.data(), c.off.data(), rec.data(), &cells, &ncells);
     if (rc) return e->fail_ctx(rc);
     const double lat = since();
+    uint64_t base = 0;
+    if (sk.packed) {  // the batch's result buffer in one piece
+      if (!sk.reserve(ncells)) return e->fail(WKG_ENOMEM, "host result buffer");
+      base = sk.ncells;
+      if (ncells) memcpy(sk.cells + base, cells, ncells * 4);
+      sk.ncells += ncells;
+    }
     for (size_t i = 0; i < c.who.size(); i++) {
       const int qi = c.who[i];
       if (rec[i].status == WKG_ENOMEM) {
         plain.push_back(qi);
         continue;
       }
-      fill_result(rec[i], cells, &res[qi]);
+      if (sk.packed) {
+        res[qi].status = rec[i].status;
+        res[qi].expanded = rec[i].expanded;
+        res[qi].rows = rec[i].status == WKG_OK ? rec[i].rows : 0;
+        res[qi].cols = rec[i].cols;
+        sk.cell_off[qi] = base + rec[i].offset;
+      } else {
+        fill_result(rec[i], cells, &res[qi]);
+      }
       res[qi].latency_s = lat;
       note(res[qi].status);
     }
@@ -542,8 +578,37 @@ int wke_run_batched(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, i
     int rc = wke_run_query(e, cmd + cmd_off[qi], (int)(cmd_off[qi + 1] - cmd_off[qi]), &res[qi]);
     res[qi].latency_s = since();
     note(rc);
+    if (sk.packed && rc == WKG_OK) {
+      const uint64_t n = res[qi].rows * (uint64_t)res[qi].cols;
+      if (!sk.reserve(n)) return e->fail(WKG_ENOMEM, "host result buffer");
+      if (n) memcpy(sk.cells + sk.ncells, res[qi].table, n * 4);
+      sk.cell_off[qi] = sk.ncells;
+      sk.ncells += n;
+      free(res[qi].table);
+      res[qi].table = nullptr;
+    }
   }
   return first;
+}
+
+}  // namespace
+
+int wke_run_batched(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, wke_result *res) {
+  if (!e || !cmd || !cmd_off || !res || nq < 0) return WKG_EINVAL;
+  Sink sk;
+  return run_batched_into(e, cmd, cmd_off, nq, res, sk);
+}
+
+int wke_run_batched_packed(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, wke_result *res,
+                           uint64_t *cell_off, int32_t **cells, uint64_t *ncells) {
+  if (!e || !cmd || !cmd_off || !res || nq < 0 || !cell_off || !cells || !ncells) return WKG_EINVAL;
+  Sink sk;
+  sk.packed = true;
+  sk.cell_off = cell_off;
+  int rc = run_batched_into(e, cmd, cmd_off, nq, res, sk);
+  *cells = sk.cells;
+  *ncells = sk.ncells;
+  return rc;
 }
 
 int wke_autotune(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, int reps, int heavy,

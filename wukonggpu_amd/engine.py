@@ -239,6 +239,33 @@ class GPUEngine:
         """Device-resident executor: whole queries on the device, one captured launch sequence."""
         return self._many(N.engine_lib().wke_run_batched, queries)
 
+    def run_batched_packed(self, queries):
+        """run_batched with all tables returned in one buffer. Returns (results,
+        seconds spent in the native call); the tables are views of one array."""
+        import time
+        L = N.engine_lib()
+        flat, off = pack_queries(queries)
+        n = len(queries)
+        res = (N.Result * max(n, 1))()
+        coff = np.zeros(max(n, 1), dtype=np.uint64)
+        cells = C.POINTER(C.c_int32)()
+        ncells = C.c_uint64()
+        t0 = time.perf_counter()
+        L.wke_run_batched_packed(self._h, _i64p(flat), _i64p(off), n, res,
+                                 coff.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(cells), C.byref(ncells))
+        wall = time.perf_counter() - t0
+        buf = (np.ctypeslib.as_array(cells, shape=(ncells.value,)).copy() if ncells.value
+               else np.zeros(0, np.int32))
+        if cells:
+            L.wke_free(C.cast(cells, C.c_void_p))
+        out = []
+        for i in range(n):
+            r = res[i]
+            c0 = int(coff[i])
+            t = buf[c0:c0 + r.rows * r.cols].reshape(r.rows, max(r.cols, 0))
+            out.append(QueryResult(t, r.status, r.expanded, r.latency_s))
+        return out, wall
+
     def run_rounds(self, queries):
         """Step-synchronous batching: one fused launch set per step round."""
         return self._many(N.engine_lib().wke_run_rounds, queries)
