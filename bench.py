@@ -488,6 +488,14 @@ def run_config5(args):
     want, wex, sec = oracle.run_query(w.gstore, sample)
     cpu_ok = bool(want.shape == rs.table.shape and np.array_equal(want, rs.table) and wex == rs.expanded)
     log(f"cpu oracle on the sample: {wex} expanded edges in {sec:.1f} s, parity={cpu_ok}")
+    # row for row at scale: the sample's first four patterns end in the hub fan-out table
+    prefix = sample[:16]
+    dev.set_table_limit(256 << 20)
+    rp = eng.run_query(prefix)
+    dev.set_table_limit(0)
+    pw, pwex, psec = oracle.run_query(w.gstore, prefix)
+    prefix_ok = bool(pw.shape == rp.table.shape and np.array_equal(pw, rp.table) and pwex == rp.expanded)
+    log(f"sample prefix (4 patterns): {rp.table.shape[0]} rows, chunked at 256 MB, equal to the oracle={prefix_ok}")
     _line("expanded edges/s + queries/s; p50/p99 latency, batched vs plain on 1 MI355X",
           r.expanded / el, "expanded edges/s", len(lat), 1, el * 1e3,
           {"workload": "6-pattern cyclic query (triangle closed by known_to_known, known_to_const filter) on a "
@@ -496,7 +504,8 @@ def run_config5(args):
            "result_rows": int(r.table.shape[0]), "expanded_edges_per_query": int(r.expanded),
            "row_half_splits": splits},
           {"batched": {"queries": 2, "ms": t_b * 1e3, "identical_to_plain": same_b},
-           "parity": {"sample_equal_cpu": cpu_ok, "executor_equals_plain": same_b},
+           "parity": {"sample_equal_cpu": cpu_ok, "executor_equals_plain": same_b,
+                      "sample_prefix_rows_equal_cpu": prefix_ok, "sample_prefix_rows": int(rp.table.shape[0])},
            "cpu_baseline": {"value": wex / sec, "unit": "expanded edges/s", "cores": 1, "kind": "port",
                             "sample": f"the same cycle from the {w.info['t1_members']}-member type t1 "
                                       f"({wex} expanded edges) on one thread", **_cpu_info()}})
