@@ -62,6 +62,19 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t v, uint32_t src) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// Lane i gets lane i-1's value, lane 0 gets 0 (DPP wave_shr:1, no LDS).
+__device__ __forceinline__ uint32_t wave_shr1(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ uint64_t wave_shr1_64(uint64_t v) {
+  return ((uint64_t)wave_shr1((uint32_t)(v >> 32)) << 32) | wave_shr1((uint32_t)v);
+}
+// Lane 63's value, in a scalar register.
+__device__ __forceinline__ uint32_t lane63(uint32_t v) { return (uint32_t)__builtin_amdgcn_readlane((int)v, 63); }
+__device__ __forceinline__ uint64_t lane63_64(uint64_t v) {
+  return ((uint64_t)lane63((uint32_t)(v >> 32)) << 32) | lane63((uint32_t)v);
+}
+
 template <bool FRAMED>
 __device__ __forceinline__ uint64_t bucket_phys(const GraphView &g, const DevMeta &m, uint64_t local) {
   if (!FRAMED) return local;  // resident: callers pass global ids
@@ -114,11 +127,17 @@ __device__ __forceinline__ uint32_t wave_probe(const GraphView &g, const DevMeta
                                                uint32_t &deg_out, uint32_t &off_out, ProbeCarry &carry) {
   const uint32_t lane = lane_id(), l8 = lane & 7, grp = lane >> 3, gbase = lane & ~7u;
   const bool valid = key != 0 && m.psz != 0;
-  const uint64_t before = shfl64(key, (lane - 1) & 63);
+  const uint64_t before = wave_shr1_64(key);
   const uint64_t prev = lane == 0 ? carry.key : before;
   const bool lead = valid && key != prev;
   const uint64_t leaders = wave_ballot(lead);
   const uint32_t L = (uint32_t)__builtin_popcountll(leaders);
+  if (L == 0) {  // every valid lane continues the carried run (wave-uniform)
+    deg_out = valid ? carry.deg : 0u;
+    off_out = valid ? carry.off : 0u;
+    if (lane63_64(key) != carry.key) carry = ProbeCarry{};  // lane 63 has no key
+    return 0;
+  }
   if (lead) {
     const uint32_t j = (uint32_t)__builtin_popcountll(leaders & ((1ull << lane) - 1));
     const uint64_t local = fastmod(mix64(key), m.psz, m.magic);
@@ -205,9 +224,9 @@ __device__ __forceinline__ uint32_t wave_probe(const GraphView &g, const DevMeta
     }
   }
   __builtin_amdgcn_wave_barrier();  // w is reused by the wave's next call
-  carry.key = shfl64(key, kWave - 1);
-  carry.deg = shfl32(deg_out, kWave - 1);
-  carry.off = shfl32(off_out, kWave - 1);
+  carry.key = lane63_64(key);
+  carry.deg = lane63(deg_out);
+  carry.off = lane63(off_out);
   return L;
 }
 
