@@ -137,3 +137,42 @@ def test_chunked_plain_path_matches_oracle(graph_hubs):
         _check(gs, q, r)
     eng.close()
     dev.close()
+
+
+@pytest.fixture(scope="module")
+def graph_mid():
+    from tests.conftest import small_graph
+    # 11.5 MB image over 12 predicates and 3 types; its largest predicate
+    # segment fits a 4 MiB graph budget, all of them together do not
+    return small_graph(seed=12, n=20000, preds=12, triples=400000, s=1.0, cap=0, types=(5000, 300, 40), hubs=1,
+                       hub_degree=20000)
+
+
+def test_framed_residency_matches_oracle(graph_mid):
+    """A graph budget below the image: per-predicate segments are loaded into
+    power-of-two frames on demand and least recently used ones are evicted;
+    every step kind, and the batched and served paths, still give the
+    oracle's answers."""
+    _, gs = graph_mid
+    dev = Device(0)
+    dev.load(gs, graph_budget_bytes=4 << 20)
+    st = dev.stats()
+    assert 0 < st["resident_bytes"] <= 4 << 20 < gs.image_nbytes()
+    eng = GPUEngine(dev)
+    rng = np.random.default_rng(41)
+    qs = random_queries(rng, 20000, 12, [14, 15, 16], 90, max_steps=4)
+    print("framed: plain path", flush=True)
+    for q in qs:
+        _check(gs, q, eng.run_query(q))
+    st = dev.stats()
+    assert st["segment_loads"] > 0 and st["segment_evictions"] > 0, st
+    print("framed: step rounds", flush=True)
+    for q, r in zip(qs, eng.run_rounds(qs)):
+        _check(gs, q, r)
+    print("framed: device-resident executor (batches whose predicates do not fit run on the plain path)",
+          flush=True)
+    eng.configure(max_queries=16, max_patterns=6, arena_bytes=64 << 20)
+    for q, r in zip(qs, eng.run_batched(qs)):
+        _check(gs, q, r)
+    eng.close()
+    dev.close()

@@ -546,6 +546,10 @@ int run_batched_into(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, 
     const int32_t *cells = nullptr;
     uint64_t ncells = 0;
     int rc = wkg_batch_run(b, (int)c.who.size(), c.steps.data(), c.off.data(), rec.data(), &cells, &ncells);
+    if (rc == WKG_ENOMEM) {  // e.g. the batch's predicates do not all fit a framed graph budget at once
+      plain.insert(plain.end(), c.who.begin(), c.who.end());
+      continue;
+    }
     if (rc) return e->fail_ctx(rc);
     const double lat = since();
     uint64_t base = 0;
@@ -772,7 +776,9 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
         }
       }
       int rc = wkg_batch_launch(l.b, (int)c.who.size(), c.steps.data(), c.off.data());
-      if (rc) {
+      if (rc == WKG_ENOMEM) {  // the batch's predicates do not all fit a framed graph budget at once
+        for (int q : c.who) run_plain(q);
+      } else if (rc) {
         e->err = wkg_last_error(e->ctx);
         for (int q : c.who) {
           res[q].status = rc;
