@@ -202,7 +202,7 @@ typedef struct wkg_batch_tuning {
   int32_t expand_wg_per_cu;  /* persistent expand grid */
   int32_t aux_wg_per_cu;     /* tile scan, merge-path partition, gather */
   int32_t probe_tile_rows;   /* rows per probe tile: 1024 or 2048 */
-  int32_t expand_tile_items; /* merged items per expand tile: 1024 or 2048 */
+  int32_t expand_tile_items; /* merged items per expand tile: 512, 1024 or 2048 */
   int32_t pad;
 } wkg_batch_tuning;
 

@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 SHAPES = [dict(probe_tile_rows=tr, expand_tile_items=ni, probe_wg_per_cu=pw, expand_wg_per_cu=xw, aux_wg_per_cu=aw)
           for tr, ni, pw, xw, aw in ((1024, 1024, 1, 1, 1), (1024, 2048, 2, 4, 2), (2048, 1024, 4, 2, 1),
-                                     (2048, 2048, 2, 2, 1))]
+                                     (2048, 2048, 2, 2, 1), (1024, 512, 2, 8, 1), (2048, 512, 4, 8, 2))]
 
 
 @pytest.fixture(scope="module")
@@ -199,7 +199,7 @@ def test_autotune_keeps_answers(dev_small, graph_small):
     e.configure(max_queries=64, max_patterns=6, arena_bytes=256 << 20)
     print("autotune over 64 queries", flush=True)
     best, ms = e.autotune(qs, reps=1)
-    assert best["probe_tile_rows"] in (1024, 2048) and best["expand_tile_items"] in (1024, 2048)
+    assert best["probe_tile_rows"] in (1024, 2048) and best["expand_tile_items"] in (512, 1024, 2048)
     assert best["probe_wg_per_cu"] >= 1 and ms > 0
     for q, r in zip(qs, e.run_batched(qs)):
         _same(gs, q, r)

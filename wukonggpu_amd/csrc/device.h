@@ -25,7 +25,7 @@ namespace wkg {
 constexpr int kWave = 64;
 constexpr int kBlock = 256;            // threads per workgroup for every step kernel
 constexpr int kTileRows = 8 * kBlock;  // probe tile: 8 rows per thread
-constexpr int kExpandVT = 8;           // merged items per thread in the expand
+constexpr int kExpandVT = 2;           // merged items per thread in the expand
 constexpr int kExpandNV = kBlock * kExpandVT;
 constexpr uint32_t kMaxHops = 1u << 20;  // chain-walk guard (images are validated)
 

@@ -746,7 +746,7 @@ wkg_batch_tuning normalized(const wkg_batch_tuning &t) {
   if (r.expand_wg_per_cu <= 0) r.expand_wg_per_cu = 2;
   if (r.aux_wg_per_cu <= 0) r.aux_wg_per_cu = 1;
   if (r.probe_tile_rows != 1024) r.probe_tile_rows = 2048;
-  if (r.expand_tile_items != 1024) r.expand_tile_items = 2048;
+  if (r.expand_tile_items != 1024 && r.expand_tile_items != 2048) r.expand_tile_items = 512;
   r.pad = 0;
   return r;
 }
@@ -816,10 +816,12 @@ int wkg_batch::enqueue(int levels) {
   k_x_begin<<<1, kBlock, 0, s>>>(args);
   const bool sorted = ctx->sorted, framed = ctx->framed;
   if (t.probe_tile_rows == 1024) {
-    if (t.expand_tile_items == 1024) enqueue_shape<1024, 4>(args, s, levels, sorted, framed, gp, gx, ga, gf);
+    if (t.expand_tile_items == 512) enqueue_shape<1024, 2>(args, s, levels, sorted, framed, gp, gx, ga, gf);
+    else if (t.expand_tile_items == 1024) enqueue_shape<1024, 4>(args, s, levels, sorted, framed, gp, gx, ga, gf);
     else enqueue_shape<1024, 8>(args, s, levels, sorted, framed, gp, gx, ga, gf);
   } else {
-    if (t.expand_tile_items == 1024) enqueue_shape<2048, 4>(args, s, levels, sorted, framed, gp, gx, ga, gf);
+    if (t.expand_tile_items == 512) enqueue_shape<2048, 2>(args, s, levels, sorted, framed, gp, gx, ga, gf);
+    else if (t.expand_tile_items == 1024) enqueue_shape<2048, 4>(args, s, levels, sorted, framed, gp, gx, ga, gf);
     else enqueue_shape<2048, 8>(args, s, levels, sorted, framed, gp, gx, ga, gf);
   }
   k_x_records<<<1, kBlock, 0, s>>>(args);
@@ -1095,9 +1097,9 @@ int wkg_batch_autotune(wkg_batch *b, int nq, const wkg_plan_step *steps, const i
   std::vector<wkg_batch_result> res((size_t)std::max(nq, 1));
   std::vector<wkg_batch_tuning> cand;
   for (int tr : {1024, 2048})
-    for (int xi : {1024, 2048})
+    for (int xi : {512, 1024, 2048})
       for (int pw : {1, 2, 4})
-        for (int xw : {1, 2, 4})
+        for (int xw : {2, 4, 8})
           for (int aw : {1, 2}) cand.push_back(wkg_batch_tuning{pw, xw, aw, tr, xi, 0});
   double best_t = 1e30;
   wkg_batch_tuning best_c = normalized(b->tune);

@@ -435,11 +435,16 @@ void wkg_ctx::prof_collect() {
 }
 
 namespace {
-// Plain 16-byte-per-lane copy: the reference point for achievable HBM
-// bandwidth and for scaling FETCH_SIZE / WRITE_SIZE counters.
+// 16-byte-per-lane streaming copy (nontemporal loads and stores, 4
+// workgroups per CU, the fastest of the shapes measured): the reference point
+// for achievable HBM bandwidth and for scaling FETCH_SIZE / WRITE_SIZE counters.
 __global__ __launch_bounds__(256) void k_copy_calibrate(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
                                                          uint64_t n) {
-  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256ull) dst[i] = src[i];
+  typedef int32_t v4i __attribute__((ext_vector_type(4)));
+  const v4i *s4 = (const v4i *)src;
+  v4i *d4 = (v4i *)dst;
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256ull)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(&s4[i]), &d4[i]);
 }
 // Random 128-byte line reads, 8 lanes per line (the way the probe reads a
 // bucket) and 8 independent lines in flight per 8-lane group: the
@@ -508,7 +513,7 @@ int wkg_bandwidth_probe(wkg_ctx *ctx, uint64_t bytes, int iters, double *copy_gb
   (void)hipEventCreate(&e1);
   int dev_cus = 256;
   (void)hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
-  const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, (uint64_t)dev_cus * 16);
+  const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, (uint64_t)dev_cus * 4);
   k_copy_calibrate<<<grid, 256, 0, s>>>((const uint4 *)a, (uint4 *)b, n);  // warm up
   (void)hipEventRecord(e0, s);
   for (int i = 0; i < iters; i++) {
