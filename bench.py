@@ -139,6 +139,31 @@ def main():
     }
     log(f"probe: {probe_rows / args.steps / 1e6:.1f} M rows and {probe_keys / args.steps / 1e6:.2f} M keys probed "
         f"per query; {probe_rate / 1e9:.2f} G keys/s = {probe_rate / lines_ps:.2f} of the random-line roofline")
+    # the probe on random keys (no runs to share): one known_to_unknown step
+    # over 64M rows of uniformly random vertex ids
+    if not args.heavy_only:
+        rng_p = np.random.default_rng(args.seed + 7)
+        n_rand = 1 << 26
+        col = (1 << 17) + rng_p.integers(0, w.info["entities"], size=(n_rand, 1), dtype=np.int64)
+        t_rand = dev.table(col.astype(np.int32))
+        del col
+        out_t, _ = dev.k2u(t_rand, 0, 2, 1)  # warm
+        out_t.free()
+        dev.profile(True)
+        for _ in range(3):
+            out_t, _ = dev.k2u(t_rand, 0, 2, 1)
+            out_t.free()
+        n_r, ms_r, _ = dev.profile_read("probe")
+        keys_r = dev.profile_read("probe_keys")[2]
+        dev.profile(False)
+        t_rand.free()
+        rand_rate = keys_r / (ms_r * 1e-3)
+        probe_roofline["random_keys"] = {
+            "rows": n_rand, "keys_probed": keys_r / 3, "ms_per_launch": ms_r / max(n_r, 1),
+            "achieved": rand_rate, "frac": rand_rate / lines_ps,
+            "workload": "known_to_unknown over 64M uniformly random vertex ids, predicate 2 OUT"}
+        log(f"probe on random keys: {keys_r / 3 / 1e6:.1f} M keys in {ms_r / max(n_r, 1):.2f} ms = "
+            f"{rand_rate / 1e9:.2f} G keys/s = {rand_rate / lines_ps:.2f} of the random-line roofline")
 
     if args.heavy_only:
         print(json.dumps({"metric": METRIC, "value": value, "unit": "expanded edges/s", "n_gpus": 1,
