@@ -106,6 +106,11 @@ const int32_t *wkg_table_device_ptr(const wkg_table *t);
  * engine then runs the step on row halves of its input, so a huge table is
  * produced and consumed in chunks of at most this size. */
 int wkg_set_table_limit(wkg_ctx *ctx, uint64_t max_table_bytes);
+/* Rows per probe tile of the per-step calls: 256, 512, 1024 or 2048, or 0
+ * (the default) to choose by input size (the largest tile that leaves at
+ * least four tiles per compute unit). A tuning knob with no counterpart in
+ * the reference; results do not depend on it. */
+int wkg_set_probe_tile_rows(wkg_ctx *ctx, int rows);
 /* A view of rows [row0, row0 + nrows) of t, sharing its memory (for running
  * a step over a table in row chunks). Free the view before t. */
 int wkg_table_slice(wkg_ctx *ctx, const wkg_table *t, uint64_t row0, uint64_t nrows, wkg_table **out);

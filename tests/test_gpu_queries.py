@@ -139,6 +139,31 @@ def test_chunked_plain_path_matches_oracle(graph_hubs):
     dev.close()
 
 
+def test_probe_tile_sizes_match_oracle(graph_hubs):
+    """Every probe tile size of the per-step path (one to eight 64-row rounds
+    per wave) gives the oracle's answers, on hub rows that span many tiles."""
+    _, gs = graph_hubs
+    dev = Device(0)
+    dev.load(gs)
+    eng = GPUEngine(dev)
+    qs = [[2, 1, 0, -1, -1, 2, 1, -2],                       # 3000-member type index, fan-out
+          [2, 1, 0, -1, -1, 2, 1, -2, -2, 3, 0, -3],          # through the hubs
+          [2, 1, 0, -1, -1, 2, 1, -2, -2, 2, 0, -1],          # closing known_to_known
+          [3, 0, 1, -1, -1, 2, 1, -2, -1, 3, 1, VB + 5],      # predicate index + known_to_const
+          [2, 1, 0, -1, -1, 2, 1, -2, -2, 3, 0, -3, -3, 2, 1, -4]]
+    rng = np.random.default_rng(77)
+    qs += random_queries(rng, 6000, 3, [2, 3], 40, max_steps=5)
+    for rows in (256, 512, 1024, 2048, 0):
+        print(f"probe tile rows {rows}", flush=True)
+        dev.set_probe_tile_rows(rows)
+        for q in qs:
+            _check(gs, q, eng.run_query(q))
+    with pytest.raises(Exception):
+        dev.set_probe_tile_rows(300)
+    eng.close()
+    dev.close()
+
+
 @pytest.fixture(scope="module")
 def graph_mid():
     from tests.conftest import small_graph

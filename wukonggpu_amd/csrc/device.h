@@ -92,7 +92,7 @@ __device__ __forceinline__ uint32_t edge_at(const GraphView &g, uint64_t edge_st
 
 // Per-wave LDS scratch of wave_probe.
 struct WaveProbeLds {
-  uint4 kb[kWave];   // run leaders' (key, first bucket)
+  uint4 kb[kWave];   // pending bucket walks: (key lo, key hi, bucket, leader index)
   uint2 res[kWave];  // run leaders' (degree, offset)
 };
 
@@ -103,15 +103,18 @@ struct WaveProbeLds {
 //     previous call's lane 63) and its result: a run of equal keys that
 //     continues from the previous call is not probed again.
 //  1. A run of equal keys in consecutive lanes is probed once: its first lane
-//     (the run leader) hashes the key and parks (key, bucket) in LDS at its
-//     rank among the leaders.
-//  2. The leaders are walked 8 lanes per key: group g takes leaders g, g+8,
+//     (the run leader) hashes the key and parks (key, bucket, its rank among
+//     the leaders) in LDS as a pending walk.
+//  2. Pending walks are taken 8 lanes per walk: group g takes walks g, g+8,
 //     g+16, ...; each lane of the group loads one 16-byte slot of the bucket
-//     (one coalesced 128-byte read), a ballot finds the matching slot and
-//     lane 7's link moves to the next bucket. A group's keys are walked
-//     together, so up to 8 bucket reads are in flight per lane, and a wave
-//     with L leaders does ceil(L/8) key slots instead of 8.
-//  3. Results go back through LDS; every lane reads its run leader's.
+//     (one coalesced 128-byte read; a group's walks are loaded together, so
+//     up to 8 bucket reads are in flight per lane). The lane whose slot
+//     holds the key writes the leader's (degree, offset) to LDS; in a group
+//     with no match, lane 7 (the link slot) either records a miss or appends
+//     the walk, moved to the linked bucket, to the next hop's pending list.
+//     The next hop only walks what is still pending, compacted.
+//  3. Every lane reads its run leader's result.
+// Buckets are addressed with 32 bits (images hold fewer than 2^32 buckets).
 // Every lane of the wave must call this with the wave's own `w`. key == 0
 // means "no key". Gives the lane's degree (0 on a miss) and list offset
 // relative to the predicate's edge_start, moves the carry to lane 63 (it
@@ -121,6 +124,12 @@ struct ProbeCarry {
   uint64_t key = 0;
   uint32_t deg = 0, off = 0;
 };
+
+// Bit mask of the lanes below this one.
+__device__ __forceinline__ uint64_t lanes_below() {
+  const uint32_t lane = lane_id();
+  return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
 
 template <bool FRAMED>
 __device__ __forceinline__ uint32_t wave_probe(const GraphView &g, const DevMeta &m, uint64_t key, WaveProbeLds &w,
@@ -138,82 +147,60 @@ __device__ __forceinline__ uint32_t wave_probe(const GraphView &g, const DevMeta
     if (lane63_64(key) != carry.key) carry = ProbeCarry{};  // lane 63 has no key
     return 0;
   }
+  const uint64_t below = lanes_below();
   if (lead) {
-    const uint32_t j = (uint32_t)__builtin_popcountll(leaders & ((1ull << lane) - 1));
+    const uint32_t j = (uint32_t)__builtin_popcountll(leaders & below);
     const uint64_t local = fastmod(mix64(key), m.psz, m.magic);
     const uint64_t b = FRAMED ? bucket_phys<true>(g, m, local) : m.main_start + local;
-    w.kb[j] = make_uint4((uint32_t)key, (uint32_t)(key >> 32), (uint32_t)b, (uint32_t)(b >> 32));
+    w.kb[j] = make_uint4((uint32_t)key, (uint32_t)(key >> 32), (uint32_t)b, j);
   }
   __builtin_amdgcn_wave_barrier();
-  const uint32_t umax = (L + 7) >> 3;
-  uint64_t keyu[8], bkt[8];
-  bool act[8];
-  uint32_t deg[8], off[8];
-#pragma unroll
-  for (int u = 0; u < 8; u++) {
-    act[u] = false;
-    keyu[u] = 0;
-    bkt[u] = 0;
-    deg[u] = 0;
-    off[u] = 0;
-    const uint32_t idx = grp + 8u * u;
-    if ((uint32_t)u < umax && idx < L) {
-      const uint4 v = w.kb[idx];
-      keyu[u] = ((uint64_t)v.y << 32) | v.x;
-      bkt[u] = ((uint64_t)v.w << 32) | v.z;
-      act[u] = true;
-    }
-  }
-  uint32_t hops = 0;
-  while (wave_ballot(act[0] | act[1] | act[2] | act[3] | act[4] | act[5] | act[6] | act[7])) {
-    uint4 s[8];
+  uint32_t n = L;  // pending walks (wave-uniform)
+  for (uint32_t hop = 0; n; hop++) {
+    const uint32_t umax = (n + 7) >> 3;
+    uint4 e[8], sl[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-      s[u] = make_uint4(0, 0, 0, 0);
-      if (act[u]) s[u] = g.slots[bkt[u] * kAssoc + l8];
+      e[u] = make_uint4(0, 0, 0, 0);
+      sl[u] = make_uint4(0, 0, 0, 0);
+      if ((uint32_t)u < umax && grp + 8u * u < n) {
+        e[u] = w.kb[grp + 8u * u];
+        sl[u] = g.slots[(uint64_t)e[u].z * kAssoc + l8];
+      }
     }
+    // (every pending entry is read above before any is rewritten below: a
+    // wave's LDS operations complete in order)
+    uint32_t nn = 0;
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       if ((uint32_t)u >= umax) break;  // wave-uniform
-      const uint64_t k = ((uint64_t)s[u].y << 32) | s[u].x;
-      const bool hit = act[u] && l8 < 7 && k == keyu[u];
-      const uint32_t gb = (uint32_t)(wave_ballot(hit) >> gbase) & 0xffu;
-      const uint32_t src = gbase + (gb ? (uint32_t)__builtin_ctz(gb) : 7u);
-      const uint32_t plo = shfl32(s[u].z, src), phi = shfl32(s[u].w, src);
-      const uint32_t klo = shfl32(s[u].x, src), khi = shfl32(s[u].y, src);
-      if (act[u]) {
-        if (gb) {
-          const uint64_t v = ((uint64_t)phi << 32) | plo;
-          deg[u] = (uint32_t)ptr_size(v);
-          off[u] = (uint32_t)(ptr_off(v) - m.edge_start);
-          act[u] = false;
-        } else {
-          const uint64_t link = ((uint64_t)khi << 32) | klo;
-          if (link == 0 || ++hops > kMaxHops) {
-            act[u] = false;
-          } else {
-            const uint64_t nb = key_vid(link);
-            bkt[u] = FRAMED ? bucket_phys<true>(g, m, nb - m.indrct_start + m.psz) : nb;
-          }
-        }
+      const bool in = grp + 8u * u < n;
+      const bool hit = in && l8 < 7 && sl[u].x == e[u].x && sl[u].y == e[u].y;
+      if (hit) {
+        const uint64_t v = ((uint64_t)sl[u].w << 32) | sl[u].z;
+        w.res[e[u].w] = make_uint2((uint32_t)ptr_size(v), (uint32_t)(ptr_off(v) - m.edge_start));
       }
+      const uint32_t gb = (uint32_t)(wave_ballot(hit) >> gbase) & 0x7fu;
+      const uint64_t link = ((uint64_t)sl[u].y << 32) | sl[u].x;
+      const bool tail = in && l8 == 7 && gb == 0;  // lane 7 of a group without a match
+      const bool follow = tail && link != 0 && hop + 1 < kMaxHops;
+      if (tail && !follow) w.res[e[u].w] = make_uint2(0u, 0u);
+      const uint64_t fb = wave_ballot(follow);
+      if (follow) {
+        const uint64_t nb = key_vid(link);
+        const uint64_t bkt = FRAMED ? bucket_phys<true>(g, m, nb - m.indrct_start + m.psz) : nb;
+        w.kb[nn + (uint32_t)__builtin_popcountll(fb & below)] = make_uint4(e[u].x, e[u].y, (uint32_t)bkt, e[u].w);
+      }
+      nn += (uint32_t)__builtin_popcountll(fb);
     }
+    __builtin_amdgcn_wave_barrier();
+    n = nn;
   }
-  // lane g*8+u publishes the result of leader g + 8u
-  uint32_t d = 0, o = 0;
-#pragma unroll
-  for (int u = 0; u < 8; u++)
-    if (l8 == (uint32_t)u) {
-      d = deg[u];
-      o = off[u];
-    }
-  if (grp + 8u * l8 < L) w.res[grp + 8u * l8] = make_uint2(d, o);
-  __builtin_amdgcn_wave_barrier();
   deg_out = 0;
   off_out = 0;
   if (valid) {
     // leaders at or below this lane; none: the lane continues the carried run
-    const uint32_t nl = (uint32_t)__builtin_popcountll(leaders & ((2ull << lane) - 1));
+    const uint32_t nl = (uint32_t)__builtin_popcountll(leaders & (below | (1ull << lane)));
     if (nl) {
       const uint2 r = w.res[nl - 1];
       deg_out = r.x;
@@ -258,6 +245,36 @@ __device__ __forceinline__ uint64_t block_excl_scan(uint64_t v, uint64_t *tmp, u
   __syncthreads();
   total = tot;
   return pre + inc - v;
+}
+
+// Exclusive block scans of N values at once (one pair of barriers): v[k]
+// becomes its exclusive prefix over the block, total[k] the block sum. `tmp`
+// is N * kBlock/kWave u64 of LDS.
+template <int N>
+__device__ __forceinline__ void block_excl_scan_n(uint64_t (&v)[N], uint64_t *tmp, uint64_t (&total)[N]) {
+  const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+  constexpr int W = kBlock / kWave;
+  uint64_t inc[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) inc[k] = wave_incl_scan(v[k]);
+  if (lane == kWave - 1) {
+#pragma unroll
+    for (int k = 0; k < N; k++) tmp[k * W + wave] = inc[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    uint64_t pre = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+      const uint64_t t = tmp[k * W + w];
+      if ((uint32_t)w < wave) pre += t;
+      tot += t;
+    }
+    v[k] = pre + inc[k] - v[k];
+    total[k] = tot;
+  }
+  __syncthreads();
 }
 
 }  // namespace wkg

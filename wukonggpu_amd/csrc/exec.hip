@@ -173,13 +173,6 @@ __device__ __forceinline__ uint32_t thread_find(const uint64_t *base, uint32_t n
   return lo;
 }
 
-// Queries [q0, q1) of this thread in a single-block plan kernel.
-__device__ __forceinline__ void my_queries(uint32_t nq, uint32_t &q0, uint32_t &q1) {
-  const uint32_t per = (nq + kBlock - 1) / kBlock;
-  q0 = min(nq, threadIdx.x * per);
-  q1 = min(nq, q0 + per);
-}
-
 __device__ __forceinline__ bool level_active(const QState &s, uint32_t level) {
   return s.status == WKG_OK && level < s.nsteps;
 }
@@ -294,118 +287,134 @@ __device__ __forceinline__ LevelNeed level_need(const ExecArgs &a, uint32_t q, u
   return n;
 }
 
+// Queries per thread per chunk in the one-block plan kernels: a chunk of
+// kBlock * kPlanK queries has its needs in registers and scans them once;
+// chunks follow one another with the running prefixes carried over.
+constexpr uint32_t kPlanK = 4;
+
 template <int TR>
 __global__ __launch_bounds__(kBlock) void k_level_plan(ExecArgs a, uint32_t level) {
-  __shared__ uint64_t s_tmp[kBlock / kWave];
+  __shared__ uint64_t s_tmp[2 * (kBlock / kWave)];
   __shared__ unsigned long long s_ut, s_us;
   ExecState *st = a.st;
   const uint32_t nq = (uint32_t)st->nq;
   const uint64_t w0 = (level & 1) ? st->lo_top : 0, w1 = (level & 1) ? a.cap : st->hi_low;
   const uint64_t room = w1 - w0;
-  uint32_t q0, q1;
-  my_queries(nq, q0, q1);
   if (threadIdx.x == 0) {  // ordered before the atomics by the scans' barriers
     s_ut = 0;
     s_us = 0;
   }
-  uint64_t ss = 0, ts = 0;
-  for (uint32_t q = q0; q < q1; q++) {
-    const LevelNeed n = level_need<TR>(a, q, level);
-    ss += n.scratch;
-    ts += n.table;
-  }
-  uint64_t tot;
-  uint64_t os = block_excl_scan(ss, s_tmp, tot);
-  uint64_t ot = block_excl_scan(ts, s_tmp, tot);
-  // fitting queries (in query order) and their tiles and scan chunks
-  uint64_t nts = 0, ncs = 0;
-  {
-    uint64_t os2 = os, ot2 = ot;
-    for (uint32_t q = q0; q < q1; q++) {
-      const LevelNeed n = level_need<TR>(a, q, level);
-      if (n.active && ot2 + n.table + os2 + n.scratch <= room) {
-        nts += n.nt;
-        ncs += n.nc;
-      }
-      os2 += n.scratch;
-      ot2 += n.table;
+  uint64_t carry_s = 0, carry_t = 0, carry_nt = 0, carry_nc = 0;
+  for (uint32_t c0 = 0; c0 < nq; c0 += kBlock * kPlanK) {
+    const uint32_t qb = c0 + threadIdx.x * kPlanK;  // this thread's queries qb .. qb + kPlanK - 1
+    LevelNeed nd[kPlanK];
+    uint64_t sz[2] = {0, 0}, szt[2];
+#pragma unroll
+    for (uint32_t i = 0; i < kPlanK; i++) {
+      nd[i] = qb + i < nq ? level_need<TR>(a, qb + i, level) : LevelNeed{false, 0, 0, 0, 0, 0};
+      sz[0] += nd[i].scratch;
+      sz[1] += nd[i].table;
     }
-  }
-  uint64_t tbase = block_excl_scan(nts, s_tmp, tot);
-  const uint64_t NT = tot;
-  uint64_t cbase = block_excl_scan(ncs, s_tmp, tot);
-  const uint64_t NC = tot;
-  uint64_t used_t = 0, used_s = 0;
-  for (uint32_t q = q0; q < q1; q++) {
-    const LevelNeed n = level_need<TR>(a, q, level);
-    a.tile_base[q] = tbase;
-    a.chunk_base[q] = cbase;
-    if (n.active) {
-      QState &s = a.qs[q];
-      if (ot + n.table + os + n.scratch > room) {
-        s.status = WKG_ENOMEM;
-      } else {
-        const wkg_plan_step &d = a.up_steps[(uint64_t)q * a.P + level];
-        StepParams p{};
-        p.pid = (uint32_t)d.pid;
-        p.dir = (uint32_t)d.dir;
-        p.R = n.R;
-        p.ntiles = n.nt;
-        if (d.kind == WKG_STEP_I2U || d.kind == WKG_STEP_C2U) {
-          p.key = d.kind == WKG_STEP_I2U ? make_key(0, (uint64_t)d.pid, (uint64_t)d.dir)
-                                         : make_key((uint64_t)d.vid, (uint64_t)d.pid, (uint64_t)d.dir);
-          p.mode = kModeK2U;
-        } else {
-          p.T = s.T;
-          p.c = s.cols;
-          p.sc = (uint32_t)d.start_col;
-          p.mode = d.kind == WKG_STEP_K2U ? kModeK2U : d.kind == WKG_STEP_K2K ? kModeK2K : kModeK2C;
-          p.ec = d.kind == WKG_STEP_K2K ? (uint32_t)d.end_col : 0u;
-          p.target = (uint32_t)d.vid;
-          p.target_ok = (d.kind == WKG_STEP_K2C && d.vid >= 0 && d.vid <= 0xffffffffLL) ? 1u : 0u;
+    block_excl_scan_n<2>(sz, s_tmp, szt);
+    const uint64_t os0 = carry_s + sz[0], ot0 = carry_t + sz[1];
+    // fitting queries (in query order) and their tiles and scan chunks
+    bool fit[kPlanK];
+    uint64_t cnt[2] = {0, 0}, cntt[2];
+    {
+      uint64_t os = os0, ot = ot0;
+#pragma unroll
+      for (uint32_t i = 0; i < kPlanK; i++) {
+        fit[i] = nd[i].active && ot + nd[i].table + os + nd[i].scratch <= room;
+        if (fit[i]) {
+          cnt[0] += nd[i].nt;
+          cnt[1] += nd[i].nc;
         }
-        char *sb = scr_at(a, level, w0, w1, os, n.scratch);
-        p.totals = (uint64_t *)sb;
-        p.tile_sum = p.totals + 4;
-        p.tile_nz = p.tile_sum + n.nt;
-        p.tile_ex = p.tile_nz + n.nt;
-        p.chunk_sum = p.tile_ex + n.nt;
-        p.chunk_nz = p.chunk_sum + n.nc;
-        p.chunk_ex = p.chunk_nz + n.nc;
-        sb += al(32 + 24 * n.nt + 24 * n.nc);
-        p.hits = (uint4 *)sb;
-        sb += n.nt * TR * 16;
-        if (p.mode == kModeK2U) {
-          p.cinfo = (uint2 *)sb;
-          p.cexcl = (uint64_t *)(sb + al(n.R * 8));
-        } else {
-          p.O = (int32_t *)out_at(a, level, w0, w1, ot, n.table);
-        }
-        a.items[q].p = p;
-        s.nchunks = (uint32_t)n.nc;
-        tbase += n.nt;
-        cbase += n.nc;
-        used_t = ot + n.table;
-        used_s = os + n.scratch;
+        os += nd[i].scratch;
+        ot += nd[i].table;
       }
     }
-    os += n.scratch;
-    ot += n.table;
+    block_excl_scan_n<2>(cnt, s_tmp, cntt);
+    uint64_t tbase = carry_nt + cnt[0], cbase = carry_nc + cnt[1];
+    uint64_t os = os0, ot = ot0, used_t = 0, used_s = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kPlanK; i++) {
+      const uint32_t q = qb + i;
+      const LevelNeed &n = nd[i];
+      if (q < nq) {
+        a.tile_base[q] = tbase;
+        a.chunk_base[q] = cbase;
+      }
+      if (n.active) {
+        QState &s = a.qs[q];
+        if (!fit[i]) {
+          s.status = WKG_ENOMEM;
+        } else {
+          const wkg_plan_step &d = a.up_steps[(uint64_t)q * a.P + level];
+          StepParams p{};
+          p.pid = (uint32_t)d.pid;
+          p.dir = (uint32_t)d.dir;
+          p.R = n.R;
+          p.ntiles = n.nt;
+          if (d.kind == WKG_STEP_I2U || d.kind == WKG_STEP_C2U) {
+            p.key = d.kind == WKG_STEP_I2U ? make_key(0, (uint64_t)d.pid, (uint64_t)d.dir)
+                                           : make_key((uint64_t)d.vid, (uint64_t)d.pid, (uint64_t)d.dir);
+            p.mode = kModeK2U;
+          } else {
+            p.T = s.T;
+            p.c = s.cols;
+            p.sc = (uint32_t)d.start_col;
+            p.mode = d.kind == WKG_STEP_K2U ? kModeK2U : d.kind == WKG_STEP_K2K ? kModeK2K : kModeK2C;
+            p.ec = d.kind == WKG_STEP_K2K ? (uint32_t)d.end_col : 0u;
+            p.target = (uint32_t)d.vid;
+            p.target_ok = (d.kind == WKG_STEP_K2C && d.vid >= 0 && d.vid <= 0xffffffffLL) ? 1u : 0u;
+          }
+          char *sb = scr_at(a, level, w0, w1, os, n.scratch);
+          p.totals = (uint64_t *)sb;
+          p.tile_sum = p.totals + 4;
+          p.tile_nz = p.tile_sum + n.nt;
+          p.tile_ex = p.tile_nz + n.nt;
+          p.chunk_sum = p.tile_ex + n.nt;
+          p.chunk_nz = p.chunk_sum + n.nc;
+          p.chunk_ex = p.chunk_nz + n.nc;
+          sb += al(32 + 24 * n.nt + 24 * n.nc);
+          p.hits = (uint4 *)sb;
+          sb += n.nt * TR * 16;
+          if (p.mode == kModeK2U) {
+            p.cinfo = (uint2 *)sb;
+            p.cexcl = (uint64_t *)(sb + al(n.R * 8));
+          } else {
+            p.O = (int32_t *)out_at(a, level, w0, w1, ot, n.table);
+          }
+          a.items[q].p = p;
+          s.nchunks = (uint32_t)n.nc;
+          tbase += n.nt;
+          cbase += n.nc;
+          used_t = ot + n.table;
+          used_s = os + n.scratch;
+        }
+      }
+      os += n.scratch;
+      ot += n.table;
+    }
+    // the fitting queries form a prefix (their cumulative needs are the
+    // smallest), so the highest ends in use are the last fitting query's
+    if (used_t) atomicMax(&s_ut, (unsigned long long)used_t);
+    if (used_s) atomicMax(&s_us, (unsigned long long)used_s);
+    carry_s += szt[0];
+    carry_t += szt[1];
+    carry_nt += cntt[0];
+    carry_nc += cntt[1];
   }
-  // the fitting queries form a prefix (their cumulative needs are the
-  // smallest), so the highest ends in use are the last fitting query's
-  if (used_t) atomicMax(&s_ut, (unsigned long long)used_t);
-  if (used_s) atomicMax(&s_us, (unsigned long long)used_s);
   __syncthreads();
   if (threadIdx.x == kBlock - 1) {
-    a.tile_base[nq] = NT;
-    a.chunk_base[nq] = NC;
+    a.tile_base[nq] = carry_nt;
+    a.chunk_base[nq] = carry_nc;
     st->w0 = w0;
     st->w1 = w1;
     st->out_used = s_ut;
     st->scr_used = s_us;
-    st->nt = NT;
-    st->nc = NC;
+    st->nt = carry_nt;
+    st->nc = carry_nc;
   }
   if (threadIdx.x < kSubQueues) st->queue[0][threadIdx.x][0] = 0;
 }
@@ -498,135 +507,148 @@ __device__ __forceinline__ ExpandNeed expand_need(const ExecArgs &a, uint32_t q,
 
 template <int NV>
 __global__ __launch_bounds__(kBlock) void k_expand_plan(ExecArgs a, uint32_t level) {
-  __shared__ uint64_t s_tmp[kBlock / kWave];
+  __shared__ uint64_t s_tmp[3 * (kBlock / kWave)];
   __shared__ unsigned long long s_ut, s_us, s_rb, s_over;
   ExecState *st = a.st;
   const uint32_t nq = (uint32_t)st->nq;
   const uint64_t w0 = st->w0, w1 = st->w1, ou = st->out_used, su = st->scr_used;
   const uint64_t room = (w1 - w0) - ou - su;
   const uint64_t rb0 = st->res_bytes;
-  uint32_t q0, q1;
-  my_queries(nq, q0, q1);
   if (threadIdx.x == 0) {
     s_ut = s_us = s_rb = s_over = 0;
   }
-  uint64_t ts = 0, ss = 0;
-  for (uint32_t q = q0; q < q1; q++) {
-    const ExpandNeed n = expand_need<NV>(a, q, level);
-    ts += n.table;
-    ss += n.scratch;
-  }
-  uint64_t tot;
-  uint64_t ot = block_excl_scan(ts, s_tmp, tot);
-  uint64_t os = block_excl_scan(ss, s_tmp, tot);
-  // pass 2: arena fit; partition entries, expand tiles, result bytes of finishing queries
-  uint64_t nps = 0, nxs = 0, rbs = 0;
-  {
-    uint64_t ot2 = ot, os2 = os;
-    for (uint32_t q = q0; q < q1; q++) {
-      const ExpandNeed n = expand_need<NV>(a, q, level);
-      if (n.active && !n.bad && ot2 + n.table + os2 + n.scratch <= room) {
-        if (n.nx) {
-          nps += n.nx + 1;
-          nxs += n.nx;
-        }
-        if (n.fin) rbs += al(n.rows * n.cols * 4);
-      }
-      ot2 += n.table;
-      os2 += n.scratch;
+  uint64_t carry_t = 0, carry_s = 0, carry_p = 0, carry_x = 0, carry_r = 0, carry_u = 0;
+  for (uint32_t c0 = 0; c0 < nq; c0 += kBlock * kPlanK) {
+    const uint32_t qb = c0 + threadIdx.x * kPlanK;
+    ExpandNeed nd[kPlanK];
+    uint64_t sz[2] = {0, 0}, szt[2];
+#pragma unroll
+    for (uint32_t i = 0; i < kPlanK; i++) {
+      nd[i] = qb + i < nq ? expand_need<NV>(a, qb + i, level) : ExpandNeed{false, false, false, false, 0, 0, 0, 0, 0};
+      sz[0] += nd[i].table;
+      sz[1] += nd[i].scratch;
     }
-  }
-  uint64_t pbase = block_excl_scan(nps, s_tmp, tot);
-  const uint64_t NP = tot;
-  uint64_t xbase = block_excl_scan(nxs, s_tmp, tot);
-  const uint64_t NX = tot;
-  uint64_t orb = block_excl_scan(rbs, s_tmp, tot);
-  // pass 3: host-buffer fit (again a prefix) and copy units
-  uint64_t nus = 0;
-  {
-    uint64_t ot2 = ot, os2 = os, orb2 = orb;
-    for (uint32_t q = q0; q < q1; q++) {
-      const ExpandNeed n = expand_need<NV>(a, q, level);
-      if (n.active && !n.bad && ot2 + n.table + os2 + n.scratch <= room && n.fin) {
-        const uint64_t b = al(n.rows * n.cols * 4);
-        if (rb0 + orb2 + b <= a.hcap) nus += (b + kCopyUnit - 1) / kCopyUnit;
-        orb2 += b;
-      }
-      ot2 += n.table;
-      os2 += n.scratch;
-    }
-  }
-  uint64_t ubase = block_excl_scan(nus, s_tmp, tot);
-  const uint64_t NU = tot;
-  uint64_t used_t = 0, used_s = 0, used_r = 0, over = 0;
-  for (uint32_t q = q0; q < q1; q++) {
-    const ExpandNeed n = expand_need<NV>(a, q, level);
-    a.part_base[q] = pbase;
-    a.xtile_base[q] = xbase;
-    a.unit_base[q] = ubase;
-    if (n.active) {
-      QState &s = a.qs[q];
-      const StepParams &p = a.items[q].p;
-      if (n.bad) {
-        s.status = WKG_ERANGE;
-      } else if (ot + n.table + os + n.scratch > room) {
-        s.status = WKG_ENOMEM;
-      } else {
-        if (n.k2u) {
-          ExpandParams x{};
-          x.pid = p.pid;
-          x.T = p.T;
-          x.c = p.c;
-          x.R = p.totals[2];
-          x.cinfo = p.cinfo;
-          x.cexcl = p.cexcl;
-          x.part = (const uint64_t *)scr_at(a, level, w0, w1, su + os, n.scratch);
-          x.E = n.rows;
-          x.O = (int32_t *)out_at(a, level, w0, w1, ou + ot, n.table);
-          a.items[q].x = x;
-          s.T = x.O;
-          s.expanded += x.E;
+    block_excl_scan_n<2>(sz, s_tmp, szt);
+    const uint64_t ot0 = carry_t + sz[0], os0 = carry_s + sz[1];
+    // arena fit; partition entries, expand tiles, result bytes of finishing queries
+    bool fit[kPlanK];
+    uint64_t cnt[3] = {0, 0, 0}, cntt[3];
+    {
+      uint64_t ot = ot0, os = os0;
+#pragma unroll
+      for (uint32_t i = 0; i < kPlanK; i++) {
+        const ExpandNeed &n = nd[i];
+        fit[i] = n.active && !n.bad && ot + n.table + os + n.scratch <= room;
+        if (fit[i]) {
           if (n.nx) {
-            pbase += n.nx + 1;
-            xbase += n.nx;
+            cnt[0] += n.nx + 1;
+            cnt[1] += n.nx;
           }
-          used_t = ot + n.table;
-          used_s = os + n.scratch;
-        } else {  // filters: the kept rows were compacted into p.O
-          s.T = p.O;
-          s.expanded += p.totals[1];
+          if (n.fin) cnt[2] += al(n.rows * n.cols * 4);
         }
-        s.rows = n.rows;
-        s.cols = (uint32_t)n.cols;
-        if (n.fin) {
+        ot += n.table;
+        os += n.scratch;
+      }
+    }
+    block_excl_scan_n<3>(cnt, s_tmp, cntt);
+    const uint64_t orb0 = carry_r + cnt[2];
+    // host-buffer fit (again a prefix) and copy units
+    uint64_t nu[1] = {0}, nut[1];
+    {
+      uint64_t orb = orb0;
+#pragma unroll
+      for (uint32_t i = 0; i < kPlanK; i++) {
+        const ExpandNeed &n = nd[i];
+        if (fit[i] && n.fin) {
           const uint64_t b = al(n.rows * n.cols * 4);
-          if (rb0 + orb + b <= a.hcap) {
-            s.out_off = (rb0 + orb) / 4;
-            ubase += (b + kCopyUnit - 1) / kCopyUnit;
-            used_r = orb + b;
-          } else {
-            s.status = WKG_ENOMEM;
-            over += b;
-          }
+          if (rb0 + orb + b <= a.hcap) nu[0] += (b + kCopyUnit - 1) / kCopyUnit;
           orb += b;
         }
       }
     }
-    ot += n.table;
-    os += n.scratch;
+    block_excl_scan_n<1>(nu, s_tmp, nut);
+    uint64_t pbase = carry_p + cnt[0], xbase = carry_x + cnt[1], ubase = carry_u + nu[0];
+    uint64_t ot = ot0, os = os0, orb = orb0;
+    uint64_t used_t = 0, used_s = 0, used_r = 0, over = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kPlanK; i++) {
+      const uint32_t q = qb + i;
+      const ExpandNeed &n = nd[i];
+      if (q < nq) {
+        a.part_base[q] = pbase;
+        a.xtile_base[q] = xbase;
+        a.unit_base[q] = ubase;
+      }
+      if (n.active) {
+        QState &s = a.qs[q];
+        const StepParams &p = a.items[q].p;
+        if (n.bad) {
+          s.status = WKG_ERANGE;
+        } else if (!fit[i]) {
+          s.status = WKG_ENOMEM;
+        } else {
+          if (n.k2u) {
+            ExpandParams x{};
+            x.pid = p.pid;
+            x.T = p.T;
+            x.c = p.c;
+            x.R = p.totals[2];
+            x.cinfo = p.cinfo;
+            x.cexcl = p.cexcl;
+            x.part = (const uint64_t *)scr_at(a, level, w0, w1, su + os, n.scratch);
+            x.E = n.rows;
+            x.O = (int32_t *)out_at(a, level, w0, w1, ou + ot, n.table);
+            a.items[q].x = x;
+            s.T = x.O;
+            s.expanded += x.E;
+            if (n.nx) {
+              pbase += n.nx + 1;
+              xbase += n.nx;
+            }
+            used_t = ot + n.table;
+            used_s = os + n.scratch;
+          } else {  // filters: the kept rows were compacted into p.O
+            s.T = p.O;
+            s.expanded += p.totals[1];
+          }
+          s.rows = n.rows;
+          s.cols = (uint32_t)n.cols;
+          if (n.fin) {
+            const uint64_t b = al(n.rows * n.cols * 4);
+            if (rb0 + orb + b <= a.hcap) {
+              s.out_off = (rb0 + orb) / 4;
+              ubase += (b + kCopyUnit - 1) / kCopyUnit;
+              used_r = orb + b;
+            } else {
+              s.status = WKG_ENOMEM;
+              over += b;
+            }
+            orb += b;
+          }
+        }
+      }
+      ot += n.table;
+      os += n.scratch;
+    }
+    if (used_t) atomicMax(&s_ut, (unsigned long long)used_t);
+    if (used_s) atomicMax(&s_us, (unsigned long long)used_s);
+    if (used_r) atomicMax(&s_rb, (unsigned long long)used_r);
+    if (over) atomicAdd(&s_over, (unsigned long long)over);
+    carry_t += szt[0];
+    carry_s += szt[1];
+    carry_p += cntt[0];
+    carry_x += cntt[1];
+    carry_r += cntt[2];
+    carry_u += nut[0];
   }
-  if (used_t) atomicMax(&s_ut, (unsigned long long)used_t);
-  if (used_s) atomicMax(&s_us, (unsigned long long)used_s);
-  if (used_r) atomicMax(&s_rb, (unsigned long long)used_r);
-  if (over) atomicAdd(&s_over, (unsigned long long)over);
   __syncthreads();
   if (threadIdx.x == kBlock - 1) {
-    a.part_base[nq] = NP;
-    a.xtile_base[nq] = NX;
-    a.unit_base[nq] = NU;
-    st->np = NP;
-    st->nx = NX;
-    st->nu = NU;
+    a.part_base[nq] = carry_p;
+    a.xtile_base[nq] = carry_x;
+    a.unit_base[nq] = carry_u;
+    st->np = carry_p;
+    st->nx = carry_x;
+    st->nu = carry_u;
     const uint64_t out = ou + s_ut, scr = su + s_us;
     if (level & 1) st->hi_low = w1 - out;
     else st->lo_top = w0 + out;

@@ -275,6 +275,8 @@ int wkg_load_graph(wkg_ctx *ctx, const uint64_t *slots, uint64_t num_slots, cons
   if (num_slots == 0 || num_slots % kAssoc)
     return ctx->fail(WKG_EINVAL, "num_slots must be a positive multiple of %d", kAssoc);
   if (num_metas > (1 << kIdxBits)) return ctx->fail(WKG_EINVAL, "too many predicate ids");
+  if (num_slots / kAssoc > 0xffffffffull)  // the probe addresses buckets with 32 bits
+    return ctx->fail(WKG_ERANGE, "images are limited to 2^32 - 1 buckets");
   (void)hipSetDevice(ctx->device);
   ctx->num_slots = num_slots;
   ctx->num_edges = num_edges;

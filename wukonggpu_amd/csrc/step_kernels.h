@@ -225,9 +225,9 @@ __device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams 
 }
 
 // At most 168 VGPRs, so three waves per SIMD hide the probe's load chains.
-template <bool SORTED, bool FRAMED>
-__global__ __launch_bounds__(kBlock, 3) void k_probe(GraphView g, StepParams p) {
-  probe_tile<kTileRows, SORTED, FRAMED>(g, p, blockIdx.x);
+template <int TR, bool SORTED, bool FRAMED>
+__global__ __launch_bounds__(kBlock, 4) void k_probe(GraphView g, StepParams p) {
+  probe_tile<TR, SORTED, FRAMED>(g, p, blockIdx.x);
 }
 
 // In-place exclusive scan of a[0, n) by one block; returns the total.
@@ -389,9 +389,10 @@ __device__ __forceinline__ void finalize_tile(const StepParams &p, uint64_t lt) 
 }
 
 // One wave per tile: tile blockIdx.x * 4 + wave.
+template <int TR>
 __global__ __launch_bounds__(kBlock) void k_finalize(StepParams p) {
   const uint64_t t = blockIdx.x * (uint64_t)(kBlock / kWave) + threadIdx.x / kWave;
-  if (t < p.ntiles) finalize_tile<kTileRows>(p, t);
+  if (t < p.ntiles) finalize_tile<TR>(p, t);
 }
 
 // Merge-path partition of the (row starts, output rows) sequence over the

@@ -228,14 +228,44 @@ int wkg_ctx::step_single_list(uint64_t key, int pid, hipStream_t s, wkg_table **
   return WKG_OK;
 }
 
-static void launch_probe(bool framed, bool sorted, const GraphView &g, const StepParams &p, hipStream_t s) {
+// Probe tile rows for an R-row input: the largest tile that still gives
+// every CU at least four tiles, down to one 64-row round per wave. Small
+// inputs otherwise run on a few blocks whose rounds' bucket walks are one
+// long chain of load latencies each.
+static int probe_tile_rows(uint64_t R, int ncu) {
+  for (int tr = kTileRows; tr > kBlock; tr /= 2)
+    if (R >= (uint64_t)tr * 4 * (uint64_t)ncu) return tr;
+  return kBlock;
+}
+
+template <int TR>
+static void launch_probe_tr(bool framed, bool sorted, const GraphView &g, const StepParams &p, hipStream_t s) {
   const unsigned grid = (unsigned)p.ntiles;
   if (framed) {
-    if (sorted) k_probe<true, true><<<grid, kBlock, 0, s>>>(g, p);
-    else k_probe<false, true><<<grid, kBlock, 0, s>>>(g, p);
+    if (sorted) k_probe<TR, true, true><<<grid, kBlock, 0, s>>>(g, p);
+    else k_probe<TR, false, true><<<grid, kBlock, 0, s>>>(g, p);
   } else {
-    if (sorted) k_probe<true, false><<<grid, kBlock, 0, s>>>(g, p);
-    else k_probe<false, false><<<grid, kBlock, 0, s>>>(g, p);
+    if (sorted) k_probe<TR, true, false><<<grid, kBlock, 0, s>>>(g, p);
+    else k_probe<TR, false, false><<<grid, kBlock, 0, s>>>(g, p);
+  }
+}
+
+static void launch_probe(int tr, bool framed, bool sorted, const GraphView &g, const StepParams &p, hipStream_t s) {
+  switch (tr) {
+    case 256: launch_probe_tr<256>(framed, sorted, g, p, s); break;
+    case 512: launch_probe_tr<512>(framed, sorted, g, p, s); break;
+    case 1024: launch_probe_tr<1024>(framed, sorted, g, p, s); break;
+    default: launch_probe_tr<2048>(framed, sorted, g, p, s); break;
+  }
+}
+
+static void launch_finalize(int tr, const StepParams &p, hipStream_t s) {
+  const unsigned grid = (unsigned)((p.ntiles + kBlock / kWave - 1) / (kBlock / kWave));
+  switch (tr) {
+    case 256: k_finalize<256><<<grid, kBlock, 0, s>>>(p); break;
+    case 512: k_finalize<512><<<grid, kBlock, 0, s>>>(p); break;
+    case 1024: k_finalize<1024><<<grid, kBlock, 0, s>>>(p); break;
+    default: k_finalize<2048><<<grid, kBlock, 0, s>>>(p); break;
   }
 }
 
@@ -257,10 +287,11 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
     if (expanded) *expanded = 0;
     return new_table(0, out_cols, s, out);
   }
-  const uint64_t ntiles = (R + kTileRows - 1) / kTileRows;
-  // scratch: totals[4] | tile_sum | tile_nz | tile_ex | hits[ntiles * kTileRows] (16 B) |
+  const int TR = probe_tile ? probe_tile : probe_tile_rows(R, ncu);
+  const uint64_t ntiles = (R + TR - 1) / TR;
+  // scratch: totals[4] | tile_sum | tile_nz | tile_ex | hits[ntiles * TR] (16 B) |
   //          K2U: cinfo[R] (8 B) | cexcl[R]
-  const uint64_t hit_words = 2 * ntiles * kTileRows;
+  const uint64_t hit_words = 2 * ntiles * TR;
   const uint64_t nchunks = (ntiles + kScanChunk - 1) / kScanChunk;
   const uint64_t scratch_words = 4 + 3 * ntiles + 3 * nchunks + hit_words + (mode == kModeK2U ? 2 * R : 0);
   uint64_t *scr = (uint64_t *)pool->alloc(scratch_words * 8, s, err);
@@ -309,7 +340,7 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
     // the filters are charged one bucket per row
     const double b = (double)R * (4.0 + (mode == kModeK2K ? 4.0 : 0.0) + (mode == kModeK2U ? 0.0 : 128.0));
     ProfScope ps(this, s, mode == kModeK2U ? "probe" : "probe_filter", b);
-    launch_probe(framed, sorted || mode == kModeK2U, g, p, s);
+    launch_probe(TR, framed, sorted || mode == kModeK2U, g, p, s);
   }
   {
     ProfScope ps(this, s, "scan_tiles", 48.0 * (double)ntiles);
@@ -318,7 +349,7 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
   }
   {
     ProfScope ps(this, s, mode == kModeK2U ? "finalize" : "compact", 0.0);
-    k_finalize<<<(unsigned)((ntiles + kBlock / kWave - 1) / (kBlock / kWave)), kBlock, 0, s>>>(p);
+    launch_finalize(TR, p, s);
   }
   hipError_t le = hipGetLastError();
   uint64_t cnt[3] = {0, 0, 0};
@@ -584,6 +615,8 @@ int wkg_init(int device, uint64_t table_budget_bytes, wkg_ctx **out) {
   if (!c) return WKG_ENOMEM;
   c->device = device;
   c->budget_request = table_budget_bytes;
+  if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || c->ncu < 1)
+    c->ncu = 256;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc((void **)&c->h_rb, 4096, hipHostMallocDefault) != hipSuccess) {
     delete c;
@@ -651,6 +684,14 @@ const int32_t *wkg_table_device_ptr(const wkg_table *t) { return t ? t->d : null
 int wkg_set_table_limit(wkg_ctx *ctx, uint64_t max_table_bytes) {
   if (!ctx) return WKG_EINVAL;
   ctx->table_limit = max_table_bytes;
+  return WKG_OK;
+}
+
+int wkg_set_probe_tile_rows(wkg_ctx *ctx, int rows) {
+  if (!ctx) return WKG_EINVAL;
+  if (rows != 0 && rows != 256 && rows != 512 && rows != 1024 && rows != 2048)
+    return ctx->fail(WKG_EINVAL, "probe tile rows must be 0, 256, 512, 1024 or 2048 (got %d)", rows);
+  ctx->probe_tile = rows;
   return WKG_OK;
 }
 

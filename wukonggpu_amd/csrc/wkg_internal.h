@@ -67,10 +67,12 @@ struct Residency {
 
 struct wkg_ctx {
   int device = 0;
+  int ncu = 256;  // compute units of the device
   hipStream_t stream = nullptr;
   Pool *pool = nullptr;
   uint64_t budget_request = 0;
   uint64_t table_limit = 0;  // largest step output table in bytes (0: no limit)
+  int probe_tile = 0;        // rows per probe tile of the per-step calls (0: by input size)
   uint64_t *h_rb = nullptr;  // pinned read-back words
   std::string err;
 

@@ -104,6 +104,10 @@ class Device:
         """Largest step output table; bigger ones are produced in row chunks (0: none)."""
         self._check(N.hip_lib().wkg_set_table_limit(self._h, nbytes))
 
+    def set_probe_tile_rows(self, rows):
+        """Rows per probe tile of the per-step path: 256, 512, 1024, 2048, or 0 (by input size)."""
+        self._check(N.hip_lib().wkg_set_probe_tile_rows(self._h, int(rows)))
+
     def random_access_rate(self, nbytes=4 << 30, iters=10):
         """Random 128-byte line reads per second (the hash probe's roofline)."""
         g = C.c_double()
