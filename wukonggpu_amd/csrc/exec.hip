@@ -5,20 +5,24 @@
 // GPU_Engine::execute_one_step dispatches, core/gpu_engine.hpp:395-467). Step
 // level l (the l-th pattern of every query that has one) runs as
 //
-//   k_level_plan     1 block: per query, size the step (rows, tiles, scan
-//                    chunks, scratch, filter output), carve it out of the HBM
-//                    arena, write the step's descriptor and the tile prefix
 //   k_x_probe        persistent: probe tiles of all queries from the work queue
-//   k_x_scan_chunks  persistent: tile sums -> chunk-local prefixes
-//   k_x_scan_top     persistent over queries of more than one chunk
+//   k_x_scan_chunks  persistent: tile sums -> chunk-local prefixes; the block
+//                    that finishes a query's last chunk scans its chunk sums
 //   k_x_finalize     persistent: compaction tiles, one wave per tile
-//   k_expand_plan    1 block: per query, read the totals, allocate the
+//   k_x_plan         1 block: per query, read the totals, allocate the
 //                    known_to_unknown output, commit the query's new table,
-//                    give finishing queries their place in the host buffer
+//                    give finishing queries their place in the host buffer;
+//                    then size level l+1 (rows, tiles, scan chunks, scratch,
+//                    filter output), carve it out of the HBM arena, write its
+//                    step descriptors and tile prefix
 //   k_x_partition    persistent: merge-path split points of all expands
 //   k_x_expand       persistent: expand tiles from a second work queue
 //   k_x_retire       persistent: finished queries' tables -> host buffer
 //
+// after a first k_x_plan that starts the query states and sizes level 0.
+// Level l+1's plan only computes places in the arena; nothing writes them
+// before level l's expand and retire are done (stream order), so planning it
+// early is safe although its window overlaps level l's scratch and inputs.
 // and k_x_records writes the per-query records that are copied back. The
 // per-tile bodies are the plain path's (step_kernels.h), so each query's
 // table is the one the per-step calls produce, cell for cell.
@@ -66,6 +70,7 @@ struct QState {
   int32_t status;     // WKG_OK, a WKG_E* code, or kInactive
   uint32_t nsteps;
   uint32_t nchunks;   // tile-scan chunks of the current step
+  uint32_t chunks_done;  // chunks scanned so far (the last one scans the chunk sums)
 };
 
 struct ExecState {
@@ -193,6 +198,14 @@ struct QueueLds {
   uint32_t need;      // tile t is outside [qlo, qhi)
 };
 
+// A block-uniform LDS value as a scalar (so what it indexes is loaded into
+// scalar registers, not once per lane).
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 struct BlockQueue {
   QueueLds &s;
   unsigned long long *head;
@@ -224,7 +237,7 @@ struct BlockQueue {
       s.need = s.t < n && (s.t < s.qlo || s.t >= s.qhi);
     }
     __syncthreads();
-    t = s.t;
+    t = uniform64(s.t);
     if (t >= n) return false;
     if (s.need) {  // block-uniform: wave 0 finds the tile's query
       if (threadIdx.x < kWave) {
@@ -237,31 +250,11 @@ struct BlockQueue {
       }
       __syncthreads();
     }
-    q = s.q;
-    lt = t - s.qlo;
+    q = (uint32_t)__builtin_amdgcn_readfirstlane((int)s.q);
+    lt = t - uniform64(s.qlo);
     return true;
   }
 };
-
-__global__ __launch_bounds__(kBlock) void k_x_begin(ExecArgs a) {
-  const uint64_t nq = *a.up_nq;
-  for (uint32_t q = threadIdx.x; q < a.Q; q += kBlock) {
-    QState s{};
-    s.T = nullptr;
-    s.status = q < nq ? WKG_OK : kInactive;
-    s.nsteps = q < nq ? a.up_nsteps[q] : 0;
-    a.qs[q] = s;
-  }
-  if (threadIdx.x == 0) {
-    ExecState &st = *a.st;
-    st.nq = nq;
-    st.lo_top = 0;
-    st.hi_low = a.cap;
-    st.w0 = st.w1 = st.out_used = st.scr_used = 0;
-    st.nt = st.nc = st.np = st.nx = st.nu = 0;
-    st.res_bytes = st.res_over = st.peak = 0;
-  }
-}
 
 // ---- level plan ------------------------------------------------------------
 
@@ -292,13 +285,44 @@ __device__ __forceinline__ LevelNeed level_need(const ExecArgs &a, uint32_t q, u
 // chunks follow one another with the running prefixes carried over.
 constexpr uint32_t kPlanK = 4;
 
+// Starts a batch: query states from the upload, the arena stacks empty.
+// Uses the plan phases' query-to-thread mapping (kPlanK consecutive queries
+// per thread), so each thread later reads back only states it wrote itself.
+__device__ __forceinline__ uint32_t begin_phase(const ExecArgs &a) {
+  const uint64_t nq = *a.up_nq;
+  for (uint32_t c0 = 0; c0 < a.Q; c0 += kBlock * kPlanK) {
+#pragma unroll
+    for (uint32_t i = 0; i < kPlanK; i++) {
+      const uint32_t q = c0 + threadIdx.x * kPlanK + i;
+      if (q < a.Q) {
+        QState s{};
+        s.T = nullptr;
+        s.status = q < nq ? WKG_OK : kInactive;
+        s.nsteps = q < nq ? a.up_nsteps[q] : 0;
+        a.qs[q] = s;
+      }
+    }
+  }
+  if (threadIdx.x == 0) {  // (the level plan writes w0, w1, out_used, scr_used, nt and nc)
+    ExecState &st = *a.st;
+    st.nq = nq;
+    st.lo_top = 0;
+    st.hi_low = a.cap;
+    st.np = st.nx = st.nu = 0;
+    st.res_bytes = st.res_over = st.peak = 0;
+  }
+  return (uint32_t)nq;
+}
+
+// Plans level `level` in the window between the stacks: [lo_top, cap) for
+// odd levels (stack 1 is refilled), [0, hi_low) for even ones.
 template <int TR>
-__global__ __launch_bounds__(kBlock) void k_level_plan(ExecArgs a, uint32_t level) {
+__device__ __forceinline__ void level_phase(const ExecArgs &a, uint32_t level, uint32_t nq, uint64_t lo_top,
+                                            uint64_t hi_low) {
   __shared__ uint64_t s_tmp[2 * (kBlock / kWave)];
   __shared__ unsigned long long s_ut, s_us;
   ExecState *st = a.st;
-  const uint32_t nq = (uint32_t)st->nq;
-  const uint64_t w0 = (level & 1) ? st->lo_top : 0, w1 = (level & 1) ? a.cap : st->hi_low;
+  const uint64_t w0 = (level & 1) ? lo_top : 0, w1 = (level & 1) ? a.cap : hi_low;
   const uint64_t room = w1 - w0;
   if (threadIdx.x == 0) {  // ordered before the atomics by the scans' barriers
     s_ut = 0;
@@ -387,6 +411,7 @@ __global__ __launch_bounds__(kBlock) void k_level_plan(ExecArgs a, uint32_t leve
           }
           a.items[q].p = p;
           s.nchunks = (uint32_t)n.nc;
+          s.chunks_done = 0;
           tbase += n.nt;
           cbase += n.nc;
           used_t = ot + n.table;
@@ -432,23 +457,28 @@ __global__ __launch_bounds__(kBlock, 3) void k_x_probe(ExecArgs a) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_x_scan_chunks(ExecArgs a) {
-  __shared__ uint32_t s_q;
+  __shared__ uint32_t s_q, s_last;
   const uint64_t NC = a.st->nc;
   const uint32_t nq = (uint32_t)a.st->nq;
   for (uint64_t u = blockIdx.x; u < NC; u += gridDim.x) {
     __syncthreads();
     const uint32_t q = block_find(a.chunk_base, nq, u, &s_q);
-    scan_chunk(a.items[q].p, u - a.chunk_base[q], a.qs[q].nchunks);
-  }
-}
-
-__global__ __launch_bounds__(kBlock) void k_x_scan_top(ExecArgs a, uint32_t level) {
-  const uint32_t nq = (uint32_t)a.st->nq;
-  for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
-    const QState &s = a.qs[q];
-    if (level_active(s, level) && s.nchunks > 1) {
+    const uint32_t nch = a.qs[q].nchunks;
+    scan_chunk(a.items[q].p, u - a.chunk_base[q], nch);
+    if (nch > 1) {
+      // thread 0 wrote this chunk's sums; the block that counts the query's
+      // last chunk scans all of its chunk sums (release before the count,
+      // acquire after it)
       __syncthreads();
-      scan_top(a.items[q].p, s.nchunks);
+      if (threadIdx.x == 0) {
+        __threadfence();
+        s_last = atomicAdd(&a.qs[q].chunks_done, 1u) + 1 == nch;
+      }
+      __syncthreads();
+      if (s_last) {
+        __threadfence();
+        scan_top(a.items[q].p, nch);
+      }
     }
   }
 }
@@ -505,12 +535,13 @@ __device__ __forceinline__ ExpandNeed expand_need(const ExecArgs &a, uint32_t q,
   return n;
 }
 
+// Commits level `level`'s results; gives the stack tops after it.
 template <int NV>
-__global__ __launch_bounds__(kBlock) void k_expand_plan(ExecArgs a, uint32_t level) {
+__device__ __forceinline__ void expand_phase(const ExecArgs &a, uint32_t level, uint32_t nq, uint64_t &lo_top,
+                                             uint64_t &hi_low) {
   __shared__ uint64_t s_tmp[3 * (kBlock / kWave)];
-  __shared__ unsigned long long s_ut, s_us, s_rb, s_over;
+  __shared__ unsigned long long s_ut, s_us, s_rb, s_over, s_lo, s_hi;
   ExecState *st = a.st;
-  const uint32_t nq = (uint32_t)st->nq;
   const uint64_t w0 = st->w0, w1 = st->w1, ou = st->out_used, su = st->scr_used;
   const uint64_t room = (w1 - w0) - ou - su;
   const uint64_t rb0 = st->res_bytes;
@@ -650,14 +681,36 @@ __global__ __launch_bounds__(kBlock) void k_expand_plan(ExecArgs a, uint32_t lev
     st->nx = carry_x;
     st->nu = carry_u;
     const uint64_t out = ou + s_ut, scr = su + s_us;
-    if (level & 1) st->hi_low = w1 - out;
-    else st->lo_top = w0 + out;
+    uint64_t lo = st->lo_top, hi = st->hi_low;
+    if (level & 1) hi = w1 - out;
+    else lo = w0 + out;
+    st->lo_top = s_lo = lo;
+    st->hi_low = s_hi = hi;
     const uint64_t in_use = a.cap - (w1 - w0) + out + scr;
     if (in_use > st->peak) st->peak = in_use;
     st->res_bytes = rb0 + s_rb;
     st->res_over += s_over;
   }
   if (threadIdx.x < kSubQueues) st->queue[1][threadIdx.x][0] = 0;
+  __syncthreads();
+  lo_top = s_lo;
+  hi_low = s_hi;
+}
+
+// The one-block plan between two levels: level `level`'s expand plan (none
+// for level -1, which starts the batch instead), then the plan of level+1
+// when it exists.
+template <int TR, int NV>
+__global__ __launch_bounds__(kBlock) void k_x_plan(ExecArgs a, int level, int levels) {
+  uint32_t nq;
+  uint64_t lo_top = 0, hi_low = a.cap;
+  if (level < 0) {
+    nq = begin_phase(a);
+  } else {
+    nq = (uint32_t)a.st->nq;
+    expand_phase<NV>(a, (uint32_t)level, nq, lo_top, hi_low);
+  }
+  if (level + 1 < levels) level_phase<TR>(a, (uint32_t)(level + 1), nq, lo_top, hi_low);
 }
 
 template <int NV>
@@ -777,13 +830,12 @@ template <int TR, int VT, bool SORTED, bool FRAMED>
 void enqueue_levels(const ExecArgs &a, hipStream_t s, int levels, unsigned gp, unsigned gx, unsigned ga,
                     unsigned gf) {
   constexpr int NV = kBlock * VT;
+  k_x_plan<TR, NV><<<1, kBlock, 0, s>>>(a, -1, levels);
   for (int l = 0; l < levels; l++) {
-    k_level_plan<TR><<<1, kBlock, 0, s>>>(a, (uint32_t)l);
     k_x_probe<TR, SORTED, FRAMED><<<gp, kBlock, 0, s>>>(a);
     k_x_scan_chunks<<<ga, kBlock, 0, s>>>(a);
-    k_x_scan_top<<<ga, kBlock, 0, s>>>(a, (uint32_t)l);
     k_x_finalize<TR><<<gf, kBlock, 0, s>>>(a);
-    k_expand_plan<NV><<<1, kBlock, 0, s>>>(a, (uint32_t)l);
+    k_x_plan<TR, NV><<<1, kBlock, 0, s>>>(a, l, levels);
     k_x_partition<NV><<<ga, kBlock, 0, s>>>(a);
     k_x_expand<VT, FRAMED><<<gx, kBlock, 0, s>>>(a);
     k_x_retire<<<ga, kBlock, 0, s>>>(a);
@@ -835,7 +887,6 @@ int wkg_batch::enqueue(int levels) {
                  ga = (unsigned)(ncu * t.aux_wg_per_cu), gf = (unsigned)(ncu * 8);
   hipError_t e = hipMemcpyAsync(d_up, h_up, up_bytes, hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return ctx->fail(WKG_EDEVICE, "plan upload failed: %s", hipGetErrorString(e));
-  k_x_begin<<<1, kBlock, 0, s>>>(args);
   const bool sorted = ctx->sorted, framed = ctx->framed;
   if (t.probe_tile_rows == 1024) {
     if (t.expand_tile_items == 512) enqueue_shape<1024, 2>(args, s, levels, sorted, framed, gp, gx, ga, gf);

@@ -335,10 +335,10 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
   p.chunk_ex = chunks + 2 * nchunks;
   const GraphView g = view();
   {
-    // start cell (+ end cell for K2K); K2U adds one 128-byte bucket per key
-    // probed and a 16-byte hit per row with edges once the counts are known;
-    // the filters are charged one bucket per row
-    const double b = (double)R * (4.0 + (mode == kModeK2K ? 4.0 : 0.0) + (mode == kModeK2U ? 0.0 : 128.0));
+    // the input table (4 R c: its rows are read whole, as cache lines); K2U
+    // adds one 128-byte bucket per key probed and a 16-byte hit per row with
+    // edges once the counts are known; the filters are charged one bucket per row
+    const double b = 4.0 * (double)R * c + (mode == kModeK2U ? 0.0 : 128.0 * (double)R);
     ProfScope ps(this, s, mode == kModeK2U ? "probe" : "probe_filter", b);
     launch_probe(TR, framed, sorted || mode == kModeK2U, g, p, s);
   }
