@@ -208,7 +208,8 @@ typedef struct wkg_batch_tuning {
   int32_t aux_wg_per_cu;     /* tile scan, merge-path partition, gather */
   int32_t probe_tile_rows;   /* rows per probe tile: 1024 or 2048 */
   int32_t expand_tile_items; /* merged items per expand tile: 512, 1024 or 2048 */
-  int32_t pad;
+  int32_t schedule;          /* probe and expand tiles: 0 drawn from a shared work
+                                queue, 1 dealt to the workgroups statically */
 } wkg_batch_tuning;
 
 typedef struct wkg_batch_result {

@@ -12,9 +12,11 @@ from wukonggpu_amd import (STEP_C2U, STEP_I2U, STEP_K2C, STEP_K2K, STEP_K2U, Bat
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [dict(probe_tile_rows=tr, expand_tile_items=ni, probe_wg_per_cu=pw, expand_wg_per_cu=xw, aux_wg_per_cu=aw)
-          for tr, ni, pw, xw, aw in ((1024, 1024, 1, 1, 1), (1024, 2048, 2, 4, 2), (2048, 1024, 4, 2, 1),
-                                     (2048, 2048, 2, 2, 1), (1024, 512, 2, 8, 1), (2048, 512, 4, 8, 2))]
+SHAPES = [dict(probe_tile_rows=tr, expand_tile_items=ni, probe_wg_per_cu=pw, expand_wg_per_cu=xw, aux_wg_per_cu=aw,
+               schedule=sc)
+          for tr, ni, pw, xw, aw, sc in ((1024, 1024, 1, 1, 1, 0), (1024, 2048, 2, 4, 2, 0), (2048, 1024, 4, 2, 1, 0),
+                                         (2048, 2048, 2, 2, 1, 0), (1024, 512, 2, 8, 1, 0), (2048, 512, 4, 8, 2, 0),
+                                         (1024, 512, 3, 4, 1, 1), (2048, 1024, 2, 8, 2, 1))]
 
 
 @pytest.fixture(scope="module")
@@ -43,7 +45,8 @@ def test_executor_random_queries_match_oracle(eng_small, graph_small):
         _same(gs, q, r)
 
 
-@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"tr{s['probe_tile_rows']}-nv{s['expand_tile_items']}")
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"tr{s['probe_tile_rows']}-nv{s['expand_tile_items']}"
+                                                        f"-{'static' if s['schedule'] else 'queue'}")
 def test_executor_shapes_match_plain(dev_small, graph_small, shape):
     _, gs = graph_small
     rng = np.random.default_rng(22)

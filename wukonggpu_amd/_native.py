@@ -46,10 +46,10 @@ class PlanStep(C.Structure):
 
 class BatchTuning(C.Structure):
     _fields_ = [("probe_wg_per_cu", i32), ("expand_wg_per_cu", i32), ("aux_wg_per_cu", i32),
-                ("probe_tile_rows", i32), ("expand_tile_items", i32), ("pad", i32)]
+                ("probe_tile_rows", i32), ("expand_tile_items", i32), ("schedule", i32)]
 
     def as_dict(self):
-        return {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "pad"}
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
 class BatchResult(C.Structure):

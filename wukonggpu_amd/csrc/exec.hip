@@ -256,6 +256,30 @@ struct BlockQueue {
   }
 };
 
+// The static deal: block b takes tiles b, b + G, b + 2G, ... of a pass's
+// flat tile space (G blocks). No atomics, and blocks differ by at most one
+// tile, which suits passes whose tiles cost about the same (large steps).
+struct StaticDeal {
+  uint32_t &sq;  // LDS slot of the query search
+  uint64_t n, t, qlo, qhi;
+  uint32_t q;
+  __device__ StaticDeal(uint32_t &lds, uint64_t total) : sq(lds), n(total), t(blockIdx.x), qlo(1), qhi(0), q(0) {}
+  __device__ bool next(const uint64_t *base, uint32_t nq, uint64_t &tt, uint32_t &qq, uint64_t &lt) {
+    __syncthreads();  // the previous tile's LDS use is over
+    if (t >= n) return false;
+    if (t < qlo || t >= qhi) {  // block-uniform
+      q = (uint32_t)__builtin_amdgcn_readfirstlane((int)block_find(base, nq, t, &sq));
+      qlo = base[q];
+      qhi = base[q + 1];
+    }
+    tt = t;
+    qq = q;
+    lt = t - qlo;
+    t += gridDim.x;
+    return true;
+  }
+};
+
 // ---- level plan ------------------------------------------------------------
 
 struct LevelNeed {
@@ -446,14 +470,20 @@ __device__ __forceinline__ void level_phase(const ExecArgs &a, uint32_t level, u
 
 // ---- probe, tile scan, finalize ----------------------------------------------
 
-template <int TR, bool SORTED, bool FRAMED>
+template <int TR, bool SORTED, bool FRAMED, bool QUEUE>
 __global__ __launch_bounds__(kBlock, 3) void k_x_probe(ExecArgs a) {
-  __shared__ QueueLds s_wq;
-  BlockQueue wq(s_wq, a.st->queue[0], a.st->nt);
   const uint32_t nq = (uint32_t)a.st->nq;
   uint64_t t, lt;
   uint32_t q;
-  while (wq.next(a.tile_base, nq, t, q, lt)) probe_tile<TR, SORTED, FRAMED>(a.g, a.items[q].p, lt);
+  if constexpr (QUEUE) {
+    __shared__ QueueLds s_wq;
+    BlockQueue wq(s_wq, a.st->queue[0], a.st->nt);
+    while (wq.next(a.tile_base, nq, t, q, lt)) probe_tile<TR, SORTED, FRAMED>(a.g, a.items[q].p, lt);
+  } else {
+    __shared__ uint32_t s_q;
+    StaticDeal wd(s_q, a.st->nt);
+    while (wd.next(a.tile_base, nq, t, q, lt)) probe_tile<TR, SORTED, FRAMED>(a.g, a.items[q].p, lt);
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_x_scan_chunks(ExecArgs a) {
@@ -725,14 +755,20 @@ __global__ __launch_bounds__(kBlock) void k_x_partition(ExecArgs a) {
   }
 }
 
-template <int VT, bool FRAMED>
+template <int VT, bool FRAMED, bool QUEUE>
 __global__ __launch_bounds__(kBlock) void k_x_expand(ExecArgs a) {
-  __shared__ QueueLds s_wq;
-  BlockQueue wq(s_wq, a.st->queue[1], a.st->nx);
   const uint32_t nq = (uint32_t)a.st->nq;
   uint64_t t, lt;
   uint32_t q;
-  while (wq.next(a.xtile_base, nq, t, q, lt)) expand_tile<VT, FRAMED>(a.g, a.items[q].x, lt);
+  if constexpr (QUEUE) {
+    __shared__ QueueLds s_wq;
+    BlockQueue wq(s_wq, a.st->queue[1], a.st->nx);
+    while (wq.next(a.xtile_base, nq, t, q, lt)) expand_tile<VT, FRAMED>(a.g, a.items[q].x, lt);
+  } else {
+    __shared__ uint32_t s_q;
+    StaticDeal wd(s_q, a.st->nx);
+    while (wd.next(a.xtile_base, nq, t, q, lt)) expand_tile<VT, FRAMED>(a.g, a.items[q].x, lt);
+  }
 }
 
 // Copies the tables of queries that finished at this level into the host
@@ -822,36 +858,43 @@ wkg_batch_tuning normalized(const wkg_batch_tuning &t) {
   if (r.aux_wg_per_cu <= 0) r.aux_wg_per_cu = 1;
   if (r.probe_tile_rows != 1024) r.probe_tile_rows = 2048;
   if (r.expand_tile_items != 1024 && r.expand_tile_items != 2048) r.expand_tile_items = 512;
-  r.pad = 0;
+  r.schedule = r.schedule == 1 ? 1 : 0;
   return r;
 }
 
-template <int TR, int VT, bool SORTED, bool FRAMED>
+template <int TR, int VT, bool SORTED, bool FRAMED, bool QUEUE>
 void enqueue_levels(const ExecArgs &a, hipStream_t s, int levels, unsigned gp, unsigned gx, unsigned ga,
                     unsigned gf) {
   constexpr int NV = kBlock * VT;
   k_x_plan<TR, NV><<<1, kBlock, 0, s>>>(a, -1, levels);
   for (int l = 0; l < levels; l++) {
-    k_x_probe<TR, SORTED, FRAMED><<<gp, kBlock, 0, s>>>(a);
+    k_x_probe<TR, SORTED, FRAMED, QUEUE><<<gp, kBlock, 0, s>>>(a);
     k_x_scan_chunks<<<ga, kBlock, 0, s>>>(a);
     k_x_finalize<TR><<<gf, kBlock, 0, s>>>(a);
     k_x_plan<TR, NV><<<1, kBlock, 0, s>>>(a, l, levels);
     k_x_partition<NV><<<ga, kBlock, 0, s>>>(a);
-    k_x_expand<VT, FRAMED><<<gx, kBlock, 0, s>>>(a);
+    k_x_expand<VT, FRAMED, QUEUE><<<gx, kBlock, 0, s>>>(a);
     k_x_retire<<<ga, kBlock, 0, s>>>(a);
   }
 }
 
-template <int TR, int VT>
-void enqueue_shape(const ExecArgs &a, hipStream_t s, int levels, bool sorted, bool framed, unsigned gp, unsigned gx,
+template <int TR, int VT, bool QUEUE>
+void enqueue_sched(const ExecArgs &a, hipStream_t s, int levels, bool sorted, bool framed, unsigned gp, unsigned gx,
                    unsigned ga, unsigned gf) {
   if (framed) {
-    if (sorted) enqueue_levels<TR, VT, true, true>(a, s, levels, gp, gx, ga, gf);
-    else enqueue_levels<TR, VT, false, true>(a, s, levels, gp, gx, ga, gf);
+    if (sorted) enqueue_levels<TR, VT, true, true, QUEUE>(a, s, levels, gp, gx, ga, gf);
+    else enqueue_levels<TR, VT, false, true, QUEUE>(a, s, levels, gp, gx, ga, gf);
   } else {
-    if (sorted) enqueue_levels<TR, VT, true, false>(a, s, levels, gp, gx, ga, gf);
-    else enqueue_levels<TR, VT, false, false>(a, s, levels, gp, gx, ga, gf);
+    if (sorted) enqueue_levels<TR, VT, true, false, QUEUE>(a, s, levels, gp, gx, ga, gf);
+    else enqueue_levels<TR, VT, false, false, QUEUE>(a, s, levels, gp, gx, ga, gf);
   }
+}
+
+template <int TR, int VT>
+void enqueue_shape(const ExecArgs &a, hipStream_t s, int levels, bool sorted, bool framed, bool queue, unsigned gp,
+                   unsigned gx, unsigned ga, unsigned gf) {
+  if (queue) enqueue_sched<TR, VT, true>(a, s, levels, sorted, framed, gp, gx, ga, gf);
+  else enqueue_sched<TR, VT, false>(a, s, levels, sorted, framed, gp, gx, ga, gf);
 }
 
 }  // namespace
@@ -887,15 +930,15 @@ int wkg_batch::enqueue(int levels) {
                  ga = (unsigned)(ncu * t.aux_wg_per_cu), gf = (unsigned)(ncu * 8);
   hipError_t e = hipMemcpyAsync(d_up, h_up, up_bytes, hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return ctx->fail(WKG_EDEVICE, "plan upload failed: %s", hipGetErrorString(e));
-  const bool sorted = ctx->sorted, framed = ctx->framed;
+  const bool sorted = ctx->sorted, framed = ctx->framed, queue = t.schedule == 0;
   if (t.probe_tile_rows == 1024) {
-    if (t.expand_tile_items == 512) enqueue_shape<1024, 2>(args, s, levels, sorted, framed, gp, gx, ga, gf);
-    else if (t.expand_tile_items == 1024) enqueue_shape<1024, 4>(args, s, levels, sorted, framed, gp, gx, ga, gf);
-    else enqueue_shape<1024, 8>(args, s, levels, sorted, framed, gp, gx, ga, gf);
+    if (t.expand_tile_items == 512) enqueue_shape<1024, 2>(args, s, levels, sorted, framed, queue, gp, gx, ga, gf);
+    else if (t.expand_tile_items == 1024) enqueue_shape<1024, 4>(args, s, levels, sorted, framed, queue, gp, gx, ga, gf);
+    else enqueue_shape<1024, 8>(args, s, levels, sorted, framed, queue, gp, gx, ga, gf);
   } else {
-    if (t.expand_tile_items == 512) enqueue_shape<2048, 2>(args, s, levels, sorted, framed, gp, gx, ga, gf);
-    else if (t.expand_tile_items == 1024) enqueue_shape<2048, 4>(args, s, levels, sorted, framed, gp, gx, ga, gf);
-    else enqueue_shape<2048, 8>(args, s, levels, sorted, framed, gp, gx, ga, gf);
+    if (t.expand_tile_items == 512) enqueue_shape<2048, 2>(args, s, levels, sorted, framed, queue, gp, gx, ga, gf);
+    else if (t.expand_tile_items == 1024) enqueue_shape<2048, 4>(args, s, levels, sorted, framed, queue, gp, gx, ga, gf);
+    else enqueue_shape<2048, 8>(args, s, levels, sorted, framed, queue, gp, gx, ga, gf);
   }
   k_x_records<<<1, kBlock, 0, s>>>(args);
   e = hipGetLastError();
@@ -1171,9 +1214,16 @@ int wkg_batch_autotune(wkg_batch *b, int nq, const wkg_plan_step *steps, const i
   std::vector<wkg_batch_tuning> cand;
   for (int tr : {1024, 2048})
     for (int xi : {512, 1024, 2048})
-      for (int pw : {1, 2, 4})
-        for (int xw : {2, 4, 8})
-          for (int aw : {1, 2}) cand.push_back(wkg_batch_tuning{pw, xw, aw, tr, xi, 0});
+      for (int aw : {1, 2}) {
+        // work queue: more workgroups than fit at once are fine (late ones
+        // find the queue drained); static deal: at most what fits (three
+        // probe workgroups per CU), or the late ones would run their share
+        // after the others are done
+        for (int pw : {1, 2, 4})
+          for (int xw : {2, 4, 8}) cand.push_back(wkg_batch_tuning{pw, xw, aw, tr, xi, 0});
+        for (int pw : {2, 3})
+          for (int xw : {4, 8}) cand.push_back(wkg_batch_tuning{pw, xw, aw, tr, xi, 1});
+      }
   double best_t = 1e30;
   wkg_batch_tuning best_c = normalized(b->tune);
   for (const wkg_batch_tuning &c : cand) {
