@@ -111,6 +111,14 @@ int wkg_set_table_limit(wkg_ctx *ctx, uint64_t max_table_bytes);
  * least four tiles per compute unit). A tuning knob with no counterpart in
  * the reference; results do not depend on it. */
 int wkg_set_probe_tile_rows(wkg_ctx *ctx, int rows);
+/* A known_to_unknown step that fans out (at least 4 output rows per input
+ * row with edges) keeps, with its output table, which input row each output
+ * row repeats. A later known_to_unknown step whose start column is one of
+ * those inherited columns (a star's next arm from the same variable) then
+ * probes once per input row instead of once per output row; its output is
+ * the same, row for row. On by default; 0 turns it off. No counterpart in the
+ * reference, which probes every row (core/gpu_hash.cu:145-316). */
+int wkg_set_lineage(wkg_ctx *ctx, int on);
 /* A view of rows [row0, row0 + nrows) of t, sharing its memory (for running
  * a step over a table in row chunks). Free the view before t. */
 int wkg_table_slice(wkg_ctx *ctx, const wkg_table *t, uint64_t row0, uint64_t nrows, wkg_table **out);

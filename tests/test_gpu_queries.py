@@ -164,6 +164,35 @@ def test_probe_tile_sizes_match_oracle(graph_hubs):
     dev.close()
 
 
+def test_lineage_star_arms_match_oracle(graph_hubs):
+    """A star's next arm after a fan-out probes once per input row of the
+    fan-out (the output keeps which input row each row repeats): the answers
+    equal the oracle's with it on and off, and it is taken."""
+    _, gs = graph_hubs
+    dev = Device(0)
+    dev.load(gs)
+    eng = GPUEngine(dev)
+    qs = [[2, 1, 0, -1, -1, 2, 1, -2, -1, 3, 1, -3],                      # star from ?x
+          [2, 1, 0, -1, -1, 2, 1, -2, -2, 3, 0, -3, -3, 2, 1, -4, -3, 3, 1, -5],  # heavy-query shape
+          [2, 1, 0, -1, -1, 2, 1, -2, -1, 3, 1, -3, -1, 4, 1, -4],        # three arms
+          [VB + 0, 2, 1, -1, -1, 3, 0, -2, -1, 3, 1, -3]]                  # hub constant start
+    rng = np.random.default_rng(91)
+    qs += random_queries(rng, 6000, 3, [2, 3], 80, max_steps=5)
+    dev.profile(True)
+    print("lineage on", flush=True)
+    for q in qs:
+        _check(gs, q, eng.run_query(q))
+    n_lineage = dev.profile_read("lineage_steps")[2]
+    dev.profile(False)
+    assert n_lineage > 0
+    dev.set_lineage(False)
+    print("lineage off", flush=True)
+    for q in qs:
+        _check(gs, q, eng.run_query(q))
+    eng.close()
+    dev.close()
+
+
 @pytest.fixture(scope="module")
 def graph_mid():
     from tests.conftest import small_graph

@@ -128,6 +128,7 @@ def hip_lib():
         _sig(L, "wkg_table_slice", C.c_int, vp, vp, u64, u64, P(vp))
         _sig(L, "wkg_set_table_limit", C.c_int, vp, u64)
         _sig(L, "wkg_set_probe_tile_rows", C.c_int, vp, C.c_int)
+        _sig(L, "wkg_set_lineage", C.c_int, vp, C.c_int)
         _sig(L, "wkg_step_i2u", C.c_int, vp, C.c_int, C.c_int, vp, P(vp), P(u64))
         _sig(L, "wkg_step_c2u", C.c_int, vp, i64, C.c_int, C.c_int, vp, P(vp), P(u64))
         _sig(L, "wkg_step_k2u", C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, vp, P(vp), P(u64))

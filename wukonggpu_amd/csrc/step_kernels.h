@@ -568,5 +568,66 @@ __global__ __launch_bounds__(kBlock) void k_copy_list(GraphView g, uint32_t pid,
     out[j] = (int32_t)edge_at<FRAMED>(g, m.edge_start, m.eframe_base, off + j);
 }
 
+// ---- known_to_unknown on an inherited column (probe once per parent row) ----
+// A known_to_unknown output repeats compacted input row i (parent row
+// info_p[i].x) in its rows [excl_p[i], excl_p[i+1]). A later step whose key
+// column is one of the parent's gathers that key once per compacted parent
+// row (k_gather_col), probes those keys as a one-column table, and spreads
+// each kept parent row (m, with d list entries) back to its f child rows:
+// child k of it gets output rows [out_pre[m] + k d, out_pre[m] + (k+1) d).
+
+// K[i] = T[info[i].x * c + sc]
+__global__ __launch_bounds__(kBlock) void k_gather_col(const int32_t *T, uint32_t c, uint32_t sc, const uint2 *info,
+                                                       uint64_t n, int32_t *K) {
+  for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+    K[i] = T[(uint64_t)info[i].x * c + sc];
+}
+
+// The kept parent rows: m < nk is key-table row info2[m].x (a compacted
+// parent row i) with list length d = excl2[m+1] - excl2[m] (the last: dtot -
+// excl2[m]); its child rows number f = excl_p[i+1] - excl_p[i] (the last
+// compacted row's: rows - excl_p[i]).
+struct KeptRows {
+  const uint2 *info2;
+  const uint64_t *excl2;
+  uint64_t nk, dtot;
+  const uint64_t *excl_p;
+  uint64_t np, rows;
+  __device__ __forceinline__ uint64_t deg(uint64_t m) const { return (m + 1 < nk ? excl2[m + 1] : dtot) - excl2[m]; }
+  __device__ __forceinline__ uint64_t fan(uint32_t i) const {
+    return (i + 1 < np ? excl_p[i + 1] : rows) - excl_p[i];
+  }
+};
+
+// cnt[m] = f, out[m] = f d
+__global__ __launch_bounds__(kBlock) void k_lift(KeptRows k, uint64_t *cnt, uint64_t *out) {
+  for (uint64_t m = blockIdx.x * (uint64_t)kBlock + threadIdx.x; m < k.nk; m += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t f = k.fan(k.info2[m].x);
+    cnt[m] = f;
+    out[m] = f * k.deg(m);
+  }
+}
+
+// Kept child row j (of kc): its parent m is the last with cnt_pre(m) <= j
+// (cnt_pre strictly ascends: every compacted parent row has a child). The
+// prefixes are the two-level tile scan's: chunk-local value + chunk prefix.
+__global__ __launch_bounds__(kBlock) void k_spread(KeptRows k, const uint64_t *cnt_loc, const uint64_t *cnt_chunk,
+                                                   const uint64_t *out_loc, const uint64_t *out_chunk, uint64_t kc,
+                                                   uint2 *cinfo, uint64_t *cexcl) {
+  auto cnt_pre = [&](uint64_t m) { return cnt_loc[m] + cnt_chunk[m / kScanChunk]; };
+  for (uint64_t j = blockIdx.x * (uint64_t)kBlock + threadIdx.x; j < kc; j += (uint64_t)gridDim.x * kBlock) {
+    uint64_t lo = 0, hi = k.nk;  // cnt_pre(lo) <= j < cnt_pre(hi)
+    while (hi - lo > 1) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (cnt_pre(mid) <= j) lo = mid;
+      else hi = mid;
+    }
+    const uint64_t kk = j - cnt_pre(lo);
+    const uint2 pi = k.info2[lo];
+    cinfo[j] = make_uint2((uint32_t)(k.excl_p[pi.x] + kk), pi.y);
+    cexcl[j] = out_loc[lo] + out_chunk[lo / kScanChunk] + kk * k.deg(lo);
+  }
+}
+
 }  // namespace
 }  // namespace wkg

@@ -173,8 +173,23 @@ int wkg_ctx::new_table(uint64_t rows, int cols, hipStream_t s, wkg_table **out) 
   return WKG_OK;
 }
 
+// The caller's handle goes away: the table's lineage is released at once (it
+// only serves steps that read this table), its cells once no child table's
+// lineage reads them any more.
 void wkg_ctx::drop_table(wkg_table *t) {
   if (!t) return;
+  if (t->lin_parent) {
+    pool->release(t->lin_alloc, t->stream);
+    wkg_table *par = t->lin_parent;
+    t->lin_parent = nullptr;
+    t->lin_alloc = nullptr;
+    unref_table(par);
+  }
+  unref_table(t);
+}
+
+void wkg_ctx::unref_table(wkg_table *t) {
+  if (--t->refs > 0) return;
   if (t->owned) pool->release(t->d, t->stream);
   delete t;
 }
@@ -269,46 +284,33 @@ static void launch_finalize(int tr, const StepParams &p, hipStream_t s) {
   }
 }
 
-int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval,
-                        hipStream_t s, wkg_table **out, uint64_t *expanded) {
-  int rc = check_pattern(pid, dir);
-  if (rc) return rc;
-  if (!in) return fail(WKG_EINVAL, "null input table");
+// Probe, tile scan and compaction of one step over `in` (the first half of
+// every known-start step): for K2U the compacted rows with edges (cinfo) and
+// their output offsets (cexcl) in pp.lin; the filters compact into O.
+int wkg_ctx::probe_pass(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval, int32_t *O,
+                        hipStream_t s, ProbePass &pp) {
   const uint64_t R = in->rows;
   const int c = in->cols;
-  if (c < 1 || sc < 0 || sc >= c) return fail(WKG_EINVAL, "start column %d outside the %d-column table", sc, c);
-  if (mode == kModeK2K && (ec < 0 || ec >= c))
-    return fail(WKG_EINVAL, "end column %d outside the %d-column table", ec, c);
-  const int out_cols = mode == kModeK2U ? c + 1 : c;
-  if (out_cols > kBlock) return fail(WKG_ERANGE, "tables are limited to %d columns", kBlock);
-  if (R >= (1ull << 32)) return fail(WKG_ERANGE, "tables are limited to 2^32 - 1 rows per step");
-  if ((rc = ensure_pred(pid, s))) return rc;
-  if (R == 0) {  // empty input gives an empty result (core/rcache.hpp:242-250)
-    if (expanded) *expanded = 0;
-    return new_table(0, out_cols, s, out);
-  }
   const int TR = probe_tile ? probe_tile : probe_tile_rows(R, ncu);
   const uint64_t ntiles = (R + TR - 1) / TR;
-  // scratch: totals[4] | tile_sum | tile_nz | tile_ex | hits[ntiles * TR] (16 B) |
-  //          K2U: cinfo[R] (8 B) | cexcl[R]
+  // scratch: totals[4] | tile_sum | tile_nz | tile_ex | chunk sums x 3 | hits[ntiles * TR] (16 B)
   const uint64_t hit_words = 2 * ntiles * TR;
   const uint64_t nchunks = (ntiles + kScanChunk - 1) / kScanChunk;
-  const uint64_t scratch_words = 4 + 3 * ntiles + 3 * nchunks + hit_words + (mode == kModeK2U ? 2 * R : 0);
-  uint64_t *scr = (uint64_t *)pool->alloc(scratch_words * 8, s, err);
-  if (!scr) return WKG_ENOMEM;
+  const uint64_t scratch_words = 4 + 3 * ntiles + 3 * nchunks + hit_words;
+  pp.scr = (uint64_t *)pool->alloc(scratch_words * 8, s, err);
+  if (!pp.scr) return WKG_ENOMEM;
+  if (mode == kModeK2U) {  // cinfo[R] (8 B) | cexcl[R]: may outlive the step as the output's lineage
+    pp.lin = pool->alloc(R * 16, s, err);
+    if (!pp.lin) {
+      release_pass(pp, s);
+      return WKG_ENOMEM;
+    }
+    pp.cinfo = (uint2 *)pp.lin;
+    pp.cexcl = (uint64_t *)((char *)pp.lin + R * 8);
+  }
+  uint64_t *scr = pp.scr;
   uint64_t *totals = scr, *tile_sum = scr + 4, *tile_nz = tile_sum + ntiles, *tile_ex = tile_nz + ntiles;
   uint64_t *chunks = tile_ex + ntiles;
-  uint4 *hits = (uint4 *)(chunks + 3 * nchunks);
-  uint2 *cinfo = (uint2 *)((uint64_t *)hits + hit_words);
-  uint64_t *cexcl = (uint64_t *)(cinfo + R);
-
-  wkg_table *t = nullptr;
-  if (mode != kModeK2U) {
-    if ((rc = new_table(R, c, s, &t))) {
-      pool->release(scr, s);
-      return rc;
-    }
-  }
   StepParams p{};
   p.T = in->d;
   p.R = R;
@@ -321,10 +323,10 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
   p.target_ok = (cval >= 0 && cval <= 0xffffffffLL) ? 1u : 0u;
   p.mode = mode;
   p.key = 0;
-  p.hits = hits;
-  p.cinfo = cinfo;
-  p.cexcl = cexcl;
-  p.O = t ? t->d : nullptr;
+  p.hits = (uint4 *)(chunks + 3 * nchunks);
+  p.cinfo = pp.cinfo;
+  p.cexcl = pp.cexcl;
+  p.O = O;
   p.tile_sum = tile_sum;
   p.tile_nz = tile_nz;
   p.tile_ex = tile_ex;
@@ -352,76 +354,243 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
     launch_finalize(TR, p, s);
   }
   hipError_t le = hipGetLastError();
-  uint64_t cnt[3] = {0, 0, 0};
-  if (le == hipSuccess) rc = read_counters(totals, 3, s, cnt);
+  int rc = le == hipSuccess ? read_counters(totals, 3, s, pp.cnt) : WKG_OK;
   if (le != hipSuccess || rc) {
-    pool->release(scr, s);
-    drop_table(t);
+    release_pass(pp, s);
     return le != hipSuccess ? fail(WKG_EDEVICE, "probe launch failed: %s", hipGetErrorString(le)) : rc;
   }
-  const uint64_t nz = cnt[2];
-  if (mode != kModeK2U) {  // filters: kept rows compacted in place
-    prof_add_bytes("compact", 16.0 * nz + 8.0 * nz * c);
-    pool->release(scr, s);
-    t->rows = cnt[0];
-    t->bytes = t->rows * (uint64_t)c * 4;
-    if (expanded) *expanded = cnt[1];
-    *out = t;
-    return WKG_OK;
+  if (mode == kModeK2U) {
+    prof_add_bytes("finalize", 32.0 * pp.cnt[2]);
+    prof_add_bytes("probe", 128.0 * (double)pp.cnt[1] + 16.0 * (double)pp.cnt[2]);
+    prof_add_bytes("probe_keys", (double)pp.cnt[1]);  // counts, not bytes: keys probed (run leaders)
+    prof_add_bytes("probe_rows", (double)R);           // and rows looked up
+  } else {
+    prof_add_bytes("compact", 16.0 * pp.cnt[2] + 8.0 * pp.cnt[2] * c);
   }
-  if (table_limit && R >= 2 && cnt[0] * (uint64_t)out_cols * 4 > table_limit) {  // (one row cannot be split)
-    pool->release(scr, s);
-    return fail(WKG_ENOMEM, "a %llu-row output is above the table limit", (unsigned long long)cnt[0]);
+  return WKG_OK;
+}
+
+void wkg_ctx::release_pass(ProbePass &pp, hipStream_t s) {
+  if (pp.scr) pool->release(pp.scr, s);
+  if (pp.lin) pool->release(pp.lin, s);
+  pp = ProbePass{};
+}
+
+// Merge-path expand of a known_to_unknown step: output row o of compacted
+// row i is (input row cinfo[i].x, edge o - cexcl[i]).
+int wkg_ctx::expand_pass(const wkg_table *in, int pid, const uint2 *cinfo, const uint64_t *cexcl, uint64_t nz,
+                         uint64_t E, hipStream_t s, wkg_table *t) {
+  if (!E) return WKG_OK;
+  const int c = in->cols;
+  const uint64_t nx = (nz + E + kExpandNV - 1) / kExpandNV;
+  uint64_t *part = (uint64_t *)pool->alloc((nx + 1) * 8, s, err);
+  if (!part) return WKG_ENOMEM;
+  {
+    ProfScope ps(this, s, "partition", 8.0 * (double)(nx + 1));
+    k_partition<<<(unsigned)((nx + 1 + kBlock - 1) / kBlock), kBlock, 0, s>>>(cexcl, nz, E, nx, part);
   }
-  prof_add_bytes("finalize", 32.0 * nz);
-  prof_add_bytes("probe", 128.0 * (double)cnt[1] + 16.0 * (double)nz);
-  prof_add_bytes("probe_keys", (double)cnt[1]);  // counts, not bytes: keys probed (run leaders)
-  prof_add_bytes("probe_rows", (double)R);        // and rows looked up
-  const uint64_t E = cnt[0];
-  if (expanded) *expanded = E;
-  if ((rc = new_table(E, out_cols, s, &t))) {
-    pool->release(scr, s);
+  ExpandParams x{};
+  x.pid = (uint32_t)pid;
+  x.T = in->d;
+  x.c = (uint32_t)c;
+  x.R = nz;
+  x.cinfo = cinfo;
+  x.cexcl = cexcl;
+  x.part = part;
+  x.E = E;
+  x.O = t->d;
+  {
+    // edges gathered (4E) + output rows written (4E(c+1)) + the input rows
+    // with edges and their offsets read once (4 nz c + 16 nz)
+    const double b = 4.0 * E + 4.0 * E * (c + 1) + 4.0 * nz * c + 16.0 * nz;
+    ProfScope ps(this, s, "expand", b);
+    const GraphView g = view();
+    if (framed) k_expand<true><<<(unsigned)nx, kBlock, 0, s>>>(g, x);
+    else k_expand<false><<<(unsigned)nx, kBlock, 0, s>>>(g, x);
+  }
+  hipError_t e2 = hipGetLastError();
+  pool->release(part, s);
+  if (e2 != hipSuccess) return fail(WKG_EDEVICE, "expand launch failed: %s", hipGetErrorString(e2));
+  return WKG_OK;
+}
+
+// Keeps (cinfo, cexcl) of a known_to_unknown step with its output `t` when
+// the step fanned out (at least 4 output rows per input row with edges) and
+// both the input and the arrays are small next to the budget: a later
+// known_to_unknown step on one of the input's columns then probes once per
+// compacted input row (step_k2u_lineage).
+void wkg_ctx::keep_lineage(const wkg_table *in, ProbePass &pp, uint64_t nz, wkg_table *t) {
+  if (!lineage || !pp.lin || !in->owned || nz == 0 || t->rows < 4 * nz) return;
+  if (in->bytes + in->rows * 16 > pool->budget / 16) return;
+  wkg_table *par = const_cast<wkg_table *>(in);
+  par->refs++;
+  t->lin_parent = par;
+  t->lin_info = pp.cinfo;
+  t->lin_excl = pp.cexcl;
+  t->lin_n = nz;
+  t->lin_alloc = pp.lin;
+  pp.lin = nullptr;
+}
+
+// known_to_unknown on column sc of a table whose rows are its parent's
+// compacted rows fanned out (t->lin_*), with sc one of the parent's columns:
+// all rows from one parent row share the key, so the keys are gathered
+// once per compacted parent row and probed as a one-column table. Kept
+// parent rows are then spread back to their child rows in order, each with
+// the parent's list, which gives the per-row step's output row for row.
+int wkg_ctx::step_k2u_lineage(const wkg_table *in, int sc, int pid, int dir, hipStream_t s, wkg_table **out,
+                              uint64_t *expanded) {
+  const wkg_table *par = in->lin_parent;
+  const uint64_t np = in->lin_n;
+  const int out_cols = in->cols + 1;
+  wkg_table *keys = nullptr;
+  int rc = new_table(np, 1, s, &keys);
+  if (rc) return rc;
+  {
+    ProfScope ps(this, s, "lineage", 12.0 * (double)np);
+    k_gather_col<<<(unsigned)std::min<uint64_t>((np + kBlock - 1) / kBlock, 65535), kBlock, 0, s>>>(
+        par->d, (uint32_t)par->cols, (uint32_t)sc, in->lin_info, np, keys->d);
+  }
+  ProbePass pp;
+  rc = probe_pass(kModeK2U, keys, 0, pid, dir, -1, -1, nullptr, s, pp);
+  if (rc) {
+    drop_table(keys);
     return rc;
   }
-  if (E) {
-    const uint64_t nx = (nz + E + kExpandNV - 1) / kExpandNV;
-    uint64_t *part = (uint64_t *)pool->alloc((nx + 1) * 8, s, err);
-    if (!part) {
-      pool->release(scr, s);
-      drop_table(t);
+  prof_add_bytes("lineage_steps", 1.0);  // a count
+  const uint64_t nk = pp.cnt[2], dtot = pp.cnt[0];  // kept parent rows, their list lengths summed
+  uint64_t kc = 0, E = 0;
+  uint64_t *lift = nullptr;
+  void *lin = nullptr;
+  if (nk) {
+    // per kept parent row: child rows and output rows, scanned like tile sums
+    const uint64_t nchunks = (nk + kScanChunk - 1) / kScanChunk;
+    lift = (uint64_t *)pool->alloc((4 + 2 * nk + 3 * nchunks) * 8, s, err);
+    if (!lift) {
+      release_pass(pp, s);
+      drop_table(keys);
+      return WKG_ENOMEM;
+    }
+    const KeptRows kept{pp.cinfo, pp.cexcl, nk, dtot, in->lin_excl, np, in->rows};
+    StepParams q{};
+    q.totals = lift;
+    q.tile_sum = lift + 4;       // output rows -> prefix
+    q.tile_nz = q.tile_sum + nk; // child rows -> prefix
+    q.tile_ex = q.tile_nz;       // (read before it is rewritten)
+    q.ntiles = nk;
+    q.chunk_sum = q.tile_nz + nk;
+    q.chunk_nz = q.chunk_sum + nchunks;
+    q.chunk_ex = q.chunk_nz + nchunks;
+    {
+      ProfScope ps(this, s, "lineage", 48.0 * (double)nk);
+      k_lift<<<(unsigned)std::min<uint64_t>((nk + kBlock - 1) / kBlock, 65535), kBlock, 0, s>>>(kept, q.tile_nz,
+                                                                                                q.tile_sum);
+      k_scan_chunks<<<(unsigned)nchunks, kBlock, 0, s>>>(q);
+      if (nchunks > 1) k_scan_top<<<1, kBlock, 0, s>>>(q, nchunks);
+    }
+    hipError_t le = hipGetLastError();
+    uint64_t cnt[3];
+    rc = le == hipSuccess ? read_counters(lift, 3, s, cnt) : fail(WKG_EDEVICE, "lineage launch failed: %s",
+                                                                    hipGetErrorString(le));
+    if (rc) {
+      pool->release(lift, s);
+      release_pass(pp, s);
+      drop_table(keys);
+      return rc;
+    }
+    E = cnt[0];
+    kc = cnt[2];
+    if (table_limit && in->rows >= 2 && E * (uint64_t)out_cols * 4 > table_limit) {
+      pool->release(lift, s);
+      release_pass(pp, s);
+      drop_table(keys);
+      return fail(WKG_ENOMEM, "a %llu-row output is above the table limit", (unsigned long long)E);
+    }
+    // kept child rows: (row, list offset) and first output row
+    lin = pool->alloc(kc * 16, s, err);
+    if (!lin) {
+      pool->release(lift, s);
+      release_pass(pp, s);
+      drop_table(keys);
       return WKG_ENOMEM;
     }
     {
-      ProfScope ps(this, s, "partition", 8.0 * (double)(nx + 1));
-      k_partition<<<(unsigned)((nx + 1 + kBlock - 1) / kBlock), kBlock, 0, s>>>(cexcl, nz, E, nx, part);
-    }
-    ExpandParams x{};
-    x.pid = (uint32_t)pid;
-    x.T = in->d;
-    x.c = (uint32_t)c;
-    x.R = nz;
-    x.cinfo = cinfo;
-    x.cexcl = cexcl;
-    x.part = part;
-    x.E = E;
-    x.O = t->d;
-    {
-      // edges gathered (4E) + output rows written (4E(c+1)) + the input rows
-      // with edges and their offsets read once (4 nz c + 16 nz)
-      const double b = 4.0 * E + 4.0 * E * (c + 1) + 4.0 * nz * c + 16.0 * nz;
-      ProfScope ps(this, s, "expand", b);
-      if (framed) k_expand<true><<<(unsigned)nx, kBlock, 0, s>>>(g, x);
-      else k_expand<false><<<(unsigned)nx, kBlock, 0, s>>>(g, x);
-    }
-    hipError_t e2 = hipGetLastError();
-    pool->release(part, s);
-    if (e2 != hipSuccess) {
-      pool->release(scr, s);
-      drop_table(t);
-      return fail(WKG_EDEVICE, "expand launch failed: %s", hipGetErrorString(e2));
+      ProfScope ps(this, s, "lineage", 28.0 * (double)kc);
+      k_spread<<<(unsigned)std::min<uint64_t>((kc + kBlock - 1) / kBlock, 1u << 20), kBlock, 0, s>>>(
+          kept, q.tile_nz, q.chunk_nz, q.tile_sum, q.chunk_sum, kc, (uint2 *)lin, (uint64_t *)((char *)lin + kc * 8));
     }
   }
-  pool->release(scr, s);
+  if (expanded) *expanded = E;
+  wkg_table *t = nullptr;
+  rc = new_table(E, out_cols, s, &t);
+  if (!rc && E) rc = expand_pass(in, pid, (const uint2 *)lin, (const uint64_t *)((char *)lin + kc * 8), kc, E, s, t);
+  if (lift) pool->release(lift, s);
+  release_pass(pp, s);
+  drop_table(keys);
+  if (rc) {
+    if (lin) pool->release(lin, s);
+    drop_table(t);
+    return rc;
+  }
+  ProbePass own;  // the spread rows are this output's own lineage
+  own.lin = lin;
+  own.cinfo = (uint2 *)lin;
+  own.cexcl = lin ? (uint64_t *)((char *)lin + kc * 8) : nullptr;
+  keep_lineage(in, own, kc, t);
+  if (own.lin) pool->release(own.lin, s);
+  *out = t;
+  return WKG_OK;
+}
+
+int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval,
+                        hipStream_t s, wkg_table **out, uint64_t *expanded) {
+  int rc = check_pattern(pid, dir);
+  if (rc) return rc;
+  if (!in) return fail(WKG_EINVAL, "null input table");
+  const uint64_t R = in->rows;
+  const int c = in->cols;
+  if (c < 1 || sc < 0 || sc >= c) return fail(WKG_EINVAL, "start column %d outside the %d-column table", sc, c);
+  if (mode == kModeK2K && (ec < 0 || ec >= c))
+    return fail(WKG_EINVAL, "end column %d outside the %d-column table", ec, c);
+  const int out_cols = mode == kModeK2U ? c + 1 : c;
+  if (out_cols > kBlock) return fail(WKG_ERANGE, "tables are limited to %d columns", kBlock);
+  if (R >= (1ull << 32)) return fail(WKG_ERANGE, "tables are limited to 2^32 - 1 rows per step");
+  if ((rc = ensure_pred(pid, s))) return rc;
+  if (R == 0) {  // empty input gives an empty result (core/rcache.hpp:242-250)
+    if (expanded) *expanded = 0;
+    return new_table(0, out_cols, s, out);
+  }
+  if (mode == kModeK2U && lineage && in->lin_parent && sc < in->lin_parent->cols)
+    return step_k2u_lineage(in, sc, pid, dir, s, out, expanded);
+
+  wkg_table *t = nullptr;
+  if (mode != kModeK2U && (rc = new_table(R, c, s, &t))) return rc;
+  ProbePass pp;
+  if ((rc = probe_pass(mode, in, sc, pid, dir, ec, cval, t ? t->d : nullptr, s, pp))) {
+    drop_table(t);
+    return rc;
+  }
+  if (mode != kModeK2U) {  // filters: kept rows compacted in place
+    t->rows = pp.cnt[0];
+    t->bytes = t->rows * (uint64_t)c * 4;
+    if (expanded) *expanded = pp.cnt[1];
+    release_pass(pp, s);
+    *out = t;
+    return WKG_OK;
+  }
+  const uint64_t E = pp.cnt[0], nz = pp.cnt[2];
+  if (table_limit && R >= 2 && E * (uint64_t)out_cols * 4 > table_limit) {  // (one row cannot be split)
+    release_pass(pp, s);
+    return fail(WKG_ENOMEM, "a %llu-row output is above the table limit", (unsigned long long)E);
+  }
+  if (expanded) *expanded = E;
+  if ((rc = new_table(E, out_cols, s, &t)) || (rc = expand_pass(in, pid, pp.cinfo, pp.cexcl, nz, E, s, t))) {
+    release_pass(pp, s);
+    drop_table(t);
+    return rc;
+  }
+  keep_lineage(in, pp, nz, t);
+  release_pass(pp, s);
   *out = t;
   return WKG_OK;
 }
@@ -692,6 +861,12 @@ int wkg_set_probe_tile_rows(wkg_ctx *ctx, int rows) {
   if (rows != 0 && rows != 256 && rows != 512 && rows != 1024 && rows != 2048)
     return ctx->fail(WKG_EINVAL, "probe tile rows must be 0, 256, 512, 1024 or 2048 (got %d)", rows);
   ctx->probe_tile = rows;
+  return WKG_OK;
+}
+
+int wkg_set_lineage(wkg_ctx *ctx, int on) {
+  if (!ctx) return WKG_EINVAL;
+  ctx->lineage = on != 0;
   return WKG_OK;
 }
 

@@ -45,6 +45,24 @@ struct wkg_table {
   uint64_t bytes;
   hipStream_t stream;  // stream that produced it; its memory returns there
   bool owned = true;   // false: a row slice of another table (wkg_table_slice)
+  int refs = 1;        // the caller's handle + child tables whose lineage reads these cells
+  // Lineage, set when this table is a known_to_unknown step's output: rows
+  // [lin_excl[i], lin_excl[i+1]) (the last up to `rows`) repeat the parent's
+  // row lin_info[i].x in their first parent->cols cells, i < lin_n.
+  wkg_table *lin_parent = nullptr;
+  const uint2 *lin_info = nullptr;
+  const uint64_t *lin_excl = nullptr;
+  uint64_t lin_n = 0;
+  void *lin_alloc = nullptr;  // pool memory holding lin_info and lin_excl
+};
+
+// One step's probe, tile scan and compaction (wkg_ctx::probe_pass).
+struct ProbePass {
+  uint64_t *scr = nullptr;    // step scratch
+  void *lin = nullptr;        // known_to_unknown: cinfo | cexcl (may become the output's lineage)
+  uint2 *cinfo = nullptr;     // compacted rows with edges: (row, list offset)
+  uint64_t *cexcl = nullptr;  // their first output row
+  uint64_t cnt[3] = {0, 0, 0};  // output rows, edges examined or keys probed, rows with output
 };
 
 // Framed residency (graph larger than its HBM budget): per-predicate
@@ -73,6 +91,7 @@ struct wkg_ctx {
   uint64_t budget_request = 0;
   uint64_t table_limit = 0;  // largest step output table in bytes (0: no limit)
   int probe_tile = 0;        // rows per probe tile of the per-step calls (0: by input size)
+  bool lineage = true;       // probe inherited columns once per parent row
   uint64_t *h_rb = nullptr;  // pinned read-back words
   std::string err;
 
@@ -114,11 +133,20 @@ struct wkg_ctx {
   hipStream_t pick(void *s) const;
   int new_table(uint64_t rows, int cols, hipStream_t s, wkg_table **out);
   void drop_table(wkg_table *t);
+  void unref_table(wkg_table *t);
   int read_counters(const uint64_t *d, int n, hipStream_t s, uint64_t *host);
   int check_pattern(int pid, int dir);
   int step_single_list(uint64_t key, int pid, hipStream_t s, wkg_table **out, uint64_t *expanded);
   int step_known(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval, hipStream_t s,
                  wkg_table **out, uint64_t *expanded);
+  int probe_pass(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval, int32_t *O,
+                 hipStream_t s, ProbePass &pp);
+  void release_pass(ProbePass &pp, hipStream_t s);
+  int expand_pass(const wkg_table *in, int pid, const uint2 *cinfo, const uint64_t *cexcl, uint64_t nz, uint64_t E,
+                  hipStream_t s, wkg_table *t);
+  void keep_lineage(const wkg_table *in, ProbePass &pp, uint64_t nz, wkg_table *t);
+  int step_k2u_lineage(const wkg_table *in, int sc, int pid, int dir, hipStream_t s, wkg_table **out,
+                       uint64_t *expanded);
   // residency
   int ensure_pred(int pid, hipStream_t s, const std::vector<int> *keep = nullptr);
   void free_graph();

@@ -108,6 +108,10 @@ class Device:
         """Rows per probe tile of the per-step path: 256, 512, 1024, 2048, or 0 (by input size)."""
         self._check(N.hip_lib().wkg_set_probe_tile_rows(self._h, int(rows)))
 
+    def set_lineage(self, on):
+        """Probe a star's next arm once per input row of the fan-out before it (default on)."""
+        self._check(N.hip_lib().wkg_set_lineage(self._h, 1 if on else 0))
+
     def random_access_rate(self, nbytes=4 << 30, iters=10):
         """Random 128-byte line reads per second (the hash probe's roofline)."""
         g = C.c_double()
