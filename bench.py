@@ -64,8 +64,15 @@ def main():
                          "graph; 5: the cyclic query on the 2B-triple graph in HBM-budgeted chunks")
     ap.add_argument("--heavy-only", action="store_true",
                     help="run only the heavy query and the copy calibration (for profiler passes)")
+    ap.add_argument("--probe-only", action="store_true",
+                    help="run only the known_to_unknown probe over 64M random keys (for profiler passes)")
+    ap.add_argument("--mix-only", action="store_true",
+                    help="run only the 512-query mix through the executor, 5 batches (for profiler passes)")
+    ap.add_argument("--shape", default="",
+                    help="with --mix-only: the executor shape as key=value pairs (probe_wg_per_cu=3,...) "
+                         "instead of autotuning")
     args = ap.parse_args()
-    if args.heavy_only:
+    if args.heavy_only or args.probe_only or args.mix_only:
         args.no_cpu = True
     if args.gpus != 1:
         print(f"bench.py measures one GPU's shard; --gpus {args.gpus} is not supported "
@@ -89,6 +96,31 @@ def main():
     dev.load(w.gstore)
     log(f"graph resident in HBM in {time.time() - t1:.1f}s")
     eng = GPUEngine(dev)
+    if args.probe_only:
+        rk = random_key_probe(dev, w, args.seed, HBM_PEAK_GBS * 1e9 / 128)
+        print(json.dumps({"probe_random_keys": rk}), flush=True)
+        eng.close()
+        dev.close()
+        return
+    if args.mix_only:
+        mix = w.queries["batched"]
+        eng.configure(max_queries=1024, max_patterns=8, arena_bytes=32 << 30)
+        if args.shape:
+            tuning = {k: int(v) for k, v in (kv.split("=") for kv in args.shape.split(","))}
+            eng.set_tuning(**tuning)
+            log(f"executor shape {tuning}")
+        else:
+            tuning, tune_ms = eng.autotune(mix, reps=2)
+            log(f"autotuned executor shape {tuning}: {tune_ms:.2f} ms per {len(mix)}-query batch")
+        ms = []
+        for _ in range(5):
+            _, tb = eng.run_batched_packed(mix)
+            ms.append(tb * 1e3)
+        log(f"5 batches: {', '.join(f'{x:.3f}' for x in ms)} ms")
+        print(json.dumps({"mix_batch_ms": ms, "tuning": tuning}), flush=True)
+        eng.close()
+        dev.close()
+        return
     copy_gbs = dev.copy_bandwidth(1 << 30, 20)
     lines_ps = dev.random_access_rate(4 << 30, 10)
     log(f"copy kernel bandwidth: {copy_gbs:.0f} GB/s; random 128-byte line reads: {lines_ps / 1e9:.2f} G/s "
@@ -156,40 +188,17 @@ def main():
     log(f"probe: {probe_rows / args.steps / 1e6:.1f} M rows and {probe_keys / args.steps / 1e6:.2f} M keys probed "
         f"per query in {ms_pr / args.steps:.3f} ms ({probe_roofline['heavy_query']['alg_GBps']:.0f} GB/s "
         f"algorithmic)")
-    # the probe on random keys (no runs to share): one known_to_unknown step
-    # over 64M rows of uniformly random vertex ids
+    # the probe on random keys (no runs to share)
     if not args.heavy_only:
-        rng_p = np.random.default_rng(args.seed + 7)
-        n_rand = 1 << 26
-        col = (1 << 17) + rng_p.integers(0, w.info["entities"], size=(n_rand, 1), dtype=np.int64)
-        t_rand = dev.table(col.astype(np.int32))
-        del col
-        out_t, _ = dev.k2u(t_rand, 0, 2, 1)  # warm
-        out_t.free()
-        dev.profile(True)
-        for _ in range(3):
-            out_t, _ = dev.k2u(t_rand, 0, 2, 1)
-            out_t.free()
-        n_r, ms_r, _ = dev.profile_read("probe")
-        keys_r = dev.profile_read("probe_keys")[2]
-        dev.profile(False)
-        t_rand.free()
-        rand_rate = keys_r / (ms_r * 1e-3)
-        m2 = w.gstore.metas()[2]  # partition_sz, main [start, end), indirect [start, end), ...
-        seg_bytes = int(m2[0] + (m2[4] - m2[3])) * 128
-        seg_lines = dev.random_access_rate(max(seg_bytes, 1 << 20), 10)
-        probe_roofline.update({"achieved": rand_rate, "frac": rand_rate / line_peak})
-        probe_roofline["random_keys"] = {
-            "rows": n_rand, "keys_probed": keys_r / 3, "ms_per_launch": ms_r / max(n_r, 1),
-            "achieved": rand_rate, "frac_of_peak": rand_rate / line_peak,
-            "frac_of_random_lines_4GiB": rand_rate / lines_ps,
-            "segment_bytes": seg_bytes, "random_lines_same_working_set": seg_lines,
-            "frac_of_random_lines_same_working_set": rand_rate / seg_lines,
-            "workload": "known_to_unknown over 64M uniformly random vertex ids, predicate 2 OUT"}
-        log(f"probe on random keys: {keys_r / 3 / 1e6:.1f} M keys in {ms_r / max(n_r, 1):.2f} ms = "
-            f"{rand_rate / 1e9:.2f} G keys/s = {rand_rate / line_peak:.2f} of HBM peak / 128 B; random lines "
-            f"over 4 GiB {lines_ps / 1e9:.1f} G/s, over the {seg_bytes / 2**20:.0f} MiB segment "
-            f"{seg_lines / 1e9:.1f} G/s ({rand_rate / seg_lines:.2f})")
+        rk = random_key_probe(dev, w, args.seed, line_peak)
+        rk["frac_of_random_lines_4GiB"] = rk["achieved"] / lines_ps
+        probe_roofline.update({"achieved": rk["achieved"], "frac": rk["frac_of_peak"]})
+        probe_roofline["random_keys"] = rk
+        log(f"probe on random keys: {rk['keys_probed'] / 1e6:.1f} M keys in {rk['ms_per_launch']:.2f} ms = "
+            f"{rk['achieved'] / 1e9:.2f} G keys/s = {rk['frac_of_peak']:.2f} of HBM peak / 128 B; random lines "
+            f"over 4 GiB {lines_ps / 1e9:.1f} G/s, over the {rk['segment_bytes'] / 2**20:.0f} MiB segment "
+            f"{rk['random_lines_same_working_set'] / 1e9:.1f} G/s "
+            f"({rk['frac_of_random_lines_same_working_set']:.2f})")
 
     if args.heavy_only:
         print(json.dumps({"metric": METRIC, "value": value, "unit": "expanded edges/s", "n_gpus": 1,
@@ -334,7 +343,7 @@ def main():
             "rounds_queries_per_s": len(mix) / t_rounds,
             "batched_queries_per_s": len(mix) / t_batched,
             "speedup": t_plain / t_batched,
-            "executor": "device-resident: whole queries in one captured hipGraph, persistent tile-ticket kernels",
+            "executor": "device-resident: whole queries in one captured hipGraph, persistent tile kernels (work queue or static deal, autotuned)",
             "tuning": tuning, "autotune_ms_per_batch": tune_ms, "autotune_s": t_tune,
             "plain_p50_ms": pct([x.latency_s for x in plain], 0.5) * 1e3,
             "plain_p99_ms": pct([x.latency_s for x in plain], 0.99) * 1e3,
@@ -375,6 +384,38 @@ def _line(metric, value, unit, steps, warmup, ms, config, extra, higher=True, dt
            "data": "synthetic (seeded generator; graph resident in HBM)", "config": config}
     out.update(extra)
     print(json.dumps(out), flush=True)
+
+
+def random_key_probe(dev, w, seed, line_peak):
+    """One known_to_unknown step (predicate 2 OUT) over 64M rows of uniformly
+    random vertex ids, timed with HIP events over 3 launches: keys probed per
+    second against HBM peak / 128 B and against random 128-byte line reads
+    over a buffer the size of the predicate's bucket segment."""
+    import numpy as np
+    rng_p = np.random.default_rng(seed + 7)
+    n_rand = 1 << 26
+    col = (1 << 17) + rng_p.integers(0, w.info["entities"], size=(n_rand, 1), dtype=np.int64)
+    t_rand = dev.table(col.astype(np.int32))
+    del col
+    out_t, _ = dev.k2u(t_rand, 0, 2, 1)  # warm
+    out_t.free()
+    dev.profile(True)
+    for _ in range(3):
+        out_t, _ = dev.k2u(t_rand, 0, 2, 1)
+        out_t.free()
+    n_r, ms_r, _ = dev.profile_read("probe")
+    keys_r = dev.profile_read("probe_keys")[2]
+    dev.profile(False)
+    t_rand.free()
+    rand_rate = keys_r / (ms_r * 1e-3)
+    m2 = w.gstore.metas()[2]  # partition_sz, main [start, end), indirect [start, end), ...
+    seg_bytes = int(m2[0] + (m2[4] - m2[3])) * 128
+    seg_lines = dev.random_access_rate(max(seg_bytes, 1 << 20), 10)
+    return {"rows": n_rand, "keys_probed": keys_r / 3, "ms_per_launch": ms_r / max(n_r, 1),
+            "achieved": rand_rate, "frac_of_peak": rand_rate / line_peak,
+            "segment_bytes": seg_bytes, "random_lines_same_working_set": seg_lines,
+            "frac_of_random_lines_same_working_set": rand_rate / seg_lines,
+            "workload": "known_to_unknown over 64M uniformly random vertex ids, predicate 2 OUT"}
 
 
 def run_config1(args):

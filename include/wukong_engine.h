@@ -84,6 +84,8 @@ int wke_configure(wke_engine *e, int max_queries, int max_patterns, uint64_t are
  * with heavy != 0 for the heavy serving lane. */
 int wke_autotune(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, int reps, int heavy,
                  wkg_batch_tuning *best, double *best_ms);
+/* Sets the executor shape without autotuning (heavy as for wke_autotune). */
+int wke_set_tuning(wke_engine *e, const wkg_batch_tuning *t, int heavy);
 
 /* Served path: an open-loop arrival stream. Query q arrives at arrival_s[q]
  * (seconds from the start, ascending). Arrived queries wait in a light or a

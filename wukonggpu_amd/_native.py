@@ -172,6 +172,7 @@ def engine_lib():
         _sig(L, "wke_configure", C.c_int, vp, C.c_int, C.c_int, u64)
         _sig(L, "wke_stats", C.c_int, vp, P(u64))
         _sig(L, "wke_autotune", C.c_int, vp, P(i64), P(i64), C.c_int, C.c_int, C.c_int, P(BatchTuning), P(dbl))
+        _sig(L, "wke_set_tuning", C.c_int, vp, P(BatchTuning), C.c_int)
         _sig(L, "wke_serve", C.c_int, vp, P(i64), P(i64), C.c_int, P(dbl), P(C.c_uint8), u64, C.c_int,
              P(Result), P(dbl))
         _engine = L

@@ -615,6 +615,18 @@ int wke_run_batched_packed(wke_engine *e, const int64_t *cmd, const int64_t *cmd
   return rc;
 }
 
+int wke_set_tuning(wke_engine *e, const wkg_batch_tuning *t, int heavy) {
+  if (!e || !t) return WKG_EINVAL;
+  if (!heavy && e->exec) {
+    if (int rc = wkg_batch_set_tuning(e->exec, t)) return e->fail_ctx(rc);
+    if (int rc = wkg_batch_get_tuning(e->exec, &e->tuning)) return e->fail_ctx(rc);
+  } else {
+    (heavy ? e->tuning_heavy : e->tuning) = *t;
+  }
+  (heavy ? e->tuned_heavy : e->tuned) = true;
+  return WKG_OK;
+}
+
 int wke_autotune(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, int reps, int heavy,
                  wkg_batch_tuning *best, double *best_ms) {
   if (!e || !cmd || !cmd_off || nq < 1 || reps < 1) return WKG_EINVAL;

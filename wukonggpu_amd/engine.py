@@ -290,6 +290,14 @@ class GPUEngine:
         if rc != 0:
             raise DeviceError(rc, "wke_configure failed")
 
+    def set_tuning(self, heavy=False, **shape):
+        """Pins the executor's launch and tile shape (the keys of autotune's result)."""
+        L = N.engine_lib()
+        t = N.BatchTuning(**shape)
+        rc = L.wke_set_tuning(self._h, C.byref(t), 1 if heavy else 0)
+        if rc != 0:
+            raise DeviceError(rc, L.wke_last_error(self._h).decode())
+
     def autotune(self, queries, reps=3, heavy=False):
         """Autotunes the executor's launch and tile shapes on a sample batch (for
         the heavy serving lane with heavy=True); returns (shape, ms)."""
