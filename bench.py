@@ -267,7 +267,9 @@ def main():
         rng = np.random.default_rng(args.seed + 12)
         for rate in [float(x) for x in args.served_rates.split(",") if x]:
             arr = np.cumsum(rng.exponential(1.0 / rate, size=len(sq)))
+            pool0 = dev.pool_stats()
             res, wall = eng.serve(sq, arr, heavy=sflags, table_budget_bytes=64 << 30, light_lanes=2)
+            pool1 = dev.pool_stats()
             ok = same(alone, res)
             served_same = served_same and ok
             lat_l = [x.latency_s for x, f in zip(res, sflags) if not f]
@@ -277,11 +279,13 @@ def main():
                    "p99_ms": pct([x.latency_s for x in res], 0.99) * 1e3,
                    "light_p50_ms": pct(lat_l, 0.5) * 1e3, "light_p99_ms": pct(lat_l, 0.99) * 1e3,
                    "heavy_p50_ms": pct(lat_h, 0.5) * 1e3, "heavy_p99_ms": pct(lat_h, 0.99) * 1e3,
-                   "identical_to_alone": bool(ok)}
+                   "identical_to_alone": bool(ok), "serve_stats": eng.serve_stats(),
+                   "pool": {"mallocs": pool1["mallocs"] - pool0["mallocs"], "trims": pool1["trims"] - pool0["trims"],
+                            "live_bytes": pool1["live_bytes"], "cached_bytes": pool1["cached_bytes"]}}
             served.append(rec)
             log(f"served @ {rate:.0f} q/s offered: {rec['achieved_qps']:.0f} q/s, p50 {rec['p50_ms']:.2f} ms, "
                 f"p99 {rec['p99_ms']:.2f} ms (light p99 {rec['light_p99_ms']:.2f}, heavy p99 "
-                f"{rec['heavy_p99_ms']:.2f}), identical={ok}")
+                f"{rec['heavy_p99_ms']:.2f}), identical={ok}; {rec['serve_stats']}, pool {rec['pool']}")
 
     # ---- CPU baseline (the oracle), parity of the heavy result --------------------------
     cpu = None

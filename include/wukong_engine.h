@@ -87,13 +87,25 @@ int wke_autotune(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int 
 /* Sets the executor shape without autotuning (heavy as for wke_autotune). */
 int wke_set_tuning(wke_engine *e, const wkg_batch_tuning *t, int heavy);
 
+/* Where wke_serve runs heavy queries: -1 (default) a worker thread on the
+ * per-step path when the graph is resident, else a heavy executor lane; 0 a
+ * heavy executor lane; 1 the worker thread (resident graphs only). */
+int wke_set_heavy_mode(wke_engine *e, int mode);
+/* Counters of the last wke_serve: plain-path fallbacks on the serving
+ * thread, queries retried alone, queries run by the worker thread, and the
+ * worker's longest backlog. */
+int wke_serve_stats(const wke_engine *e, uint64_t *fallbacks, uint64_t *solo, uint64_t *worker_queries,
+                    uint64_t *worker_backlog);
+
 /* Served path: an open-loop arrival stream. Query q arrives at arrival_s[q]
  * (seconds from the start, ascending). Arrived queries wait in a light or a
  * heavy queue (heavy[q] != 0; heavy may be NULL). Light queries run on
- * `light_lanes` device executors, heavy queries on one more, each with its
- * own stream and arena; the arenas together are the table budget
- * (table_budget_bytes; half to the heavy lane, the rest split over the light
- * lanes). Whenever a lane is free, the queries waiting in its queue are
+ * `light_lanes` device executors, each with its own stream and arena; heavy
+ * queries run as wke_set_heavy_mode says: on a worker thread, one at a time
+ * on the per-step path (whose star arms use lineage), or on one more
+ * executor lane. The arenas together are the table budget
+ * (table_budget_bytes; half to a heavy lane if there is one, the rest split
+ * over the light lanes). Whenever a lane is free, the queries waiting in its queue are
  * admitted as one device batch, as many as its arena holds at the bytes per
  * query the lane has seen (its arena high-water mark over its batch size; a
  * heavy lane starts with one query), so work that arrives while other

@@ -13,7 +13,8 @@
  *
  * Conventions
  *  - Every entry point returns 0 (WKG_OK) or a negative WKG_E* status; none
- *    aborts. wkg_last_error(ctx) describes the most recent failure.
+ *    aborts. wkg_last_error(ctx) describes the calling thread's most recent
+ *    failure.
  *  - A result table is row-major int32, rows x cols, like
  *    request_or_reply::result_table (core/query.hpp:69). Row counts are 64-bit.
  *  - Column numbers are 0-based; the reference's variable -k is column k-1
@@ -87,6 +88,9 @@ int wkg_ensure_predicate(wkg_ctx *ctx, int pid);
 /* Graph residency statistics: bytes resident, frame loads, evictions. */
 int wkg_graph_stats(const wkg_ctx *ctx, uint64_t *resident_bytes,
                     uint64_t *segment_loads, uint64_t *segment_evictions);
+/* 1 when the loaded graph is framed (segments loaded on demand under a graph
+ * budget), 0 when the whole image is resident, negative on error. */
+int wkg_graph_framed(const wkg_ctx *ctx);
 
 /* ---- tables ------------------------------------------------------------------ */
 
@@ -119,6 +123,20 @@ int wkg_set_probe_tile_rows(wkg_ctx *ctx, int rows);
  * the same, row for row. On by default; 0 turns it off. No counterpart in the
  * reference, which probes every row (core/gpu_hash.cu:145-316). */
 int wkg_set_lineage(wkg_ctx *ctx, int on);
+/* Per-thread arena: until wkg_thread_arena_end, the tables and step scratch
+ * that the calling thread's per-step calls allocate are carved from `bytes`
+ * of pool memory reserved now, with no hipMalloc; wkg_thread_arena_reset
+ * drops them all at once (call it only when every table the thread made is
+ * freed and its work has completed, e.g. after a query's last table reached
+ * the host). An allocation that does not fit takes pool memory as usual.
+ * For a worker thread that runs one query at a time beside other streams. */
+int wkg_thread_arena_begin(wkg_ctx *ctx, uint64_t bytes);
+int wkg_thread_arena_reset(wkg_ctx *ctx);
+int wkg_thread_arena_end(wkg_ctx *ctx);
+/* Table/scratch pool: bytes live and cached, hipMalloc calls, and cache drops
+ * (each drop synchronises the device and frees every cached block). */
+int wkg_pool_stats(wkg_ctx *ctx, uint64_t *live_bytes, uint64_t *cached_bytes, uint64_t *mallocs,
+                   uint64_t *trims);
 /* A view of rows [row0, row0 + nrows) of t, sharing its memory (for running
  * a step over a table in row chunks). Free the view before t. */
 int wkg_table_slice(wkg_ctx *ctx, const wkg_table *t, uint64_t row0, uint64_t nrows, wkg_table **out);

@@ -228,5 +228,11 @@ def test_framed_residency_matches_oracle(graph_mid):
     eng.configure(max_queries=16, max_patterns=6, arena_bytes=64 << 20)
     for q, r in zip(qs, eng.run_batched(qs)):
         _check(gs, q, r)
+    print("framed: served (heavy queries on a heavy executor lane, not the worker thread)", flush=True)
+    heavy = np.array([len(q) >= 12 for q in qs], dtype=np.uint8)
+    arrivals = np.cumsum(rng.exponential(0.0002, size=len(qs)))
+    res, _ = eng.serve(qs, arrivals, heavy=heavy, table_budget_bytes=256 << 20, light_lanes=2)
+    for q, r in zip(qs, res):
+        _check(gs, q, r)
     eng.close()
     dev.close()

@@ -129,6 +129,7 @@ def hip_lib():
         _sig(L, "wkg_set_table_limit", C.c_int, vp, u64)
         _sig(L, "wkg_set_probe_tile_rows", C.c_int, vp, C.c_int)
         _sig(L, "wkg_set_lineage", C.c_int, vp, C.c_int)
+        _sig(L, "wkg_pool_stats", C.c_int, vp, P(u64), P(u64), P(u64), P(u64))
         _sig(L, "wkg_step_i2u", C.c_int, vp, C.c_int, C.c_int, vp, P(vp), P(u64))
         _sig(L, "wkg_step_c2u", C.c_int, vp, i64, C.c_int, C.c_int, vp, P(vp), P(u64))
         _sig(L, "wkg_step_k2u", C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, vp, P(vp), P(u64))
@@ -173,6 +174,8 @@ def engine_lib():
         _sig(L, "wke_stats", C.c_int, vp, P(u64))
         _sig(L, "wke_autotune", C.c_int, vp, P(i64), P(i64), C.c_int, C.c_int, C.c_int, P(BatchTuning), P(dbl))
         _sig(L, "wke_set_tuning", C.c_int, vp, P(BatchTuning), C.c_int)
+        _sig(L, "wke_set_heavy_mode", C.c_int, vp, C.c_int)
+        _sig(L, "wke_serve_stats", C.c_int, vp, P(u64), P(u64), P(u64), P(u64))
         _sig(L, "wke_serve", C.c_int, vp, P(i64), P(i64), C.c_int, P(dbl), P(C.c_uint8), u64, C.c_int,
              P(Result), P(dbl))
         _engine = L

@@ -92,7 +92,7 @@ extern "C" int wkg_step_batch(wkg_ctx *ctx, int n, const wkg_step_desc *steps, v
   for (int i = 0; i < n; i++) {
     const wkg_step_desc &d = steps[i];
     int rc = ctx->check_pattern(d.pid, d.dir);
-    if (rc) return ctx->fail(rc, "step %d: %s", i, ctx->err.c_str());
+    if (rc) return ctx->fail(rc, "step %d: %s", i, wkg_ctx::err_slot().c_str());
     switch (d.kind) {
       case WKG_STEP_I2U:
         if (!is_tpid(d.pid)) return ctx->fail(WKG_EINVAL, "step %d: index id %d is not a predicate/type id", i, d.pid);
@@ -134,7 +134,7 @@ extern "C" int wkg_step_batch(wkg_ctx *ctx, int n, const wkg_step_desc *steps, v
   const uint64_t w_items = (sizeof(BatchItem) * n + 7) / 8;
   const uint64_t hit_words = 2 * NT * kTileRows;
   const uint64_t words = 4 * (uint64_t)n + (n + 1) + 3 * NT + hit_words + 2 * NR + w_items;
-  uint64_t *scr = (uint64_t *)ctx->pool->alloc(words * 8, s, ctx->err);
+  uint64_t *scr = (uint64_t *)ctx->pool->alloc(words * 8, s, wkg_ctx::err_slot());
   if (!scr) return WKG_ENOMEM;
   uint64_t *totals = scr;
   uint64_t *d_tileb = totals + 4 * n;
@@ -255,7 +255,7 @@ extern "C" int wkg_step_batch(wkg_ctx *ctx, int n, const wkg_step_desc *steps, v
   ctx->prof_add_bytes("batch_finalize", fin_bytes);
   const uint64_t NX = xtileb[n], NP = partb[n];
   if (NX) {
-    uint64_t *xs = (uint64_t *)ctx->pool->alloc((NP + 2 * (n + 1)) * 8, s, ctx->err);
+    uint64_t *xs = (uint64_t *)ctx->pool->alloc((NP + 2 * (n + 1)) * 8, s, wkg_ctx::err_slot());
     if (!xs) return bail(WKG_ENOMEM);
     uint64_t *d_part = xs, *d_partb = xs + NP, *d_xtileb = d_partb + (n + 1);
     for (int i = 0; i < n; i++) items[i].x.part = d_part + partb[i];

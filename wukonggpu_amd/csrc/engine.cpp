@@ -13,7 +13,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <deque>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -149,7 +151,18 @@ void fill_result(const wkg_batch_result &r, const int32_t *cells, wke_result *ou
 
 struct wke_engine {
   wkg_ctx *ctx = nullptr;
-  std::string err;
+  // The calling thread's last failure message (wke_last_error): the served
+  // path runs heavy queries on a second host thread.
+  static std::string &err_slot() {
+    static thread_local std::string s;
+    return s;
+  }
+  // Held around every per-step (plain) query while a served loop's worker
+  // thread may run one: the per-step path keeps one read-back buffer and
+  // stream per context.
+  std::mutex plain_mu;
+  int heavy_mode = -1;  // wke_set_heavy_mode
+  uint64_t sv_fallbacks = 0, sv_solo = 0, sv_worker = 0, sv_backlog = 0;  // last wke_serve
   // device-resident executor shape (wke_configure) and tuning (wke_autotune)
   int cfg_queries = 1024, cfg_patterns = 8;
   uint64_t cfg_arena = 16ull << 30;
@@ -182,17 +195,17 @@ struct wke_engine {
     return WKG_OK;
   }
   int fail(int code, const std::string &m) {
-    err = m;
+    err_slot() = m;
     return code;
   }
   int fail_ctx(int code) {
-    err = wkg_last_error(ctx);
+    err_slot() = wkg_last_error(ctx);
     return code;
   }
 
   // The step descriptor of r's current step.
   int plan(const Request &r, wkg_step_desc *d, int *new_cols) {
-    int rc = dispatch(r.cmd, r.step, r.col_num, d, new_cols, err);
+    int rc = dispatch(r.cmd, r.step, r.col_num, d, new_cols, err_slot());
     d->in = r.t;
     return rc;
   }
@@ -244,7 +257,7 @@ struct wke_engine {
           free(res->table);
           res->table = nullptr;
           res->status = rc;
-          err = wkg_last_error(ctx);
+          err_slot() = wkg_last_error(ctx);
         }
       }
     }
@@ -349,7 +362,7 @@ int wke_destroy(wke_engine *e) {
   return WKG_OK;
 }
 
-const char *wke_last_error(const wke_engine *e) { return e ? e->err.c_str() : "null engine"; }
+const char *wke_last_error(const wke_engine *e) { return e ? wke_engine::err_slot().c_str() : "null engine"; }
 void wke_free(void *p) { free(p); }
 
 int wke_run_query(wke_engine *e, const int64_t *cmd, int ncmd, wke_result *res) {
@@ -420,7 +433,7 @@ int batched_round(wke_engine *e, std::vector<Request *> &act, void *stream) {
     return WKG_OK;
   }
   if (rc != WKG_ENOMEM) {
-    e->err = wkg_last_error(e->ctx);
+    e->err_slot() = wkg_last_error(e->ctx);
     for (Request *r : who) r->status = rc;
     return rc;
   }
@@ -527,7 +540,7 @@ int run_batched_into(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, 
     std::string why;
     if (int rc = compile_query(cmd + cmd_off[q], n, plans[q], why)) {
       res[q].status = rc;
-      e->err = why;
+      e->err_slot() = why;
       note(rc);
     } else if ((int)plans[q].size() > e->cfg_patterns) {
       plain.push_back(q);
@@ -615,6 +628,22 @@ int wke_run_batched_packed(wke_engine *e, const int64_t *cmd, const int64_t *cmd
   return rc;
 }
 
+int wke_set_heavy_mode(wke_engine *e, int mode) {
+  if (!e || mode < -1 || mode > 1) return WKG_EINVAL;
+  e->heavy_mode = mode;
+  return WKG_OK;
+}
+
+int wke_serve_stats(const wke_engine *e, uint64_t *fallbacks, uint64_t *solo, uint64_t *worker_queries,
+                    uint64_t *worker_backlog) {
+  if (!e) return WKG_EINVAL;
+  if (fallbacks) *fallbacks = e->sv_fallbacks;
+  if (solo) *solo = e->sv_solo;
+  if (worker_queries) *worker_queries = e->sv_worker;
+  if (worker_backlog) *worker_backlog = e->sv_backlog;
+  return WKG_OK;
+}
+
 int wke_set_tuning(wke_engine *e, const wkg_batch_tuning *t, int heavy) {
   if (!e || !t) return WKG_EINVAL;
   if (!heavy && e->exec) {
@@ -672,6 +701,23 @@ struct Lane {
   std::vector<int> run;
 };
 
+// The heavy lane on a resident graph: a host thread that runs heavy queries
+// one at a time on the per-step path (whose star arms use lineage), on the
+// context's own stream, while the light lanes' executors run on theirs.
+struct PlainWorker {
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<int> in;
+  struct Done {
+    int q, rc;
+    std::string why;
+  };
+  std::vector<Done> out;
+  bool stop = false, ready = false;
+  int pending = 0;  // queued or running (main thread only)
+  std::thread th;
+};
+
 }  // namespace
 
 int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, const double *arrival_s,
@@ -682,10 +728,26 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
   bool any_heavy = false;
   for (int q = 0; heavy && q < nq; q++) any_heavy |= heavy[q] != 0;
   if (table_budget_bytes == 0) table_budget_bytes = 64ull << 30;
-  const uint64_t heavy_arena = any_heavy ? table_budget_bytes / 2 : 0;
+  // heavy queries: a worker thread on the per-step path when the graph is
+  // resident; with a framed graph (segment loads and evictions are not
+  // thread-safe) a heavy executor lane instead
+  const bool resident = wkg_graph_framed(e->ctx) == 0;
+  const bool worker = any_heavy && resident && e->heavy_mode != 0;
+  e->sv_fallbacks = e->sv_solo = e->sv_worker = e->sv_backlog = 0;
+  const bool heavy_lane = any_heavy && !worker;
+  const uint64_t heavy_arena = (heavy_lane || worker) ? table_budget_bytes / 2 : 0;
   const uint64_t light_arena = (table_budget_bytes - heavy_arena) / (uint64_t)light_lanes;
   std::vector<Lane> lanes;
+  PlainWorker pw;
   auto release = [&] {
+    if (pw.th.joinable()) {
+      {
+        std::lock_guard<std::mutex> lk(pw.mu);
+        pw.stop = true;
+      }
+      pw.cv.notify_one();
+      pw.th.join();
+    }
     for (Lane &l : lanes) {
       if (l.busy) {
         std::vector<wkg_batch_result> sink(l.run.size());
@@ -694,7 +756,7 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
       wkg_batch_destroy(l.b);
     }
   };
-  for (int i = 0; i < light_lanes + (any_heavy ? 1 : 0); i++) {
+  for (int i = 0; i < light_lanes + (heavy_lane ? 1 : 0); i++) {
     Lane l;
     l.heavy = i == light_lanes;
     l.arena = l.heavy ? heavy_arena : light_arena;
@@ -707,6 +769,41 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
   std::vector<std::vector<wkg_plan_step>> plans(nq);
   std::vector<char> solo(nq, 0);
   std::deque<int> wait[2];  // light, heavy
+  if (worker) {
+    // The worker's tables and scratch come from its own arena (the heavy
+    // share of the budget), reserved before arrivals start and reset after
+    // every query: no hipMalloc or hipFree (which wait on the device) while
+    // the light lanes keep it busy.
+    pw.th = std::thread([&] {
+      const bool arena = wkg_thread_arena_begin(e->ctx, heavy_arena) == WKG_OK;
+      {
+        std::lock_guard<std::mutex> lk(pw.mu);
+        pw.ready = true;
+      }
+      pw.cv.notify_all();
+      for (;;) {
+        int q;
+        {
+          std::unique_lock<std::mutex> lk(pw.mu);
+          pw.cv.wait(lk, [&] { return pw.stop || !pw.in.empty(); });
+          if (pw.stop) break;
+          q = pw.in.front();
+          pw.in.pop_front();
+        }
+        int rc;
+        {
+          std::lock_guard<std::mutex> pl(e->plain_mu);
+          rc = wke_run_query(e, cmd + cmd_off[q], (int)(cmd_off[q + 1] - cmd_off[q]), &res[q]);
+          if (arena) (void)wkg_thread_arena_reset(e->ctx);
+        }
+        std::lock_guard<std::mutex> lk(pw.mu);
+        pw.out.push_back({q, rc, rc ? std::string(wke_last_error(e)) : std::string()});
+      }
+      if (arena) (void)wkg_thread_arena_end(e->ctx);
+    });
+    std::unique_lock<std::mutex> lk(pw.mu);
+    pw.cv.wait(lk, [&] { return pw.ready; });
+  }
   const auto t0 = Clock::now();
   auto now = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
   int next = 0, completed = 0, first = WKG_OK;
@@ -716,7 +813,12 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
     completed++;
   };
   auto run_plain = [&](int q) {
-    int rc = wke_run_query(e, cmd + cmd_off[q], (int)(cmd_off[q + 1] - cmd_off[q]), &res[q]);
+    e->sv_fallbacks++;
+    int rc;
+    {
+      std::lock_guard<std::mutex> pl(e->plain_mu);
+      rc = wke_run_query(e, cmd + cmd_off[q], (int)(cmd_off[q + 1] - cmd_off[q]), &res[q]);
+    }
     done(q, rc);
   };
   std::vector<wkg_batch_result> rec((size_t)e->cfg_queries);
@@ -731,14 +833,37 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
       int rc = compile_query(cmd + cmd_off[q], (int)(cmd_off[q + 1] - cmd_off[q]), plans[q], why);
       if (rc) {
         res[q].status = rc;
-        e->err = why;
+        e->err_slot() = why;
         done(q, rc);
       } else if ((int)plans[q].size() > e->cfg_patterns) {
         run_plain(q);
+      } else if (worker && heavy[q]) {
+        {
+          std::lock_guard<std::mutex> lk(pw.mu);
+          pw.in.push_back(q);
+        }
+        pw.cv.notify_one();
+        pw.pending++;
+        e->sv_worker++;
+        e->sv_backlog = std::max<uint64_t>(e->sv_backlog, (uint64_t)pw.pending);
       } else {
         wait[any_heavy && heavy[q] ? 1 : 0].push_back(q);
       }
       progressed = true;
+    }
+    // heavy queries the worker finished
+    if (worker) {
+      std::vector<PlainWorker::Done> fin;
+      {
+        std::lock_guard<std::mutex> lk(pw.mu);
+        fin.swap(pw.out);
+      }
+      for (const PlainWorker::Done &d : fin) {
+        if (d.rc) e->err_slot() = d.why;
+        pw.pending--;
+        done(d.q, d.rc);
+        progressed = true;
+      }
     }
     // retire finished batches
     for (Lane &l : lanes) {
@@ -758,6 +883,7 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
           done(q, rc);
         } else if (rec[i].status == WKG_ENOMEM && l.run.size() > 1) {
           solo[q] = 1;  // retried alone
+          e->sv_solo++;
           w.push_front(q);
         } else if (rec[i].status == WKG_ENOMEM) {
           run_plain(q);
@@ -791,7 +917,7 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
       if (rc == WKG_ENOMEM) {  // the batch's predicates do not all fit a framed graph budget at once
         for (int q : c.who) run_plain(q);
       } else if (rc) {
-        e->err = wkg_last_error(e->ctx);
+        e->err_slot() = wkg_last_error(e->ctx);
         for (int q : c.who) {
           res[q].status = rc;
           done(q, rc);
@@ -803,7 +929,7 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
       progressed = true;
     }
     if (progressed) continue;
-    bool any_busy = false;
+    bool any_busy = pw.pending > 0;
     for (Lane &l : lanes) any_busy |= l.busy;
     if (!any_busy && next < nq) {
       const double dt = arrival_s[next] - now();

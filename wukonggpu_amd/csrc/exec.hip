@@ -1012,9 +1012,9 @@ int wkg_batch_create(wkg_ctx *ctx, int max_queries, int max_patterns, uint64_t a
     return rc;
   };
   if (e != hipSuccess) return bail(ctx->fail(WKG_EDEVICE, "batch stream setup failed: %s", hipGetErrorString(e)));
-  b->dctl = (char *)ctx->pool->alloc(b->ctl_bytes, b->s, ctx->err);
+  b->dctl = (char *)ctx->pool->alloc(b->ctl_bytes, b->s, wkg_ctx::err_slot());
   if (!b->dctl) return bail(WKG_ENOMEM);
-  b->arena = (char *)ctx->pool->alloc(b->cap, b->s, ctx->err);
+  b->arena = (char *)ctx->pool->alloc(b->cap, b->s, wkg_ctx::err_slot());
   if (!b->arena) return bail(WKG_ENOMEM);
   e = hipHostMalloc((void **)&b->h_up, b->up_bytes, hipHostMallocDefault);
   if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_down, b->down_bytes, hipHostMallocDefault);
@@ -1124,7 +1124,7 @@ int wkg_batch_launch(wkg_batch *b, int nq, const wkg_plan_step *steps, const int
     int cols = 0;
     for (int l = 0; l < n; l++) {
       const wkg_plan_step &d = steps[step_off[q] + l];
-      if (int rc = ctx->check_pattern(d.pid, d.dir)) return ctx->fail(rc, "query %d step %d: %s", q, l, ctx->err.c_str());
+      if (int rc = ctx->check_pattern(d.pid, d.dir)) return ctx->fail(rc, "query %d step %d: %s", q, l, wkg_ctx::err_slot().c_str());
       switch (d.kind) {
         case WKG_STEP_I2U:
         case WKG_STEP_C2U:

@@ -444,6 +444,12 @@ int wkg_ensure_predicate(wkg_ctx *ctx, int pid) {
   return ctx->ensure_pred(pid, ctx->stream);
 }
 
+int wkg_graph_framed(const wkg_ctx *ctx) {
+  if (!ctx) return WKG_EINVAL;
+  if (!ctx->loaded) return WKG_ESTATE;
+  return ctx->framed ? 1 : 0;
+}
+
 int wkg_graph_stats(const wkg_ctx *ctx, uint64_t *resident_bytes, uint64_t *segment_loads,
                     uint64_t *segment_evictions) {
   if (!ctx) return WKG_EINVAL;

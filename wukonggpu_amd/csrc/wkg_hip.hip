@@ -33,7 +33,7 @@ int wkg_ctx::fail(int code, const char *fmt, ...) {
   va_start(ap, fmt);
   vsnprintf(buf, sizeof buf, fmt, ap);
   va_end(ap);
-  err = buf;
+  err_slot() = buf;
   return code;
 }
 
@@ -70,6 +70,14 @@ size_t round_size(size_t n) {
 }  // namespace
 
 void *Pool::alloc(size_t bytes, hipStream_t s, std::string &err) {
+  if (BumpArena *b = thread_arena()) {
+    const uint64_t sz = ((uint64_t)(bytes ? bytes : 1) + 255) & ~uint64_t(255);
+    if (b->used + sz <= b->cap) {
+      void *p = b->base + b->used;
+      b->used += sz;
+      return p;
+    }  // (a query bigger than the region takes pool memory as usual)
+  }
   std::lock_guard<std::mutex> lk(mu);
   const size_t sz = round_size(bytes ? bytes : 1);
   // reuse a cached block at most a quarter larger: a small request must not
@@ -89,11 +97,11 @@ void *Pool::alloc(size_t bytes, hipStream_t s, std::string &err) {
           " left";
     return nullptr;
   }
-  if (live_bytes + cached_bytes + sz > budget) trim_locked();
+  if (live_bytes + cached_bytes + sz > budget) trim_locked(sz);
   void *p = nullptr;
   hipError_t e = hipMalloc(&p, sz);
   if (e != hipSuccess) {
-    trim_locked();
+    trim_locked(0);
     e = hipMalloc(&p, sz);
   }
   if (e != hipSuccess) {
@@ -103,11 +111,14 @@ void *Pool::alloc(size_t bytes, hipStream_t s, std::string &err) {
   }
   live[p] = sz;
   live_bytes += sz;
+  mallocs++;
   return p;
 }
 
 void Pool::release(void *p, hipStream_t s) {
   if (!p) return;
+  if (BumpArena *b = thread_arena())
+    if ((char *)p >= b->base && (char *)p < b->base + b->cap) return;
   std::lock_guard<std::mutex> lk(mu);
   auto it = live.find(p);
   if (it == live.end()) return;
@@ -118,17 +129,32 @@ void Pool::release(void *p, hipStream_t s) {
   cached_bytes += sz;
 }
 
-void Pool::trim_locked() {
+// Frees cached blocks, largest first, until `need` more bytes fit the
+// budget (need 0: all of them). hipFree waits for the device, so the cache
+// is only cut as far as needed.
+void Pool::trim_locked(uint64_t need) {
   if (cached.empty()) return;
+  trims++;
   (void)hipDeviceSynchronize();
-  for (auto &kv : cached) (void)hipFree(kv.second.ptr);
-  cached.clear();
-  cached_bytes = 0;
+  while (!cached.empty() && (need == 0 || live_bytes + cached_bytes + need > budget)) {
+    auto it = std::prev(cached.end());
+    (void)hipFree(it->second.ptr);
+    cached_bytes -= it->first;
+    cached.erase(it);
+  }
+}
+
+void Pool::stats(uint64_t *live, uint64_t *cached_b, uint64_t *nmalloc, uint64_t *ntrim) {
+  std::lock_guard<std::mutex> lk(mu);
+  if (live) *live = live_bytes;
+  if (cached_b) *cached_b = cached_bytes;
+  if (nmalloc) *nmalloc = mallocs;
+  if (ntrim) *ntrim = trims;
 }
 
 void Pool::trim() {
   std::lock_guard<std::mutex> lk(mu);
-  trim_locked();
+  trim_locked(0);
 }
 
 Pool::~Pool() {
@@ -152,7 +178,12 @@ GraphView wkg_ctx::view() const {
   return g;
 }
 
-hipStream_t wkg_ctx::pick(void *s) const { return s ? (hipStream_t)s : stream; }
+// The launch stream of an entry point; also makes the context's device
+// current on the calling thread (a context may be driven from several).
+hipStream_t wkg_ctx::pick(void *s) const {
+  (void)hipSetDevice(device);
+  return s ? (hipStream_t)s : stream;
+}
 
 int wkg_ctx::new_table(uint64_t rows, int cols, hipStream_t s, wkg_table **out) {
   wkg_table *t = new (std::nothrow) wkg_table;
@@ -163,7 +194,7 @@ int wkg_ctx::new_table(uint64_t rows, int cols, hipStream_t s, wkg_table **out) 
   t->bytes = rows * (uint64_t)cols * sizeof(int32_t);
   t->d = nullptr;
   if (t->bytes) {
-    t->d = (int32_t *)pool->alloc(t->bytes, s, err);
+    t->d = (int32_t *)pool->alloc(t->bytes, s, err_slot());
     if (!t->d) {
       delete t;
       return WKG_ENOMEM;
@@ -214,7 +245,7 @@ int wkg_ctx::check_pattern(int pid, int dir) {
 int wkg_ctx::step_single_list(uint64_t key, int pid, hipStream_t s, wkg_table **out, uint64_t *expanded) {
   int rc = ensure_pred(pid, s);
   if (rc) return rc;
-  uint64_t *d_rb = (uint64_t *)pool->alloc(2 * sizeof(uint64_t), s, err);
+  uint64_t *d_rb = (uint64_t *)pool->alloc(2 * sizeof(uint64_t), s, err_slot());
   if (!d_rb) return WKG_ENOMEM;
   GraphView g = view();
   {
@@ -297,10 +328,10 @@ int wkg_ctx::probe_pass(int mode, const wkg_table *in, int sc, int pid, int dir,
   const uint64_t hit_words = 2 * ntiles * TR;
   const uint64_t nchunks = (ntiles + kScanChunk - 1) / kScanChunk;
   const uint64_t scratch_words = 4 + 3 * ntiles + 3 * nchunks + hit_words;
-  pp.scr = (uint64_t *)pool->alloc(scratch_words * 8, s, err);
+  pp.scr = (uint64_t *)pool->alloc(scratch_words * 8, s, err_slot());
   if (!pp.scr) return WKG_ENOMEM;
   if (mode == kModeK2U) {  // cinfo[R] (8 B) | cexcl[R]: may outlive the step as the output's lineage
-    pp.lin = pool->alloc(R * 16, s, err);
+    pp.lin = pool->alloc(R * 16, s, err_slot());
     if (!pp.lin) {
       release_pass(pp, s);
       return WKG_ENOMEM;
@@ -383,7 +414,7 @@ int wkg_ctx::expand_pass(const wkg_table *in, int pid, const uint2 *cinfo, const
   if (!E) return WKG_OK;
   const int c = in->cols;
   const uint64_t nx = (nz + E + kExpandNV - 1) / kExpandNV;
-  uint64_t *part = (uint64_t *)pool->alloc((nx + 1) * 8, s, err);
+  uint64_t *part = (uint64_t *)pool->alloc((nx + 1) * 8, s, err_slot());
   if (!part) return WKG_ENOMEM;
   {
     ProfScope ps(this, s, "partition", 8.0 * (double)(nx + 1));
@@ -465,7 +496,7 @@ int wkg_ctx::step_k2u_lineage(const wkg_table *in, int sc, int pid, int dir, hip
   if (nk) {
     // per kept parent row: child rows and output rows, scanned like tile sums
     const uint64_t nchunks = (nk + kScanChunk - 1) / kScanChunk;
-    lift = (uint64_t *)pool->alloc((4 + 2 * nk + 3 * nchunks) * 8, s, err);
+    lift = (uint64_t *)pool->alloc((4 + 2 * nk + 3 * nchunks) * 8, s, err_slot());
     if (!lift) {
       release_pass(pp, s);
       drop_table(keys);
@@ -507,7 +538,7 @@ int wkg_ctx::step_k2u_lineage(const wkg_table *in, int sc, int pid, int dir, hip
       return fail(WKG_ENOMEM, "a %llu-row output is above the table limit", (unsigned long long)E);
     }
     // kept child rows: (row, list offset) and first output row
-    lin = pool->alloc(kc * 16, s, err);
+    lin = pool->alloc(kc * 16, s, err_slot());
     if (!lin) {
       pool->release(lift, s);
       release_pass(pp, s);
@@ -810,7 +841,7 @@ int wkg_destroy(wkg_ctx *ctx) {
   return WKG_OK;
 }
 
-const char *wkg_last_error(const wkg_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char *wkg_last_error(const wkg_ctx *ctx) { return ctx ? wkg_ctx::err_slot().c_str() : "null context"; }
 
 int wkg_table_from_host(wkg_ctx *ctx, const int32_t *host, uint64_t rows, int cols, void *stream,
                         wkg_table **out) {
@@ -861,6 +892,42 @@ int wkg_set_probe_tile_rows(wkg_ctx *ctx, int rows) {
   if (rows != 0 && rows != 256 && rows != 512 && rows != 1024 && rows != 2048)
     return ctx->fail(WKG_EINVAL, "probe tile rows must be 0, 256, 512, 1024 or 2048 (got %d)", rows);
   ctx->probe_tile = rows;
+  return WKG_OK;
+}
+
+int wkg_thread_arena_begin(wkg_ctx *ctx, uint64_t bytes) {
+  if (!ctx || bytes < 4096) return WKG_EINVAL;
+  if (thread_arena()) return ctx->fail(WKG_ESTATE, "this thread already has an arena");
+  hipStream_t s = ctx->pick(nullptr);
+  char *base = (char *)ctx->pool->alloc(bytes, s, wkg_ctx::err_slot());
+  if (!base) return WKG_ENOMEM;
+  thread_arena() = new BumpArena{base, bytes, 0};
+  return WKG_OK;
+}
+
+int wkg_thread_arena_reset(wkg_ctx *ctx) {
+  if (!ctx) return WKG_EINVAL;
+  BumpArena *b = thread_arena();
+  if (!b) return ctx->fail(WKG_ESTATE, "this thread has no arena");
+  HIPCHK(ctx, hipStreamSynchronize(ctx->pick(nullptr)));  // (a failed query may have left work queued)
+  b->used = 0;
+  return WKG_OK;
+}
+
+int wkg_thread_arena_end(wkg_ctx *ctx) {
+  if (!ctx) return WKG_EINVAL;
+  BumpArena *b = thread_arena();
+  if (!b) return ctx->fail(WKG_ESTATE, "this thread has no arena");
+  thread_arena() = nullptr;
+  ctx->pool->release(b->base, ctx->pick(nullptr));
+  delete b;
+  return WKG_OK;
+}
+
+int wkg_pool_stats(wkg_ctx *ctx, uint64_t *live_bytes, uint64_t *cached_bytes, uint64_t *mallocs,
+                   uint64_t *trims) {
+  if (!ctx) return WKG_EINVAL;
+  ctx->pool->stats(live_bytes, cached_bytes, mallocs, trims);
   return WKG_OK;
 }
 

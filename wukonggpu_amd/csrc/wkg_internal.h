@@ -18,6 +18,18 @@
 // budget. A freed block is reused only by later work on the stream that
 // freed it (stream order makes that safe without a synchronisation); blocks
 // cached for other streams are returned to HIP when the budget needs them.
+// A per-thread bump region (wkg_thread_arena_*): while it is set, the
+// calling thread's pool allocations are carved from it (no hipMalloc) and its
+// releases inside it are no-ops; a reset drops them all at once.
+struct BumpArena {
+  char *base = nullptr;
+  uint64_t cap = 0, used = 0;
+};
+inline BumpArena *&thread_arena() {
+  static thread_local BumpArena *a = nullptr;
+  return a;
+}
+
 struct Pool {
   struct Cached {
     void *ptr;
@@ -28,11 +40,13 @@ struct Pool {
   void *alloc(size_t bytes, hipStream_t s, std::string &err);
   void release(void *p, hipStream_t s);
   void trim();
+  void stats(uint64_t *live, uint64_t *cached_b, uint64_t *nmalloc, uint64_t *ntrim);
   uint64_t budget;
   uint64_t live_bytes = 0, cached_bytes = 0, peak_bytes = 0;
+  uint64_t mallocs = 0, trims = 0;  // hipMalloc calls; cache drops (each a device synchronisation)
 
  private:
-  void trim_locked();
+  void trim_locked(uint64_t need);
   std::mutex mu;
   std::multimap<size_t, Cached> cached;
   std::unordered_map<void *, size_t> live;
@@ -93,7 +107,12 @@ struct wkg_ctx {
   int probe_tile = 0;        // rows per probe tile of the per-step calls (0: by input size)
   bool lineage = true;       // probe inherited columns once per parent row
   uint64_t *h_rb = nullptr;  // pinned read-back words
-  std::string err;
+  // The calling thread's last failure message (wkg_last_error): the served
+  // path drives one context from two host threads, so it is kept per thread.
+  static std::string &err_slot() {
+    static thread_local std::string s;
+    return s;
+  }
 
   // graph
   bool loaded = false, framed = false, sorted = false;

@@ -108,6 +108,12 @@ class Device:
         """Rows per probe tile of the per-step path: 256, 512, 1024, 2048, or 0 (by input size)."""
         self._check(N.hip_lib().wkg_set_probe_tile_rows(self._h, int(rows)))
 
+    def pool_stats(self):
+        """Table pool: live and cached bytes, hipMalloc calls, cache drops."""
+        v = [C.c_uint64() for _ in range(4)]
+        self._check(N.hip_lib().wkg_pool_stats(self._h, *[C.byref(x) for x in v]))
+        return {k: int(x.value) for k, x in zip(("live_bytes", "cached_bytes", "mallocs", "trims"), v)}
+
     def set_lineage(self, on):
         """Probe a star's next arm once per input row of the fan-out before it (default on)."""
         self._check(N.hip_lib().wkg_set_lineage(self._h, 1 if on else 0))
@@ -289,6 +295,21 @@ class GPUEngine:
         rc = L.wke_configure(self._h, max_queries, max_patterns, arena_bytes)
         if rc != 0:
             raise DeviceError(rc, "wke_configure failed")
+
+    def set_heavy_mode(self, mode):
+        """Heavy served queries: -1 auto, 0 a heavy executor lane, 1 the per-step worker thread."""
+        L = N.engine_lib()
+        rc = L.wke_set_heavy_mode(self._h, int(mode))
+        if rc != 0:
+            raise DeviceError(rc, "wke_set_heavy_mode failed")
+
+    def serve_stats(self):
+        """Counters of the last serve(): fallbacks, solo retries, worker queries, worker backlog."""
+        L = N.engine_lib()
+        v = [C.c_uint64() for _ in range(4)]
+        L.wke_serve_stats(self._h, *[C.byref(x) for x in v])
+        return {k: int(x.value) for k, x in zip(("plain_fallbacks", "solo_retries", "worker_queries",
+                                                  "worker_backlog"), v)}
 
     def set_tuning(self, heavy=False, **shape):
         """Pins the executor's launch and tile shape (the keys of autotune's result)."""
