@@ -91,6 +91,9 @@ def main():
     w = workloads.config2_3(seed=args.seed)
     log(f"graph: {w.info['triples']} triples, {w.info['keys']} keys, image {w.info['image_bytes'] / 1e9:.2f} GB "
         f"built in {time.time() - t0:.1f}s")
+    device_build = None
+    if not (args.heavy_only or args.probe_only or args.mix_only):
+        device_build = compare_device_build(w, Device)
     dev = Device(0)
     t1 = time.time()
     dev.load(w.gstore)
@@ -340,6 +343,7 @@ def main():
             "traffic_source": traffic_src, "copy_kernel_gbs": copy_gbs,
         },
         "probe_roofline": probe_roofline,
+        "device_build": device_build,
         "cpu_baseline": cpu,
         "batched": {
             "workload": f"{len(mix)} selective 3-4 pattern queries from type indexes (sizes log-uniform 1..1e5)",
@@ -388,6 +392,29 @@ def _line(metric, value, unit, steps, warmup, ms, config, extra, higher=True, dt
            "data": "synthetic (seeded generator; graph resident in HBM)", "config": config}
     out.update(extra)
     print(json.dumps(out), flush=True)
+
+
+def compare_device_build(w, Device, main_load=0.75):
+    """Builds the same graph on the device (wkg_build_graph) in a second
+    context, times it against the host builder (all host threads) and checks
+    the two images are equal, slot for slot."""
+    import numpy as np
+    from wukonggpu_amd import GStore
+    t = time.perf_counter()
+    gs = GStore.build(w.triples, main_load=main_load)
+    host_s = time.perf_counter() - t
+    d2 = Device(0)
+    dev_s = d2.build(w.triples, main_load=main_load)
+    slots, edges, metas = d2.image()
+    same = bool(np.array_equal(metas, gs.metas()) and np.array_equal(edges, gs.edges()) and
+                np.array_equal(slots, gs.slots().reshape(-1, 2)))
+    d2.close()
+    gs.close()
+    n = len(w.triples)
+    log(f"device build: {n} triples in {dev_s:.3f} s ({n / dev_s / 1e6:.0f} M triples/s, upload included); host "
+        f"builder {host_s:.2f} s on {os.cpu_count()} threads; images identical={same}")
+    return {"triples": n, "device_s": dev_s, "host_s": host_s, "host_threads": os.cpu_count(),
+            "device_triples_per_s": n / dev_s, "image_equal_host": same}
 
 
 def random_key_probe(dev, w, seed, line_peak):

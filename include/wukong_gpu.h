@@ -85,6 +85,19 @@ int wkg_load_graph(wkg_ctx *ctx, const uint64_t *slots, uint64_t num_slots,
  * whole image is resident. */
 int wkg_ensure_predicate(wkg_ctx *ctx, int pid);
 
+/* Builds the gstore image on the device from n ID triples (host arrays, as
+ * read by wks_triples_read_file) and makes it the context's resident graph:
+ * the same image, slot for slot, that wks_build makes with the same
+ * main_load (0: 0.75) -- the reference builds on the host (core/gstore.hpp:
+ * 600-659, 665-870, 957-978; core/dgraph.hpp:268-330). *build_s (may be
+ * NULL) gets the build time, upload included. */
+int wkg_build_graph(wkg_ctx *ctx, const uint32_t *s, const uint32_t *p, const uint32_t *o, uint64_t n,
+                    double main_load, double *build_s);
+/* Sizes of the loaded image, and (resident graphs) a copy of it to the host:
+ * num_slots x 2 u64, num_edges u32, num_metas pred metas. */
+int wkg_graph_shape(const wkg_ctx *ctx, uint64_t *num_slots, uint64_t *num_edges, int *num_metas);
+int wkg_graph_download(wkg_ctx *ctx, uint64_t *slots, uint32_t *edges, wkg_pred_meta_abi *metas);
+
 /* Graph residency statistics: bytes resident, frame loads, evictions. */
 int wkg_graph_stats(const wkg_ctx *ctx, uint64_t *resident_bytes,
                     uint64_t *segment_loads, uint64_t *segment_evictions);

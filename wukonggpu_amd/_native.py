@@ -129,6 +129,9 @@ def hip_lib():
         _sig(L, "wkg_set_table_limit", C.c_int, vp, u64)
         _sig(L, "wkg_set_probe_tile_rows", C.c_int, vp, C.c_int)
         _sig(L, "wkg_set_lineage", C.c_int, vp, C.c_int)
+        _sig(L, "wkg_build_graph", C.c_int, vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32), u64, dbl, P(dbl))
+        _sig(L, "wkg_graph_shape", C.c_int, vp, P(u64), P(u64), P(C.c_int))
+        _sig(L, "wkg_graph_download", C.c_int, vp, vp, vp, vp)
         _sig(L, "wkg_pool_stats", C.c_int, vp, P(u64), P(u64), P(u64), P(u64))
         _sig(L, "wkg_step_i2u", C.c_int, vp, C.c_int, C.c_int, vp, P(vp), P(u64))
         _sig(L, "wkg_step_c2u", C.c_int, vp, i64, C.c_int, C.c_int, vp, P(vp), P(u64))

@@ -265,41 +265,36 @@ static int validate_host(wkg_ctx *c, const uint64_t *slots, const uint32_t *edge
 
 extern "C" {
 
-int wkg_load_graph(wkg_ctx *ctx, const uint64_t *slots, uint64_t num_slots, const uint32_t *edges,
-                   uint64_t num_edges, const wkg_pred_meta_abi *metas, int num_metas, uint64_t graph_budget_bytes,
-                   uint32_t flags) {
-  if (!ctx) return WKG_EINVAL;
-  if (ctx->loaded) return ctx->fail(WKG_ESTATE, "a graph is already loaded");
-  if (!slots || !metas || num_metas < 2 || (num_edges && !edges))
-    return ctx->fail(WKG_EINVAL, "null image array");
-  if (num_slots == 0 || num_slots % kAssoc)
-    return ctx->fail(WKG_EINVAL, "num_slots must be a positive multiple of %d", kAssoc);
-  if (num_metas > (1 << kIdxBits)) return ctx->fail(WKG_EINVAL, "too many predicate ids");
-  if (num_slots / kAssoc > 0xffffffffull)  // the probe addresses buckets with 32 bits
-    return ctx->fail(WKG_ERANGE, "images are limited to 2^32 - 1 buckets");
-  (void)hipSetDevice(ctx->device);
-  ctx->num_slots = num_slots;
-  ctx->num_edges = num_edges;
-  ctx->num_buckets_total = num_slots / kAssoc;
-  ctx->num_metas = (uint32_t)num_metas;
-  ctx->metas.assign(metas, metas + num_metas);
-  uint64_t num_main = 0;
-  for (int p = 0; p < num_metas; p++) {
-    const wkg_pred_meta_abi &m = metas[p];
+// Checks an image's sizes and pred metas and derives the device metas
+// (first half of a graph load, shared with the device builder).
+int wkg_ctx::set_image_metas(uint64_t nslots, uint64_t nedges, const wkg_pred_meta_abi *m_in, int nmetas) {
+  if (nslots == 0 || nslots % kAssoc)
+    return fail(WKG_EINVAL, "num_slots must be a positive multiple of %d", kAssoc);
+  if (nmetas > (1 << kIdxBits)) return fail(WKG_EINVAL, "too many predicate ids");
+  if (nslots / kAssoc > 0xffffffffull)  // the probe addresses buckets with 32 bits
+    return fail(WKG_ERANGE, "images are limited to 2^32 - 1 buckets");
+  num_slots = nslots;
+  num_edges = nedges;
+  num_buckets_total = nslots / kAssoc;
+  num_metas = (uint32_t)nmetas;
+  metas.assign(m_in, m_in + nmetas);
+  uint64_t nmain = 0;
+  for (int p = 0; p < nmetas; p++) {
+    const wkg_pred_meta_abi &m = m_in[p];
     if (!m.partition_sz) continue;
-    if (m.main_hdr_end - m.main_hdr_start != m.partition_sz || m.main_hdr_end > ctx->num_buckets_total ||
-        m.indrct_hdr_start > m.indrct_hdr_end || m.indrct_hdr_end > ctx->num_buckets_total ||
-        m.edge_start > m.edge_end || m.edge_end > num_edges || m.edge_end - m.edge_start >= (1ull << 32)) {
-      ctx->metas.clear();
-      return ctx->fail(WKG_EINVAL, "pred_meta %d is inconsistent with the image", p);
+    if (m.main_hdr_end - m.main_hdr_start != m.partition_sz || m.main_hdr_end > num_buckets_total ||
+        m.indrct_hdr_start > m.indrct_hdr_end || m.indrct_hdr_end > num_buckets_total ||
+        m.edge_start > m.edge_end || m.edge_end > nedges || m.edge_end - m.edge_start >= (1ull << 32)) {
+      metas.clear();
+      return fail(WKG_EINVAL, "pred_meta %d is inconsistent with the image", p);
     }
-    num_main = std::max(num_main, m.main_hdr_end);
+    nmain = std::max(nmain, m.main_hdr_end);
   }
-  ctx->num_main = num_main;
-  ctx->h_meta.assign(num_metas, DevMeta{});
-  for (int p = 0; p < num_metas; p++) {
-    const wkg_pred_meta_abi &m = metas[p];
-    DevMeta &d = ctx->h_meta[p];
+  num_main = nmain;
+  h_meta.assign(nmetas, DevMeta{});
+  for (int p = 0; p < nmetas; p++) {
+    const wkg_pred_meta_abi &m = m_in[p];
+    DevMeta &d = h_meta[p];
     d.main_start = m.main_hdr_start;
     d.psz = m.partition_sz;
     d.magic = m.partition_sz ? fastmod_magic(m.partition_sz) : 0;
@@ -307,13 +302,72 @@ int wkg_load_graph(wkg_ctx *ctx, const uint64_t *slots, uint64_t num_slots, cons
     d.edge_start = m.edge_start;
     d.edge_len = m.edge_end - m.edge_start;
   }
+  return WKG_OK;
+}
+
+// Second half of a resident load, with d_slots and d_edges in place: device
+// metas, the image checks on the device, and the table budget.
+int wkg_ctx::finish_resident(uint32_t flags, hipStream_t s) {
+  if (hipMalloc(&d_meta, sizeof(DevMeta) * num_metas) != hipSuccess) {
+    (void)hipGetLastError();
+    free_graph();
+    return fail(WKG_ENOMEM, "cannot allocate the predicate metadata");
+  }
+  hipError_t e = hipMemcpy(d_meta, h_meta.data(), sizeof(DevMeta) * num_metas, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    free_graph();
+    return fail(WKG_EDEVICE, "metadata upload failed: %s", hipGetErrorString(e));
+  }
+  const uint64_t nwords = (num_buckets_total + 31) / 32;
+  uint32_t *tmp = nullptr;
+  if (hipMalloc(&tmp, (nwords + 1) * 4) != hipSuccess) {
+    free_graph();
+    return fail(WKG_ENOMEM, "cannot allocate validation bitmap");
+  }
+  (void)hipMemsetAsync(tmp, 0, (nwords + 1) * 4, s);
+  const int check_sorted = (flags & WKG_LOAD_SORTED_LISTS) ? 0 : 1;
+  const uint64_t blocks = std::min<uint64_t>((num_slots + 255) / 256, 1u << 16);
+  k_validate<<<(unsigned)blocks, 256, 0, s>>>(d_slots, num_buckets_total, num_main, num_edges, d_meta, num_metas,
+                                               d_edges, tmp + 1, check_sorted, tmp);
+  uint32_t vflags = 0;
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(&vflags, tmp, 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) {
+    free_graph();
+    return fail(WKG_EDEVICE, "image validation failed: %s", hipGetErrorString(e));
+  }
+  if (vflags & 3u) {
+    free_graph();
+    return fail(WKG_EINVAL, "corrupt image (%s)", (vflags & 1u) ? "list outside its segment" : "bad chain link");
+  }
+  sorted = (flags & WKG_LOAD_SORTED_LISTS) || !(vflags & 4u);
+  resident_bytes = num_slots * 16 + std::max<uint64_t>(num_edges, 1) * 4;
+  loaded = true;
+  if (!budget_request) {  // default table budget: 75% of what is left
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess) pool->budget = (uint64_t)(fr * 0.75);
+  }
+  return WKG_OK;
+}
+
+int wkg_load_graph(wkg_ctx *ctx, const uint64_t *slots, uint64_t num_slots, const uint32_t *edges,
+                   uint64_t num_edges, const wkg_pred_meta_abi *metas, int num_metas, uint64_t graph_budget_bytes,
+                   uint32_t flags) {
+  if (!ctx) return WKG_EINVAL;
+  if (ctx->loaded) return ctx->fail(WKG_ESTATE, "a graph is already loaded");
+  if (!slots || !metas || num_metas < 2 || (num_edges && !edges))
+    return ctx->fail(WKG_EINVAL, "null image array");
+  (void)hipSetDevice(ctx->device);
+  if (int rc = ctx->set_image_metas(num_slots, num_edges, metas, num_metas)) return rc;
+  const uint64_t num_main = ctx->num_main;
   const uint64_t slot_bytes = num_slots * 16, edge_bytes = std::max<uint64_t>(num_edges, 1) * 4;
   const bool framed = graph_budget_bytes && graph_budget_bytes < slot_bytes + edge_bytes;
   hipStream_t s = ctx->stream;
 
   if (!framed) {
-    if (hipMalloc(&ctx->d_slots, slot_bytes) != hipSuccess || hipMalloc(&ctx->d_edges, edge_bytes) != hipSuccess ||
-        hipMalloc(&ctx->d_meta, sizeof(DevMeta) * num_metas) != hipSuccess) {
+    if (hipMalloc(&ctx->d_slots, slot_bytes) != hipSuccess || hipMalloc(&ctx->d_edges, edge_bytes) != hipSuccess) {
       (void)hipGetLastError();
       ctx->free_graph();
       return ctx->fail(WKG_ENOMEM, "cannot allocate %llu bytes for the resident image",
@@ -321,40 +375,11 @@ int wkg_load_graph(wkg_ctx *ctx, const uint64_t *slots, uint64_t num_slots, cons
     }
     hipError_t e = hipMemcpy(ctx->d_slots, slots, slot_bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess && num_edges) e = hipMemcpy(ctx->d_edges, edges, num_edges * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-      e = hipMemcpy(ctx->d_meta, ctx->h_meta.data(), sizeof(DevMeta) * num_metas, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
       ctx->free_graph();
       return ctx->fail(WKG_EDEVICE, "image upload failed: %s", hipGetErrorString(e));
     }
-    // validate on the device
-    const uint64_t nwords = (ctx->num_buckets_total + 31) / 32;
-    uint32_t *tmp = nullptr;
-    if (hipMalloc(&tmp, (nwords + 1) * 4) != hipSuccess) {
-      ctx->free_graph();
-      return ctx->fail(WKG_ENOMEM, "cannot allocate validation bitmap");
-    }
-    (void)hipMemsetAsync(tmp, 0, (nwords + 1) * 4, s);
-    const int check_sorted = (flags & WKG_LOAD_SORTED_LISTS) ? 0 : 1;
-    const uint64_t blocks = std::min<uint64_t>((num_slots + 255) / 256, 1u << 16);
-    k_validate<<<(unsigned)blocks, 256, 0, s>>>(ctx->d_slots, ctx->num_buckets_total, num_main, num_edges,
-                                                 ctx->d_meta, ctx->num_metas, ctx->d_edges, tmp + 1,
-                                                 check_sorted, tmp);
-    uint32_t vflags = 0;
-    e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(&vflags, tmp, 4, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    (void)hipFree(tmp);
-    if (e != hipSuccess) {
-      ctx->free_graph();
-      return ctx->fail(WKG_EDEVICE, "image validation failed: %s", hipGetErrorString(e));
-    }
-    if (vflags & 3u) {
-      ctx->free_graph();
-      return ctx->fail(WKG_EINVAL, "corrupt image (%s)", (vflags & 1u) ? "list outside its segment" : "bad chain link");
-    }
-    ctx->sorted = (flags & WKG_LOAD_SORTED_LISTS) || !(vflags & 4u);
-    ctx->resident_bytes = slot_bytes + edge_bytes;
+    return ctx->finish_resident(flags, s);
   } else {
     bool srt = true;
     int rc = validate_host(ctx, slots, edges, !(flags & WKG_LOAD_SORTED_LISTS), &srt);

@@ -19,6 +19,7 @@
 // the overflow goes to linked indirect buckets; indirect buckets of an epoch
 // are numbered in main-bucket order.
 #include "gstore_format.h"
+#include "gstore_layout.h"
 #include "../../include/wukong_gstore.h"
 
 #include <omp.h>
@@ -371,16 +372,7 @@ void wks_triples_free(wks_triples *t) { delete t; }
 
 namespace {
 
-// One insertion epoch: a contiguous run of keys sharing a main partition.
-struct Epoch {
-  uint32_t id;        // predicate or type id whose partition the keys use
-  int kind;           // 0: rdf:type (s,1,OUT) keys, 1: type index, 2: predicate
-  uint64_t edge_base; // first edge of this epoch
-  uint64_t nedges;
-  uint64_t nkeys;
-  uint64_t ext_count = 0;  // indirect buckets the epoch needs
-  uint64_t ext_base = 0;   // its first indirect bucket (relative to num_buckets)
-};
+using wkg::Epoch;
 
 struct Grouped {
   // per predicate id: [so_off[p], so_off[p+1]) of (s<<32|o), deduplicated
@@ -586,53 +578,24 @@ int wks_build(const wks_triples *t, const wks_build_params *bp, wks_gstore **out
         return fail(WKS_EINVAL, "id %u is used both as a predicate and as a type", p);
   }
 
-  // ---- partition sizes (core/gstore.hpp:618-651) ------------------------------
-  std::vector<uint64_t> part_cnt(num_metas, 0);
-  uint64_t num_edges_count = 0, num_keys = 0;
-  for (uint32_t p = 1; p < num_metas; p++) {
-    part_cnt[p] = g.so_cnt[p] + g.os_cnt[p] + type_cnt[p];
-    num_edges_count += part_cnt[p];
-    num_keys += g.nsubj[p] + g.nobj[p];
-    if (p >= 2 && g.so_cnt[p]) num_keys++;
-    if (p >= 2 && g.os_cnt[p]) num_keys++;
-    if (type_cnt[p]) num_keys++;
+  // ---- partition sizes and epochs (core/gstore.hpp:618-651) -----------------
+  Layout lay;
+  {
+    IdCounts cnt;
+    cnt.so_cnt = g.so_cnt;
+    cnt.os_cnt = g.os_cnt;
+    cnt.nsubj = g.nsubj;
+    cnt.nobj = g.nobj;
+    cnt.type_cnt = type_cnt;
+    plan_layout(cnt, num_metas, load, lay);
   }
-  // rdf:type triples count once for pid 1 (so_cnt) and once for their type
-  // (type_cnt), as core/gstore.hpp:619-626 does
-  const uint64_t num_buckets = std::max<uint64_t>(1, (uint64_t)std::ceil((double)num_keys / (7.0 * load)));
-
   wks_gstore *G = new (std::nothrow) wks_gstore;
   if (!G) return fail(WKS_ENOMEM, "out of memory");
-  G->num_buckets = num_buckets;
-  G->num_keys = num_keys;
-  G->metas.assign(num_metas, wkg_pred_meta{});
-  uint64_t main_cnt = 0;
-  for (uint32_t p = 1; p < num_metas; p++) {
-    uint64_t b = num_edges_count ? (uint64_t)((double)part_cnt[p] / (double)num_edges_count * (double)num_buckets) : 0;
-    if (b == 0 && part_cnt[p] > 0) b = 1;
-    G->metas[p].partition_sz = b;
-    G->metas[p].main_hdr_start = main_cnt;
-    G->metas[p].main_hdr_end = main_cnt + b;
-    main_cnt += b;
-  }
-  // the "at least one bucket" rule can push the sum past num_buckets
-  if (main_cnt > num_buckets) G->num_buckets = main_cnt;
-
-  // ---- epochs ------------------------------------------------------------------
-  std::vector<Epoch> ep;
-  if (g.so_cnt[kTypeId]) ep.push_back(Epoch{(uint32_t)kTypeId, 0, 0, g.so_cnt[kTypeId], g.nsubj[kTypeId]});
-  for (uint32_t p = 2; p < num_metas; p++)
-    if (type_cnt[p]) ep.push_back(Epoch{p, 1, 0, type_cnt[p], 1});
-  for (uint32_t p = 2; p < num_metas; p++)
-    if (g.so_cnt[p] || g.os_cnt[p]) {
-      uint64_t nk = g.nsubj[p] + g.nobj[p] + (g.so_cnt[p] ? 1 : 0) + (g.os_cnt[p] ? 1 : 0);
-      ep.push_back(Epoch{p, 2, 0, g.so_cnt[p] + g.os_cnt[p] + g.nsubj[p] + g.nobj[p], nk});
-    }
-  uint64_t total_edges = 0;
-  for (auto &e : ep) {
-    e.edge_base = total_edges;
-    total_edges += e.nedges;
-  }
+  G->num_buckets = lay.num_buckets;
+  G->num_keys = lay.num_keys;
+  G->metas = lay.metas;
+  std::vector<Epoch> &ep = lay.ep;
+  const uint64_t total_edges = lay.total_edges;
   try {
     G->edges.resize(total_edges);
   } catch (...) {
@@ -654,15 +617,12 @@ int wks_build(const wks_triples *t, const wks_build_params *bp, wks_gstore **out
                  [&](uint64_t key, uint64_t, uint64_t) { cnt[fastmod(mix64(key), psz, mag)]++; });
       uint64_t ext = 0;
       for (uint64_t b = 0; b < psz; b++)
-        if (cnt[b] > 7) ext += (cnt[b] - 7 + 6) / 7;
+        ext += chain_buckets(cnt[b]);
       e.ext_count = ext;
     }
   }
-  uint64_t ext_total = 0;
-  for (auto &e : ep) {
-    e.ext_base = ext_total;
-    ext_total += e.ext_count;
-  }
+  const uint64_t ext_total = place_epochs(lay);
+  G->metas = lay.metas;
   G->num_buckets_ext = ext_total;
   const uint64_t nb_all = G->num_buckets + ext_total;
   try {
@@ -681,7 +641,7 @@ int wks_build(const wks_triples *t, const wks_build_params *bp, wks_gstore **out
 #pragma omp for schedule(dynamic, 1)
     for (size_t k = 0; k < ep.size(); k++) {
       const Epoch &e = ep[k];
-      wkg_pred_meta &m = G->metas[e.id];
+      const wkg_pred_meta &m = G->metas[e.id];
       const uint64_t psz = m.partition_sz, mag = fastmod_magic(psz);
       cur.assign(psz, 0);
       walk_epoch(e, g, type_off, type_members, nullptr,
@@ -690,12 +650,12 @@ int wks_build(const wks_triples *t, const wks_build_params *bp, wks_gstore **out
       uint64_t next = G->num_buckets + e.ext_base;
       for (uint64_t b = 0; b < psz; b++) {
         chain[b] = next;
-        if (cur[b] > 7) next += (cur[b] - 7 + 6) / 7;
+        next += chain_buckets(cur[b]);
       }
       // link chains: main bucket -> first indirect -> ... -> last (link 0)
       for (uint64_t b = 0; b < psz; b++) {
-        if (cur[b] <= 7) continue;
-        const uint64_t nchain = (cur[b] - 7 + 6) / 7;
+        const uint64_t nchain = chain_buckets(cur[b]);
+        if (!nchain) continue;
         uint64_t prev = m.main_hdr_start + b;
         for (uint64_t j = 0; j < nchain; j++) {
           G->slots[(prev * kAssoc + 7) * 2] = make_key(chain[b] + j, 0, 0);
@@ -719,10 +679,6 @@ int wks_build(const wks_triples *t, const wks_build_params *bp, wks_gstore **out
         sp[0] = key;
         sp[1] = make_ptr(size & kSizeMask, e.edge_base + off);
       });
-      m.indrct_hdr_start = G->num_buckets + e.ext_base;
-      m.indrct_hdr_end = G->num_buckets + e.ext_base + e.ext_count;
-      m.edge_start = e.edge_base;
-      m.edge_end = e.edge_base + e.nedges;
     }
   }
   if (too_big) {

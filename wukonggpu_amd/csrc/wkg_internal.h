@@ -167,6 +167,8 @@ struct wkg_ctx {
   int step_k2u_lineage(const wkg_table *in, int sc, int pid, int dir, hipStream_t s, wkg_table **out,
                        uint64_t *expanded);
   // residency
+  int set_image_metas(uint64_t nslots, uint64_t nedges, const wkg_pred_meta_abi *m_in, int nmetas);
+  int finish_resident(uint32_t flags, hipStream_t s);
   int ensure_pred(int pid, hipStream_t s, const std::vector<int> *keep = nullptr);
   void free_graph();
 };

@@ -84,6 +84,30 @@ class Device:
                                      gstore.metas_ptr(), gstore.num_metas, graph_budget_bytes,
                                      1 if sorted_lists else 0))
 
+    def build(self, triples, main_load=0.75):
+        """Builds the gstore image on the device from ID triples (the image
+        GStore.build makes, slot for slot) and keeps it resident. Returns the
+        build time in seconds, upload included."""
+        s, p, o = triples.arrays()
+        s, p, o = (np.ascontiguousarray(x, dtype=np.uint32) for x in (s, p, o))
+        t = C.c_double()
+        u32p = C.POINTER(C.c_uint32)
+        self._check(N.hip_lib().wkg_build_graph(self._h, s.ctypes.data_as(u32p), p.ctypes.data_as(u32p),
+                                                 o.ctypes.data_as(u32p), len(s), float(main_load), C.byref(t)))
+        return float(t.value)
+
+    def image(self):
+        """The resident image as host arrays: (slots (n, 2) u64, edges u32, metas (m, 8) u64)."""
+        L = N.hip_lib()
+        ns, ne, nm = C.c_uint64(), C.c_uint64(), C.c_int()
+        self._check(L.wkg_graph_shape(self._h, C.byref(ns), C.byref(ne), C.byref(nm)))
+        slots = np.empty((ns.value, 2), dtype=np.uint64)
+        edges = np.empty(ne.value, dtype=np.uint32)
+        metas = np.empty((nm.value, 8), dtype=np.uint64)
+        self._check(L.wkg_graph_download(self._h, slots.ctypes.data_as(C.c_void_p), edges.ctypes.data_as(C.c_void_p),
+                                         metas.ctypes.data_as(C.c_void_p)))
+        return slots, edges, metas
+
     def profile(self, on=True):
         """Turns per-kernel HIP-event timing on (clearing totals) or off."""
         self._check(N.hip_lib().wkg_profile_enable(self._h, 1 if on else 0))
