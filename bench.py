@@ -58,6 +58,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--served-n", type=int, default=2000, help="queries per served rate (0: skip)")
     ap.add_argument("--served-rates", default="1000,2000,4000", help="offered Poisson rates, queries/s")
+    ap.add_argument("--batch-s", type=float, default=2.0,
+                    help="closed-loop batch mode: seconds per in-flight cap, a quarter of it warmup (0: skip)")
+    ap.add_argument("--batch-p", default="20,200,1000",
+                    help="closed-loop batch mode: light in-flight caps (heavy cap min(p, 50))")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the closed-loop batch mode on the 100M-triple graph")
     ap.add_argument("--config", default="2", choices=["1", "2", "4", "5"],
                     help="2: heavy query + batched mix + served sweep on the 100M-triple graph (default); "
                          "1: plain single query on the 10M-triple graph; 4: served sweep on the 1B-triple "
@@ -72,7 +78,7 @@ def main():
                     help="with --mix-only: the executor shape as key=value pairs (probe_wg_per_cu=3,...) "
                          "instead of autotuning")
     args = ap.parse_args()
-    if args.heavy_only or args.probe_only or args.mix_only:
+    if args.heavy_only or args.probe_only or args.mix_only or args.batch_only:
         args.no_cpu = True
     if args.gpus != 1:
         print(f"bench.py measures one GPU's shard; --gpus {args.gpus} is not supported "
@@ -92,7 +98,7 @@ def main():
     log(f"graph: {w.info['triples']} triples, {w.info['keys']} keys, image {w.info['image_bytes'] / 1e9:.2f} GB "
         f"built in {time.time() - t0:.1f}s")
     device_build = None
-    if not (args.heavy_only or args.probe_only or args.mix_only):
+    if not (args.heavy_only or args.probe_only or args.mix_only or args.batch_only):
         device_build = compare_device_build(w, Device)
     dev = Device(0)
     t1 = time.time()
@@ -102,6 +108,15 @@ def main():
     if args.probe_only:
         rk = random_key_probe(dev, w, args.seed, HBM_PEAK_GBS * 1e9 / 128)
         print(json.dumps({"probe_random_keys": rk}), flush=True)
+        eng.close()
+        dev.close()
+        return
+    if args.batch_only:
+        P = w.info["predicates"]
+        types = list(range(2 + P + 1, 2 + P + w.info["types"]))
+        eng.configure(max_queries=1024, max_patterns=8, arena_bytes=16 << 30)
+        bm = batch_mode(eng, w.gstore, P, types, w.info["t0"], args)
+        print(json.dumps({"batch_mode": bm}), flush=True)
         eng.close()
         dev.close()
         return
@@ -290,11 +305,18 @@ def main():
                 f"p99 {rec['p99_ms']:.2f} ms (light p99 {rec['light_p99_ms']:.2f}, heavy p99 "
                 f"{rec['heavy_p99_ms']:.2f}), identical={ok}; {rec['serve_stats']}, pool {rec['pool']}")
 
+    # ---- closed-loop batch mode: templates with random constants, in-flight caps -------------
+    bm = None
+    if args.batch_s > 0:
+        P = w.info["predicates"]
+        bm = batch_mode(eng, w.gstore, P, list(range(2 + P + 1, 2 + P + w.info["types"])), w.info["t0"], args)
+
     # ---- CPU baseline (the oracle), parity of the heavy result --------------------------
     cpu = None
     parity = {"heavy_rows_equal": None, "heavy_executor_equals_plain": heavy_exec_same,
               "mix_plain_equals_executor": bool(same_exec), "mix_plain_equals_rounds": bool(same_rounds),
-              "served_equals_alone": served_same}
+              "served_equals_alone": served_same,
+              "batch_mode_equals_alone": bm["equals_alone"] if bm else None}
     if not args.no_cpu:
         import oracle
         want, wex, sec = oracle.run_query(w.gstore, heavy)
@@ -360,11 +382,86 @@ def main():
         },
         "heavy_via_executor_ms": pct(hx_lat, 0.5) * 1e3, "heavy_tuning": heavy_tuning,
         "served": served,
+        "batch_mode": bm,
         "parity": parity,
     }
     print(json.dumps(line), flush=True)
     eng.close()
     dev.close()
+
+
+# Latency points of the reference logger's per-query-type CDF (core/logger.hpp:250-313).
+CDF_RATES = [0.01] + [0.05 * i for i in range(1, 20)] + [0.95 + 0.01 * i for i in range(1, 6)]
+
+
+def batch_mode(eng, gs, P, light_types, t0, args, budget=64 << 30):
+    """Closed-loop batch mode (core/proxy.hpp:287-457) at each light in-flight
+    cap p of --batch-p, the heavy cap min(p, 50) (global_heavy_flying_cnt of
+    scripts/config), --batch-s seconds with the first quarter warmup. Per cap,
+    as the reference's Logger reports it: light and heavy queries completed
+    per second after the warmup, light and heavy median and 99th latency
+    (sorted[n/2], sorted[int(0.99 n)], core/logger.hpp:196-238), and the
+    25-point latency CDF per query type (core/logger.hpp:250-313). The first
+    2048 light and 64 heavy issued queries are checked against the same
+    query run alone on the per-step path."""
+    import numpy as np
+    from wukonggpu_amd import workloads
+    dur = args.batch_s
+    warm = dur / 4
+    caps = [int(x) for x in args.batch_p.split(",") if x]
+    n = 3_000_000
+    t = time.perf_counter()
+    flat, off, heavy, types = workloads.batch_stream(gs, P, light_types, t0, n, args.seed + 51)
+    log(f"batch mode: {n} queries from 6 light templates and 4 heavy queries in {time.perf_counter() - t:.1f}s")
+    light_pos, heavy_pos = np.nonzero(heavy == 0)[0], np.nonzero(heavy)[0]
+    keep = np.zeros(n, dtype=bool)
+    keep[light_pos[:2048]] = True
+    keep[heavy_pos[:64]] = True
+    kidx = np.nonzero(keep)[0]
+    kq = [flat[off[i]:off[i + 1]].tolist() for i in kidx]
+    alone = eng.run_plain(kq)
+    out, all_same = [], True
+    for p in caps:
+        hcap = min(p, 50)
+        res, issue, wall = eng.serve_closed((flat, off), heavy, p, hcap, dur, table_budget_bytes=budget,
+                                            light_lanes=2, keep=keep)
+        issued = issue >= 0
+        lat = np.array([r.latency_s for r in res])
+        done = issue + lat
+        st = np.array([r.status for r in res])
+        win = issued & (done >= warm) & (done <= dur)
+        ok = bool((st[issued] == 0).all())
+        for j, i in enumerate(kidx):
+            if issued[i]:
+                a, r = alone[j], res[i]
+                ok = ok and a.status == 0 and r.table.shape == a.table.shape and \
+                    np.array_equal(r.table, a.table) and r.expanded == a.expanded
+        all_same = all_same and ok
+        rec = {"light_inflight": p, "heavy_inflight": hcap, "wall_s": wall,
+               "issued": int(issued.sum()), "issued_heavy": int((issued & (heavy == 1)).sum()),
+               "stream_exhausted": bool(issued[light_pos].all()),
+               "light_qps": float((win & (heavy == 0)).sum() / (dur - warm)),
+               "heavy_qps": float((win & (heavy == 1)).sum() / (dur - warm)), "equals_alone": ok}
+        for cls, name in ((0, "light"), (1, "heavy")):
+            v = np.sort(lat[issued & (heavy == cls)])
+            rec[f"{name}_median_us"] = round(float(v[len(v) // 2] * 1e6), 1) if len(v) else None
+            rec[f"{name}_p99_us"] = round(float(v[int(len(v) * 0.99)] * 1e6), 1) if len(v) else None
+        cdf = {}
+        for k in range(10):
+            v = np.sort(lat[issued & (types == k)])
+            if len(v):
+                cdf[f"Q{k + 1}"] = [round(float(v[min(int(len(v) * r), len(v) - 1)] * 1e6), 1) for r in CDF_RATES]
+        rec["cdf_us"] = cdf
+        rec["cdf_rates"] = [round(r, 2) for r in CDF_RATES]
+        out.append(rec)
+        log(f"batch mode p={p} (heavy {hcap}): light {rec['light_qps']:.0f} q/s, heavy {rec['heavy_qps']:.0f} q/s; "
+            f"light median {rec['light_median_us']} us p99 {rec['light_p99_us']} us, heavy median "
+            f"{rec['heavy_median_us']} us p99 {rec['heavy_p99_us']} us; issued {rec['issued']} "
+            f"({rec['issued_heavy']} heavy), exhausted={rec['stream_exhausted']}, identical={ok}")
+        del res
+    return {"duration_s": dur, "warmup_s": warm, "runs": out, "equals_alone": all_same,
+            "templates": "6 light (constant start from a random type member: 2-hop, 3-arm star, 3-hop chain) and "
+                         "4 heavy (index start, 6 patterns), loads of the reference's batch mix_config"}
 
 
 def _same(xs, ys):
@@ -394,22 +491,26 @@ def _line(metric, value, unit, steps, warmup, ms, config, extra, higher=True, dt
     print(json.dumps(out), flush=True)
 
 
-def compare_device_build(w, Device, main_load=0.75):
+def compare_device_build(w, Device, main_load=0.75, host_gs=None, host_s=None):
     """Builds the same graph on the device (wkg_build_graph) in a second
-    context, times it against the host builder (all host threads) and checks
-    the two images are equal, slot for slot."""
+    context, times it against the host builder (all host threads; rebuilt
+    here unless its image and time are given) and checks the two images are
+    equal, slot for slot."""
     import numpy as np
     from wukonggpu_amd import GStore
-    t = time.perf_counter()
-    gs = GStore.build(w.triples, main_load=main_load)
-    host_s = time.perf_counter() - t
+    gs = host_gs
+    if gs is None:
+        t = time.perf_counter()
+        gs = GStore.build(w.triples, main_load=main_load)
+        host_s = time.perf_counter() - t
     d2 = Device(0)
     dev_s = d2.build(w.triples, main_load=main_load)
     slots, edges, metas = d2.image()
     same = bool(np.array_equal(metas, gs.metas()) and np.array_equal(edges, gs.edges()) and
                 np.array_equal(slots, gs.slots().reshape(-1, 2)))
     d2.close()
-    gs.close()
+    if host_gs is None:
+        gs.close()
     n = len(w.triples)
     log(f"device build: {n} triples in {dev_s:.3f} s ({n / dev_s / 1e6:.0f} M triples/s, upload included); host "
         f"builder {host_s:.2f} s on {os.cpu_count()} threads; images identical={same}")
@@ -498,9 +599,12 @@ def run_config4(args):
     import oracle
     from wukonggpu_amd import Device, GPUEngine, workloads
     t0 = time.time()
-    w = workloads.config4(seed=4)
+    w = workloads.config4(seed=4, keep_triples=True)
     log(f"graph: {w.info['triples']} triples, {w.info['keys']} keys, image {w.info['image_bytes'] / 1e9:.2f} GB "
         f"built in {time.time() - t0:.1f}s")
+    device_build = compare_device_build(w, Device, host_gs=w.gstore, host_s=w.info["host_build_s"])
+    w.triples.close()
+    w.triples = None
     dev = Device(0)
     dev.load(w.gstore)
     eng = GPUEngine(dev)
@@ -541,6 +645,7 @@ def run_config4(args):
         log(f"served @ {rate:.0f} q/s offered: {rec['achieved_qps']:.0f} q/s, p50 {rec['p50_ms']:.2f} ms, "
             f"p99 {rec['p99_ms']:.2f} ms (light p99 {rec['light_p99_ms']:.2f}, heavy p99 "
             f"{rec['heavy_p99_ms']:.2f}), identical={ok}")
+    bm = batch_mode(eng, w.gstore, P, w.info["light_types"], w.info["t0"], args) if args.batch_s > 0 else None
     # CPU oracle on a bounded sample: the light queries, and one heavy query
     samp = light[:256] + [sq[i] for i in heavy_idx[:1]]
     t = time.perf_counter()
@@ -561,9 +666,10 @@ def run_config4(args):
                        "1B-triple graph: 100M entities, 256 predicates, Zipf 1.0; 64 GB table budget",
            **{k: v for k, v in w.info.items() if k != "light_types"},
            "queries_per_rate": len(sq), "heavy_queries": len(heavy_idx)},
-          {"served": served, "plain_queries_per_s": len(sq) / t_plain,
+          {"served": served, "batch_mode": bm, "plain_queries_per_s": len(sq) / t_plain,
+           "device_build": device_build,
            "light_batched_queries_per_s": len(light) / t_light, "tuning": tuning, "heavy_tuning": htuning,
-           "parity": {"served_equals_alone": all_same,
+           "parity": {"served_equals_alone": all_same, "batch_mode_equals_alone": bm["equals_alone"] if bm else None,
                       "light_batched_equals_plain": _same([alone[i] for i in light_idx], light_b),
                       "cpu_sample_equal": bool(cpu_ok)},
            "cpu_baseline": {"value": len(samp) / cpu_s, "unit": "queries/s", "cores": 1, "kind": "port",

@@ -115,6 +115,18 @@ int wke_serve_stats(const wke_engine *e, uint64_t *fallbacks, uint64_t *solo, ui
 int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, const double *arrival_s,
               const uint8_t *heavy, uint64_t table_budget_bytes, int light_lanes, wke_result *res,
               double *wall_s);
+/* Closed-loop batch mode, the reference's batch driver (core/proxy.hpp:
+ * 287-457): the light queries are issued in order while fewer than
+ * light_inflight of them are in flight, the heavy ones while fewer than
+ * heavy_inflight are (its heavy in-flight cap), until duration_s has passed;
+ * then everything issued completes. Lanes, admission and the heavy worker
+ * are wke_serve's. issue_s[q] gets each query's issue time in seconds from
+ * the start (-1: not issued before the end, its result zeroed); latency_s is
+ * completion - issue (send to reply, as the reference's Logger records it).
+ * Queries are issued in order within their class. */
+int wke_serve_closed(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, const uint8_t *heavy,
+                     int light_inflight, int heavy_inflight, double duration_s, uint64_t table_budget_bytes,
+                     int light_lanes, wke_result *res, double *issue_s, double *wall_s);
 
 #ifdef __cplusplus
 }

@@ -194,6 +194,72 @@ def test_served_lanes_alone_equals_in_mix(eng_small, graph_small):
         _same(gs, q, r)
 
 
+def test_closed_loop_batch_mode(eng_small, graph_small):
+    """The reference's batch driver semantics: light and heavy queries issued
+    in order below per-class in-flight caps for a duration; every issued
+    query completes with the answer it gets alone, none is issued after the
+    duration, and the caps hold (a class never has more queries issued than
+    completed plus its cap)."""
+    _, gs = graph_small
+    rng = np.random.default_rng(27)
+    qs = random_queries(rng, 300, 4, [6, 7], 3000, max_steps=6)
+    heavy = np.array([len(q) >= 20 for q in qs], dtype=np.uint8)
+    eng_small.configure(max_queries=256, max_patterns=8, arena_bytes=1 << 30)
+    print("closed loop: caps 16 light, 2 heavy, 0.25 s", flush=True)
+    res, issue, wall = eng_small.serve_closed(qs, heavy, 16, 2, 0.25, table_budget_bytes=2 << 30, light_lanes=2)
+    issued = issue >= 0
+    assert issued.sum() > 0 and wall > 0
+    assert (issue[issued] < 0.25).all()
+    # the loop ends at the duration, or earlier once every query was issued
+    assert issued.all() or wall >= 0.25
+    # in order within a class: an issued query's class predecessors were issued
+    for cls in (0, 1):
+        f = issued[heavy == cls]
+        assert not (f[1:] & ~f[:-1]).any()
+    for q, r, f in zip(qs, res, issued):
+        if not f:
+            continue
+        alone = eng_small.run_query(q)
+        assert r.status == 0 and r.table.shape == alone.table.shape and np.array_equal(r.table, alone.table), q
+        assert r.expanded == alone.expanded and r.latency_s > 0
+    # in-flight caps: at any issue time, earlier-issued queries of the class
+    # that had not completed yet number fewer than the cap
+    done_t = issue + np.array([r.latency_s for r in res])
+    for cls, cap in ((0, 16), (1, 2)):
+        idx = np.where(issued & (heavy == cls))[0]
+        for i in idx:
+            others = idx[(issue[idx] < issue[i]) & (done_t[idx] > issue[i])]
+            assert len(others) < cap + 1, (cls, len(others))
+    # a duration must be positive
+    with pytest.raises(DeviceError):
+        eng_small.serve_closed(qs[:50], heavy[:50], 16, 2, 0.0, table_budget_bytes=2 << 30, light_lanes=2)
+
+
+def test_closed_loop_template_stream(eng_small, graph_small):
+    """The benchmark's batch-mode stream, packed, through the closed loop with
+    only some result tables kept: kept tables equal the query run alone, the
+    rest report the same row counts."""
+    from wukonggpu_amd import workloads
+    _, gs = graph_small
+    preds = gs.num_metas - 4  # predicates 2..preds+1, then the two types
+    t0 = 2 + preds
+    flat, off, heavy, types = workloads.batch_stream(gs, preds, [t0 + 1], t0, 4000, seed=9)
+    keep = np.zeros(len(heavy), dtype=bool)
+    keep[::3] = True
+    eng_small.configure(max_queries=256, max_patterns=8, arena_bytes=1 << 30)
+    print("closed loop over the template stream: caps 64 light, 4 heavy, 0.1 s", flush=True)
+    res, issue, _ = eng_small.serve_closed((flat, off), heavy, 64, 4, 0.1, table_budget_bytes=2 << 30,
+                                           light_lanes=2, keep=keep)
+    for i in np.nonzero(issue >= 0)[0]:
+        alone = eng_small.run_query(flat[off[i]:off[i + 1]])
+        r = res[i]
+        assert r.status == 0 and r.rows == alone.table.shape[0] and r.expanded == alone.expanded, i
+        if keep[i]:
+            assert np.array_equal(r.table, alone.table), i
+        else:
+            assert r.table is None
+
+
 def test_autotune_keeps_answers(dev_small, graph_small):
     _, gs = graph_small
     rng = np.random.default_rng(26)

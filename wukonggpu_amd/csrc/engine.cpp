@@ -720,11 +720,20 @@ struct PlainWorker {
 
 }  // namespace
 
-int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, const double *arrival_s,
-              const uint8_t *heavy, uint64_t table_budget_bytes, int light_lanes, wke_result *res,
-              double *wall_s) {
-  if (!e || !cmd || !cmd_off || !arrival_s || !res || nq < 0 || light_lanes < 1 || light_lanes > 16)
-    return WKG_EINVAL;
+namespace {
+
+// Closed-loop admission (the reference's batch driver): per-class in-flight
+// caps and a duration instead of arrival times.
+struct Closed {
+  int cap[2];        // light, heavy
+  double duration;   // no issue after this many seconds
+  double *issue_s;   // per query: issue time, -1 if never issued
+};
+
+// The served loop, open (arrival_s) or closed (cl).
+int serve_loop(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, const double *arrival_s,
+               const Closed *cl, const uint8_t *heavy, uint64_t table_budget_bytes, int light_lanes, wke_result *res,
+               double *wall_s) {
   bool any_heavy = false;
   for (int q = 0; heavy && q < nq; q++) any_heavy |= heavy[q] != 0;
   if (table_budget_bytes == 0) table_budget_bytes = 64ull << 30;
@@ -807,10 +816,23 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
   const auto t0 = Clock::now();
   auto now = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
   int next = 0, completed = 0, first = WKG_OK;
+  // closed loop: each class's queries in order, issued below its cap
+  std::vector<int> order[2];
+  size_t onext[2] = {0, 0};
+  int inflight[2] = {0, 0}, issued = 0;
+  if (cl) {
+    for (int q = 0; q < nq; q++) {
+      order[heavy && heavy[q] ? 1 : 0].push_back(q);
+      cl->issue_s[q] = -1;
+      memset(&res[q], 0, sizeof res[q]);  // what a query never issued returns
+    }
+  }
+  auto cls = [&](int q) { return heavy && heavy[q] ? 1 : 0; };
   auto done = [&](int q, int rc) {
-    res[q].latency_s = now() - arrival_s[q];
+    res[q].latency_s = now() - (cl ? cl->issue_s[q] : arrival_s[q]);
     if (rc && !first) first = rc;
     completed++;
+    inflight[cls(q)]--;
   };
   auto run_plain = [&](int q) {
     e->sv_fallbacks++;
@@ -822,12 +844,30 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
     done(q, rc);
   };
   std::vector<wkg_batch_result> rec((size_t)e->cfg_queries);
-  while (completed < nq) {
+  for (;;) {
+    if (cl) {  // issued all it will issue, and all of that completed
+      const bool more = now() < cl->duration && (onext[0] < order[0].size() || onext[1] < order[1].size());
+      if (!more && completed == issued) break;
+    } else if (completed >= nq) {
+      break;
+    }
     bool progressed = false;
-    // admit arrivals
+    // admit arrivals (open loop) or issue below the caps (closed loop)
     const double t = now();
-    while (next < nq && arrival_s[next] <= t) {
-      const int q = next++;
+    for (;;) {
+      int q = -1;
+      if (cl) {
+        if (t >= cl->duration) break;
+        for (int k = 0; k < 2 && q < 0; k++)
+          if (inflight[k] < cl->cap[k] && onext[k] < order[k].size()) q = order[k][onext[k]++];
+        if (q < 0) break;
+        cl->issue_s[q] = t;
+      } else {
+        if (next >= nq || arrival_s[next] > t) break;
+        q = next++;
+      }
+      issued++;
+      inflight[cls(q)]++;
       memset(&res[q], 0, sizeof res[q]);
       std::string why;
       int rc = compile_query(cmd + cmd_off[q], (int)(cmd_off[q + 1] - cmd_off[q]), plans[q], why);
@@ -931,7 +971,7 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
     if (progressed) continue;
     bool any_busy = pw.pending > 0;
     for (Lane &l : lanes) any_busy |= l.busy;
-    if (!any_busy && next < nq) {
+    if (!cl && !any_busy && next < nq) {
       const double dt = arrival_s[next] - now();
       if (dt > 0) std::this_thread::sleep_for(std::chrono::duration<double>(dt));
     } else {
@@ -941,6 +981,26 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
   if (wall_s) *wall_s = now();
   release();
   return first;
+}
+
+}  // namespace
+
+int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, const double *arrival_s,
+              const uint8_t *heavy, uint64_t table_budget_bytes, int light_lanes, wke_result *res,
+              double *wall_s) {
+  if (!e || !cmd || !cmd_off || !arrival_s || !res || nq < 0 || light_lanes < 1 || light_lanes > 16)
+    return WKG_EINVAL;
+  return serve_loop(e, cmd, cmd_off, nq, arrival_s, nullptr, heavy, table_budget_bytes, light_lanes, res, wall_s);
+}
+
+int wke_serve_closed(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, const uint8_t *heavy,
+                     int light_inflight, int heavy_inflight, double duration_s, uint64_t table_budget_bytes,
+                     int light_lanes, wke_result *res, double *issue_s, double *wall_s) {
+  if (!e || !cmd || !cmd_off || !res || !issue_s || nq < 0 || light_lanes < 1 || light_lanes > 16 ||
+      light_inflight < 0 || heavy_inflight < 0 || duration_s <= 0)
+    return WKG_EINVAL;
+  Closed cl{{light_inflight, heavy_inflight}, duration_s, issue_s};
+  return serve_loop(e, cmd, cmd_off, nq, nullptr, &cl, heavy, table_budget_bytes, light_lanes, res, wall_s);
 }
 
 }  // extern "C"

@@ -205,7 +205,10 @@ class Device:
 
 
 def pack_queries(queries):
-    """List of cmd_chains (sequences of int) -> (flat int64 array, int64 offsets)."""
+    """List of cmd_chains (sequences of int) -> (flat int64 array, int64 offsets).
+    A (flat, offsets) pair of arrays passes through."""
+    if isinstance(queries, tuple) and len(queries) == 2 and isinstance(queries[0], np.ndarray):
+        return np.ascontiguousarray(queries[0], dtype=np.int64), np.ascontiguousarray(queries[1], dtype=np.int64)
     off = np.zeros(len(queries) + 1, dtype=np.int64)
     for i, q in enumerate(queries):
         off[i + 1] = off[i] + len(q)
@@ -216,10 +219,11 @@ def pack_queries(queries):
 
 
 class QueryResult:
-    __slots__ = ("table", "status", "expanded", "latency_s")
+    __slots__ = ("table", "status", "expanded", "latency_s", "rows")
 
-    def __init__(self, table, status, expanded, latency_s):
+    def __init__(self, table, status, expanded, latency_s, rows=None):
         self.table, self.status, self.expanded, self.latency_s = table, status, expanded, latency_s
+        self.rows = table.shape[0] if rows is None else rows
 
 
 class GPUEngine:
@@ -241,11 +245,18 @@ class GPUEngine:
 
     __del__ = close
 
-    def _collect(self, res, n):
+    def _collect(self, res, n, keep=None):
+        """QueryResults of res[:n]; where keep[i] is false the table is
+        released unread and the result's table is None."""
         L = N.engine_lib()
         out = []
         for i in range(n):
             r = res[i]
+            if keep is not None and not keep[i]:
+                if r.table:
+                    L.wke_free(C.cast(r.table, C.c_void_p))
+                out.append(QueryResult(None, r.status, r.expanded, r.latency_s, r.rows))
+                continue
             if r.table and r.rows and r.cols:
                 t = np.ctypeslib.as_array(r.table, shape=(r.rows * r.cols,)).reshape(r.rows, r.cols).copy()
                 L.wke_free(C.cast(r.table, C.c_void_p))
@@ -368,6 +379,26 @@ class GPUEngine:
                     hv.ctypes.data_as(C.POINTER(C.c_uint8)), table_budget_bytes, light_lanes, res,
                     C.byref(wall))
         return self._collect(res, n), wall.value
+
+    def serve_closed(self, queries, heavy, light_inflight, heavy_inflight, duration_s, table_budget_bytes=0,
+                     light_lanes=2, keep=None):
+        """Closed-loop batch mode: (results, issue_s, wall); issue_s[q] < 0 for
+        queries never issued. queries: a list of cmd_chains or a (flat,
+        offsets) pair; keep: optional mask of the result tables to return (the
+        others come back as None, their rows still counted)."""
+        L = N.engine_lib()
+        flat, off = pack_queries(queries)
+        n = len(off) - 1
+        hv = np.ascontiguousarray(heavy, dtype=np.uint8)
+        res = (N.Result * max(n, 1))()
+        issue = np.zeros(max(n, 1), dtype=np.float64)
+        wall = C.c_double()
+        rc = L.wke_serve_closed(self._h, _i64p(flat), _i64p(off), n, hv.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                int(light_inflight), int(heavy_inflight), float(duration_s), table_budget_bytes,
+                                light_lanes, res, issue.ctypes.data_as(C.POINTER(C.c_double)), C.byref(wall))
+        if rc != 0:
+            raise DeviceError(rc, L.wke_last_error(self._h).decode())
+        return self._collect(res, n, keep), issue[:n], wall.value
 
 
 class Batch:

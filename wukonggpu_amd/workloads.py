@@ -9,6 +9,8 @@ triples; objects are uniform and distinct per subject.
 Queries are cmd_chains (core/query.hpp:68): (start, predicate, direction,
 end) per pattern, variables -1, -2, ... and dir IN = 0 / OUT = 1.
 """
+import time
+
 import numpy as np
 
 from .gstore import GStore, Triples, VID_BASE
@@ -131,7 +133,87 @@ def served_mix(P, type_ids, t0, n, seed, heavy_frac=0.05):
     return qs, np.array(flags, dtype=np.uint8)
 
 
-def config4(seed=4, n_light_types=1023):
+# The reference's batch mix (scripts/batch/mix_config): six light templates
+# (q1-q6) and four heavy queries (lubm_q1, q2, q3, q7) with these loads.
+BATCH_LIGHT_LOADS = (2800, 2800, 290, 900, 2800, 200)
+BATCH_HEAVY_LOADS = (5, 100, 100, 5)
+
+
+def _best_hop(gs, rng, frm, P, tries=16):
+    """(pid, dir) among `tries` random candidates whose lists from the
+    vertices `frm` are most often non-empty (ties: larger total)."""
+    best, key = None, (-1.0, -1)
+    for _ in range(tries):
+        pid, d = int(rng.integers(2, 2 + P)), int(rng.integers(0, 2))
+        degs = np.array([gs.degree(int(v), pid, d) for v in frm])
+        k = (float((degs > 0).mean()), int(degs.sum()))
+        if k > key:
+            best, key = (pid, d), k
+    return best
+
+
+def batch_stream(gs, P, light_types, t0, n, seed):
+    """Closed-loop batch-mode stream modelled on the reference's batch mix
+    (scripts/batch/mix_config), drawn by load as core/proxy.hpp:356-377 does
+    (mymath::get_distribution). Each light template starts from a random
+    constant of its own type (request_template::instantiate over the type's
+    members, the candidates of core/proxy.hpp:75-115):
+        q1, q2, q3, q5:  %c pa ?X . ?X pb ?Y                (const_to_unknown, known_to_unknown)
+        q4:              %c pa ?X . ?X pb ?Y1 . ?X pc ?Y2 . ?X pd ?Y3   (a star of three arms)
+        q6:              %c pa ?X . ?X pb ?Y . ?Y pc ?Z     (a chain)
+    with predicates and directions picked on the graph so that the hops are
+    mostly non-empty. The templates' rdf:type checks are left out: on this
+    graph about 1% of the entities are typed, so they would empty nearly every
+    answer. The heavy queries start from the t0 type index (start_from_index,
+    which the reference counts as heavy): the heavy 6-pattern shape with four
+    predicate triples. Returns (flat int64 cmd_chains, int64 offsets, heavy
+    flags uint8, template index per query: 0-5 light, 6-9 heavy)."""
+    rng = np.random.default_rng(seed)
+    cand = [t for t in light_types if gs.degree(0, t, IN) >= 64] or \
+        [t for t in light_types if gs.degree(0, t, IN) > 0]
+    tmpls, members = [], []
+    for k in range(len(BATCH_LIGHT_LOADS)):
+        t = int(rng.choice(cand))
+        m = gs.get_edges(0, t, IN).astype(np.int64)
+        samp = rng.choice(m, size=min(32, len(m)), replace=False)
+        pa, da = _best_hop(gs, rng, samp, P)
+        xs = np.concatenate([gs.get_edges(int(c), pa, da) for c in samp] + [np.zeros(0, np.uint32)])
+        xs = xs[:64] if len(xs) else samp[:1]
+        q = [0, pa, da, -1]
+        if k == 3:  # star: three arms from ?X
+            for j in range(3):
+                pb, db = _best_hop(gs, rng, xs, P)
+                q += [-1, pb, db, -(2 + j)]
+        else:
+            pb, db = _best_hop(gs, rng, xs, P)
+            q += [-1, pb, db, -2]
+            if k == 5:  # chain: one more hop from ?Y
+                ys = np.concatenate([gs.get_edges(int(x), pb, db) for x in xs] + [np.zeros(0, np.uint32)])
+                pc, dc = _best_hop(gs, rng, ys[:64] if len(ys) else xs[:1], P)
+                q += [-2, pc, dc, -3]
+        tmpls.append(np.array(q, dtype=np.int64))
+        members.append(m)
+    trip = [(2, 3, 4)] + [tuple(int(x) for x in 2 + rng.choice(P, size=3, replace=False)) for _ in range(3)]
+    for pa, pb, pc in trip:
+        tmpls.append(np.array(heavy_query(P, t0, pa, pb, pc), dtype=np.int64))
+    loads = np.array(BATCH_LIGHT_LOADS + BATCH_HEAVY_LOADS, dtype=np.float64)
+    types = rng.choice(len(loads), size=n, p=loads / loads.sum())
+    lens = np.array([len(x) for x in tmpls], dtype=np.int64)
+    off = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(lens[types], out=off[1:])
+    flat = np.empty(int(off[-1]), dtype=np.int64)
+    for k in range(len(tmpls)):
+        idx = np.nonzero(types == k)[0]
+        if not idx.size:
+            continue
+        flat[off[idx][:, None] + np.arange(lens[k])] = tmpls[k]
+        if k < len(BATCH_LIGHT_LOADS):
+            flat[off[idx]] = rng.choice(members[k], size=idx.size)
+    heavy = (types >= len(BATCH_LIGHT_LOADS)).astype(np.uint8)
+    return flat, off, heavy, types.astype(np.int32)
+
+
+def config4(seed=4, n_light_types=1023, keep_triples=False):
     """Served graph: 100M entities, 256 predicates, ~1B triples (Zipf 1.0),
     type t0 with 500k members for the heavy queries and n_light_types types
     with log-uniform sizes in [1, 1e4] for the light ones. The triples are
@@ -143,13 +225,18 @@ def config4(seed=4, n_light_types=1023):
     tr = Triples.generate(seed=seed, num_entities=N, num_preds=P, num_triples=1_000_000_000, zipf_s=1.0,
                           type_sizes=sizes)
     n_tr = len(tr)
+    t_build = time.perf_counter()
     gs = GStore.build(tr)
-    tr.close()
+    t_build = time.perf_counter() - t_build
+    if not keep_triples:
+        tr.close()
+        tr = None
     t0 = 2 + P
     info = {"entities": N, "predicates": P, "triples": n_tr, "types": len(sizes), "t0": t0,
             "t0_members": sizes[0], "light_types": list(range(t0 + 1, t0 + len(sizes))), "keys": gs.num_keys,
             "edges": gs.num_edges, "image_bytes": gs.image_nbytes()}
-    return Workload("config4", None, gs, {}, info)
+    info["host_build_s"] = t_build
+    return Workload("config4", tr, gs, {}, info)
 
 
 def cyclic_query(t, pa, ph, pc, pq, const):
