@@ -281,6 +281,10 @@ def main():
         type_ids = list(range(2 + P + 1, 2 + P + w.info["types"]))
         sq, sflags = workloads.served_mix(P, type_ids, w.info["t0"], args.served_n, args.seed + 11)
         alone = eng.run_plain(sq)
+        hx = [int(a.expanded) for a, f in zip(alone, sflags) if f]
+        hl = [a.latency_s for a, f in zip(alone, sflags) if f]
+        log(f"served mix: {len(sq)} queries, {len(hx)} heavy expanding {min(hx, default=0)}..{max(hx, default=0)} "
+            f"edges, {min(hl, default=0) * 1e3:.2f}..{max(hl, default=0) * 1e3:.2f} ms alone")
         served_same = True
         rng = np.random.default_rng(args.seed + 12)
         for rate in [float(x) for x in args.served_rates.split(",") if x]:
@@ -409,7 +413,7 @@ def batch_mode(eng, gs, P, light_types, t0, args, budget=64 << 30):
     dur = args.batch_s
     warm = dur / 4
     caps = [int(x) for x in args.batch_p.split(",") if x]
-    n = 3_000_000
+    n = 4_000_000
     t = time.perf_counter()
     flat, off, heavy, types = workloads.batch_stream(gs, P, light_types, t0, n, args.seed + 51)
     log(f"batch mode: {n} queries from 6 light templates and 4 heavy queries in {time.perf_counter() - t:.1f}s")
@@ -426,14 +430,14 @@ def batch_mode(eng, gs, P, light_types, t0, args, budget=64 << 30):
         res, issue, wall = eng.serve_closed((flat, off), heavy, p, hcap, dur, table_budget_bytes=budget,
                                             light_lanes=2, keep=keep)
         issued = issue >= 0
-        lat = np.array([r.latency_s for r in res])
+        lat = res.latency_s
         done = issue + lat
-        st = np.array([r.status for r in res])
+        st = res.status
         win = issued & (done >= warm) & (done <= dur)
         ok = bool((st[issued] == 0).all())
         for j, i in enumerate(kidx):
             if issued[i]:
-                a, r = alone[j], res[i]
+                a, r = alone[j], res[int(i)]
                 ok = ok and a.status == 0 and r.table.shape == a.table.shape and \
                     np.array_equal(r.table, a.table) and r.expanded == a.expanded
         all_same = all_same and ok

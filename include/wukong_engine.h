@@ -123,10 +123,13 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
  * are wke_serve's. issue_s[q] gets each query's issue time in seconds from
  * the start (-1: not issued before the end, its result zeroed); latency_s is
  * completion - issue (send to reply, as the reference's Logger records it).
- * Queries are issued in order within their class. */
+ * Queries are issued in order within their class. keep (may be NULL: all)
+ * marks the queries whose tables are returned; the others are blind, as the
+ * reference's batch requests are (core/proxy.hpp:384): rows, cols, status
+ * and expanded only, table NULL. */
 int wke_serve_closed(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, const uint8_t *heavy,
                      int light_inflight, int heavy_inflight, double duration_s, uint64_t table_budget_bytes,
-                     int light_lanes, wke_result *res, double *issue_s, double *wall_s);
+                     int light_lanes, const uint8_t *keep, wke_result *res, double *issue_s, double *wall_s);
 
 #ifdef __cplusplus
 }

@@ -257,7 +257,7 @@ def test_closed_loop_template_stream(eng_small, graph_small):
         if keep[i]:
             assert np.array_equal(r.table, alone.table), i
         else:
-            assert r.table is None
+            assert int(i) not in res.tables  # blind: row counts only
 
 
 def test_autotune_keeps_answers(dev_small, graph_small):

@@ -182,6 +182,6 @@ def engine_lib():
         _sig(L, "wke_serve", C.c_int, vp, P(i64), P(i64), C.c_int, P(dbl), P(C.c_uint8), u64, C.c_int,
              P(Result), P(dbl))
         _sig(L, "wke_serve_closed", C.c_int, vp, P(i64), P(i64), C.c_int, P(C.c_uint8), C.c_int, C.c_int, dbl,
-             u64, C.c_int, P(Result), P(dbl), P(dbl))
+             u64, C.c_int, P(C.c_uint8), P(Result), P(dbl), P(dbl))
         _engine = L
     return _engine

@@ -129,14 +129,15 @@ int compile_query(const int64_t *cmd, int ncmd, std::vector<wkg_plan_step> &out,
 }
 
 // Copies batch record r (cells of the whole batch) into a wke_result.
-void fill_result(const wkg_batch_result &r, const int32_t *cells, wke_result *out) {
+// with_table false: rows and cols only (a blind request's reply)
+void fill_result(const wkg_batch_result &r, const int32_t *cells, wke_result *out, bool with_table = true) {
   out->status = r.status;
   out->expanded = r.expanded;
   out->rows = r.status == WKG_OK ? r.rows : 0;
   out->cols = r.cols;
   out->table = nullptr;
   const uint64_t n = out->rows * (uint64_t)(r.cols > 0 ? r.cols : 0);
-  if (r.status == WKG_OK && n) {
+  if (r.status == WKG_OK && n && with_table) {
     out->table = (int32_t *)malloc(n * 4);
     if (!out->table) {
       out->status = WKG_ENOMEM;
@@ -725,9 +726,11 @@ namespace {
 // Closed-loop admission (the reference's batch driver): per-class in-flight
 // caps and a duration instead of arrival times.
 struct Closed {
-  int cap[2];        // light, heavy
-  double duration;   // no issue after this many seconds
-  double *issue_s;   // per query: issue time, -1 if never issued
+  int cap[2];            // light, heavy
+  double duration;       // no issue after this many seconds
+  double *issue_s;       // per query: issue time, -1 if never issued
+  const uint8_t *keep;   // per query: return the table (NULL: all)
+  bool blind(int q) const { return keep && !keep[q]; }
 };
 
 // The served loop, open (arrival_s) or closed (cl).
@@ -830,6 +833,10 @@ int serve_loop(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq
   auto cls = [&](int q) { return heavy && heavy[q] ? 1 : 0; };
   auto done = [&](int q, int rc) {
     res[q].latency_s = now() - (cl ? cl->issue_s[q] : arrival_s[q]);
+    if (cl && cl->blind(q) && res[q].table) {
+      free(res[q].table);
+      res[q].table = nullptr;
+    }
     if (rc && !first) first = rc;
     completed++;
     inflight[cls(q)]--;
@@ -928,7 +935,7 @@ int serve_loop(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq
         } else if (rec[i].status == WKG_ENOMEM) {
           run_plain(q);
         } else {
-          fill_result(rec[i], cells, &res[q]);
+          fill_result(rec[i], cells, &res[q], !(cl && cl->blind(q)));
           done(q, res[q].status);
         }
       }
@@ -995,11 +1002,11 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
 
 int wke_serve_closed(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, const uint8_t *heavy,
                      int light_inflight, int heavy_inflight, double duration_s, uint64_t table_budget_bytes,
-                     int light_lanes, wke_result *res, double *issue_s, double *wall_s) {
+                     int light_lanes, const uint8_t *keep, wke_result *res, double *issue_s, double *wall_s) {
   if (!e || !cmd || !cmd_off || !res || !issue_s || nq < 0 || light_lanes < 1 || light_lanes > 16 ||
       light_inflight < 0 || heavy_inflight < 0 || duration_s <= 0)
     return WKG_EINVAL;
-  Closed cl{{light_inflight, heavy_inflight}, duration_s, issue_s};
+  Closed cl{{light_inflight, heavy_inflight}, duration_s, issue_s, keep};
   return serve_loop(e, cmd, cmd_off, nq, nullptr, &cl, heavy, table_budget_bytes, light_lanes, res, wall_s);
 }
 

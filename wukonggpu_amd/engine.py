@@ -245,18 +245,11 @@ class GPUEngine:
 
     __del__ = close
 
-    def _collect(self, res, n, keep=None):
-        """QueryResults of res[:n]; where keep[i] is false the table is
-        released unread and the result's table is None."""
+    def _collect(self, res, n):
         L = N.engine_lib()
         out = []
         for i in range(n):
             r = res[i]
-            if keep is not None and not keep[i]:
-                if r.table:
-                    L.wke_free(C.cast(r.table, C.c_void_p))
-                out.append(QueryResult(None, r.status, r.expanded, r.latency_s, r.rows))
-                continue
             if r.table and r.rows and r.cols:
                 t = np.ctypeslib.as_array(r.table, shape=(r.rows * r.cols,)).reshape(r.rows, r.cols).copy()
                 L.wke_free(C.cast(r.table, C.c_void_p))
@@ -382,23 +375,64 @@ class GPUEngine:
 
     def serve_closed(self, queries, heavy, light_inflight, heavy_inflight, duration_s, table_budget_bytes=0,
                      light_lanes=2, keep=None):
-        """Closed-loop batch mode: (results, issue_s, wall); issue_s[q] < 0 for
-        queries never issued. queries: a list of cmd_chains or a (flat,
+        """Closed-loop batch mode: (ClosedResults, issue_s, wall); issue_s[q] <
+        0 for queries never issued. queries: a list of cmd_chains or a (flat,
         offsets) pair; keep: optional mask of the result tables to return (the
-        others come back as None, their rows still counted)."""
+        others are blind: row counts only)."""
         L = N.engine_lib()
         flat, off = pack_queries(queries)
         n = len(off) - 1
         hv = np.ascontiguousarray(heavy, dtype=np.uint8)
+        kp = None if keep is None else np.ascontiguousarray(keep, dtype=np.uint8)
         res = (N.Result * max(n, 1))()
         issue = np.zeros(max(n, 1), dtype=np.float64)
         wall = C.c_double()
         rc = L.wke_serve_closed(self._h, _i64p(flat), _i64p(off), n, hv.ctypes.data_as(C.POINTER(C.c_uint8)),
                                 int(light_inflight), int(heavy_inflight), float(duration_s), table_budget_bytes,
-                                light_lanes, res, issue.ctypes.data_as(C.POINTER(C.c_double)), C.byref(wall))
+                                light_lanes, None if kp is None else kp.ctypes.data_as(C.POINTER(C.c_uint8)), res,
+                                issue.ctypes.data_as(C.POINTER(C.c_double)), C.byref(wall))
         if rc != 0:
             raise DeviceError(rc, L.wke_last_error(self._h).decode())
-        return self._collect(res, n, keep), issue[:n], wall.value
+        return ClosedResults(res, n), issue[:n], wall.value
+
+
+# Layout of wke_result (include/wukong_engine.h) as a numpy record.
+_RESULT_DTYPE = np.dtype([("table", "<u8"), ("rows", "<u8"), ("cols", "<i4"), ("status", "<i4"),
+                          ("expanded", "<u8"), ("latency_s", "<f8")])
+
+
+class ClosedResults:
+    """Results of a closed-loop run as per-query arrays (rows, cols, status,
+    expanded, latency_s) and the returned tables by query index; indexing
+    gives a QueryResult (table None for a blind query)."""
+
+    def __init__(self, res, n):
+        assert C.sizeof(N.Result) == _RESULT_DTYPE.itemsize
+        rec = np.frombuffer((C.c_char * (n * _RESULT_DTYPE.itemsize)).from_address(C.addressof(res)),
+                            dtype=_RESULT_DTYPE) if n else np.zeros(0, dtype=_RESULT_DTYPE)
+        self.rows, self.cols = rec["rows"].copy(), rec["cols"].copy()
+        self.status, self.expanded = rec["status"].copy(), rec["expanded"].copy()
+        self.latency_s = rec["latency_s"].copy()
+        self.tables = {}
+        L = N.engine_lib()
+        for q in np.nonzero(rec["table"])[0]:
+            r = res[int(q)]
+            t = np.ctypeslib.as_array(r.table, shape=(r.rows * r.cols,)).reshape(r.rows, r.cols).copy()
+            L.wke_free(C.cast(r.table, C.c_void_p))
+            self.tables[int(q)] = t
+
+    def __len__(self):
+        return len(self.rows)
+
+    def __getitem__(self, q):
+        t = self.tables.get(q)
+        if t is None and self.rows[q] == 0:
+            t = np.zeros((0, max(int(self.cols[q]), 0)), dtype=np.int32)
+        return QueryResult(t, int(self.status[q]), int(self.expanded[q]), float(self.latency_s[q]),
+                           int(self.rows[q]))
+
+    def __iter__(self):
+        return (self[q] for q in range(len(self)))
 
 
 class Batch:
