@@ -64,6 +64,8 @@ def main():
                     help="closed-loop batch mode: light in-flight caps (heavy cap min(p, 50))")
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the closed-loop batch mode on the 100M-triple graph")
+    ap.add_argument("--exchange-only", action="store_true",
+                    help="run only the fork-join table split (for profiler passes)")
     ap.add_argument("--config", default="2", choices=["1", "2", "4", "5"],
                     help="2: heavy query + batched mix + served sweep on the 100M-triple graph (default); "
                          "1: plain single query on the 10M-triple graph; 4: served sweep on the 1B-triple "
@@ -78,8 +80,14 @@ def main():
                     help="with --mix-only: the executor shape as key=value pairs (probe_wg_per_cu=3,...) "
                          "instead of autotuning")
     args = ap.parse_args()
-    if args.heavy_only or args.probe_only or args.mix_only or args.batch_only:
+    if args.heavy_only or args.probe_only or args.mix_only or args.batch_only or args.exchange_only:
         args.no_cpu = True
+    if args.exchange_only:
+        from wukonggpu_amd import Device
+        dev = Device(0)
+        print(json.dumps({"table_exchange": table_exchange(dev, args.seed)}), flush=True)
+        dev.close()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU's shard; --gpus {args.gpus} is not supported "
               "(this build runs on a single MI355X)", file=sys.stderr)
@@ -315,12 +323,16 @@ def main():
         P = w.info["predicates"]
         bm = batch_mode(eng, w.gstore, P, list(range(2 + P + 1, 2 + P + w.info["types"])), w.info["t0"], args)
 
+    # ---- fork-join table exchange: the split a sharded deployment adds per step ---------------
+    exchange = table_exchange(dev, args.seed)
+
     # ---- CPU baseline (the oracle), parity of the heavy result --------------------------
     cpu = None
     parity = {"heavy_rows_equal": None, "heavy_executor_equals_plain": heavy_exec_same,
               "mix_plain_equals_executor": bool(same_exec), "mix_plain_equals_rounds": bool(same_rounds),
               "served_equals_alone": served_same,
-              "batch_mode_equals_alone": bm["equals_alone"] if bm else None}
+              "batch_mode_equals_alone": bm["equals_alone"] if bm else None,
+              "table_split_correct": all(r["correct"] for r in exchange)}
     if not args.no_cpu:
         import oracle
         want, wex, sec = oracle.run_query(w.gstore, heavy)
@@ -387,6 +399,7 @@ def main():
         "heavy_via_executor_ms": pct(hx_lat, 0.5) * 1e3, "heavy_tuning": heavy_tuning,
         "served": served,
         "batch_mode": bm,
+        "table_exchange": exchange,
         "parity": parity,
     }
     print(json.dumps(line), flush=True)
@@ -552,6 +565,61 @@ def random_key_probe(dev, w, seed, line_peak):
             "segment_bytes": seg_bytes, "random_lines_same_working_set": seg_lines,
             "frac_of_random_lines_same_working_set": rand_rate / seg_lines,
             "workload": "known_to_unknown over 64M uniformly random vertex ids, predicate 2 OUT"}
+
+
+def table_exchange(dev, seed, rows=1 << 25, cols=4, parts=(8, 64, 1024)):
+    """Fork-join split (wkg_table_split) of a rows x cols table of random
+    vertex ids by the owner of column 0, for each part count (8: one 8-GPU
+    node), HIP events over 5 launches after a warmup: split rows/s and the
+    HBM roofline of the split (algorithmic bytes: the table read once and
+    written once, 8 * rows * cols) over the count, scan and scatter kernels,
+    and of the scatter kernel alone. Each split is checked by what defines
+    it: sub-table p holds exactly the input rows whose column-0 cell is p mod
+    the part count, in input order, at the offsets the sizes give; and the
+    merge of its sub-tables (wkg_table_concat) is the split table."""
+    import numpy as np
+    rng = np.random.default_rng(seed + 61)
+    t = ((1 << 17) + rng.integers(0, 100_000_000, size=(rows, cols), dtype=np.int64)).astype(np.int32)
+    d = dev.table(t)
+    alg = 8.0 * rows * cols
+    out = []
+    for n in parts:
+        o, _, _ = dev.table_split(d, 0, n)  # warm
+        o.free()
+        dev.profile(True)
+        tw = time.perf_counter()
+        for _ in range(5):
+            o, prow, phead = dev.table_split(d, 0, n)
+            if _ < 4:
+                o.free()
+        wall = (time.perf_counter() - tw) / 5
+        k = {name: dev.profile_read(name) for name in ("split_count", "split_scan", "split_scatter")}
+        dev.profile(False)
+        views = [dev.table_slice(o, int(h), int(c)) for h, c in zip(phead, prow)]
+        merged = dev.table_concat(views).to_host()
+        got = o.to_host()
+        for v in views:
+            v.free()
+        o.free()
+        owner = t[:, 0].view(np.uint32) % np.uint32(n)
+        ok = bool(np.array_equal(merged, got) and int(prow.sum()) == rows and phead[0] == 0 and
+                  (phead[1:] == np.cumsum(prow)[:-1]).all())
+        for p in range(n) if ok else ():
+            ok = ok and np.array_equal(got[int(phead[p]):int(phead[p] + prow[p])], t[owner == p])
+        kms = sum(v[1] for v in k.values()) / 5
+        sms = k["split_scatter"][1] / max(k["split_scatter"][0], 1)
+        rec = {"parts": n, "rows": rows, "cols": cols, "wall_ms": wall * 1e3, "kernels_ms": kms,
+               "rows_per_s": rows / (kms * 1e-3), "achieved_GBps": alg / (kms * 1e-3) / 1e9,
+               "frac_of_peak": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+               "scatter_ms": sms, "scatter_GBps": alg / (sms * 1e-3) / 1e9,
+               "per_kernel_ms": {name: v[1] / max(v[0], 1) for name, v in k.items()},
+               "correct": ok}
+        out.append(rec)
+        log(f"table split {rows}x{cols} into {n}: {kms:.3f} ms in kernels ({rec['achieved_GBps']:.0f} GB/s, "
+            f"{rec['frac_of_peak']:.2f} of peak; scatter {sms:.3f} ms = {rec['scatter_GBps']:.0f} GB/s), "
+            f"{wall * 1e3:.3f} ms per call; correct={ok}")
+    d.free()
+    return out
 
 
 def run_config1(args):

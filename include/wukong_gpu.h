@@ -155,6 +155,22 @@ int wkg_pool_stats(wkg_ctx *ctx, uint64_t *live_bytes, uint64_t *cached_bytes, u
 int wkg_table_slice(wkg_ctx *ctx, const wkg_table *t, uint64_t row0, uint64_t nrows, wkg_table **out);
 int wkg_table_free(wkg_ctx *ctx, wkg_table *t);
 
+/* Fork-join split (GPU_Engine::generate_sub_query, core/gpu_engine.hpp:293-350,
+ * over RCache::generate_sub_query, core/rcache.hpp:964-1043, and the kernels
+ * of core/gpu_hash.cu:698-873): row i of `in` goes to sub-table
+ * in[i][col] mod nparts, the cell read as an unsigned 32-bit vertex id (the
+ * owner rule vid % servers, utils/mymath.hpp:55-59). *out holds the
+ * sub-tables back to back: sub-table p is part_rows[p] rows starting at row
+ * part_head[p] (part_head may be NULL); wkg_table_slice gives views of them.
+ * Inside a sub-table rows keep their input order (the reference's atomic
+ * slots allow any order; this is the stable one). 1 <= nparts <= 1024. */
+int wkg_table_split(wkg_ctx *ctx, const wkg_table *in, int col, int nparts, void *stream, wkg_table **out,
+                    uint64_t *part_rows, uint64_t *part_head);
+/* Merge of sub-replies (RMap::put_reply, core/reply_map.hpp:43-80): the
+ * parts' rows appended in the order given. Empty parts are skipped; the
+ * others must have the same column count. */
+int wkg_table_concat(wkg_ctx *ctx, int n, const wkg_table *const *parts, void *stream, wkg_table **out);
+
 /* ---- pattern steps (one RCache handler each) -------------------------------------
  * `expanded` (may be NULL) receives the step's work count: edges emitted for
  * index/const/known_to_unknown, edges examined by the reference's early-exit

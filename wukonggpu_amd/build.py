@@ -42,7 +42,7 @@ def build_gstore():
     return out
 
 
-HIP_SOURCES = ("wkg_hip.hip", "graph.hip", "batch.hip", "exec.hip", "builder.hip")
+HIP_SOURCES = ("wkg_hip.hip", "graph.hip", "batch.hip", "exec.hip", "builder.hip", "split.hip")
 
 
 def build_hip():

@@ -163,6 +163,33 @@ class Device:
                                                      a.shape[0], a.shape[1], None, C.byref(h)))
         return Table(self, h.value)
 
+    def table_split(self, t, col, nparts):
+        """Fork-join split of a device table by the owner (cell % nparts) of
+        column `col`: (Table holding the sub-tables back to back, rows per
+        sub-table, first row of each)."""
+        h = C.c_void_p()
+        rows = np.zeros(max(nparts, 1), dtype=np.uint64)
+        head = np.zeros(max(nparts, 1), dtype=np.uint64)
+        u64p = C.POINTER(C.c_uint64)
+        self._check(N.hip_lib().wkg_table_split(self._h, t._h, int(col), int(nparts), None, C.byref(h),
+                                                 rows.ctypes.data_as(u64p), head.ctypes.data_as(u64p)))
+        return Table(self, h.value), rows[:nparts], head[:nparts]
+
+    def table_concat(self, tables):
+        """Sub-replies merged: the tables' rows appended in order."""
+        arr = (C.c_void_p * max(len(tables), 1))(*[t._h.value for t in tables])
+        h = C.c_void_p()
+        self._check(N.hip_lib().wkg_table_concat(self._h, len(tables), arr, None, C.byref(h)))
+        return Table(self, h.value)
+
+    def table_slice(self, t, row0, nrows):
+        """View of rows [row0, row0 + nrows) of t (free it before t)."""
+        h = C.c_void_p()
+        self._check(N.hip_lib().wkg_table_slice(self._h, t._h, int(row0), int(nrows), C.byref(h)))
+        v = Table(self, h.value)
+        v._parent = t  # the view's memory is t's
+        return v
+
     def _step(self, fn, *args):
         h = C.c_void_p()
         ex = C.c_uint64()
