@@ -119,7 +119,8 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
  * 287-457): the light queries are issued in order while fewer than
  * light_inflight of them are in flight, the heavy ones while fewer than
  * heavy_inflight are (its heavy in-flight cap), until duration_s has passed;
- * then everything issued completes. Lanes, admission and the heavy worker
+ * then everything issued completes. Two issue rounds are at least
+ * interval_s apart (its send interval, -i; 0: issue whenever a slot frees). Lanes, admission and the heavy worker
  * are wke_serve's. issue_s[q] gets each query's issue time in seconds from
  * the start (-1: not issued before the end, its result zeroed); latency_s is
  * completion - issue (send to reply, as the reference's Logger records it).
@@ -128,8 +129,9 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
  * reference's batch requests are (core/proxy.hpp:384): rows, cols, status
  * and expanded only, table NULL. */
 int wke_serve_closed(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, const uint8_t *heavy,
-                     int light_inflight, int heavy_inflight, double duration_s, uint64_t table_budget_bytes,
-                     int light_lanes, const uint8_t *keep, wke_result *res, double *issue_s, double *wall_s);
+                     int light_inflight, int heavy_inflight, double duration_s, double interval_s,
+                     uint64_t table_budget_bytes, int light_lanes, const uint8_t *keep, wke_result *res,
+                     double *issue_s, double *wall_s);
 
 #ifdef __cplusplus
 }

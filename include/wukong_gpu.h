@@ -112,7 +112,9 @@ int wkg_graph_framed(const wkg_ctx *ctx);
 int wkg_table_from_host(wkg_ctx *ctx, const int32_t *host, uint64_t rows, int cols,
                         void *stream, wkg_table **out);
 /* Device table -> host (the last-step D2H copy, core/rcache.hpp:401-412).
- * `host` must hold rows*cols int32. */
+ * `host` must hold rows*cols int32. Tables up to 64 KB are copied by a
+ * kernel into the context's pinned staging block, whose flag the host polls
+ * (no stream synchronisation); larger ones by a copy command. */
 int wkg_table_to_host(wkg_ctx *ctx, const wkg_table *t, int32_t *host, void *stream);
 int wkg_table_shape(const wkg_table *t, uint64_t *rows, int *cols);
 /* Device address of the table's first cell (row-major int32). */
@@ -136,6 +138,11 @@ int wkg_set_probe_tile_rows(wkg_ctx *ctx, int rows);
  * the same, row for row. On by default; 0 turns it off. No counterpart in the
  * reference, which probes every row (core/gpu_hash.cu:145-316). */
 int wkg_set_lineage(wkg_ctx *ctx, int on);
+/* A known-start step over at most 512 input rows runs its probe, scan,
+ * compaction and (up to 2048 output rows) expand in one launch instead of
+ * five; its output is the same, row for row. On by default; 0 turns it off.
+ * A launch-count optimisation with no counterpart in the reference. */
+int wkg_set_small_steps(wkg_ctx *ctx, int on);
 /* Per-thread arena: until wkg_thread_arena_end, the tables and step scratch
  * that the calling thread's per-step calls allocate are carved from `bytes`
  * of pool memory reserved now, with no hipMalloc; wkg_thread_arena_reset

@@ -233,6 +233,12 @@ def test_closed_loop_batch_mode(eng_small, graph_small):
     # a duration must be positive
     with pytest.raises(DeviceError):
         eng_small.serve_closed(qs[:50], heavy[:50], 16, 2, 0.0, table_budget_bytes=2 << 30, light_lanes=2)
+    # a send interval: issue rounds at least 20 ms apart, so at most 6 in 0.1 s
+    res_i, issue_i, _ = eng_small.serve_closed(qs, heavy, 16, 2, 0.1, table_budget_bytes=2 << 30, light_lanes=2,
+                                               interval_s=0.02)
+    rounds = np.unique(issue_i[issue_i >= 0])
+    assert 1 <= len(rounds) <= 6 and (np.diff(rounds) >= 0.02 - 1e-9).all(), rounds
+    assert (issue_i >= 0).sum() <= len(rounds) * 18  # each round fills at most the free slots of both caps
 
 
 def test_closed_loop_template_stream(eng_small, graph_small):

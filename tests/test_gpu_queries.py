@@ -164,6 +164,41 @@ def test_probe_tile_sizes_match_oracle(graph_hubs):
     dev.close()
 
 
+def test_small_steps_match_oracle(graph_hubs):
+    """Steps over at most 512 rows run in one launch: the answers equal the
+    oracle's with it on and off, including small inputs whose output is
+    above the in-launch expand's 2048 rows (hub rows of 5000 edges)."""
+    _, gs = graph_hubs
+    dev = Device(0)
+    dev.load(gs)
+    eng = GPUEngine(dev)
+    qs = [[VB + 0, 2, 1, -1, -1, 3, 0, -2],                              # hub constant: 5000-row fan-out
+          [VB + 0, 2, 1, -1, -1, 2, 0, -2, -2, 3, 1, -3],
+          [3, 0, 1, -1, -1, 3, 1, VB + 5],                               # filter on a predicate index
+          [VB + 1, 2, 1, -1, -1, 2, 0, -2, -2, 2, 1, -1],                # closing known_to_known
+          [VB + 2, 3, 1, -1, -1, 2, 1, -2, -1, 3, 0, -3, -3, 2, 1, -4]]
+    rng = np.random.default_rng(79)
+    qs += random_queries(rng, 6000, 3, [2, 3], 60, max_steps=5)
+    for on in (True, False):
+        print(f"small steps {'on' if on else 'off'}", flush=True)
+        dev.set_small_steps(on)
+        for q in qs:
+            _check(gs, q, eng.run_query(q))
+    # one step of each kind on 1..512-row tables
+    dev.set_small_steps(True)
+    for rows in (1, 63, 257, 512, 513):
+        t = (VB + rng.integers(0, 6000, size=(rows, 2))).astype(np.int32)
+        t[0, 0] = VB  # a hub row
+        for cmd, fn in (([-1, 2, 1, -3], lambda d: dev.k2u(d, 0, 2, 1)),
+                        ([-1, 3, 0, -2], lambda d: dev.k2k(d, 0, 3, 0, 1)),
+                        ([-2, 2, 1, VB + 7], lambda d: dev.k2c(d, 1, 2, 1, VB + 7))):
+            want, wex = oracle.run_step(gs, [VB, 2, 1, -1] + cmd, 1, t)
+            got, ex = fn(dev.table(t))
+            assert np.array_equal(got.to_host(), want) and ex == wex, (rows, cmd)
+    eng.close()
+    dev.close()
+
+
 def test_lineage_star_arms_match_oracle(graph_hubs):
     """A star's next arm after a fan-out probes once per input row of the
     fan-out (the output keeps which input row each row repeats): the answers

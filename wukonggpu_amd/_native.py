@@ -128,6 +128,7 @@ def hip_lib():
         _sig(L, "wkg_table_slice", C.c_int, vp, vp, u64, u64, P(vp))
         _sig(L, "wkg_table_split", C.c_int, vp, vp, C.c_int, C.c_int, vp, P(vp), P(u64), P(u64))
         _sig(L, "wkg_table_concat", C.c_int, vp, C.c_int, P(vp), vp, P(vp))
+        _sig(L, "wkg_set_small_steps", C.c_int, vp, C.c_int)
         _sig(L, "wkg_set_table_limit", C.c_int, vp, u64)
         _sig(L, "wkg_set_probe_tile_rows", C.c_int, vp, C.c_int)
         _sig(L, "wkg_set_lineage", C.c_int, vp, C.c_int)
@@ -184,6 +185,6 @@ def engine_lib():
         _sig(L, "wke_serve", C.c_int, vp, P(i64), P(i64), C.c_int, P(dbl), P(C.c_uint8), u64, C.c_int,
              P(Result), P(dbl))
         _sig(L, "wke_serve_closed", C.c_int, vp, P(i64), P(i64), C.c_int, P(C.c_uint8), C.c_int, C.c_int, dbl,
-             u64, C.c_int, P(C.c_uint8), P(Result), P(dbl), P(dbl))
+             dbl, u64, C.c_int, P(C.c_uint8), P(Result), P(dbl), P(dbl))
         _engine = L
     return _engine

@@ -29,6 +29,23 @@ constexpr int kExpandVT = 2;           // merged items per thread in the expand
 constexpr int kExpandNV = kBlock * kExpandVT;
 constexpr uint32_t kMaxHops = 1u << 20;  // chain-walk guard (images are validated)
 
+// Read-back to the host without a stream synchronisation: a kernel writes
+// counters into a pinned host block and then a sequence number into the
+// block's flag word (system-scope release); the host polls the flag.
+constexpr int kRbWords = 8;   // counters per read-back, at words [0, kRbWords)
+constexpr int kRbFlag = 8;    // the flag word
+constexpr uint64_t kStageOff = 4096;         // byte offset of the staging area for small tables
+constexpr uint64_t kStageBytes = 64 << 10;   // tables up to this size reach the host through it
+struct Signal {
+  uint64_t *h;   // the pinned block (device address)
+  uint64_t seq;
+};
+// One thread: v[0, n) then the flag.
+__device__ __forceinline__ void publish(const Signal &sg, const uint64_t *v, int n) {
+  for (int i = 0; i < n; i++) sg.h[i] = v[i];
+  __hip_atomic_store(&sg.h[kRbFlag], sg.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 struct DevMeta {
   uint64_t main_start;    // first main bucket, global id
   uint64_t psz;           // partition_sz (0: the id has no keys)

@@ -728,6 +728,7 @@ namespace {
 struct Closed {
   int cap[2];            // light, heavy
   double duration;       // no issue after this many seconds
+  double interval;       // at least this long between two issue rounds (0: none)
   double *issue_s;       // per query: issue time, -1 if never issued
   const uint8_t *keep;   // per query: return the table (NULL: all)
   bool blind(int q) const { return keep && !keep[q]; }
@@ -823,6 +824,7 @@ int serve_loop(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq
   std::vector<int> order[2];
   size_t onext[2] = {0, 0};
   int inflight[2] = {0, 0}, issued = 0;
+  double next_round = 0;  // closed loop: earliest time of the next issue round
   if (cl) {
     for (int q = 0; q < nq; q++) {
       order[heavy && heavy[q] ? 1 : 0].push_back(q);
@@ -861,10 +863,11 @@ int serve_loop(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq
     bool progressed = false;
     // admit arrivals (open loop) or issue below the caps (closed loop)
     const double t = now();
+    int round_issued = 0;
     for (;;) {
       int q = -1;
       if (cl) {
-        if (t >= cl->duration) break;
+        if (t >= cl->duration || t < next_round) break;
         for (int k = 0; k < 2 && q < 0; k++)
           if (inflight[k] < cl->cap[k] && onext[k] < order[k].size()) q = order[k][onext[k]++];
         if (q < 0) break;
@@ -874,6 +877,7 @@ int serve_loop(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq
         q = next++;
       }
       issued++;
+      round_issued++;
       inflight[cls(q)]++;
       memset(&res[q], 0, sizeof res[q]);
       std::string why;
@@ -898,6 +902,7 @@ int serve_loop(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq
       }
       progressed = true;
     }
+    if (cl && round_issued && cl->interval > 0) next_round = t + cl->interval;
     // heavy queries the worker finished
     if (worker) {
       std::vector<PlainWorker::Done> fin;
@@ -1001,12 +1006,13 @@ int wke_serve(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq,
 }
 
 int wke_serve_closed(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq, const uint8_t *heavy,
-                     int light_inflight, int heavy_inflight, double duration_s, uint64_t table_budget_bytes,
-                     int light_lanes, const uint8_t *keep, wke_result *res, double *issue_s, double *wall_s) {
+                     int light_inflight, int heavy_inflight, double duration_s, double interval_s,
+                     uint64_t table_budget_bytes, int light_lanes, const uint8_t *keep, wke_result *res,
+                     double *issue_s, double *wall_s) {
   if (!e || !cmd || !cmd_off || !res || !issue_s || nq < 0 || light_lanes < 1 || light_lanes > 16 ||
-      light_inflight < 0 || heavy_inflight < 0 || duration_s <= 0)
+      light_inflight < 0 || heavy_inflight < 0 || duration_s <= 0 || interval_s < 0)
     return WKG_EINVAL;
-  Closed cl{{light_inflight, heavy_inflight}, duration_s, issue_s, keep};
+  Closed cl{{light_inflight, heavy_inflight}, duration_s, interval_s, issue_s, keep};
   return serve_loop(e, cmd, cmd_off, nq, nullptr, &cl, heavy, table_budget_bytes, light_lanes, res, wall_s);
 }
 

@@ -544,18 +544,65 @@ __global__ __launch_bounds__(kBlock) void k_expand(GraphView g, ExpandParams a) 
   expand_tile<kExpandVT, FRAMED>(g, a, blockIdx.x);
 }
 
-// Single-key probe for index_to_unknown / const_to_unknown: out[0] = degree,
-// out[1] = offset in the predicate segment. One wave.
+// ---- a small step in one launch --------------------------------------------
+// A known-start step over at most kSmallRows input rows runs in one
+// workgroup: the probe tiles (kBlock rows each), the tile scan, the
+// finalize and, for known_to_unknown with at most `cap` output rows, the
+// merge-path split points and the expand into x.O (cap rows reserved) --
+// the multi-launch path's own tile bodies, separated by workgroup barriers
+// (after which the block sees its own global writes), so the tables are the
+// same. A bigger output leaves the compacted rows for the multi-launch
+// expand. Saves four launches on the per-step path's small steps, whose
+// time is launch latency.
+constexpr uint64_t kSmallRows = 2 * kBlock;
+constexpr uint64_t kSmallCap = 4 * kExpandNV;  // output rows expanded in the same launch
+
+template <bool SORTED, bool FRAMED>
+__global__ __launch_bounds__(kBlock) void k_small_step(GraphView g, StepParams p, ExpandParams x, uint64_t cap,
+                                                       uint64_t *part, Signal sg) {
+  for (uint64_t lt = 0; lt < p.ntiles; lt++) {
+    probe_tile<kBlock, SORTED, FRAMED>(g, p, lt);
+    __syncthreads();  // the tile's LDS is reused by the next
+  }
+  scan_chunk(p, 0, 1);
+  __syncthreads();
+  for (uint64_t lt = threadIdx.x / kWave; lt < p.ntiles; lt += kBlock / kWave) finalize_tile<kBlock>(p, lt);
+  __syncthreads();
+  const uint64_t E = p.totals[0], nz = p.totals[2];
+  if (p.mode == kModeK2U && E != 0 && E <= cap) {
+    const uint64_t nx = (nz + E + kExpandNV - 1) / kExpandNV;
+    for (uint64_t k = threadIdx.x; k <= nx; k += kBlock) part[k] = merge_split<kExpandNV>(p.cexcl, nz, E, k);
+    __syncthreads();
+    ExpandParams xx = x;
+    xx.R = nz;
+    xx.E = E;
+    xx.part = part;
+    for (uint64_t t = 0; t < nx; t++) expand_tile<kExpandVT, FRAMED>(g, xx, t);
+  }
+  if (threadIdx.x == 0) publish(sg, p.totals, 3);  // (thread 0 wrote the totals)
+}
+
+// A small table to the host: copies n cells into the pinned staging area,
+// then publishes (every wave's stores complete at system scope first).
+__global__ __launch_bounds__(kBlock) void k_stage(const int32_t *src, uint64_t n, int32_t *dst, Signal sg) {
+  for (uint64_t i = threadIdx.x; i < n; i += kBlock) dst[i] = src[i];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) publish(sg, nullptr, 0);
+}
+
+// Single-key probe for index_to_unknown / const_to_unknown: publishes the
+// degree and the offset in the predicate segment. One wave.
 template <bool FRAMED>
-__global__ __launch_bounds__(kWave) void k_probe_one(GraphView g, uint64_t key, uint32_t pid, uint64_t *out) {
+__global__ __launch_bounds__(kWave) void k_probe_one(GraphView g, uint64_t key, uint32_t pid, Signal sg) {
   __shared__ WaveProbeLds s_wp;
   const DevMeta m = g.meta[pid];
   uint32_t deg, off;
   ProbeCarry carry;
   wave_probe<FRAMED>(g, m, threadIdx.x == 0 ? key : 0, s_wp, deg, off, carry);
   if (threadIdx.x == 0) {
-    out[0] = deg;
-    out[1] = off;
+    const uint64_t v[2] = {deg, off};
+    publish(sg, v, 2);
   }
 }
 

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -225,9 +226,40 @@ void wkg_ctx::unref_table(wkg_table *t) {
   delete t;
 }
 
+// Copies n (<= kRbWords) device counters to the host through the pinned block.
+__global__ void k_signal(const uint64_t *d, int n, Signal sg) {
+  if (threadIdx.x == 0) {
+    uint64_t v[kRbWords];
+    for (int i = 0; i < n; i++) v[i] = d[i];
+    publish(sg, v, n);
+  }
+}
+
+// The host learns a step's counts by polling the flag that a kernel queued
+// behind the step's work sets in pinned host memory: no copy command and no
+// stream synchronisation, whose wake-up latency dominated small steps. After
+// 20 ms of polling it waits on the stream instead (which also reports a
+// failed kernel whose signal never comes).
+int wkg_ctx::wait_signal(const Signal &sg, hipStream_t s) {
+  const uint64_t *flag = h_rb + kRbFlag;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t spins = 0; __atomic_load_n(flag, __ATOMIC_ACQUIRE) != sg.seq; spins++) {
+    if ((spins & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) {
+      HIPCHK(this, hipStreamSynchronize(s));
+      if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) != sg.seq)
+        return fail(WKG_EDEVICE, "read-back signal %llu missing after the stream completed",
+                    (unsigned long long)sg.seq);
+      break;
+    }
+  }
+  return WKG_OK;
+}
+
 int wkg_ctx::read_counters(const uint64_t *d, int n, hipStream_t s, uint64_t *host) {
-  HIPCHK(this, hipMemcpyAsync(h_rb, d, n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  HIPCHK(this, hipStreamSynchronize(s));
+  const Signal sg = next_signal();
+  k_signal<<<1, kWave, 0, s>>>(d, n, sg);
+  LAUNCHCHK(this);
+  if (int rc = wait_signal(sg, s)) return rc;
   memcpy(host, h_rb, n * sizeof(uint64_t));
   return WKG_OK;
 }
@@ -245,19 +277,16 @@ int wkg_ctx::check_pattern(int pid, int dir) {
 int wkg_ctx::step_single_list(uint64_t key, int pid, hipStream_t s, wkg_table **out, uint64_t *expanded) {
   int rc = ensure_pred(pid, s);
   if (rc) return rc;
-  uint64_t *d_rb = (uint64_t *)pool->alloc(2 * sizeof(uint64_t), s, err_slot());
-  if (!d_rb) return WKG_ENOMEM;
   GraphView g = view();
+  const Signal sg = next_signal();
   {
     ProfScope ps(this, s, "probe_one", 128.0 + 16.0);
-    if (framed) k_probe_one<true><<<1, kWave, 0, s>>>(g, key, (uint32_t)pid, d_rb);
-    else k_probe_one<false><<<1, kWave, 0, s>>>(g, key, (uint32_t)pid, d_rb);
+    if (framed) k_probe_one<true><<<1, kWave, 0, s>>>(g, key, (uint32_t)pid, sg);
+    else k_probe_one<false><<<1, kWave, 0, s>>>(g, key, (uint32_t)pid, sg);
   }
   LAUNCHCHK(this);
-  uint64_t hv[2];
-  rc = read_counters(d_rb, 2, s, hv);
-  pool->release(d_rb, s);
-  if (rc) return rc;
+  if ((rc = wait_signal(sg, s))) return rc;
+  const uint64_t hv[2] = {h_rb[0], h_rb[1]};
   wkg_table *t;
   if ((rc = new_table(hv[0], 1, s, &t))) return rc;
   if (hv[0]) {
@@ -315,25 +344,24 @@ static void launch_finalize(int tr, const StepParams &p, hipStream_t s) {
   }
 }
 
-// Probe, tile scan and compaction of one step over `in` (the first half of
-// every known-start step): for K2U the compacted rows with edges (cinfo) and
-// their output offsets (cexcl) in pp.lin; the filters compact into O.
-int wkg_ctx::probe_pass(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval, int32_t *O,
-                        hipStream_t s, ProbePass &pp) {
+// Scratch and parameters of one step's probe / scan / finalize passes over
+// TR-row tiles, with `extra` more scratch words after them (at p's end: the
+// returned pointer is pp.scr + pp's own words).
+static int pass_setup(wkg_ctx *ctx, int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval,
+                      int32_t *O, hipStream_t s, int TR, uint64_t extra, ProbePass &pp, StepParams &p) {
   const uint64_t R = in->rows;
   const int c = in->cols;
-  const int TR = probe_tile ? probe_tile : probe_tile_rows(R, ncu);
   const uint64_t ntiles = (R + TR - 1) / TR;
-  // scratch: totals[4] | tile_sum | tile_nz | tile_ex | chunk sums x 3 | hits[ntiles * TR] (16 B)
+  // scratch: totals[4] | tile_sum | tile_nz | tile_ex | chunk sums x 3 | hits[ntiles * TR] (16 B) | extra
   const uint64_t hit_words = 2 * ntiles * TR;
   const uint64_t nchunks = (ntiles + kScanChunk - 1) / kScanChunk;
-  const uint64_t scratch_words = 4 + 3 * ntiles + 3 * nchunks + hit_words;
-  pp.scr = (uint64_t *)pool->alloc(scratch_words * 8, s, err_slot());
+  const uint64_t scratch_words = 4 + 3 * ntiles + 3 * nchunks + hit_words + extra;
+  pp.scr = (uint64_t *)ctx->pool->alloc(scratch_words * 8, s, wkg_ctx::err_slot());
   if (!pp.scr) return WKG_ENOMEM;
   if (mode == kModeK2U) {  // cinfo[R] (8 B) | cexcl[R]: may outlive the step as the output's lineage
-    pp.lin = pool->alloc(R * 16, s, err_slot());
+    pp.lin = ctx->pool->alloc(R * 16, s, wkg_ctx::err_slot());
     if (!pp.lin) {
-      release_pass(pp, s);
+      ctx->release_pass(pp, s);
       return WKG_ENOMEM;
     }
     pp.cinfo = (uint2 *)pp.lin;
@@ -342,7 +370,7 @@ int wkg_ctx::probe_pass(int mode, const wkg_table *in, int sc, int pid, int dir,
   uint64_t *scr = pp.scr;
   uint64_t *totals = scr, *tile_sum = scr + 4, *tile_nz = tile_sum + ntiles, *tile_ex = tile_nz + ntiles;
   uint64_t *chunks = tile_ex + ntiles;
-  StepParams p{};
+  p = StepParams{};
   p.T = in->d;
   p.R = R;
   p.c = (uint32_t)c;
@@ -366,6 +394,22 @@ int wkg_ctx::probe_pass(int mode, const wkg_table *in, int sc, int pid, int dir,
   p.chunk_sum = chunks;
   p.chunk_nz = chunks + nchunks;
   p.chunk_ex = chunks + 2 * nchunks;
+  return WKG_OK;
+}
+
+// Probe, tile scan and compaction of one step over `in` (the first half of
+// every known-start step): for K2U the compacted rows with edges (cinfo) and
+// their output offsets (cexcl) in pp.lin; the filters compact into O.
+int wkg_ctx::probe_pass(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval, int32_t *O,
+                        hipStream_t s, ProbePass &pp) {
+  const uint64_t R = in->rows;
+  const int c = in->cols;
+  const int TR = probe_tile ? probe_tile : probe_tile_rows(R, ncu);
+  StepParams p{};
+  int rc = pass_setup(this, mode, in, sc, pid, dir, ec, cval, O, s, TR, 0, pp, p);
+  if (rc) return rc;
+  const uint64_t ntiles = p.ntiles, nchunks = (ntiles + kScanChunk - 1) / kScanChunk;
+  uint64_t *totals = p.totals;
   const GraphView g = view();
   {
     // the input table (4 R c: its rows are read whole, as cache lines); K2U
@@ -385,7 +429,7 @@ int wkg_ctx::probe_pass(int mode, const wkg_table *in, int sc, int pid, int dir,
     launch_finalize(TR, p, s);
   }
   hipError_t le = hipGetLastError();
-  int rc = le == hipSuccess ? read_counters(totals, 3, s, pp.cnt) : WKG_OK;
+  rc = le == hipSuccess ? read_counters(totals, 3, s, pp.cnt) : WKG_OK;
   if (le != hipSuccess || rc) {
     release_pass(pp, s);
     return le != hipSuccess ? fail(WKG_EDEVICE, "probe launch failed: %s", hipGetErrorString(le)) : rc;
@@ -593,6 +637,7 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
   }
   if (mode == kModeK2U && lineage && in->lin_parent && sc < in->lin_parent->cols)
     return step_k2u_lineage(in, sc, pid, dir, s, out, expanded);
+  if (small_steps && R <= kSmallRows) return step_small(mode, in, sc, pid, dir, ec, cval, s, out, expanded);
 
   wkg_table *t = nullptr;
   if (mode != kModeK2U && (rc = new_table(R, c, s, &t))) return rc;
@@ -623,6 +668,94 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
   keep_lineage(in, pp, nz, t);
   release_pass(pp, s);
   *out = t;
+  return WKG_OK;
+}
+
+// A known-start step over at most kSmallRows rows in one launch
+// (k_small_step) and one read-back of its totals. A known_to_unknown output
+// of more than kSmallCap rows is expanded by the multi-launch expand from
+// the compacted rows the launch left.
+int wkg_ctx::step_small(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval,
+                        hipStream_t s, wkg_table **out, uint64_t *expanded) {
+  const uint64_t R = in->rows;
+  const int c = in->cols;
+  const int out_cols = mode == kModeK2U ? c + 1 : c;
+  wkg_table *t = nullptr;
+  int rc = new_table(mode == kModeK2U ? kSmallCap : R, out_cols, s, &t);
+  if (rc) return rc;
+  ProbePass pp;
+  StepParams p{};
+  const uint64_t part_words = kSmallCap / kExpandNV + 2;  // split points of the in-launch expand
+  if ((rc = pass_setup(this, mode, in, sc, pid, dir, ec, cval, mode == kModeK2U ? nullptr : t->d, s, kBlock,
+                       part_words, pp, p))) {
+    drop_table(t);
+    return rc;
+  }
+  uint64_t *part = (uint64_t *)(p.hits + p.ntiles * kBlock);
+  ExpandParams x{};
+  x.pid = (uint32_t)pid;
+  x.T = in->d;
+  x.c = (uint32_t)c;
+  x.cinfo = p.cinfo;
+  x.cexcl = p.cexcl;
+  x.O = t->d;
+  const GraphView g = view();
+  const bool srt = sorted || mode == kModeK2U;
+  const Signal sg = next_signal();
+  {
+    ProfScope ps(this, s, "small_step", 4.0 * (double)R * c + 128.0 * (double)R);
+    if (framed) {
+      if (srt) k_small_step<true, true><<<1, kBlock, 0, s>>>(g, p, x, kSmallCap, part, sg);
+      else k_small_step<false, true><<<1, kBlock, 0, s>>>(g, p, x, kSmallCap, part, sg);
+    } else {
+      if (srt) k_small_step<true, false><<<1, kBlock, 0, s>>>(g, p, x, kSmallCap, part, sg);
+      else k_small_step<false, false><<<1, kBlock, 0, s>>>(g, p, x, kSmallCap, part, sg);
+    }
+  }
+  hipError_t le = hipGetLastError();
+  rc = le == hipSuccess ? wait_signal(sg, s) : fail(WKG_EDEVICE, "small step launch failed: %s", hipGetErrorString(le));
+  if (rc) {
+    release_pass(pp, s);
+    drop_table(t);
+    return rc;
+  }
+  memcpy(pp.cnt, h_rb, 3 * sizeof(uint64_t));
+  if (mode != kModeK2U) {  // filters: kept rows compacted in place
+    t->rows = pp.cnt[0];
+    t->bytes = t->rows * (uint64_t)c * 4;
+    if (expanded) *expanded = pp.cnt[1];
+    release_pass(pp, s);
+    *out = t;
+    return WKG_OK;
+  }
+  const uint64_t E = pp.cnt[0], nz = pp.cnt[2];
+  if (table_limit && R >= 2 && E * (uint64_t)out_cols * 4 > table_limit) {  // (one row cannot be split)
+    release_pass(pp, s);
+    drop_table(t);
+    return fail(WKG_ENOMEM, "a %llu-row output is above the table limit", (unsigned long long)E);
+  }
+  if (E <= kSmallCap) {  // expanded in the launch
+    t->rows = E;
+    t->bytes = E * (uint64_t)out_cols * 4;
+  } else {
+    drop_table(t);
+    t = nullptr;
+    if ((rc = new_table(E, out_cols, s, &t)) || (rc = expand_pass(in, pid, pp.cinfo, pp.cexcl, nz, E, s, t))) {
+      release_pass(pp, s);
+      drop_table(t);
+      return rc;
+    }
+  }
+  if (expanded) *expanded = E;
+  keep_lineage(in, pp, nz, t);
+  release_pass(pp, s);
+  *out = t;
+  return WKG_OK;
+}
+
+int wkg_set_small_steps(wkg_ctx *ctx, int on) {
+  if (!ctx) return WKG_EINVAL;
+  ctx->small_steps = on != 0;
   return WKG_OK;
 }
 
@@ -818,10 +951,15 @@ int wkg_init(int device, uint64_t table_budget_bytes, wkg_ctx **out) {
   if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || c->ncu < 1)
     c->ncu = 256;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipHostMalloc((void **)&c->h_rb, 4096, hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void **)&c->h_rb, kStageOff + kStageBytes, hipHostMallocCoherent | hipHostMallocMapped) !=
+          hipSuccess ||
+      hipHostGetDevicePointer((void **)&c->h_rb_dev, c->h_rb, 0) != hipSuccess) {
+    if (c->h_rb) (void)hipHostFree(c->h_rb);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return WKG_EDEVICE;
   }
+  memset(c->h_rb, 0, kStageOff);  // the signal flag starts below every sequence number
   size_t fr = 0, tot = 0;
   (void)hipMemGetInfo(&fr, &tot);
   c->pool = new Pool(table_budget_bytes ? table_budget_bytes : (uint64_t)(fr * 0.75));
@@ -867,6 +1005,14 @@ int wkg_table_to_host(wkg_ctx *ctx, const wkg_table *t, int32_t *host, void *str
   if (!t->bytes) return WKG_OK;
   if (!host) return ctx->fail(WKG_EINVAL, "null host buffer");
   hipStream_t s = ctx->pick(stream);
+  if (t->bytes <= kStageBytes) {  // through the pinned staging area, polled
+    const Signal sg = ctx->next_signal();
+    k_stage<<<1, kBlock, 0, s>>>(t->d, t->bytes / 4, (int32_t *)((char *)ctx->h_rb_dev + kStageOff), sg);
+    LAUNCHCHK(ctx);
+    if (int rc = ctx->wait_signal(sg, s)) return rc;
+    memcpy(host, (const char *)ctx->h_rb + kStageOff, t->bytes);
+    return WKG_OK;
+  }
   HIPCHK(ctx, hipMemcpyAsync(host, t->d, t->bytes, hipMemcpyDeviceToHost, s));
   HIPCHK(ctx, hipStreamSynchronize(s));
   return WKG_OK;

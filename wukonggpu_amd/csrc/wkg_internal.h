@@ -106,7 +106,10 @@ struct wkg_ctx {
   uint64_t table_limit = 0;  // largest step output table in bytes (0: no limit)
   int probe_tile = 0;        // rows per probe tile of the per-step calls (0: by input size)
   bool lineage = true;       // probe inherited columns once per parent row
-  uint64_t *h_rb = nullptr;  // pinned read-back words
+  bool small_steps = true;   // steps over at most kSmallRows rows in one launch
+  uint64_t *h_rb = nullptr;  // pinned read-back block: counters, the signal flag, small-table staging
+  uint64_t *h_rb_dev = nullptr;  // the same block's device address
+  uint64_t rb_seq = 0;       // the last read-back's signal value
   // The calling thread's last failure message (wkg_last_error): the served
   // path drives one context from two host threads, so it is kept per thread.
   static std::string &err_slot() {
@@ -154,12 +157,16 @@ struct wkg_ctx {
   void drop_table(wkg_table *t);
   void unref_table(wkg_table *t);
   int read_counters(const uint64_t *d, int n, hipStream_t s, uint64_t *host);
+  wkg::Signal next_signal() { return wkg::Signal{h_rb_dev, ++rb_seq}; }
+  int wait_signal(const wkg::Signal &sg, hipStream_t s);
   int check_pattern(int pid, int dir);
   int step_single_list(uint64_t key, int pid, hipStream_t s, wkg_table **out, uint64_t *expanded);
   int step_known(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval, hipStream_t s,
                  wkg_table **out, uint64_t *expanded);
   int probe_pass(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval, int32_t *O,
                  hipStream_t s, ProbePass &pp);
+  int step_small(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval, hipStream_t s,
+                 wkg_table **out, uint64_t *expanded);
   void release_pass(ProbePass &pp, hipStream_t s);
   int expand_pass(const wkg_table *in, int pid, const uint2 *cinfo, const uint64_t *cexcl, uint64_t nz, uint64_t E,
                   hipStream_t s, wkg_table *t);

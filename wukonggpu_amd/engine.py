@@ -138,6 +138,10 @@ class Device:
         self._check(N.hip_lib().wkg_pool_stats(self._h, *[C.byref(x) for x in v]))
         return {k: int(x.value) for k, x in zip(("live_bytes", "cached_bytes", "mallocs", "trims"), v)}
 
+    def set_small_steps(self, on):
+        """Known-start steps over at most 512 rows in one launch (on by default)."""
+        self._check(N.hip_lib().wkg_set_small_steps(self._h, 1 if on else 0))
+
     def set_lineage(self, on):
         """Probe a star's next arm once per input row of the fan-out before it (default on)."""
         self._check(N.hip_lib().wkg_set_lineage(self._h, 1 if on else 0))
@@ -401,11 +405,12 @@ class GPUEngine:
         return self._collect(res, n), wall.value
 
     def serve_closed(self, queries, heavy, light_inflight, heavy_inflight, duration_s, table_budget_bytes=0,
-                     light_lanes=2, keep=None):
+                     light_lanes=2, keep=None, interval_s=0.0):
         """Closed-loop batch mode: (ClosedResults, issue_s, wall); issue_s[q] <
         0 for queries never issued. queries: a list of cmd_chains or a (flat,
         offsets) pair; keep: optional mask of the result tables to return (the
-        others are blind: row counts only)."""
+        others are blind: row counts only); interval_s: least time between
+        two issue rounds."""
         L = N.engine_lib()
         flat, off = pack_queries(queries)
         n = len(off) - 1
@@ -415,7 +420,8 @@ class GPUEngine:
         issue = np.zeros(max(n, 1), dtype=np.float64)
         wall = C.c_double()
         rc = L.wke_serve_closed(self._h, _i64p(flat), _i64p(off), n, hv.ctypes.data_as(C.POINTER(C.c_uint8)),
-                                int(light_inflight), int(heavy_inflight), float(duration_s), table_budget_bytes,
+                                int(light_inflight), int(heavy_inflight), float(duration_s), float(interval_s),
+                                table_budget_bytes,
                                 light_lanes, None if kp is None else kp.ctypes.data_as(C.POINTER(C.c_uint8)), res,
                                 issue.ctypes.data_as(C.POINTER(C.c_double)), C.byref(wall))
         if rc != 0:
