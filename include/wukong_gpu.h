@@ -181,7 +181,9 @@ int wkg_table_concat(wkg_ctx *ctx, int n, const wkg_table *const *parts, void *s
 /* ---- pattern steps (one RCache handler each) -------------------------------------
  * `expanded` (may be NULL) receives the step's work count: edges emitted for
  * index/const/known_to_unknown, edges examined by the reference's early-exit
- * linear scan for known_to_known/known_to_const. */
+ * linear scan for known_to_known/known_to_const. A step returns once its
+ * row count is known; its output table is complete in the order of its
+ * stream (later work there sees it; other streams must wait for it). */
 
 /* index_to_unknown: rows = the index list of key (0, idx_id, dir).
  * RCache::index_to_unknown, core/rcache.hpp:811-962. */

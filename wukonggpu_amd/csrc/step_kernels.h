@@ -54,6 +54,9 @@ struct StepParams {
   // two-level tile scan (null when one block scans all tiles): per chunk of
   // kScanChunk tiles, the chunk's prefix of output rows / rows with output
   uint64_t *chunk_sum, *chunk_nz, *chunk_ex;
+  // per-step path: the totals are also published to the host by the kernel
+  // that computes them (null: not published)
+  Signal sg = {nullptr, 0};
 };
 
 constexpr int kScanChunk = 8 * kBlock;  // tiles per block in the two-level tile scan
@@ -268,6 +271,10 @@ __device__ __forceinline__ void scan_step_tiles(const StepParams &p) {
     p.totals[0] = rows;
     p.totals[1] = ex;
     p.totals[2] = nz;
+    if (p.sg.h) {
+      const uint64_t v[3] = {rows, ex, nz};
+      publish(p.sg, v, 3);
+    }
   }
 }
 
@@ -309,6 +316,10 @@ __device__ __forceinline__ void scan_chunk(const StepParams &p, uint64_t chunk, 
       p.totals[2] = tn;
       p.chunk_sum[0] = 0;
       p.chunk_nz[0] = 0;
+      if (p.sg.h) {
+        const uint64_t v[3] = {ts, te, tn};
+        publish(p.sg, v, 3);
+      }
     } else {
       p.chunk_sum[chunk] = ts;
       p.chunk_nz[chunk] = tn;
@@ -331,6 +342,10 @@ __device__ __forceinline__ void scan_top(const StepParams &p, uint64_t nchunks) 
     p.totals[0] = rows;
     p.totals[1] = ex;
     p.totals[2] = nz;
+    if (p.sg.h) {
+      const uint64_t v[3] = {rows, ex, nz};
+      publish(p.sg, v, 3);
+    }
   }
 }
 

@@ -419,6 +419,8 @@ int wkg_ctx::probe_pass(int mode, const wkg_table *in, int sc, int pid, int dir,
     ProfScope ps(this, s, mode == kModeK2U ? "probe" : "probe_filter", b);
     launch_probe(TR, framed, sorted || mode == kModeK2U, g, p, s);
   }
+  // the scan publishes the totals; the host goes on while the finalize runs
+  p.sg = next_signal();
   {
     ProfScope ps(this, s, "scan_tiles", 48.0 * (double)ntiles);
     k_scan_chunks<<<(unsigned)nchunks, kBlock, 0, s>>>(p);
@@ -429,11 +431,12 @@ int wkg_ctx::probe_pass(int mode, const wkg_table *in, int sc, int pid, int dir,
     launch_finalize(TR, p, s);
   }
   hipError_t le = hipGetLastError();
-  rc = le == hipSuccess ? read_counters(totals, 3, s, pp.cnt) : WKG_OK;
+  rc = le == hipSuccess ? wait_signal(p.sg, s) : WKG_OK;
   if (le != hipSuccess || rc) {
     release_pass(pp, s);
     return le != hipSuccess ? fail(WKG_EDEVICE, "probe launch failed: %s", hipGetErrorString(le)) : rc;
   }
+  memcpy(pp.cnt, h_rb, 3 * sizeof(uint64_t));
   if (mode == kModeK2U) {
     prof_add_bytes("finalize", 32.0 * pp.cnt[2]);
     prof_add_bytes("probe", 128.0 * (double)pp.cnt[1] + 16.0 * (double)pp.cnt[2]);
