@@ -217,11 +217,7 @@ extern "C" int wkg_step_batch(wkg_ctx *ctx, int n, const wkg_step_desc *steps, v
   if ((e = hipGetLastError()) != hipSuccess)
     return bail(ctx->fail(WKG_EDEVICE, "batched probe launch failed: %s", hipGetErrorString(e)));
   std::vector<uint64_t> tot(4 * (size_t)n);
-  {
-    e = hipMemcpyAsync(tot.data(), totals, tot.size() * 8, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return bail(ctx->fail(WKG_EDEVICE, "totals read-back failed: %s", hipGetErrorString(e)));
-  }
+  if (int rc = ctx->read_words(totals, tot.size(), s, tot.data())) return bail(rc);
 
   // ---- outputs and the expand pass --------------------------------------------------
   std::vector<uint64_t> partb(n + 1, 0), xtileb(n + 1, 0);

@@ -226,15 +226,6 @@ void wkg_ctx::unref_table(wkg_table *t) {
   delete t;
 }
 
-// Copies n (<= kRbWords) device counters to the host through the pinned block.
-__global__ void k_signal(const uint64_t *d, int n, Signal sg) {
-  if (threadIdx.x == 0) {
-    uint64_t v[kRbWords];
-    for (int i = 0; i < n; i++) v[i] = d[i];
-    publish(sg, v, n);
-  }
-}
-
 // The host learns a step's counts by polling the flag that a kernel queued
 // behind the step's work sets in pinned host memory: no copy command and no
 // stream synchronisation, whose wake-up latency dominated small steps. After
@@ -255,12 +246,20 @@ int wkg_ctx::wait_signal(const Signal &sg, hipStream_t s) {
   return WKG_OK;
 }
 
-int wkg_ctx::read_counters(const uint64_t *d, int n, hipStream_t s, uint64_t *host) {
+// n device words to the host: through the pinned staging area (a one-block
+// copy kernel, then the polled flag) when they fit, else a copy command.
+int wkg_ctx::read_words(const uint64_t *d, uint64_t n, hipStream_t s, uint64_t *host) {
+  if (!n) return WKG_OK;
+  if (n * 8 > kStageBytes) {
+    HIPCHK(this, hipMemcpyAsync(host, d, n * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(this, hipStreamSynchronize(s));
+    return WKG_OK;
+  }
   const Signal sg = next_signal();
-  k_signal<<<1, kWave, 0, s>>>(d, n, sg);
+  k_stage<<<1, kBlock, 0, s>>>((const int32_t *)d, 2 * n, (int32_t *)((char *)h_rb_dev + kStageOff), sg);
   LAUNCHCHK(this);
   if (int rc = wait_signal(sg, s)) return rc;
-  memcpy(host, h_rb, n * sizeof(uint64_t));
+  memcpy(host, (const char *)h_rb + kStageOff, n * 8);
   return WKG_OK;
 }
 
@@ -559,6 +558,7 @@ int wkg_ctx::step_k2u_lineage(const wkg_table *in, int sc, int pid, int dir, hip
     q.chunk_sum = q.tile_nz + nk;
     q.chunk_nz = q.chunk_sum + nchunks;
     q.chunk_ex = q.chunk_nz + nchunks;
+    q.sg = next_signal();  // the scan publishes the totals
     {
       ProfScope ps(this, s, "lineage", 48.0 * (double)nk);
       k_lift<<<(unsigned)std::min<uint64_t>((nk + kBlock - 1) / kBlock, 65535), kBlock, 0, s>>>(kept, q.tile_nz,
@@ -568,8 +568,9 @@ int wkg_ctx::step_k2u_lineage(const wkg_table *in, int sc, int pid, int dir, hip
     }
     hipError_t le = hipGetLastError();
     uint64_t cnt[3];
-    rc = le == hipSuccess ? read_counters(lift, 3, s, cnt) : fail(WKG_EDEVICE, "lineage launch failed: %s",
-                                                                    hipGetErrorString(le));
+    rc = le == hipSuccess ? wait_signal(q.sg, s) : fail(WKG_EDEVICE, "lineage launch failed: %s",
+                                                         hipGetErrorString(le));
+    if (!rc) memcpy(cnt, h_rb, sizeof cnt);
     if (rc) {
       pool->release(lift, s);
       release_pass(pp, s);
@@ -1008,7 +1009,9 @@ int wkg_table_to_host(wkg_ctx *ctx, const wkg_table *t, int32_t *host, void *str
   if (!t->bytes) return WKG_OK;
   if (!host) return ctx->fail(WKG_EINVAL, "null host buffer");
   hipStream_t s = ctx->pick(stream);
-  if (t->bytes <= kStageBytes) {  // through the pinned staging area, polled
+  if (t->bytes <= kStageBytes && t->bytes % 8 == 0)  // through the pinned staging area, polled
+    return ctx->read_words((const uint64_t *)t->d, t->bytes / 8, s, (uint64_t *)host);
+  if (t->bytes <= kStageBytes) {
     const Signal sg = ctx->next_signal();
     k_stage<<<1, kBlock, 0, s>>>(t->d, t->bytes / 4, (int32_t *)((char *)ctx->h_rb_dev + kStageOff), sg);
     LAUNCHCHK(ctx);

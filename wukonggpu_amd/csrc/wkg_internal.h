@@ -156,7 +156,7 @@ struct wkg_ctx {
   int new_table(uint64_t rows, int cols, hipStream_t s, wkg_table **out);
   void drop_table(wkg_table *t);
   void unref_table(wkg_table *t);
-  int read_counters(const uint64_t *d, int n, hipStream_t s, uint64_t *host);
+  int read_words(const uint64_t *d, uint64_t n, hipStream_t s, uint64_t *host);
   wkg::Signal next_signal() { return wkg::Signal{h_rb_dev, ++rb_seq}; }
   int wait_signal(const wkg::Signal &sg, hipStream_t s);
   int check_pattern(int pid, int dir);
