@@ -9,7 +9,8 @@
 // one-thread-per-row update (core/gpu_hash.cu:145-316, 527-610, 685-694).
 // known_to_known / known_to_const (core/gpu_hash.cu:320-466, 613-682) run as
 //   k_probe -> k_scan_tiles -> k_finalize (stable compaction of kept rows)
-// index_to_unknown / const_to_unknown are k_probe_one + k_copy_list.
+// index_to_unknown / const_to_unknown are k_probe_copy (+ k_copy_list for a
+// list above kListCap entries).
 //
 // The probe writes, per tile, the list of rows that have edges (or that a
 // filter keeps) and the tile's sums; the tile scan turns the sums into
@@ -606,15 +607,29 @@ __global__ __launch_bounds__(kBlock) void k_stage(const int32_t *src, uint64_t n
   if (threadIdx.x == 0) publish(sg, nullptr, 0);
 }
 
-// Single-key probe for index_to_unknown / const_to_unknown: publishes the
-// degree and the offset in the predicate segment. One wave.
+// Index / const start in one launch: wave 0 probes the key; when the list
+// has at most `cap` entries the workgroup copies it to out; then the degree
+// and offset are published.
+constexpr uint64_t kListCap = 8192;
 template <bool FRAMED>
-__global__ __launch_bounds__(kWave) void k_probe_one(GraphView g, uint64_t key, uint32_t pid, Signal sg) {
+__global__ __launch_bounds__(kBlock) void k_probe_copy(GraphView g, uint64_t key, uint32_t pid, uint64_t cap,
+                                                       int32_t *out, Signal sg) {
   __shared__ WaveProbeLds s_wp;
+  __shared__ uint32_t s_deg, s_off;
   const DevMeta m = g.meta[pid];
-  uint32_t deg, off;
-  ProbeCarry carry;
-  wave_probe<FRAMED>(g, m, threadIdx.x == 0 ? key : 0, s_wp, deg, off, carry);
+  if (threadIdx.x < kWave) {
+    uint32_t deg, off;
+    ProbeCarry carry;
+    wave_probe<FRAMED>(g, m, threadIdx.x == 0 ? key : 0, s_wp, deg, off, carry);
+    if (threadIdx.x == 0) {
+      s_deg = deg;
+      s_off = off;
+    }
+  }
+  __syncthreads();
+  const uint64_t deg = s_deg, off = s_off;
+  if (deg <= cap)
+    for (uint64_t j = threadIdx.x; j < deg; j += kBlock) out[j] = (int32_t)edge_at<FRAMED>(g, m.edge_start, m.eframe_base, off + j);
   if (threadIdx.x == 0) {
     const uint64_t v[2] = {deg, off};
     publish(sg, v, 2);

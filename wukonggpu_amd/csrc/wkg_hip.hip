@@ -276,28 +276,38 @@ int wkg_ctx::check_pattern(int pid, int dir) {
 int wkg_ctx::step_single_list(uint64_t key, int pid, hipStream_t s, wkg_table **out, uint64_t *expanded) {
   int rc = ensure_pred(pid, s);
   if (rc) return rc;
+  // a list of at most kListCap entries is copied by the probing launch
+  wkg_table *t;
+  if ((rc = new_table(kListCap, 1, s, &t))) return rc;
   GraphView g = view();
   const Signal sg = next_signal();
   {
     ProfScope ps(this, s, "probe_one", 128.0 + 16.0);
-    if (framed) k_probe_one<true><<<1, kWave, 0, s>>>(g, key, (uint32_t)pid, sg);
-    else k_probe_one<false><<<1, kWave, 0, s>>>(g, key, (uint32_t)pid, sg);
+    if (framed) k_probe_copy<true><<<1, kBlock, 0, s>>>(g, key, (uint32_t)pid, kListCap, t->d, sg);
+    else k_probe_copy<false><<<1, kBlock, 0, s>>>(g, key, (uint32_t)pid, kListCap, t->d, sg);
   }
-  LAUNCHCHK(this);
-  if ((rc = wait_signal(sg, s))) return rc;
-  const uint64_t hv[2] = {h_rb[0], h_rb[1]};
-  wkg_table *t;
-  if ((rc = new_table(hv[0], 1, s, &t))) return rc;
-  if (hv[0]) {
-    const uint64_t blocks = std::min<uint64_t>((hv[0] + kBlock - 1) / kBlock, 4096);
+  hipError_t le = hipGetLastError();
+  rc = le == hipSuccess ? wait_signal(sg, s) : fail(WKG_EDEVICE, "probe launch failed: %s", hipGetErrorString(le));
+  if (rc) {
+    drop_table(t);
+    return rc;
+  }
+  const uint64_t deg = h_rb[0], off = h_rb[1];
+  if (deg <= kListCap) {
+    t->rows = deg;
+    t->bytes = deg * 4;
+  } else {
+    drop_table(t);
+    if ((rc = new_table(deg, 1, s, &t))) return rc;
+    const uint64_t blocks = std::min<uint64_t>((deg + kBlock - 1) / kBlock, 4096);
     {
-      ProfScope ps(this, s, "copy_list", 8.0 * (double)hv[0]);
-      if (framed) k_copy_list<true><<<blocks, kBlock, 0, s>>>(g, (uint32_t)pid, hv[1], hv[0], t->d);
-      else k_copy_list<false><<<blocks, kBlock, 0, s>>>(g, (uint32_t)pid, hv[1], hv[0], t->d);
+      ProfScope ps(this, s, "copy_list", 8.0 * (double)deg);
+      if (framed) k_copy_list<true><<<blocks, kBlock, 0, s>>>(g, (uint32_t)pid, off, deg, t->d);
+      else k_copy_list<false><<<blocks, kBlock, 0, s>>>(g, (uint32_t)pid, off, deg, t->d);
     }
     LAUNCHCHK(this);
   }
-  if (expanded) *expanded = hv[0];
+  if (expanded) *expanded = deg;
   *out = t;
   return WKG_OK;
 }
