@@ -426,7 +426,7 @@ def batch_mode(eng, gs, P, light_types, t0, args, budget=64 << 30):
     dur = args.batch_s
     warm = dur / 4
     caps = [int(x) for x in args.batch_p.split(",") if x]
-    n = 4_000_000
+    n = 6_000_000
     t = time.perf_counter()
     flat, off, heavy, types = workloads.batch_stream(gs, P, light_types, t0, n, args.seed + 51)
     log(f"batch mode: {n} queries from 6 light templates and 4 heavy queries in {time.perf_counter() - t:.1f}s")
