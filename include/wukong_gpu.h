@@ -143,6 +143,14 @@ int wkg_set_lineage(wkg_ctx *ctx, int on);
  * five; its output is the same, row for row. On by default; 0 turns it off.
  * A launch-count optimisation with no counterpart in the reference. */
 int wkg_set_small_steps(wkg_ctx *ctx, int on);
+/* Input rows per launch of the calling thread's per-step probe and
+ * compaction passes (the expand: rows / 64 tiles of 512 items per launch);
+ * 0 restores the default, 2^24. The GPU runs another queue's kernels only
+ * between this queue's kernels, so smaller launches let concurrent work (a
+ * served light batch, another process) in sooner, at a few microseconds per
+ * extra launch. A scheduling knob with no counterpart in the reference;
+ * results do not depend on it. */
+int wkg_thread_launch_rows(wkg_ctx *ctx, uint64_t rows);
 /* Per-thread arena: until wkg_thread_arena_end, the tables and step scratch
  * that the calling thread's per-step calls allocate are carved from `bytes`
  * of pool memory reserved now, with no hipMalloc; wkg_thread_arena_reset

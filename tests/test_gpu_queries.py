@@ -199,6 +199,33 @@ def test_small_steps_match_oracle(graph_hubs):
     dev.close()
 
 
+def test_split_launches_match_oracle(graph_hubs):
+    """Probe, compaction and expand passes split into launches of 4096 input
+    rows (64 expand tiles) give the oracle's answers."""
+    _, gs = graph_hubs
+    dev = Device(0)
+    dev.load(gs)
+    eng = GPUEngine(dev)
+    qs = [[2, 1, 0, -1, -1, 2, 1, -2],
+          [2, 1, 0, -1, -1, 2, 1, -2, -2, 3, 0, -3],
+          [2, 1, 0, -1, -1, 2, 1, -2, -2, 2, 0, -1],
+          [2, 1, 0, -1, -1, 2, 1, -2, -2, 3, 0, -3, -3, 2, 1, -4, -3, 3, 1, -5]]
+    rng = np.random.default_rng(81)
+    qs += random_queries(rng, 6000, 3, [2, 3], 30, max_steps=5)
+    try:
+        for rows in (4096, 0):
+            print(f"launch rows {rows}", flush=True)
+            dev.set_launch_rows(rows)
+            for q in qs:
+                _check(gs, q, eng.run_query(q))
+        with pytest.raises(Exception):
+            dev.set_launch_rows(100)
+    finally:
+        dev.set_launch_rows(0)
+    eng.close()
+    dev.close()
+
+
 def test_lineage_star_arms_match_oracle(graph_hubs):
     """A star's next arm after a fan-out probes once per input row of the
     fan-out (the output keeps which input row each row repeats): the answers

@@ -129,6 +129,7 @@ def hip_lib():
         _sig(L, "wkg_table_split", C.c_int, vp, vp, C.c_int, C.c_int, vp, P(vp), P(u64), P(u64))
         _sig(L, "wkg_table_concat", C.c_int, vp, C.c_int, P(vp), vp, P(vp))
         _sig(L, "wkg_set_small_steps", C.c_int, vp, C.c_int)
+        _sig(L, "wkg_thread_launch_rows", C.c_int, vp, u64)
         _sig(L, "wkg_set_table_limit", C.c_int, vp, u64)
         _sig(L, "wkg_set_probe_tile_rows", C.c_int, vp, C.c_int)
         _sig(L, "wkg_set_lineage", C.c_int, vp, C.c_int)

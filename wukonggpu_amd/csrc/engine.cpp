@@ -789,6 +789,8 @@ int serve_loop(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq
     // the light lanes keep it busy.
     pw.th = std::thread([&] {
       const bool arena = wkg_thread_arena_begin(e->ctx, heavy_arena) == WKG_OK;
+      // short launches, so that light batches get the GPU between them
+      (void)wkg_thread_launch_rows(e->ctx, 1ull << 22);
       {
         std::lock_guard<std::mutex> lk(pw.mu);
         pw.ready = true;
@@ -813,6 +815,7 @@ int serve_loop(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq
         pw.out.push_back({q, rc, rc ? std::string(wke_last_error(e)) : std::string()});
       }
       if (arena) (void)wkg_thread_arena_end(e->ctx);
+      (void)wkg_thread_launch_rows(e->ctx, 0);
     });
     std::unique_lock<std::mutex> lk(pw.mu);
     pw.cv.wait(lk, [&] { return pw.ready; });

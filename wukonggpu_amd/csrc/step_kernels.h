@@ -58,6 +58,7 @@ struct StepParams {
   // per-step path: the totals are also published to the host by the kernel
   // that computes them (null: not published)
   Signal sg = {nullptr, 0};
+  uint64_t tile0 = 0;  // k_probe / k_finalize: the launch's first tile (the per-step path's launches are split)
 };
 
 constexpr int kScanChunk = 8 * kBlock;  // tiles per block in the two-level tile scan
@@ -231,7 +232,7 @@ __device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams 
 // At most 168 VGPRs, so three waves per SIMD hide the probe's load chains.
 template <int TR, bool SORTED, bool FRAMED>
 __global__ __launch_bounds__(kBlock, 4) void k_probe(GraphView g, StepParams p) {
-  probe_tile<TR, SORTED, FRAMED>(g, p, blockIdx.x);
+  probe_tile<TR, SORTED, FRAMED>(g, p, p.tile0 + blockIdx.x);
 }
 
 // In-place exclusive scan of a[0, n) by one block; returns the total.
@@ -407,7 +408,7 @@ __device__ __forceinline__ void finalize_tile(const StepParams &p, uint64_t lt) 
 // One wave per tile: tile blockIdx.x * 4 + wave.
 template <int TR>
 __global__ __launch_bounds__(kBlock) void k_finalize(StepParams p) {
-  const uint64_t t = blockIdx.x * (uint64_t)(kBlock / kWave) + threadIdx.x / kWave;
+  const uint64_t t = p.tile0 + blockIdx.x * (uint64_t)(kBlock / kWave) + threadIdx.x / kWave;
   if (t < p.ntiles) finalize_tile<TR>(p, t);
 }
 
@@ -440,6 +441,7 @@ struct ExpandParams {
   const uint64_t *part;  // ntiles + 1 split points
   uint64_t E;
   int32_t *O;  // E x (c+1)
+  uint64_t tile0;  // k_expand: the launch's first tile
 };
 
 // Load-balanced expand (known_to_unknown result update): a tile owns
@@ -557,7 +559,7 @@ __device__ __forceinline__ void expand_tile(const GraphView &g, const ExpandPara
 
 template <bool FRAMED>
 __global__ __launch_bounds__(kBlock) void k_expand(GraphView g, ExpandParams a) {
-  expand_tile<kExpandVT, FRAMED>(g, a, blockIdx.x);
+  expand_tile<kExpandVT, FRAMED>(g, a, a.tile0 + blockIdx.x);
 }
 
 // ---- a small step in one launch --------------------------------------------

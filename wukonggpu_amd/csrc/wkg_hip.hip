@@ -322,15 +322,26 @@ static int probe_tile_rows(uint64_t R, int ncu) {
   return kBlock;
 }
 
+// The per-step path splits its probe, finalize and expand passes into
+// launches of thread_launch_rows() input rows (expand: that / 64 tiles of
+// 512 items): the GPU runs another queue's kernels only between this
+// queue's kernels, so one launch over a huge table would hold up every
+// concurrent batch (a served light query) for its whole length. 2^24 rows
+// is ~0.35 ms of probing; the served path's heavy worker uses 2^22.
 template <int TR>
-static void launch_probe_tr(bool framed, bool sorted, const GraphView &g, const StepParams &p, hipStream_t s) {
-  const unsigned grid = (unsigned)p.ntiles;
-  if (framed) {
-    if (sorted) k_probe<TR, true, true><<<grid, kBlock, 0, s>>>(g, p);
-    else k_probe<TR, false, true><<<grid, kBlock, 0, s>>>(g, p);
-  } else {
-    if (sorted) k_probe<TR, true, false><<<grid, kBlock, 0, s>>>(g, p);
-    else k_probe<TR, false, false><<<grid, kBlock, 0, s>>>(g, p);
+static void launch_probe_tr(bool framed, bool sorted, const GraphView &g, const StepParams &p0, hipStream_t s) {
+  const uint64_t chunk = std::max<uint64_t>(thread_launch_rows() / TR, 4);
+  StepParams p = p0;
+  for (uint64_t t0 = 0; t0 < p0.ntiles; t0 += chunk) {
+    p.tile0 = t0;
+    const unsigned grid = (unsigned)std::min<uint64_t>(chunk, p0.ntiles - t0);
+    if (framed) {
+      if (sorted) k_probe<TR, true, true><<<grid, kBlock, 0, s>>>(g, p);
+      else k_probe<TR, false, true><<<grid, kBlock, 0, s>>>(g, p);
+    } else {
+      if (sorted) k_probe<TR, true, false><<<grid, kBlock, 0, s>>>(g, p);
+      else k_probe<TR, false, false><<<grid, kBlock, 0, s>>>(g, p);
+    }
   }
 }
 
@@ -343,13 +354,19 @@ static void launch_probe(int tr, bool framed, bool sorted, const GraphView &g, c
   }
 }
 
-static void launch_finalize(int tr, const StepParams &p, hipStream_t s) {
-  const unsigned grid = (unsigned)((p.ntiles + kBlock / kWave - 1) / (kBlock / kWave));
-  switch (tr) {
-    case 256: k_finalize<256><<<grid, kBlock, 0, s>>>(p); break;
-    case 512: k_finalize<512><<<grid, kBlock, 0, s>>>(p); break;
-    case 1024: k_finalize<1024><<<grid, kBlock, 0, s>>>(p); break;
-    default: k_finalize<2048><<<grid, kBlock, 0, s>>>(p); break;
+static void launch_finalize(int tr, const StepParams &p0, hipStream_t s) {
+  const uint64_t chunk = std::max<uint64_t>(thread_launch_rows() / (uint64_t)tr, 4) & ~uint64_t(3);  // tiles, whole blocks of 4 waves
+  StepParams p = p0;
+  for (uint64_t t0 = 0; t0 < p0.ntiles; t0 += chunk) {
+    p.tile0 = t0;
+    const uint64_t n = std::min<uint64_t>(chunk, p0.ntiles - t0);
+    const unsigned grid = (unsigned)((n + kBlock / kWave - 1) / (kBlock / kWave));
+    switch (tr) {
+      case 256: k_finalize<256><<<grid, kBlock, 0, s>>>(p); break;
+      case 512: k_finalize<512><<<grid, kBlock, 0, s>>>(p); break;
+      case 1024: k_finalize<1024><<<grid, kBlock, 0, s>>>(p); break;
+      default: k_finalize<2048><<<grid, kBlock, 0, s>>>(p); break;
+    }
   }
 }
 
@@ -492,8 +509,14 @@ int wkg_ctx::expand_pass(const wkg_table *in, int pid, const uint2 *cinfo, const
     const double b = 4.0 * E + 4.0 * E * (c + 1) + 4.0 * nz * c + 16.0 * nz;
     ProfScope ps(this, s, "expand", b);
     const GraphView g = view();
-    if (framed) k_expand<true><<<(unsigned)nx, kBlock, 0, s>>>(g, x);
-    else k_expand<false><<<(unsigned)nx, kBlock, 0, s>>>(g, x);
+    // split launches, as the probe's
+    const uint64_t per = std::max<uint64_t>(thread_launch_rows() / 64, 1);
+    for (uint64_t t0 = 0; t0 < nx; t0 += per) {
+      x.tile0 = t0;
+      const unsigned grid = (unsigned)std::min<uint64_t>(per, nx - t0);
+      if (framed) k_expand<true><<<grid, kBlock, 0, s>>>(g, x);
+      else k_expand<false><<<grid, kBlock, 0, s>>>(g, x);
+    }
   }
   hipError_t e2 = hipGetLastError();
   pool->release(part, s);
@@ -764,6 +787,14 @@ int wkg_ctx::step_small(int mode, const wkg_table *in, int sc, int pid, int dir,
   keep_lineage(in, pp, nz, t);
   release_pass(pp, s);
   *out = t;
+  return WKG_OK;
+}
+
+int wkg_thread_launch_rows(wkg_ctx *ctx, uint64_t rows) {
+  if (!ctx) return WKG_EINVAL;
+  if (rows != 0 && (rows < 4096 || rows > (1ull << 32)))
+    return ctx->fail(WKG_EINVAL, "launch rows %llu outside [4096, 2^32] (0: the default)", (unsigned long long)rows);
+  thread_launch_rows() = rows ? rows : kDefaultLaunchRows;
   return WKG_OK;
 }
 

@@ -138,6 +138,10 @@ class Device:
         self._check(N.hip_lib().wkg_pool_stats(self._h, *[C.byref(x) for x in v]))
         return {k: int(x.value) for k, x in zip(("live_bytes", "cached_bytes", "mallocs", "trims"), v)}
 
+    def set_launch_rows(self, rows):
+        """Input rows per probe/compaction launch of the calling thread's steps (0: the default, 2^24)."""
+        self._check(N.hip_lib().wkg_thread_launch_rows(self._h, int(rows)))
+
     def set_small_steps(self, on):
         """Known-start steps over at most 512 rows in one launch (on by default)."""
         self._check(N.hip_lib().wkg_set_small_steps(self._h, 1 if on else 0))
