@@ -145,7 +145,7 @@ int wkg_set_lineage(wkg_ctx *ctx, int on);
 int wkg_set_small_steps(wkg_ctx *ctx, int on);
 /* Input rows per launch of the calling thread's per-step probe and
  * compaction passes (the expand: rows / 64 tiles of 512 items per launch);
- * 0 restores the default, 2^24. The GPU runs another queue's kernels only
+ * 0 restores the default, 2^26. The GPU runs another queue's kernels only
  * between this queue's kernels, so smaller launches let concurrent work (a
  * served light batch, another process) in sooner, at a few microseconds per
  * extra launch. A scheduling knob with no counterpart in the reference;

@@ -326,8 +326,8 @@ static int probe_tile_rows(uint64_t R, int ncu) {
 // launches of thread_launch_rows() input rows (expand: that / 64 tiles of
 // 512 items): the GPU runs another queue's kernels only between this
 // queue's kernels, so one launch over a huge table would hold up every
-// concurrent batch (a served light query) for its whole length. 2^24 rows
-// is ~0.35 ms of probing; the served path's heavy worker uses 2^22.
+// concurrent batch (a served light query) for its whole length. The default,
+// 2^26 rows, is ~1.4 ms of probing; the served path's heavy worker uses 2^22.
 template <int TR>
 static void launch_probe_tr(bool framed, bool sorted, const GraphView &g, const StepParams &p0, hipStream_t s) {
   const uint64_t chunk = std::max<uint64_t>(thread_launch_rows() / TR, 4);

@@ -31,7 +31,7 @@ inline BumpArena *&thread_arena() {
 }
 // Input rows per probe / finalize launch of the calling thread's per-step
 // calls (expands: rows / 64 tiles per launch); wkg_thread_launch_rows.
-constexpr uint64_t kDefaultLaunchRows = 1ull << 24;
+constexpr uint64_t kDefaultLaunchRows = 1ull << 26;
 inline uint64_t &thread_launch_rows() {
   static thread_local uint64_t r = kDefaultLaunchRows;
   return r;

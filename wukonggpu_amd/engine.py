@@ -139,7 +139,7 @@ class Device:
         return {k: int(x.value) for k, x in zip(("live_bytes", "cached_bytes", "mallocs", "trims"), v)}
 
     def set_launch_rows(self, rows):
-        """Input rows per probe/compaction launch of the calling thread's steps (0: the default, 2^24)."""
+        """Input rows per probe/compaction launch of the calling thread's steps (0: the default, 2^26)."""
         self._check(N.hip_lib().wkg_thread_launch_rows(self._h, int(rows)))
 
     def set_small_steps(self, on):
