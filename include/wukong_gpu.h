@@ -138,6 +138,15 @@ int wkg_set_probe_tile_rows(wkg_ctx *ctx, int rows);
  * the same, row for row. On by default; 0 turns it off. No counterpart in the
  * reference, which probes every row (core/gpu_hash.cu:145-316). */
 int wkg_set_lineage(wkg_ctx *ctx, int on);
+/* With lineage on, the output of such a once-per-input-row step also keeps
+ * its groups (an input row's list, joined to each of that row's children).
+ * A known_to_known / known_to_const step from that output's last column
+ * (end column, if any, one of the input's) of at least `rows` rows (default
+ * 2^22; 0: any size), and at least 4 per list entry of its groups, decides
+ * each group's list entry once and reads the decision back for every row
+ * holding it; its output and work count are the per-row filter's. No
+ * counterpart in the reference (core/gpu_hash.cu:320-466 tests every row). */
+int wkg_set_group_filter_rows(wkg_ctx *ctx, uint64_t rows);
 /* A known-start step over at most 512 input rows runs its probe, scan,
  * compaction and (up to 2048 output rows) expand in one launch instead of
  * five; its output is the same, row for row. On by default; 0 turns it off.

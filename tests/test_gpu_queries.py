@@ -255,6 +255,49 @@ def test_lineage_star_arms_match_oracle(graph_hubs):
     dev.close()
 
 
+def test_lineage_group_filters_match_oracle(graph_hubs):
+    """A filter from a star arm's new column (end column, if any, inherited
+    from before the fan-out) decides each (input row, list entry) once and
+    reads the decision back for every row holding it: the rows and the
+    examined-edge count equal the oracle's, with it on at every size, at its
+    default size and off, and it is taken."""
+    _, gs = graph_hubs
+    dev = Device(0)
+    dev.load(gs)
+    eng = GPUEngine(dev)
+    qs = []
+    for pa, pb, pc in [(4, 3, 2), (3, 4, 2), (4, 4, 3)]:
+        head = [2, 1, 0, -1, -1, pa, 1, -2, -2, pb, 0, -3, -3, pb, 1, -4, -3, pc, 1, -5]
+        qs += [head + [-5, pb, 0, -3],        # known_to_known back to the arm's start
+               head + [-5, pa, 0, -2],        # end column from before the fan-out
+               head + [-5, pc, 1, -1],
+               head + [-5, pb, 0, -4],        # end column of the fan-out itself: per row
+               head + [-5, pb, 0, VB + 7],    # known_to_const
+               head + [-5, pc, 1, VB + 5999]]
+    qs += [[2, 1, 0, -1, -1, 2, 1, -2, -1, 3, 1, -3, -3, 2, 0, -1],       # star from ?x, closed on ?x
+           [2, 1, 0, -1, -1, 2, 1, -2, -1, 3, 1, -3, -3, 4, 0, -1, -3, 3, 0, -1]]  # then a second filter
+    want = [oracle.run_query(gs, q)[:2] for q in qs]
+    for rows, lineage in [(0, True), (1 << 22, True), (0, False)]:
+        dev.set_lineage(lineage)
+        dev.set_group_filter_rows(rows)
+        dev.profile(True)
+        print("group filter rows", rows, "lineage", lineage, flush=True)
+        for q, (wt, wex) in zip(qs, want):
+            res = eng.run_query(q)
+            assert res.status == 0, q
+            assert res.table.shape[0] == wt.shape[0], q
+            assert np.array_equal(res.table.reshape(wt.shape), wt), q
+            assert res.expanded == wex, q
+        taken = dev.profile_read("lineage_filters")[2]
+        dev.profile(False)
+        if rows == 0 and lineage:
+            assert taken > 0
+        elif not lineage:
+            assert taken == 0
+    eng.close()
+    dev.close()
+
+
 @pytest.fixture(scope="module")
 def graph_mid():
     from tests.conftest import small_graph

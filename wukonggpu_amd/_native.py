@@ -133,6 +133,7 @@ def hip_lib():
         _sig(L, "wkg_set_table_limit", C.c_int, vp, u64)
         _sig(L, "wkg_set_probe_tile_rows", C.c_int, vp, C.c_int)
         _sig(L, "wkg_set_lineage", C.c_int, vp, C.c_int)
+        _sig(L, "wkg_set_group_filter_rows", C.c_int, vp, C.c_uint64)
         _sig(L, "wkg_build_graph", C.c_int, vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32), u64, dbl, P(dbl))
         _sig(L, "wkg_graph_shape", C.c_int, vp, P(u64), P(u64), P(C.c_int))
         _sig(L, "wkg_graph_download", C.c_int, vp, vp, vp, vp)

@@ -59,6 +59,9 @@ struct StepParams {
   // that computes them (null: not published)
   Signal sg = {nullptr, 0};
   uint64_t tile0 = 0;  // k_probe / k_finalize: the launch's first tile (the per-step path's launches are split)
+  // filters: each row's edges examined count `T[row * c + wc]` times (a
+  // row standing for that many rows of another table)
+  uint32_t weighted = 0, wc = 0;
 };
 
 constexpr int kScanChunk = 8 * kBlock;  // tiles per block in the two-level tile scan
@@ -206,7 +209,12 @@ __device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams 
     const uint32_t c = s_cnt[tid * kRowsPerThread + k];
     sum += c;
     nz += c ? 1u : 0u;
-    ex += s_aux[tid * kRowsPerThread + k];
+    uint64_t w = 1;
+    if (p.weighted) {
+      const uint64_t row = base + tid * kRowsPerThread + k;
+      w = row < p.R ? (uint32_t)p.T[row * p.c + p.wc] : 0u;
+    }
+    ex += (uint64_t)s_aux[tid * kRowsPerThread + k] * w;
   }
   uint64_t nz_total, sum_total, ex_total = 0;
   uint32_t pos = (uint32_t)block_excl_scan(nz, s_tmp, nz_total);
@@ -705,6 +713,129 @@ __global__ __launch_bounds__(kBlock) void k_spread(KeptRows k, const uint64_t *c
     const uint2 pi = k.info2[lo];
     cinfo[j] = make_uint2((uint32_t)(k.excl_p[pi.x] + kk), pi.y);
     cexcl[j] = out_loc[lo] + out_chunk[lo / kScanChunk] + kk * k.deg(lo);
+  }
+}
+
+// ---- filters on a star arm's output (decide once per parent group) ----
+// An output A of the inherited-column known_to_unknown above consists of
+// groups: group m < ng is kept key row m, i.e. grandparent row ginfo[m].x
+// whose list (segment offset ginfo[m].y, d_m = gs[m+1] - gs[m] entries)
+// was joined to each of its f_m child rows. Its rows are
+// [go[m], go[m] + f_m d_m), child by child, so row r of group m holds list
+// entry (r - go[m]) mod d_m in its last column and the grandparent row's
+// cells in its first columns. A known_to_known / known_to_const step from
+// that last column (with a grandparent column as the end) gives the same
+// answer for all f_m rows that hold one entry: it is decided once per
+// (group, entry) -- the d_m rows of a table S -- and each decision is
+// counted f_m times.
+
+// Group m's first child row of the output, its grandparent row and list.
+__global__ __launch_bounds__(kBlock) void k_keep_groups(KeptRows k, const uint2 *pinfo, const uint64_t *out_loc,
+                                                        const uint64_t *out_chunk, uint2 *ginfo, uint64_t *gs,
+                                                        uint64_t *go) {
+  for (uint64_t m = blockIdx.x * (uint64_t)kBlock + threadIdx.x; m < k.nk; m += (uint64_t)gridDim.x * kBlock) {
+    const uint2 i2 = k.info2[m];
+    ginfo[m] = make_uint2(pinfo[i2.x].x, i2.y);
+    gs[m] = k.excl2[m];
+    go[m] = out_loc[m] + out_chunk[m / kScanChunk];
+  }
+}
+
+// The group table: G[m] = (end cell of the grandparent row (0 without an
+// end column), f_m), the list of G row m at segment offset ginfo[m].y.
+__global__ __launch_bounds__(kBlock) void k_group_table(const int32_t *P, uint32_t pc, int32_t ec, const uint2 *ginfo,
+                                                        const uint64_t *gs, const uint64_t *go, uint64_t ng,
+                                                        uint64_t dtot, uint64_t rows, int32_t *G, uint2 *gci) {
+  for (uint64_t m = blockIdx.x * (uint64_t)kBlock + threadIdx.x; m < ng; m += (uint64_t)gridDim.x * kBlock) {
+    const uint2 gi = ginfo[m];
+    const uint64_t d = (m + 1 < ng ? gs[m + 1] : dtot) - gs[m];
+    const uint64_t f = ((m + 1 < ng ? go[m + 1] : rows) - go[m]) / d;
+    G[2 * m] = ec >= 0 ? P[(uint64_t)gi.x * pc + (uint32_t)ec] : 0;
+    G[2 * m + 1] = (int32_t)(uint32_t)f;
+    gci[m] = make_uint2((uint32_t)m, gi.y);
+  }
+}
+
+// D[row] = 1 for every row of a filter's probe pass that was kept: tile lt's
+// kept rows are hits[lt * tr + i], i below its count (tile prefixes of the
+// two-level scan).
+__global__ __launch_bounds__(kBlock) void k_hits_flags(const uint4 *hits, const uint64_t *tile_nz,
+                                                       const uint64_t *chunk_nz, uint64_t ntiles, uint64_t nz_total,
+                                                       uint32_t tr, uint8_t *D) {
+  auto pre = [&](uint64_t t) -> uint64_t {
+    return t >= ntiles ? nz_total : tile_nz[t] + chunk_nz[t / kScanChunk];
+  };
+  for (uint64_t lt = blockIdx.x; lt < ntiles; lt += gridDim.x) {
+    const uint64_t a = pre(lt), n = pre(lt + 1) - a;
+    for (uint64_t i = threadIdx.x; i < n; i += kBlock) D[hits[lt * tr + i].x] = 1;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero_bytes(uint8_t *D, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) D[i] = 0;
+}
+
+// Largest i in [lo, hi] with a[i] <= x (a ascending, a[lo] <= x).
+__device__ __forceinline__ uint64_t last_le(const uint64_t *a, uint64_t lo, uint64_t hi, uint64_t x) {
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi + 1) >> 1;
+    if (a[mid] <= x) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Filter pass over one TR-row tile of a group-structured table: row r of
+// group m is kept iff D[gs[m] + (r - go[m]) mod d_m]. Writes the tile's
+// kept rows and sums exactly as the probe pass of a filter does (no edges
+// examined: they were counted on S), so the tile scan and the compaction
+// that follow are the filter's own.
+template <int TR>
+__global__ __launch_bounds__(kBlock) void k_flag_groups(StepParams p, const uint64_t *go, const uint64_t *gs,
+                                                        uint64_t ng, uint64_t dtot, const uint8_t *D) {
+  constexpr int kRowsPerThread = TR / kBlock;
+  static_assert(TR % kBlock == 0, "tile rows must be a multiple of the block");
+  __shared__ uint64_t s_tmp[kBlock / kWave];
+  __shared__ uint64_t s_m[2];
+  const uint64_t lt = p.tile0 + blockIdx.x;
+  const uint32_t tid = threadIdx.x;
+  const uint64_t base = lt * TR;
+  if (tid == 0) {  // the groups this tile's rows lie in: [m0, m1]
+    const uint64_t last = (base + TR < p.R ? base + TR : p.R) - 1;
+    const uint64_t m0 = last_le(go, 0, ng - 1, base);
+    s_m[0] = m0;
+    s_m[1] = last_le(go, m0, ng - 1, last);
+  }
+  __syncthreads();
+  const uint64_t m0 = s_m[0], m1 = s_m[1];
+  const uint64_t r0 = base + (uint64_t)tid * kRowsPerThread;
+  uint32_t keep = 0, nz = 0;  // bit k: row r0 + k kept
+  if (r0 < p.R) {
+    uint64_t m = last_le(go, m0, m1, r0);
+#pragma unroll
+    for (int k = 0; k < kRowsPerThread; k++) {
+      const uint64_t r = r0 + k;
+      if (r < p.R) {
+        while (m < m1 && go[m + 1] <= r) m++;  // every group has rows: at most one step
+        const uint64_t d = (m + 1 < ng ? gs[m + 1] : dtot) - gs[m];
+        const uint32_t e = (uint32_t)(r - go[m]) % (uint32_t)d;
+        if (D[gs[m] + e]) {
+          keep |= 1u << k;
+          nz++;
+        }
+      }
+    }
+  }
+  uint64_t nz_total;
+  uint32_t pos = (uint32_t)block_excl_scan(nz, s_tmp, nz_total);
+  uint4 *hits = p.hits + lt * TR;
+#pragma unroll
+  for (int k = 0; k < kRowsPerThread; k++)
+    if (keep >> k & 1u) hits[pos++] = make_uint4((uint32_t)(r0 + k), 0u, 1u, 0u);
+  if (tid == 0) {
+    p.tile_sum[lt] = nz_total;
+    p.tile_nz[lt] = nz_total;
+    p.tile_ex[lt] = 0;
   }
 }
 

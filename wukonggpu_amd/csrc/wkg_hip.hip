@@ -217,6 +217,13 @@ void wkg_ctx::drop_table(wkg_table *t) {
     t->lin_alloc = nullptr;
     unref_table(par);
   }
+  if (t->glin_parent) {
+    pool->release(t->glin_alloc, t->stream);
+    wkg_table *gp = t->glin_parent;
+    t->glin_parent = nullptr;
+    t->glin_alloc = nullptr;
+    unref_table(gp);
+  }
   unref_table(t);
 }
 
@@ -427,13 +434,22 @@ static int pass_setup(wkg_ctx *ctx, int mode, const wkg_table *in, int sc, int p
 // every known-start step): for K2U the compacted rows with edges (cinfo) and
 // their output offsets (cexcl) in pp.lin; the filters compact into O.
 int wkg_ctx::probe_pass(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval, int32_t *O,
-                        hipStream_t s, ProbePass &pp) {
+                        hipStream_t s, ProbePass &pp, int wcol) {
   const uint64_t R = in->rows;
   const int c = in->cols;
   const int TR = probe_tile ? probe_tile : probe_tile_rows(R, ncu);
   StepParams p{};
   int rc = pass_setup(this, mode, in, sc, pid, dir, ec, cval, O, s, TR, 0, pp, p);
   if (rc) return rc;
+  if (wcol >= 0) {
+    p.weighted = 1;
+    p.wc = (uint32_t)wcol;
+  }
+  pp.hits = p.hits;
+  pp.tile_nz = p.tile_nz;
+  pp.chunk_nz = p.chunk_nz;
+  pp.ntiles = p.ntiles;
+  pp.tr = TR;
   const uint64_t ntiles = p.ntiles, nchunks = (ntiles + kScanChunk - 1) / kScanChunk;
   uint64_t *totals = p.totals;
   const GraphView g = view();
@@ -452,7 +468,7 @@ int wkg_ctx::probe_pass(int mode, const wkg_table *in, int sc, int pid, int dir,
     k_scan_chunks<<<(unsigned)nchunks, kBlock, 0, s>>>(p);
     if (nchunks > 1) k_scan_top<<<1, kBlock, 0, s>>>(p, nchunks);
   }
-  {
+  if (mode == kModeK2U || O) {  // (a filter without O leaves its kept rows in pp.hits)
     ProfScope ps(this, s, mode == kModeK2U ? "finalize" : "compact", 0.0);
     launch_finalize(TR, p, s);
   }
@@ -468,7 +484,7 @@ int wkg_ctx::probe_pass(int mode, const wkg_table *in, int sc, int pid, int dir,
     prof_add_bytes("probe", 128.0 * (double)pp.cnt[1] + 16.0 * (double)pp.cnt[2]);
     prof_add_bytes("probe_keys", (double)pp.cnt[1]);  // counts, not bytes: keys probed (run leaders)
     prof_add_bytes("probe_rows", (double)R);           // and rows looked up
-  } else {
+  } else if (O) {
     prof_add_bytes("compact", 16.0 * pp.cnt[2] + 8.0 * pp.cnt[2] * c);
   }
   return WKG_OK;
@@ -483,7 +499,7 @@ void wkg_ctx::release_pass(ProbePass &pp, hipStream_t s) {
 // Merge-path expand of a known_to_unknown step: output row o of compacted
 // row i is (input row cinfo[i].x, edge o - cexcl[i]).
 int wkg_ctx::expand_pass(const wkg_table *in, int pid, const uint2 *cinfo, const uint64_t *cexcl, uint64_t nz,
-                         uint64_t E, hipStream_t s, wkg_table *t) {
+                         uint64_t E, hipStream_t s, wkg_table *t, const char *tag) {
   if (!E) return WKG_OK;
   const int c = in->cols;
   const uint64_t nx = (nz + E + kExpandNV - 1) / kExpandNV;
@@ -507,7 +523,7 @@ int wkg_ctx::expand_pass(const wkg_table *in, int pid, const uint2 *cinfo, const
     // edges gathered (4E) + output rows written (4E(c+1)) + the input rows
     // with edges and their offsets read once (4 nz c + 16 nz)
     const double b = 4.0 * E + 4.0 * E * (c + 1) + 4.0 * nz * c + 16.0 * nz;
-    ProfScope ps(this, s, "expand", b);
+    ProfScope ps(this, s, tag, b);
     const GraphView g = view();
     // split launches, as the probe's
     const uint64_t per = std::max<uint64_t>(thread_launch_rows() / 64, 1);
@@ -571,7 +587,7 @@ int wkg_ctx::step_k2u_lineage(const wkg_table *in, int sc, int pid, int dir, hip
   const uint64_t nk = pp.cnt[2], dtot = pp.cnt[0];  // kept parent rows, their list lengths summed
   uint64_t kc = 0, E = 0;
   uint64_t *lift = nullptr;
-  void *lin = nullptr;
+  void *lin = nullptr, *glin = nullptr;
   if (nk) {
     // per kept parent row: child rows and output rows, scanned like tile sums
     const uint64_t nchunks = (nk + kScanChunk - 1) / kScanChunk;
@@ -631,6 +647,17 @@ int wkg_ctx::step_k2u_lineage(const wkg_table *in, int sc, int pid, int dir, hip
       k_spread<<<(unsigned)std::min<uint64_t>((kc + kBlock - 1) / kBlock, 1u << 20), kBlock, 0, s>>>(
           kept, q.tile_nz, q.chunk_nz, q.tile_sum, q.chunk_sum, kc, (uint2 *)lin, (uint64_t *)((char *)lin + kc * 8));
     }
+    // the output's groups, when a group's list repeats over 4+ rows on
+    // average (filters from the new column then decide once per entry)
+    if (E >= 4 * dtot && nk * 24 <= pool->budget / 16) {
+      glin = pool->alloc(nk * 24, s, err_slot());
+      if (glin) {
+        ProfScope ps(this, s, "lineage", 40.0 * (double)nk);
+        k_keep_groups<<<(unsigned)std::min<uint64_t>((nk + kBlock - 1) / kBlock, 65535), kBlock, 0, s>>>(
+            kept, in->lin_info, q.tile_sum, q.chunk_sum, (uint2 *)glin, (uint64_t *)glin + nk,
+            (uint64_t *)glin + 2 * nk);
+      }
+    }
   }
   if (expanded) *expanded = E;
   wkg_table *t = nullptr;
@@ -641,8 +668,21 @@ int wkg_ctx::step_k2u_lineage(const wkg_table *in, int sc, int pid, int dir, hip
   drop_table(keys);
   if (rc) {
     if (lin) pool->release(lin, s);
+    if (glin) pool->release(glin, s);
     drop_table(t);
     return rc;
+  }
+  if (glin) {
+    wkg_table *gp = const_cast<wkg_table *>(par);
+    gp->refs++;
+    t->glin_parent = gp;
+    t->glin_info = (const uint2 *)glin;
+    t->glin_s = (const uint64_t *)glin + nk;
+    t->glin_o = (const uint64_t *)glin + 2 * nk;
+    t->glin_n = nk;
+    t->glin_dtot = dtot;
+    t->glin_pid = pid;
+    t->glin_alloc = glin;
   }
   ProbePass own;  // the spread rows are this output's own lineage
   own.lin = lin;
@@ -652,6 +692,100 @@ int wkg_ctx::step_k2u_lineage(const wkg_table *in, int sc, int pid, int dir, hip
   if (own.lin) pool->release(own.lin, s);
   *out = t;
   return WKG_OK;
+}
+
+// known_to_known / known_to_const from the last column of a table with
+// group lineage (the end column, if any, one of the grandparent's): each
+// group's list entry is decided once, on the table S of (end cell, child
+// rows, entry) rows -- the group table expanded by its lists -- with each
+// entry's examined edges counted once per child row. The decisions are then
+// read back for every row of the input and the kept rows compacted in
+// order, as the per-row filter's compaction does, which gives its output
+// and its work count.
+int wkg_ctx::step_filter_groups(int mode, const wkg_table *in, int pid, int dir, int ec, int64_t cval,
+                                hipStream_t s, wkg_table **out, uint64_t *expanded) {
+  const wkg_table *gp = in->glin_parent;
+  const uint64_t ng = in->glin_n, dtot = in->glin_dtot, R = in->rows;
+  const int c = in->cols;
+  int rc = ensure_pred(in->glin_pid, s);
+  if (rc) return rc;
+  prof_add_bytes("lineage_filters", 1.0);  // a count
+  wkg_table *G = nullptr, *S = nullptr, *t = nullptr;
+  uint2 *gci = nullptr;
+  uint8_t *D = nullptr;
+  ProbePass sp, pp;
+  auto done = [&](int r) {
+    release_pass(sp, s);
+    release_pass(pp, s);
+    if (gci) pool->release(gci, s);
+    if (D) pool->release(D, s);
+    drop_table(G);
+    drop_table(S);
+    if (r) drop_table(t);
+    return r;
+  };
+  if ((rc = new_table(ng, 2, s, &G))) return done(rc);
+  gci = (uint2 *)pool->alloc(ng * 8, s, err_slot());
+  if (!gci) return done(WKG_ENOMEM);
+  {
+    ProfScope ps(this, s, "lineage", 40.0 * (double)ng);
+    k_group_table<<<(unsigned)std::min<uint64_t>((ng + kBlock - 1) / kBlock, 65535), kBlock, 0, s>>>(
+        gp->d, (uint32_t)gp->cols, mode == kModeK2K ? ec : -1, in->glin_info, in->glin_s, in->glin_o, ng, dtot, R,
+        G->d, gci);
+  }
+  if ((rc = new_table(dtot, 3, s, &S)) ||
+      (rc = expand_pass(G, in->glin_pid, gci, in->glin_s, ng, dtot, s, S, "lineage")))
+    return done(rc);
+  // decisions per (group, entry): kept S rows, examined edges weighted by column 1
+  if ((rc = probe_pass(mode, S, 2, pid, dir, mode == kModeK2K ? 0 : -1, cval, nullptr, s, sp, 1)))
+    return done(rc);
+  const uint64_t examined = sp.cnt[1];
+  D = (uint8_t *)pool->alloc(dtot, s, err_slot());
+  if (!D) return done(WKG_ENOMEM);
+  {
+    ProfScope ps(this, s, "lineage", (double)dtot + 16.0 * (double)sp.cnt[2]);
+    k_zero_bytes<<<(unsigned)std::min<uint64_t>((dtot + kBlock - 1) / kBlock, 65535), kBlock, 0, s>>>(D, dtot);
+    if (sp.cnt[2])
+      k_hits_flags<<<(unsigned)std::min<uint64_t>(sp.ntiles, 65535), kBlock, 0, s>>>(
+          sp.hits, sp.tile_nz, sp.chunk_nz, sp.ntiles, sp.cnt[2], (uint32_t)sp.tr, D);
+  }
+  // the input's rows: kept flags, tile scan, compaction
+  constexpr int TR = 2048;
+  if ((rc = new_table(R, c, s, &t))) return done(rc);
+  StepParams p{};
+  if ((rc = pass_setup(this, mode, in, c - 1, pid, dir, ec, cval, t->d, s, TR, 0, pp, p))) return done(rc);
+  const uint64_t ntiles = p.ntiles, nchunks = (ntiles + kScanChunk - 1) / kScanChunk;
+  {
+    ProfScope ps(this, s, "lineage", 1.0 * (double)R);
+    const uint64_t per = std::max<uint64_t>(thread_launch_rows() / TR, 1);
+    for (uint64_t t0 = 0; t0 < ntiles; t0 += per) {
+      p.tile0 = t0;
+      k_flag_groups<TR><<<(unsigned)std::min<uint64_t>(per, ntiles - t0), kBlock, 0, s>>>(
+          p, in->glin_o, in->glin_s, ng, dtot, D);
+    }
+    p.tile0 = 0;
+  }
+  p.sg = next_signal();
+  {
+    ProfScope ps(this, s, "scan_tiles", 48.0 * (double)ntiles);
+    k_scan_chunks<<<(unsigned)nchunks, kBlock, 0, s>>>(p);
+    if (nchunks > 1) k_scan_top<<<1, kBlock, 0, s>>>(p, nchunks);
+  }
+  {
+    ProfScope ps(this, s, "compact", 0.0);
+    launch_finalize(TR, p, s);
+  }
+  hipError_t le = hipGetLastError();
+  if (le != hipSuccess) return done(fail(WKG_EDEVICE, "filter launch failed: %s", hipGetErrorString(le)));
+  if ((rc = wait_signal(p.sg, s))) return done(rc);
+  uint64_t cnt[3];
+  memcpy(cnt, h_rb, sizeof cnt);
+  prof_add_bytes("compact", 16.0 * cnt[2] + 8.0 * cnt[2] * c);
+  t->rows = cnt[0];
+  t->bytes = t->rows * (uint64_t)c * 4;
+  if (expanded) *expanded = examined;
+  *out = t;
+  return done(WKG_OK);
 }
 
 int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval,
@@ -675,6 +809,9 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
   if (mode == kModeK2U && lineage && in->lin_parent && sc < in->lin_parent->cols)
     return step_k2u_lineage(in, sc, pid, dir, s, out, expanded);
   if (small_steps && R <= kSmallRows) return step_small(mode, in, sc, pid, dir, ec, cval, s, out, expanded);
+  if (mode != kModeK2U && lineage && !framed && in->glin_parent && sc == c - 1 &&
+      (mode == kModeK2C || ec < in->glin_parent->cols) && R >= group_filter_rows && R >= 4 * in->glin_dtot)
+    return step_filter_groups(mode, in, pid, dir, ec, cval, s, out, expanded);
 
   wkg_table *t = nullptr;
   if (mode != kModeK2U && (rc = new_table(R, c, s, &t))) return rc;
@@ -1127,6 +1264,12 @@ int wkg_pool_stats(wkg_ctx *ctx, uint64_t *live_bytes, uint64_t *cached_bytes, u
 int wkg_set_lineage(wkg_ctx *ctx, int on) {
   if (!ctx) return WKG_EINVAL;
   ctx->lineage = on != 0;
+  return WKG_OK;
+}
+
+int wkg_set_group_filter_rows(wkg_ctx *ctx, uint64_t rows) {
+  if (!ctx) return WKG_EINVAL;
+  ctx->group_filter_rows = rows;
   return WKG_OK;
 }
 

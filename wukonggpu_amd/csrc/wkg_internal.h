@@ -75,6 +75,18 @@ struct wkg_table {
   const uint64_t *lin_excl = nullptr;
   uint64_t lin_n = 0;
   void *lin_alloc = nullptr;  // pool memory holding lin_info and lin_excl
+  // Group lineage, set when this table is a known_to_unknown output over a
+  // column its input inherited (step_k2u_lineage): group m < glin_n is
+  // grandparent row glin_info[m].x joined with the list at segment offset
+  // glin_info[m].y of predicate glin_pid (glin_s[m+1] - glin_s[m] entries,
+  // the last up to glin_dtot), repeated for each child row, in rows
+  // [glin_o[m], glin_o[m+1]) (the last up to `rows`).
+  wkg_table *glin_parent = nullptr;
+  const uint2 *glin_info = nullptr;
+  const uint64_t *glin_s = nullptr, *glin_o = nullptr;
+  uint64_t glin_n = 0, glin_dtot = 0;
+  int glin_pid = 0;
+  void *glin_alloc = nullptr;
 };
 
 // One step's probe, tile scan and compaction (wkg_ctx::probe_pass).
@@ -84,6 +96,11 @@ struct ProbePass {
   uint2 *cinfo = nullptr;     // compacted rows with edges: (row, list offset)
   uint64_t *cexcl = nullptr;  // their first output row
   uint64_t cnt[3] = {0, 0, 0};  // output rows, edges examined or keys probed, rows with output
+  // the probe's kept-row lists and their tile prefixes (valid until release)
+  const uint4 *hits = nullptr;
+  const uint64_t *tile_nz = nullptr, *chunk_nz = nullptr;
+  uint64_t ntiles = 0;
+  int tr = 0;
 };
 
 // Framed residency (graph larger than its HBM budget): per-predicate
@@ -113,6 +130,7 @@ struct wkg_ctx {
   uint64_t table_limit = 0;  // largest step output table in bytes (0: no limit)
   int probe_tile = 0;        // rows per probe tile of the per-step calls (0: by input size)
   bool lineage = true;       // probe inherited columns once per parent row
+  uint64_t group_filter_rows = 1ull << 22;  // smallest input a filter decides per lineage group
   bool small_steps = true;   // steps over at most kSmallRows rows in one launch
   uint64_t *h_rb = nullptr;  // pinned read-back block: counters, the signal flag, small-table staging
   uint64_t *h_rb_dev = nullptr;  // the same block's device address
@@ -171,15 +189,17 @@ struct wkg_ctx {
   int step_known(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval, hipStream_t s,
                  wkg_table **out, uint64_t *expanded);
   int probe_pass(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval, int32_t *O,
-                 hipStream_t s, ProbePass &pp);
+                 hipStream_t s, ProbePass &pp, int wcol = -1);
   int step_small(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval, hipStream_t s,
                  wkg_table **out, uint64_t *expanded);
   void release_pass(ProbePass &pp, hipStream_t s);
   int expand_pass(const wkg_table *in, int pid, const uint2 *cinfo, const uint64_t *cexcl, uint64_t nz, uint64_t E,
-                  hipStream_t s, wkg_table *t);
+                  hipStream_t s, wkg_table *t, const char *tag = "expand");
   void keep_lineage(const wkg_table *in, ProbePass &pp, uint64_t nz, wkg_table *t);
   int step_k2u_lineage(const wkg_table *in, int sc, int pid, int dir, hipStream_t s, wkg_table **out,
                        uint64_t *expanded);
+  int step_filter_groups(int mode, const wkg_table *in, int pid, int dir, int ec, int64_t cval, hipStream_t s,
+                         wkg_table **out, uint64_t *expanded);
   // residency
   int set_image_metas(uint64_t nslots, uint64_t nedges, const wkg_pred_meta_abi *m_in, int nmetas);
   int finish_resident(uint32_t flags, hipStream_t s);

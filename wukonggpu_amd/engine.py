@@ -150,6 +150,11 @@ class Device:
         """Probe a star's next arm once per input row of the fan-out before it (default on)."""
         self._check(N.hip_lib().wkg_set_lineage(self._h, 1 if on else 0))
 
+    def set_group_filter_rows(self, rows):
+        """Smallest table a filter from a star arm's new column decides once per
+        lineage group (default 2^22 rows; 0: any size)."""
+        self._check(N.hip_lib().wkg_set_group_filter_rows(self._h, int(rows)))
+
     def random_access_rate(self, nbytes=4 << 30, iters=10):
         """Random 128-byte line reads per second (the hash probe's roofline)."""
         g = C.c_double()
