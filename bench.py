@@ -66,6 +66,9 @@ def main():
                     help="run only the closed-loop batch mode on the 100M-triple graph")
     ap.add_argument("--exchange-only", action="store_true",
                     help="run only the fork-join table split (for profiler passes)")
+    ap.add_argument("--served-heavy-only", action="store_true",
+                    help="only the served mix's heavy queries, each alone on the plain path, with the filters "
+                         "decided per lineage group and per row")
     ap.add_argument("--config", default="2", choices=["1", "2", "4", "5"],
                     help="2: heavy query + batched mix + served sweep on the 100M-triple graph (default); "
                          "1: plain single query on the 10M-triple graph; 4: served sweep on the 1B-triple "
@@ -80,6 +83,8 @@ def main():
                     help="with --mix-only: the executor shape as key=value pairs (probe_wg_per_cu=3,...) "
                          "instead of autotuning")
     args = ap.parse_args()
+    if args.served_heavy_only:
+        args.batch_only = True
     if args.heavy_only or args.probe_only or args.mix_only or args.batch_only or args.exchange_only:
         args.no_cpu = True
     if args.exchange_only:
@@ -116,6 +121,31 @@ def main():
     if args.probe_only:
         rk = random_key_probe(dev, w, args.seed, HBM_PEAK_GBS * 1e9 / 128)
         print(json.dumps({"probe_random_keys": rk}), flush=True)
+        eng.close()
+        dev.close()
+        return
+    if args.served_heavy_only:
+        P = w.info["predicates"]
+        type_ids = list(range(2 + P + 1, 2 + P + w.info["types"]))
+        sq, sflags = workloads.served_mix(P, type_ids, w.info["t0"], args.served_n, args.seed + 11)
+        hq = [q for q, f in zip(sq, sflags) if f]
+        out = {"workload": f"the {len(hq)} heavy queries of the {len(sq)}-query served mix, each alone"}
+        ref = None
+        for name, rows in (("group_filters", 1 << 22), ("per_row_filters", 1 << 62)):
+            dev.set_group_filter_rows(rows)
+            eng.run_plain(hq)  # warm-up
+            res = eng.run_plain(hq)
+            ms = [r.latency_s * 1e3 for r in res]
+            rows_ex = [(r.table.shape[0], int(r.expanded)) for r in res]
+            if ref is None:
+                ref = [r.table for r in res]
+            same = all(np.array_equal(a.table, b) for a, b in zip(res, ref))
+            out[name] = {"total_ms": sum(ms), "max_ms": max(ms), "p50_ms": sorted(ms)[len(ms) // 2],
+                         "expanded": sum(e for _, e in rows_ex), "identical": same}
+            log(f"{name}: {len(hq)} heavy queries alone in {sum(ms):.1f} ms (max {max(ms):.2f} ms), "
+                f"identical={same}")
+        dev.set_group_filter_rows(1 << 22)
+        print(json.dumps({"served_heavy": out}), flush=True)
         eng.close()
         dev.close()
         return
