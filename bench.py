@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--served-heavy-only", action="store_true",
                     help="only the served mix's heavy queries, each alone on the plain path, with the filters "
                          "decided per lineage group and per row")
+    ap.add_argument("--group-filter-rows", type=int, default=1 << 22,
+                    help="with --served-heavy-only: smallest table a filter decides per lineage group")
     ap.add_argument("--config", default="2", choices=["1", "2", "4", "5"],
                     help="2: heavy query + batched mix + served sweep on the 100M-triple graph (default); "
                          "1: plain single query on the 10M-triple graph; 4: served sweep on the 1B-triple "
@@ -131,7 +133,7 @@ def main():
         hq = [q for q, f in zip(sq, sflags) if f]
         out = {"workload": f"the {len(hq)} heavy queries of the {len(sq)}-query served mix, each alone"}
         ref = None
-        for name, rows in (("group_filters", 1 << 22), ("per_row_filters", 1 << 62)):
+        for name, rows in (("group_filters", args.group_filter_rows), ("per_row_filters", 1 << 62)):
             dev.set_group_filter_rows(rows)
             eng.run_plain(hq)  # warm-up
             res = eng.run_plain(hq)
@@ -142,8 +144,8 @@ def main():
             same = all(np.array_equal(a.table, b) for a, b in zip(res, ref))
             out[name] = {"total_ms": sum(ms), "max_ms": max(ms), "p50_ms": sorted(ms)[len(ms) // 2],
                          "expanded": sum(e for _, e in rows_ex), "identical": same}
-            log(f"{name}: {len(hq)} heavy queries alone in {sum(ms):.1f} ms (max {max(ms):.2f} ms), "
-                f"identical={same}")
+            log(f"{name} (from {rows} rows): {len(hq)} heavy queries alone in {sum(ms):.1f} ms "
+                f"(max {max(ms):.2f} ms), identical={same}")
         dev.set_group_filter_rows(1 << 22)
         print(json.dumps({"served_heavy": out}), flush=True)
         eng.close()
