@@ -36,7 +36,23 @@ typedef struct wks_gstore wks_gstore;
  * min(degree_cap, floor(C * r^-zipf_s)) with C solved so that the degrees sum to
  * about num_triples/num_preds; objects are uniform and distinct per subject.
  * Predicates 0..num_hubs-1 (in order) raise the degree of their rank-1 subject to
- * hub_degree. Type k gets type_sizes[k] distinct uniformly chosen members. */
+ * hub_degree. Type k gets type_sizes[k] distinct uniformly chosen members.
+ *
+ * Two planted correlations give multi-pattern cycles non-empty answers (on
+ * uniform objects a closing edge almost never exists):
+ *  - linked pairs: for k < num_linked, predicate 2+2k+1 is linked to
+ *    predicate 2+2k. A subject s of the linked predicate with out-degree d
+ *    that is also a subject of the source predicate takes
+ *    min(ceil(link_frac * d), |source list of s|) of its d objects from its
+ *    own source list (a seeded subset), the rest uniform and distinct as
+ *    usual; so (s linked o) implies (s source o) for about link_frac of its
+ *    edges. Degrees and subject ranks are unchanged.
+ *  - sibling pairs: for each k < num_sibling, predicate sibling_src[k] (an
+ *    id in 2..num_preds+1) plants edges of predicate sibling_dst[k]: for
+ *    every subject of sibling_src[k] with ascending objects o_0 < o_1 < ...,
+ *    the triples (o_2i, sibling_dst[k], o_2i+1). Every such pair of objects
+ *    then closes a triangle. These triples come after the type triples; a
+ *    duplicate of an existing triple is removed by the builders. */
 typedef struct wks_gen_params {
   uint64_t seed;
   uint64_t num_entities;
@@ -48,6 +64,11 @@ typedef struct wks_gen_params {
   const uint64_t *type_sizes; /* num_types entries */
   uint32_t num_hubs;
   uint64_t hub_degree;
+  uint32_t num_linked;        /* linked predicate pairs (0: none) */
+  double link_frac;           /* share of a linked list taken from its source list */
+  uint32_t num_sibling;       /* sibling plants (0: none) */
+  const uint32_t *sibling_src; /* num_sibling predicate ids */
+  const uint32_t *sibling_dst; /* num_sibling predicate ids */
 } wks_gen_params;
 
 int wks_generate(const wks_gen_params *params, wks_triples **out);

@@ -122,6 +122,7 @@ struct Req {  // request_or_reply, the fields the engine uses
   int col_num = 0;
   std::vector<int> table;
   uint64_t expanded = 0;
+  int threads = 1;  // known-start steps over row ranges on this many threads
 
   bool is_finished() const { return step * 4 >= ncmd; }
   bool start_from_index() const { return is_tpid(cmd[0]); }
@@ -156,15 +157,17 @@ int const_to_unknown(const Image &g, Req &req) {
   return OK;
 }
 
-int known_to_unknown(const Image &g, Req &req) {
+// The three known-start handlers below process input rows [r0, r1) into
+// `upd` (appending) and return the step's work count over those rows; a
+// step is the concatenation of its row ranges, in order, since each row's
+// output depends on that row alone (run_known).
+uint64_t known_to_unknown_rows(const Image &g, const Req &req, uint64_t r0, uint64_t r1,
+                               std::vector<int> &upd) {
   int64_t start = req.cmd[req.step * 4], pid = req.cmd[req.step * 4 + 1];
-  int64_t d = req.cmd[req.step * 4 + 2], end = req.cmd[req.step * 4 + 3];
-  if (req.col_num != req.var2column(end)) return E_PLAN;
-  std::vector<int> upd;
-  upd.reserve(req.table.size());
-  const uint64_t n = req.rows();
+  int64_t d = req.cmd[req.step * 4 + 2];
   const int64_t sc = req.var2column(start);
-  for (uint64_t i = 0; i < n; i++) {
+  uint64_t expanded = 0;
+  for (uint64_t i = r0; i < r1; i++) {
     int64_t prev_id = req.get(i, sc);
     uint64_t sz = 0;
     const uint32_t *res = get_edges(g, prev_id, d, pid, &sz);
@@ -172,21 +175,18 @@ int known_to_unknown(const Image &g, Req &req) {
       req.append_row_to(i, upd);
       upd.push_back((int)res[k]);
     }
-    req.expanded += sz;
+    expanded += sz;
   }
-  req.col_num = req.col_num + 1;
-  req.table.swap(upd);
-  req.step++;
-  return OK;
+  return expanded;
 }
 
-int known_to_known(const Image &g, Req &req) {
+uint64_t known_to_known_rows(const Image &g, const Req &req, uint64_t r0, uint64_t r1,
+                             std::vector<int> &upd) {
   int64_t start = req.cmd[req.step * 4], pid = req.cmd[req.step * 4 + 1];
   int64_t d = req.cmd[req.step * 4 + 2], end = req.cmd[req.step * 4 + 3];
-  std::vector<int> upd;
-  const uint64_t n = req.rows();
   const int64_t sc = req.var2column(start), ec = req.var2column(end);
-  for (uint64_t i = 0; i < n; i++) {
+  uint64_t expanded = 0;
+  for (uint64_t i = r0; i < r1; i++) {
     int64_t prev_id = req.get(i, sc);
     uint64_t sz = 0;
     const uint32_t *res = get_edges(g, prev_id, d, pid, &sz);
@@ -198,20 +198,18 @@ int known_to_known(const Image &g, Req &req) {
         break;
       }
     }
-    req.expanded += k < sz ? k + 1 : sz;
+    expanded += k < sz ? k + 1 : sz;
   }
-  req.table.swap(upd);
-  req.step++;
-  return OK;
+  return expanded;
 }
 
-int known_to_const(const Image &g, Req &req) {
+uint64_t known_to_const_rows(const Image &g, const Req &req, uint64_t r0, uint64_t r1,
+                             std::vector<int> &upd) {
   int64_t start = req.cmd[req.step * 4], pid = req.cmd[req.step * 4 + 1];
   int64_t d = req.cmd[req.step * 4 + 2], end = req.cmd[req.step * 4 + 3];
-  std::vector<int> upd;
-  const uint64_t n = req.rows();
   const int64_t sc = req.var2column(start);
-  for (uint64_t i = 0; i < n; i++) {
+  uint64_t expanded = 0;
+  for (uint64_t i = r0; i < r1; i++) {
     int64_t prev_id = req.get(i, sc);
     uint64_t sz = 0;
     const uint32_t *res = get_edges(g, prev_id, d, pid, &sz);
@@ -222,8 +220,67 @@ int known_to_const(const Image &g, Req &req) {
         break;
       }
     }
-    req.expanded += k < sz ? k + 1 : sz;
+    expanded += k < sz ? k + 1 : sz;
   }
+  return expanded;
+}
+
+typedef uint64_t (*RowsFn)(const Image &, const Req &, uint64_t, uint64_t, std::vector<int> &);
+
+// Runs one known-start handler over all rows: on one thread in one range, or
+// (req.threads > 1 and enough rows) in contiguous row ranges on several
+// threads whose outputs are concatenated in range order.
+void run_known(const Image &g, Req &req, RowsFn fn, std::vector<int> &upd) {
+  const uint64_t n = req.rows();
+  const int T = req.threads;
+  if (T <= 1 || n < (1u << 16)) {
+    req.expanded += fn(g, req, 0, n, upd);
+    return;
+  }
+  const uint64_t parts = (uint64_t)T * 8;
+  std::vector<std::vector<int>> outs(parts);
+  std::vector<uint64_t> ex(parts, 0);
+  std::atomic<uint64_t> next{0};
+  auto work = [&]() {
+    for (uint64_t c; (c = next.fetch_add(1)) < parts;)
+      ex[c] = fn(g, req, n * c / parts, n * (c + 1) / parts, outs[c]);
+  };
+  std::vector<std::thread> ts;
+  for (int i = 1; i < T; i++) ts.emplace_back(work);
+  work();
+  for (auto &t : ts) t.join();
+  uint64_t total = 0;
+  for (auto &o : outs) total += o.size();
+  upd.reserve(total);
+  for (uint64_t c = 0; c < parts; c++) {
+    upd.insert(upd.end(), outs[c].begin(), outs[c].end());
+    std::vector<int>().swap(outs[c]);
+    req.expanded += ex[c];
+  }
+}
+
+int known_to_unknown(const Image &g, Req &req) {
+  int64_t end = req.cmd[req.step * 4 + 3];
+  if (req.col_num != req.var2column(end)) return E_PLAN;
+  std::vector<int> upd;
+  run_known(g, req, known_to_unknown_rows, upd);
+  req.col_num = req.col_num + 1;
+  req.table.swap(upd);
+  req.step++;
+  return OK;
+}
+
+int known_to_known(const Image &g, Req &req) {
+  std::vector<int> upd;
+  run_known(g, req, known_to_known_rows, upd);
+  req.table.swap(upd);
+  req.step++;
+  return OK;
+}
+
+int known_to_const(const Image &g, Req &req) {
+  std::vector<int> upd;
+  run_known(g, req, known_to_const_rows, upd);
   req.table.swap(upd);
   req.step++;
   return OK;
@@ -265,6 +322,26 @@ int execute_one_step(const Image &g, Req &req) {
   return rc < 0 ? rc : 1;
 }
 
+// Order-sensitive 64-bit digest of a row-major table: fmix64(rows * 256 +
+// cols) plus, over cells i = 0, 1, ..., fmix64((i + 1) * golden ^ cell_i)
+// (cell as its u32 bit pattern), all mod 2^64. oracle.table_digest computes
+// the same for a numpy table.
+uint64_t fmix64(uint64_t x) {  // MurmurHash3's 64-bit finalizer
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+
+uint64_t table_digest(const std::vector<int> &t, uint64_t rows, int cols) {
+  uint64_t h = fmix64(rows * 256 + (uint64_t)cols);
+  for (uint64_t i = 0; i < t.size(); i++)
+    h += fmix64((i + 1) * 0x9E3779B97F4A7C15ull ^ (uint64_t)(uint32_t)t[i]);
+  return h;
+}
+
 Image make_image(const uint64_t *slots, uint64_t num_slots, const uint32_t *edges, uint64_t num_edges,
                  const void *metas, int num_metas) {
   return Image{slots, num_slots, edges, num_edges, (const Meta *)metas, num_metas};
@@ -274,16 +351,19 @@ Image make_image(const uint64_t *slots, uint64_t num_slots, const uint32_t *edge
 
 extern "C" {
 
-// Runs steps [0, stop_step) of a query (stop_step <= 0: all). On success the
-// result table is returned in *out (free with or_free), *rows x *cols.
-int or_run_query(const uint64_t *slots, uint64_t num_slots, const uint32_t *edges,
-                 uint64_t num_edges, const void *metas, int num_metas, const int64_t *cmd, int ncmd,
-                 int stop_step, int32_t **out, uint64_t *rows, int *cols, uint64_t *expanded,
-                 double *seconds) {
+// Runs steps [0, stop_step) of a query (stop_step <= 0: all), the
+// known-start steps on `threads` threads over row ranges (<= 1: one thread;
+// the result is the same). On success the result table is returned in *out
+// (free with or_free), *rows x *cols.
+int or_run_query_mt(const uint64_t *slots, uint64_t num_slots, const uint32_t *edges,
+                    uint64_t num_edges, const void *metas, int num_metas, const int64_t *cmd, int ncmd,
+                    int stop_step, int threads, int32_t **out, uint64_t *rows, int *cols,
+                    uint64_t *expanded, double *seconds) {
   Image g = make_image(slots, num_slots, edges, num_edges, metas, num_metas);
   Req req;
   req.cmd = cmd;
   req.ncmd = ncmd;
+  req.threads = threads;
   if (ncmd <= 0 || ncmd % 4) return E_PLAN;
   auto t0 = std::chrono::steady_clock::now();
   while (stop_step <= 0 || req.step < stop_step) {
@@ -305,16 +385,26 @@ int or_run_query(const uint64_t *slots, uint64_t num_slots, const uint32_t *edge
   return 0;
 }
 
-// One step on a given input table (the state a request has before `step`).
+int or_run_query(const uint64_t *slots, uint64_t num_slots, const uint32_t *edges,
+                 uint64_t num_edges, const void *metas, int num_metas, const int64_t *cmd, int ncmd,
+                 int stop_step, int32_t **out, uint64_t *rows, int *cols, uint64_t *expanded,
+                 double *seconds) {
+  return or_run_query_mt(slots, num_slots, edges, num_edges, metas, num_metas, cmd, ncmd, stop_step, 1,
+                         out, rows, cols, expanded, seconds);
+}
+
+// One step on a given input table (the state a request has before `step`),
+// a known-start step on `threads` threads over row ranges (<= 1: one).
 int or_run_step(const uint64_t *slots, uint64_t num_slots, const uint32_t *edges,
                 uint64_t num_edges, const void *metas, int num_metas, const int64_t *cmd, int ncmd,
-                int step, const int32_t *in, uint64_t in_rows, int in_cols, int32_t **out,
+                int step, const int32_t *in, uint64_t in_rows, int in_cols, int threads, int32_t **out,
                 uint64_t *rows, int *cols, uint64_t *expanded) {
   Image g = make_image(slots, num_slots, edges, num_edges, metas, num_metas);
   Req req;
   req.cmd = cmd;
   req.ncmd = ncmd;
   req.step = step;
+  req.threads = threads;
   req.col_num = in_cols;
   if (ncmd <= 0 || ncmd % 4 || step < 0 || step * 4 >= ncmd) return E_PLAN;
   req.table.assign(in, in + in_rows * (uint64_t)in_cols);
@@ -336,10 +426,11 @@ int or_run_step(const uint64_t *slots, uint64_t num_slots, const uint32_t *edges
 // run to completion on `threads` threads, each query on one thread like one
 // reference Engine thread (core/engine.hpp:659-725). Per-query latencies and
 // result sizes are written to lat_s[q], rows[q]; returns wall seconds in *wall.
+// digest (may be NULL) gets each result table's table_digest.
 int or_run_many(const uint64_t *slots, uint64_t num_slots, const uint32_t *edges,
                 uint64_t num_edges, const void *metas, int num_metas, const int64_t *cmd,
                 const int64_t *cmd_off, int nq, int threads, double *lat_s, uint64_t *rows,
-                uint64_t *expanded, double *wall) {
+                uint64_t *expanded, uint64_t *digest, double *wall) {
   Image g = make_image(slots, num_slots, edges, num_edges, metas, num_metas);
   std::atomic<int> next{0}, err{0};
   auto work = [&]() {
@@ -357,6 +448,7 @@ int or_run_many(const uint64_t *slots, uint64_t num_slots, const uint32_t *edges
       lat_s[q] = std::chrono::duration<double>(t1 - t0).count();
       rows[q] = req.rows();
       expanded[q] = req.expanded;
+      if (digest) digest[q] = table_digest(req.table, req.rows(), req.col_num);
     }
   };
   auto t0 = std::chrono::steady_clock::now();

@@ -277,23 +277,31 @@ def test_lineage_group_filters_match_oracle(graph_hubs):
     qs += [[2, 1, 0, -1, -1, 2, 1, -2, -1, 3, 1, -3, -3, 2, 0, -1],       # star from ?x, closed on ?x
            [2, 1, 0, -1, -1, 2, 1, -2, -1, 3, 1, -3, -3, 4, 0, -1, -3, 3, 0, -1]]  # then a second filter
     want = [oracle.run_query(gs, q)[:2] for q in qs]
-    for rows, lineage in [(0, True), (1 << 22, True), (0, False)]:
-        dev.set_lineage(lineage)
-        dev.set_group_filter_rows(rows)
-        dev.profile(True)
-        print("group filter rows", rows, "lineage", lineage, flush=True)
-        for q, (wt, wex) in zip(qs, want):
-            res = eng.run_query(q)
-            assert res.status == 0, q
-            assert res.table.shape[0] == wt.shape[0], q
-            assert np.array_equal(res.table.reshape(wt.shape), wt), q
-            assert res.expanded == wex, q
-        taken = dev.profile_read("lineage_filters")[2]
-        dev.profile(False)
-        if rows == 0 and lineage:
-            assert taken > 0
-        elif not lineage:
-            assert taken == 0
+    # (group filter rows, lineage, launch rows): 4096 input rows per launch
+    # splits the group pass into launches that start past tile 0
+    try:
+        for rows, lineage, launch in [(0, True, 0), (0, True, 4096), (1 << 22, True, 0), (0, False, 0)]:
+            dev.set_lineage(lineage)
+            dev.set_group_filter_rows(rows)
+            dev.set_launch_rows(launch)
+            dev.profile(True)
+            print("group filter rows", rows, "lineage", lineage, "launch rows", launch, flush=True)
+            for q, (wt, wex) in zip(qs, want):
+                res = eng.run_query(q)
+                assert res.status == 0, q
+                assert res.table.shape[0] == wt.shape[0], q
+                assert np.array_equal(res.table.reshape(wt.shape), wt), q
+                assert res.expanded == wex, q
+            taken = dev.profile_read("lineage_filters")[2]
+            dev.profile(False)
+            if rows == 0 and lineage:
+                assert taken > 0
+            elif not lineage:
+                assert taken == 0
+    finally:
+        dev.set_launch_rows(0)
+        dev.set_group_filter_rows(1 << 22)
+        dev.set_lineage(True)
     eng.close()
     dev.close()
 
@@ -339,5 +347,29 @@ def test_framed_residency_matches_oracle(graph_mid):
     res, _ = eng.serve(qs, arrivals, heavy=heavy, table_budget_bytes=256 << 20, light_lanes=2)
     for q, r in zip(qs, res):
         _check(gs, q, r)
+    eng.close()
+    dev.close()
+
+
+def test_long_index_lists_resident(graph_mid):
+    """Predicate-index lists longer than one probe-and-copy launch (8192
+    entries) on a resident graph: each list and each query that starts from
+    one, against the oracle."""
+    _, gs = graph_mid
+    dev = Device(0)
+    dev.load(gs)
+    assert dev.stats()["resident_bytes"] >= gs.image_nbytes()
+    eng = GPUEngine(dev)
+    longest = 0
+    for pid in range(2, 14):
+        for d in (0, 1):
+            want, wex, _ = oracle.run_query(gs, [pid, 0, d, -1])
+            longest = max(longest, want.shape[0])
+            t, ex = dev.i2u(pid, d)
+            assert np.array_equal(t.to_host(), want) and ex == wex, (pid, d)
+            t.free()
+            q = [pid, 0, d, -1, -1, 2 + (pid % 12), 1, -2]
+            _check(gs, q, eng.run_query(q))
+    assert longest > 8192
     eng.close()
     dev.close()

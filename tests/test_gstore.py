@@ -148,3 +148,51 @@ def test_builder_rejects_bad_triples():
         GStore.build(Triples.from_arrays([5], [2], [VB]))       # subject must be a normal vertex
     with pytest.raises(RuntimeError):
         GStore.build(Triples.from_arrays([VB], [0], [VB]))      # predicate 0 is reserved
+
+
+def test_generator_planted_links_and_siblings():
+    # linked pairs (2k+3 takes a share of its lists from 2k+2's list of the
+    # same subject) and sibling plants ((o_2i, dst, o_2i+1) for consecutive
+    # objects of every source list) leave every other list and every degree
+    # as the plain generator makes them
+    import math
+    kw = dict(seed=3, num_entities=20000, num_preds=6, num_triples=200000, zipf_s=1.0, type_sizes=[100])
+    a = Triples.generate(**kw)
+    b = Triples.generate(**kw, num_linked=2, link_frac=0.25, siblings=[(2, 7), (4, 3)])
+    sa, pa, oa = (x.copy() for x in a.arrays())
+    sb, pb, ob = (x.copy() for x in b.arrays())
+    n = len(sa)
+
+    def lists(s, p, o, pid):
+        m = p == pid
+        out = {}
+        for x, y in zip(s[m].tolist(), o[m].tolist()):
+            out.setdefault(x, []).append(y)
+        return out
+
+    for pid in range(2, 8):
+        la, lb = lists(sa, pa, oa, pid), lists(sb[:n], pb[:n], ob[:n], pid)
+        assert {k: len(v) for k, v in la.items()} == {k: len(v) for k, v in lb.items()}, pid
+        if pid in (2, 4, 6, 7):  # sources and the unlinked pair
+            assert la == lb, pid
+    for src, dst in ((2, 3), (4, 5)):
+        ls, ld = lists(sb[:n], pb[:n], ob[:n], src), lists(sb[:n], pb[:n], ob[:n], dst)
+        both = 0
+        for s, objs in ld.items():
+            assert len(set(objs)) == len(objs)
+            if s in ls:
+                both += 1
+                need = min(math.ceil(0.25 * len(objs)), len(ls[s]))
+                assert len(set(objs) & set(ls[s])) >= need, (dst, s)
+        assert both > 100
+    planted = {}
+    for x, p, y in zip(sb[n:].tolist(), pb[n:].tolist(), ob[n:].tolist()):
+        planted.setdefault(p, []).append((x, y))
+    for src, dst in ((2, 7), (4, 3)):
+        want = []
+        for s, objs in sorted(lists(sb[:n], pb[:n], ob[:n], src).items()):
+            objs = sorted(objs)
+            want += [(objs[i], objs[i + 1]) for i in range(0, len(objs) - 1, 2)]
+        assert sorted(planted[dst]) == sorted(want), dst
+    a.close()
+    b.close()

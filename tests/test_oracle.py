@@ -94,3 +94,34 @@ def test_oracle_threads_agree(graph_small):
     lat1, rows1, ex1, _ = oracle.run_many(gs, qs, threads=1)
     lat4, rows4, ex4, _ = oracle.run_many(gs, qs, threads=4)
     assert np.array_equal(rows1, rows4) and np.array_equal(ex1, ex4)
+
+
+def test_oracle_threaded_steps_and_digest(graph_hubs):
+    # known-start steps split over row ranges on 8 threads give the one-thread
+    # table (each row's output depends on that row alone); the oracle's table
+    # digest equals the numpy one over the same table, and moves when one
+    # cell or the row order changes
+    _, gs = graph_hubs
+    tid = 2 + 3
+    qs = [[tid, 1, 0, -1, -1, 3, 1, -2, -2, 3, 0, -3, -3, 2, 1, -4],
+          [tid, 1, 0, -1, -1, 2, 1, -2, -2, 2, 0, -3, -3, 3, 1, -4, -4, 2, 0, -3],
+          [tid, 1, 0, -1, -1, 2, 1, -2, -2, 2, 0, -3, -3, 2, 1, VB + 17]]
+    big = 0
+    for q in qs:
+        a, ea, _ = oracle.run_query(gs, q)
+        b, eb, _ = oracle.run_query(gs, q, threads=8)
+        assert a.shape == b.shape and np.array_equal(a, b) and ea == eb, q
+        big = max(big, oracle.run_query(gs, q, stop_step=3)[0].shape[0])
+    assert big >= 1 << 16  # the threaded split ran
+    _, rows, ex, _, dg = oracle.run_many(gs, qs, threads=2, digests=True)
+    for q, d in zip(qs, dg):
+        t, _, _ = oracle.run_query(gs, q)
+        assert int(d) == oracle.table_digest(t)
+    t, _, _ = oracle.run_query(gs, qs[0])
+    assert t.shape[0] > 2
+    h = oracle.table_digest(t)
+    t2 = t.copy()
+    t2[1, 0] ^= 1
+    assert oracle.table_digest(t2) != h
+    assert oracle.table_digest(t[::-1]) != h
+    assert oracle.table_digest(t[:, :3]) != h

@@ -21,7 +21,9 @@ vp = C.c_void_p
 class GenParams(C.Structure):
     _fields_ = [("seed", u64), ("num_entities", u64), ("num_preds", u32), ("num_types", u32),
                 ("num_triples", u64), ("zipf_s", dbl), ("degree_cap", u64),
-                ("type_sizes", P(u64)), ("num_hubs", u32), ("hub_degree", u64)]
+                ("type_sizes", P(u64)), ("num_hubs", u32), ("hub_degree", u64),
+                ("num_linked", u32), ("link_frac", dbl), ("num_sibling", u32),
+                ("sibling_src", P(u32)), ("sibling_dst", P(u32))]
 
 
 class BuildParams(C.Structure):

@@ -109,10 +109,26 @@ struct Perm {
     }
     return (l << half) | r;
   }
+  uint64_t dec(uint64_t y) const {  // one round of enc undone per step, last key first
+    uint64_t l = y >> half, r = y & mask;
+    for (int i = 3; i >= 0; i--) {
+      uint64_t pr = l;
+      l = r ^ (wkg::mix64(l ^ keys[i]) & mask);
+      r = pr;
+    }
+    return (l << half) | r;
+  }
   uint64_t operator()(uint64_t x) const {
     uint64_t y = enc(x);
     while (y >= n) y = enc(y);
     return y;
+  }
+  // inverse of operator(): walking dec back from y passes the out-of-range
+  // values the forward walk passed and stops at its in-range start
+  uint64_t inverse(uint64_t y) const {
+    uint64_t x = dec(y);
+    while (x >= n) x = dec(x);
+    return x;
   }
 };
 
@@ -169,6 +185,40 @@ void sample_distinct(Rng &rng, uint64_t n, uint64_t d, std::vector<uint32_t> &ou
   }
 }
 
+// A linked list: `out` starts with m distinct values taken from the source
+// list `src` (a seeded partial shuffle), then uniform distinct values of
+// [0, n) are added until it holds d values; returned ascending.
+void sample_linked(Rng &rng, uint64_t n, uint64_t d, uint64_t m, std::vector<uint32_t> src,
+                   std::vector<uint32_t> &out) {
+  out.clear();
+  m = std::min<uint64_t>(std::min<uint64_t>(m, d), src.size());
+  for (uint64_t i = 0; i < m; i++) {
+    std::swap(src[i], src[i + rng.below(src.size() - i)]);
+    out.push_back(src[i]);
+  }
+  std::sort(out.begin(), out.end());
+  if (d * 2 > n) {
+    std::vector<char> used(n, 0);
+    for (uint32_t v : out) used[v] = 1;
+    std::vector<uint32_t> rest;
+    rest.reserve(n - out.size());
+    for (uint64_t v = 0; v < n; v++)
+      if (!used[v]) rest.push_back((uint32_t)v);
+    for (uint64_t i = 0; out.size() < d; i++) {
+      std::swap(rest[i], rest[i + rng.below(rest.size() - i)]);
+      out.push_back(rest[i]);
+    }
+    std::sort(out.begin(), out.end());
+    return;
+  }
+  while (out.size() < d) {
+    size_t need = d - out.size();
+    for (size_t i = 0; i < need; i++) out.push_back((uint32_t)rng.below(n));
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -209,12 +259,31 @@ int wks_generate(const wks_gen_params *prm, wks_triples **out) {
     z.C = hi;
   }
   const uint64_t rmax = Tp > 0 ? z.max_rank() : 0;
+  if (prm->num_linked > P / 2) return fail(WKS_EINVAL, "num_linked must be <= num_preds / 2");
+  if (prm->num_linked && !(prm->link_frac >= 0 && prm->link_frac <= 1))
+    return fail(WKS_EINVAL, "link_frac must be in [0, 1]");
+  if (prm->num_sibling && (!prm->sibling_src || !prm->sibling_dst))
+    return fail(WKS_EINVAL, "sibling_src / sibling_dst is null");
+  for (uint32_t k = 0; k < prm->num_sibling; k++)
+    if (prm->sibling_src[k] < 2 || prm->sibling_src[k] >= 2 + P || prm->sibling_dst[k] < 2 ||
+        prm->sibling_dst[k] >= 2 + P)
+      return fail(WKS_EINVAL, "sibling plant %u names a predicate outside 2..%u", k, P + 1);
 
-  // per-predicate totals (hub predicates differ only in rank 1)
+  // degree of rank r in predicate j (hub predicates differ only in rank 1)
+  const uint64_t hub_deg = std::min<uint64_t>(prm->hub_degree, N);
+  auto degree = [&](uint32_t j, uint64_t r) -> uint64_t {
+    uint64_t d = r <= rmax ? z(r) : 0;
+    if (r == 1 && j < prm->num_hubs && hub_deg > d) d = hub_deg;
+    return d;
+  };
+  auto top_rank = [&](uint32_t j) -> uint64_t {
+    return (j < prm->num_hubs && rmax == 0 && hub_deg > 0) ? 1 : rmax;
+  };
+
+  // per-predicate totals
   std::vector<uint64_t> pred_total(P), pred_base(P + 1, 0);
   uint64_t base_total = 0;
   for (uint64_t r = 1; r <= rmax; r++) base_total += z(r);
-  const uint64_t hub_deg = std::min<uint64_t>(prm->hub_degree, N);
   for (uint32_t j = 0; j < P; j++) {
     uint64_t t = base_total;
     if (j < prm->num_hubs) {
@@ -229,7 +298,15 @@ int wks_generate(const wks_gen_params *prm, wks_triples **out) {
     if (prm->type_sizes[k] > N) return fail(WKS_EINVAL, "type %u larger than num_entities", k);
     type_total += prm->type_sizes[k];
   }
-  const uint64_t total = pred_base[P] + type_total;
+  // sibling plants: floor(d / 2) triples per subject of the source predicate
+  std::vector<uint64_t> sib_base(prm->num_sibling + 1, pred_base[P] + type_total);
+  for (uint32_t k = 0; k < prm->num_sibling; k++) {
+    const uint32_t j = prm->sibling_src[k] - 2;
+    uint64_t c = 0;
+    for (uint64_t r = 1; r <= top_rank(j); r++) c += degree(j, r) / 2;
+    sib_base[k + 1] = sib_base[k] + c;
+  }
+  const uint64_t total = sib_base[prm->num_sibling];
 
   wks_triples *t = new (std::nothrow) wks_triples;
   if (!t) return fail(WKS_ENOMEM, "out of memory");
@@ -244,27 +321,56 @@ int wks_generate(const wks_gen_params *prm, wks_triples **out) {
 
 #pragma omp parallel
   {
-    std::vector<uint32_t> objs;
+    std::vector<uint32_t> objs, src_objs;
 #pragma omp for schedule(dynamic, 1)
     for (uint32_t j = 0; j < P; j++) {
       Perm subj(N, derive(prm->seed, 1, j));
+      // a linked predicate (odd index below 2 * num_linked) reads its source's ranks
+      const bool linked = (j & 1) && j < 2 * prm->num_linked;
+      Perm src_subj(N, derive(prm->seed, 1, linked ? j - 1 : j));
+      std::vector<uint64_t> sib_pos;  // plants this predicate sources
+      std::vector<uint32_t> sib_pid;
+      for (uint32_t k = 0; k < prm->num_sibling; k++)
+        if (prm->sibling_src[k] == 2 + j) {
+          sib_pos.push_back(sib_base[k]);
+          sib_pid.push_back(prm->sibling_dst[k]);
+        }
       uint64_t pos = pred_base[j];
       const uint32_t pid = 2 + j;
-      uint64_t rtop = rmax;
-      if (j < prm->num_hubs && rtop == 0 && hub_deg > 0) rtop = 1;
+      const uint64_t rtop = top_rank(j);
       for (uint64_t r = 1; r <= rtop; r++) {
-        uint64_t d = r <= rmax ? z(r) : 0;
-        if (r == 1 && j < prm->num_hubs && hub_deg > d) d = hub_deg;
+        const uint64_t d = degree(j, r);
         if (!d) continue;
-        Rng rng(derive(prm->seed, 2 + j, r));
-        sample_distinct(rng, N, d, objs);
-        const uint32_t s = (uint32_t)(kVidBase + subj(r - 1));
+        const uint64_t sv = subj(r - 1);
+        uint64_t d_src = 0, r_src = 0;
+        if (linked) {
+          r_src = src_subj.inverse(sv) + 1;
+          d_src = r_src <= top_rank(j - 1) ? degree(j - 1, r_src) : 0;
+        }
+        if (d_src) {
+          Rng rs(derive(prm->seed, 2 + (j - 1), r_src));
+          sample_distinct(rs, N, d_src, src_objs);
+          Rng rl(derive(prm->seed, (0x5ull << 40) | j, r));
+          const uint64_t m = (uint64_t)std::ceil(prm->link_frac * (double)d);
+          sample_linked(rl, N, d, m, src_objs, objs);
+        } else {
+          Rng rng(derive(prm->seed, 2 + j, r));
+          sample_distinct(rng, N, d, objs);
+        }
+        const uint32_t s = (uint32_t)(kVidBase + sv);
         for (uint32_t ob : objs) {
           t->s[pos] = s;
           t->p[pos] = pid;
           t->o[pos] = (uint32_t)(kVidBase + ob);
           pos++;
         }
+        for (size_t k = 0; k < sib_pos.size(); k++)
+          for (size_t i = 0; i + 1 < objs.size(); i += 2) {
+            const uint64_t q = sib_pos[k]++;
+            t->s[q] = (uint32_t)(kVidBase + objs[i]);
+            t->p[q] = sib_pid[k];
+            t->o[q] = (uint32_t)(kVidBase + objs[i + 1]);
+          }
       }
     }
   }

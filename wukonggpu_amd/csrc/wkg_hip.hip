@@ -648,8 +648,10 @@ int wkg_ctx::step_k2u_lineage(const wkg_table *in, int sc, int pid, int dir, hip
           kept, q.tile_nz, q.chunk_nz, q.tile_sum, q.chunk_sum, kc, (uint2 *)lin, (uint64_t *)((char *)lin + kc * 8));
     }
     // the output's groups, when a group's list repeats over 4+ rows on
-    // average (filters from the new column then decide once per entry)
-    if (E >= 4 * dtot && nk * 24 <= pool->budget / 16) {
+    // average (filters from the new column then decide once per entry).
+    // They keep the grandparent table alive with the output, so its bytes
+    // count against the same share of the budget as a kept lineage parent.
+    if (E >= 4 * dtot && par->bytes + nk * 24 <= pool->budget / 16) {
       glin = pool->alloc(nk * 24, s, err_slot());
       if (glin) {
         ProfScope ps(this, s, "lineage", 40.0 * (double)nk);

@@ -34,11 +34,20 @@ class Triples:
 
     @classmethod
     def generate(cls, seed, num_entities, num_preds, num_triples, zipf_s, degree_cap=0,
-                 type_sizes=(), num_hubs=0, hub_degree=0):
+                 type_sizes=(), num_hubs=0, hub_degree=0, num_linked=0, link_frac=0.25, siblings=()):
+        """Seeded synthetic graph (wks_generate). num_linked pairs predicate
+        2+2k+1 to 2+2k (a share link_frac of its lists taken from the source
+        predicate's list of the same subject); siblings is a list of
+        (source pid, planted pid): consecutive objects of every source list
+        get a planted edge."""
         L = N.gstore_lib()
         ts = (C.c_uint64 * max(len(type_sizes), 1))(*[int(x) for x in type_sizes])
+        ss = (C.c_uint32 * max(len(siblings), 1))(*[int(a) for a, _ in siblings])
+        sd = (C.c_uint32 * max(len(siblings), 1))(*[int(b) for _, b in siblings])
         prm = N.GenParams(seed, num_entities, num_preds, len(type_sizes), num_triples, zipf_s,
-                          degree_cap, C.cast(ts, C.POINTER(C.c_uint64)), num_hubs, hub_degree)
+                          degree_cap, C.cast(ts, C.POINTER(C.c_uint64)), num_hubs, hub_degree,
+                          num_linked, link_frac, len(siblings), C.cast(ss, C.POINTER(C.c_uint32)),
+                          C.cast(sd, C.POINTER(C.c_uint32)))
         h = C.c_void_p()
         _check(L.wks_generate(C.byref(prm), C.byref(h)), "generate")
         return cls(h.value)

@@ -42,21 +42,26 @@ def config1(seed=1):
                                                                                         deg[subj == c][0])})
 
 
+LINK_FRAC = 0.25
+
+
 def _cfg23_graph(seed):
     """The 100M-triple graph shared by the heavy and batched configs: 10M
     entities, 64 predicates, Zipf 1.0 (uncapped), type t0 with 500k members and
-    511 more types with log-uniform sizes in [1, 1e5]."""
+    511 more types with log-uniform sizes in [1, 1e5]. Predicates are linked
+    in pairs (2k+3 to 2k+2, a quarter of a list from the same subject's
+    source list), so a star whose arms follow a linked pair can close."""
     N, P = 10_000_000, 64
     rng = np.random.default_rng(seed)
     extra = np.exp(rng.uniform(0.0, np.log(1e5), 511)).astype(np.int64)
     sizes = [500_000] + [int(x) for x in extra]
     tr = Triples.generate(seed=seed, num_entities=N, num_preds=P, num_triples=100_000_000, zipf_s=1.0,
-                          type_sizes=sizes)
+                          type_sizes=sizes, num_linked=P // 2, link_frac=LINK_FRAC)
     gs = GStore.build(tr)
     return tr, gs, N, P, sizes
 
 
-def heavy_query(P, t0, pa=2, pb=3, pc=4):
+def heavy_query(P, t0, pa=2, pb=4, pc=5):
     """Config-2 heavy query from type index t0 (2-hop chain, star of 2 outgoing
     predicates, closing known_to_known):
         ?x rdf:type t0 .       index_to_unknown                 (500k rows)
@@ -65,6 +70,8 @@ def heavy_query(P, t0, pa=2, pb=3, pc=4):
         ?z pb ?u .             known_to_unknown  OUT  (hub fan-out, the big table)
         ?z pc ?w .             known_to_unknown  OUT  (second star arm)
         ?z pb ?w               known_to_known, evaluated from ?w's IN list
+    On a graph whose pc is linked to pb (part of a subject's pc list is taken
+    from its pb list) the closing edge exists for about that part of the rows.
     """
     return [t0, 1, IN, -1,
             -1, pa, OUT, -2,
@@ -114,6 +121,14 @@ def config2_3(seed=2, n_batched=512):
     return Workload("config2_3", tr, gs, {"heavy": heavy, "batched": batch}, info)
 
 
+def linked_triple(rng, P):
+    """(pa, pb, pc) for a heavy query: pb a source predicate of a linked pair
+    (even offset from 2), pc its linked partner, pa any other predicate."""
+    pb = 2 + 2 * int(rng.integers(0, P // 2))
+    pa = int(rng.choice([p for p in range(2, 2 + P) if p not in (pb, pb + 1)]))
+    return pa, pb, pb + 1
+
+
 def served_mix(P, type_ids, t0, n, seed, heavy_frac=0.05):
     """Config-4 style mix over the given graph: light 2-3 pattern queries from
     type indexes and a fraction of heavy 5-6 pattern ones (the heavy query with
@@ -123,7 +138,7 @@ def served_mix(P, type_ids, t0, n, seed, heavy_frac=0.05):
     qs, flags = [], []
     for i in range(n):
         if rng.random() < heavy_frac:
-            pa, pb, pc = (int(x) for x in 2 + rng.choice(P, size=3, replace=False))
+            pa, pb, pc = linked_triple(rng, P)
             qs.append(heavy_query(P, t0, pa, pb, pc))
             flags.append(1)
         else:
@@ -193,7 +208,7 @@ def batch_stream(gs, P, light_types, t0, n, seed):
                 q += [-2, pc, dc, -3]
         tmpls.append(np.array(q, dtype=np.int64))
         members.append(m)
-    trip = [(2, 3, 4)] + [tuple(int(x) for x in 2 + rng.choice(P, size=3, replace=False)) for _ in range(3)]
+    trip = [(2, 4, 5)] + [linked_triple(rng, P) for _ in range(3)]
     for pa, pb, pc in trip:
         tmpls.append(np.array(heavy_query(P, t0, pa, pb, pc), dtype=np.int64))
     loads = np.array(BATCH_LIGHT_LOADS + BATCH_HEAVY_LOADS, dtype=np.float64)
@@ -213,17 +228,18 @@ def batch_stream(gs, P, light_types, t0, n, seed):
     return flat, off, heavy, types.astype(np.int32)
 
 
-def config4(seed=4, n_light_types=1023, keep_triples=False):
-    """Served graph: 100M entities, 256 predicates, ~1B triples (Zipf 1.0),
-    type t0 with 500k members for the heavy queries and n_light_types types
-    with log-uniform sizes in [1, 1e4] for the light ones. The triples are
-    released once the image is built."""
-    N, P = 100_000_000, 256
+def config4(seed=4, n_light_types=1023, keep_triples=False, N=100_000_000, P=256, T=1_000_000_000,
+            t0_members=500_000, light_max=1e4):
+    """Served graph: 100M entities, 256 predicates, ~1B triples (Zipf 1.0,
+    predicates linked in pairs), type t0 with 500k members for the heavy
+    queries and n_light_types types with log-uniform sizes in [1, 1e4] for
+    the light ones. Smaller N, P, T give the same shape at a stated scale.
+    The triples are released once the image is built unless kept."""
     rng = np.random.default_rng(seed)
-    extra = np.exp(rng.uniform(0.0, np.log(1e4), n_light_types)).astype(np.int64)
-    sizes = [500_000] + [int(x) for x in extra]
-    tr = Triples.generate(seed=seed, num_entities=N, num_preds=P, num_triples=1_000_000_000, zipf_s=1.0,
-                          type_sizes=sizes)
+    extra = np.exp(rng.uniform(0.0, np.log(light_max), n_light_types)).astype(np.int64)
+    sizes = [t0_members] + [int(x) for x in extra]
+    tr = Triples.generate(seed=seed, num_entities=N, num_preds=P, num_triples=T, zipf_s=1.0,
+                          type_sizes=sizes, num_linked=P // 2, link_frac=LINK_FRAC)
     n_tr = len(tr)
     t_build = time.perf_counter()
     gs = GStore.build(tr)
@@ -247,6 +263,9 @@ def cyclic_query(t, pa, ph, pc, pq, const):
         ?c ph ?d .        known_to_unknown  OUT  (hub fan-out: the ~1B-row table)
         ?d pc ?b .        known_to_known         (closes the triangle b-c-d)
         ?c pq const .     known_to_const
+    On a graph where ph plants pc edges between consecutive objects of each
+    ph list (sibling plants), the triangle closes for about half of the
+    (b, c) pairs, on the d that is b's planted partner.
     """
     return [t, 1, IN, -1,
             -1, pa, OUT, -2,
@@ -256,31 +275,51 @@ def cyclic_query(t, pa, ph, pc, pq, const):
             -3, pq, OUT, const]
 
 
-def config5(seed=5, t0_members=2_000_000, sample_members=20_000):
-    """Large graph: 200M entities, 512 predicates, ~2B triples, Zipf 0.9, and
-    hubs: the rank-1 subject of each of the first 8 predicates has 2M
-    edges. Types: t0 (t0_members) for the full query and t1
-    (sample_members) for a parity sample small enough for the oracle's
-    per-row work. The cyclic query goes through hub predicate ph = 2; the
-    known_to_const constant is one of the hub's own ph objects."""
-    N, P = 200_000_000, 512
-    sizes = [t0_members, sample_members]
-    tr = Triples.generate(seed=seed, num_entities=N, num_preds=P, num_triples=2_000_000_000, zipf_s=0.9,
-                          type_sizes=sizes, num_hubs=8, hub_degree=2_000_000)
-    n_tr = len(tr)
-    gs = GStore.build(tr)
-    tr.close()
+CONFIG5_PREDS = {"ph": 2, "pa": 100, "pc": 200, "pq": 2}
+
+
+def config5_triples(seed=5, N=200_000_000, P=512, T=2_000_000_000, hub_degree=2_000_000, t0_members=2_000_000,
+                    sample_members=20_000):
+    """The config-5 triples: Zipf 0.9, the rank-1 subject of each of the
+    first 8 predicates a hub of hub_degree edges, types t0 and t1, and pc
+    edges planted between consecutive objects of every ph list."""
+    c = CONFIG5_PREDS
+    return Triples.generate(seed=seed, num_entities=N, num_preds=P, num_triples=T, zipf_s=0.9,
+                            type_sizes=[t0_members, sample_members], num_hubs=8, hub_degree=hub_degree,
+                            siblings=[(c["ph"], c["pc"])])
+
+
+def config5_queries(gs, P):
+    """The cycle from t0 (full) and from t1 (sample), through hub predicate
+    ph; the known_to_const constant is one of the ph hub's own objects."""
+    c = CONFIG5_PREDS
     t0, t1 = 2 + P, 2 + P + 1
-    ph, pa, pc, pq = 2, 100, 200, 2
-    # the hub of ph: the subject with the longest ph OUT list; the constant is its first object
+    ph = c["ph"]
     hub, best = None, -1
     for s in gs.get_edges(0, ph, IN).tolist():
         d = gs.degree(s, ph, OUT)
         if d > best:
             hub, best = s, d
     const = int(gs.get_edges(hub, ph, OUT)[0])
-    queries = {"full": cyclic_query(t0, pa, ph, pc, pq, const), "sample": cyclic_query(t1, pa, ph, pc, pq, const)}
-    info = {"entities": N, "predicates": P, "triples": n_tr, "t0": t0, "t1": t1, "t0_members": t0_members,
-            "t1_members": sample_members, "ph": ph, "pa": pa, "pc": pc, "pq": pq, "hub": hub, "hub_degree": best,
-            "const": const, "keys": gs.num_keys, "edges": gs.num_edges, "image_bytes": gs.image_nbytes()}
+    queries = {"full": cyclic_query(t0, c["pa"], ph, c["pc"], c["pq"], const),
+               "sample": cyclic_query(t1, c["pa"], ph, c["pc"], c["pq"], const)}
+    return queries, {"t0": t0, "t1": t1, **c, "hub": hub, "hub_degree": best, "const": const}
+
+
+def config5(seed=5, t0_members=2_000_000, sample_members=20_000):
+    """Large graph: 200M entities, 512 predicates, ~2B triples, Zipf 0.9, and
+    hubs: the rank-1 subject of each of the first 8 predicates has 2M
+    edges. Types: t0 (t0_members) for the full query and t1
+    (sample_members) for a sample. The cyclic query goes through hub
+    predicate ph = 2; the known_to_const constant is one of the hub's own
+    ph objects."""
+    N, P = 200_000_000, 512
+    tr = config5_triples(seed, N, P, t0_members=t0_members, sample_members=sample_members)
+    n_tr = len(tr)
+    gs = GStore.build(tr)
+    tr.close()
+    queries, qi = config5_queries(gs, P)
+    info = {"entities": N, "predicates": P, "triples": n_tr, "t0_members": t0_members,
+            "t1_members": sample_members, **qi, "keys": gs.num_keys, "edges": gs.num_edges,
+            "image_bytes": gs.image_nbytes()}
     return Workload("config5", None, gs, queries, info)
