@@ -220,6 +220,27 @@ int wkg_step_c2u(wkg_ctx *ctx, int64_t vid, int pid, int dir, void *stream,
 int wkg_step_k2u(wkg_ctx *ctx, const wkg_table *in, int start_col, int pid, int dir,
                  void *stream, wkg_table **out, uint64_t *expanded);
 
+/* known_to_unknown that allocates nothing. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * result is wkg_step_k2u's, row for row. `in` is rows x cols device cells
+ * (row-major int32) and `out` a device buffer of out_cap_rows x (cols + 1)
+ * cells, both the caller's. The step's scratch (tile sums, compacted rows,
+ * split points) is a fixed device array of WKG_NOALLOC_SCRATCH_BYTES per
+ * GPU, set up by wkg_init and held until the process ends; an input or
+ * output too big for it fails with WKG_ERANGE before any launch or before
+ * the expand (about 32 bytes per input row plus 1 byte per 64 output rows).
+ * *out_rows gets the output's row count; when it exceeds out_cap_rows the
+ * call fails with WKG_ERANGE and writes nothing to `out`. The graph must be
+ * resident (WKG_ESTATE when framed). Every call on one GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (other streams, other threads,
+ * other contexts) race; calls on one stream are ordered. The output is
+ * complete in stream order. */
+#define WKG_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int wkg_step_k2u_noalloc(wkg_ctx *ctx, const int32_t *in, uint64_t rows, int cols, int start_col, int pid, int dir,
+                         int32_t *out, uint64_t out_cap_rows, void *stream, uint64_t *out_rows,
+                         uint64_t *expanded);
+
 /* known_to_known: keep the rows whose row[end_col] is in the edge list of
  * (row[start_col], pid, dir). RCache::known_to_known, core/rcache.hpp:417-612. */
 int wkg_step_k2k(wkg_ctx *ctx, const wkg_table *in, int start_col, int pid, int dir,

@@ -143,6 +143,8 @@ def hip_lib():
         _sig(L, "wkg_step_i2u", C.c_int, vp, C.c_int, C.c_int, vp, P(vp), P(u64))
         _sig(L, "wkg_step_c2u", C.c_int, vp, i64, C.c_int, C.c_int, vp, P(vp), P(u64))
         _sig(L, "wkg_step_k2u", C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, vp, P(vp), P(u64))
+        _sig(L, "wkg_step_k2u_noalloc", C.c_int, vp, vp, u64, C.c_int, C.c_int, C.c_int, C.c_int, vp, u64, vp,
+             P(u64), P(u64))
         _sig(L, "wkg_step_k2k", C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, P(vp), P(u64))
         _sig(L, "wkg_step_k2c", C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, i64, vp, P(vp), P(u64))
         _sig(L, "wkg_step_batch", C.c_int, vp, C.c_int, P(StepDesc), vp, P(vp), P(u64))

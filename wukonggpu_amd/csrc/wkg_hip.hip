@@ -377,22 +377,36 @@ static void launch_finalize(int tr, const StepParams &p0, hipStream_t s) {
   }
 }
 
+// Scratch words of one step's probe / scan / finalize passes over R rows in
+// TR-row tiles: totals[4] | tile_sum | tile_nz | tile_ex | chunk sums x 3 |
+// hits[ntiles * TR] (16 B each).
+static uint64_t pass_scratch_words(uint64_t R, int TR) {
+  const uint64_t ntiles = (R + TR - 1) / TR;
+  const uint64_t nchunks = (ntiles + kScanChunk - 1) / kScanChunk;
+  return 4 + 3 * ntiles + 3 * nchunks + 2 * ntiles * TR;
+}
+
 // Scratch and parameters of one step's probe / scan / finalize passes over
 // TR-row tiles, with `extra` more scratch words after them (at p's end: the
-// returned pointer is pp.scr + pp's own words).
+// returned pointer is pp.scr + pp's own words). With ext_scr (and, for
+// known_to_unknown, ext_lin of 16 B per row) the scratch is the caller's and
+// nothing is allocated.
 static int pass_setup(wkg_ctx *ctx, int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval,
-                      int32_t *O, hipStream_t s, int TR, uint64_t extra, ProbePass &pp, StepParams &p) {
+                      int32_t *O, hipStream_t s, int TR, uint64_t extra, ProbePass &pp, StepParams &p,
+                      uint64_t *ext_scr = nullptr, void *ext_lin = nullptr) {
   const uint64_t R = in->rows;
   const int c = in->cols;
   const uint64_t ntiles = (R + TR - 1) / TR;
-  // scratch: totals[4] | tile_sum | tile_nz | tile_ex | chunk sums x 3 | hits[ntiles * TR] (16 B) | extra
-  const uint64_t hit_words = 2 * ntiles * TR;
   const uint64_t nchunks = (ntiles + kScanChunk - 1) / kScanChunk;
-  const uint64_t scratch_words = 4 + 3 * ntiles + 3 * nchunks + hit_words + extra;
-  pp.scr = (uint64_t *)ctx->pool->alloc(scratch_words * 8, s, wkg_ctx::err_slot());
-  if (!pp.scr) return WKG_ENOMEM;
+  if (ext_scr) {
+    pp.external = true;
+    pp.scr = ext_scr;
+  } else {
+    pp.scr = (uint64_t *)ctx->pool->alloc((pass_scratch_words(R, TR) + extra) * 8, s, wkg_ctx::err_slot());
+    if (!pp.scr) return WKG_ENOMEM;
+  }
   if (mode == kModeK2U) {  // cinfo[R] (8 B) | cexcl[R]: may outlive the step as the output's lineage
-    pp.lin = ctx->pool->alloc(R * 16, s, wkg_ctx::err_slot());
+    pp.lin = ext_scr ? ext_lin : ctx->pool->alloc(R * 16, s, wkg_ctx::err_slot());
     if (!pp.lin) {
       ctx->release_pass(pp, s);
       return WKG_ENOMEM;
@@ -434,12 +448,12 @@ static int pass_setup(wkg_ctx *ctx, int mode, const wkg_table *in, int sc, int p
 // every known-start step): for K2U the compacted rows with edges (cinfo) and
 // their output offsets (cexcl) in pp.lin; the filters compact into O.
 int wkg_ctx::probe_pass(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval, int32_t *O,
-                        hipStream_t s, ProbePass &pp, int wcol) {
+                        hipStream_t s, ProbePass &pp, int wcol, uint64_t *ext_scr, void *ext_lin, int tile_rows) {
   const uint64_t R = in->rows;
   const int c = in->cols;
-  const int TR = probe_tile ? probe_tile : probe_tile_rows(R, ncu);
+  const int TR = tile_rows ? tile_rows : probe_tile ? probe_tile : probe_tile_rows(R, ncu);
   StepParams p{};
-  int rc = pass_setup(this, mode, in, sc, pid, dir, ec, cval, O, s, TR, 0, pp, p);
+  int rc = pass_setup(this, mode, in, sc, pid, dir, ec, cval, O, s, TR, 0, pp, p, ext_scr, ext_lin);
   if (rc) return rc;
   if (wcol >= 0) {
     p.weighted = 1;
@@ -491,19 +505,22 @@ int wkg_ctx::probe_pass(int mode, const wkg_table *in, int sc, int pid, int dir,
 }
 
 void wkg_ctx::release_pass(ProbePass &pp, hipStream_t s) {
-  if (pp.scr) pool->release(pp.scr, s);
-  if (pp.lin) pool->release(pp.lin, s);
+  if (!pp.external) {
+    if (pp.scr) pool->release(pp.scr, s);
+    if (pp.lin) pool->release(pp.lin, s);
+  }
   pp = ProbePass{};
 }
 
 // Merge-path expand of a known_to_unknown step: output row o of compacted
 // row i is (input row cinfo[i].x, edge o - cexcl[i]).
 int wkg_ctx::expand_pass(const wkg_table *in, int pid, const uint2 *cinfo, const uint64_t *cexcl, uint64_t nz,
-                         uint64_t E, hipStream_t s, wkg_table *t, const char *tag) {
+                         uint64_t E, hipStream_t s, wkg_table *t, const char *tag, uint64_t *ext_part) {
   if (!E) return WKG_OK;
   const int c = in->cols;
   const uint64_t nx = (nz + E + kExpandNV - 1) / kExpandNV;
-  uint64_t *part = (uint64_t *)pool->alloc((nx + 1) * 8, s, err_slot());
+  // split points: (nx + 1) words, the caller's (ext_part) or the pool's
+  uint64_t *part = ext_part ? ext_part : (uint64_t *)pool->alloc((nx + 1) * 8, s, err_slot());
   if (!part) return WKG_ENOMEM;
   {
     ProfScope ps(this, s, "partition", 8.0 * (double)(nx + 1));
@@ -535,7 +552,7 @@ int wkg_ctx::expand_pass(const wkg_table *in, int pid, const uint2 *cinfo, const
     }
   }
   hipError_t e2 = hipGetLastError();
-  pool->release(part, s);
+  if (!ext_part) pool->release(part, s);
   if (e2 != hipSuccess) return fail(WKG_EDEVICE, "expand launch failed: %s", hipGetErrorString(e2));
   return WKG_OK;
 }
@@ -1121,6 +1138,32 @@ int wkg_random_access_probe(wkg_ctx *ctx, uint64_t bytes, int iters, double *lin
   return WKG_OK;
 }
 
+// Step scratch of wkg_step_k2u_noalloc, one copy per GPU, held until the
+// process ends. Host code never uses this name for memory: it is a host-side
+// stand-in for the symbol. Each device's address comes from
+// hipGetSymbolAddress once per device (noalloc_scratch_of, called by
+// wkg_init with that device current) and is what the kernels are given.
+static __device__ __align__(256) uint8_t scratch[WKG_NOALLOC_SCRATCH_BYTES];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];  // per device: the address of its copy of `scratch`
+
+// The current device's scratch address (looked up on its first use).
+static void *noalloc_scratch_of(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    g_scratch_ptr[device] = p;
+  }
+  return g_scratch_ptr[device];
+}
+
 int wkg_init(int device, uint64_t table_budget_bytes, wkg_ctx **out) {
   if (!out) return WKG_EINVAL;
   *out = nullptr;
@@ -1144,6 +1187,13 @@ int wkg_init(int device, uint64_t table_budget_bytes, wkg_ctx **out) {
     return WKG_EDEVICE;
   }
   memset(c->h_rb, 0, kStageOff);  // the signal flag starts below every sequence number
+  c->noalloc_scratch = (uint64_t *)noalloc_scratch_of(device);
+  if (!c->noalloc_scratch) {
+    (void)hipHostFree(c->h_rb);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return WKG_EDEVICE;
+  }
   size_t fr = 0, tot = 0;
   (void)hipMemGetInfo(&fr, &tot);
   c->pool = new Pool(table_budget_bytes ? table_budget_bytes : (uint64_t)(fr * 0.75));
@@ -1321,6 +1371,70 @@ int wkg_step_k2u(wkg_ctx *ctx, const wkg_table *in, int start_col, int pid, int 
                  wkg_table **out, uint64_t *expanded) {
   if (!ctx || !out) return WKG_EINVAL;
   return ctx->step_known(kModeK2U, in, start_col, pid, dir, -1, 0, ctx->pick(stream), out, expanded);
+}
+
+int wkg_step_k2u_noalloc(wkg_ctx *ctx, const int32_t *in, uint64_t rows, int cols, int start_col, int pid, int dir,
+                         int32_t *out, uint64_t out_cap_rows, void *stream, uint64_t *out_rows, uint64_t *expanded) {
+  if (!ctx || !out_rows) return WKG_EINVAL;
+  *out_rows = 0;
+  if (expanded) *expanded = 0;
+  int rc = ctx->check_pattern(pid, dir);
+  if (rc) return rc;
+  if (ctx->framed)
+    return ctx->fail(WKG_ESTATE, "the no-alloc step needs a resident graph (framed segments load on demand)");
+  if (cols < 1 || start_col < 0 || start_col >= cols)
+    return ctx->fail(WKG_EINVAL, "start column %d outside the %d-column table", start_col, cols);
+  if (cols + 1 > kBlock) return ctx->fail(WKG_ERANGE, "tables are limited to %d columns", kBlock);
+  if (rows >= (1ull << 32)) return ctx->fail(WKG_ERANGE, "tables are limited to 2^32 - 1 rows per step");
+  if (rows && !in) return ctx->fail(WKG_EINVAL, "null input table");
+  if (!ctx->noalloc_scratch) return ctx->fail(WKG_ESTATE, "no scratch for the no-alloc step on this device");
+  if (rows == 0) return WKG_OK;
+  hipStream_t s = ctx->pick(stream);
+  // scratch: the probe passes' words | compacted rows (16 B per row) | the expand's split points
+  auto up = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
+  const int TR = ctx->probe_tile ? ctx->probe_tile : probe_tile_rows(rows, ctx->ncu);
+  const uint64_t pass_b = up(pass_scratch_words(rows, TR) * 8), lin_b = up(rows * 16);
+  if (pass_b + lin_b + 256 > WKG_NOALLOC_SCRATCH_BYTES)
+    return ctx->fail(WKG_ERANGE, "a %llu-row input needs %llu bytes of step scratch, above the %llu of the no-alloc step",
+                     (unsigned long long)rows, (unsigned long long)(pass_b + lin_b + 256),
+                     (unsigned long long)WKG_NOALLOC_SCRATCH_BYTES);
+  char *base = (char *)ctx->noalloc_scratch;
+  wkg_table tin{};
+  tin.d = const_cast<int32_t *>(in);
+  tin.rows = rows;
+  tin.cols = cols;
+  tin.bytes = rows * (uint64_t)cols * 4;
+  tin.stream = s;
+  tin.owned = false;
+  if ((rc = ctx->ensure_pred(pid, s))) return rc;
+  ProbePass pp;
+  if ((rc = ctx->probe_pass(kModeK2U, &tin, start_col, pid, dir, -1, 0, nullptr, s, pp, -1, (uint64_t *)base,
+                            base + pass_b, TR)))
+    return rc;
+  const uint64_t E = pp.cnt[0], nz = pp.cnt[2];
+  *out_rows = E;
+  if (expanded) *expanded = E;
+  const uint64_t part_b = ((nz + E + kExpandNV - 1) / kExpandNV + 1) * 8;
+  if (E > out_cap_rows || (E && !out) || pass_b + lin_b + part_b > WKG_NOALLOC_SCRATCH_BYTES) {
+    ctx->release_pass(pp, s);
+    if (E > out_cap_rows)
+      return ctx->fail(WKG_ERANGE, "the output has %llu rows; the buffer holds %llu", (unsigned long long)E,
+                       (unsigned long long)out_cap_rows);
+    if (E && !out) return ctx->fail(WKG_EINVAL, "null output buffer");
+    return ctx->fail(WKG_ERANGE, "a %llu-row output needs %llu bytes of step scratch, above the %llu of the "
+                     "no-alloc step", (unsigned long long)E, (unsigned long long)(pass_b + lin_b + part_b),
+                     (unsigned long long)WKG_NOALLOC_SCRATCH_BYTES);
+  }
+  wkg_table tout{};
+  tout.d = out;
+  tout.rows = E;
+  tout.cols = cols + 1;
+  tout.bytes = E * (uint64_t)(cols + 1) * 4;
+  tout.stream = s;
+  tout.owned = false;
+  rc = ctx->expand_pass(&tin, pid, pp.cinfo, pp.cexcl, nz, E, s, &tout, "expand", (uint64_t *)(base + pass_b + lin_b));
+  ctx->release_pass(pp, s);
+  return rc;
 }
 
 int wkg_step_k2k(wkg_ctx *ctx, const wkg_table *in, int start_col, int pid, int dir, int end_col,

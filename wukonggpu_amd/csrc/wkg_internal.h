@@ -101,6 +101,7 @@ struct ProbePass {
   const uint64_t *tile_nz = nullptr, *chunk_nz = nullptr;
   uint64_t ntiles = 0;
   int tr = 0;
+  bool external = false;  // scr / lin are the caller's memory, not the pool's
 };
 
 // Framed residency (graph larger than its HBM budget): per-predicate
@@ -132,6 +133,7 @@ struct wkg_ctx {
   bool lineage = true;       // probe inherited columns once per parent row
   uint64_t group_filter_rows = 1ull << 22;  // smallest input a filter decides per lineage group
   bool small_steps = true;   // steps over at most kSmallRows rows in one launch
+  uint64_t *noalloc_scratch = nullptr;  // this device's no-alloc step scratch (wkg_step_k2u_noalloc)
   uint64_t *h_rb = nullptr;  // pinned read-back block: counters, the signal flag, small-table staging
   uint64_t *h_rb_dev = nullptr;  // the same block's device address
   uint64_t rb_seq = 0;       // the last read-back's signal value
@@ -189,12 +191,13 @@ struct wkg_ctx {
   int step_known(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval, hipStream_t s,
                  wkg_table **out, uint64_t *expanded);
   int probe_pass(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval, int32_t *O,
-                 hipStream_t s, ProbePass &pp, int wcol = -1);
+                 hipStream_t s, ProbePass &pp, int wcol = -1, uint64_t *ext_scr = nullptr, void *ext_lin = nullptr,
+                 int tile_rows = 0);
   int step_small(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval, hipStream_t s,
                  wkg_table **out, uint64_t *expanded);
   void release_pass(ProbePass &pp, hipStream_t s);
   int expand_pass(const wkg_table *in, int pid, const uint2 *cinfo, const uint64_t *cexcl, uint64_t nz, uint64_t E,
-                  hipStream_t s, wkg_table *t, const char *tag = "expand");
+                  hipStream_t s, wkg_table *t, const char *tag = "expand", uint64_t *ext_part = nullptr);
   void keep_lineage(const wkg_table *in, ProbePass &pp, uint64_t nz, wkg_table *t);
   int step_k2u_lineage(const wkg_table *in, int sc, int pid, int dir, hipStream_t s, wkg_table **out,
                        uint64_t *expanded);

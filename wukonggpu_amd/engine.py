@@ -219,6 +219,20 @@ class Device:
     def k2u(self, t, start_col, pid, d):
         return self._step(N.hip_lib().wkg_step_k2u, t._h, start_col, pid, d)
 
+    def k2u_noalloc(self, t, start_col, pid, d, out):
+        """known_to_unknown into the caller's buffer `out` (a Table of at
+        least the output's rows, cols + 1 columns) with the library's fixed
+        scratch (wkg_step_k2u_noalloc): returns (rows written, expanded)."""
+        rows, cols = t.shape
+        cap, ocols = out.shape
+        if ocols != cols + 1:
+            raise ValueError("the output buffer needs cols + 1 columns")
+        L = N.hip_lib()
+        n, ex = C.c_uint64(), C.c_uint64()
+        self._check(L.wkg_step_k2u_noalloc(self._h, L.wkg_table_device_ptr(t._h), rows, cols, start_col, pid, d,
+                                           L.wkg_table_device_ptr(out._h), cap, None, C.byref(n), C.byref(ex)))
+        return int(n.value), int(ex.value)
+
     def k2k(self, t, start_col, pid, d, end_col):
         return self._step(N.hip_lib().wkg_step_k2k, t._h, start_col, pid, d, end_col)
 
