@@ -1,26 +1,27 @@
 #!/usr/bin/env python3
 """GPU graph-exploration benchmark on one MI355X.
 
-Default workload (one "step" = one heavy query end to end, graph resident in
-HBM): the heavy 6-pattern query on the 100M-triple graph (10M entities, 64
-predicates, Zipf 1.0, a 500k-member type index), run on the plain GPU path
-(one pattern step at a time through the C++ engine, tables kept on the
-device, final table copied to the host). `value` = expanded edges/s.
+The N=1 line is the largest single-GPU configuration: the 6-pattern cyclic
+query (a triangle closed by known_to_known, then a known_to_const filter) on
+the 2B-triple graph (200M entities, 512 predicates, Zipf 0.9, 2M-edge hubs;
+an 89.8 GB image resident in HBM), on the plain GPU path with its ~1B-row
+hub fan-out produced and consumed in HBM-budgeted 4 GB chunks. One "step" is
+one such query end to end. `value` = expanded edges/s. Beside it: the same
+query through the device-resident executor (batched with the 100k-member
+sample cycle), both answers checked row for row against the CPU oracle, and
+the oracle on one host thread as the reported CPU baseline.
 
-Beside it, on the same graph, with identical answers required everywhere:
-  - the roofline of the dominant kernel (the expand of the hub fan-out step)
-    from HIP events over the timed region;
-  - the heavy query through the device-resident executor;
-  - the batched mix (512 selective 3-4 pattern queries from type indexes of
-    log-uniform size): plain path, step-synchronous rounds, and the
-    device-resident executor under the shape its autotuner picks;
-  - the served form: open-loop Poisson arrivals of a light/heavy mix at a
-    sweep of offered rates, p50/p99 latency and achieved queries/s, each
-    answer compared with the same query run alone;
-  - the CPU oracle (a restatement of the reference's engine.hpp handlers)
-    timed on the host as the reported baseline.
+The other configurations ride along as extra keys, each checked against the
+oracle (the same keys `--config N` prints as a line of its own):
+  config2_3  the heavy 6-pattern query on the 100M-triple graph (plain and
+             executor), its 151M-row fan-out table, the 512-query batched mix
+             (plain, step rounds, autotuned executor; every table checked),
+             the served Poisson sweep, the closed-loop batch mode and the
+             fork-join table split;
+  config4    the served Poisson sweep and batch mode on the 1B-triple graph;
+  config1    the 2-hop query from a constant on the 10M-triple graph.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config all|1|2|4|5]
 Progress goes to stderr; one JSON line goes to stdout.
 """
 import argparse
@@ -34,10 +35,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "expanded edges/s + queries/s; p50/p99 latency, batched vs plain on 1 MI355X"
 HBM_PEAK_GBS = 8000.0
+ORACLE_THREADS = 16   # the GPU host's CPU share
+C5_SAMPLE = 100_000   # members of the config-5 sample type (the bounded CPU baseline)
+C5_LIMIT = 4 << 30    # config 5's output-table limit: the hub fan-out in 4 GB chunks
+T_START = time.time()
 
 
 def log(*a):
-    print(*a, file=sys.stderr, flush=True)
+    print(f"[{time.time() - T_START:6.1f}s]", *a, file=sys.stderr, flush=True)
 
 
 def pct(vals, q):
@@ -49,13 +54,89 @@ def pct(vals, q):
     return s[min(int(q * len(s)), len(s) - 1)]
 
 
+def _cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
+
+
+def _same(xs, ys):
+    import numpy as np
+    return all(a.status == 0 and b.status == 0 and a.table.shape == b.table.shape and
+               np.array_equal(a.table, b.table) and a.expanded == b.expanded for a, b in zip(xs, ys))
+
+
+def _eq(res, want, wex):
+    """A GPU result equals an oracle table, row for row, and its work count."""
+    import numpy as np
+    return bool(res.status == 0 and res.table.shape[0] == want.shape[0] and
+                (want.size == 0 or np.array_equal(res.table.reshape(want.shape), want)) and res.expanded == wex)
+
+
+def _oracle_digests(gs, queries, results):
+    """Every result against the oracle's table for the same query: row count,
+    work count and an order-sensitive digest of the cells. Returns (all
+    equal, queries checked, oracle wall seconds)."""
+    import oracle
+    _, rows, ex, wall, dg = oracle.run_many(gs, queries, threads=ORACLE_THREADS, digests=True)
+    ok = all(r.status == 0 and r.table.shape[0] == rows[i] and r.expanded == ex[i] and
+             oracle.table_digest(r.table) == int(dg[i]) for i, r in enumerate(results))
+    return bool(ok), len(queries), wall
+
+
+def _kernel_table(dev, names, n):
+    out = {}
+    for k in names:
+        v = dev.profile_read(k)
+        if v[0]:
+            out[k] = {"launches": v[0] / n, "ms": v[1] / n, "alg_GBps": (v[2] / (v[1] * 1e-3) / 1e9) if v[1] > 0 else 0.0}
+    return out
+
+
+STEP_KERNELS = ("probe", "probe_filter", "scan_tiles", "finalize", "compact", "partition", "expand", "probe_one",
+                "copy_list", "small_step", "lineage")
+
+
+def _expand_roofline(dev, n_queries, traffic_file, extra=None):
+    """HBM roofline of k_expand from HIP events over the profiled queries:
+    algorithmic bytes per launch / average launch time; traffic (PMC bytes
+    per launch) from the committed rocprofv3 summary when there is one."""
+    n_exp, ms_exp, b_exp = dev.profile_read("expand")
+    achieved = b_exp / (ms_exp * 1e-3) / 1e9 if ms_exp > 0 else 0.0
+    traffic, src = None, None
+    tp = os.path.join(ROOT, "profiles", traffic_file)
+    if os.path.exists(tp):
+        with open(tp) as f:
+            t = json.load(f)
+        traffic, src = t["hbm_bytes_per_launch"], t["source"]
+    r = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "expand (k_expand)",
+         "launches_per_query": n_exp / max(n_queries, 1), "alg_bytes_per_launch": b_exp / max(n_exp, 1),
+         "avg_launch_ms": ms_exp / max(n_exp, 1), "traffic_source": src}
+    if extra:
+        r.update(extra)
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--config", default="all", choices=["all", "1", "2", "4", "5"],
+                    help="all (default): config 5 as the line, configs 4, 2/3 and 1 as extra keys; "
+                         "N: that configuration's line alone")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle parity legs")
+    ap.add_argument("--time-budget", type=float, default=480.0,
+                    help="seconds after which the default run leaves out its optional legs")
     ap.add_argument("--served-n", type=int, default=2000, help="queries per served rate (0: skip)")
     ap.add_argument("--served-rates", default="1000,2000,4000", help="offered Poisson rates, queries/s")
     ap.add_argument("--batch-s", type=float, default=2.0,
@@ -71,23 +152,27 @@ def main():
                          "decided per lineage group and per row")
     ap.add_argument("--group-filter-rows", type=int, default=1 << 22,
                     help="with --served-heavy-only: smallest table a filter decides per lineage group")
-    ap.add_argument("--config", default="2", choices=["1", "2", "4", "5"],
-                    help="2: heavy query + batched mix + served sweep on the 100M-triple graph (default); "
-                         "1: plain single query on the 10M-triple graph; 4: served sweep on the 1B-triple "
-                         "graph; 5: the cyclic query on the 2B-triple graph in HBM-budgeted chunks")
     ap.add_argument("--heavy-only", action="store_true",
-                    help="run only the heavy query and the copy calibration (for profiler passes)")
+                    help="run only the config-2 heavy query and the copy calibration (for profiler passes)")
     ap.add_argument("--probe-only", action="store_true",
                     help="run only the known_to_unknown probe over 64M random keys (for profiler passes)")
     ap.add_argument("--mix-only", action="store_true",
                     help="run only the 512-query mix through the executor, 5 batches (for profiler passes)")
+    ap.add_argument("--cycle-only", action="store_true",
+                    help="run only the config-5 cyclic query, timed and with HIP events (for profiler passes)")
     ap.add_argument("--shape", default="",
                     help="with --mix-only: the executor shape as key=value pairs (probe_wg_per_cu=3,...) "
                          "instead of autotuning")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU's shard; --gpus {args.gpus} is not supported "
+              "(this build runs on a single MI355X)", file=sys.stderr)
+        sys.exit(2)
     if args.served_heavy_only:
         args.batch_only = True
-    if args.heavy_only or args.probe_only or args.mix_only or args.batch_only or args.exchange_only:
+    profiler_pass = (args.heavy_only or args.probe_only or args.mix_only or args.batch_only or args.exchange_only
+                     or args.cycle_only)
+    if profiler_pass:
         args.no_cpu = True
     if args.exchange_only:
         from wukonggpu_amd import Device
@@ -95,40 +180,193 @@ def main():
         print(json.dumps({"table_exchange": table_exchange(dev, args.seed)}), flush=True)
         dev.close()
         return
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU's shard; --gpus {args.gpus} is not supported "
-              "(this build runs on a single MI355X)", file=sys.stderr)
-        sys.exit(2)
-
-    import numpy as np
-
-    from wukonggpu_amd import Device, GPUEngine, workloads
-
-    if args.config != "2":
-        {"1": run_config1, "4": run_config4, "5": run_config5}[args.config](args)
+    if args.cycle_only:
+        print(json.dumps(run_config5(args, quick=True)), flush=True)
+        return
+    if profiler_pass:
+        print(json.dumps(run_config23(args)), flush=True)
+        return
+    if args.config != "all":
+        fn = {"1": run_config1, "2": run_config23, "4": run_config4, "5": run_config5}[args.config]
+        print(json.dumps(fn(args)), flush=True)
         return
 
+    line = run_config5(args)
+    extra = {}
+    for name, fn in (("config4", run_config4), ("config2_3", run_config23), ("config1", run_config1)):
+        t = time.time()
+        try:
+            extra[name] = fn(args, nested=True)
+        except Exception as e:  # a failing extra leg must not lose the headline line
+            extra[name] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"{name} failed: {e}")
+        extra[name]["leg_s"] = time.time() - t
+    line.update(extra)
+    line["bench_wall_s"] = time.time() - T_START
+    print(json.dumps(line), flush=True)
+
+
+def _line(value, unit, steps, warmup, ms, config, extra, higher=True, dtype="u32"):
+    out = {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": ms, "higher_is_better": higher, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+           "data": "synthetic (seeded generator; graph resident in HBM)", "config": config}
+    out.update(extra)
+    return out
+
+
+def time_left(args):
+    return args.time_budget - (time.time() - T_START)
+
+
+# ---- config 5: the cyclic query on the 2B-triple graph (the N=1 line) --------------------------------
+
+def run_config5(args, quick=False, nested=False):
+    """The cyclic 6-pattern query on the 2B-triple graph: plain path with a
+    4 GB output-table limit (the ~1B-row hub fan-out produced and consumed in
+    row chunks, depth-first), unchunked for comparison, the device-resident
+    executor on a batch of the cycle from both start types, the CPU oracle
+    on the full query (parallel, as the checker) and on the sample (one
+    thread, as the baseline)."""
+    import numpy as np
+    import oracle
+    from wukonggpu_amd import Device, GPUEngine, workloads
     t0 = time.time()
-    w = workloads.config2_3(seed=args.seed)
-    log(f"graph: {w.info['triples']} triples, {w.info['keys']} keys, image {w.info['image_bytes'] / 1e9:.2f} GB "
-        f"built in {time.time() - t0:.1f}s")
-    device_build = None
-    if not (args.heavy_only or args.probe_only or args.mix_only or args.batch_only):
-        device_build = compare_device_build(w, Device)
-    dev = Device(0)
+    w = workloads.config5(seed=5, sample_members=C5_SAMPLE)
+    build_s = time.time() - t0
+    log(f"config 5 graph: {w.info['triples']} triples, {w.info['keys']} keys, image "
+        f"{w.info['image_bytes'] / 1e9:.2f} GB, generated and built in {build_s:.1f}s; hub {w.info['hub']} "
+        f"degree {w.info['hub_degree']}")
+    dev = Device(0, table_budget_bytes=96 << 30)
     t1 = time.time()
     dev.load(w.gstore)
-    log(f"graph resident in HBM in {time.time() - t1:.1f}s")
+    load_s = time.time() - t1
     eng = GPUEngine(dev)
-    if args.probe_only:
-        rk = random_key_probe(dev, w, args.seed, HBM_PEAK_GBS * 1e9 / 128)
-        print(json.dumps({"probe_random_keys": rk}), flush=True)
+    full, sample = w.queries["full"], w.queries["sample"]
+    dev.set_table_limit(C5_LIMIT)
+    for _ in range(max(1, args.warmup)):
+        r = eng.run_query(full)
+    s0 = eng.splits()
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        q0 = time.perf_counter()
+        r = eng.run_query(full)
+        lat.append(time.perf_counter() - q0)
+    elapsed = time.perf_counter() - t_start
+    splits = (eng.splits() - s0) / max(args.steps, 1)
+    value = r.expanded * args.steps / elapsed
+    log(f"cycle, plain chunked: {r.table.shape[0]} rows, {r.expanded} expanded edges, "
+        f"{elapsed / args.steps * 1e3:.2f} ms/query, {value / 1e9:.2f} G expanded edges/s, {splits:.0f} splits/query")
+    n_prof = max(1, min(args.steps, 5))
+    dev.profile(True)
+    t_prof = time.perf_counter()
+    for _ in range(n_prof):
+        eng.run_query(full)
+    prof_ms = (time.perf_counter() - t_prof) / n_prof * 1e3
+    kernels = _kernel_table(dev, STEP_KERNELS, n_prof)
+    roof = _expand_roofline(dev, n_prof, "expand_cycle_2b.traffic.json")
+    dev.profile(False)
+    log(f"k_expand: {roof['launches_per_query']:.0f} launches/query, {roof['avg_launch_ms']:.3f} ms avg, "
+        f"{roof['achieved']:.0f} GB/s algorithmic = {roof['frac']:.2f} of peak")
+    config = {"workload": "6-pattern cyclic query (triangle closed by known_to_known, known_to_const filter) on a "
+                          "2B-triple graph: 200M entities, 512 predicates, Zipf 0.9, 2M-edge hubs, pc edges planted "
+                          "between consecutive ph objects; 4 GB output-table limit (HBM-budgeted chunks)",
+              **{k: v for k, v in w.info.items()}, "query": full, "result_rows": int(r.table.shape[0]),
+              "expanded_edges_per_query": int(r.expanded), "row_half_splits_per_query": splits,
+              "p50_ms": pct(lat, 0.5) * 1e3, "p99_ms": pct(lat, 0.99) * 1e3, "graph_build_s": build_s,
+              "graph_load_s": load_s, "ms_per_query_with_kernel_events": prof_ms,
+              "kernels_per_query": kernels, "gpu_kernel_ms_per_query": sum(v["ms"] for v in kernels.values())}
+    if quick:
+        out = _line(value, "expanded edges/s", args.steps, args.warmup, elapsed / args.steps * 1e3, config,
+                    {"roofline": roof})
         eng.close()
         dev.close()
-        return
+        return out
+    # the same query with whole tables (no output limit)
+    dev.set_table_limit(0)
+    rw = eng.run_query(full)
+    tw = []
+    for _ in range(3):
+        q0 = time.perf_counter()
+        rw = eng.run_query(full)
+        tw.append(time.perf_counter() - q0)
+    whole_same = _same([r], [rw])
+    log(f"cycle, whole tables: {pct(tw, 0.5) * 1e3:.2f} ms/query, identical to chunked={whole_same}")
+    # the executor: the cycle from both start types in one device batch
+    eng.configure(max_queries=16, max_patterns=8, arena_bytes=64 << 30)
+    bx = eng.run_batched([full, sample])
+    tb = []
+    for _ in range(3):
+        q0 = time.perf_counter()
+        bx = eng.run_batched([full, sample])
+        tb.append(time.perf_counter() - q0)
+    rs = eng.run_query(sample)
+    exec_same = _same([r, rs], bx)
+    log(f"executor batch [full, sample]: {pct(tb, 0.5) * 1e3:.2f} ms, identical to plain={exec_same}")
+    eng.close()
+    dev.close()
+    parity = {"executor_equals_plain": exec_same, "whole_equals_chunked": whole_same}
+    cpu = None
+    if not args.no_cpu:
+        want, wex, sec = oracle.run_query(w.gstore, full, threads=ORACLE_THREADS)
+        parity["full_rows_equal_cpu"] = _eq(r, want, wex)
+        parity["full_result_rows"] = int(want.shape[0])
+        log(f"oracle, full cycle on {ORACLE_THREADS} threads: {want.shape[0]} rows in {sec:.1f} s, "
+            f"equal={parity['full_rows_equal_cpu']}")
+        del want
+        sw, swex, ssec = oracle.run_query(w.gstore, sample)
+        parity["sample_rows_equal_cpu"] = _eq(rs, sw, swex)
+        parity["sample_result_rows"] = int(sw.shape[0])
+        log(f"oracle, sample cycle on one thread: {sw.shape[0]} rows, {swex} expanded edges in {ssec:.1f} s, "
+            f"equal={parity['sample_rows_equal_cpu']}")
+        cpu = {"value": swex / ssec, "unit": "expanded edges/s", "cores": 1, "kind": "port",
+               "sample": f"the same cycle from the {C5_SAMPLE}-member type t1 ({swex} expanded edges) once on one "
+                         f"thread, {ssec:.1f} s", **_cpu_info()}
+    w.gstore.close()
+    return _line(value, "expanded edges/s", args.steps, args.warmup, elapsed / args.steps * 1e3, config,
+                 {"roofline": roof, "cpu_baseline": cpu,
+                  "whole_tables_ms": pct(tw, 0.5) * 1e3,
+                  "batched": {"queries": 2, "ms": pct(tb, 0.5) * 1e3, "identical_to_plain": exec_same,
+                              "what": "the full cycle and the sample cycle in one device-resident executor batch"},
+                  "parity": parity})
+
+
+# ---- configs 2 and 3: the heavy query and the 512-query mix on the 100M-triple graph ------------------
+
+def run_config23(args, nested=False):
+    """The heavy 6-pattern query on the 100M-triple graph (plain path, timed
+    uninstrumented, then with HIP events per launch; through the executor),
+    its fan-out table against the oracle, the random-key probe roofline, the
+    512-query batched mix three ways, the served sweep, the closed-loop
+    batch mode and the fork-join split. The profiler passes (--heavy-only,
+    --probe-only, --mix-only, --batch-only, --served-heavy-only) stop early."""
+    import numpy as np
+    import oracle
+    from wukonggpu_amd import Device, GPUEngine, workloads
+    t0 = time.time()
+    w = workloads.config2_3(seed=args.seed)
+    log(f"config 2/3 graph: {w.info['triples']} triples, {w.info['keys']} keys, image "
+        f"{w.info['image_bytes'] / 1e9:.2f} GB built in {time.time() - t0:.1f}s")
+    profiler_pass = args.heavy_only or args.probe_only or args.mix_only or args.batch_only
+    device_build = None
+    if not profiler_pass and not nested:
+        device_build = compare_device_build(w, Device)
+    w.triples.close()
+    w.triples = None
+    dev = Device(0)
+    dev.load(w.gstore)
+    eng = GPUEngine(dev)
+    P = w.info["predicates"]
+    type_ids = list(range(2 + P + 1, 2 + P + w.info["types"]))
+
+    def done(obj):
+        eng.close()
+        dev.close()
+        return obj
+
+    if args.probe_only:
+        return done({"probe_random_keys": random_key_probe(dev, w, args.seed, HBM_PEAK_GBS * 1e9 / 128)})
     if args.served_heavy_only:
-        P = w.info["predicates"]
-        type_ids = list(range(2 + P + 1, 2 + P + w.info["types"]))
         sq, sflags = workloads.served_mix(P, type_ids, w.info["t0"], args.served_n, args.seed + 11)
         hq = [q for q, f in zip(sq, sflags) if f]
         out = {"workload": f"the {len(hq)} heavy queries of the {len(sq)}-query served mix, each alone"}
@@ -138,28 +376,18 @@ def main():
             eng.run_plain(hq)  # warm-up
             res = eng.run_plain(hq)
             ms = [r.latency_s * 1e3 for r in res]
-            rows_ex = [(r.table.shape[0], int(r.expanded)) for r in res]
             if ref is None:
                 ref = [r.table for r in res]
             same = all(np.array_equal(a.table, b) for a, b in zip(res, ref))
             out[name] = {"total_ms": sum(ms), "max_ms": max(ms), "p50_ms": sorted(ms)[len(ms) // 2],
-                         "expanded": sum(e for _, e in rows_ex), "identical": same}
+                         "expanded": sum(int(r.expanded) for r in res), "identical": same}
             log(f"{name} (from {rows} rows): {len(hq)} heavy queries alone in {sum(ms):.1f} ms "
                 f"(max {max(ms):.2f} ms), identical={same}")
         dev.set_group_filter_rows(1 << 22)
-        print(json.dumps({"served_heavy": out}), flush=True)
-        eng.close()
-        dev.close()
-        return
+        return done({"served_heavy": out})
     if args.batch_only:
-        P = w.info["predicates"]
-        types = list(range(2 + P + 1, 2 + P + w.info["types"]))
         eng.configure(max_queries=1024, max_patterns=8, arena_bytes=16 << 30)
-        bm = batch_mode(eng, w.gstore, P, types, w.info["t0"], args)
-        print(json.dumps({"batch_mode": bm}), flush=True)
-        eng.close()
-        dev.close()
-        return
+        return done({"batch_mode": batch_mode(eng, w.gstore, P, type_ids, w.info["t0"], args)})
     if args.mix_only:
         mix = w.queries["batched"]
         eng.configure(max_queries=1024, max_patterns=8, arena_bytes=32 << 30)
@@ -175,15 +403,12 @@ def main():
             _, tb = eng.run_batched_packed(mix)
             ms.append(tb * 1e3)
         log(f"5 batches: {', '.join(f'{x:.3f}' for x in ms)} ms")
-        print(json.dumps({"mix_batch_ms": ms, "tuning": tuning}), flush=True)
-        eng.close()
-        dev.close()
-        return
+        return done({"mix_batch_ms": ms, "tuning": tuning})
+
     copy_gbs = dev.copy_bandwidth(1 << 30, 20)
     lines_ps = dev.random_access_rate(4 << 30, 10)
     log(f"copy kernel bandwidth: {copy_gbs:.0f} GB/s; random 128-byte line reads: {lines_ps / 1e9:.2f} G/s "
         f"({lines_ps * 128 / 1e9:.0f} GB/s)")
-
     heavy = w.queries["heavy"]
     for _ in range(args.warmup):
         eng.run_query(heavy)
@@ -201,70 +426,42 @@ def main():
     for _ in range(args.steps):
         eng.run_query(heavy)
     prof_ms = (time.perf_counter() - t_prof) / args.steps * 1e3
-    kernels = {k: dev.profile_read(k) for k in ("probe", "probe_filter", "scan_tiles", "finalize", "compact",
-                                                "partition", "expand", "probe_one", "copy_list")}
+    kernels = _kernel_table(dev, STEP_KERNELS, args.steps)
     probe_keys = dev.profile_read("probe_keys")[2]  # counts kept in the bytes field
     probe_rows = dev.profile_read("probe_rows")[2]
+    ms_pr = dev.profile_read("probe")[1]
+    alg_pr = dev.profile_read("probe")[2]
+    roof = _expand_roofline(dev, args.steps, "expand_heavy_query.traffic.json", {"copy_kernel_gbs": copy_gbs})
     dev.profile(False)
     edges = r.expanded
     value = edges * args.steps / elapsed
     log(f"heavy query: {r.table.shape[0]} result rows, {edges} expanded edges, "
-        f"{elapsed / args.steps * 1e3:.2f} ms/query, {value / 1e9:.2f} G expanded edges/s")
-
-    n_exp, ms_exp, b_exp = kernels["expand"]
-    achieved = b_exp / (ms_exp * 1e-3) / 1e9 if ms_exp > 0 else 0.0
-    # HBM bytes per expand launch from the committed rocprofv3 FETCH_SIZE /
-    # WRITE_SIZE summary of this workload (scaled by a copy kernel of known size)
-    traffic, traffic_src = None, None
-    tp = os.path.join(ROOT, "profiles", "expand_heavy_query.traffic.json")
-    if os.path.exists(tp):
-        with open(tp) as f:
-            t = json.load(f)
-        traffic, traffic_src = t["hbm_bytes_per_launch"], t["source"]
-    gpu_kernel_ms = sum(v[1] for v in kernels.values()) / args.steps
-    # The probe against the random-access roofline: HBM peak / 128 B = 62.5 G
-    # bucket lines/s (every key probed reads at least one random 128-byte
-    # bucket). Next to it: the measured random-line rate over 4 GiB (HBM) and,
-    # for the random-key probe, over a buffer the size of the probed
-    # predicate's bucket segment (its working set, largely cache-resident).
+        f"{elapsed / args.steps * 1e3:.2f} ms/query, {value / 1e9:.2f} G expanded edges/s; k_expand "
+        f"{roof['achieved']:.0f} GB/s = {roof['frac']:.2f} of peak")
     line_peak = HBM_PEAK_GBS * 1e9 / 128
-    ms_pr = kernels["probe"][1]
-    probe_rate = probe_keys / (ms_pr * 1e-3) if ms_pr > 0 else 0.0
     probe_roofline = {
         "bound": "random-access", "peak": line_peak, "unit": "128-byte bucket lines/s",
-        "kernel": "probe (k_probe, known_to_unknown steps)",
-        "measured_random_lines_4GiB": lines_ps,
+        "kernel": "probe (k_probe, known_to_unknown steps)", "measured_random_lines_4GiB": lines_ps,
         "heavy_query": {
             "keys_probed_per_query": probe_keys / args.steps, "rows_per_query": probe_rows / args.steps,
-            "keys_per_s": probe_rate, "rows_per_s": probe_rows / (ms_pr * 1e-3) if ms_pr > 0 else 0.0,
-            "alg_GBps": kernels["probe"][2] / (ms_pr * 1e-3) / 1e9 if ms_pr > 0 else 0.0,
+            "keys_per_s": probe_keys / (ms_pr * 1e-3) if ms_pr > 0 else 0.0,
+            "rows_per_s": probe_rows / (ms_pr * 1e-3) if ms_pr > 0 else 0.0,
+            "alg_GBps": alg_pr / (ms_pr * 1e-3) / 1e9 if ms_pr > 0 else 0.0,
             "note": "runs of equal keys share one bucket walk, so this probe is bound by reading its "
                     "input table (4 R c bytes), not by bucket lines; alg_GBps counts 4 R c + 128 per key "
                     "probed + 16 per row with edges",
         },
     }
-    log(f"probe: {probe_rows / args.steps / 1e6:.1f} M rows and {probe_keys / args.steps / 1e6:.2f} M keys probed "
-        f"per query in {ms_pr / args.steps:.3f} ms ({probe_roofline['heavy_query']['alg_GBps']:.0f} GB/s "
-        f"algorithmic)")
-    # the probe on random keys (no runs to share)
-    if not args.heavy_only:
-        rk = random_key_probe(dev, w, args.seed, line_peak)
-        rk["frac_of_random_lines_4GiB"] = rk["achieved"] / lines_ps
-        probe_roofline.update({"achieved": rk["achieved"], "frac": rk["frac_of_peak"]})
-        probe_roofline["random_keys"] = rk
-        log(f"probe on random keys: {rk['keys_probed'] / 1e6:.1f} M keys in {rk['ms_per_launch']:.2f} ms = "
-            f"{rk['achieved'] / 1e9:.2f} G keys/s = {rk['frac_of_peak']:.2f} of HBM peak / 128 B; random lines "
-            f"over 4 GiB {lines_ps / 1e9:.1f} G/s, over the {rk['segment_bytes'] / 2**20:.0f} MiB segment "
-            f"{rk['random_lines_same_working_set'] / 1e9:.1f} G/s "
-            f"({rk['frac_of_random_lines_same_working_set']:.2f})")
-
     if args.heavy_only:
-        print(json.dumps({"metric": METRIC, "value": value, "unit": "expanded edges/s", "n_gpus": 1,
-                          "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
-                          "expand_GBps": achieved, "copy_kernel_gbs": copy_gbs}), flush=True)
-        eng.close()
-        dev.close()
-        return
+        return done(_line(value, "expanded edges/s", args.steps, args.warmup, elapsed / args.steps * 1e3,
+                          {"workload": "config-2 heavy query"}, {"roofline": roof}))
+    rk = random_key_probe(dev, w, args.seed, line_peak)
+    rk["frac_of_random_lines_4GiB"] = rk["achieved"] / lines_ps
+    probe_roofline.update({"achieved": rk["achieved"], "frac": rk["frac_of_peak"], "random_keys": rk})
+    log(f"probe on random keys: {rk['keys_probed'] / 1e6:.1f} M keys in {rk['ms_per_launch']:.2f} ms = "
+        f"{rk['achieved'] / 1e9:.2f} G keys/s = {rk['frac_of_peak']:.2f} of HBM peak / 128 B; random lines "
+        f"over 4 GiB {lines_ps / 1e9:.1f} G/s, over the {rk['segment_bytes'] / 2**20:.0f} MiB segment "
+        f"{rk['random_lines_same_working_set'] / 1e9:.1f} G/s ({rk['frac_of_random_lines_same_working_set']:.2f})")
 
     # ---- the heavy query through the device-resident executor ---------------------------
     eng.configure(max_queries=1024, max_patterns=8, arena_bytes=32 << 30)
@@ -279,7 +476,7 @@ def main():
         hx = eng_h.run_batched([heavy])[0]
         hx_lat.append(time.perf_counter() - q0)
     eng_h.close()
-    heavy_exec_same = bool(hx.status == 0 and np.array_equal(hx.table, r.table) and hx.expanded == r.expanded)
+    heavy_exec_same = _same([hx], [r])
     log(f"heavy query via the executor (shape {heavy_tuning}): {pct(hx_lat, 0.5) * 1e3:.2f} ms "
         f"(identical={heavy_exec_same})")
 
@@ -304,139 +501,268 @@ def main():
         t_exec.append(tb)
     t_batched = sorted(t_exec)[len(t_exec) // 2]
     batched_1 = eng.run_batched(mix)  # the per-query-table form gives the same answers
-
-    def same(xs, ys):
-        return all(a.status == 0 and b.status == 0 and a.table.shape == b.table.shape and
-                   np.array_equal(a.table, b.table) and a.expanded == b.expanded for a, b in zip(xs, ys))
-    same_exec, same_rounds = same(plain, batched) and same(plain, batched_1), same(plain, rounds)
+    same_exec, same_rounds = _same(plain, batched) and _same(plain, batched_1), _same(plain, rounds)
     mix_edges = sum(x.expanded for x in batched)
     log(f"batched mix: plain {len(mix) / t_plain:.0f} q/s, step rounds {len(mix) / t_rounds:.0f} q/s, "
         f"executor {len(mix) / t_batched:.0f} q/s (x{t_plain / t_batched:.1f}); identical={same_exec and same_rounds}")
 
     # ---- served: open-loop Poisson arrivals, light + heavy lanes ----------------------------
-    served = []
-    served_same = None
+    served, served_same = [], None
     if args.served_n > 0:
-        P = w.info["predicates"]
-        type_ids = list(range(2 + P + 1, 2 + P + w.info["types"]))
         sq, sflags = workloads.served_mix(P, type_ids, w.info["t0"], args.served_n, args.seed + 11)
         alone = eng.run_plain(sq)
-        hx = [int(a.expanded) for a, f in zip(alone, sflags) if f]
-        hl = [a.latency_s for a, f in zip(alone, sflags) if f]
-        log(f"served mix: {len(sq)} queries, {len(hx)} heavy expanding {min(hx, default=0)}..{max(hx, default=0)} "
-            f"edges, {min(hl, default=0) * 1e3:.2f}..{max(hl, default=0) * 1e3:.2f} ms alone")
-        served_same = True
-        rng = np.random.default_rng(args.seed + 12)
-        for rate in [float(x) for x in args.served_rates.split(",") if x]:
-            arr = np.cumsum(rng.exponential(1.0 / rate, size=len(sq)))
-            pool0 = dev.pool_stats()
-            res, wall = eng.serve(sq, arr, heavy=sflags, table_budget_bytes=64 << 30, light_lanes=2)
-            pool1 = dev.pool_stats()
-            ok = same(alone, res)
-            served_same = served_same and ok
-            lat_l = [x.latency_s for x, f in zip(res, sflags) if not f]
-            lat_h = [x.latency_s for x, f in zip(res, sflags) if f]
-            rec = {"offered_qps": rate, "queries": len(sq), "heavy": int(sflags.sum()),
-                   "achieved_qps": len(sq) / wall, "p50_ms": pct([x.latency_s for x in res], 0.5) * 1e3,
-                   "p99_ms": pct([x.latency_s for x in res], 0.99) * 1e3,
-                   "light_p50_ms": pct(lat_l, 0.5) * 1e3, "light_p99_ms": pct(lat_l, 0.99) * 1e3,
-                   "heavy_p50_ms": pct(lat_h, 0.5) * 1e3, "heavy_p99_ms": pct(lat_h, 0.99) * 1e3,
-                   "identical_to_alone": bool(ok), "serve_stats": eng.serve_stats(),
-                   "pool": {"mallocs": pool1["mallocs"] - pool0["mallocs"], "trims": pool1["trims"] - pool0["trims"],
-                            "live_bytes": pool1["live_bytes"], "cached_bytes": pool1["cached_bytes"]}}
-            served.append(rec)
-            log(f"served @ {rate:.0f} q/s offered: {rec['achieved_qps']:.0f} q/s, p50 {rec['p50_ms']:.2f} ms, "
-                f"p99 {rec['p99_ms']:.2f} ms (light p99 {rec['light_p99_ms']:.2f}, heavy p99 "
-                f"{rec['heavy_p99_ms']:.2f}), identical={ok}; {rec['serve_stats']}, pool {rec['pool']}")
+        served, served_same = served_sweep(eng, dev, sq, sflags, alone, args.served_rates, args.seed + 12,
+                                           64 << 30)
 
     # ---- closed-loop batch mode: templates with random constants, in-flight caps -------------
     bm = None
-    if args.batch_s > 0:
-        P = w.info["predicates"]
-        bm = batch_mode(eng, w.gstore, P, list(range(2 + P + 1, 2 + P + w.info["types"])), w.info["t0"], args)
+    if args.batch_s > 0 and (not nested or time_left(args) > 120):
+        bm = batch_mode(eng, w.gstore, P, type_ids, w.info["t0"], args)
 
     # ---- fork-join table exchange: the split a sharded deployment adds per step ---------------
     exchange = table_exchange(dev, args.seed)
+    # the 151M-row hub fan-out table (the heavy query's first four patterns), for the oracle below
+    rp = eng.run_query(heavy[:16]) if not args.no_cpu else None
+    eng.close()
+    dev.close()
 
-    # ---- CPU baseline (the oracle), parity of the heavy result --------------------------
+    # ---- the oracle: parity of every answer, the CPU baseline -----------------------------------
     cpu = None
-    parity = {"heavy_rows_equal": None, "heavy_executor_equals_plain": heavy_exec_same,
+    parity = {"heavy_executor_equals_plain": heavy_exec_same,
               "mix_plain_equals_executor": bool(same_exec), "mix_plain_equals_rounds": bool(same_rounds),
-              "served_equals_alone": served_same,
-              "batch_mode_equals_alone": bm["equals_alone"] if bm else None,
-              "table_split_correct": all(r["correct"] for r in exchange)}
+              "served_equals_alone": served_same, "batch_mode_equals_alone": bm["equals_alone"] if bm else None,
+              "table_split_correct": all(x["correct"] for x in exchange)}
     if not args.no_cpu:
-        import oracle
         want, wex, sec = oracle.run_query(w.gstore, heavy)
-        parity["heavy_rows_equal"] = bool(want.shape == r.table.shape and np.array_equal(want, r.table)
-                                          and wex == r.expanded)
+        parity["heavy_rows_equal_cpu"] = _eq(r, want, wex)
+        parity["heavy_result_rows"] = int(want.shape[0])
         cpu = {"value": wex / sec, "unit": "expanded edges/s", "cores": 1, "kind": "port",
-               "sample": f"the full heavy query once on one thread ({wex} expanded edges in {sec:.1f} s)"}
-        log(f"cpu oracle: {wex / sec / 1e6:.1f} M expanded edges/s, parity={parity['heavy_rows_equal']}")
-        threads = os.cpu_count() or 1
-        lat_c, rows_c, ex_c, wall_c = oracle.run_many(w.gstore, mix, threads=threads)
-        parity["mix_rows_equal_cpu"] = bool(all(int(rows_c[i]) == batched[i].table.shape[0] and
-                                                int(ex_c[i]) == batched[i].expanded for i in range(len(mix))))
+               "sample": f"the full heavy query once on one thread ({wex} expanded edges in {sec:.1f} s)",
+               **_cpu_info()}
+        log(f"cpu oracle: {wex / sec / 1e6:.1f} M expanded edges/s, heavy rows {want.shape[0]}, "
+            f"parity={parity['heavy_rows_equal_cpu']}")
+        ok, n, wall = _oracle_digests(w.gstore, mix, batched)
+        parity["mix_tables_equal_cpu"] = ok
+        parity["mix_tables_checked"] = n
+        parity["mix_nonempty_answers"] = int(sum(1 for x in batched if x.table.shape[0]))
+        log(f"512-query mix: every executor table against the oracle: {ok}")
+        # the hub fan-out table, row for row
+        pw, pex, psec = oracle.run_query(w.gstore, heavy[:16], threads=ORACLE_THREADS)
+        parity["fanout_rows_equal_cpu"] = _eq(rp, pw, pex)
+        parity["fanout_rows"] = int(pw.shape[0])
+        log(f"fan-out table ({pw.shape[0]} rows x {pw.shape[1]}): equal to the oracle={parity['fanout_rows_equal_cpu']}")
+        del pw, rp
+        threads = ORACLE_THREADS
+        _, _, _, wall_c = oracle.run_many(w.gstore, mix, threads=threads)
         cpu["mix_queries_per_s"] = len(mix) / wall_c
         cpu["mix_threads"] = threads
-
-    line = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "expanded edges/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u32",
-        "data": "synthetic (seeded generator; graph resident in HBM)",
-        "config": {
-            "workload": "heavy 6-pattern query (index start, 2-hop chain, 2-arm star, closing known_to_known) "
-                        "on a 100M-triple graph: 10M entities, 64 predicates, Zipf s=1.0, 500k-member type",
-            "triples": w.info["triples"], "keys": w.info["keys"], "edges": w.info["edges"],
-            "query": heavy, "result_rows": int(r.table.shape[0]), "expanded_edges_per_query": int(edges),
-            "p50_ms": pct(lat, 0.5) * 1e3, "p99_ms": pct(lat, 0.99) * 1e3,
-            "gpu_kernel_ms_per_query": gpu_kernel_ms,
-            "ms_per_query_with_kernel_events": prof_ms,
-            "kernels_per_query": {k: {"launches": v[0] / args.steps, "ms": v[1] / args.steps,
-                                      "alg_GBps": (v[2] / (v[1] * 1e-3) / 1e9) if v[1] > 0 else 0.0}
-                                  for k, v in kernels.items() if v[0]},
-        },
-        "roofline": {
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": "expand (k_expand)", "launches_per_query": n_exp / args.steps,
-            "alg_bytes_per_launch": b_exp / max(n_exp, 1), "avg_launch_ms": ms_exp / max(n_exp, 1),
-            "traffic_source": traffic_src, "copy_kernel_gbs": copy_gbs,
-        },
-        "probe_roofline": probe_roofline,
-        "device_build": device_build,
-        "cpu_baseline": cpu,
+    w.gstore.close()
+    return _line(value, "expanded edges/s", args.steps, args.warmup, elapsed / args.steps * 1e3, {
+        "workload": "heavy 6-pattern query (index start, 2-hop chain, 2-arm star over a linked predicate pair, "
+                    "closing known_to_known) on a 100M-triple graph: 10M entities, 64 predicates (linked in "
+                    "pairs), Zipf s=1.0, 500k-member type",
+        "triples": w.info["triples"], "keys": w.info["keys"], "edges": w.info["edges"],
+        "query": heavy, "result_rows": int(r.table.shape[0]), "expanded_edges_per_query": int(edges),
+        "p50_ms": pct(lat, 0.5) * 1e3, "p99_ms": pct(lat, 0.99) * 1e3,
+        "gpu_kernel_ms_per_query": sum(v["ms"] for v in kernels.values()),
+        "ms_per_query_with_kernel_events": prof_ms, "kernels_per_query": kernels}, {
+        "roofline": roof, "probe_roofline": probe_roofline, "device_build": device_build, "cpu_baseline": cpu,
         "batched": {
             "workload": f"{len(mix)} selective 3-4 pattern queries from type indexes (sizes log-uniform 1..1e5)",
-            "plain_queries_per_s": len(mix) / t_plain,
-            "rounds_queries_per_s": len(mix) / t_rounds,
-            "batched_queries_per_s": len(mix) / t_batched,
-            "speedup": t_plain / t_batched,
-            "executor": "device-resident: whole queries in one captured hipGraph, persistent tile kernels (work queue or static deal, autotuned)",
+            "plain_queries_per_s": len(mix) / t_plain, "rounds_queries_per_s": len(mix) / t_rounds,
+            "batched_queries_per_s": len(mix) / t_batched, "speedup": t_plain / t_batched,
+            "executor": "device-resident: whole queries in one captured hipGraph, persistent tile kernels "
+                        "(work queue or static deal, autotuned)",
             "tuning": tuning, "autotune_ms_per_batch": tune_ms, "autotune_s": t_tune,
             "plain_p50_ms": pct([x.latency_s for x in plain], 0.5) * 1e3,
             "plain_p99_ms": pct([x.latency_s for x in plain], 0.99) * 1e3,
-            "batched_ms_per_batch": t_batched * 1e3,
-            "expanded_edges": int(mix_edges),
-        },
+            "batched_ms_per_batch": t_batched * 1e3, "expanded_edges": int(mix_edges)},
         "heavy_via_executor_ms": pct(hx_lat, 0.5) * 1e3, "heavy_tuning": heavy_tuning,
-        "served": served,
-        "batch_mode": bm,
-        "table_exchange": exchange,
-        "parity": parity,
-    }
-    print(json.dumps(line), flush=True)
+        "served": served, "batch_mode": bm, "table_exchange": exchange, "parity": parity})
+
+
+def served_sweep(eng, dev, sq, sflags, alone, rates, seed, budget):
+    """Open-loop Poisson arrivals at each offered rate: achieved queries/s,
+    p50/p99 overall and per class, each answer against the query run alone."""
+    import numpy as np
+    hx = [int(a.expanded) for a, f in zip(alone, sflags) if f]
+    hl = [a.latency_s for a, f in zip(alone, sflags) if f]
+    log(f"served mix: {len(sq)} queries, {len(hx)} heavy expanding {min(hx, default=0)}..{max(hx, default=0)} "
+        f"edges, {min(hl, default=0) * 1e3:.2f}..{max(hl, default=0) * 1e3:.2f} ms alone")
+    out, all_same = [], True
+    rng = np.random.default_rng(seed)
+    for rate in [float(x) for x in rates.split(",") if x]:
+        arr = np.cumsum(rng.exponential(1.0 / rate, size=len(sq)))
+        pool0 = dev.pool_stats()
+        res, wall = eng.serve(sq, arr, heavy=sflags, table_budget_bytes=budget, light_lanes=2)
+        pool1 = dev.pool_stats()
+        ok = _same(alone, res)
+        all_same = all_same and ok
+        lat_l = [x.latency_s for x, f in zip(res, sflags) if not f]
+        lat_h = [x.latency_s for x, f in zip(res, sflags) if f]
+        rec = {"offered_qps": rate, "queries": len(sq), "heavy": int(sflags.sum()),
+               "achieved_qps": len(sq) / wall, "p50_ms": pct([x.latency_s for x in res], 0.5) * 1e3,
+               "p99_ms": pct([x.latency_s for x in res], 0.99) * 1e3,
+               "light_p50_ms": pct(lat_l, 0.5) * 1e3, "light_p99_ms": pct(lat_l, 0.99) * 1e3,
+               "heavy_p50_ms": pct(lat_h, 0.5) * 1e3, "heavy_p99_ms": pct(lat_h, 0.99) * 1e3,
+               "identical_to_alone": bool(ok), "serve_stats": eng.serve_stats(),
+               "pool": {"mallocs": pool1["mallocs"] - pool0["mallocs"], "trims": pool1["trims"] - pool0["trims"]}}
+        out.append(rec)
+        log(f"served @ {rate:.0f} q/s offered: {rec['achieved_qps']:.0f} q/s, p50 {rec['p50_ms']:.2f} ms, "
+            f"p99 {rec['p99_ms']:.2f} ms (light p99 {rec['light_p99_ms']:.2f}, heavy p99 "
+            f"{rec['heavy_p99_ms']:.2f}), identical={ok}")
+    return out, all_same
+
+
+# ---- config 4: served on the 1B-triple graph -------------------------------------------------------
+
+def run_config4(args, nested=False):
+    """Served: open-loop Poisson arrivals over the 1B-triple graph, 95% light
+    (2-3 patterns from small type indexes) and 5% heavy (the 6-pattern heavy
+    shape from a 500k-member type, >= 10M-row tables), 64 GB table budget;
+    every served answer against the query alone, all of them against the
+    oracle's digests; the batch mode; the heavy queries' expand roofline."""
+    import numpy as np
+    import oracle
+    from wukonggpu_amd import Device, GPUEngine, workloads
+    t0 = time.time()
+    w = workloads.config4(seed=4, keep_triples=not nested)
+    log(f"config 4 graph: {w.info['triples']} triples, {w.info['keys']} keys, image "
+        f"{w.info['image_bytes'] / 1e9:.2f} GB built in {time.time() - t0:.1f}s")
+    device_build = None
+    if not nested:
+        device_build = compare_device_build(w, Device, host_gs=w.gstore, host_s=w.info["host_build_s"])
+        w.triples.close()
+        w.triples = None
+    dev = Device(0)
+    dev.load(w.gstore)
+    eng = GPUEngine(dev)
+    P = w.info["predicates"]
+    sq, flags = workloads.served_mix(P, w.info["light_types"], w.info["t0"], args.served_n, args.seed + 41)
+    heavy_idx = [i for i in range(len(sq)) if flags[i]]
+    eng.configure(max_queries=1024, max_patterns=8, arena_bytes=16 << 30)
+    light_idx = [i for i in range(len(sq)) if not flags[i]][:512]
+    light = [sq[i] for i in light_idx]
+    tuning, tune_ms = eng.autotune(light, reps=2)
+    htuning, _ = eng.autotune([sq[i] for i in heavy_idx[:2]], reps=1, heavy=True)
+    log(f"autotuned: light {tuning}, heavy {htuning}")
+    tp = time.perf_counter()
+    alone = eng.run_plain(sq)
+    t_plain = time.perf_counter() - tp
+    # the heavy queries' expand on the plain path, with HIP events
+    hq = [sq[i] for i in heavy_idx]
+    dev.profile(True)
+    hres = eng.run_plain(hq)
+    kernels = _kernel_table(dev, STEP_KERNELS, max(len(hq), 1))
+    roof = _expand_roofline(dev, len(hq), "expand_served_1b.traffic.json",
+                            {"workload": f"the {len(hq)} heavy queries of the served mix, each alone on the plain path"})
+    dev.profile(False)
+    hrows = [int(alone[i].expanded) for i in heavy_idx]
+    log(f"{len(sq)} queries alone on the plain path: {len(sq) / t_plain:.0f} q/s; heavy queries expand "
+        f"{min(hrows) if hrows else 0}..{max(hrows) if hrows else 0} edges; k_expand {roof['achieved']:.0f} GB/s "
+        f"({roof['frac']:.2f} of peak)")
+    tb = time.perf_counter()
+    light_b = eng.run_batched(light)
+    t_light = time.perf_counter() - tb
+    served, all_same = served_sweep(eng, dev, sq, flags, alone, args.served_rates, args.seed + 42, 64 << 30)
+    bm = None
+    if args.batch_s > 0 and (not nested or time_left(args) > 200):
+        bm = batch_mode(eng, w.gstore, P, w.info["light_types"], w.info["t0"], args)
     eng.close()
     dev.close()
+    parity = {"served_equals_alone": all_same, "batch_mode_equals_alone": bm["equals_alone"] if bm else None,
+              "light_batched_equals_plain": _same([alone[i] for i in light_idx], light_b),
+              "heavy_profiled_equals_alone": _same(hres, [alone[i] for i in heavy_idx])}
+    cpu = None
+    if not args.no_cpu:
+        ok, n, wall = _oracle_digests(w.gstore, sq, alone)
+        parity["all_served_queries_equal_cpu"] = ok
+        parity["queries_checked"] = n
+        parity["heavy_nonempty_answers"] = int(sum(1 for i in heavy_idx if alone[i].table.shape[0]))
+        log(f"oracle digests of all {n} served queries ({ORACLE_THREADS} threads, {wall:.1f} s): {ok}")
+        samp = light[:256] + [sq[i] for i in heavy_idx[:1]]
+        t = time.perf_counter()
+        for q in samp:
+            oracle.run_query(w.gstore, q)
+        cpu_s = time.perf_counter() - t
+        cpu = {"value": len(samp) / cpu_s, "unit": "queries/s", "cores": 1, "kind": "port",
+               "sample": f"{len(samp)} of the served queries (256 light, 1 heavy) on one thread, {cpu_s:.1f} s",
+               **_cpu_info()}
+    best = max(served, key=lambda x: x["achieved_qps"]) if served else None
+    w.gstore.close()
+    return _line(best["achieved_qps"] if best else 0.0, "queries/s", len(sq), 0,
+                 1e3 / best["achieved_qps"] if best else 0.0,
+                 {"workload": "served open-loop Poisson mix (95% light 2-3 patterns, 5% heavy 6 patterns) on a "
+                              "1B-triple graph: 100M entities, 256 predicates (linked in pairs), Zipf 1.0; 64 GB "
+                              "table budget",
+                  **{k: v for k, v in w.info.items() if k != "light_types"},
+                  "queries_per_rate": len(sq), "heavy_queries": len(heavy_idx)},
+                 {"roofline": roof, "kernels_per_heavy_query": kernels, "served": served, "batch_mode": bm,
+                  "plain_queries_per_s": len(sq) / t_plain, "device_build": device_build,
+                  "light_batched_queries_per_s": len(light) / t_light, "tuning": tuning, "heavy_tuning": htuning,
+                  "parity": parity, "cpu_baseline": cpu})
+
+
+# ---- config 1: the 2-hop query from a constant on the 10M-triple graph -----------------------------------
+
+def run_config1(args, nested=False):
+    """Plain single query on the 10M-triple graph: the 2-hop chain from the
+    constant of p1-degree ~1000, on the GPU (plain path and executor) and on
+    the CPU oracle."""
+    import oracle
+    from wukonggpu_amd import Device, GPUEngine, workloads
+    w = workloads.config1(seed=1)
+    q = w.queries[0]
+    dev = Device(0)
+    dev.load(w.gstore)
+    eng = GPUEngine(dev)
+    steps = max(args.steps, 200)
+    for _ in range(max(args.warmup, 10)):
+        r = eng.run_query(q)
+    lat = []
+    for _ in range(steps):
+        t = time.perf_counter()
+        r = eng.run_query(q)
+        lat.append(time.perf_counter() - t)
+    el = sum(lat)
+    dev.profile(True)
+    for _ in range(20):
+        eng.run_query(q)
+    kernels = _kernel_table(dev, STEP_KERNELS, 20)
+    dev.profile(False)
+    # the dominant kernel by time, against the HBM roofline (this query is launch-bound)
+    dom = max(kernels.items(), key=lambda kv: kv[1]["ms"]) if kernels else (None, None)
+    roof = None
+    if dom[0]:
+        roof = {"bound": "hbm", "achieved": dom[1]["alg_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": dom[1]["alg_GBps"] / HBM_PEAK_GBS, "traffic": None, "kernel": dom[0],
+                "launches_per_query": dom[1]["launches"], "avg_launch_ms": dom[1]["ms"] / max(dom[1]["launches"], 1),
+                "note": "a 1.2k-edge query: every launch is a few workgroups, so the GPU time is launch latency"}
+    eng.configure(max_queries=64, max_patterns=4, arena_bytes=1 << 30)
+    bx = eng.run_batched([q])[0]
+    want, wex, sec = oracle.run_query(w.gstore, q)
+    reps = max(1, int(2.0 / max(sec, 1e-4)))
+    t = time.perf_counter()
+    for _ in range(reps):
+        oracle.run_query(w.gstore, q)
+    cpu_s = (time.perf_counter() - t) / reps
+    same = _eq(r, want, wex)
+    log(f"config 1: {r.table.shape[0]} rows, {r.expanded} expanded edges, GPU p50 {pct(lat, 0.5) * 1e3:.3f} ms, "
+        f"CPU {cpu_s * 1e3:.3f} ms/query, identical={same}")
+    eng.close()
+    dev.close()
+    w.gstore.close()
+    return _line(r.expanded * steps / el, "expanded edges/s", steps, max(args.warmup, 10), el / steps * 1e3,
+                 {"workload": "plain 2-hop chain from one constant (p1 degree ~1k) on a 10M-triple graph: 1M "
+                              "entities, 16 predicates, Zipf 1.1 capped at 10k",
+                  **{k: v for k, v in w.info.items()}, "query": q, "result_rows": int(r.table.shape[0]),
+                  "p50_ms": pct(lat, 0.5) * 1e3, "p99_ms": pct(lat, 0.99) * 1e3, "queries_per_s": steps / el,
+                  "kernels_per_query": kernels},
+                 {"roofline": roof,
+                  "cpu_baseline": {"value": wex / cpu_s, "unit": "expanded edges/s", "cores": 1, "kind": "port",
+                                   "sample": f"the full query, {reps} runs on one thread", "p50_ms": cpu_s * 1e3,
+                                   **_cpu_info()},
+                  "parity": {"rows_equal_cpu": same, "executor_equals_cpu": _eq(bx, want, wex)}})
 
 
 # Latency points of the reference logger's per-query-type CDF (core/logger.hpp:250-313).
@@ -511,33 +837,6 @@ def batch_mode(eng, gs, P, light_types, t0, args, budget=64 << 30):
     return {"duration_s": dur, "warmup_s": warm, "runs": out, "equals_alone": all_same,
             "templates": "6 light (constant start from a random type member: 2-hop, 3-arm star, 3-hop chain) and "
                          "4 heavy (index start, 6 patterns), loads of the reference's batch mix_config"}
-
-
-def _same(xs, ys):
-    import numpy as np
-    return all(a.status == 0 and b.status == 0 and a.table.shape == b.table.shape and
-               np.array_equal(a.table, b.table) and a.expanded == b.expanded for a, b in zip(xs, ys))
-
-
-def _cpu_info():
-    model = None
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
-    return {"cpu_model": model, "nproc": os.cpu_count()}
-
-
-def _line(metric, value, unit, steps, warmup, ms, config, extra, higher=True, dtype="u32"):
-    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
-           "ms_per_step": ms, "higher_is_better": higher, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
-           "data": "synthetic (seeded generator; graph resident in HBM)", "config": config}
-    out.update(extra)
-    print(json.dumps(out), flush=True)
 
 
 def compare_device_build(w, Device, main_load=0.75, host_gs=None, host_s=None):
@@ -654,197 +953,6 @@ def table_exchange(dev, seed, rows=1 << 25, cols=4, parts=(8, 64, 1024)):
     return out
 
 
-def run_config1(args):
-    """Plain single query on the 10M-triple graph: the 2-hop chain from the
-    constant of p1-degree ~1000, on the GPU and on the CPU oracle."""
-    import numpy as np
-    import oracle
-    from wukonggpu_amd import Device, GPUEngine, workloads
-    w = workloads.config1(seed=1)
-    q = w.queries[0]
-    dev = Device(0)
-    dev.load(w.gstore)
-    eng = GPUEngine(dev)
-    for _ in range(max(args.warmup, 1)):
-        r = eng.run_query(q)
-    lat = []
-    for _ in range(args.steps):
-        t = time.perf_counter()
-        r = eng.run_query(q)
-        lat.append(time.perf_counter() - t)
-    el = sum(lat)
-    want, wex, sec = oracle.run_query(w.gstore, q)
-    reps = max(1, int(2.0 / max(sec, 1e-4)))
-    t = time.perf_counter()
-    for _ in range(reps):
-        oracle.run_query(w.gstore, q)
-    cpu_s = (time.perf_counter() - t) / reps
-    same = bool(want.shape == r.table.shape and np.array_equal(want, r.table) and wex == r.expanded)
-    log(f"config 1: {r.table.shape[0]} rows, {r.expanded} expanded edges, GPU {el / args.steps * 1e3:.3f} ms/query, "
-        f"CPU {cpu_s * 1e3:.3f} ms/query, identical={same}")
-    _line("expanded edges/s + queries/s; p50/p99 latency, batched vs plain on 1 MI355X",
-          r.expanded * args.steps / el, "expanded edges/s", args.steps, args.warmup, el / args.steps * 1e3,
-          {"workload": "plain 2-hop chain from one constant (p1 degree ~1k) on a 10M-triple graph: 1M entities, "
-                       "16 predicates, Zipf 1.1 capped at 10k", **{k: v for k, v in w.info.items()},
-           "query": q, "result_rows": int(r.table.shape[0]), "p50_ms": pct(lat, 0.5) * 1e3,
-           "p99_ms": pct(lat, 0.99) * 1e3, "queries_per_s": args.steps / el},
-          {"cpu_baseline": {"value": wex / cpu_s, "unit": "expanded edges/s", "cores": 1, "kind": "port",
-                            "sample": f"the full query, {reps} runs on one thread", **_cpu_info()},
-           "parity": {"rows_equal_cpu": same}})
-    eng.close()
-    dev.close()
-
-
-def run_config4(args):
-    """Served: open-loop Poisson arrivals over the 1B-triple graph, 95% light
-    (2-3 patterns from small type indexes) and 5% heavy (the 6-pattern heavy
-    shape from a 500k-member type, >= 10M-row tables), 64 GB table budget."""
-    import numpy as np
-    import oracle
-    from wukonggpu_amd import Device, GPUEngine, workloads
-    t0 = time.time()
-    w = workloads.config4(seed=4, keep_triples=True)
-    log(f"graph: {w.info['triples']} triples, {w.info['keys']} keys, image {w.info['image_bytes'] / 1e9:.2f} GB "
-        f"built in {time.time() - t0:.1f}s")
-    device_build = compare_device_build(w, Device, host_gs=w.gstore, host_s=w.info["host_build_s"])
-    w.triples.close()
-    w.triples = None
-    dev = Device(0)
-    dev.load(w.gstore)
-    eng = GPUEngine(dev)
-    P = w.info["predicates"]
-    sq, flags = workloads.served_mix(P, w.info["light_types"], w.info["t0"], args.served_n, args.seed + 41)
-    heavy_idx = [i for i in range(len(sq)) if flags[i]]
-    eng.configure(max_queries=1024, max_patterns=8, arena_bytes=16 << 30)
-    light_idx = [i for i in range(len(sq)) if not flags[i]][:512]
-    light = [sq[i] for i in light_idx]
-    tuning, tune_ms = eng.autotune(light, reps=2)
-    htuning, _ = eng.autotune([sq[i] for i in heavy_idx[:2]], reps=1, heavy=True)
-    log(f"autotuned: light {tuning}, heavy {htuning}")
-    tp = time.perf_counter()
-    alone = eng.run_plain(sq)
-    t_plain = time.perf_counter() - tp
-    hrows = [int(alone[i].expanded) for i in heavy_idx]
-    log(f"{len(sq)} queries alone on the plain path: {len(sq) / t_plain:.0f} q/s; heavy queries expand "
-        f"{min(hrows) if hrows else 0}..{max(hrows) if hrows else 0} edges")
-    tb = time.perf_counter()
-    light_b = eng.run_batched(light)
-    t_light = time.perf_counter() - tb
-    served, all_same = [], True
-    rng = np.random.default_rng(args.seed + 42)
-    for rate in [float(x) for x in args.served_rates.split(",") if x]:
-        arr = np.cumsum(rng.exponential(1.0 / rate, size=len(sq)))
-        res, wall = eng.serve(sq, arr, heavy=flags, table_budget_bytes=64 << 30, light_lanes=2)
-        ok = _same(alone, res)
-        all_same = all_same and ok
-        lat_l = [x.latency_s for x, f in zip(res, flags) if not f]
-        lat_h = [x.latency_s for x, f in zip(res, flags) if f]
-        rec = {"offered_qps": rate, "achieved_qps": len(sq) / wall,
-               "p50_ms": pct([x.latency_s for x in res], 0.5) * 1e3,
-               "p99_ms": pct([x.latency_s for x in res], 0.99) * 1e3,
-               "light_p50_ms": pct(lat_l, 0.5) * 1e3, "light_p99_ms": pct(lat_l, 0.99) * 1e3,
-               "heavy_p50_ms": pct(lat_h, 0.5) * 1e3, "heavy_p99_ms": pct(lat_h, 0.99) * 1e3,
-               "identical_to_alone": bool(ok)}
-        served.append(rec)
-        log(f"served @ {rate:.0f} q/s offered: {rec['achieved_qps']:.0f} q/s, p50 {rec['p50_ms']:.2f} ms, "
-            f"p99 {rec['p99_ms']:.2f} ms (light p99 {rec['light_p99_ms']:.2f}, heavy p99 "
-            f"{rec['heavy_p99_ms']:.2f}), identical={ok}")
-    bm = batch_mode(eng, w.gstore, P, w.info["light_types"], w.info["t0"], args) if args.batch_s > 0 else None
-    # CPU oracle on a bounded sample: the light queries, and one heavy query
-    samp = light[:256] + [sq[i] for i in heavy_idx[:1]]
-    t = time.perf_counter()
-    cpu_ok, cpu_ex = True, 0
-    ref = {tuple(q): a for q, a in zip(sq, alone)}
-    for q in samp:
-        want, wex, _ = oracle.run_query(w.gstore, q)
-        a = ref[tuple(q)]
-        cpu_ok = cpu_ok and want.shape == a.table.shape and np.array_equal(want, a.table) and wex == a.expanded
-        cpu_ex += wex
-    cpu_s = time.perf_counter() - t
-    best = max(served, key=lambda r: r["achieved_qps"]) if served else None
-    log(f"cpu oracle: {len(samp)} queries in {cpu_s:.1f} s ({len(samp) / cpu_s:.0f} q/s), parity={cpu_ok}")
-    _line("expanded edges/s + queries/s; p50/p99 latency, batched vs plain on 1 MI355X",
-          best["achieved_qps"] if best else 0.0, "queries/s", len(sq), 0,
-          1e3 / best["achieved_qps"] if best else 0.0,
-          {"workload": "served open-loop Poisson mix (95% light 2-3 patterns, 5% heavy 6 patterns) on a "
-                       "1B-triple graph: 100M entities, 256 predicates, Zipf 1.0; 64 GB table budget",
-           **{k: v for k, v in w.info.items() if k != "light_types"},
-           "queries_per_rate": len(sq), "heavy_queries": len(heavy_idx)},
-          {"served": served, "batch_mode": bm, "plain_queries_per_s": len(sq) / t_plain,
-           "device_build": device_build,
-           "light_batched_queries_per_s": len(light) / t_light, "tuning": tuning, "heavy_tuning": htuning,
-           "parity": {"served_equals_alone": all_same, "batch_mode_equals_alone": bm["equals_alone"] if bm else None,
-                      "light_batched_equals_plain": _same([alone[i] for i in light_idx], light_b),
-                      "cpu_sample_equal": bool(cpu_ok)},
-           "cpu_baseline": {"value": len(samp) / cpu_s, "unit": "queries/s", "cores": 1, "kind": "port",
-                            "sample": f"{len(samp)} of the served queries (256 light, 1 heavy) on one thread",
-                            **_cpu_info()}})
-    eng.close()
-    dev.close()
-
-
-def run_config5(args):
-    """The cyclic 6-pattern query on the 2B-triple graph: plain path with a
-    4 GB output-table limit (the ~1B-row hub fan-out is produced and consumed
-    in row chunks, depth-first), the device-resident executor on a batch of
-    the same cycle from both start types, and the CPU oracle on the sample."""
-    import numpy as np
-    import oracle
-    from wukonggpu_amd import Device, GPUEngine, workloads
-    t0 = time.time()
-    w = workloads.config5(seed=5)
-    log(f"graph: {w.info['triples']} triples, {w.info['keys']} keys, image {w.info['image_bytes'] / 1e9:.2f} GB "
-        f"built in {time.time() - t0:.1f}s; hub {w.info['hub']} degree {w.info['hub_degree']}")
-    dev = Device(0, table_budget_bytes=96 << 30)
-    dev.load(w.gstore)
-    eng = GPUEngine(dev)
-    full, sample = w.queries["full"], w.queries["sample"]
-    dev.set_table_limit(4 << 30)
-    r = eng.run_query(full)  # warm
-    lat = []
-    for _ in range(max(1, min(args.steps, 3))):
-        t = time.perf_counter()
-        r = eng.run_query(full)
-        lat.append(time.perf_counter() - t)
-    el = sum(lat) / len(lat)
-    splits = eng.splits()
-    log(f"full query, plain chunked: {r.table.shape[0]} rows, {r.expanded} expanded edges, {el * 1e3:.1f} ms "
-        f"({r.expanded / el / 1e9:.2f} G edges/s), {splits} row-half splits so far")
-    dev.set_table_limit(0)
-    eng.configure(max_queries=16, max_patterns=8, arena_bytes=64 << 30)
-    bx = eng.run_batched([full, sample])
-    tb = time.perf_counter()
-    bx = eng.run_batched([full, sample])
-    t_b = time.perf_counter() - tb
-    rs = eng.run_query(sample)
-    same_b = _same([r, rs], bx)
-    log(f"executor batch [full, sample]: {t_b * 1e3:.1f} ms, identical to plain={same_b}")
-    want, wex, sec = oracle.run_query(w.gstore, sample)
-    cpu_ok = bool(want.shape == rs.table.shape and np.array_equal(want, rs.table) and wex == rs.expanded)
-    log(f"cpu oracle on the sample: {wex} expanded edges in {sec:.1f} s, parity={cpu_ok}")
-    # row for row at scale: the sample's first four patterns end in the hub fan-out table
-    prefix = sample[:16]
-    dev.set_table_limit(256 << 20)
-    rp = eng.run_query(prefix)
-    dev.set_table_limit(0)
-    pw, pwex, psec = oracle.run_query(w.gstore, prefix)
-    prefix_ok = bool(pw.shape == rp.table.shape and np.array_equal(pw, rp.table) and pwex == rp.expanded)
-    log(f"sample prefix (4 patterns): {rp.table.shape[0]} rows, chunked at 256 MB, equal to the oracle={prefix_ok}")
-    _line("expanded edges/s + queries/s; p50/p99 latency, batched vs plain on 1 MI355X",
-          r.expanded / el, "expanded edges/s", len(lat), 1, el * 1e3,
-          {"workload": "6-pattern cyclic query (triangle closed by known_to_known, known_to_const filter) on a "
-                       "2B-triple graph: 200M entities, 512 predicates, Zipf 0.9, 2M-edge hubs; 4 GB output-table "
-                       "limit (HBM-budgeted chunks)", **w.info, "query": full,
-           "result_rows": int(r.table.shape[0]), "expanded_edges_per_query": int(r.expanded),
-           "row_half_splits": splits},
-          {"batched": {"queries": 2, "ms": t_b * 1e3, "identical_to_plain": same_b},
-           "parity": {"sample_equal_cpu": cpu_ok, "executor_equals_plain": same_b,
-                      "sample_prefix_rows_equal_cpu": prefix_ok, "sample_prefix_rows": int(rp.table.shape[0])},
-           "cpu_baseline": {"value": wex / sec, "unit": "expanded edges/s", "cores": 1, "kind": "port",
-                            "sample": f"the same cycle from the {w.info['t1_members']}-member type t1 "
-                                      f"({wex} expanded edges) on one thread", **_cpu_info()}})
-    eng.close()
-    dev.close()
 
 
 if __name__ == "__main__":
