@@ -100,8 +100,8 @@ def _kernel_table(dev, names, n):
     return out
 
 
-STEP_KERNELS = ("probe", "probe_filter", "scan_tiles", "finalize", "compact", "partition", "expand", "probe_one",
-                "copy_list", "small_step", "lineage")
+STEP_KERNELS = ("probe", "probe_filter", "fanout_filter", "scan_tiles", "finalize", "compact", "partition", "expand",
+                "probe_one", "copy_list", "small_step", "lineage")
 
 
 def _expand_roofline(dev, n_queries, traffic_file, extra=None):

@@ -373,3 +373,56 @@ def test_long_index_lists_resident(graph_mid):
     assert longest > 8192
     eng.close()
     dev.close()
+
+
+def test_fanout_filters_match_oracle(graph_hubs):
+    """A known_to_known / known_to_const from a fan-out's new column (end
+    column from before the fan-out, or a constant) is decided from the
+    fan-out's lineage: each distinct list of its parents is probed once and
+    every row tests its parent's target against its entry's list. Rows and
+    work counts equal the oracle's, with it taken at every size, with split
+    launches, in HBM-budgeted chunks (fan-outs of row slices) and off."""
+    _, gs = graph_hubs
+    dev = Device(0)
+    dev.load(gs)
+    eng = GPUEngine(dev)
+    head = [7, 1, 0, -1, -1, 2, 1, -2, -2, 2, 0, -3, -3, 2, 1, -4]   # ... ?3 p2 ?4: the hub fan-out (675k rows)
+    qs = [head + [-4, 3, 1, -2],            # triangle-style close on an inherited column
+          head + [-4, 3, 0, -3],            # IN lists, end column the fan-out's parent column
+          head + [-4, 2, 1, -1],
+          head + [-4, 3, 1, VB + 17],       # known_to_const
+          head + [-4, 2, 0, VB + 5999],
+          head + [-4, 3, 1, -2, -3, 2, 1, VB + 3],   # then a per-row filter on the small result
+          head + [-4, 2, 1, -4],            # end column the new column itself: per row
+          [VB + 0, 2, 1, -1, -1, 2, 0, -2, -2, 3, 1, -3, -3, 3, 0, -1],  # constant start
+          [6, 1, 0, -1, -1, 3, 1, -2, -2, 2, 1, -3, -3, 4, 1, -1]]
+    want = [oracle.run_query(gs, q)[:2] for q in qs]
+    try:
+        for rows, lineage, launch, limit in [(0, True, 0, 0), (0, True, 4096, 0), (0, True, 0, 4 << 20),
+                                             (1 << 22, True, 0, 0), (0, False, 0, 0)]:
+            dev.set_lineage(lineage)
+            dev.set_group_filter_rows(rows)
+            dev.set_launch_rows(launch)
+            dev.set_table_limit(limit)
+            dev.profile(True)
+            print("fan-out filters: group filter rows", rows, "lineage", lineage, "launch rows", launch,
+                  "table limit", limit, flush=True)
+            for q, (wt, wex) in zip(qs, want):
+                res = eng.run_query(q)
+                assert res.status == 0, q
+                assert res.table.shape[0] == wt.shape[0], q
+                assert np.array_equal(res.table.reshape(wt.shape), wt), q
+                assert res.expanded == wex, q
+            taken = dev.profile_read("fanout_filters")[2]
+            dev.profile(False)
+            if rows == 0 and lineage:
+                assert taken > 0
+            elif not lineage:
+                assert taken == 0
+    finally:
+        dev.set_launch_rows(0)
+        dev.set_group_filter_rows(1 << 22)
+        dev.set_lineage(True)
+        dev.set_table_limit(0)
+    eng.close()
+    dev.close()

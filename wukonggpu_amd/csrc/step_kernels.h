@@ -839,5 +839,215 @@ __global__ __launch_bounds__(kBlock) void k_flag_groups(StepParams p, const uint
   }
 }
 
+
+// ---- filters from a fan-out's new column (the parent's list decided per entry once) ----
+// A known_to_unknown output T with lineage repeats compacted parent row i
+// (parent table row info[i].x, list at segment offset info[i].y) in its rows
+// [excl[i], excl[i+1]); row excl[i] + j holds the list's entry j in its last
+// column. A known_to_known / known_to_const step from that column (the end
+// column, if any, one of the parent's) keeps row excl[i] + j iff the filter
+// target of parent i is in the list of (entry j, pid, dir). Parents that
+// share a list share its entries' keys, so each distinct list's entries are
+// probed once (D: (segment offset, degree) per entry under the filter's
+// predicate); each row then only tests its target against that list, and
+// the input table's cells are read for kept rows alone.
+struct FanoutFilter {
+  const uint64_t *excl;   // first child row per compacted parent (ascending)
+  const uint2 *info;      // (parent row, list offset) per compacted parent
+  uint64_t np;            // compacted parents
+  const uint64_t *pbase;  // per compacted parent: its list's first entry in D
+  const uint2 *D;         // per distinct-list entry: (segment offset, or the entry if degree 1; degree) of its key
+  const int32_t *P;       // parent table
+  uint32_t pc;            // parent columns
+  int32_t ec;             // known_to_known end column (a parent column); -1: known_to_const
+  uint32_t *bits;         // per tile: TR / 32 words, bit k of word w: row 32 w + k kept
+};
+
+// Keys of the distinct lists: K[k] = entry k - lbase[L] of list L (segment
+// offset ly[L] of pid), L the list holding k.
+template <bool FRAMED>
+__global__ __launch_bounds__(kBlock) void k_gather_lists(GraphView g, uint32_t pid, const uint64_t *lbase,
+                                                         const uint32_t *ly, uint64_t nl, uint64_t total, int32_t *K) {
+  const DevMeta m = g.meta[pid];
+  for (uint64_t k = blockIdx.x * (uint64_t)kBlock + threadIdx.x; k < total; k += (uint64_t)gridDim.x * kBlock) {
+    uint64_t lo = 0, hi = nl - 1;  // last L with lbase[L] <= k
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi + 1) >> 1;
+      if (lbase[mid] <= k) lo = mid;
+      else hi = mid - 1;
+    }
+    K[k] = (int32_t)edge_at<FRAMED>(g, m.edge_start, m.eframe_base, (uint64_t)ly[lo] + (k - lbase[lo]));
+  }
+}
+
+// D[k] = (0, 0) for every key, then (x, degree) for the keys with edges
+// (the compacted rows of a known_to_unknown probe pass over K): x is the
+// list's only entry when the degree is 1 (a row then needs no list read),
+// else the list's segment offset.
+__global__ __launch_bounds__(kBlock) void k_zero_u64(uint64_t *a, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) a[i] = 0;
+}
+__global__ __launch_bounds__(kBlock) void k_key_degrees(GraphView g, uint32_t pid, const uint2 *cinfo,
+                                                        const uint64_t *cexcl, uint64_t nz, uint64_t E, uint2 *D) {
+  const DevMeta m = g.meta[pid];
+  for (uint64_t k = blockIdx.x * (uint64_t)kBlock + threadIdx.x; k < nz; k += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t deg = (k + 1 < nz ? cexcl[k + 1] : E) - cexcl[k];
+    const uint32_t x = deg == 1 ? edge_at<false>(g, m.edge_start, m.eframe_base, cinfo[k].y) : cinfo[k].y;
+    D[cinfo[k].x] = make_uint2(x, (uint32_t)deg);
+  }
+}
+
+// Filter pass over one TR-row tile of T. Each thread takes RPT consecutive
+// rows: its parent (a binary search for the first row, then walking), list
+// entry, D entry and target; a list of at most kLaneList entries is scanned
+// by the row's own lane (the reference's early-exit scan, so its work count
+// is the scan's: first match + 1, or the degree). Longer lists go through
+// the probe pass's 8-lane membership test in a second phase, which a block
+// skips when it has none. Kept flags become the tile's bit words; the tile's
+// sums are kept rows and edges examined.
+template <int TR, bool SORTED>
+__global__ __launch_bounds__(kBlock) void k_fanout_filter(GraphView g, StepParams p, FanoutFilter f) {
+  constexpr int RPT = TR / kBlock;
+  constexpr uint32_t kLaneList = 8;
+  static_assert(TR % kBlock == 0 && (TR / 32) <= kBlock, "tile rows");
+  __shared__ uint32_t s_off[TR], s_deg[TR], s_tgt[TR];
+  __shared__ uint8_t s_keep[TR], s_big[TR];
+  __shared__ uint64_t s_m[2];
+  __shared__ uint32_t s_any;
+  const uint64_t lt = p.tile0 + blockIdx.x;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), gbase = lane & ~7u, l8 = lane & 7;
+  const uint64_t base = lt * TR;
+  const DevMeta m = g.meta[p.pid < g.num_metas ? p.pid : 0];
+  if (tid == 0) {  // the parents this tile's rows lie in: [i0, i1]
+    const uint64_t last = (base + TR < p.R ? base + TR : p.R) - 1;
+    const uint64_t i0 = last_le(f.excl, 0, f.np - 1, base);
+    s_m[0] = i0;
+    s_m[1] = last_le(f.excl, i0, f.np - 1, last);
+    s_any = 0;
+  }
+  __syncthreads();
+  uint64_t ex = 0;
+  {
+    const uint64_t i0 = s_m[0], i1 = s_m[1];
+    const uint64_t r0 = base + (uint64_t)tid * RPT;
+    uint64_t i = r0 < p.R ? last_le(f.excl, i0, i1, r0) : i0;
+    uint32_t off[RPT], deg[RPT], tgt[RPT];
+    bool valid[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {  // the D entries and targets, loads issued together
+      const uint64_t r = r0 + k;
+      off[k] = 0;
+      deg[k] = 0;
+      tgt[k] = 0;
+      valid[k] = false;
+      if (r < p.R) {
+        while (i < i1 && f.excl[i + 1] <= r) i++;  // every parent has rows: one step at most per row
+        const uint2 e = f.D[f.pbase[i] + (r - f.excl[i])];
+        off[k] = e.x;
+        deg[k] = e.y;
+        if (f.ec >= 0) {
+          const int32_t cell = f.P[(uint64_t)f.info[i].x * f.pc + (uint32_t)f.ec];
+          tgt[k] = (uint32_t)cell;
+          valid[k] = cell >= 0;
+        } else {
+          tgt[k] = p.target;
+          valid[k] = p.target_ok != 0;
+        }
+      }
+    }
+    bool big_any = false;
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+      const uint32_t local = tid * RPT + k;
+      uint8_t keep = 0, big = 0;
+      if (valid[k] && deg[k] > kLaneList) {
+        big = 1;
+        s_off[local] = off[k];
+        s_deg[local] = deg[k];
+        s_tgt[local] = tgt[k];
+      } else if (deg[k] == 1) {  // D holds the list's only entry
+        keep = valid[k] && off[k] == tgt[k] ? 1 : 0;
+        ex += 1;
+      } else {
+        uint32_t x = deg[k];
+        if (valid[k])
+          for (uint32_t j = 0; j < deg[k]; j++)
+            if (edge_at<false>(g, m.edge_start, m.eframe_base, (uint64_t)off[k] + j) == tgt[k]) {
+              keep = 1;
+              x = j + 1;
+              break;
+            }
+        ex += x;
+      }
+      s_keep[local] = keep;
+      s_big[local] = big;
+      big_any |= big != 0;
+    }
+    if (big_any) s_any = 1;
+  }
+  __syncthreads();
+  if (s_any) {  // long lists: the 8-lane membership test, 64 rows per wave per round
+    constexpr uint32_t kWaveRows = TR / (kBlock / kWave);
+    const uint32_t wave = tid / kWave;
+    for (int rd = 0; rd < TR / kBlock; rd++) {
+      const uint32_t local = wave * kWaveRows + rd * kWave + lane;
+      const bool big = s_big[local] != 0;
+      const uint32_t off = big ? s_off[local] : 0u, deg = big ? s_deg[local] : 0u, tgt = big ? s_tgt[local] : 0u;
+      if (!wave_ballot(big)) continue;
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const uint32_t src = gbase + u;
+        const uint32_t du = shfl32(deg, src), ou = shfl32(off, src), tu = shfl32(tgt, src);
+        const bool bu = shfl32(big ? 1u : 0u, src) != 0;
+        bool found;
+        uint32_t examined;
+        group_member<SORTED, false>(g, m.edge_start, m.eframe_base, ou, du, tu, bu, found, examined);
+        if (l8 == (uint32_t)u && big) {
+          s_keep[local] = found ? 1 : 0;
+          ex += examined;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  uint32_t word = 0, nz = 0;
+  if (tid < TR / 32) {
+#pragma unroll
+    for (int b = 0; b < 32; b++) word |= (uint32_t)s_keep[tid * 32 + b] << b;
+    nz = __builtin_popcount(word);
+    f.bits[lt * (TR / 32) + tid] = word;
+  }
+  uint64_t v[2] = {nz, ex}, tot[2];
+  __shared__ uint64_t s_tmp2[2 * (kBlock / kWave)];
+  block_excl_scan_n<2>(v, s_tmp2, tot);
+  if (tid == 0) {
+    p.tile_sum[lt] = tot[0];
+    p.tile_nz[lt] = tot[0];
+    p.tile_ex[lt] = tot[1];
+  }
+}
+
+// Compaction of a fan-out filter's kept rows: one wave per tile reads the
+// tile's TR / 32 bit words and copies each kept row of T to its place in O
+// (the tile's prefix from the two-level scan, then bit order).
+template <int TR>
+__global__ __launch_bounds__(kBlock) void k_compact_bits(StepParams p, const uint32_t *bits) {
+  static_assert(TR / 32 <= kWave, "one word per lane");
+  const uint64_t lt = p.tile0 + blockIdx.x * (uint64_t)(kBlock / kWave) + threadIdx.x / kWave;
+  if (lt >= p.ntiles) return;
+  const uint32_t lane = lane_id();
+  const uint32_t w = lane < TR / 32 ? bits[lt * (TR / 32) + lane] : 0u;
+  if (!wave_ballot(w != 0)) return;
+  const uint64_t pre = p.tile_nz[lt] + (p.chunk_nz ? p.chunk_nz[lt / kScanChunk] : 0);
+  const uint64_t inc = wave_incl_scan((uint64_t)__builtin_popcount(w));
+  uint64_t o = pre + inc - (uint64_t)__builtin_popcount(w);
+  const uint32_t c = p.c;
+  for (uint32_t x = w; x; x &= x - 1) {
+    const uint64_t r = lt * TR + (uint64_t)lane * 32 + (uint32_t)__builtin_ctz(x);
+    for (uint32_t col = 0; col < c; col++) p.O[o * c + col] = p.T[r * c + col];
+    o++;
+  }
+}
+
 }  // namespace
 }  // namespace wkg

@@ -230,7 +230,9 @@ void wkg_ctx::drop_table(wkg_table *t) {
 void wkg_ctx::unref_table(wkg_table *t) {
   if (--t->refs > 0) return;
   if (t->owned) pool->release(t->d, t->stream);
+  wkg_table *owner = t->slice_of;
   delete t;
+  if (owner) unref_table(owner);
 }
 
 // The host learns a step's counts by polling the flag that a kernel queued
@@ -562,8 +564,9 @@ int wkg_ctx::expand_pass(const wkg_table *in, int pid, const uint2 *cinfo, const
 // both the input and the arrays are small next to the budget: a later
 // known_to_unknown step on one of the input's columns then probes once per
 // compacted input row (step_k2u_lineage).
-void wkg_ctx::keep_lineage(const wkg_table *in, ProbePass &pp, uint64_t nz, wkg_table *t) {
-  if (!lineage || !pp.lin || !in->owned || nz == 0 || t->rows < 4 * nz) return;
+void wkg_ctx::keep_lineage(const wkg_table *in, ProbePass &pp, uint64_t nz, wkg_table *t, int pid) {
+  if (!lineage || !pp.lin || nz == 0 || t->rows < 4 * nz) return;
+  if (!in->owned && !in->slice_of) return;  // a caller's buffer (no-alloc step): nothing keeps it alive
   if (in->bytes + in->rows * 16 > pool->budget / 16) return;
   wkg_table *par = const_cast<wkg_table *>(in);
   par->refs++;
@@ -571,6 +574,7 @@ void wkg_ctx::keep_lineage(const wkg_table *in, ProbePass &pp, uint64_t nz, wkg_
   t->lin_info = pp.cinfo;
   t->lin_excl = pp.cexcl;
   t->lin_n = nz;
+  t->lin_pid = pid;
   t->lin_alloc = pp.lin;
   pp.lin = nullptr;
 }
@@ -707,7 +711,7 @@ int wkg_ctx::step_k2u_lineage(const wkg_table *in, int sc, int pid, int dir, hip
   own.lin = lin;
   own.cinfo = (uint2 *)lin;
   own.cexcl = lin ? (uint64_t *)((char *)lin + kc * 8) : nullptr;
-  keep_lineage(in, own, kc, t);
+  keep_lineage(in, own, kc, t, pid);
   if (own.lin) pool->release(own.lin, s);
   *out = t;
   return WKG_OK;
@@ -807,6 +811,168 @@ int wkg_ctx::step_filter_groups(int mode, const wkg_table *in, int pid, int dir,
   return done(WKG_OK);
 }
 
+// known_to_known / known_to_const from the last column of a table with
+// lineage (a known_to_unknown output that fanned out; the end column, if
+// any, one of the parent's): the distinct lists of its compacted parents are
+// found on the host from the lineage (parents sharing a list offset share
+// the list), each distinct list's entries are probed once under the
+// filter's predicate, and every row then tests its parent's target against
+// its entry's list (k_fanout_filter), with the reference's work count. Only
+// kept rows are read from the input. Returns kNotTaken (nothing done) when
+// the rows do not repeat the distinct entries at least 4 times, or the
+// parents are too many to dedup on the host.
+int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir, int ec, int64_t cval, hipStream_t s,
+                                wkg_table **out, uint64_t *expanded) {
+  constexpr uint64_t kMaxParents = 1u << 20;
+  const uint64_t np = in->lin_n, R = in->rows;
+  const int c = in->cols;
+  if (np == 0 || np > kMaxParents) return kNotTaken;
+  // the compacted parents' lists: (offset, first child row) per parent
+  std::vector<uint64_t> h((size_t)np * 2);
+  {
+    std::vector<uint2> info(np);
+    HIPCHK(this, hipMemcpyAsync(info.data(), in->lin_info, np * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(this, hipMemcpyAsync(h.data() + np, in->lin_excl, np * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(this, hipStreamSynchronize(s));
+    for (uint64_t i = 0; i < np; i++) h[i] = info[i].y;
+  }
+  // distinct lists by offset: ly (offset), lbase (first entry in D); pbase per parent
+  std::unordered_map<uint64_t, uint64_t> seen;
+  seen.reserve(np * 2);
+  std::vector<uint64_t> lbase, pbase(np);
+  std::vector<uint32_t> ly;
+  uint64_t total = 0;
+  for (uint64_t i = 0; i < np; i++) {
+    const uint64_t n = (i + 1 < np ? h[np + i + 1] : R) - h[np + i];
+    auto it = seen.find(h[i]);
+    if (it == seen.end()) {
+      it = seen.emplace(h[i], lbase.size()).first;
+      lbase.push_back(total);
+      ly.push_back((uint32_t)h[i]);
+      total += n;
+    }
+    pbase[i] = lbase[it->second];
+  }
+  if (R < 4 * total) return kNotTaken;
+  const uint64_t nl = lbase.size();
+  int rc = ensure_pred(in->lin_pid, s);
+  if (rc) return rc;
+  prof_add_bytes("fanout_filters", 1.0);  // a count
+  // device copies of the list tables, the keys, their (offset, degree) under pid
+  constexpr int TR = 1024;
+  const uint64_t ntiles = (R + TR - 1) / TR, nchunks = (ntiles + kScanChunk - 1) / kScanChunk;
+  const uint64_t up_words = np + nl + (nl + 1) / 2;  // pbase | lbase | ly (u32)
+  const uint64_t scan_words = 4 + 3 * ntiles + 3 * nchunks;
+  const uint64_t bit_words = (ntiles * (TR / 32) + 1) / 2;
+  uint64_t *buf = (uint64_t *)pool->alloc((up_words + scan_words + bit_words + total) * 8, s, err_slot());
+  if (!buf) return WKG_ENOMEM;
+  uint64_t *d_pbase = buf, *d_lbase = buf + np;
+  uint32_t *d_ly = (uint32_t *)(d_lbase + nl);
+  uint64_t *scr = buf + up_words;
+  uint32_t *bits = (uint32_t *)(scr + scan_words);
+  uint2 *D = (uint2 *)(scr + scan_words + bit_words);
+  wkg_table *K = nullptr, *t = nullptr;
+  ProbePass kp;
+  auto done = [&](int r) {
+    release_pass(kp, s);
+    drop_table(K);
+    if (r) drop_table(t);
+    pool->release(buf, s);
+    return r;
+  };
+  {
+    std::vector<uint64_t> up(up_words, 0);
+    memcpy(up.data(), pbase.data(), np * 8);
+    memcpy(up.data() + np, lbase.data(), nl * 8);
+    memcpy((uint32_t *)(up.data() + np + nl), ly.data(), nl * 4);
+    if (hipMemcpyAsync(buf, up.data(), up_words * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return done(fail(WKG_EDEVICE, "fan-out filter upload failed"));
+  }
+  if ((rc = new_table(total, 1, s, &K))) return done(rc);
+  const GraphView g = view();
+  {
+    ProfScope ps(this, s, "lineage", 8.0 * (double)total + 8.0 * (double)total);
+    const unsigned grid = (unsigned)std::min<uint64_t>((total + kBlock - 1) / kBlock, 65535);
+    k_gather_lists<false><<<grid, kBlock, 0, s>>>(g, (uint32_t)in->lin_pid, d_lbase, d_ly, nl, total, K->d);
+    k_zero_u64<<<grid, kBlock, 0, s>>>((uint64_t *)D, total);
+  }
+  if ((rc = probe_pass(kModeK2U, K, 0, pid, dir, -1, -1, nullptr, s, kp))) return done(rc);
+  if (kp.cnt[2]) {
+    ProfScope ps(this, s, "lineage", 24.0 * (double)kp.cnt[2]);
+    k_key_degrees<<<(unsigned)std::min<uint64_t>((kp.cnt[2] + kBlock - 1) / kBlock, 65535), kBlock, 0, s>>>(
+        g, (uint32_t)pid, kp.cinfo, kp.cexcl, kp.cnt[2], kp.cnt[0], D);
+  }
+  // the rows: decisions as tile bit words, the tile scan, then the kept rows
+  StepParams p{};
+  p.T = in->d;
+  p.R = R;
+  p.c = (uint32_t)c;
+  p.pid = (uint32_t)pid;
+  p.dir = (uint32_t)dir;
+  p.mode = mode;
+  p.target = (uint32_t)cval;
+  p.target_ok = (cval >= 0 && cval <= 0xffffffffLL) ? 1u : 0u;
+  p.totals = scr;
+  p.tile_sum = scr + 4;
+  p.tile_nz = p.tile_sum + ntiles;
+  p.tile_ex = p.tile_nz + ntiles;
+  p.chunk_sum = p.tile_ex + ntiles;
+  p.chunk_nz = p.chunk_sum + nchunks;
+  p.chunk_ex = p.chunk_nz + nchunks;
+  p.ntiles = ntiles;
+  FanoutFilter f{};
+  f.excl = in->lin_excl;
+  f.info = in->lin_info;
+  f.np = np;
+  f.pbase = d_pbase;
+  f.D = D;
+  f.P = in->lin_parent->d;
+  f.pc = (uint32_t)in->lin_parent->cols;
+  f.ec = mode == kModeK2K ? ec : -1;
+  f.bits = bits;
+  {
+    // per row: a D entry (8 B, mostly cached) and the list it names; the
+    // input rows are not read
+    ProfScope ps(this, s, "fanout_filter", 8.0 * (double)R);
+    const uint64_t per = std::max<uint64_t>(thread_launch_rows() / TR, 1);
+    for (uint64_t t0 = 0; t0 < ntiles; t0 += per) {
+      p.tile0 = t0;
+      const unsigned grid = (unsigned)std::min<uint64_t>(per, ntiles - t0);
+      if (sorted) k_fanout_filter<TR, true><<<grid, kBlock, 0, s>>>(g, p, f);
+      else k_fanout_filter<TR, false><<<grid, kBlock, 0, s>>>(g, p, f);
+    }
+    p.tile0 = 0;
+  }
+  p.sg = next_signal();
+  {
+    ProfScope ps(this, s, "scan_tiles", 48.0 * (double)ntiles);
+    k_scan_chunks<<<(unsigned)nchunks, kBlock, 0, s>>>(p);
+    if (nchunks > 1) k_scan_top<<<1, kBlock, 0, s>>>(p, nchunks);
+  }
+  hipError_t le = hipGetLastError();
+  if (le != hipSuccess) return done(fail(WKG_EDEVICE, "fan-out filter launch failed: %s", hipGetErrorString(le)));
+  if ((rc = wait_signal(p.sg, s))) return done(rc);
+  const uint64_t kept = h_rb[0], examined = h_rb[1];
+  if ((rc = new_table(kept, c, s, &t))) return done(rc);
+  if (kept) {
+    p.O = t->d;
+    ProfScope ps(this, s, "compact", 8.0 * (double)kept * c);
+    const uint64_t per = std::max<uint64_t>(thread_launch_rows() / TR, 4) & ~uint64_t(3);
+    for (uint64_t t0 = 0; t0 < ntiles; t0 += per) {
+      p.tile0 = t0;
+      const uint64_t n = std::min<uint64_t>(per, ntiles - t0);
+      k_compact_bits<TR><<<(unsigned)((n + kBlock / kWave - 1) / (kBlock / kWave)), kBlock, 0, s>>>(p, bits);
+    }
+    if ((le = hipGetLastError()) != hipSuccess)
+      return done(fail(WKG_EDEVICE, "fan-out compaction launch failed: %s", hipGetErrorString(le)));
+  }
+  if (expanded) *expanded = examined;
+  *out = t;
+  t = nullptr;
+  return done(WKG_OK);
+}
+
 int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval,
                         hipStream_t s, wkg_table **out, uint64_t *expanded) {
   int rc = check_pattern(pid, dir);
@@ -831,6 +997,11 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
   if (mode != kModeK2U && lineage && !framed && in->glin_parent && sc == c - 1 &&
       (mode == kModeK2C || ec < in->glin_parent->cols) && R >= group_filter_rows && R >= 4 * in->glin_dtot)
     return step_filter_groups(mode, in, pid, dir, ec, cval, s, out, expanded);
+  if (mode != kModeK2U && lineage && !framed && in->lin_parent && sc == c - 1 &&
+      (mode == kModeK2C || ec < in->lin_parent->cols) && R >= group_filter_rows) {
+    rc = step_filter_fanout(mode, in, pid, dir, ec, cval, s, out, expanded);
+    if (rc != kNotTaken) return rc;
+  }
 
   wkg_table *t = nullptr;
   if (mode != kModeK2U && (rc = new_table(R, c, s, &t))) return rc;
@@ -858,7 +1029,7 @@ int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir,
     drop_table(t);
     return rc;
   }
-  keep_lineage(in, pp, nz, t);
+  keep_lineage(in, pp, nz, t, pid);
   release_pass(pp, s);
   *out = t;
   return WKG_OK;
@@ -940,7 +1111,7 @@ int wkg_ctx::step_small(int mode, const wkg_table *in, int sc, int pid, int dir,
     }
   }
   if (expanded) *expanded = E;
-  keep_lineage(in, pp, nz, t);
+  keep_lineage(in, pp, nz, t, pid);
   release_pass(pp, s);
   *out = t;
   return WKG_OK;
@@ -1338,6 +1509,9 @@ int wkg_table_slice(wkg_ctx *ctx, const wkg_table *t, uint64_t row0, uint64_t nr
   v->bytes = nrows * (uint64_t)t->cols * sizeof(int32_t);
   v->stream = t->stream;
   v->owned = false;
+  // the view keeps its table's cells alive (a child step's lineage may read them)
+  v->slice_of = const_cast<wkg_table *>(t);
+  v->slice_of->refs++;
   *out = v;
   return WKG_OK;
 }

@@ -66,6 +66,7 @@ struct wkg_table {
   uint64_t bytes;
   hipStream_t stream;  // stream that produced it; its memory returns there
   bool owned = true;   // false: a row slice of another table (wkg_table_slice)
+  wkg_table *slice_of = nullptr;  // a slice: the table whose cells it views (held by a reference)
   int refs = 1;        // the caller's handle + child tables whose lineage reads these cells
   // Lineage, set when this table is a known_to_unknown step's output: rows
   // [lin_excl[i], lin_excl[i+1]) (the last up to `rows`) repeat the parent's
@@ -74,6 +75,7 @@ struct wkg_table {
   const uint2 *lin_info = nullptr;
   const uint64_t *lin_excl = nullptr;
   uint64_t lin_n = 0;
+  int lin_pid = 0;            // the step's predicate (lin_info[i].y is an offset in its segment)
   void *lin_alloc = nullptr;  // pool memory holding lin_info and lin_excl
   // Group lineage, set when this table is a known_to_unknown output over a
   // column its input inherited (step_k2u_lineage): group m < glin_n is
@@ -88,6 +90,10 @@ struct wkg_table {
   int glin_pid = 0;
   void *glin_alloc = nullptr;
 };
+
+// Returned by a specialised step path that declined the step (nothing was
+// done; the general path runs it). Positive, so never a WKG_* status.
+constexpr int kNotTaken = 1;
 
 // One step's probe, tile scan and compaction (wkg_ctx::probe_pass).
 struct ProbePass {
@@ -198,7 +204,9 @@ struct wkg_ctx {
   void release_pass(ProbePass &pp, hipStream_t s);
   int expand_pass(const wkg_table *in, int pid, const uint2 *cinfo, const uint64_t *cexcl, uint64_t nz, uint64_t E,
                   hipStream_t s, wkg_table *t, const char *tag = "expand", uint64_t *ext_part = nullptr);
-  void keep_lineage(const wkg_table *in, ProbePass &pp, uint64_t nz, wkg_table *t);
+  void keep_lineage(const wkg_table *in, ProbePass &pp, uint64_t nz, wkg_table *t, int pid);
+  int step_filter_fanout(int mode, const wkg_table *in, int pid, int dir, int ec, int64_t cval, hipStream_t s,
+                         wkg_table **out, uint64_t *expanded);
   int step_k2u_lineage(const wkg_table *in, int sc, int pid, int dir, hipStream_t s, wkg_table **out,
                        uint64_t *expanded);
   int step_filter_groups(int mode, const wkg_table *in, int pid, int dir, int ec, int64_t cval, hipStream_t s,
