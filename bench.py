@@ -138,7 +138,10 @@ def main():
     ap.add_argument("--time-budget", type=float, default=480.0,
                     help="seconds after which the default run leaves out its optional legs")
     ap.add_argument("--served-n", type=int, default=2000, help="queries per served rate (0: skip)")
-    ap.add_argument("--served-rates", default="1000,2000,4000", help="offered Poisson rates, queries/s")
+    ap.add_argument("--served-rates", default="1000,2000,4000",
+                    help="offered Poisson rates of the 100M-triple served mix, queries/s")
+    ap.add_argument("--served-sweep-rates", default="4000,16000,32000,64000,128000,256000,1000000",
+                    help="offered Poisson rates of the 1B-triple served sweep (to saturation), queries/s")
     ap.add_argument("--batch-s", type=float, default=2.0,
                     help="closed-loop batch mode: seconds per in-flight cap, a quarter of it warmup (0: skip)")
     ap.add_argument("--batch-p", default="20,200,1000",
@@ -508,7 +511,7 @@ def run_config23(args, nested=False):
 
     # ---- served: open-loop Poisson arrivals, light + heavy lanes ----------------------------
     served, served_same = [], None
-    if args.served_n > 0:
+    if args.served_n > 0 and not nested:
         sq, sflags = workloads.served_mix(P, type_ids, w.info["t0"], args.served_n, args.seed + 11)
         alone = eng.run_plain(sq)
         served, served_same = served_sweep(eng, dev, sq, sflags, alone, args.served_rates, args.seed + 12,
@@ -581,9 +584,11 @@ def run_config23(args, nested=False):
         "served": served, "batch_mode": bm, "table_exchange": exchange, "parity": parity})
 
 
-def served_sweep(eng, dev, sq, sflags, alone, rates, seed, budget):
+def served_sweep(eng, dev, sq, sflags, alone, rates, seed, budget, span_s=0.0):
     """Open-loop Poisson arrivals at each offered rate: achieved queries/s,
-    p50/p99 overall and per class, each answer against the query run alone."""
+    p50/p99 overall and per class, each answer against the query run alone.
+    With span_s, a rate serves the first max(2000, rate * span_s) queries of
+    the stream (arrivals over about span_s seconds); else all of them."""
     import numpy as np
     hx = [int(a.expanded) for a, f in zip(alone, sflags) if f]
     hl = [a.latency_s for a, f in zip(alone, sflags) if f]
@@ -592,26 +597,42 @@ def served_sweep(eng, dev, sq, sflags, alone, rates, seed, budget):
     out, all_same = [], True
     rng = np.random.default_rng(seed)
     for rate in [float(x) for x in rates.split(",") if x]:
-        arr = np.cumsum(rng.exponential(1.0 / rate, size=len(sq)))
+        n = len(sq) if span_s <= 0 else min(len(sq), max(2000, int(rate * span_s)))
+        qs, fl, al_ = sq[:n], sflags[:n], alone[:n]
+        arr = np.cumsum(rng.exponential(1.0 / rate, size=n))
         pool0 = dev.pool_stats()
-        res, wall = eng.serve(sq, arr, heavy=sflags, table_budget_bytes=budget, light_lanes=2)
+        res, wall = eng.serve(qs, arr, heavy=fl, table_budget_bytes=budget, light_lanes=2)
         pool1 = dev.pool_stats()
-        ok = _same(alone, res)
+        ok = _same(al_, res)
         all_same = all_same and ok
-        lat_l = [x.latency_s for x, f in zip(res, sflags) if not f]
-        lat_h = [x.latency_s for x, f in zip(res, sflags) if f]
-        rec = {"offered_qps": rate, "queries": len(sq), "heavy": int(sflags.sum()),
-               "achieved_qps": len(sq) / wall, "p50_ms": pct([x.latency_s for x in res], 0.5) * 1e3,
+        lat_l = [x.latency_s for x, f in zip(res, fl) if not f]
+        lat_h = [x.latency_s for x, f in zip(res, fl) if f]
+        rec = {"offered_qps": rate, "queries": n, "heavy": int(fl.sum()), "arrival_span_s": float(arr[-1]),
+               "achieved_qps": n / wall, "p50_ms": pct([x.latency_s for x in res], 0.5) * 1e3,
                "p99_ms": pct([x.latency_s for x in res], 0.99) * 1e3,
                "light_p50_ms": pct(lat_l, 0.5) * 1e3, "light_p99_ms": pct(lat_l, 0.99) * 1e3,
                "heavy_p50_ms": pct(lat_h, 0.5) * 1e3, "heavy_p99_ms": pct(lat_h, 0.99) * 1e3,
                "identical_to_alone": bool(ok), "serve_stats": eng.serve_stats(),
                "pool": {"mallocs": pool1["mallocs"] - pool0["mallocs"], "trims": pool1["trims"] - pool0["trims"]}}
         out.append(rec)
-        log(f"served @ {rate:.0f} q/s offered: {rec['achieved_qps']:.0f} q/s, p50 {rec['p50_ms']:.2f} ms, "
-            f"p99 {rec['p99_ms']:.2f} ms (light p99 {rec['light_p99_ms']:.2f}, heavy p99 "
+        log(f"served @ {rate:.0f} q/s offered ({n} queries): {rec['achieved_qps']:.0f} q/s, p50 {rec['p50_ms']:.2f} "
+            f"ms, p99 {rec['p99_ms']:.2f} ms (light p99 {rec['light_p99_ms']:.2f}, heavy p99 "
             f"{rec['heavy_p99_ms']:.2f}), identical={ok}")
     return out, all_same
+
+
+def served_knee(served):
+    """The highest offered rate the served loop kept up with (achieved at
+    least 95% of offered), with its latencies, and the first rate past it."""
+    kept = [r for r in served if r["achieved_qps"] >= 0.95 * r["offered_qps"]]
+    past = [r for r in served if r["achieved_qps"] < 0.95 * r["offered_qps"]]
+    k = max(kept, key=lambda r: r["offered_qps"]) if kept else None
+    p = min(past, key=lambda r: r["offered_qps"]) if past else None
+    pick = ("offered_qps", "achieved_qps", "p50_ms", "p99_ms", "light_p50_ms", "light_p99_ms", "heavy_p50_ms",
+            "heavy_p99_ms")
+    return {"last_kept_up": {x: k[x] for x in pick} if k else None,
+            "first_saturated": {x: p[x] for x in pick} if p else None,
+            "rule": "kept up: achieved >= 95% of offered"}
 
 
 # ---- config 4: served on the 1B-triple graph -------------------------------------------------------
@@ -638,7 +659,7 @@ def run_config4(args, nested=False):
     dev.load(w.gstore)
     eng = GPUEngine(dev)
     P = w.info["predicates"]
-    sq, flags = workloads.served_mix(P, w.info["light_types"], w.info["t0"], args.served_n, args.seed + 41)
+    sq, flags = workloads.served_mix(P, w.info["light_types"], w.info["t0"], args.served_n * 10, args.seed + 41)
     heavy_idx = [i for i in range(len(sq)) if flags[i]]
     eng.configure(max_queries=1024, max_patterns=8, arena_bytes=16 << 30)
     light_idx = [i for i in range(len(sq)) if not flags[i]][:512]
@@ -664,7 +685,10 @@ def run_config4(args, nested=False):
     tb = time.perf_counter()
     light_b = eng.run_batched(light)
     t_light = time.perf_counter() - tb
-    served, all_same = served_sweep(eng, dev, sq, flags, alone, args.served_rates, args.seed + 42, 64 << 30)
+    served, all_same = served_sweep(eng, dev, sq, flags, alone, args.served_sweep_rates, args.seed + 42, 64 << 30,
+                                    span_s=0.5)
+    knee = served_knee(served)
+    log(f"served knee: {knee}")
     bm = None
     if args.batch_s > 0 and (not nested or time_left(args) > 200):
         bm = batch_mode(eng, w.gstore, P, w.info["light_types"], w.info["t0"], args)
@@ -697,7 +721,8 @@ def run_config4(args, nested=False):
                               "table budget",
                   **{k: v for k, v in w.info.items() if k != "light_types"},
                   "queries_per_rate": len(sq), "heavy_queries": len(heavy_idx)},
-                 {"roofline": roof, "kernels_per_heavy_query": kernels, "served": served, "batch_mode": bm,
+                 {"roofline": roof, "kernels_per_heavy_query": kernels, "served": served, "served_knee": knee,
+                  "batch_mode": bm,
                   "plain_queries_per_s": len(sq) / t_plain, "device_build": device_build,
                   "light_batched_queries_per_s": len(light) / t_light, "tuning": tuning, "heavy_tuning": htuning,
                   "parity": parity, "cpu_baseline": cpu})
@@ -728,16 +753,36 @@ def run_config1(args, nested=False):
     dev.profile(True)
     for _ in range(20):
         eng.run_query(q)
-    kernels = _kernel_table(dev, STEP_KERNELS, 20)
+    kernels = _kernel_table(dev, STEP_KERNELS + ("small_query",), 20)
     dev.profile(False)
-    # the dominant kernel by time, against the HBM roofline (this query is launch-bound)
-    dom = max(kernels.items(), key=lambda kv: kv[1]["ms"]) if kernels else (None, None)
+    # the same query step by step (one launch sequence and read-back per step)
+    eng.set_small_queries(False)
+    for _ in range(10):
+        rs = eng.run_query(q)
+    lat_steps = []
+    for _ in range(steps):
+        t = time.perf_counter()
+        rs = eng.run_query(q)
+        lat_steps.append(time.perf_counter() - t)
+    eng.set_small_queries(True)
+    # the one launch (the whole query) against the HBM roofline: algorithmic
+    # bytes of its two steps -- the constant's bucket and list read and
+    # written (128 + 8 D), then known_to_unknown over D rows (4 D read, 128 per
+    # key probed, the E gathered edges, E rows of 2 cells written, 8 D of
+    # offsets) -- over the launch's time; the query is latency-bound
+    D = int(w.info["const_degree"])
+    E = int(r.table.shape[0])
+    alg = 128 + 8 * D + 4 * D + 128 * D + 4 * E + 8 * E + 8 * D
+    sq = kernels.get("small_query")
     roof = None
-    if dom[0]:
-        roof = {"bound": "hbm", "achieved": dom[1]["alg_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": dom[1]["alg_GBps"] / HBM_PEAK_GBS, "traffic": None, "kernel": dom[0],
-                "launches_per_query": dom[1]["launches"], "avg_launch_ms": dom[1]["ms"] / max(dom[1]["launches"], 1),
-                "note": "a 1.2k-edge query: every launch is a few workgroups, so the GPU time is launch latency"}
+    if sq and sq["ms"] > 0:
+        ach = alg / (sq["ms"] / sq["launches"] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                "traffic": None, "kernel": "small_query (k_small_query: the whole query)",
+                "launches_per_query": sq["launches"], "avg_launch_ms": sq["ms"] / sq["launches"],
+                "alg_bytes_per_launch": alg,
+                "note": "a 1.2k-edge query in one workgroup: the launch is a chain of dependent bucket and list "
+                        "reads, latency-bound by design (one launch instead of seven)"}
     eng.configure(max_queries=64, max_patterns=4, arena_bytes=1 << 30)
     bx = eng.run_batched([q])[0]
     want, wex, sec = oracle.run_query(w.gstore, q)
@@ -747,8 +792,8 @@ def run_config1(args, nested=False):
         oracle.run_query(w.gstore, q)
     cpu_s = (time.perf_counter() - t) / reps
     same = _eq(r, want, wex)
-    log(f"config 1: {r.table.shape[0]} rows, {r.expanded} expanded edges, GPU p50 {pct(lat, 0.5) * 1e3:.3f} ms, "
-        f"CPU {cpu_s * 1e3:.3f} ms/query, identical={same}")
+    log(f"config 1: {r.table.shape[0]} rows, {r.expanded} expanded edges, GPU p50 {pct(lat, 0.5) * 1e3:.3f} ms "
+        f"(step by step {pct(lat_steps, 0.5) * 1e3:.3f} ms), CPU {cpu_s * 1e3:.3f} ms/query, identical={same}")
     eng.close()
     dev.close()
     w.gstore.close()
@@ -757,12 +802,16 @@ def run_config1(args, nested=False):
                               "entities, 16 predicates, Zipf 1.1 capped at 10k",
                   **{k: v for k, v in w.info.items()}, "query": q, "result_rows": int(r.table.shape[0]),
                   "p50_ms": pct(lat, 0.5) * 1e3, "p99_ms": pct(lat, 0.99) * 1e3, "queries_per_s": steps / el,
+                  "step_by_step_p50_ms": pct(lat_steps, 0.5) * 1e3,
+                  "path": "the whole query in one launch (wkg_query_small); step_by_step_p50_ms: one launch "
+                          "sequence and read-back per step",
                   "kernels_per_query": kernels},
                  {"roofline": roof,
                   "cpu_baseline": {"value": wex / cpu_s, "unit": "expanded edges/s", "cores": 1, "kind": "port",
                                    "sample": f"the full query, {reps} runs on one thread", "p50_ms": cpu_s * 1e3,
                                    **_cpu_info()},
-                  "parity": {"rows_equal_cpu": same, "executor_equals_cpu": _eq(bx, want, wex)}})
+                  "parity": {"rows_equal_cpu": same, "executor_equals_cpu": _eq(bx, want, wex),
+                             "step_by_step_equals_cpu": _eq(rs, want, wex)}})
 
 
 # Latency points of the reference logger's per-query-type CDF (core/logger.hpp:250-313).

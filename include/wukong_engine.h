@@ -50,6 +50,12 @@ void wke_free(void *p);
  * chunks); the result is the same table an unchunked run gives. */
 int wke_run_query(wke_engine *e, const int64_t *cmd, int ncmd, wke_result *res);
 
+/* The plain path first offers an index- or constant-start query of at most
+ * 8 patterns to wkg_query_small (all steps in one launch while its tables
+ * stay small) and runs it step by step when that declines. On by default; 0
+ * turns it off. The result is the same either way. */
+int wke_set_small_queries(wke_engine *e, int on);
+
 /* Row-half splits made so far by the plain path's chunking. */
 int wke_stats(const wke_engine *e, uint64_t *splits);
 

@@ -119,6 +119,14 @@ int wkg_table_to_host(wkg_ctx *ctx, const wkg_table *t, int32_t *host, void *str
 int wkg_table_shape(const wkg_table *t, uint64_t *rows, int *cols);
 /* Device address of the table's first cell (row-major int32). */
 const int32_t *wkg_table_device_ptr(const wkg_table *t);
+/* A table over rows x cols device cells the caller owns -- the history
+ * buffers the reference receives into (GPU::history_inbuf /
+ * load_history_data, core/gpu.hpp:64-92; the GPUDirect receive of
+ * core/rdma_transport.hpp:138-150) -- so a step can read a history that is
+ * only a device pointer (request_or_reply::gpu_history_ptr,
+ * core/query.hpp:63). The view never frees the cells; they must stay valid
+ * until the view is freed and every step reading it has completed. */
+int wkg_table_wrap_device(wkg_ctx *ctx, const int32_t *cells, uint64_t rows, int cols, wkg_table **out);
 /* A known_to_unknown step of two or more input rows whose output table would
  * exceed max_table_bytes fails with WKG_ENOMEM before allocating it (0: no
  * limit, the default). The
@@ -302,6 +310,20 @@ typedef struct wkg_plan_step {
   int32_t pad;
   int64_t vid;       /* C2U start constant / K2C end constant */
 } wkg_plan_step;
+
+/* ---- a whole small query in one launch --------------------------------------------------
+ * Runs every step of a compiled plan (wkg_plan_step, above; at most 8 steps,
+ * the first an index or constant start) in one launch and one read-back,
+ * while every table stays within 4096 input rows per step and 64K cells;
+ * the result is the per-step calls' table, cell for cell. The reference
+ * runs a query's steps one handler at a time (GPU_Engine::execute_request,
+ * core/gpu_engine.hpp:471-526); for a query this small each step costs
+ * launch and read-back latency, not work. Returns WKG_ERANGE when a table
+ * outgrew those limits (or the graph is framed): nothing is returned and
+ * the caller runs the query step by step. *cells points at rows x cols
+ * cells valid until the next call on this context. */
+int wkg_query_small(wkg_ctx *ctx, const wkg_plan_step *steps, int n, void *stream, const int32_t **cells,
+                    uint64_t *rows, int *cols, uint64_t *expanded);
 
 /* Launch and tile shapes (0 = default). The autotuner below picks them. */
 typedef struct wkg_batch_tuning {

@@ -78,6 +78,24 @@ int wks_triples_from_arrays(const uint32_t *s, const uint32_t *p, const uint32_t
 /* Reads an ID triple file, one "s\tp\to" per line (datagen/generate_data.cpp:146-150). */
 int wks_triples_read_file(const char *path, wks_triples **out);
 int wks_triples_write_file(const wks_triples *t, const char *path);
+
+/* A dataset directory in the reference's ID format (datagen/generate_data.cpp
+ * :146-167, read by core/dgraph.hpp:362-417): every id_* file holds
+ * "s\tp\to" lines; str_index maps the index ids (__PREDICATE__ 0, rdf:type
+ * 1, predicates and types) and str_normal the normal vertex ids, one
+ * "<string>\t<id>" per line. wks_dataset_read reads the id_* files in name
+ * order, and, when str_index is there, checks that every predicate (and
+ * every type a triple names) is one of its ids. */
+typedef struct wks_dataset_info {
+  uint64_t files, triples;
+  uint64_t index_ids, max_index_id;   /* str_index, without __PREDICATE__ */
+  uint64_t normal_ids, max_normal_id; /* str_normal */
+  int32_t has_str_index, has_str_normal;
+} wks_dataset_info;
+int wks_dataset_read(const char *dir, wks_triples **out, wks_dataset_info *info);
+/* Writes t as `files` id_* files plus str_index and str_normal (with
+ * made-up strings) into dir (created if missing). */
+int wks_dataset_write(const wks_triples *t, const char *dir, int files);
 uint64_t wks_triples_count(const wks_triples *t);
 const uint32_t *wks_triples_s(const wks_triples *t);
 const uint32_t *wks_triples_p(const wks_triples *t);

@@ -396,6 +396,11 @@ def test_fanout_filters_match_oracle(graph_hubs):
           head + [-4, 2, 1, -4],            # end column the new column itself: per row
           [VB + 0, 2, 1, -1, -1, 2, 0, -2, -2, 3, 1, -3, -3, 3, 0, -1],  # constant start
           [6, 1, 0, -1, -1, 3, 1, -2, -2, 2, 1, -3, -3, 4, 1, -1]]
+    # short lists shared by many parents (the hub repeated 5000 times with a
+    # 2-entry list): row tiles instead of parent-aligned units
+    hub = max(gs.get_edges(0, 2, 0).tolist(), key=lambda v: gs.degree(v, 2, 1))
+    qs += [[hub, 2, 1, -1, -1, 2, 0, -2, -2, 3, 1, -3, -3, 3, 1, -1],
+           [hub, 2, 1, -1, -1, 2, 0, -2, -2, 3, 1, -3, -3, 2, 0, -2]]
     want = [oracle.run_query(gs, q)[:2] for q in qs]
     try:
         for rows, lineage, launch, limit in [(0, True, 0, 0), (0, True, 4096, 0), (0, True, 0, 4 << 20),
@@ -414,9 +419,12 @@ def test_fanout_filters_match_oracle(graph_hubs):
                 assert np.array_equal(res.table.reshape(wt.shape), wt), q
                 assert res.expanded == wex, q
             taken = dev.profile_read("fanout_filters")[2]
+            by_units, by_rows = dev.profile_read("fanout_units")[2], dev.profile_read("fanout_row_tiles")[2]
             dev.profile(False)
             if rows == 0 and lineage:
                 assert taken > 0
+                if limit == 0:
+                    assert by_units > 0 and by_rows > 0, (by_units, by_rows)
             elif not lineage:
                 assert taken == 0
     finally:
@@ -424,5 +432,72 @@ def test_fanout_filters_match_oracle(graph_hubs):
         dev.set_group_filter_rows(1 << 22)
         dev.set_lineage(True)
         dev.set_table_limit(0)
+    eng.close()
+    dev.close()
+
+
+def test_dataset_directory_and_persisted_image(tmp_path):
+    """A graph read from a dataset directory in the reference's ID format
+    (id_* files, str_index, str_normal), built, saved as an image file and
+    loaded back, then loaded on the GPU: queries give the oracle's answers
+    on the directly built image."""
+    from tests.conftest import small_graph
+    tr, gs = small_graph(seed=33, n=20000, preds=6, triples=300000, s=1.0, cap=0, types=(4000, 300, 30),
+                         hubs=1, hub_degree=8000)
+    d = tmp_path / "ds"
+    tr.write_dataset(d, files=4)
+    back, info = Triples.read_dataset(d)
+    assert info["files"] == 4 and info["has_str_index"] and info["has_str_normal"]
+    img = tmp_path / "graph.img"
+    GStore.build(back).save(img)
+    loaded = GStore.load(img)
+    dev = Device(0)
+    dev.load(loaded)
+    eng = GPUEngine(dev)
+    rng = np.random.default_rng(35)
+    qs = random_queries(rng, 20000, 6, [8, 9, 10], 80, max_steps=5)
+    print("queries on the file-loaded, persisted image", flush=True)
+    for q in qs:
+        _check(gs, q, eng.run_query(q))
+    for q, r in zip(qs, eng.run_batched(qs)):
+        _check(gs, q, r)
+    eng.close()
+    dev.close()
+
+
+def test_small_queries_in_one_launch(graph_hubs):
+    """Index- and constant-start queries whose tables stay within 4096 input
+    rows a step and 64K cells run all their steps in one launch
+    (wkg_query_small); larger ones are declined and run step by step. The
+    answers equal the oracle's either way, including a result too big for
+    the pinned staging area (16K cells) and filters, empty tables and hub
+    lists inside the limits."""
+    _, gs = graph_hubs
+    dev = Device(0)
+    dev.load(gs)
+    eng = GPUEngine(dev)
+    hub = max(gs.get_edges(0, 2, 0).tolist(), key=lambda v: gs.degree(v, 2, 1))
+    qs = [[5, 1, 0, -1, -1, 2, 1, -2],                        # 3000 members -> ~10k x 2 (past the staging area)
+          [7, 1, 0, -1, -1, 2, 1, -2, -2, 3, 0, -3],          # a chain
+          [7, 1, 0, -1, -1, 2, 1, -2, -1, 3, 1, -3, -3, 2, 0, -1],  # a star, closed
+          [VB + 1, 2, 1, -1, -1, 3, 1, VB + 7],                # constant start, known_to_const
+          [VB + 99999, 2, 1, -1, -1, 3, 1, -2],                # a constant with no edges: empty
+          [hub, 2, 1, -1, -1, 3, 1, -2],                       # a 5000-entry list: declined
+          [2, 0, 1, -1, -1, 3, 1, -2]]                         # a predicate index
+    rng = np.random.default_rng(93)
+    qs += random_queries(rng, 6000, 3, [5, 6, 7], 80, max_steps=5)
+    for on in (True, False):
+        eng.set_small_queries(on)
+        dev.profile(True)
+        print("small queries", "on" if on else "off", flush=True)
+        for q in qs:
+            _check(gs, q, eng.run_query(q))
+        taken, declined = dev.profile_read("small_query_taken")[2], dev.profile_read("small_query_declined")[2]
+        dev.profile(False)
+        if on:
+            assert taken > 0 and declined > 0, (taken, declined)
+        else:
+            assert taken == 0 and declined == 0
+    eng.set_small_queries(True)
     eng.close()
     dev.close()

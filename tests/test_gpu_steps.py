@@ -244,3 +244,33 @@ def test_bad_arguments_rejected(dev_small):
         dev_small.k2u(t, 0, 2, 3)        # direction
     with pytest.raises(DeviceError):
         dev_small.i2u(VB, 0)             # not an index id
+
+
+def test_steps_on_wrapped_device_cells(dev_hubs, graph_hubs):
+    """A table over device cells the caller owns (wkg_table_wrap_device: the
+    reference's history buffers) runs every known-start step like a table
+    of the library's own, and its fan-outs keep no lineage on it."""
+    _, gs = graph_hubs
+    rng = np.random.default_rng(71)
+    tab = _input_table(rng, 30000, 3, 6000)
+    owner = dev_hubs.table(tab)
+    for rows in (30000, 1000):  # the whole buffer, then a prefix of it
+        v = dev_hubs.wrap_device(owner, rows=rows)
+        sub = tab[:rows]
+        for cmd, fn in (([-1, 2, 1, -4], lambda t: dev_hubs.k2u(t, 0, 2, 1)),
+                        ([-2, 3, 0, -1], lambda t: dev_hubs.k2k(t, 1, 3, 0, 0)),
+                        ([-3, 2, 1, VB + 11], lambda t: dev_hubs.k2c(t, 2, 2, 1, VB + 11))):
+            want, wex = oracle.run_step(gs, [VB, 2, 1, -1] + cmd, 1, sub)
+            got, ex = fn(v)
+            _eq(got.to_host(), want, ("wrapped", rows, cmd))
+            assert ex == wex
+            # a second step over the fan-out of a wrapped table (no lineage kept on it)
+            if cmd[3] == -4:
+                want2, wex2 = oracle.run_step(gs, [VB, 2, 1, -1, -1, 2, 1, -4] + [-1, 3, 1, -5], 2, want)
+                got2, ex2 = dev_hubs.k2u(got, 0, 3, 1)
+                _eq(got2.to_host(), want2, ("wrapped fan-out, star arm", rows))
+                assert ex2 == wex2
+                got2.free()
+            got.free()
+        v.free()
+    owner.free()

@@ -196,3 +196,35 @@ def test_generator_planted_links_and_siblings():
         assert sorted(planted[dst]) == sorted(want), dst
     a.close()
     b.close()
+
+
+def test_dataset_directory_roundtrip(tmp_path):
+    # the reference's ID-format dataset (id_* files, str_index, str_normal):
+    # written and read back it gives the same triples, in file order, and the
+    # same image; a predicate missing from str_index is rejected
+    tr, gs = small_graph(seed=31, n=400, preds=5, triples=4000, types=(40, 7))
+    d = tmp_path / "ds"
+    tr.write_dataset(d, files=3)
+    names = sorted(p.name for p in d.iterdir())
+    assert names == ["id_0000.nt", "id_0001.nt", "id_0002.nt", "str_index", "str_normal"]
+    back, info = Triples.read_dataset(d)
+    a, b = tr.arrays(), back.arrays()
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    s, p, o = a
+    assert info["files"] == 3 and info["triples"] == len(s)
+    ids = set(p.tolist()) | set(o[p == 1].tolist())
+    assert info["has_str_index"] and info["index_ids"] == len(ids | {1}) and info["max_index_id"] == max(ids)
+    normal = set(s.tolist()) | set(o[p != 1].tolist())
+    assert info["has_str_normal"] and info["normal_ids"] == len(normal) and info["max_normal_id"] == max(normal)
+    g2 = GStore.build(back)
+    assert np.array_equal(g2.slots(), gs.slots()) and np.array_equal(g2.edges(), gs.edges())
+    assert np.array_equal(g2.metas(), gs.metas())
+    # str_index without predicate 3: the dataset is inconsistent
+    lines = (d / "str_index").read_text().splitlines()
+    (d / "str_index").write_text("\n".join(x for x in lines if not x.endswith("\t3")) + "\n")
+    with pytest.raises(RuntimeError):
+        Triples.read_dataset(d)
+    # a directory with no id_* files
+    (tmp_path / "empty").mkdir()
+    with pytest.raises(RuntimeError):
+        Triples.read_dataset(tmp_path / "empty")

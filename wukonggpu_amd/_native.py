@@ -26,6 +26,11 @@ class GenParams(C.Structure):
                 ("sibling_src", P(u32)), ("sibling_dst", P(u32))]
 
 
+class DatasetInfo(C.Structure):
+    _fields_ = [("files", u64), ("triples", u64), ("index_ids", u64), ("max_index_id", u64),
+                ("normal_ids", u64), ("max_normal_id", u64), ("has_str_index", i32), ("has_str_normal", i32)]
+
+
 class BuildParams(C.Structure):
     _fields_ = [("main_load", dbl), ("num_threads", C.c_int)]
 
@@ -92,6 +97,8 @@ def gstore_lib():
         _sig(L, "wks_triples_from_arrays", C.c_int, P(u32), P(u32), P(u32), u64, P(vp))
         _sig(L, "wks_triples_read_file", C.c_int, C.c_char_p, P(vp))
         _sig(L, "wks_triples_write_file", C.c_int, vp, C.c_char_p)
+        _sig(L, "wks_dataset_read", C.c_int, C.c_char_p, P(vp), P(DatasetInfo))
+        _sig(L, "wks_dataset_write", C.c_int, vp, C.c_char_p, C.c_int)
         _sig(L, "wks_triples_count", u64, vp)
         for n in ("s", "p", "o"):
             _sig(L, f"wks_triples_{n}", P(u32), vp)
@@ -126,6 +133,7 @@ def hip_lib():
         _sig(L, "wkg_table_to_host", C.c_int, vp, vp, P(i32), vp)
         _sig(L, "wkg_table_shape", C.c_int, vp, P(u64), P(C.c_int))
         _sig(L, "wkg_table_device_ptr", vp, vp)
+        _sig(L, "wkg_table_wrap_device", C.c_int, vp, vp, u64, C.c_int, P(vp))
         _sig(L, "wkg_table_free", C.c_int, vp, vp)
         _sig(L, "wkg_table_slice", C.c_int, vp, vp, u64, u64, P(vp))
         _sig(L, "wkg_table_split", C.c_int, vp, vp, C.c_int, C.c_int, vp, P(vp), P(u64), P(u64))
@@ -147,6 +155,7 @@ def hip_lib():
              P(u64), P(u64))
         _sig(L, "wkg_step_k2k", C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, P(vp), P(u64))
         _sig(L, "wkg_step_k2c", C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, i64, vp, P(vp), P(u64))
+        _sig(L, "wkg_query_small", C.c_int, vp, P(PlanStep), C.c_int, vp, P(P(i32)), P(u64), P(C.c_int), P(u64))
         _sig(L, "wkg_step_batch", C.c_int, vp, C.c_int, P(StepDesc), vp, P(vp), P(u64))
         _sig(L, "wkg_profile_enable", C.c_int, vp, C.c_int)
         _sig(L, "wkg_profile_read", C.c_int, vp, C.c_char_p, P(u64), P(dbl), P(dbl))
@@ -184,6 +193,7 @@ def engine_lib():
              P(P(i32)), P(u64))
         _sig(L, "wke_configure", C.c_int, vp, C.c_int, C.c_int, u64)
         _sig(L, "wke_stats", C.c_int, vp, P(u64))
+        _sig(L, "wke_set_small_queries", C.c_int, vp, C.c_int)
         _sig(L, "wke_autotune", C.c_int, vp, P(i64), P(i64), C.c_int, C.c_int, C.c_int, P(BatchTuning), P(dbl))
         _sig(L, "wke_set_tuning", C.c_int, vp, P(BatchTuning), C.c_int)
         _sig(L, "wke_set_heavy_mode", C.c_int, vp, C.c_int)

@@ -163,6 +163,7 @@ struct wke_engine {
   // stream per context.
   std::mutex plain_mu;
   int heavy_mode = -1;  // wke_set_heavy_mode
+  bool small_queries = true;  // wke_set_small_queries: try wkg_query_small first
   uint64_t sv_fallbacks = 0, sv_solo = 0, sv_worker = 0, sv_backlog = 0;  // last wke_serve
   // device-resident executor shape (wke_configure) and tuning (wke_autotune)
   int cfg_queries = 1024, cfg_patterns = 8;
@@ -370,6 +371,32 @@ int wke_run_query(wke_engine *e, const int64_t *cmd, int ncmd, wke_result *res) 
   if (!e || !cmd || !res || ncmd <= 0 || ncmd % 4) return WKG_EINVAL;
   memset(res, 0, sizeof *res);
   const auto t0 = Clock::now();
+  // a query whose tables stay small: all of its steps in one launch (the
+  // library declines with WKG_ERANGE once a table outgrows that; then, and
+  // on any other status, the query runs step by step, which also reports
+  // the canonical error of a bad plan)
+  if (e->small_queries && ncmd / 4 <= 8) {
+    std::vector<wkg_plan_step> plan;
+    std::string why;
+    if (compile_query(cmd, ncmd, plan, why) == WKG_OK &&
+        (plan[0].kind == WKG_STEP_I2U || plan[0].kind == WKG_STEP_C2U)) {
+      const int32_t *cells = nullptr;
+      uint64_t rows = 0, ex = 0;
+      int cols = 0;
+      if (wkg_query_small(e->ctx, plan.data(), (int)plan.size(), nullptr, &cells, &rows, &cols, &ex) == WKG_OK) {
+        const uint64_t n = rows * (uint64_t)cols;
+        res->table = n ? (int32_t *)malloc(n * 4) : nullptr;
+        if (n && !res->table) return e->fail(WKG_ENOMEM, "host result table");
+        if (n) memcpy(res->table, cells, n * 4);
+        res->rows = rows;
+        res->cols = cols;
+        res->expanded = ex;
+        res->status = WKG_OK;
+        res->latency_s = std::chrono::duration<double>(Clock::now() - t0).count();
+        return WKG_OK;
+      }
+    }
+  }
   Request r;
   r.cmd = cmd;
   r.ncmd = ncmd;
@@ -386,6 +413,12 @@ int wke_run_query(wke_engine *e, const int64_t *cmd, int ncmd, wke_result *res) 
   }
   res->latency_s = std::chrono::duration<double>(Clock::now() - t0).count();
   return rc;
+}
+
+int wke_set_small_queries(wke_engine *e, int on) {
+  if (!e) return WKG_EINVAL;
+  e->small_queries = on != 0;
+  return WKG_OK;
 }
 
 int wke_stats(const wke_engine *e, uint64_t *splits) {

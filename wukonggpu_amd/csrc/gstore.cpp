@@ -22,7 +22,9 @@
 #include "gstore_layout.h"
 #include "../../include/wukong_gstore.h"
 
+#include <dirent.h>
 #include <omp.h>
+#include <sys/stat.h>
 
 #include <algorithm>
 #include <atomic>
@@ -406,11 +408,13 @@ int wks_triples_from_arrays(const uint32_t *s, const uint32_t *p, const uint32_t
   return WKS_OK;
 }
 
-int wks_triples_read_file(const char *path, wks_triples **out) {
-  if (!path || !out) return fail(WKS_EINVAL, "null argument");
+}  // extern "C"
+
+namespace {
+// Appends the "s\tp\to" lines of one ID-format file to t.
+int read_triples_into(const char *path, wks_triples *t) {
   FILE *f = fopen(path, "rb");
   if (!f) return fail(WKS_EIO, "cannot open %s: %s", path, strerror(errno));
-  wks_triples *t = new wks_triples;
   std::vector<char> buf(1 << 22);
   uint64_t v[3];
   int field = 0;
@@ -453,8 +457,149 @@ int wks_triples_read_file(const char *path, wks_triples **out) {
   }
   if (rc == WKS_OK) end_line();
   fclose(f);
+  return rc;
+}
+
+// The "<string>\t<id>" lines of an ID-mapping file (str_index / str_normal,
+// datagen/generate_data.cpp:155-167): calls fn(id) per line.
+template <class Fn>
+int read_id_map(const std::string &path, Fn fn) {
+  FILE *f = fopen(path.c_str(), "rb");
+  if (!f) return fail(WKS_EIO, "cannot open %s: %s", path.c_str(), strerror(errno));
+  std::vector<char> line(1 << 16);
+  uint64_t n = 0;
+  int rc = WKS_OK;
+  while (fgets(line.data(), (int)line.size(), f)) {
+    n++;
+    char *tab = strrchr(line.data(), '\t');
+    if (!tab) {
+      if (line[0] == '\n' || line[0] == '\0') continue;
+      rc = fail(WKS_EIO, "%s:%llu: expected <string>\\t<id>", path.c_str(), (unsigned long long)n);
+      break;
+    }
+    char *end = nullptr;
+    const unsigned long long id = strtoull(tab + 1, &end, 10);
+    if (end == tab + 1 || id > 0xffffffffull) {
+      rc = fail(WKS_EIO, "%s:%llu: bad id", path.c_str(), (unsigned long long)n);
+      break;
+    }
+    fn((uint32_t)id);
+  }
+  fclose(f);
+  return rc;
+}
+}  // namespace
+
+extern "C" {
+
+int wks_triples_read_file(const char *path, wks_triples **out) {
+  if (!path || !out) return fail(WKS_EINVAL, "null argument");
+  wks_triples *t = new wks_triples;
+  int rc = read_triples_into(path, t);
   if (rc != WKS_OK) { delete t; return rc; }
   *out = t;
+  return WKS_OK;
+}
+
+int wks_dataset_read(const char *dir, wks_triples **out, wks_dataset_info *info) {
+  if (!dir || !out) return fail(WKS_EINVAL, "null argument");
+  *out = nullptr;
+  DIR *d = opendir(dir);
+  if (!d) return fail(WKS_EIO, "cannot open directory %s: %s", dir, strerror(errno));
+  std::vector<std::string> files;
+  bool has_index = false, has_normal = false;
+  for (struct dirent *e; (e = readdir(d));) {
+    const std::string n = e->d_name;
+    if (n.empty() || n[0] == '.') continue;
+    if (n.compare(0, 3, "id_") == 0) files.push_back(n);
+    if (n == "str_index") has_index = true;
+    if (n == "str_normal") has_normal = true;
+  }
+  closedir(d);
+  if (files.empty()) return fail(WKS_EIO, "no id_* files in %s", dir);
+  std::sort(files.begin(), files.end());
+  const std::string base = std::string(dir) + "/";
+  wks_dataset_info inf{};
+  inf.files = files.size();
+  wks_triples *t = new wks_triples;
+  for (const std::string &n : files)
+    if (int rc = read_triples_into((base + n).c_str(), t)) {
+      delete t;
+      return rc;
+    }
+  inf.triples = t->s.size();
+  // index ids (predicates and types): the id space of the pred metas
+  std::vector<char> index_id;
+  if (has_index) {
+    int rc = read_id_map(base + "str_index", [&](uint32_t id) {
+      if (id >= wkg::kMaxIdx + 1u) return;
+      if (id >= index_id.size()) index_id.resize(id + 1, 0);
+      if (id && !index_id[id]) inf.index_ids++;
+      index_id[id] = 1;
+      inf.max_index_id = std::max<uint64_t>(inf.max_index_id, id);
+    });
+    if (rc) { delete t; return rc; }
+    for (size_t i = 0; i < t->s.size(); i++) {
+      const uint32_t pid = t->p[i];
+      const bool ok = pid < index_id.size() && index_id[pid] &&
+                      (pid != wkg::kTypeId || (t->o[i] < index_id.size() && index_id[t->o[i]]));
+      if (!ok) {
+        delete t;
+        return fail(WKS_EINVAL, "triple %llu names an index id missing from str_index", (unsigned long long)i);
+      }
+    }
+  }
+  if (has_normal) {
+    int rc = read_id_map(base + "str_normal", [&](uint32_t id) {
+      inf.normal_ids++;
+      inf.max_normal_id = std::max<uint64_t>(inf.max_normal_id, id);
+    });
+    if (rc) { delete t; return rc; }
+  }
+  inf.has_str_index = has_index;
+  inf.has_str_normal = has_normal;
+  if (info) *info = inf;
+  *out = t;
+  return WKS_OK;
+}
+
+int wks_dataset_write(const wks_triples *t, const char *dir, int files) {
+  if (!t || !dir || files < 1) return fail(WKS_EINVAL, "null argument or no files");
+  if (mkdir(dir, 0755) != 0 && errno != EEXIST) return fail(WKS_EIO, "cannot create %s: %s", dir, strerror(errno));
+  const std::string base = std::string(dir) + "/";
+  const size_t n = t->s.size();
+  for (int k = 0; k < files; k++) {
+    char name[64];
+    snprintf(name, sizeof name, "id_%04d.nt", k);
+    FILE *f = fopen((base + name).c_str(), "wb");
+    if (!f) return fail(WKS_EIO, "cannot open %s%s: %s", base.c_str(), name, strerror(errno));
+    for (size_t i = n * k / files; i < n * (k + 1) / files; i++) fprintf(f, "%u\t%u\t%u\n", t->s[i], t->p[i], t->o[i]);
+    if (fclose(f) != 0) return fail(WKS_EIO, "write to %s%s failed", base.c_str(), name);
+  }
+  // ID mappings with made-up names in the reference's layout: __PREDICATE__ 0,
+  // rdf:type 1, then every predicate and type id; every normal vertex id
+  std::vector<char> idx, normal;
+  auto mark = [](std::vector<char> &v, uint32_t id) {
+    if (id >= v.size()) v.resize((size_t)id + 1, 0);
+    v[id] = 1;
+  };
+  for (size_t i = 0; i < n; i++) {
+    mark(idx, t->p[i]);
+    mark(normal, t->s[i]);
+    if (t->p[i] == wkg::kTypeId) mark(idx, t->o[i]);
+    else mark(normal, t->o[i]);
+  }
+  FILE *f = fopen((base + "str_index").c_str(), "wb");
+  if (!f) return fail(WKS_EIO, "cannot open %sstr_index: %s", base.c_str(), strerror(errno));
+  fprintf(f, "__PREDICATE__\t0\n<http://www.w3.org/1999/02/22-rdf-syntax-ns#type>\t1\n");
+  for (size_t id = 2; id < idx.size(); id++)
+    if (idx[id]) fprintf(f, "<http://wukong.example/index/%zu>\t%zu\n", id, id);
+  if (fclose(f) != 0) return fail(WKS_EIO, "write to %sstr_index failed", base.c_str());
+  f = fopen((base + "str_normal").c_str(), "wb");
+  if (!f) return fail(WKS_EIO, "cannot open %sstr_normal: %s", base.c_str(), strerror(errno));
+  for (size_t id = 0; id < normal.size(); id++)
+    if (normal[id]) fprintf(f, "<http://wukong.example/vertex/%zu>\t%zu\n", id, id);
+  if (fclose(f) != 0) return fail(WKS_EIO, "write to %sstr_normal failed", base.c_str());
   return WKS_OK;
 }
 

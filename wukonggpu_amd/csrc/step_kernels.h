@@ -23,6 +23,7 @@
 // independent steps in one launch.
 #pragma once
 
+#include "../../include/wukong_gpu.h"
 #include "device.h"
 
 namespace wkg {
@@ -608,6 +609,157 @@ __global__ __launch_bounds__(kBlock) void k_small_step(GraphView g, StepParams p
   if (threadIdx.x == 0) publish(sg, p.totals, 3);  // (thread 0 wrote the totals)
 }
 
+// ---- a whole small query in one launch ----
+// Every step of a query whose tables stay small runs in one workgroup with
+// the per-step path's own tile bodies (probe, tile scan, finalize,
+// merge-path partition, expand), between workgroup barriers, on two
+// ping-pong tables in device scratch; the last table goes to the pinned
+// staging area and the results are published with one flag. A table past
+// the caps ends the launch early with status 1 (the caller then runs the
+// query step by step), so the result is always the per-step path's.
+constexpr uint32_t kSQSteps = 8;                 // patterns
+constexpr uint64_t kSQRows = 16 * kBlock;        // input rows of a step (16 probe tiles)
+constexpr uint64_t kSQCells = 1u << 16;          // cells of a table (64K: 256 KB)
+struct SmallQuery {
+  wkg_plan_step steps[kSQSteps];
+  uint32_t nsteps;
+  int32_t *buf[2];     // ping-pong tables, kSQCells cells each
+  uint64_t *scr;       // step scratch (SQ_SCRATCH_WORDS)
+  int32_t *stage;      // pinned staging area (device address), stage_cells cells
+  uint64_t stage_cells;
+};
+constexpr uint64_t kSQTiles = kSQRows / kBlock;
+// scratch words: totals 4 | tile sums 3 x tiles | chunk sums 3 | hits (16 B a row, 16-byte aligned) | cinfo (8 B a
+// row) | cexcl (8 B a row) | merge-path split points
+constexpr uint64_t kSQOffHits = (4 + 3 * kSQTiles + 3 + 1) & ~uint64_t(1);
+constexpr uint64_t kSQOffCinfo = kSQOffHits + 2 * kSQRows;
+constexpr uint64_t kSQOffCexcl = kSQOffCinfo + kSQRows;
+constexpr uint64_t kSQOffPart = kSQOffCexcl + kSQRows;
+constexpr uint64_t kSQScratchWords = kSQOffPart + (kSQRows + kSQCells) / kExpandNV + 2;
+
+template <bool SORTED, bool FRAMED>
+__global__ __launch_bounds__(kBlock) void k_small_query(GraphView g, SmallQuery q, Signal sg) {
+  __shared__ WaveProbeLds s_wp;
+  __shared__ uint32_t s_deg, s_off;
+  uint64_t rows = 0, expanded = 0, status = 0;
+  uint32_t cols = 0, cur = 0;
+  for (uint32_t l = 0; l < q.nsteps && status == 0; l++) {
+    const wkg_plan_step d = q.steps[l];
+    int32_t *in = q.buf[cur], *out = q.buf[cur ^ 1];
+    if (d.kind == WKG_STEP_I2U || d.kind == WKG_STEP_C2U) {  // one key: its list is the table
+      const uint64_t key = d.kind == WKG_STEP_I2U ? make_key(0, (uint64_t)d.pid, (uint64_t)d.dir)
+                                                  : make_key((uint64_t)d.vid, (uint64_t)d.pid, (uint64_t)d.dir);
+      const DevMeta m = g.meta[d.pid];
+      if (threadIdx.x < kWave) {
+        uint32_t deg, off;
+        ProbeCarry carry;
+        wave_probe<FRAMED>(g, m, threadIdx.x == 0 ? key : 0, s_wp, deg, off, carry);
+        if (threadIdx.x == 0) {
+          s_deg = deg;
+          s_off = off;
+        }
+      }
+      __syncthreads();
+      const uint64_t deg = s_deg, off = s_off;
+      expanded += deg;
+      if (deg > kSQRows) {
+        status = 1;
+        break;
+      }
+      for (uint64_t j = threadIdx.x; j < deg; j += kBlock)
+        out[j] = (int32_t)edge_at<FRAMED>(g, m.edge_start, m.eframe_base, off + j);
+      rows = deg;
+      cols = 1;
+    } else {
+      if (rows > kSQRows) {
+        status = 1;
+        break;
+      }
+      const int mode = d.kind == WKG_STEP_K2U ? kModeK2U : d.kind == WKG_STEP_K2K ? kModeK2K : kModeK2C;
+      StepParams p{};
+      p.T = in;
+      p.R = rows;
+      p.c = cols;
+      p.sc = (uint32_t)d.start_col;
+      p.ec = mode == kModeK2K ? (uint32_t)d.end_col : 0u;
+      p.pid = (uint32_t)d.pid;
+      p.dir = (uint32_t)d.dir;
+      p.target = (uint32_t)d.vid;
+      p.target_ok = (mode == kModeK2C && d.vid >= 0 && d.vid <= 0xffffffffLL) ? 1u : 0u;
+      p.mode = mode;
+      p.ntiles = (rows + kBlock - 1) / kBlock;
+      p.totals = q.scr;
+      p.tile_sum = q.scr + 4;
+      p.tile_nz = p.tile_sum + kSQTiles;
+      p.tile_ex = p.tile_nz + kSQTiles;
+      p.chunk_sum = p.tile_ex + kSQTiles;
+      p.chunk_nz = p.chunk_sum + 1;
+      p.chunk_ex = p.chunk_nz + 1;
+      p.hits = (uint4 *)(q.scr + kSQOffHits);
+      p.cinfo = (uint2 *)(q.scr + kSQOffCinfo);
+      p.cexcl = q.scr + kSQOffCexcl;
+      p.O = mode == kModeK2U ? nullptr : out;
+      for (uint64_t lt = 0; lt < p.ntiles; lt++) {
+        if (mode == kModeK2U) probe_tile<kBlock, true, FRAMED>(g, p, lt);
+        else probe_tile<kBlock, SORTED, FRAMED>(g, p, lt);
+        __syncthreads();  // the tile's LDS is reused by the next
+      }
+      if (p.ntiles == 0 && threadIdx.x == 0) {  // an empty input: empty output, no work
+        p.totals[0] = p.totals[1] = p.totals[2] = 0;
+      } else {
+        scan_chunk(p, 0, 1);
+      }
+      __syncthreads();
+      for (uint64_t lt = threadIdx.x / kWave; lt < p.ntiles; lt += kBlock / kWave) finalize_tile<kBlock>(p, lt);
+      __syncthreads();
+      if (mode == kModeK2U) {
+        const uint64_t E = p.totals[0], nz = p.totals[2];
+        expanded += E;
+        if (E * (uint64_t)(cols + 1) > kSQCells) {
+          status = 1;
+          break;
+        }
+        if (E) {
+          uint64_t *part = q.scr + kSQOffPart;
+          const uint64_t nx = (nz + E + kExpandNV - 1) / kExpandNV;
+          for (uint64_t k = threadIdx.x; k <= nx; k += kBlock) part[k] = merge_split<kExpandNV>(p.cexcl, nz, E, k);
+          __syncthreads();
+          ExpandParams x{};
+          x.pid = p.pid;
+          x.T = in;
+          x.c = cols;
+          x.R = nz;
+          x.cinfo = p.cinfo;
+          x.cexcl = p.cexcl;
+          x.part = part;
+          x.E = E;
+          x.O = out;
+          for (uint64_t t = 0; t < nx; t++) expand_tile<kExpandVT, FRAMED>(g, x, t);
+        }
+        rows = E;
+        cols += 1;
+      } else {
+        rows = p.totals[0];
+        expanded += p.totals[1];
+      }
+    }
+    __syncthreads();  // the output is complete before it is the next step's input
+    cur ^= 1;
+  }
+  // the last table to the host staging area when it fits there (status 2:
+  // it stays in buf[cur] for a copy command)
+  const uint64_t cells = rows * (uint64_t)cols;
+  if (status == 0 && cells > q.stage_cells) status = 2;
+  if (status == 0)
+    for (uint64_t i = threadIdx.x; i < cells; i += kBlock) q.stage[i] = q.buf[cur][i];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint64_t v[5] = {status, rows, cols, expanded, cur};
+    publish(sg, v, 5);
+  }
+}
+
 // A small table to the host: copies n cells into the pinned staging area,
 // then publishes (every wave's stores complete at system scope first).
 __global__ __launch_bounds__(kBlock) void k_stage(const int32_t *src, uint64_t n, int32_t *dst, Signal sg) {
@@ -1027,14 +1179,149 @@ __global__ __launch_bounds__(kBlock) void k_fanout_filter(GraphView g, StepParam
   }
 }
 
+// The same decisions over parent-aligned units: unit (i, jb) is rows
+// [excl[i] + jb TR, ...) of parent i, at most TR of them, all with one
+// target. Units are ranked in row order (rank = ubase[i] + jb: the order of
+// the scan and the compaction) but executed list by list, j-block by
+// j-block, the parents sharing a list innermost (block e of list L: j-block
+// (e - xbase[L]) / |parents of L|), so the blocks that read one slice of a
+// shared list's D entries run together and find it in their L2.
+struct FanoutUnits {
+  const uint64_t *xbase;  // per distinct list (+1): first execution index
+  const uint64_t *poff;   // per distinct list (+1): its parents' first slot in par
+  const uint32_t *par;    // compacted parents, grouped by list, row order within a list
+  const uint64_t *ubase;  // per compacted parent: rank of its first unit
+  uint64_t nl;            // distinct lists
+  uint64_t *row0;         // per unit rank: its first row
+};
+
+template <int TR, bool SORTED>
+__global__ __launch_bounds__(kBlock) void k_fanout_units(GraphView g, StepParams p, FanoutFilter f, FanoutUnits u) {
+  constexpr int RPT = TR / kBlock;
+  constexpr uint32_t kLaneList = 8;
+  static_assert(TR % kBlock == 0 && (TR / 32) <= kBlock, "tile rows");
+  __shared__ uint32_t s_off[TR], s_deg[TR];
+  __shared__ uint8_t s_keep[TR], s_big[TR];
+  __shared__ uint32_t s_any;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), gbase = lane & ~7u, l8 = lane & 7;
+  const uint64_t e = p.tile0 + blockIdx.x;
+  // the unit: list L, j-block, parent
+  uint64_t L = 0, hi = u.nl - 1;
+  while (L < hi) {
+    const uint64_t mid = (L + hi + 1) >> 1;
+    if (u.xbase[mid] <= e) L = mid;
+    else hi = mid - 1;
+  }
+  const uint64_t npl = u.poff[L + 1] - u.poff[L], e2 = e - u.xbase[L];
+  const uint64_t jb = e2 / npl;
+  const uint32_t i = u.par[u.poff[L] + e2 % npl];
+  const uint64_t first = f.excl[i];
+  const uint64_t n = (i + 1 < f.np ? f.excl[i + 1] : p.R) - first;
+  const uint64_t j0 = jb * TR;
+  const uint32_t nrows = (uint32_t)(n - j0 < (uint64_t)TR ? n - j0 : (uint64_t)TR);
+  const uint64_t rank = u.ubase[i] + jb;
+  uint32_t tgt;
+  bool valid;
+  if (f.ec >= 0) {
+    const int32_t cell = f.P[(uint64_t)f.info[i].x * f.pc + (uint32_t)f.ec];
+    tgt = (uint32_t)cell;
+    valid = cell >= 0;
+  } else {
+    tgt = p.target;
+    valid = p.target_ok != 0;
+  }
+  const DevMeta m = g.meta[p.pid < g.num_metas ? p.pid : 0];
+  if (tid == 0) s_any = 0;
+  const uint2 *Dp = f.D + f.pbase[i] + j0;
+  uint2 d[RPT];
+#pragma unroll
+  for (int k = 0; k < RPT; k++) {  // the unit's D entries, loads issued together
+    const uint32_t local = tid * RPT + k;
+    d[k] = local < nrows ? Dp[local] : make_uint2(0u, 0u);
+  }
+  __syncthreads();
+  uint64_t ex = 0;
+  bool big_any = false;
+#pragma unroll
+  for (int k = 0; k < RPT; k++) {
+    const uint32_t local = tid * RPT + k;
+    const uint32_t off = d[k].x, deg = d[k].y;
+    uint8_t keep = 0, big = 0;
+    if (valid && deg > kLaneList) {
+      big = 1;
+      s_off[local] = off;
+      s_deg[local] = deg;
+    } else if (deg == 1) {  // D holds the list's only entry
+      keep = valid && off == tgt ? 1 : 0;
+      ex += 1;
+    } else {
+      uint32_t x = deg;
+      if (valid)
+        for (uint32_t j = 0; j < deg; j++)
+          if (edge_at<false>(g, m.edge_start, m.eframe_base, (uint64_t)off + j) == tgt) {
+            keep = 1;
+            x = j + 1;
+            break;
+          }
+      ex += x;
+    }
+    s_keep[local] = keep;
+    s_big[local] = big;
+    big_any |= big != 0;
+  }
+  if (big_any) s_any = 1;
+  __syncthreads();
+  if (s_any) {  // long lists: the 8-lane membership test, 64 rows per wave per round
+    constexpr uint32_t kWaveRows = TR / (kBlock / kWave);
+    const uint32_t wave = tid / kWave;
+    for (int rd = 0; rd < TR / kBlock; rd++) {
+      const uint32_t local = wave * kWaveRows + rd * kWave + lane;
+      const bool big = s_big[local] != 0;
+      const uint32_t off = big ? s_off[local] : 0u, deg = big ? s_deg[local] : 0u;
+      if (!wave_ballot(big)) continue;
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const uint32_t src = gbase + q;
+        const uint32_t du = shfl32(deg, src), ou = shfl32(off, src);
+        const bool bu = shfl32(big ? 1u : 0u, src) != 0;
+        bool found;
+        uint32_t examined;
+        group_member<SORTED, false>(g, m.edge_start, m.eframe_base, ou, du, tgt, bu, found, examined);
+        if (l8 == (uint32_t)q && big) {
+          s_keep[local] = found ? 1 : 0;
+          ex += examined;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  uint32_t word = 0, nz = 0;
+  if (tid < TR / 32) {
+#pragma unroll
+    for (int b = 0; b < 32; b++) word |= (uint32_t)s_keep[tid * 32 + b] << b;
+    nz = __builtin_popcount(word);
+    f.bits[rank * (TR / 32) + tid] = word;
+  }
+  uint64_t v[2] = {nz, ex}, tot[2];
+  __shared__ uint64_t s_tmp2[2 * (kBlock / kWave)];
+  block_excl_scan_n<2>(v, s_tmp2, tot);
+  if (tid == 0) {
+    p.tile_sum[rank] = tot[0];
+    p.tile_nz[rank] = tot[0];
+    p.tile_ex[rank] = tot[1];
+    u.row0[rank] = first + j0;
+  }
+}
+
 // Compaction of a fan-out filter's kept rows: one wave per tile reads the
 // tile's TR / 32 bit words and copies each kept row of T to its place in O
 // (the tile's prefix from the two-level scan, then bit order).
 template <int TR>
-__global__ __launch_bounds__(kBlock) void k_compact_bits(StepParams p, const uint32_t *bits) {
+__global__ __launch_bounds__(kBlock) void k_compact_bits(StepParams p, const uint32_t *bits, const uint64_t *row0) {
   static_assert(TR / 32 <= kWave, "one word per lane");
   const uint64_t lt = p.tile0 + blockIdx.x * (uint64_t)(kBlock / kWave) + threadIdx.x / kWave;
   if (lt >= p.ntiles) return;
+  const uint64_t r_base = row0 ? row0[lt] : lt * TR;  // unit-ranked tiles carry their first row
   const uint32_t lane = lane_id();
   const uint32_t w = lane < TR / 32 ? bits[lt * (TR / 32) + lane] : 0u;
   if (!wave_ballot(w != 0)) return;
@@ -1043,7 +1330,7 @@ __global__ __launch_bounds__(kBlock) void k_compact_bits(StepParams p, const uin
   uint64_t o = pre + inc - (uint64_t)__builtin_popcount(w);
   const uint32_t c = p.c;
   for (uint32_t x = w; x; x &= x - 1) {
-    const uint64_t r = lt * TR + (uint64_t)lane * 32 + (uint32_t)__builtin_ctz(x);
+    const uint64_t r = r_base + (uint64_t)lane * 32 + (uint32_t)__builtin_ctz(x);
     for (uint32_t col = 0; col < c; col++) p.O[o * c + col] = p.T[r * c + col];
     o++;
   }

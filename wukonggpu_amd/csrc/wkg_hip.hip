@@ -836,10 +836,11 @@ int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir,
     HIPCHK(this, hipStreamSynchronize(s));
     for (uint64_t i = 0; i < np; i++) h[i] = info[i].y;
   }
-  // distinct lists by offset: ly (offset), lbase (first entry in D); pbase per parent
+  // distinct lists by offset: ly (offset), lbase (first entry in D), ln
+  // (length); per parent its list and that list's first D entry
   std::unordered_map<uint64_t, uint64_t> seen;
   seen.reserve(np * 2);
-  std::vector<uint64_t> lbase, pbase(np);
+  std::vector<uint64_t> lbase, ln, pbase(np), plist(np);
   std::vector<uint32_t> ly;
   uint64_t total = 0;
   for (uint64_t i = 0; i < np; i++) {
@@ -849,25 +850,68 @@ int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir,
       it = seen.emplace(h[i], lbase.size()).first;
       lbase.push_back(total);
       ly.push_back((uint32_t)h[i]);
+      ln.push_back(n);
       total += n;
     }
+    plist[i] = it->second;
     pbase[i] = lbase[it->second];
   }
   if (R < 4 * total) return kNotTaken;
   const uint64_t nl = lbase.size();
+  // parent-aligned units of TRU rows when they average at least 256 rows
+  // (long shared lists), else row tiles of TR rows
+  constexpr int TR = 1024, TRU = 2048;
+  uint64_t nunits = 0;
+  std::vector<uint64_t> ubase(np);
+  for (uint64_t i = 0; i < np; i++) {
+    ubase[i] = nunits;
+    nunits += (ln[plist[i]] + TRU - 1) / TRU;
+  }
+  const bool units = nunits * 256 <= R;
   int rc = ensure_pred(in->lin_pid, s);
   if (rc) return rc;
-  prof_add_bytes("fanout_filters", 1.0);  // a count
-  // device copies of the list tables, the keys, their (offset, degree) under pid
-  constexpr int TR = 1024;
-  const uint64_t ntiles = (R + TR - 1) / TR, nchunks = (ntiles + kScanChunk - 1) / kScanChunk;
-  const uint64_t up_words = np + nl + (nl + 1) / 2;  // pbase | lbase | ly (u32)
-  const uint64_t scan_words = 4 + 3 * ntiles + 3 * nchunks;
-  const uint64_t bit_words = (ntiles * (TR / 32) + 1) / 2;
+  prof_add_bytes("fanout_filters", 1.0);  // counts: taken, and by tile kind
+  prof_add_bytes(units ? "fanout_units" : "fanout_row_tiles", 1.0);
+  const int tr = units ? TRU : TR;
+  const uint64_t ntiles = units ? nunits : (R + TR - 1) / TR, nchunks = (ntiles + kScanChunk - 1) / kScanChunk;
+  // uploads: pbase | lbase | ly (u32) [| ubase | xbase | poff | par (u32)]
+  const uint64_t w_ly = (nl + 1) / 2, w_par = (np + 1) / 2;
+  const uint64_t up_words = np + nl + w_ly + (units ? np + 2 * (nl + 1) + w_par : 0);
+  const uint64_t scan_words = 4 + 3 * ntiles + 3 * nchunks + (units ? ntiles : 0);  // (+ unit first rows)
+  const uint64_t bit_words = (ntiles * (uint64_t)(tr / 32) + 1) / 2;
   uint64_t *buf = (uint64_t *)pool->alloc((up_words + scan_words + bit_words + total) * 8, s, err_slot());
   if (!buf) return WKG_ENOMEM;
+  std::vector<uint64_t> up(up_words, 0);
+  {
+    uint64_t o = 0;
+    memcpy(up.data() + o, pbase.data(), np * 8);
+    o += np;
+    memcpy(up.data() + o, lbase.data(), nl * 8);
+    o += nl;
+    memcpy((uint32_t *)(up.data() + o), ly.data(), nl * 4);
+    o += w_ly;
+    if (units) {
+      memcpy(up.data() + o, ubase.data(), np * 8);
+      o += np;
+      // parents grouped by list (row order within a list); execution bases
+      std::vector<uint64_t> cnt(nl + 1, 0);
+      for (uint64_t i = 0; i < np; i++) cnt[plist[i] + 1]++;
+      uint64_t *xbase = up.data() + o, *poff = xbase + nl + 1;
+      uint32_t *par = (uint32_t *)(poff + nl + 1);
+      poff[0] = 0;
+      xbase[0] = 0;
+      for (uint64_t L = 0; L < nl; L++) {
+        poff[L + 1] = poff[L] + cnt[L + 1];
+        xbase[L + 1] = xbase[L] + cnt[L + 1] * ((ln[L] + TRU - 1) / TRU);
+      }
+      std::vector<uint64_t> fill(poff, poff + nl);
+      for (uint64_t i = 0; i < np; i++) par[fill[plist[i]]++] = (uint32_t)i;
+    }
+  }
   uint64_t *d_pbase = buf, *d_lbase = buf + np;
   uint32_t *d_ly = (uint32_t *)(d_lbase + nl);
+  uint64_t *d_ubase = buf + np + nl + w_ly, *d_xbase = d_ubase + np, *d_poff = d_xbase + nl + 1;
+  uint32_t *d_par = (uint32_t *)(d_poff + nl + 1);
   uint64_t *scr = buf + up_words;
   uint32_t *bits = (uint32_t *)(scr + scan_words);
   uint2 *D = (uint2 *)(scr + scan_words + bit_words);
@@ -880,15 +924,9 @@ int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir,
     pool->release(buf, s);
     return r;
   };
-  {
-    std::vector<uint64_t> up(up_words, 0);
-    memcpy(up.data(), pbase.data(), np * 8);
-    memcpy(up.data() + np, lbase.data(), nl * 8);
-    memcpy((uint32_t *)(up.data() + np + nl), ly.data(), nl * 4);
-    if (hipMemcpyAsync(buf, up.data(), up_words * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-      return done(fail(WKG_EDEVICE, "fan-out filter upload failed"));
-  }
+  if (hipMemcpyAsync(buf, up.data(), up_words * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return done(fail(WKG_EDEVICE, "fan-out filter upload failed"));
   if ((rc = new_table(total, 1, s, &K))) return done(rc);
   const GraphView g = view();
   {
@@ -921,6 +959,7 @@ int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir,
   p.chunk_nz = p.chunk_sum + nchunks;
   p.chunk_ex = p.chunk_nz + nchunks;
   p.ntiles = ntiles;
+  uint64_t *row0 = units ? p.chunk_ex + nchunks : nullptr;
   FanoutFilter f{};
   f.excl = in->lin_excl;
   f.info = in->lin_info;
@@ -931,16 +970,22 @@ int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir,
   f.pc = (uint32_t)in->lin_parent->cols;
   f.ec = mode == kModeK2K ? ec : -1;
   f.bits = bits;
+  FanoutUnits u{d_xbase, d_poff, d_par, d_ubase, nl, row0};
   {
     // per row: a D entry (8 B, mostly cached) and the list it names; the
     // input rows are not read
     ProfScope ps(this, s, "fanout_filter", 8.0 * (double)R);
-    const uint64_t per = std::max<uint64_t>(thread_launch_rows() / TR, 1);
+    const uint64_t per = std::max<uint64_t>(thread_launch_rows() / (uint64_t)tr, 1);
     for (uint64_t t0 = 0; t0 < ntiles; t0 += per) {
       p.tile0 = t0;
       const unsigned grid = (unsigned)std::min<uint64_t>(per, ntiles - t0);
-      if (sorted) k_fanout_filter<TR, true><<<grid, kBlock, 0, s>>>(g, p, f);
-      else k_fanout_filter<TR, false><<<grid, kBlock, 0, s>>>(g, p, f);
+      if (units) {
+        if (sorted) k_fanout_units<TRU, true><<<grid, kBlock, 0, s>>>(g, p, f, u);
+        else k_fanout_units<TRU, false><<<grid, kBlock, 0, s>>>(g, p, f, u);
+      } else {
+        if (sorted) k_fanout_filter<TR, true><<<grid, kBlock, 0, s>>>(g, p, f);
+        else k_fanout_filter<TR, false><<<grid, kBlock, 0, s>>>(g, p, f);
+      }
     }
     p.tile0 = 0;
   }
@@ -958,11 +1003,12 @@ int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir,
   if (kept) {
     p.O = t->d;
     ProfScope ps(this, s, "compact", 8.0 * (double)kept * c);
-    const uint64_t per = std::max<uint64_t>(thread_launch_rows() / TR, 4) & ~uint64_t(3);
+    const uint64_t per = std::max<uint64_t>(thread_launch_rows() / (uint64_t)tr, 4) & ~uint64_t(3);
     for (uint64_t t0 = 0; t0 < ntiles; t0 += per) {
       p.tile0 = t0;
-      const uint64_t n = std::min<uint64_t>(per, ntiles - t0);
-      k_compact_bits<TR><<<(unsigned)((n + kBlock / kWave - 1) / (kBlock / kWave)), kBlock, 0, s>>>(p, bits);
+      const unsigned grid = (unsigned)((std::min<uint64_t>(per, ntiles - t0) + kBlock / kWave - 1) / (kBlock / kWave));
+      if (units) k_compact_bits<TRU><<<grid, kBlock, 0, s>>>(p, bits, row0);
+      else k_compact_bits<TR><<<grid, kBlock, 0, s>>>(p, bits, nullptr);
     }
     if ((le = hipGetLastError()) != hipSuccess)
       return done(fail(WKG_EDEVICE, "fan-out compaction launch failed: %s", hipGetErrorString(le)));
@@ -1376,6 +1422,7 @@ int wkg_destroy(wkg_ctx *ctx) {
   if (!ctx) return WKG_EINVAL;
   (void)hipSetDevice(ctx->device);
   (void)hipDeviceSynchronize();
+  if (ctx->sq_mem) ctx->pool->release(ctx->sq_mem, ctx->stream);
   ctx->free_graph();
   delete ctx->pool;
   if (ctx->h_rb) (void)hipHostFree(ctx->h_rb);
@@ -1516,6 +1563,22 @@ int wkg_table_slice(wkg_ctx *ctx, const wkg_table *t, uint64_t row0, uint64_t nr
   return WKG_OK;
 }
 
+int wkg_table_wrap_device(wkg_ctx *ctx, const int32_t *cells, uint64_t rows, int cols, wkg_table **out) {
+  if (!ctx || !out) return WKG_EINVAL;
+  if (cols < 1 || cols > kBlock) return ctx->fail(WKG_EINVAL, "a table has 1..%d columns (got %d)", kBlock, cols);
+  if (rows && !cells) return ctx->fail(WKG_EINVAL, "null cells for a %llu-row table", (unsigned long long)rows);
+  wkg_table *v = new (std::nothrow) wkg_table;
+  if (!v) return ctx->fail(WKG_ENOMEM, "out of host memory");
+  v->d = const_cast<int32_t *>(cells);
+  v->rows = rows;
+  v->cols = cols;
+  v->bytes = rows * (uint64_t)cols * sizeof(int32_t);
+  v->stream = ctx->stream;
+  v->owned = false;  // (and no owner: steps keep no lineage on it)
+  *out = v;
+  return WKG_OK;
+}
+
 int wkg_table_free(wkg_ctx *ctx, wkg_table *t) {
   if (!ctx) return WKG_EINVAL;
   ctx->drop_table(t);
@@ -1609,6 +1672,81 @@ int wkg_step_k2u_noalloc(wkg_ctx *ctx, const int32_t *in, uint64_t rows, int col
   rc = ctx->expand_pass(&tin, pid, pp.cinfo, pp.cexcl, nz, E, s, &tout, "expand", (uint64_t *)(base + pass_b + lin_b));
   ctx->release_pass(pp, s);
   return rc;
+}
+
+int wkg_query_small(wkg_ctx *ctx, const wkg_plan_step *steps, int n, void *stream, const int32_t **cells,
+                    uint64_t *rows, int *cols, uint64_t *expanded) {
+  if (!ctx || !steps || !cells || !rows || !cols) return WKG_EINVAL;
+  *cells = nullptr;
+  *rows = 0;
+  *cols = 0;
+  if (expanded) *expanded = 0;
+  if (!ctx->loaded) return ctx->fail(WKG_ESTATE, "no graph loaded");
+  if (n < 1 || n > (int)kSQSteps) return ctx->fail(WKG_ERANGE, "a small query has 1..%u steps", kSQSteps);
+  if (ctx->framed) return ctx->fail(WKG_ERANGE, "small queries run on resident graphs");
+  int c = 0;  // the plan's column bookkeeping, as the executor checks it
+  for (int l = 0; l < n; l++) {
+    const wkg_plan_step &d = steps[l];
+    if (int rc = ctx->check_pattern(d.pid, d.dir)) return rc;
+    const bool start = d.kind == WKG_STEP_I2U || d.kind == WKG_STEP_C2U;
+    if (start != (l == 0)) return ctx->fail(WKG_EINVAL, "step %d: an index or constant start only at step 0", l);
+    if (d.kind == WKG_STEP_I2U && !is_tpid(d.pid)) return ctx->fail(WKG_EINVAL, "index id %d is not a predicate/type id", d.pid);
+    if (d.kind == WKG_STEP_C2U && (d.vid < 0 || (uint64_t)d.vid > kMaxVid))
+      return ctx->fail(WKG_EINVAL, "constant %lld outside the vertex id range", (long long)d.vid);
+    if (start) {
+      c = 1;
+      continue;
+    }
+    if (d.kind != WKG_STEP_K2U && d.kind != WKG_STEP_K2K && d.kind != WKG_STEP_K2C)
+      return ctx->fail(WKG_EINVAL, "step %d: unknown kind %d", l, d.kind);
+    if (d.start_col < 0 || d.start_col >= c) return ctx->fail(WKG_EINVAL, "step %d: start column %d outside %d", l, d.start_col, c);
+    if (d.kind == WKG_STEP_K2K && (d.end_col < 0 || d.end_col >= c))
+      return ctx->fail(WKG_EINVAL, "step %d: end column %d outside %d", l, d.end_col, c);
+    if (d.kind == WKG_STEP_K2U) c++;
+  }
+  hipStream_t s = ctx->pick(stream);
+  const uint64_t tbl = (kSQCells * 4 + 255) & ~uint64_t(255);
+  if (!ctx->sq_mem) {
+    ctx->sq_mem = ctx->pool->alloc(2 * tbl + kSQScratchWords * 8, s, wkg_ctx::err_slot());
+    if (!ctx->sq_mem) return WKG_ENOMEM;
+  }
+  SmallQuery q{};
+  for (int l = 0; l < n; l++) q.steps[l] = steps[l];
+  q.nsteps = (uint32_t)n;
+  q.buf[0] = (int32_t *)ctx->sq_mem;
+  q.buf[1] = (int32_t *)((char *)ctx->sq_mem + tbl);
+  q.scr = (uint64_t *)((char *)ctx->sq_mem + 2 * tbl);
+  q.stage = (int32_t *)((char *)ctx->h_rb_dev + kStageOff);
+  q.stage_cells = kStageBytes / 4;
+  const GraphView g = ctx->view();
+  const Signal sg = ctx->next_signal();
+  {
+    ProfScope ps(ctx, s, "small_query", 0.0);
+    if (ctx->sorted) k_small_query<true, false><<<1, kBlock, 0, s>>>(g, q, sg);
+    else k_small_query<false, false><<<1, kBlock, 0, s>>>(g, q, sg);
+  }
+  hipError_t le = hipGetLastError();
+  if (le != hipSuccess) return ctx->fail(WKG_EDEVICE, "small query launch failed: %s", hipGetErrorString(le));
+  if (int rc = ctx->wait_signal(sg, s)) return rc;
+  const uint64_t status = ctx->h_rb[0], r = ctx->h_rb[1], cc = ctx->h_rb[2], ex = ctx->h_rb[3], cur = ctx->h_rb[4];
+  if (status == 1) {
+    ctx->prof_add_bytes("small_query_declined", 1.0);  // a count
+    return ctx->fail(WKG_ERANGE, "a table outgrew the small-query limits (%llu input rows, %llu cells)",
+                     (unsigned long long)kSQRows, (unsigned long long)kSQCells);
+  }
+  ctx->prof_add_bytes("small_query_taken", 1.0);
+  *rows = r;
+  *cols = (int)cc;
+  if (expanded) *expanded = ex;
+  if (status == 0) {
+    *cells = (const int32_t *)((const char *)ctx->h_rb + kStageOff);
+  } else {  // in device scratch: a copy command
+    ctx->sq_host.resize(r * cc);
+    HIPCHK(ctx, hipMemcpyAsync(ctx->sq_host.data(), q.buf[cur], r * cc * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    *cells = ctx->sq_host.data();
+  }
+  return WKG_OK;
 }
 
 int wkg_step_k2k(wkg_ctx *ctx, const wkg_table *in, int start_col, int pid, int dir, int end_col,

@@ -176,6 +176,18 @@ class Device:
                                                      a.shape[0], a.shape[1], None, C.byref(h)))
         return Table(self, h.value)
 
+    def wrap_device(self, t, rows=None, cols=None):
+        """A Table viewing the device cells of table t (or its first rows x
+        cols cells) without owning them (wkg_table_wrap_device)."""
+        L = N.hip_lib()
+        r0, c0 = t.shape
+        h = C.c_void_p()
+        self._check(L.wkg_table_wrap_device(self._h, L.wkg_table_device_ptr(t._h), r0 if rows is None else rows,
+                                            c0 if cols is None else cols, C.byref(h)))
+        v = Table(self, h.value)
+        v._parent = t  # the view's memory is t's
+        return v
+
     def table_split(self, t, col, nparts):
         """Fork-join split of a device table by the owner (cell % nparts) of
         column `col`: (Table holding the sub-tables back to back, rows per
@@ -365,6 +377,10 @@ class GPUEngine:
     def run_rounds(self, queries):
         """Step-synchronous batching: one fused launch set per step round."""
         return self._many(N.engine_lib().wke_run_rounds, queries)
+
+    def set_small_queries(self, on):
+        """Try a whole small query in one launch before the step-by-step path (default on)."""
+        N.engine_lib().wke_set_small_queries(self._h, 1 if on else 0)
 
     def splits(self):
         """Row-half splits made so far by the plain path's HBM-budgeted chunking."""

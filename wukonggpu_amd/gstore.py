@@ -70,6 +70,18 @@ class Triples:
         _check(N.gstore_lib().wks_triples_read_file(str(path).encode(), C.byref(h)), "read triples")
         return cls(h.value)
 
+    @classmethod
+    def read_dataset(cls, path):
+        """A dataset directory in the reference's ID format (id_* files,
+        str_index, str_normal): (Triples, info dict)."""
+        h = C.c_void_p()
+        info = N.DatasetInfo()
+        _check(N.gstore_lib().wks_dataset_read(str(path).encode(), C.byref(h), C.byref(info)), "read dataset")
+        return cls(h.value), {k: int(getattr(info, k)) for k, _ in N.DatasetInfo._fields_}
+
+    def write_dataset(self, path, files=1):
+        _check(N.gstore_lib().wks_dataset_write(self._h, str(path).encode(), int(files)), "write dataset")
+
     def write(self, path):
         _check(N.gstore_lib().wks_triples_write_file(self._h, str(path).encode()), "write triples")
 
