@@ -1199,9 +1199,11 @@ template <int TR, bool SORTED>
 __global__ __launch_bounds__(kBlock) void k_fanout_units(GraphView g, StepParams p, FanoutFilter f, FanoutUnits u) {
   constexpr int RPT = TR / kBlock;
   constexpr uint32_t kLaneList = 8;
-  static_assert(TR % kBlock == 0 && (TR / 32) <= kBlock, "tile rows");
+  static_assert(RPT == 8 && (TR / 32) <= kBlock, "a thread's rows are one byte of kept bits");
   __shared__ uint32_t s_off[TR], s_deg[TR];
-  __shared__ uint8_t s_keep[TR], s_big[TR];
+  __shared__ uint8_t s_keep[TR];             // long lists' decisions (phase 2)
+  __shared__ uint8_t s_bigb[kBlock];         // per thread: which of its rows have long lists
+  __shared__ __align__(4) uint8_t s_kb[kBlock];  // per thread: its rows' kept bits
   __shared__ uint32_t s_any;
   const uint32_t tid = threadIdx.x, lane = lane_id(), gbase = lane & ~7u, l8 = lane & 7;
   const uint64_t e = p.tile0 + blockIdx.x;
@@ -1241,42 +1243,39 @@ __global__ __launch_bounds__(kBlock) void k_fanout_units(GraphView g, StepParams
   }
   __syncthreads();
   uint64_t ex = 0;
-  bool big_any = false;
+  uint32_t kb = 0, bigb = 0;  // this thread's rows: kept bits, long-list bits
 #pragma unroll
   for (int k = 0; k < RPT; k++) {
     const uint32_t local = tid * RPT + k;
     const uint32_t off = d[k].x, deg = d[k].y;
-    uint8_t keep = 0, big = 0;
     if (valid && deg > kLaneList) {
-      big = 1;
+      bigb |= 1u << k;
       s_off[local] = off;
       s_deg[local] = deg;
     } else if (deg == 1) {  // D holds the list's only entry
-      keep = valid && off == tgt ? 1 : 0;
+      kb |= (valid && off == tgt ? 1u : 0u) << k;
       ex += 1;
     } else {
       uint32_t x = deg;
       if (valid)
         for (uint32_t j = 0; j < deg; j++)
           if (edge_at<false>(g, m.edge_start, m.eframe_base, (uint64_t)off + j) == tgt) {
-            keep = 1;
+            kb |= 1u << k;
             x = j + 1;
             break;
           }
       ex += x;
     }
-    s_keep[local] = keep;
-    s_big[local] = big;
-    big_any |= big != 0;
   }
-  if (big_any) s_any = 1;
+  s_bigb[tid] = (uint8_t)bigb;
+  if (bigb) s_any = 1;
   __syncthreads();
   if (s_any) {  // long lists: the 8-lane membership test, 64 rows per wave per round
     constexpr uint32_t kWaveRows = TR / (kBlock / kWave);
     const uint32_t wave = tid / kWave;
     for (int rd = 0; rd < TR / kBlock; rd++) {
       const uint32_t local = wave * kWaveRows + rd * kWave + lane;
-      const bool big = s_big[local] != 0;
+      const bool big = (s_bigb[local / RPT] >> (local % RPT)) & 1u;
       const uint32_t off = big ? s_off[local] : 0u, deg = big ? s_deg[local] : 0u;
       if (!wave_ballot(big)) continue;
 #pragma unroll
@@ -1293,12 +1292,16 @@ __global__ __launch_bounds__(kBlock) void k_fanout_units(GraphView g, StepParams
         }
       }
     }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < RPT; k++)
+      if (bigb >> k & 1u) kb |= (uint32_t)s_keep[tid * RPT + k] << k;
   }
+  s_kb[tid] = (uint8_t)kb;
   __syncthreads();
   uint32_t word = 0, nz = 0;
-  if (tid < TR / 32) {
-#pragma unroll
-    for (int b = 0; b < 32; b++) word |= (uint32_t)s_keep[tid * 32 + b] << b;
+  if (tid < TR / 32) {  // rows 32 tid .. 32 tid + 31: four threads' bytes
+    word = ((const uint32_t *)s_kb)[tid];
     nz = __builtin_popcount(word);
     f.bits[rank * (TR / 32) + tid] = word;
   }
