@@ -314,8 +314,9 @@ typedef struct wkg_plan_step {
 /* ---- a whole small query in one launch --------------------------------------------------
  * Runs every step of a compiled plan (wkg_plan_step, above; at most 8 steps,
  * the first an index or constant start) in one launch and one read-back,
- * while every table stays within 4096 input rows per step and 64K cells;
- * the result is the per-step calls' table, cell for cell. The reference
+ * while every table stays within 4096 input rows per step and 8192 cells
+ * (one workgroup holding the tables in LDS); the result is the per-step
+ * calls' table, cell for cell. The reference
  * runs a query's steps one handler at a time (GPU_Engine::execute_request,
  * core/gpu_engine.hpp:471-526); for a query this small each step costs
  * launch and read-back latency, not work. Returns WKG_ERANGE when a table

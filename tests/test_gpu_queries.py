@@ -467,17 +467,19 @@ def test_dataset_directory_and_persisted_image(tmp_path):
 
 def test_small_queries_in_one_launch(graph_hubs):
     """Index- and constant-start queries whose tables stay within 4096 input
-    rows a step and 64K cells run all their steps in one launch
-    (wkg_query_small); larger ones are declined and run step by step. The
-    answers equal the oracle's either way, including a result too big for
-    the pinned staging area (16K cells) and filters, empty tables and hub
-    lists inside the limits."""
+    rows a step and 8192 cells run all their steps in one launch
+    (wkg_query_small, one workgroup with the tables in LDS); larger ones are
+    declined and run step by step. The answers equal the oracle's either
+    way, including filters over 3000 rows (three probe rounds a wave),
+    empty tables and hub lists inside the limits."""
     _, gs = graph_hubs
     dev = Device(0)
     dev.load(gs)
     eng = GPUEngine(dev)
     hub = max(gs.get_edges(0, 2, 0).tolist(), key=lambda v: gs.degree(v, 2, 1))
-    qs = [[5, 1, 0, -1, -1, 2, 1, -2],                        # 3000 members -> ~10k x 2 (past the staging area)
+    qs = [[5, 1, 0, -1, -1, 2, 1, -2],                        # 3000 members -> ~10k x 2: declined
+          [5, 1, 0, -1, -1, 2, 1, VB + 7],                     # 3000 rows, known_to_const
+          [5, 1, 0, -1, -1, 3, 0, -1],                         # 3000 rows, known_to_known on one column
           [7, 1, 0, -1, -1, 2, 1, -2, -2, 3, 0, -3],          # a chain
           [7, 1, 0, -1, -1, 2, 1, -2, -1, 3, 1, -3, -3, 2, 0, -1],  # a star, closed
           [VB + 1, 2, 1, -1, -1, 3, 1, VB + 7],                # constant start, known_to_const

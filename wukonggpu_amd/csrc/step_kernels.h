@@ -610,64 +610,90 @@ __global__ __launch_bounds__(kBlock) void k_small_step(GraphView g, StepParams p
 }
 
 // ---- a whole small query in one launch ----
-// Every step of a query whose tables stay small runs in one workgroup with
-// the per-step path's own tile bodies (probe, tile scan, finalize,
-// merge-path partition, expand), between workgroup barriers, on two
-// ping-pong tables in device scratch; the last table goes to the pinned
-// staging area and the results are published with one flag. A table past
-// the caps ends the launch early with status 1 (the caller then runs the
-// query step by step), so the result is always the per-step path's.
-constexpr uint32_t kSQSteps = 8;                 // patterns
-constexpr uint64_t kSQRows = 16 * kBlock;        // input rows of a step (16 probe tiles)
-constexpr uint64_t kSQCells = 1u << 16;          // cells of a table (64K: 256 KB)
+// Every step of a query whose tables stay small runs in one 1024-thread
+// workgroup with every table in LDS (two ping-pong tables of kSQCells
+// cells), so the only global accesses are the graph's buckets and lists and
+// the final copy of the result into the pinned staging area:
+//  - an index / constant start: wave 0 probes the key, the workgroup copies
+//    its list into the table;
+//  - a known-start step: wave w probes rows [64 w r, 64 (w + 1) r) of the
+//    input (r = rounds of 64 rows, at most 4), filters test membership with
+//    the 8-lane group test, then a workgroup scan of the degrees (kept
+//    flags) gives every row its first output row; known_to_unknown writes
+//    output row k = (the last row whose first output row is <= k, its list
+//    entry k - that first row), filters copy the kept rows in order. Output
+//    order is the reference's (core/engine.hpp:126-194).
+// A table past the caps ends the launch early with status 1 (the caller
+// then runs the query step by step), so the result is always the per-step
+// path's. Status and counters go out with one published flag.
+constexpr int kSQThreads = 1024;
+constexpr int kSQWaves = kSQThreads / kWave;
+constexpr uint32_t kSQSteps = 8;        // patterns
+constexpr uint32_t kSQRows = 4096;      // input rows of a known-start step
+constexpr uint32_t kSQCells = 8192;     // cells of a table (32 KB)
+static_assert(kSQCells * 4 <= kStageBytes, "a small query's result fits the staging area");
 struct SmallQuery {
   wkg_plan_step steps[kSQSteps];
+  DevMeta meta[kSQSteps];  // each step's predicate metadata (passed in, not loaded)
   uint32_t nsteps;
-  int32_t *buf[2];     // ping-pong tables, kSQCells cells each
-  uint64_t *scr;       // step scratch (SQ_SCRATCH_WORDS)
-  int32_t *stage;      // pinned staging area (device address), stage_cells cells
-  uint64_t stage_cells;
+  int32_t *stage;  // pinned staging area (device address), kSQCells cells
 };
-constexpr uint64_t kSQTiles = kSQRows / kBlock;
-// scratch words: totals 4 | tile sums 3 x tiles | chunk sums 3 | hits (16 B a row, 16-byte aligned) | cinfo (8 B a
-// row) | cexcl (8 B a row) | merge-path split points
-constexpr uint64_t kSQOffHits = (4 + 3 * kSQTiles + 3 + 1) & ~uint64_t(1);
-constexpr uint64_t kSQOffCinfo = kSQOffHits + 2 * kSQRows;
-constexpr uint64_t kSQOffCexcl = kSQOffCinfo + kSQRows;
-constexpr uint64_t kSQOffPart = kSQOffCexcl + kSQRows;
-constexpr uint64_t kSQScratchWords = kSQOffPart + (kSQRows + kSQCells) / kExpandNV + 2;
+
+// Exclusive u64 scan over the kSQThreads threads; `total` gets the sum.
+__device__ __forceinline__ uint64_t sq_block_scan(uint64_t v, uint64_t *tmp, uint64_t &total) {
+  const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+  const uint64_t inc = wave_incl_scan(v);
+  if (lane == kWave - 1) tmp[wave] = inc;
+  __syncthreads();
+  uint64_t pre = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kSQWaves; w++) {
+    const uint64_t t = tmp[w];
+    if ((uint32_t)w < wave) pre += t;
+    tot += t;
+  }
+  __syncthreads();
+  total = tot;
+  return pre + inc - v;
+}
 
 template <bool SORTED, bool FRAMED>
-__global__ __launch_bounds__(kBlock) void k_small_query(GraphView g, SmallQuery q, Signal sg) {
-  __shared__ WaveProbeLds s_wp;
-  __shared__ uint32_t s_deg, s_off;
-  uint64_t rows = 0, expanded = 0, status = 0;
-  uint32_t cols = 0, cur = 0;
-  for (uint32_t l = 0; l < q.nsteps && status == 0; l++) {
+__global__ __launch_bounds__(kSQThreads) void k_small_query(GraphView g, SmallQuery q, Signal sg) {
+  __shared__ WaveProbeLds s_wp[kSQWaves];
+  __shared__ uint32_t s_deg[kSQRows];  // degree (known_to_unknown) or kept flag (filters)
+  __shared__ uint32_t s_off[kSQRows];  // list offset (known_to_unknown)
+  __shared__ uint32_t s_pos[kSQRows];  // first output row
+  __shared__ int32_t s_tab[2][kSQCells];
+  __shared__ uint64_t s_tmp[2][kSQWaves];
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave, l8 = lane & 7, gbase = lane & ~7u;
+  uint64_t expanded = 0, status = 0;
+  uint32_t rows = 0, cols = 0, cur = 0;
+  for (uint32_t l = 0; l < q.nsteps; l++) {
     const wkg_plan_step d = q.steps[l];
-    int32_t *in = q.buf[cur], *out = q.buf[cur ^ 1];
+    const int32_t *in = s_tab[cur];
+    int32_t *out = s_tab[cur ^ 1];
+    const DevMeta m = q.meta[l];
     if (d.kind == WKG_STEP_I2U || d.kind == WKG_STEP_C2U) {  // one key: its list is the table
       const uint64_t key = d.kind == WKG_STEP_I2U ? make_key(0, (uint64_t)d.pid, (uint64_t)d.dir)
                                                   : make_key((uint64_t)d.vid, (uint64_t)d.pid, (uint64_t)d.dir);
-      const DevMeta m = g.meta[d.pid];
-      if (threadIdx.x < kWave) {
+      if (wave == 0) {
         uint32_t deg, off;
         ProbeCarry carry;
-        wave_probe<FRAMED>(g, m, threadIdx.x == 0 ? key : 0, s_wp, deg, off, carry);
-        if (threadIdx.x == 0) {
-          s_deg = deg;
-          s_off = off;
+        wave_probe<FRAMED>(g, m, lane == 0 ? key : 0, s_wp[0], deg, off, carry);
+        if (lane == 0) {
+          s_deg[0] = deg;
+          s_off[0] = off;
         }
       }
       __syncthreads();
-      const uint64_t deg = s_deg, off = s_off;
+      const uint32_t deg = s_deg[0], off = s_off[0];
       expanded += deg;
-      if (deg > kSQRows) {
+      if (deg > kSQCells) {
         status = 1;
         break;
       }
-      for (uint64_t j = threadIdx.x; j < deg; j += kBlock)
-        out[j] = (int32_t)edge_at<FRAMED>(g, m.edge_start, m.eframe_base, off + j);
+      for (uint32_t j = tid; j < deg; j += kSQThreads)
+        out[j] = (int32_t)edge_at<FRAMED>(g, m.edge_start, m.eframe_base, (uint64_t)off + j);
       rows = deg;
       cols = 1;
     } else {
@@ -676,87 +702,113 @@ __global__ __launch_bounds__(kBlock) void k_small_query(GraphView g, SmallQuery 
         break;
       }
       const int mode = d.kind == WKG_STEP_K2U ? kModeK2U : d.kind == WKG_STEP_K2K ? kModeK2K : kModeK2C;
-      StepParams p{};
-      p.T = in;
-      p.R = rows;
-      p.c = cols;
-      p.sc = (uint32_t)d.start_col;
-      p.ec = mode == kModeK2K ? (uint32_t)d.end_col : 0u;
-      p.pid = (uint32_t)d.pid;
-      p.dir = (uint32_t)d.dir;
-      p.target = (uint32_t)d.vid;
-      p.target_ok = (mode == kModeK2C && d.vid >= 0 && d.vid <= 0xffffffffLL) ? 1u : 0u;
-      p.mode = mode;
-      p.ntiles = (rows + kBlock - 1) / kBlock;
-      p.totals = q.scr;
-      p.tile_sum = q.scr + 4;
-      p.tile_nz = p.tile_sum + kSQTiles;
-      p.tile_ex = p.tile_nz + kSQTiles;
-      p.chunk_sum = p.tile_ex + kSQTiles;
-      p.chunk_nz = p.chunk_sum + 1;
-      p.chunk_ex = p.chunk_nz + 1;
-      p.hits = (uint4 *)(q.scr + kSQOffHits);
-      p.cinfo = (uint2 *)(q.scr + kSQOffCinfo);
-      p.cexcl = q.scr + kSQOffCexcl;
-      p.O = mode == kModeK2U ? nullptr : out;
-      for (uint64_t lt = 0; lt < p.ntiles; lt++) {
-        if (mode == kModeK2U) probe_tile<kBlock, true, FRAMED>(g, p, lt);
-        else probe_tile<kBlock, SORTED, FRAMED>(g, p, lt);
-        __syncthreads();  // the tile's LDS is reused by the next
-      }
-      if (p.ntiles == 0 && threadIdx.x == 0) {  // an empty input: empty output, no work
-        p.totals[0] = p.totals[1] = p.totals[2] = 0;
-      } else {
-        scan_chunk(p, 0, 1);
+      const uint32_t sc = (uint32_t)d.start_col, ec = mode == kModeK2K ? (uint32_t)d.end_col : 0u;
+      const bool target_ok = mode == kModeK2C && d.vid >= 0 && d.vid <= 0xffffffffLL;
+      const uint32_t rounds = (rows + kSQThreads - 1) / kSQThreads;  // 64-row rounds of a wave
+      ProbeCarry carry;
+      uint64_t ex = 0;  // filters: edges examined by this lane's rows
+      for (uint32_t rd = 0; rd < rounds; rd++) {
+        const uint32_t row = (wave * rounds + rd) * kWave + lane;
+        uint64_t key = 0;
+        int32_t end_cell = -1;
+        if (row < rows) {
+          const int32_t cell = in[row * cols + sc];
+          if (cell >= 0) key = make_key((uint32_t)cell, (uint64_t)d.pid, (uint64_t)d.dir);
+          if (mode == kModeK2K) end_cell = in[row * cols + ec];
+        }
+        uint32_t deg, off;
+        wave_probe<FRAMED>(g, m, key, s_wp[wave], deg, off, carry);
+        if (mode == kModeK2U) {
+          if (row < rows) {
+            s_deg[row] = deg;
+            s_off[row] = off;
+          }
+        } else {
+          // the 8 lanes of a group test the group's 8 rows one after another
+          uint32_t my_found = 0, my_ex = 0;
+#pragma unroll 1
+          for (int u = 0; u < 8; u++) {
+            const uint32_t src = gbase + u;
+            const uint32_t du = shfl32(deg, src), ou = shfl32(off, src);
+            uint32_t target = (uint32_t)d.vid;
+            bool valid = target_ok;
+            if (mode == kModeK2K) {
+              const int32_t e = (int32_t)shfl32((uint32_t)end_cell, src);
+              target = (uint32_t)e;
+              valid = e >= 0;
+            }
+            bool found;
+            uint32_t examined;
+            group_member<SORTED, FRAMED>(g, m.edge_start, m.eframe_base, ou, du, target, valid, found, examined);
+            if (l8 == (uint32_t)u) {
+              my_found = found ? 1u : 0u;
+              my_ex = examined;
+            }
+          }
+          if (row < rows) s_deg[row] = my_found;
+          ex += my_ex;
+        }
       }
       __syncthreads();
-      for (uint64_t lt = threadIdx.x / kWave; lt < p.ntiles; lt += kBlock / kWave) finalize_tile<kBlock>(p, lt);
+      // thread t owns rows [t r, t r + r) of the scan
+      uint64_t cnt = 0;
+      for (uint32_t k = 0; k < rounds; k++) {
+        const uint32_t row = tid * rounds + k;
+        if (row < rows) cnt += s_deg[row];
+      }
+      uint64_t total, ex_total = 0;
+      uint64_t pre = sq_block_scan(cnt, s_tmp[0], total);
+      if (mode != kModeK2U) (void)sq_block_scan(ex, s_tmp[1], ex_total);
+      for (uint32_t k = 0; k < rounds; k++) {
+        const uint32_t row = tid * rounds + k;
+        if (row < rows) {
+          s_pos[row] = (uint32_t)pre;  // (a total past the caps declines below before any use)
+          pre += s_deg[row];
+        }
+      }
       __syncthreads();
       if (mode == kModeK2U) {
-        const uint64_t E = p.totals[0], nz = p.totals[2];
-        expanded += E;
-        if (E * (uint64_t)(cols + 1) > kSQCells) {
+        expanded += total;
+        const uint32_t nc = cols + 1;
+        if (total * nc > kSQCells) {
           status = 1;
           break;
         }
-        if (E) {
-          uint64_t *part = q.scr + kSQOffPart;
-          const uint64_t nx = (nz + E + kExpandNV - 1) / kExpandNV;
-          for (uint64_t k = threadIdx.x; k <= nx; k += kBlock) part[k] = merge_split<kExpandNV>(p.cexcl, nz, E, k);
-          __syncthreads();
-          ExpandParams x{};
-          x.pid = p.pid;
-          x.T = in;
-          x.c = cols;
-          x.R = nz;
-          x.cinfo = p.cinfo;
-          x.cexcl = p.cexcl;
-          x.part = part;
-          x.E = E;
-          x.O = out;
-          for (uint64_t t = 0; t < nx; t++) expand_tile<kExpandVT, FRAMED>(g, x, t);
+        const uint32_t E = (uint32_t)total;
+        for (uint32_t r = tid; r < E; r += kSQThreads) {
+          uint32_t lo = 0, hi = rows - 1;  // the last row whose first output row is <= r
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (s_pos[mid] <= r) lo = mid;
+            else hi = mid - 1;
+          }
+          const uint32_t v = edge_at<FRAMED>(g, m.edge_start, m.eframe_base, (uint64_t)s_off[lo] + (r - s_pos[lo]));
+          int32_t *o = out + r * nc;
+          for (uint32_t j = 0; j < cols; j++) o[j] = in[lo * cols + j];
+          o[cols] = (int32_t)v;
         }
         rows = E;
-        cols += 1;
+        cols = nc;
       } else {
-        rows = p.totals[0];
-        expanded += p.totals[1];
+        expanded += ex_total;
+        for (uint32_t row = tid; row < rows; row += kSQThreads)
+          if (s_deg[row])
+            for (uint32_t j = 0; j < cols; j++) out[s_pos[row] * cols + j] = in[row * cols + j];
+        rows = (uint32_t)total;
       }
     }
     __syncthreads();  // the output is complete before it is the next step's input
     cur ^= 1;
   }
-  // the last table to the host staging area when it fits there (status 2:
-  // it stays in buf[cur] for a copy command)
-  const uint64_t cells = rows * (uint64_t)cols;
-  if (status == 0 && cells > q.stage_cells) status = 2;
-  if (status == 0)
-    for (uint64_t i = threadIdx.x; i < cells; i += kBlock) q.stage[i] = q.buf[cur][i];
+  if (status == 0) {
+    const uint32_t cells = rows * cols;
+    for (uint32_t i = tid; i < cells; i += kSQThreads) q.stage[i] = s_tab[cur][i];
+  }
   __threadfence_system();
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint64_t v[5] = {status, rows, cols, expanded, cur};
-    publish(sg, v, 5);
+  if (tid == 0) {
+    const uint64_t v[4] = {status, rows, cols, expanded};
+    publish(sg, v, 4);
   }
 }
 

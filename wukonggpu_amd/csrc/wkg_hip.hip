@@ -1422,7 +1422,6 @@ int wkg_destroy(wkg_ctx *ctx) {
   if (!ctx) return WKG_EINVAL;
   (void)hipSetDevice(ctx->device);
   (void)hipDeviceSynchronize();
-  if (ctx->sq_mem) ctx->pool->release(ctx->sq_mem, ctx->stream);
   ctx->free_graph();
   delete ctx->pool;
   if (ctx->h_rb) (void)hipHostFree(ctx->h_rb);
@@ -1705,47 +1704,34 @@ int wkg_query_small(wkg_ctx *ctx, const wkg_plan_step *steps, int n, void *strea
     if (d.kind == WKG_STEP_K2U) c++;
   }
   hipStream_t s = ctx->pick(stream);
-  const uint64_t tbl = (kSQCells * 4 + 255) & ~uint64_t(255);
-  if (!ctx->sq_mem) {
-    ctx->sq_mem = ctx->pool->alloc(2 * tbl + kSQScratchWords * 8, s, wkg_ctx::err_slot());
-    if (!ctx->sq_mem) return WKG_ENOMEM;
-  }
   SmallQuery q{};
-  for (int l = 0; l < n; l++) q.steps[l] = steps[l];
+  for (int l = 0; l < n; l++) {
+    q.steps[l] = steps[l];
+    q.meta[l] = ctx->h_meta[steps[l].pid];
+  }
   q.nsteps = (uint32_t)n;
-  q.buf[0] = (int32_t *)ctx->sq_mem;
-  q.buf[1] = (int32_t *)((char *)ctx->sq_mem + tbl);
-  q.scr = (uint64_t *)((char *)ctx->sq_mem + 2 * tbl);
   q.stage = (int32_t *)((char *)ctx->h_rb_dev + kStageOff);
-  q.stage_cells = kStageBytes / 4;
   const GraphView g = ctx->view();
   const Signal sg = ctx->next_signal();
   {
     ProfScope ps(ctx, s, "small_query", 0.0);
-    if (ctx->sorted) k_small_query<true, false><<<1, kBlock, 0, s>>>(g, q, sg);
-    else k_small_query<false, false><<<1, kBlock, 0, s>>>(g, q, sg);
+    if (ctx->sorted) k_small_query<true, false><<<1, kSQThreads, 0, s>>>(g, q, sg);
+    else k_small_query<false, false><<<1, kSQThreads, 0, s>>>(g, q, sg);
   }
   hipError_t le = hipGetLastError();
   if (le != hipSuccess) return ctx->fail(WKG_EDEVICE, "small query launch failed: %s", hipGetErrorString(le));
   if (int rc = ctx->wait_signal(sg, s)) return rc;
-  const uint64_t status = ctx->h_rb[0], r = ctx->h_rb[1], cc = ctx->h_rb[2], ex = ctx->h_rb[3], cur = ctx->h_rb[4];
-  if (status == 1) {
+  const uint64_t status = ctx->h_rb[0], r = ctx->h_rb[1], cc = ctx->h_rb[2], ex = ctx->h_rb[3];
+  if (status != 0) {
     ctx->prof_add_bytes("small_query_declined", 1.0);  // a count
-    return ctx->fail(WKG_ERANGE, "a table outgrew the small-query limits (%llu input rows, %llu cells)",
-                     (unsigned long long)kSQRows, (unsigned long long)kSQCells);
+    return ctx->fail(WKG_ERANGE, "a table outgrew the small-query limits (%u input rows, %u cells)", kSQRows,
+                     kSQCells);
   }
   ctx->prof_add_bytes("small_query_taken", 1.0);
   *rows = r;
   *cols = (int)cc;
   if (expanded) *expanded = ex;
-  if (status == 0) {
-    *cells = (const int32_t *)((const char *)ctx->h_rb + kStageOff);
-  } else {  // in device scratch: a copy command
-    ctx->sq_host.resize(r * cc);
-    HIPCHK(ctx, hipMemcpyAsync(ctx->sq_host.data(), q.buf[cur], r * cc * 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipStreamSynchronize(s));
-    *cells = ctx->sq_host.data();
-  }
+  *cells = (const int32_t *)((const char *)ctx->h_rb + kStageOff);
   return WKG_OK;
 }
 

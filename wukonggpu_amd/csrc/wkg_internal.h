@@ -140,8 +140,6 @@ struct wkg_ctx {
   uint64_t group_filter_rows = 1ull << 22;  // smallest input a filter decides per lineage group
   bool small_steps = true;   // steps over at most kSmallRows rows in one launch
   uint64_t *noalloc_scratch = nullptr;  // this device's no-alloc step scratch (wkg_step_k2u_noalloc)
-  void *sq_mem = nullptr;              // whole-small-query tables and scratch (wkg_query_small), pool memory
-  std::vector<int32_t> sq_host;        // a small query's result too big for the staging area
   uint64_t *h_rb = nullptr;  // pinned read-back block: counters, the signal flag, small-table staging
   uint64_t *h_rb_dev = nullptr;  // the same block's device address
   uint64_t rb_seq = 0;       // the last read-back's signal value
