@@ -40,6 +40,7 @@ extern "C" {
 #define WKG_EDEVICE (-3)   /* HIP runtime error */
 #define WKG_ERANGE (-4)    /* a size exceeds a format limit */
 #define WKG_ESTATE (-5)    /* call out of order (e.g. no graph loaded) */
+#define WKG_EAGAIN (-6)    /* not finished yet / no free slot: call again */
 
 typedef struct wkg_ctx wkg_ctx;
 typedef struct wkg_table wkg_table;
@@ -311,20 +312,39 @@ typedef struct wkg_plan_step {
   int64_t vid;       /* C2U start constant / K2C end constant */
 } wkg_plan_step;
 
-/* ---- a whole small query in one launch --------------------------------------------------
+/* ---- small queries: one workgroup each, a persistent server ---------------------------
  * Runs every step of a compiled plan (wkg_plan_step, above; at most 8 steps,
- * the first an index or constant start) in one launch and one read-back,
- * while every table stays within 4096 input rows per step and 8192 cells
- * (one workgroup holding the tables in LDS); the result is the per-step
- * calls' table, cell for cell. The reference
- * runs a query's steps one handler at a time (GPU_Engine::execute_request,
- * core/gpu_engine.hpp:471-526); for a query this small each step costs
- * launch and read-back latency, not work. Returns WKG_ERANGE when a table
- * outgrew those limits (or the graph is framed): nothing is returned and
- * the caller runs the query step by step. *cells points at rows x cols
- * cells valid until the next call on this context. */
-int wkg_query_small(wkg_ctx *ctx, const wkg_plan_step *steps, int n, void *stream, const int32_t **cells,
+ * the first an index or constant start) in one 1024-thread workgroup with
+ * the tables in LDS, while every table stays within 4096 input rows per step
+ * and 8192 cells; the result is the per-step calls' table, cell for cell.
+ * The reference runs a query's steps one handler at a time
+ * (GPU_Engine::execute_request, core/gpu_engine.hpp:471-526); for a query
+ * this small each step costs launch and read-back latency, not work.
+ *
+ * By default the queries go to a persistent server: a kernel of a few
+ * workgroups (wkg_set_small_query_server) that takes queries from a ring of
+ * 64 slots in pinned host memory as they arrive, with no launch per query,
+ * and exits by itself after 200 us without a query (the next query launches
+ * it again). With 0 workgroups each query is one launch instead.
+ *
+ * wkg_query_small runs one query and waits: WKG_OK with the table copied
+ * into cells (cap cells; WKG_SMALL_QUERY_CELLS always suffices), or
+ * WKG_ERANGE when a table outgrew the limits (or the graph is framed):
+ * then nothing is returned and the caller runs the query step by step.
+ * wkg_query_small_submit / _collect are the same, split: submit returns a
+ * ticket at once (WKG_EAGAIN when all 64 slots are in flight), collect
+ * returns its answer, or WKG_EAGAIN while it runs when wait is 0. Every
+ * submitted ticket must be collected once. A context's calls may come from
+ * several host threads. */
+#define WKG_SMALL_QUERY_CELLS 8192
+int wkg_query_small(wkg_ctx *ctx, const wkg_plan_step *steps, int n, void *stream, int32_t *cells, uint64_t cap,
                     uint64_t *rows, int *cols, uint64_t *expanded);
+int wkg_query_small_submit(wkg_ctx *ctx, const wkg_plan_step *steps, int n, uint64_t *ticket);
+int wkg_query_small_collect(wkg_ctx *ctx, uint64_t ticket, int wait, int32_t *cells, uint64_t cap, uint64_t *rows,
+                            int *cols, uint64_t *expanded);
+/* Server workgroups (default 4, at most half the CUs; 0: one launch per
+ * query). Stops a running server; the next query starts the new shape. */
+int wkg_set_small_query_server(wkg_ctx *ctx, int workgroups);
 
 /* Launch and tile shapes (0 = default). The autotuner below picks them. */
 typedef struct wkg_batch_tuning {

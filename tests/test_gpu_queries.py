@@ -488,10 +488,11 @@ def test_small_queries_in_one_launch(graph_hubs):
           [2, 0, 1, -1, -1, 3, 1, -2]]                         # a predicate index
     rng = np.random.default_rng(93)
     qs += random_queries(rng, 6000, 3, [5, 6, 7], 80, max_steps=5)
-    for on in (True, False):
+    for on, server in ((True, 4), (True, 0), (False, 4)):
         eng.set_small_queries(on)
+        dev.set_small_query_server(server)
         dev.profile(True)
-        print("small queries", "on" if on else "off", flush=True)
+        print("small queries", "on" if on else "off", "server workgroups", server, flush=True)
         for q in qs:
             _check(gs, q, eng.run_query(q))
         taken, declined = dev.profile_read("small_query_taken")[2], dev.profile_read("small_query_declined")[2]
@@ -501,5 +502,68 @@ def test_small_queries_in_one_launch(graph_hubs):
         else:
             assert taken == 0 and declined == 0
     eng.set_small_queries(True)
+    dev.set_small_query_server(4)
     eng.close()
+    dev.close()
+
+
+def test_small_query_server_tickets(graph_hubs, graph_small):
+    """The persistent server over many tickets in flight: up to all 64
+    slots submitted at once and collected out of order, a full ring
+    refusing one more, collection without waiting, a server that went idle
+    and exited (the next ticket launches it again), one workgroup and
+    eight, and a context destroyed while its server runs (it is stopped),
+    then a new context on another graph. Every answer equals the oracle's;
+    declined tickets are the ones the one-launch path declines."""
+    import time
+    from wukonggpu_amd import plan_of
+    _, gs = graph_hubs
+    dev = Device(0)
+    dev.load(gs)
+    rng = np.random.default_rng(94)
+    qs = random_queries(rng, 6000, 3, [5, 6, 7], 80, max_steps=5)
+    qs = [q for q in qs if q[0] > 1][:200]  # index starts
+
+    def check(q, res):
+        st, table, ex = res
+        want, wex, _ = oracle.run_query(gs, q)
+        if st == "declined":
+            dev.set_small_query_server(0)
+            st1 = dev.query_small(plan_of(q))[0]
+            dev.set_small_query_server(wg)
+            assert st1 == "declined", q
+            return 0
+        assert st == "ok", (q, st)
+        assert table.shape == want.shape and np.array_equal(table, want), q
+        assert ex == wex, (q, ex, wex)
+        return 1
+
+    for wg in (1, 8):
+        dev.set_small_query_server(wg)
+        print("server workgroups", wg, flush=True)
+        ok = 0
+        for start in range(0, len(qs), 64):
+            chunk = qs[start:start + 64]
+            tickets = [dev.query_small_submit(plan_of(q)) for q in chunk]
+            assert all(t is not None for t in tickets)
+            if len(chunk) == 64:
+                assert dev.query_small_submit(plan_of(chunk[0])) is None  # the ring is full
+            order = list(range(len(chunk)))
+            rng.shuffle(order)
+            for i in order:
+                res = dev.query_small_collect(tickets[i], wait=bool(i % 2))
+                while res[0] == "pending":
+                    res = dev.query_small_collect(tickets[i], wait=False)
+                ok += check(chunk[i], res)
+        assert ok > len(qs) // 2, ok
+        time.sleep(0.01)  # every workgroup idles out
+        ok += check(qs[0], dev.query_small(plan_of(qs[0])))
+    # a context destroyed while its server runs, then a new one on another graph
+    dev.query_small(plan_of(qs[1]))
+    dev.close()
+    _, gs = graph_small
+    dev = Device(0)
+    dev.load(gs)
+    for q in random_queries(np.random.default_rng(95), 300, 4, [6, 7], 40, max_steps=4):
+        check(q, dev.query_small(plan_of(q)))
     dev.close()

@@ -155,7 +155,10 @@ def hip_lib():
              P(u64), P(u64))
         _sig(L, "wkg_step_k2k", C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, P(vp), P(u64))
         _sig(L, "wkg_step_k2c", C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, i64, vp, P(vp), P(u64))
-        _sig(L, "wkg_query_small", C.c_int, vp, P(PlanStep), C.c_int, vp, P(P(i32)), P(u64), P(C.c_int), P(u64))
+        _sig(L, "wkg_query_small", C.c_int, vp, P(PlanStep), C.c_int, vp, P(i32), u64, P(u64), P(C.c_int), P(u64))
+        _sig(L, "wkg_query_small_submit", C.c_int, vp, P(PlanStep), C.c_int, P(u64))
+        _sig(L, "wkg_query_small_collect", C.c_int, vp, u64, C.c_int, P(i32), u64, P(u64), P(C.c_int), P(u64))
+        _sig(L, "wkg_set_small_query_server", C.c_int, vp, C.c_int)
         _sig(L, "wkg_step_batch", C.c_int, vp, C.c_int, P(StepDesc), vp, P(vp), P(u64))
         _sig(L, "wkg_profile_enable", C.c_int, vp, C.c_int)
         _sig(L, "wkg_profile_read", C.c_int, vp, C.c_char_p, P(u64), P(dbl), P(dbl))

@@ -380,14 +380,19 @@ int wke_run_query(wke_engine *e, const int64_t *cmd, int ncmd, wke_result *res) 
     std::string why;
     if (compile_query(cmd, ncmd, plan, why) == WKG_OK &&
         (plan[0].kind == WKG_STEP_I2U || plan[0].kind == WKG_STEP_C2U)) {
-      const int32_t *cells = nullptr;
+      int32_t *cells = (int32_t *)malloc(WKG_SMALL_QUERY_CELLS * 4);
+      if (!cells) return e->fail(WKG_ENOMEM, "host result table");
       uint64_t rows = 0, ex = 0;
       int cols = 0;
-      if (wkg_query_small(e->ctx, plan.data(), (int)plan.size(), nullptr, &cells, &rows, &cols, &ex) == WKG_OK) {
+      if (wkg_query_small(e->ctx, plan.data(), (int)plan.size(), nullptr, cells, WKG_SMALL_QUERY_CELLS, &rows, &cols,
+                          &ex) == WKG_OK) {
         const uint64_t n = rows * (uint64_t)cols;
-        res->table = n ? (int32_t *)malloc(n * 4) : nullptr;
-        if (n && !res->table) return e->fail(WKG_ENOMEM, "host result table");
-        if (n) memcpy(res->table, cells, n * 4);
+        if (n) {
+          int32_t *t = (int32_t *)realloc(cells, n * 4);  // (shrinks in place)
+          res->table = t ? t : cells;
+        } else {
+          free(cells);
+        }
         res->rows = rows;
         res->cols = cols;
         res->expanded = ex;
@@ -395,6 +400,7 @@ int wke_run_query(wke_engine *e, const int64_t *cmd, int ncmd, wke_result *res) 
         res->latency_s = std::chrono::duration<double>(Clock::now() - t0).count();
         return WKG_OK;
       }
+      free(cells);
     }
   }
   Request r;

@@ -70,6 +70,7 @@ int log2_floor(uint64_t x) {
 }  // namespace
 
 void wkg_ctx::free_graph() {
+  sq_stop();  // the server reads the graph it was launched with
   if (d_slots) (void)hipFree(d_slots);
   if (d_edges) (void)hipFree(d_edges);
   if (d_meta) (void)hipFree(d_meta);

@@ -609,11 +609,11 @@ __global__ __launch_bounds__(kBlock) void k_small_step(GraphView g, StepParams p
   if (threadIdx.x == 0) publish(sg, p.totals, 3);  // (thread 0 wrote the totals)
 }
 
-// ---- a whole small query in one launch ----
+// ---- a whole small query in one workgroup ----
 // Every step of a query whose tables stay small runs in one 1024-thread
 // workgroup with every table in LDS (two ping-pong tables of kSQCells
 // cells), so the only global accesses are the graph's buckets and lists and
-// the final copy of the result into the pinned staging area:
+// the final copy of the result to the host:
 //  - an index / constant start: wave 0 probes the key, the workgroup copies
 //    its list into the table;
 //  - a known-start step: wave w probes rows [64 w r, 64 (w + 1) r) of the
@@ -623,20 +623,34 @@ __global__ __launch_bounds__(kBlock) void k_small_step(GraphView g, StepParams p
 //    output row k = (the last row whose first output row is <= k, its list
 //    entry k - that first row), filters copy the kept rows in order. Output
 //    order is the reference's (core/engine.hpp:126-194).
-// A table past the caps ends the launch early with status 1 (the caller
-// then runs the query step by step), so the result is always the per-step
-// path's. Status and counters go out with one published flag.
+// A table past the caps ends the query early with status 1 (the caller then
+// runs it step by step), so the result is always the per-step path's.
+// Two kernels run this body: k_small_query, one launch per query, and
+// k_sq_server, a persistent server that takes queries from a ring in pinned
+// host memory as they arrive.
 constexpr int kSQThreads = 1024;
 constexpr int kSQWaves = kSQThreads / kWave;
 constexpr uint32_t kSQSteps = 8;        // patterns
 constexpr uint32_t kSQRows = 4096;      // input rows of a known-start step
 constexpr uint32_t kSQCells = 8192;     // cells of a table (32 KB)
+constexpr int kSQHalf = 4;  // list entries / output rows a thread gathers at once
 static_assert(kSQCells * 4 <= kStageBytes, "a small query's result fits the staging area");
 struct SmallQuery {
   wkg_plan_step steps[kSQSteps];
   DevMeta meta[kSQSteps];  // each step's predicate metadata (passed in, not loaded)
   uint32_t nsteps;
-  int32_t *stage;  // pinned staging area (device address), kSQCells cells
+  uint32_t pad[15];
+};
+static_assert(sizeof(SmallQuery) % 64 == 0, "whole cache lines");
+
+struct SQShared {
+  WaveProbeLds wp[kSQWaves];
+  uint32_t deg[kSQRows];  // degree (known_to_unknown) or kept flag (filters)
+  uint32_t off[kSQRows];  // list offset (known_to_unknown)
+  uint32_t pos[kSQRows];  // first output row
+  alignas(16) int32_t tab[2][kSQCells];
+  uint64_t tmp[2][kSQWaves];
+  SmallQuery q;           // the query (copied in)
 };
 
 // Exclusive u64 scan over the kSQThreads threads; `total` gets the sum.
@@ -657,43 +671,75 @@ __device__ __forceinline__ uint64_t sq_block_scan(uint64_t v, uint64_t *tmp, uin
   return pre + inc - v;
 }
 
+__device__ __forceinline__ uint32_t sq_uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ uint64_t sq_uniform(uint64_t v) {
+  return ((uint64_t)sq_uniform((uint32_t)(v >> 32)) << 32) | sq_uniform((uint32_t)v);
+}
+
+// Runs sh.q; wave 0 writes the result table to dst (rows x cols cells, 16-
+// byte aligned) unless the status is 1. res = {status, rows, cols,
+// expanded}. Every thread of the workgroup calls it; wave 0 then releases
+// its stores to the host.
 template <bool SORTED, bool FRAMED>
-__global__ __launch_bounds__(kSQThreads) void k_small_query(GraphView g, SmallQuery q, Signal sg) {
-  __shared__ WaveProbeLds s_wp[kSQWaves];
-  __shared__ uint32_t s_deg[kSQRows];  // degree (known_to_unknown) or kept flag (filters)
-  __shared__ uint32_t s_off[kSQRows];  // list offset (known_to_unknown)
-  __shared__ uint32_t s_pos[kSQRows];  // first output row
-  __shared__ int32_t s_tab[2][kSQCells];
-  __shared__ uint64_t s_tmp[2][kSQWaves];
+__device__ __forceinline__ void sq_run(const GraphView &g, SQShared &sh, int32_t *dst, uint64_t (&res)[4]) {
+  const SmallQuery &q = sh.q;
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave, l8 = lane & 7, gbase = lane & ~7u;
   uint64_t expanded = 0, status = 0;
   uint32_t rows = 0, cols = 0, cur = 0;
-  for (uint32_t l = 0; l < q.nsteps; l++) {
-    const wkg_plan_step d = q.steps[l];
-    const int32_t *in = s_tab[cur];
-    int32_t *out = s_tab[cur ^ 1];
-    const DevMeta m = q.meta[l];
+  const uint32_t nsteps = sq_uniform((uint32_t)q.nsteps);
+  for (uint32_t l = 0; l < nsteps; l++) {
+    // the step and its metadata are the same in every lane: into scalar registers
+    wkg_plan_step d;
+    d.kind = (int32_t)sq_uniform((uint32_t)q.steps[l].kind);
+    d.pid = (int32_t)sq_uniform((uint32_t)q.steps[l].pid);
+    d.dir = (int32_t)sq_uniform((uint32_t)q.steps[l].dir);
+    d.start_col = (int32_t)sq_uniform((uint32_t)q.steps[l].start_col);
+    d.end_col = (int32_t)sq_uniform((uint32_t)q.steps[l].end_col);
+    d.vid = (int64_t)sq_uniform((uint64_t)q.steps[l].vid);
+    DevMeta m;
+    m.main_start = sq_uniform(q.meta[l].main_start);
+    m.psz = sq_uniform(q.meta[l].psz);
+    m.magic = sq_uniform(q.meta[l].magic);
+    m.indrct_start = sq_uniform(q.meta[l].indrct_start);
+    m.edge_start = sq_uniform(q.meta[l].edge_start);
+    m.edge_len = 0;
+    m.vframe_base = sq_uniform(q.meta[l].vframe_base);
+    m.eframe_base = sq_uniform(q.meta[l].eframe_base);
+    const int32_t *in = sh.tab[cur];
+    int32_t *out = sh.tab[cur ^ 1];
     if (d.kind == WKG_STEP_I2U || d.kind == WKG_STEP_C2U) {  // one key: its list is the table
       const uint64_t key = d.kind == WKG_STEP_I2U ? make_key(0, (uint64_t)d.pid, (uint64_t)d.dir)
                                                   : make_key((uint64_t)d.vid, (uint64_t)d.pid, (uint64_t)d.dir);
       if (wave == 0) {
         uint32_t deg, off;
         ProbeCarry carry;
-        wave_probe<FRAMED>(g, m, lane == 0 ? key : 0, s_wp[0], deg, off, carry);
+        wave_probe<FRAMED>(g, m, lane == 0 ? key : 0, sh.wp[0], deg, off, carry);
         if (lane == 0) {
-          s_deg[0] = deg;
-          s_off[0] = off;
+          sh.deg[0] = deg;
+          sh.off[0] = off;
         }
       }
       __syncthreads();
-      const uint32_t deg = s_deg[0], off = s_off[0];
+      const uint32_t deg = sh.deg[0], off = sh.off[0];
       expanded += deg;
       if (deg > kSQCells) {
         status = 1;
         break;
       }
-      for (uint32_t j = tid; j < deg; j += kSQThreads)
-        out[j] = (int32_t)edge_at<FRAMED>(g, m.edge_start, m.eframe_base, (uint64_t)off + j);
+#pragma unroll 1
+      for (uint32_t j0 = 0; j0 < deg; j0 += kSQHalf * kSQThreads) {
+        uint32_t v[kSQHalf];  // a thread's list loads in flight together
+#pragma unroll
+        for (int k = 0; k < kSQHalf; k++) {
+          const uint32_t j = j0 + tid + k * kSQThreads;
+          if (j < deg) v[k] = edge_at<FRAMED>(g, m.edge_start, m.eframe_base, (uint64_t)off + j);
+        }
+#pragma unroll
+        for (int k = 0; k < kSQHalf; k++) {
+          const uint32_t j = j0 + tid + k * kSQThreads;
+          if (j < deg) out[j] = (int32_t)v[k];
+        }
+      }
       rows = deg;
       cols = 1;
     } else {
@@ -717,11 +763,11 @@ __global__ __launch_bounds__(kSQThreads) void k_small_query(GraphView g, SmallQu
           if (mode == kModeK2K) end_cell = in[row * cols + ec];
         }
         uint32_t deg, off;
-        wave_probe<FRAMED>(g, m, key, s_wp[wave], deg, off, carry);
+        wave_probe<FRAMED>(g, m, key, sh.wp[wave], deg, off, carry);
         if (mode == kModeK2U) {
           if (row < rows) {
-            s_deg[row] = deg;
-            s_off[row] = off;
+            sh.deg[row] = deg;
+            sh.off[row] = off;
           }
         } else {
           // the 8 lanes of a group test the group's 8 rows one after another
@@ -745,7 +791,7 @@ __global__ __launch_bounds__(kSQThreads) void k_small_query(GraphView g, SmallQu
               my_ex = examined;
             }
           }
-          if (row < rows) s_deg[row] = my_found;
+          if (row < rows) sh.deg[row] = my_found;
           ex += my_ex;
         }
       }
@@ -754,16 +800,16 @@ __global__ __launch_bounds__(kSQThreads) void k_small_query(GraphView g, SmallQu
       uint64_t cnt = 0;
       for (uint32_t k = 0; k < rounds; k++) {
         const uint32_t row = tid * rounds + k;
-        if (row < rows) cnt += s_deg[row];
+        if (row < rows) cnt += sh.deg[row];
       }
       uint64_t total, ex_total = 0;
-      uint64_t pre = sq_block_scan(cnt, s_tmp[0], total);
-      if (mode != kModeK2U) (void)sq_block_scan(ex, s_tmp[1], ex_total);
+      uint64_t pre = sq_block_scan(cnt, sh.tmp[0], total);
+      if (mode != kModeK2U) (void)sq_block_scan(ex, sh.tmp[1], ex_total);
       for (uint32_t k = 0; k < rounds; k++) {
         const uint32_t row = tid * rounds + k;
         if (row < rows) {
-          s_pos[row] = (uint32_t)pre;  // (a total past the caps declines below before any use)
-          pre += s_deg[row];
+          sh.pos[row] = (uint32_t)pre;  // (a total past the caps declines below before any use)
+          pre += sh.deg[row];
         }
       }
       __syncthreads();
@@ -774,41 +820,161 @@ __global__ __launch_bounds__(kSQThreads) void k_small_query(GraphView g, SmallQu
           status = 1;
           break;
         }
-        const uint32_t E = (uint32_t)total;
-        for (uint32_t r = tid; r < E; r += kSQThreads) {
-          uint32_t lo = 0, hi = rows - 1;  // the last row whose first output row is <= r
-          while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) >> 1;
-            if (s_pos[mid] <= r) lo = mid;
-            else hi = mid - 1;
+        const uint32_t E = (uint32_t)total;  // <= kSQCells
+#pragma unroll 1
+        for (uint32_t r0 = 0; r0 < E; r0 += kSQHalf * kSQThreads) {  // kSQHalf output rows a thread at once
+          uint32_t par[kSQHalf], v[kSQHalf];
+#pragma unroll
+          for (int k = 0; k < kSQHalf; k++) {  // the last row whose first output row is <= r
+            const uint32_t r = r0 + tid + k * kSQThreads;
+            uint32_t lo = 0;
+#pragma unroll
+            for (uint32_t step = kSQRows / 2; step; step >>= 1)
+              if (lo + step < rows && sh.pos[lo + step] <= r) lo += step;
+            par[k] = lo;
           }
-          const uint32_t v = edge_at<FRAMED>(g, m.edge_start, m.eframe_base, (uint64_t)s_off[lo] + (r - s_pos[lo]));
-          int32_t *o = out + r * nc;
-          for (uint32_t j = 0; j < cols; j++) o[j] = in[lo * cols + j];
-          o[cols] = (int32_t)v;
+#pragma unroll
+          for (int k = 0; k < kSQHalf; k++) {  // the gathered edges in flight together
+            const uint32_t r = r0 + tid + k * kSQThreads;
+            if (r < E) v[k] = edge_at<FRAMED>(g, m.edge_start, m.eframe_base, (uint64_t)sh.off[par[k]] + (r - sh.pos[par[k]]));
+          }
+#pragma unroll
+          for (int k = 0; k < kSQHalf; k++) {
+            const uint32_t r = r0 + tid + k * kSQThreads;
+            if (r < E) {
+              int32_t *o = out + r * nc;
+              for (uint32_t j = 0; j < cols; j++) o[j] = in[par[k] * cols + j];
+              o[cols] = (int32_t)v[k];
+            }
+          }
         }
         rows = E;
         cols = nc;
       } else {
         expanded += ex_total;
         for (uint32_t row = tid; row < rows; row += kSQThreads)
-          if (s_deg[row])
-            for (uint32_t j = 0; j < cols; j++) out[s_pos[row] * cols + j] = in[row * cols + j];
+          if (sh.deg[row])
+            for (uint32_t j = 0; j < cols; j++) out[sh.pos[row] * cols + j] = in[row * cols + j];
         rows = (uint32_t)total;
       }
     }
     __syncthreads();  // the output is complete before it is the next step's input
     cur ^= 1;
   }
-  if (status == 0) {
-    const uint32_t cells = rows * cols;
-    for (uint32_t i = tid; i < cells; i += kSQThreads) q.stage[i] = s_tab[cur][i];
+  // the result to the host: wave 0 alone (one wave's system-scope release
+  // then covers every store), 16 bytes a lane
+  if (status == 0 && wave == 0) {
+    const uint32_t cells = rows * cols, nq = cells / 4;
+    const uint4 *src = (const uint4 *)sh.tab[cur];
+    for (uint32_t i = lane; i < nq; i += kWave) ((uint4 *)dst)[i] = src[i];
+    if (lane < cells - 4 * nq) dst[4 * nq + lane] = sh.tab[cur][4 * nq + lane];
   }
-  __threadfence_system();
+  res[0] = status;
+  res[1] = rows;
+  res[2] = cols;
+  res[3] = expanded;
+}
+
+// One query, one launch: the result into the pinned staging area, then
+// {status, rows, cols, expanded} and the flag.
+template <bool SORTED, bool FRAMED>
+__global__ __launch_bounds__(kSQThreads) void k_small_query(GraphView g, SmallQuery q, int32_t *stage, Signal sg) {
+  __shared__ SQShared sh;
+  if (threadIdx.x == 0) sh.q = q;
   __syncthreads();
-  if (tid == 0) {
-    const uint64_t v[4] = {status, rows, cols, expanded};
-    publish(sg, v, 4);
+  uint64_t res[4];
+  sq_run<SORTED, FRAMED>(g, sh, stage, res);
+  if (threadIdx.x < kWave) {
+    __threadfence_system();
+    if (threadIdx.x == 0) publish(sg, res, 4);
+  }
+}
+
+// ---- the persistent small-query server ----
+// A ring of slots in pinned host memory: the host writes a query into slot
+// t mod nslots, then publishes t + 1 as the ring's tail. Each workgroup of
+// k_sq_server claims the next unclaimed ticket (a compare-and-swap on a
+// device-memory head counter, so every ticket is taken once, in order),
+// copies the query into LDS, runs it, writes the result table and
+// {status, rows, cols, expanded, device ticks} into the slot, then t + 1 as
+// the slot's done word (system-scope release). A workgroup exits when no
+// ticket arrived for idle_ticks, when it has run for life_ticks, or when
+// the host sets stop; every wait is bounded by those clocks (the device's
+// constant wall clock), so the grid always drains. Queries arriving while
+// others run are admitted as soon as a workgroup is free.
+struct SQResp {
+  uint64_t status, rows, cols, expanded, ticks;
+  uint64_t pad[2];
+  uint64_t done;  // ticket + 1, written last
+};
+struct SQSlot {
+  SmallQuery q;
+  SQResp resp;
+  int32_t cells[kSQCells];
+};
+struct SQBox {
+  const uint64_t *tail;  // pinned: tickets issued (host-written)
+  const uint64_t *stop;  // pinned: nonzero asks the workgroups to exit
+  uint64_t *head;        // device memory: tickets claimed
+  SQSlot *slots;         // pinned ring (device address)
+  uint64_t nslots;       // a power of two
+  uint64_t idle_ticks, life_ticks;
+};
+constexpr uint64_t kSQNoClaim = ~uint64_t(0);
+
+template <bool SORTED>
+__global__ __launch_bounds__(kSQThreads) void k_sq_server(GraphView g, SQBox box) {
+  __shared__ SQShared sh;
+  __shared__ uint64_t s_claim;
+  const uint64_t t_start = wall_clock64();
+  uint64_t t_idle = t_start;  // (thread 0) since when the ring is idle
+  // (thread 0) the head as last seen: a stale value only fails one
+  // compare-and-swap, which returns the current one
+  uint64_t h = threadIdx.x == 0 ? __hip_atomic_load(box.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      uint64_t claim = kSQNoClaim;
+      for (;;) {
+        const uint64_t now = wall_clock64();
+        if (now - t_start > box.life_ticks || now - t_idle > box.idle_ticks) break;
+        if (__hip_atomic_load(box.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) break;
+        const uint64_t tail = __hip_atomic_load(box.tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (h < tail) {
+          if (__hip_atomic_compare_exchange_strong(box.head, &h, h + 1, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT)) {
+            claim = h++;
+            break;
+          }
+          continue;  // another workgroup took it (h is now the current head)
+        }
+        __builtin_amdgcn_s_sleep(4);
+      }
+      s_claim = claim;
+    }
+    __syncthreads();
+    const uint64_t claim = s_claim;
+    if (claim == kSQNoClaim) break;
+    const uint64_t t0 = wall_clock64();
+    SQSlot *slot = box.slots + (claim & (box.nslots - 1));
+    // the query into LDS, word by word past every cache
+    const uint32_t *src = (const uint32_t *)&slot->q;
+    uint32_t *qw = (uint32_t *)&sh.q;
+    for (uint32_t i = threadIdx.x; i < sizeof(SmallQuery) / 4; i += kSQThreads)
+      qw[i] = __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __syncthreads();
+    uint64_t res[4];
+    sq_run<SORTED, false>(g, sh, slot->cells, res);
+    if (threadIdx.x < kWave) __threadfence_system();  // (wave 0 wrote the cells)
+    if (threadIdx.x == 0) {
+      slot->resp.status = res[0];
+      slot->resp.rows = res[1];
+      slot->resp.cols = res[2];
+      slot->resp.expanded = res[3];
+      slot->resp.ticks = wall_clock64() - t0;
+      __hip_atomic_store(&slot->resp.done, claim + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      t_idle = wall_clock64();
+    }
+    __syncthreads();  // s_claim and sh are reused
   }
 }
 

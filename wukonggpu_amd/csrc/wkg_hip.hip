@@ -1421,6 +1421,7 @@ int wkg_init(int device, uint64_t table_budget_bytes, wkg_ctx **out) {
 int wkg_destroy(wkg_ctx *ctx) {
   if (!ctx) return WKG_EINVAL;
   (void)hipSetDevice(ctx->device);
+  ctx->sq_free();
   (void)hipDeviceSynchronize();
   ctx->free_graph();
   delete ctx->pool;
@@ -1673,13 +1674,40 @@ int wkg_step_k2u_noalloc(wkg_ctx *ctx, const int32_t *in, uint64_t rows, int col
   return rc;
 }
 
-int wkg_query_small(wkg_ctx *ctx, const wkg_plan_step *steps, int n, void *stream, const int32_t **cells,
-                    uint64_t *rows, int *cols, uint64_t *expanded) {
-  if (!ctx || !steps || !cells || !rows || !cols) return WKG_EINVAL;
-  *cells = nullptr;
-  *rows = 0;
-  *cols = 0;
-  if (expanded) *expanded = 0;
+}  // extern "C"
+
+// ---- small queries: the persistent server's host side ----
+struct SQServer {
+  std::mutex mu;
+  hipStream_t stream = nullptr;
+  hipEvent_t end_ev = nullptr;  // recorded after the last launch
+  bool launched = false;
+  char *h = nullptr;            // pinned: tail line, stop line, then the slots
+  char *d = nullptr;            // the same block's device address
+  uint64_t *d_head = nullptr;   // device memory: tickets claimed
+  uint64_t tail = 0;            // tickets issued
+  std::vector<uint8_t> busy;    // per slot: issued and not yet collected
+  // host clock of the last launch and of the last ticket: within the idle
+  // time of the last ticket and the life time of the launch a workgroup
+  // surely runs (or is about to start), so no event query is needed
+  std::chrono::steady_clock::time_point launch_t, submit_t;
+  double tick_hz = 1e8;         // the device wall clock
+  uint64_t *tail_h() { return (uint64_t *)h; }
+  uint64_t *stop_h() { return (uint64_t *)(h + 64); }
+  SQSlot *slot_h(uint64_t t) { return (SQSlot *)(h + 128) + (t & (kSQSlots - 1)); }
+  static constexpr uint64_t kSQSlots = 64;
+};
+
+namespace {
+
+// A server workgroup exits after kSQIdleS without a query and takes no query
+// after kSQLifeS (the next query launches it again). Its stream may share a
+// hardware queue with other streams of the process: work queued there waits
+// at most that long behind it.
+constexpr double kSQIdleS = 200e-6;
+constexpr double kSQLifeS = 0.02;
+
+int sq_check_plan(wkg_ctx *ctx, const wkg_plan_step *steps, int n) {
   if (!ctx->loaded) return ctx->fail(WKG_ESTATE, "no graph loaded");
   if (n < 1 || n > (int)kSQSteps) return ctx->fail(WKG_ERANGE, "a small query has 1..%u steps", kSQSteps);
   if (ctx->framed) return ctx->fail(WKG_ERANGE, "small queries run on resident graphs");
@@ -1703,35 +1731,253 @@ int wkg_query_small(wkg_ctx *ctx, const wkg_plan_step *steps, int n, void *strea
       return ctx->fail(WKG_EINVAL, "step %d: end column %d outside %d", l, d.end_col, c);
     if (d.kind == WKG_STEP_K2U) c++;
   }
-  hipStream_t s = ctx->pick(stream);
-  SmallQuery q{};
+  return WKG_OK;
+}
+
+void sq_fill(wkg_ctx *ctx, const wkg_plan_step *steps, int n, SmallQuery &q) {
+  memset(&q, 0, sizeof q);
   for (int l = 0; l < n; l++) {
     q.steps[l] = steps[l];
     q.meta[l] = ctx->h_meta[steps[l].pid];
   }
   q.nsteps = (uint32_t)n;
-  q.stage = (int32_t *)((char *)ctx->h_rb_dev + kStageOff);
+}
+
+int sq_init(wkg_ctx *ctx) {
+  if (ctx->sqs) return WKG_OK;
+  SQServer *v = new (std::nothrow) SQServer;
+  if (!v) return ctx->fail(WKG_ENOMEM, "small-query server state");
+  const size_t bytes = 128 + SQServer::kSQSlots * sizeof(SQSlot);
+  int khz = 0;
+  bool ok = hipHostMalloc((void **)&v->h, bytes, hipHostMallocCoherent | hipHostMallocMapped) == hipSuccess &&
+            hipHostGetDevicePointer((void **)&v->d, v->h, 0) == hipSuccess &&
+            hipMalloc((void **)&v->d_head, 256) == hipSuccess && hipMemset(v->d_head, 0, 256) == hipSuccess &&
+            hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&v->end_ev, hipEventDisableTiming) == hipSuccess &&
+            hipDeviceSynchronize() == hipSuccess;
+  if (ok && hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device) == hipSuccess && khz > 0)
+    v->tick_hz = std::min(1e9, std::max(1e6, khz * 1e3));
+  if (!ok) {
+    if (v->end_ev) (void)hipEventDestroy(v->end_ev);
+    if (v->stream) (void)hipStreamDestroy(v->stream);
+    if (v->d_head) (void)hipFree(v->d_head);
+    if (v->h) (void)hipHostFree(v->h);
+    delete v;
+    return ctx->fail(WKG_EDEVICE, "small-query server set-up failed");
+  }
+  memset(v->h, 0, bytes);
+  v->busy.assign(SQServer::kSQSlots, 0);
+  ctx->sqs = v;
+  return WKG_OK;
+}
+
+// Whether the last launch still runs (v->mu held).
+int sq_alive(wkg_ctx *ctx, SQServer *v, bool &alive) {
+  alive = false;
+  if (!v->launched) return WKG_OK;
+  const hipError_t e = hipEventQuery(v->end_ev);
+  if (e == hipErrorNotReady) {
+    alive = true;
+    return WKG_OK;
+  }
+  v->launched = false;
+  if (e != hipSuccess) return ctx->fail(WKG_EDEVICE, "small-query server: %s", hipGetErrorString(e));
+  return WKG_OK;
+}
+
+// Launches the server unless it runs (v->mu held).
+int sq_ensure(wkg_ctx *ctx, SQServer *v) {
+  const auto now = std::chrono::steady_clock::now();
+  if (v->launched && now - v->submit_t < std::chrono::duration<double>(kSQIdleS / 2) &&
+      now - v->launch_t < std::chrono::duration<double>(kSQLifeS / 2))
+    return WKG_OK;
+  bool alive;
+  if (int rc = sq_alive(ctx, v, alive)) return rc;
+  if (alive) return WKG_OK;
+  SQBox b;
+  b.tail = (const uint64_t *)v->d;
+  b.stop = (const uint64_t *)(v->d + 64);
+  b.head = v->d_head;
+  b.slots = (SQSlot *)(v->d + 128);
+  b.nslots = SQServer::kSQSlots;
+  b.idle_ticks = (uint64_t)(kSQIdleS * v->tick_hz);
+  b.life_ticks = (uint64_t)(kSQLifeS * v->tick_hz);
+  const int nwg = std::max(1, std::min(ctx->sq_workgroups, ctx->ncu / 2));
+  if (ctx->sorted) k_sq_server<true><<<nwg, kSQThreads, 0, v->stream>>>(ctx->view(), b);
+  else k_sq_server<false><<<nwg, kSQThreads, 0, v->stream>>>(ctx->view(), b);
+  const hipError_t le = hipGetLastError();
+  if (le != hipSuccess) return ctx->fail(WKG_EDEVICE, "small-query server launch failed: %s", hipGetErrorString(le));
+  HIPCHK(ctx, hipEventRecord(v->end_ev, v->stream));
+  v->launched = true;
+  v->launch_t = std::chrono::steady_clock::now();
+  return WKG_OK;
+}
+
+int sq_submit(wkg_ctx *ctx, const wkg_plan_step *steps, int n, uint64_t *ticket) {
+  if (int rc = sq_init(ctx)) return rc;
+  SQServer *v = ctx->sqs;
+  std::lock_guard<std::mutex> lk(v->mu);
+  const uint64_t t = v->tail;
+  if (v->busy[t & (SQServer::kSQSlots - 1)])
+    return ctx->fail(WKG_EAGAIN, "all %llu small-query slots are in flight", (unsigned long long)SQServer::kSQSlots);
+  SQSlot *sl = v->slot_h(t);
+  sq_fill(ctx, steps, n, sl->q);
+  v->busy[t & (SQServer::kSQSlots - 1)] = 1;
+  __atomic_store_n(v->tail_h(), t + 1, __ATOMIC_RELEASE);  // the slot is written before the tail moves
+  v->tail = t + 1;
+  *ticket = t;
+  const int rc = sq_ensure(ctx, v);
+  v->submit_t = std::chrono::steady_clock::now();  // (after: a workgroup then runs until at least idle past it)
+  return rc;
+}
+
+// Copies ticket t's answer out and frees its slot; WKG_EAGAIN while it is
+// not answered (wait = 0).
+int sq_collect(wkg_ctx *ctx, uint64_t t, bool wait, int32_t *cells, uint64_t cap, uint64_t *rows, int *cols,
+               uint64_t *expanded) {
+  SQServer *v = ctx->sqs;
+  if (!v) return ctx->fail(WKG_EINVAL, "ticket %llu is not in flight", (unsigned long long)t);
+  {
+    std::lock_guard<std::mutex> lk(v->mu);
+    if (t >= v->tail || t + SQServer::kSQSlots < v->tail || !v->busy[t & (SQServer::kSQSlots - 1)])
+      return ctx->fail(WKG_EINVAL, "ticket %llu is not in flight", (unsigned long long)t);
+  }
+  SQSlot *sl = v->slot_h(t);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t spins = 0; __atomic_load_n(&sl->resp.done, __ATOMIC_ACQUIRE) != t + 1; spins++) {
+    if (!wait && spins == 0) {
+      // not answered yet: make sure a server runs to answer it
+      std::lock_guard<std::mutex> lk(v->mu);
+      if (__atomic_load_n(&sl->resp.done, __ATOMIC_ACQUIRE) == t + 1) break;
+      if (int rc = sq_ensure(ctx, v)) return rc;
+      return WKG_EAGAIN;
+    }
+    if ((spins & 255) == 255) {
+      std::lock_guard<std::mutex> lk(v->mu);
+      // every workgroup of the last launch may have gone idle just before
+      // this ticket arrived: a fresh launch takes it
+      if (__atomic_load_n(&sl->resp.done, __ATOMIC_ACQUIRE) == t + 1) break;
+      if (int rc = sq_ensure(ctx, v)) return rc;
+      if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(10))
+        return ctx->fail(WKG_EDEVICE, "small-query ticket %llu not answered in 10 s", (unsigned long long)t);
+    }
+  }
+  const uint64_t status = sl->resp.status, r = sl->resp.rows, c = sl->resp.cols;
+  if (expanded) *expanded = sl->resp.expanded;
+  if (ctx->profiling) {
+    std::lock_guard<std::mutex> lk(ctx->prof_mu);
+    wkg_ctx::KStat &k = ctx->prof_stats["small_query"];
+    k.launches++;
+    k.ms += sl->resp.ticks / v->tick_hz * 1e3;
+    ctx->prof_stats[status ? "small_query_declined" : "small_query_taken"].bytes += 1.0;
+  }
+  int rc = WKG_OK;
+  if (status != 0) {
+    rc = ctx->fail(WKG_ERANGE, "a table outgrew the small-query limits (%u input rows, %u cells)", kSQRows, kSQCells);
+  } else if (r * c > cap) {
+    rc = ctx->fail(WKG_ERANGE, "a %llu-cell answer does not fit %llu cells", (unsigned long long)(r * c),
+                   (unsigned long long)cap);
+  } else {
+    *rows = r;
+    *cols = (int)c;
+    if (r * c) memcpy(cells, sl->cells, r * c * 4);
+  }
+  std::lock_guard<std::mutex> lk(v->mu);
+  v->busy[t & (SQServer::kSQSlots - 1)] = 0;
+  return rc;
+}
+
+// One launch per query (no server).
+int sq_one_launch(wkg_ctx *ctx, const wkg_plan_step *steps, int n, void *stream, int32_t *cells, uint64_t cap,
+                  uint64_t *rows, int *cols, uint64_t *expanded) {
+  hipStream_t s = ctx->pick(stream);
+  SmallQuery q;
+  sq_fill(ctx, steps, n, q);
+  int32_t *stage = (int32_t *)((char *)ctx->h_rb_dev + kStageOff);
   const GraphView g = ctx->view();
   const Signal sg = ctx->next_signal();
   {
     ProfScope ps(ctx, s, "small_query", 0.0);
-    if (ctx->sorted) k_small_query<true, false><<<1, kSQThreads, 0, s>>>(g, q, sg);
-    else k_small_query<false, false><<<1, kSQThreads, 0, s>>>(g, q, sg);
+    if (ctx->sorted) k_small_query<true, false><<<1, kSQThreads, 0, s>>>(g, q, stage, sg);
+    else k_small_query<false, false><<<1, kSQThreads, 0, s>>>(g, q, stage, sg);
   }
   hipError_t le = hipGetLastError();
   if (le != hipSuccess) return ctx->fail(WKG_EDEVICE, "small query launch failed: %s", hipGetErrorString(le));
   if (int rc = ctx->wait_signal(sg, s)) return rc;
-  const uint64_t status = ctx->h_rb[0], r = ctx->h_rb[1], cc = ctx->h_rb[2], ex = ctx->h_rb[3];
-  if (status != 0) {
-    ctx->prof_add_bytes("small_query_declined", 1.0);  // a count
-    return ctx->fail(WKG_ERANGE, "a table outgrew the small-query limits (%u input rows, %u cells)", kSQRows,
-                     kSQCells);
-  }
-  ctx->prof_add_bytes("small_query_taken", 1.0);
+  const uint64_t status = ctx->h_rb[0], r = ctx->h_rb[1], c = ctx->h_rb[2];
+  if (expanded) *expanded = ctx->h_rb[3];
+  ctx->prof_add_bytes(status ? "small_query_declined" : "small_query_taken", 1.0);  // counts
+  if (status != 0)
+    return ctx->fail(WKG_ERANGE, "a table outgrew the small-query limits (%u input rows, %u cells)", kSQRows, kSQCells);
+  if (r * c > cap)
+    return ctx->fail(WKG_ERANGE, "a %llu-cell answer does not fit %llu cells", (unsigned long long)(r * c),
+                     (unsigned long long)cap);
   *rows = r;
-  *cols = (int)cc;
-  if (expanded) *expanded = ex;
-  *cells = (const int32_t *)((const char *)ctx->h_rb + kStageOff);
+  *cols = (int)c;
+  if (r * c) memcpy(cells, (const char *)ctx->h_rb + kStageOff, r * c * 4);
+  return WKG_OK;
+}
+
+}  // namespace
+
+void wkg_ctx::sq_stop() {
+  SQServer *v = sqs;
+  if (!v) return;
+  std::lock_guard<std::mutex> lk(v->mu);
+  if (v->launched) {
+    __atomic_store_n(v->stop_h(), 1, __ATOMIC_RELEASE);
+    (void)hipStreamSynchronize(v->stream);
+    __atomic_store_n(v->stop_h(), 0, __ATOMIC_RELEASE);
+    v->launched = false;
+  }
+}
+
+void wkg_ctx::sq_free() {
+  sq_stop();
+  SQServer *v = sqs;
+  if (!v) return;
+  (void)hipEventDestroy(v->end_ev);
+  (void)hipStreamDestroy(v->stream);
+  (void)hipFree(v->d_head);
+  (void)hipHostFree(v->h);
+  delete v;
+  sqs = nullptr;
+}
+
+extern "C" {
+
+int wkg_query_small(wkg_ctx *ctx, const wkg_plan_step *steps, int n, void *stream, int32_t *cells, uint64_t cap,
+                    uint64_t *rows, int *cols, uint64_t *expanded) {
+  if (!ctx || !steps || !rows || !cols || (cap && !cells)) return WKG_EINVAL;
+  *rows = 0;
+  *cols = 0;
+  if (expanded) *expanded = 0;
+  if (int rc = sq_check_plan(ctx, steps, n)) return rc;
+  if (ctx->sq_workgroups == 0) return sq_one_launch(ctx, steps, n, stream, cells, cap, rows, cols, expanded);
+  uint64_t t;
+  if (int rc = sq_submit(ctx, steps, n, &t)) return rc;
+  return sq_collect(ctx, t, true, cells, cap, rows, cols, expanded);
+}
+
+int wkg_query_small_submit(wkg_ctx *ctx, const wkg_plan_step *steps, int n, uint64_t *ticket) {
+  if (!ctx || !steps || !ticket) return WKG_EINVAL;
+  if (int rc = sq_check_plan(ctx, steps, n)) return rc;
+  return sq_submit(ctx, steps, n, ticket);
+}
+
+int wkg_query_small_collect(wkg_ctx *ctx, uint64_t ticket, int wait, int32_t *cells, uint64_t cap, uint64_t *rows,
+                            int *cols, uint64_t *expanded) {
+  if (!ctx || !rows || !cols || (cap && !cells)) return WKG_EINVAL;
+  *rows = 0;
+  *cols = 0;
+  if (expanded) *expanded = 0;
+  return sq_collect(ctx, ticket, wait != 0, cells, cap, rows, cols, expanded);
+}
+
+int wkg_set_small_query_server(wkg_ctx *ctx, int workgroups) {
+  if (!ctx || workgroups < 0) return WKG_EINVAL;
+  ctx->sq_stop();  // the next query launches the new shape
+  ctx->sq_workgroups = workgroups;
   return WKG_OK;
 }
 

@@ -128,6 +128,8 @@ struct Residency {
   hipStream_t copy_stream = nullptr;
 };
 
+struct SQServer;
+
 struct wkg_ctx {
   int device = 0;
   int ncu = 256;  // compute units of the device
@@ -140,6 +142,8 @@ struct wkg_ctx {
   uint64_t group_filter_rows = 1ull << 22;  // smallest input a filter decides per lineage group
   bool small_steps = true;   // steps over at most kSmallRows rows in one launch
   uint64_t *noalloc_scratch = nullptr;  // this device's no-alloc step scratch (wkg_step_k2u_noalloc)
+  SQServer *sqs = nullptr;   // the persistent small-query server (created on first use)
+  int sq_workgroups = 4;     // its workgroups (0: one launch per small query)
   uint64_t *h_rb = nullptr;  // pinned read-back block: counters, the signal flag, small-table staging
   uint64_t *h_rb_dev = nullptr;  // the same block's device address
   uint64_t rb_seq = 0;       // the last read-back's signal value
@@ -184,6 +188,8 @@ struct wkg_ctx {
   void prof_add_bytes(const char *name, double bytes);  // bytes known only after a launch
 
   int fail(int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
+  void sq_stop();  // stops the small-query server (it holds the graph view it was launched with)
+  void sq_free();
   wkg::GraphView view() const;
   hipStream_t pick(void *s) const;
   int new_table(uint64_t rows, int cols, hipStream_t s, wkg_table **out);

@@ -10,6 +10,7 @@ import ctypes as C
 import numpy as np
 
 from . import _native as N
+from ._native import STEP_C2U, STEP_I2U, STEP_K2C, STEP_K2K, STEP_K2U
 
 
 class DeviceError(RuntimeError):
@@ -245,6 +246,59 @@ class Device:
                                            L.wkg_table_device_ptr(out._h), cap, None, C.byref(n), C.byref(ex)))
         return int(n.value), int(ex.value)
 
+    # ---- small queries (wkg_query_small*) ----
+    def set_small_query_server(self, workgroups):
+        """Persistent small-query server workgroups (0: one launch per query)."""
+        self._check(N.hip_lib().wkg_set_small_query_server(self._h, workgroups))
+
+    @staticmethod
+    def _plan_array(plan):
+        arr = (N.PlanStep * max(len(plan), 1))()
+        for i, s in enumerate(plan):
+            arr[i].kind, arr[i].pid, arr[i].dir = s["kind"], s["pid"], s["dir"]
+            arr[i].start_col, arr[i].end_col = s.get("start_col", 0), s.get("end_col", 0)
+            arr[i].vid = s.get("vid", 0)
+        return arr
+
+    _SQ_CELLS = 8192
+
+    def _sq_result(self, rc, buf, rows, cols, ex):
+        if rc == -4:  # WKG_ERANGE: declined, run it step by step
+            return "declined", None, int(ex.value)
+        self._check(rc)
+        n = rows.value * cols.value
+        return "ok", np.ctypeslib.as_array(buf)[:n].reshape(rows.value, cols.value).copy(), int(ex.value)
+
+    def query_small(self, plan):
+        """One small query (a plan of step dicts): ("ok", table, expanded) or
+        ("declined", None, expanded so far)."""
+        buf = (C.c_int32 * self._SQ_CELLS)()
+        rows, cols, ex = C.c_uint64(), C.c_int(), C.c_uint64()
+        rc = N.hip_lib().wkg_query_small(self._h, self._plan_array(plan), len(plan), None, buf, self._SQ_CELLS,
+                                         C.byref(rows), C.byref(cols), C.byref(ex))
+        return self._sq_result(rc, buf, rows, cols, ex)
+
+    def query_small_submit(self, plan):
+        """Submits a small query to the server; returns its ticket, or None
+        when every slot is in flight."""
+        t = C.c_uint64()
+        rc = N.hip_lib().wkg_query_small_submit(self._h, self._plan_array(plan), len(plan), C.byref(t))
+        if rc == -6:  # WKG_EAGAIN
+            return None
+        self._check(rc)
+        return int(t.value)
+
+    def query_small_collect(self, ticket, wait=True):
+        """("ok", table, expanded), ("declined", None, expanded) or, with
+        wait False, ("pending", None, 0) while the ticket runs."""
+        buf = (C.c_int32 * self._SQ_CELLS)()
+        rows, cols, ex = C.c_uint64(), C.c_int(), C.c_uint64()
+        rc = N.hip_lib().wkg_query_small_collect(self._h, ticket, 1 if wait else 0, buf, self._SQ_CELLS,
+                                                 C.byref(rows), C.byref(cols), C.byref(ex))
+        if rc == -6:
+            return "pending", None, 0
+        return self._sq_result(rc, buf, rows, cols, ex)
+
     def k2k(self, t, start_col, pid, d, end_col):
         return self._step(N.hip_lib().wkg_step_k2k, t._h, start_col, pid, d, end_col)
 
@@ -268,6 +322,29 @@ class Device:
         ex = (C.c_uint64 * n)()
         self._check(N.hip_lib().wkg_step_batch(self._h, n, arr, None, outs, ex))
         return [Table(self, outs[i]) for i in range(n)], [int(x) for x in ex]
+
+
+def plan_of(cmd):
+    """cmd_chain -> plan step dicts {kind, pid, dir, start_col, end_col, vid}:
+    GPU_Engine::execute_one_step's dispatch (core/gpu_engine.hpp:395-467)
+    with the table's column bookkeeping (a known variable -k is column k-1)."""
+    steps, cols = [], 0
+    for i in range(0, len(cmd), 4):
+        s, p, d, e = (int(x) for x in cmd[i:i + 4])
+        if i == 0 and 1 < s < (1 << 17):
+            steps.append(dict(kind=STEP_I2U, pid=s, dir=d))
+            cols = 1
+        elif s >= 0:
+            steps.append(dict(kind=STEP_C2U, pid=p, dir=d, vid=s))
+            cols = 1
+        elif e >= 0:
+            steps.append(dict(kind=STEP_K2C, pid=p, dir=d, start_col=-s - 1, vid=e))
+        elif e < 0 and -e <= cols:
+            steps.append(dict(kind=STEP_K2K, pid=p, dir=d, start_col=-s - 1, end_col=-e - 1))
+        else:
+            steps.append(dict(kind=STEP_K2U, pid=p, dir=d, start_col=-s - 1))
+            cols += 1
+    return steps
 
 
 def pack_queries(queries):
