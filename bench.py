@@ -685,10 +685,24 @@ def run_config4(args, nested=False):
     tb = time.perf_counter()
     light_b = eng.run_batched(light)
     t_light = time.perf_counter() - tb
+    # one untimed served run first: the lanes' arenas and captured sequences
+    # are set up there, not in the first measured rate
+    warm = np.cumsum(np.random.default_rng(args.seed + 40).exponential(1.0 / 4000, size=2000))
+    eng.serve(sq[:2000], warm, heavy=flags[:2000], table_budget_bytes=64 << 30, light_lanes=2)
     served, all_same = served_sweep(eng, dev, sq, flags, alone, args.served_sweep_rates, args.seed + 42, 64 << 30,
                                     span_s=0.5)
     knee = served_knee(served)
     log(f"served knee: {knee}")
+    # the same sweep with every light query on the light lanes' batches (no
+    # small-query server): admission at batch boundaries only
+    eng.set_serve_small(0)
+    log("the same sweep without the small-query server:")
+    served_lanes, same_lanes = served_sweep(eng, dev, sq, flags, alone, args.served_sweep_rates, args.seed + 42,
+                                            64 << 30, span_s=0.5)
+    eng.set_serve_small(16)
+    knee_lanes = served_knee(served_lanes)
+    log(f"served knee without the server: {knee_lanes}")
+    all_same = all_same and same_lanes
     bm = None
     if args.batch_s > 0 and (not nested or time_left(args) > 200):
         bm = batch_mode(eng, w.gstore, P, w.info["light_types"], w.info["t0"], args)
@@ -722,6 +736,7 @@ def run_config4(args, nested=False):
                   **{k: v for k, v in w.info.items() if k != "light_types"},
                   "queries_per_rate": len(sq), "heavy_queries": len(heavy_idx)},
                  {"roofline": roof, "kernels_per_heavy_query": kernels, "served": served, "served_knee": knee,
+                  "served_lanes_only": served_lanes, "served_knee_lanes_only": knee_lanes,
                   "batch_mode": bm,
                   "plain_queries_per_s": len(sq) / t_plain, "device_build": device_build,
                   "light_batched_queries_per_s": len(light) / t_light, "tuning": tuning, "heavy_tuning": htuning,

@@ -97,6 +97,15 @@ int wke_set_tuning(wke_engine *e, const wkg_batch_tuning *t, int heavy);
  * per-step path when the graph is resident, else a heavy executor lane; 0 a
  * heavy executor lane; 1 the worker thread (resident graphs only). */
 int wke_set_heavy_mode(wke_engine *e, int mode);
+/* Light queries of wke_serve / wke_serve_closed that start from an index or
+ * a constant and have at most 8 patterns go first to the small-query server
+ * (wkg_query_small_submit), which admits each one as soon as one of its
+ * `workgroups` workgroups is free, with no batch boundary; one that
+ * outgrows its limits goes on to the light lanes. Default 16; 0 sends
+ * every light query to the light lanes. The counters of the last serve:
+ * queries the server answered and queries it declined. */
+int wke_set_serve_small(wke_engine *e, int workgroups);
+int wke_serve_small_stats(const wke_engine *e, uint64_t *answered, uint64_t *declined);
 /* Counters of the last wke_serve: plain-path fallbacks on the serving
  * thread, queries retried alone, queries run by the worker thread, and the
  * worker's longest backlog. */

@@ -345,6 +345,11 @@ int wkg_query_small_collect(wkg_ctx *ctx, uint64_t ticket, int wait, int32_t *ce
 /* Server workgroups (default 4, at most half the CUs; 0: one launch per
  * query). Stops a running server; the next query starts the new shape. */
 int wkg_set_small_query_server(wkg_ctx *ctx, int workgroups);
+/* A server workgroup exits after idle_s without a query (default 200 us)
+ * and takes no query after life_s (default 20 ms, at most 10 s); the next
+ * query launches it again. Work on a stream that shares the server's
+ * hardware queue waits behind it up to that long. */
+int wkg_set_small_query_server_times(wkg_ctx *ctx, double idle_s, double life_s);
 
 /* Launch and tile shapes (0 = default). The autotuner below picks them. */
 typedef struct wkg_batch_tuning {

@@ -179,14 +179,22 @@ def test_served_lanes_alone_equals_in_mix(eng_small, graph_small):
     heavy = np.array([len(q) >= 20 for q in qs], dtype=np.uint8)
     arrivals = np.cumsum(rng.exponential(0.0002, size=len(qs)))
     eng_small.configure(max_queries=256, max_patterns=8, arena_bytes=1 << 30)
-    print("served: 200 arrivals, 2 light lanes + heavy lane", flush=True)
-    res, wall = eng_small.serve(qs, arrivals, heavy=heavy, table_budget_bytes=2 << 30, light_lanes=2)
-    assert wall > 0
-    for q, r in zip(qs, res):
-        alone = eng_small.run_query(q)
-        assert r.status == 0 and r.table.shape == alone.table.shape and np.array_equal(r.table, alone.table), q
-        assert r.expanded == alone.expanded, q
-        assert r.latency_s > 0
+    # light queries through the small-query server (admitted one by one),
+    # then through the light lanes only
+    for wg in (16, 0):
+        eng_small.set_serve_small(wg)
+        print(f"served: 200 arrivals, small-query server {wg} workgroups, 2 light lanes + heavy lane", flush=True)
+        res, wall = eng_small.serve(qs, arrivals, heavy=heavy, table_budget_bytes=2 << 30, light_lanes=2)
+        assert wall > 0
+        st = eng_small.serve_stats()
+        assert (st["small_answered"] > 50) if wg else (st["small_answered"] == 0), st
+        for q, r in zip(qs, res):
+            _same(gs, q, r)  # the oracle's answer
+            alone = eng_small.run_query(q)
+            assert r.status == 0 and r.table.shape == alone.table.shape and np.array_equal(r.table, alone.table), q
+            assert r.expanded == alone.expanded, q
+            assert r.latency_s > 0
+    eng_small.set_serve_small(16)
     # a budget too small for most batches still answers every query
     res2, _ = eng_small.serve(qs[:60], arrivals[:60] * 0, heavy=heavy[:60], table_budget_bytes=4 << 20,
                               light_lanes=2)

@@ -164,7 +164,9 @@ struct wke_engine {
   std::mutex plain_mu;
   int heavy_mode = -1;  // wke_set_heavy_mode
   bool small_queries = true;  // wke_set_small_queries: try wkg_query_small first
+  int serve_small = 16;       // wke_set_serve_small: small-query server workgroups while serving (0: none)
   uint64_t sv_fallbacks = 0, sv_solo = 0, sv_worker = 0, sv_backlog = 0;  // last wke_serve
+  uint64_t sv_small = 0, sv_small_declined = 0;
   // device-resident executor shape (wke_configure) and tuning (wke_autotune)
   int cfg_queries = 1024, cfg_patterns = 8;
   uint64_t cfg_arena = 16ull << 30;
@@ -674,6 +676,19 @@ int wke_set_heavy_mode(wke_engine *e, int mode) {
   return WKG_OK;
 }
 
+int wke_set_serve_small(wke_engine *e, int workgroups) {
+  if (!e || workgroups < 0) return WKG_EINVAL;
+  e->serve_small = workgroups;
+  return WKG_OK;
+}
+
+int wke_serve_small_stats(const wke_engine *e, uint64_t *answered, uint64_t *declined) {
+  if (!e) return WKG_EINVAL;
+  if (answered) *answered = e->sv_small;
+  if (declined) *declined = e->sv_small_declined;
+  return WKG_OK;
+}
+
 int wke_serve_stats(const wke_engine *e, uint64_t *fallbacks, uint64_t *solo, uint64_t *worker_queries,
                     uint64_t *worker_backlog) {
   if (!e) return WKG_EINVAL;
@@ -786,11 +801,21 @@ int serve_loop(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq
   const bool resident = wkg_graph_framed(e->ctx) == 0;
   const bool worker = any_heavy && resident && e->heavy_mode != 0;
   e->sv_fallbacks = e->sv_solo = e->sv_worker = e->sv_backlog = 0;
+  e->sv_small = e->sv_small_declined = 0;
+  // light queries whose tables stay small: the persistent small-query
+  // server, admitted one by one as they arrive (a query its limits decline
+  // goes on to the light lanes)
+  const bool small = resident && e->serve_small > 0;
+  if (small && wkg_set_small_query_server(e->ctx, e->serve_small) != WKG_OK) return e->fail_ctx(WKG_EINVAL);
   const bool heavy_lane = any_heavy && !worker;
   const uint64_t heavy_arena = (heavy_lane || worker) ? table_budget_bytes / 2 : 0;
   const uint64_t light_arena = (table_budget_bytes - heavy_arena) / (uint64_t)light_lanes;
   std::vector<Lane> lanes;
   PlainWorker pw;
+  std::deque<int> sq_wait;                          // light queries waiting for a server slot
+  std::vector<std::pair<int, uint64_t>> sq_run;     // (query, ticket) on the server
+  std::vector<int32_t> sq_buf(WKG_SMALL_QUERY_CELLS);
+  const int sq_keep = 4;                            // the context's server shape after serving
   auto release = [&] {
     if (pw.th.joinable()) {
       {
@@ -807,6 +832,14 @@ int serve_loop(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq
       }
       wkg_batch_destroy(l.b);
     }
+    std::vector<int32_t> sink(WKG_SMALL_QUERY_CELLS);
+    for (const auto &qt : sq_run) {  // every ticket is collected
+      uint64_t r, x;
+      int c;
+      (void)wkg_query_small_collect(e->ctx, qt.second, 1, sink.data(), sink.size(), &r, &c, &x);
+    }
+    sq_run.clear();
+    if (small) (void)wkg_set_small_query_server(e->ctx, sq_keep);
   };
   for (int i = 0; i < light_lanes + (heavy_lane ? 1 : 0); i++) {
     Lane l;
@@ -930,6 +963,9 @@ int serve_loop(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq
         done(q, rc);
       } else if ((int)plans[q].size() > e->cfg_patterns) {
         run_plain(q);
+      } else if (small && !(any_heavy && heavy[q]) && plans[q].size() <= 8 &&
+                 (plans[q][0].kind == WKG_STEP_I2U || plans[q][0].kind == WKG_STEP_C2U) && sq_wait.size() < 64) {
+        sq_wait.push_back(q);
       } else if (worker && heavy[q]) {
         {
           std::lock_guard<std::mutex> lk(pw.mu);
@@ -958,6 +994,51 @@ int serve_loop(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq
         done(d.q, d.rc);
         progressed = true;
       }
+    }
+    // the small-query server: answers, then new tickets for free slots
+    for (size_t i = 0; i < sq_run.size();) {
+      const int q = sq_run[i].first;
+      uint64_t r = 0, x = 0;
+      int c = 0;
+      const int rc = wkg_query_small_collect(e->ctx, sq_run[i].second, 0, sq_buf.data(), sq_buf.size(), &r, &c, &x);
+      if (rc == WKG_EAGAIN) {
+        i++;
+        continue;
+      }
+      sq_run[i] = sq_run.back();
+      sq_run.pop_back();
+      progressed = true;
+      if (rc == WKG_ERANGE) {  // past the limits: a light lane runs it
+        e->sv_small_declined++;
+        wait[0].push_back(q);
+        continue;
+      }
+      e->sv_small++;
+      res[q].status = rc;
+      if (rc == WKG_OK) {
+        res[q].rows = r;
+        res[q].cols = c;
+        res[q].expanded = x;
+        const uint64_t n = r * (uint64_t)c;
+        if (n && !(cl && cl->blind(q))) {
+          res[q].table = (int32_t *)malloc(n * 4);
+          if (res[q].table) memcpy(res[q].table, sq_buf.data(), n * 4);
+          else res[q].status = WKG_ENOMEM, res[q].rows = 0;
+        }
+      } else {
+        e->err_slot() = wkg_last_error(e->ctx);
+      }
+      done(q, res[q].status);
+    }
+    while (!sq_wait.empty()) {
+      const int q = sq_wait.front();
+      uint64_t ticket;
+      const int rc = wkg_query_small_submit(e->ctx, plans[q].data(), (int)plans[q].size(), &ticket);
+      if (rc == WKG_EAGAIN) break;  // every slot in flight
+      sq_wait.pop_front();
+      progressed = true;
+      if (rc == WKG_OK) sq_run.emplace_back(q, ticket);
+      else wait[0].push_back(q);  // (e.g. a framed graph)
     }
     // retire finished batches
     for (Lane &l : lanes) {
@@ -1023,6 +1104,10 @@ int serve_loop(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq
       progressed = true;
     }
     if (progressed) continue;
+    if (!sq_run.empty()) {  // answers come within microseconds: poll, do not sleep
+      std::this_thread::yield();
+      continue;
+    }
     bool any_busy = pw.pending > 0;
     for (Lane &l : lanes) any_busy |= l.busy;
     if (!cl && !any_busy && next < nq) {

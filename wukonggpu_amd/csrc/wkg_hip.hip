@@ -1700,12 +1700,10 @@ struct SQServer {
 
 namespace {
 
-// A server workgroup exits after kSQIdleS without a query and takes no query
-// after kSQLifeS (the next query launches it again). Its stream may share a
-// hardware queue with other streams of the process: work queued there waits
-// at most that long behind it.
-constexpr double kSQIdleS = 200e-6;
-constexpr double kSQLifeS = 0.02;
+// A server workgroup exits after ctx->sq_idle_s without a query and takes
+// no query after ctx->sq_life_s (the next query launches it again). Its
+// stream may share a hardware queue with other streams of the process: work
+// queued there waits at most that long behind it.
 
 int sq_check_plan(wkg_ctx *ctx, const wkg_plan_step *steps, int n) {
   if (!ctx->loaded) return ctx->fail(WKG_ESTATE, "no graph loaded");
@@ -1744,6 +1742,8 @@ void sq_fill(wkg_ctx *ctx, const wkg_plan_step *steps, int n, SmallQuery &q) {
 }
 
 int sq_init(wkg_ctx *ctx) {
+  static std::mutex init_mu;  // (two host threads may make a context's first small query)
+  std::lock_guard<std::mutex> lk(init_mu);
   if (ctx->sqs) return WKG_OK;
   SQServer *v = new (std::nothrow) SQServer;
   if (!v) return ctx->fail(WKG_ENOMEM, "small-query server state");
@@ -1788,8 +1788,8 @@ int sq_alive(wkg_ctx *ctx, SQServer *v, bool &alive) {
 // Launches the server unless it runs (v->mu held).
 int sq_ensure(wkg_ctx *ctx, SQServer *v) {
   const auto now = std::chrono::steady_clock::now();
-  if (v->launched && now - v->submit_t < std::chrono::duration<double>(kSQIdleS / 2) &&
-      now - v->launch_t < std::chrono::duration<double>(kSQLifeS / 2))
+  if (v->launched && now - v->submit_t < std::chrono::duration<double>(ctx->sq_idle_s / 2) &&
+      now - v->launch_t < std::chrono::duration<double>(ctx->sq_life_s / 2))
     return WKG_OK;
   bool alive;
   if (int rc = sq_alive(ctx, v, alive)) return rc;
@@ -1800,8 +1800,8 @@ int sq_ensure(wkg_ctx *ctx, SQServer *v) {
   b.head = v->d_head;
   b.slots = (SQSlot *)(v->d + 128);
   b.nslots = SQServer::kSQSlots;
-  b.idle_ticks = (uint64_t)(kSQIdleS * v->tick_hz);
-  b.life_ticks = (uint64_t)(kSQLifeS * v->tick_hz);
+  b.idle_ticks = (uint64_t)(ctx->sq_idle_s * v->tick_hz);
+  b.life_ticks = (uint64_t)(ctx->sq_life_s * v->tick_hz);
   const int nwg = std::max(1, std::min(ctx->sq_workgroups, ctx->ncu / 2));
   if (ctx->sorted) k_sq_server<true><<<nwg, kSQThreads, 0, v->stream>>>(ctx->view(), b);
   else k_sq_server<false><<<nwg, kSQThreads, 0, v->stream>>>(ctx->view(), b);
@@ -1978,6 +1978,14 @@ int wkg_set_small_query_server(wkg_ctx *ctx, int workgroups) {
   if (!ctx || workgroups < 0) return WKG_EINVAL;
   ctx->sq_stop();  // the next query launches the new shape
   ctx->sq_workgroups = workgroups;
+  return WKG_OK;
+}
+
+int wkg_set_small_query_server_times(wkg_ctx *ctx, double idle_s, double life_s) {
+  if (!ctx || !(idle_s > 0) || !(life_s >= idle_s) || life_s > 10.0) return WKG_EINVAL;
+  ctx->sq_stop();
+  ctx->sq_idle_s = idle_s;
+  ctx->sq_life_s = life_s;
   return WKG_OK;
 }
 

@@ -144,6 +144,7 @@ struct wkg_ctx {
   uint64_t *noalloc_scratch = nullptr;  // this device's no-alloc step scratch (wkg_step_k2u_noalloc)
   SQServer *sqs = nullptr;   // the persistent small-query server (created on first use)
   int sq_workgroups = 4;     // its workgroups (0: one launch per small query)
+  double sq_idle_s = 200e-6, sq_life_s = 0.02;  // a server workgroup's idle exit and life
   uint64_t *h_rb = nullptr;  // pinned read-back block: counters, the signal flag, small-table staging
   uint64_t *h_rb_dev = nullptr;  // the same block's device address
   uint64_t rb_seq = 0;       // the last read-back's signal value

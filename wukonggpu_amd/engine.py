@@ -251,6 +251,10 @@ class Device:
         """Persistent small-query server workgroups (0: one launch per query)."""
         self._check(N.hip_lib().wkg_set_small_query_server(self._h, workgroups))
 
+    def set_small_query_server_times(self, idle_s, life_s):
+        """A server workgroup's idle exit and life (seconds)."""
+        self._check(N.hip_lib().wkg_set_small_query_server_times(self._h, float(idle_s), float(life_s)))
+
     @staticmethod
     def _plan_array(plan):
         arr = (N.PlanStep * max(len(plan), 1))()
@@ -478,13 +482,21 @@ class GPUEngine:
         if rc != 0:
             raise DeviceError(rc, "wke_set_heavy_mode failed")
 
+    def set_serve_small(self, workgroups):
+        """Small-query server workgroups for served light queries (0: light lanes only)."""
+        rc = N.engine_lib().wke_set_serve_small(self._h, int(workgroups))
+        if rc != 0:
+            raise DeviceError(rc, "wke_set_serve_small failed")
+
     def serve_stats(self):
-        """Counters of the last serve(): fallbacks, solo retries, worker queries, worker backlog."""
+        """Counters of the last serve(): fallbacks, solo retries, worker queries, worker backlog,
+        queries the small-query server answered and declined."""
         L = N.engine_lib()
-        v = [C.c_uint64() for _ in range(4)]
-        L.wke_serve_stats(self._h, *[C.byref(x) for x in v])
+        v = [C.c_uint64() for _ in range(6)]
+        L.wke_serve_stats(self._h, *[C.byref(x) for x in v[:4]])
+        L.wke_serve_small_stats(self._h, C.byref(v[4]), C.byref(v[5]))
         return {k: int(x.value) for k, x in zip(("plain_fallbacks", "solo_retries", "worker_queries",
-                                                  "worker_backlog"), v)}
+                                                  "worker_backlog", "small_answered", "small_declined"), v)}
 
     def set_tuning(self, heavy=False, **shape):
         """Pins the executor's launch and tile shape (the keys of autotune's result)."""
