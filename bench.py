@@ -244,6 +244,9 @@ def run_config5(args, quick=False, nested=False):
     dev.load(w.gstore)
     load_s = time.time() - t1
     eng = GPUEngine(dev)
+    # a plain 1 GiB device copy (k_copy_calibrate): the reference point of
+    # the HBM counters in a profiled run of this command
+    copy_gbs = dev.copy_bandwidth(1 << 30, 3)
     full, sample = w.queries["full"], w.queries["sample"]
     dev.set_table_limit(C5_LIMIT)
     for _ in range(max(1, args.warmup)):
@@ -277,7 +280,7 @@ def run_config5(args, quick=False, nested=False):
               **{k: v for k, v in w.info.items()}, "query": full, "result_rows": int(r.table.shape[0]),
               "expanded_edges_per_query": int(r.expanded), "row_half_splits_per_query": splits,
               "p50_ms": pct(lat, 0.5) * 1e3, "p99_ms": pct(lat, 0.99) * 1e3, "graph_build_s": build_s,
-              "graph_load_s": load_s, "ms_per_query_with_kernel_events": prof_ms,
+              "graph_load_s": load_s, "ms_per_query_with_kernel_events": prof_ms, "copy_kernel_gbs": copy_gbs,
               "kernels_per_query": kernels, "gpu_kernel_ms_per_query": sum(v["ms"] for v in kernels.values())}
     if quick:
         out = _line(value, "expanded edges/s", args.steps, args.warmup, elapsed / args.steps * 1e3, config,

@@ -331,8 +331,12 @@ typedef struct wkg_plan_step {
  * into cells (cap cells; WKG_SMALL_QUERY_CELLS always suffices), or
  * WKG_ERANGE when a table outgrew the limits (or the graph is framed):
  * then nothing is returned and the caller runs the query step by step.
+ * An index or constant start whose list alone outgrew the limits is
+ * remembered while the graph is loaded: the next query from it is declined
+ * at once, with no launch.
  * wkg_query_small_submit / _collect are the same, split: submit returns a
- * ticket at once (WKG_EAGAIN when all 64 slots are in flight), collect
+ * ticket at once (WKG_EAGAIN when all 64 slots are in flight; WKG_ERANGE for
+ * a start remembered as too big), collect
  * returns its answer, or WKG_EAGAIN while it runs when wait is 0. Every
  * submitted ticket must be collected once. A context's calls may come from
  * several host threads. */

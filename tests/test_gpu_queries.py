@@ -8,7 +8,7 @@ import pytest
 
 import oracle
 from tests.conftest import VB, random_queries
-from wukonggpu_amd import Device, GPUEngine, GStore, Triples
+from wukonggpu_amd import Device, DeviceError, GPUEngine, GStore, Triples
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -558,6 +558,28 @@ def test_small_query_server_tickets(graph_hubs, graph_small):
         assert ok > len(qs) // 2, ok
         time.sleep(0.01)  # every workgroup idles out
         ok += check(qs[0], dev.query_small(plan_of(qs[0])))
+    # a start list past the limits: declined by the server once, then at once
+    from tests.conftest import small_graph
+    _, gs_big = small_graph(seed=12, n=20000, preds=2, triples=40000, types=(10000,))
+    dev2 = Device(0)
+    dev2.load(gs_big)
+    q_big = plan_of([4, 1, 0, -1])  # the 10000-member type's index
+    dev2.profile(True)
+    assert dev2.query_small(q_big)[0] == "declined"
+    n1 = dev2.profile_read("small_query")[0]
+    assert dev2.query_small(q_big)[0] == "declined"
+    with pytest.raises(DeviceError) as ei:  # submitting it is declined at once as well
+        dev2.query_small_submit(q_big)
+    assert ei.value.rc == -4
+    n2 = dev2.profile_read("small_query")[0]
+    dev2.profile(False)
+    assert n1 == 1 and n2 == 1, (n1, n2)  # no second run on the device
+    eng2 = GPUEngine(dev2)
+    want, wex, _ = oracle.run_query(gs_big, [4, 1, 0, -1])
+    r = eng2.run_query([4, 1, 0, -1])
+    assert np.array_equal(r.table, want) and r.expanded == wex
+    eng2.close()
+    dev2.close()
     # a context destroyed while its server runs, then a new one on another graph
     dev.query_small(plan_of(qs[1]))
     dev.close()

@@ -71,6 +71,10 @@ int log2_floor(uint64_t x) {
 
 void wkg_ctx::free_graph() {
   sq_stop();  // the server reads the graph it was launched with
+  {
+    std::lock_guard<std::mutex> lk(sq_big_mu);
+    sq_big_starts.clear();
+  }
   if (d_slots) (void)hipFree(d_slots);
   if (d_edges) (void)hipFree(d_edges);
   if (d_meta) (void)hipFree(d_meta);

@@ -678,16 +678,16 @@ __device__ __forceinline__ uint64_t sq_uniform(uint64_t v) {
 
 // Runs sh.q; wave 0 writes the result table to dst (rows x cols cells, 16-
 // byte aligned) unless the status is 1. res = {status, rows, cols,
-// expanded}. Every thread of the workgroup calls it; wave 0 then releases
-// its stores to the host.
+// expanded, the step that declined}. Every thread of the workgroup calls
+// it; wave 0 then releases its stores to the host.
 template <bool SORTED, bool FRAMED>
-__device__ __forceinline__ void sq_run(const GraphView &g, SQShared &sh, int32_t *dst, uint64_t (&res)[4]) {
+__device__ __forceinline__ void sq_run(const GraphView &g, SQShared &sh, int32_t *dst, uint64_t (&res)[5]) {
   const SmallQuery &q = sh.q;
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave, l8 = lane & 7, gbase = lane & ~7u;
   uint64_t expanded = 0, status = 0;
-  uint32_t rows = 0, cols = 0, cur = 0;
+  uint32_t rows = 0, cols = 0, cur = 0, l = 0;
   const uint32_t nsteps = sq_uniform((uint32_t)q.nsteps);
-  for (uint32_t l = 0; l < nsteps; l++) {
+  for (; l < nsteps; l++) {
     // the step and its metadata are the same in every lane: into scalar registers
     wkg_plan_step d;
     d.kind = (int32_t)sq_uniform((uint32_t)q.steps[l].kind);
@@ -873,6 +873,7 @@ __device__ __forceinline__ void sq_run(const GraphView &g, SQShared &sh, int32_t
   res[1] = rows;
   res[2] = cols;
   res[3] = expanded;
+  res[4] = l;
 }
 
 // One query, one launch: the result into the pinned staging area, then
@@ -882,11 +883,11 @@ __global__ __launch_bounds__(kSQThreads) void k_small_query(GraphView g, SmallQu
   __shared__ SQShared sh;
   if (threadIdx.x == 0) sh.q = q;
   __syncthreads();
-  uint64_t res[4];
+  uint64_t res[5];
   sq_run<SORTED, FRAMED>(g, sh, stage, res);
   if (threadIdx.x < kWave) {
     __threadfence_system();
-    if (threadIdx.x == 0) publish(sg, res, 4);
+    if (threadIdx.x == 0) publish(sg, res, 5);
   }
 }
 
@@ -904,7 +905,8 @@ __global__ __launch_bounds__(kSQThreads) void k_small_query(GraphView g, SmallQu
 // others run are admitted as soon as a workgroup is free.
 struct SQResp {
   uint64_t status, rows, cols, expanded, ticks;
-  uint64_t pad[2];
+  uint64_t step;  // status 1: the step that outgrew the limits
+  uint64_t pad;
   uint64_t done;  // ticket + 1, written last
 };
 struct SQSlot {
@@ -962,7 +964,7 @@ __global__ __launch_bounds__(kSQThreads) void k_sq_server(GraphView g, SQBox box
     for (uint32_t i = threadIdx.x; i < sizeof(SmallQuery) / 4; i += kSQThreads)
       qw[i] = __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __syncthreads();
-    uint64_t res[4];
+    uint64_t res[5];
     sq_run<SORTED, false>(g, sh, slot->cells, res);
     if (threadIdx.x < kWave) __threadfence_system();  // (wave 0 wrote the cells)
     if (threadIdx.x == 0) {
@@ -970,6 +972,7 @@ __global__ __launch_bounds__(kSQThreads) void k_sq_server(GraphView g, SQBox box
       slot->resp.rows = res[1];
       slot->resp.cols = res[2];
       slot->resp.expanded = res[3];
+      slot->resp.step = res[4];
       slot->resp.ticks = wall_clock64() - t0;
       __hip_atomic_store(&slot->resp.done, claim + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
       t_idle = wall_clock64();

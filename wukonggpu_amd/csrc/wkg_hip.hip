@@ -1732,6 +1732,25 @@ int sq_check_plan(wkg_ctx *ctx, const wkg_plan_step *steps, int n) {
   return WKG_OK;
 }
 
+std::tuple<int, int, int, int64_t> sq_start_key(const wkg_plan_step &d) {
+  return {d.kind, d.pid, d.dir, d.kind == WKG_STEP_C2U ? d.vid : 0};
+}
+
+// A start list known to outgrow the limits: WKG_ERANGE at once.
+int sq_known_big(wkg_ctx *ctx, const wkg_plan_step *steps) {
+  std::lock_guard<std::mutex> lk(ctx->sq_big_mu);
+  if (!ctx->sq_big_starts.count(sq_start_key(steps[0]))) return WKG_OK;
+  ctx->prof_add_bytes("small_query_declined", 1.0);  // a count
+  return ctx->fail(WKG_ERANGE, "the start list outgrows the small-query limits (%u cells)", kSQCells);
+}
+
+void sq_note_decline(wkg_ctx *ctx, const wkg_plan_step &start, uint64_t step) {
+  if (step != 0) return;
+  std::lock_guard<std::mutex> lk(ctx->sq_big_mu);
+  if (ctx->sq_big_starts.size() >= 65536) ctx->sq_big_starts.clear();  // (bounded)
+  ctx->sq_big_starts.insert(sq_start_key(start));
+}
+
 void sq_fill(wkg_ctx *ctx, const wkg_plan_step *steps, int n, SmallQuery &q) {
   memset(&q, 0, sizeof q);
   for (int l = 0; l < n; l++) {
@@ -1864,6 +1883,7 @@ int sq_collect(wkg_ctx *ctx, uint64_t t, bool wait, int32_t *cells, uint64_t cap
   }
   const uint64_t status = sl->resp.status, r = sl->resp.rows, c = sl->resp.cols;
   if (expanded) *expanded = sl->resp.expanded;
+  if (status != 0) sq_note_decline(ctx, sl->q.steps[0], sl->resp.step);
   if (ctx->profiling) {
     std::lock_guard<std::mutex> lk(ctx->prof_mu);
     wkg_ctx::KStat &k = ctx->prof_stats["small_query"];
@@ -1906,6 +1926,7 @@ int sq_one_launch(wkg_ctx *ctx, const wkg_plan_step *steps, int n, void *stream,
   if (int rc = ctx->wait_signal(sg, s)) return rc;
   const uint64_t status = ctx->h_rb[0], r = ctx->h_rb[1], c = ctx->h_rb[2];
   if (expanded) *expanded = ctx->h_rb[3];
+  if (status != 0) sq_note_decline(ctx, steps[0], ctx->h_rb[4]);
   ctx->prof_add_bytes(status ? "small_query_declined" : "small_query_taken", 1.0);  // counts
   if (status != 0)
     return ctx->fail(WKG_ERANGE, "a table outgrew the small-query limits (%u input rows, %u cells)", kSQRows, kSQCells);
@@ -1953,6 +1974,7 @@ int wkg_query_small(wkg_ctx *ctx, const wkg_plan_step *steps, int n, void *strea
   *cols = 0;
   if (expanded) *expanded = 0;
   if (int rc = sq_check_plan(ctx, steps, n)) return rc;
+  if (int rc = sq_known_big(ctx, steps)) return rc;
   if (ctx->sq_workgroups == 0) return sq_one_launch(ctx, steps, n, stream, cells, cap, rows, cols, expanded);
   uint64_t t;
   if (int rc = sq_submit(ctx, steps, n, &t)) return rc;
@@ -1962,6 +1984,7 @@ int wkg_query_small(wkg_ctx *ctx, const wkg_plan_step *steps, int n, void *strea
 int wkg_query_small_submit(wkg_ctx *ctx, const wkg_plan_step *steps, int n, uint64_t *ticket) {
   if (!ctx || !steps || !ticket) return WKG_EINVAL;
   if (int rc = sq_check_plan(ctx, steps, n)) return rc;
+  if (int rc = sq_known_big(ctx, steps)) return rc;
   return sq_submit(ctx, steps, n, ticket);
 }
 

@@ -7,6 +7,8 @@
 #include <list>
 #include <map>
 #include <mutex>
+#include <set>
+#include <tuple>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -145,6 +147,10 @@ struct wkg_ctx {
   SQServer *sqs = nullptr;   // the persistent small-query server (created on first use)
   int sq_workgroups = 4;     // its workgroups (0: one launch per small query)
   double sq_idle_s = 200e-6, sq_life_s = 0.02;  // a server workgroup's idle exit and life
+  // index / constant starts whose list alone outgrew the small-query limits
+  // (the graph does not change while loaded: such a query is declined at once)
+  std::set<std::tuple<int, int, int, int64_t>> sq_big_starts;
+  std::mutex sq_big_mu;
   uint64_t *h_rb = nullptr;  // pinned read-back block: counters, the signal flag, small-table staging
   uint64_t *h_rb_dev = nullptr;  // the same block's device address
   uint64_t rb_seq = 0;       // the last read-back's signal value
