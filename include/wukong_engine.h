@@ -97,6 +97,13 @@ int wke_set_tuning(wke_engine *e, const wkg_batch_tuning *t, int heavy);
  * per-step path when the graph is resident, else a heavy executor lane; 0 a
  * heavy executor lane; 1 the worker thread (resident graphs only). */
 int wke_set_heavy_mode(wke_engine *e, int mode);
+/* Plain path: a known_to_unknown step whose new column the next pattern
+ * filters (known_to_known back to an earlier column, or known_to_const)
+ * runs with that filter in one call (wkg_step_k2u_filter), which does not
+ * write the fan-out when the filter can run on its lineage. Same tables and
+ * work counts; on by default, 0 runs the two steps one after the other. */
+int wke_set_fused_filters(wke_engine *e, int on);
+
 /* Light queries of wke_serve / wke_serve_closed that start from an index or
  * a constant and have at most 8 patterns go first to the small-query server
  * (wkg_query_small_submit), which admits each one as soon as one of its

@@ -252,6 +252,19 @@ int wkg_step_k2u_noalloc(wkg_ctx *ctx, const int32_t *in, uint64_t rows, int col
 
 /* known_to_known: keep the rows whose row[end_col] is in the edge list of
  * (row[start_col], pid, dir). RCache::known_to_known, core/rcache.hpp:417-612. */
+/* known_to_unknown (as wkg_step_k2u) followed by a filter whose start
+ * column is the new column: filter_kind WKG_STEP_K2K with end_col an input
+ * column, or WKG_STEP_K2C with const_end, on filter_pid / filter_dir. When
+ * the filter can decide the fan-out's rows from its lineage (a parent row
+ * and a list entry each) the fan-out is never written: *fused = 1 and *out
+ * is the filter's output. Otherwise *fused = 0 and *out is the
+ * known_to_unknown output (the caller runs the filter on it). Either way
+ * the tables and work counts are those of the two calls made one after the
+ * other (core/rcache.hpp:232-414, then the filter's handler); this call has
+ * no counterpart in the reference, it fuses two of its handlers. */
+int wkg_step_k2u_filter(wkg_ctx *ctx, const wkg_table *in, int start_col, int pid, int dir, int filter_kind,
+                        int filter_pid, int filter_dir, int end_col, int64_t const_end, void *stream, wkg_table **out,
+                        int *fused, uint64_t *expanded_k2u, uint64_t *expanded_filter);
 int wkg_step_k2k(wkg_ctx *ctx, const wkg_table *in, int start_col, int pid, int dir,
                  int end_col, void *stream, wkg_table **out, uint64_t *expanded);
 

@@ -403,15 +403,18 @@ def test_fanout_filters_match_oracle(graph_hubs):
            [hub, 2, 1, -1, -1, 2, 0, -2, -2, 3, 1, -3, -3, 2, 0, -2]]
     want = [oracle.run_query(gs, q)[:2] for q in qs]
     try:
-        for rows, lineage, launch, limit in [(0, True, 0, 0), (0, True, 4096, 0), (0, True, 0, 4 << 20),
-                                             (1 << 22, True, 0, 0), (0, False, 0, 0)]:
+        for rows, lineage, launch, limit, fuse in [(0, True, 0, 0, True), (0, True, 4096, 0, True),
+                                                   (0, True, 0, 4 << 20, True), (1 << 22, True, 0, 0, True),
+                                                   (0, False, 0, 0, True), (0, True, 0, 0, False),
+                                                   (0, True, 0, 4 << 20, False)]:
             dev.set_lineage(lineage)
             dev.set_group_filter_rows(rows)
             dev.set_launch_rows(launch)
             dev.set_table_limit(limit)
+            eng.set_fused_filters(fuse)
             dev.profile(True)
             print("fan-out filters: group filter rows", rows, "lineage", lineage, "launch rows", launch,
-                  "table limit", limit, flush=True)
+                  "table limit", limit, "fused with the fan-out", fuse, flush=True)
             for q, (wt, wex) in zip(qs, want):
                 res = eng.run_query(q)
                 assert res.status == 0, q
@@ -420,19 +423,64 @@ def test_fanout_filters_match_oracle(graph_hubs):
                 assert res.expanded == wex, q
             taken = dev.profile_read("fanout_filters")[2]
             by_units, by_rows = dev.profile_read("fanout_units")[2], dev.profile_read("fanout_row_tiles")[2]
+            fused = dev.profile_read("fused_fanouts")[2]
             dev.profile(False)
             if rows == 0 and lineage:
                 assert taken > 0
                 if limit == 0:
                     assert by_units > 0 and by_rows > 0, (by_units, by_rows)
+                assert (fused > 0) if fuse else (fused == 0), fused
             elif not lineage:
-                assert taken == 0
+                assert taken == 0 and fused == 0
     finally:
         dev.set_launch_rows(0)
         dev.set_group_filter_rows(1 << 22)
         dev.set_lineage(True)
         dev.set_table_limit(0)
+        eng.set_fused_filters(True)
     eng.close()
+    dev.close()
+
+
+def test_k2u_filter_call(graph_hubs):
+    """wkg_step_k2u_filter against the two calls it stands for and the
+    oracle: fused (the fan-out never written) for a large fan-out, not fused
+    for a small input (the known_to_unknown output comes back and the filter
+    runs on it), with known_to_known and known_to_const filters."""
+    from wukonggpu_amd import STEP_K2C, STEP_K2K
+    _, gs = graph_hubs
+    dev = Device(0)
+    dev.load(gs)
+    dev.set_group_filter_rows(0)
+    try:
+        head = [7, 1, 0, -1, -1, 2, 1, -2, -2, 2, 0, -3]
+        t1, _ = dev.i2u(7, 0)
+        t2, _ = dev.k2u(t1, 0, 2, 1)
+        t3, _ = dev.k2u(t2, 1, 2, 0)  # ?1 ?2 ?3: the fan-out's input (hub rows)
+        _, wex_head, _ = oracle.run_query(gs, head)
+        _, wex_fan, _ = oracle.run_query(gs, head + [-3, 2, 1, -4])
+        cases = [(STEP_K2K, 3, 1, 1, -1, [-4, 3, 1, -2]),
+                 (STEP_K2K, 3, 0, 2, -1, [-4, 3, 0, -3]),
+                 (STEP_K2C, 3, 1, -1, VB + 17, [-4, 3, 1, VB + 17])]
+        for kind, fp, fd, ec, cv, tail in cases:
+            q = head + [-3, 2, 1, -4] + tail
+            want, wex, _ = oracle.run_query(gs, q)
+            out, fused, e1, e2 = dev.k2u_filter(t3, 2, 2, 1, kind, fp, fd, ec, cv)
+            assert fused, (kind, fp, fd)
+            assert np.array_equal(out.to_host().reshape(want.shape), want), q
+            # the work counts of the two steps, as the oracle counts them
+            assert e1 == wex_fan - wex_head and e1 + e2 == wex - wex_head, (e1, e2)
+            out.free()
+        # a small input: not fused, the known_to_unknown output comes back
+        small, _ = dev.c2u(VB + 1, 2, 1)
+        out, fused, e1, e2 = dev.k2u_filter(small, 0, 2, 1, STEP_K2C, 3, 1, -1, VB + 7)
+        assert not fused and e2 == 0
+        want, wex, _ = oracle.run_query(gs, [VB + 1, 2, 1, -1, -1, 2, 1, -2])
+        assert np.array_equal(out.to_host().reshape(want.shape), want)
+        _, wex1, _ = oracle.run_query(gs, [VB + 1, 2, 1, -1])
+        assert e1 == wex - wex1
+    finally:
+        dev.set_group_filter_rows(1 << 22)
     dev.close()
 
 

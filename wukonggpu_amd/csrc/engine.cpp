@@ -165,6 +165,7 @@ struct wke_engine {
   int heavy_mode = -1;  // wke_set_heavy_mode
   bool small_queries = true;  // wke_set_small_queries: try wkg_query_small first
   int serve_small = 16;       // wke_set_serve_small: small-query server workgroups while serving (0: none)
+  bool fuse_filters = true;   // wke_set_fused_filters: known_to_unknown + filter on its new column in one call
   uint64_t sv_fallbacks = 0, sv_solo = 0, sv_worker = 0, sv_backlog = 0;  // last wke_serve
   uint64_t sv_small = 0, sv_small_declined = 0;
   // device-resident executor shape (wke_configure) and tuning (wke_autotune)
@@ -221,6 +222,29 @@ struct wke_engine {
     if (rc) return rc;
     wkg_table *o = nullptr;
     uint64_t ex = 0;
+    // known_to_unknown whose new column the next pattern filters: one call,
+    // and the fan-out it would write is not written when the filter can run
+    // on its lineage
+    if (fuse_filters && d.kind == WKG_STEP_K2U && (r.step + 1) * 4 < r.ncmd) {
+      wkg_step_desc d2;
+      int nc2;
+      std::string why;
+      if (dispatch(r.cmd, r.step + 1, nc, &d2, &nc2, why) == WKG_OK && d2.start_col == nc - 1 &&
+          (d2.kind == WKG_STEP_K2C || (d2.kind == WKG_STEP_K2K && d2.end_col < nc - 1))) {
+        int fused = 0;
+        uint64_t ex2 = 0;
+        rc = wkg_step_k2u_filter(ctx, r.t, d.start_col, d.pid, d.dir, d2.kind, d2.pid, d2.dir, d2.end_col, d2.vid,
+                                 stream, &o, &fused, &ex, &ex2);
+        if (rc) return fail_ctx(rc);
+        advance(r, o, nc, ex);
+        if (fused) {  // the filter is done too: its output is the table
+          r.step++;
+          r.col_num = nc2;
+          r.expanded += ex2;
+        }
+        return WKG_OK;
+      }
+    }
     switch (d.kind) {
       case WKG_STEP_I2U: rc = wkg_step_i2u(ctx, d.pid, d.dir, stream, &o, &ex); break;
       case WKG_STEP_C2U: rc = wkg_step_c2u(ctx, d.vid, d.pid, d.dir, stream, &o, &ex); break;
@@ -673,6 +697,12 @@ int wke_run_batched_packed(wke_engine *e, const int64_t *cmd, const int64_t *cmd
 int wke_set_heavy_mode(wke_engine *e, int mode) {
   if (!e || mode < -1 || mode > 1) return WKG_EINVAL;
   e->heavy_mode = mode;
+  return WKG_OK;
+}
+
+int wke_set_fused_filters(wke_engine *e, int on) {
+  if (!e) return WKG_EINVAL;
+  e->fuse_filters = on != 0;
   return WKG_OK;
 }
 

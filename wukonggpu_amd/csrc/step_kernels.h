@@ -1537,6 +1537,186 @@ __global__ __launch_bounds__(kBlock) void k_fanout_units(GraphView g, StepParams
   }
 }
 
+// The fan-out filter for long shared lists, a block of list entries at a
+// time: a workgroup takes kFBRows entries of one distinct list and up to
+// kFBGroup (= 64) of the parents sharing that list, and tests every parent
+// against every entry's list, holding the block in registers:
+//  - thread t takes entries t, t + 256, ...: their D entries are read once;
+//    an entry whose list has one element keeps it in a register (D holds
+//    it), so a (parent, entry) test is one compare;
+//  - the few entries with longer lists are tested before the parent loop,
+//    each by one wave with lane pi testing parent pi (the list's entries
+//    broadcast lane by lane, or a binary search per lane for a long sorted
+//    list), leaving a 64-bit parent mask per entry;
+//  - per parent, each wave's compares are ballots: 64 rows' kept flags at
+//    once. Kept rows are rare: the bit words start zeroed and only words
+//    with a kept row are written.
+// The output is the units' layout of k_fanout_units (unit = parent x block
+// of kFBRows rows, ranked ubase[parent] + block: bit words, kept rows,
+// edges examined, first row), so the tile scan and the compaction are
+// unchanged. Work counts follow the reference's scan: first match + 1, or
+// the degree.
+constexpr int kFBRows = 2048;
+constexpr int kFBPer = kFBRows / kBlock;
+constexpr int kFBGroup = 64;
+struct FanoutBlocks {
+  const uint64_t *xb;     // per distinct list (+1): first work index (blocks x parent groups)
+  const uint64_t *poff;   // per distinct list (+1): its parents' first slot in par
+  const uint32_t *par;    // compacted parents, grouped by list, row order within a list
+  const uint64_t *ubase;  // per compacted parent: rank of its first unit of kFBRows rows
+  const uint64_t *ln;     // per distinct list: its entries
+  uint64_t nl;
+  uint64_t *row0;         // per unit rank: its first row
+};
+
+template <bool SORTED>
+__global__ __launch_bounds__(kBlock) void k_fanout_blocks(GraphView g, StepParams p, FanoutFilter f, FanoutBlocks u) {
+  constexpr int W = kBlock / kWave;
+  constexpr uint32_t kNone = 0xffffffffu, kNoTarget = 0xfffffffeu;  // (ids and valid targets are below 2^31)
+  static_assert(kFBGroup == kWave, "lane pi tests parent pi");
+  __shared__ uint32_t s_boff[kFBRows], s_bdeg[kFBRows];
+  __shared__ uint16_t s_rowb[kFBRows];               // a multi-entry row's index among them
+  __shared__ uint64_t s_bmask[kFBRows];              // per multi-entry row: the parents in its list
+  __shared__ uint32_t s_tgt[kFBGroup], s_par[kFBGroup];
+  __shared__ uint64_t s_rank[kFBGroup];
+  __shared__ uint32_t s_kept[kFBGroup][W];
+  __shared__ uint64_t s_bex[W][kWave], s_tmp[W];
+  __shared__ uint32_t s_nb;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+  const uint64_t e = p.tile0 + blockIdx.x;
+  uint64_t L = 0, hi = u.nl - 1;  // the list: the last with xb <= e
+  while (L < hi) {
+    const uint64_t mid = (L + hi + 1) >> 1;
+    if (u.xb[mid] <= e) L = mid;
+    else hi = mid - 1;
+  }
+  const uint64_t cnt = u.poff[L + 1] - u.poff[L], ngr = (cnt + kFBGroup - 1) / kFBGroup;
+  const uint64_t e2 = e - u.xb[L], jb = e2 / ngr, grp = e2 % ngr;
+  const uint64_t n = u.ln[L], j0 = jb * kFBRows;
+  const uint32_t nrows = (uint32_t)(n - j0 < (uint64_t)kFBRows ? n - j0 : (uint64_t)kFBRows);
+  const uint32_t npg = (uint32_t)(cnt - grp * kFBGroup < (uint64_t)kFBGroup ? cnt - grp * kFBGroup : kFBGroup);
+  const DevMeta m = g.meta[p.pid < g.num_metas ? p.pid : 0];
+  if (tid == 0) s_nb = 0;
+  if (tid < kFBGroup) {
+    uint32_t tg = kNoTarget, i = 0;
+    if (tid < npg) {
+      i = u.par[u.poff[L] + grp * kFBGroup + tid];
+      if (f.ec >= 0) {
+        const int32_t cell = f.P[(uint64_t)f.info[i].x * f.pc + (uint32_t)f.ec];
+        if (cell >= 0) tg = (uint32_t)cell;
+      } else if (p.target_ok) {
+        tg = p.target;
+      }
+      s_rank[tid] = u.ubase[i] + jb;
+    }
+    s_par[tid] = i;
+    s_tgt[tid] = tg;
+  }
+  s_kept[tid / W][tid % W] = 0;  // (kFBGroup x W = kBlock entries)
+  __syncthreads();
+  // the block's D entries (the list's D entries start at any of its parents' pbase)
+  const uint2 *Dp = f.D + f.pbase[s_par[0]] + j0;
+  uint32_t c0[kFBPer], dg[kFBPer];
+  uint64_t ex1 = 0;  // single-entry rows: one edge examined each, for every parent
+#pragma unroll
+  for (int k = 0; k < kFBPer; k++) {
+    const uint32_t row = tid + k * kBlock;
+    const uint2 d = row < nrows ? Dp[row] : make_uint2(0u, 0u);
+    dg[k] = d.y;
+    c0[k] = d.y == 1 ? d.x : kNone;
+    ex1 += d.y == 1 ? 1u : 0u;
+    if (d.y >= 2) {
+      const uint32_t b = atomicAdd(&s_nb, 1u);
+      s_rowb[row] = (uint16_t)b;
+      s_boff[b] = d.x;
+      s_bdeg[b] = d.y;
+    }
+  }
+  __syncthreads();
+  const uint32_t nb = s_nb;
+  {
+    // each multi-entry list by one wave, lane pi testing parent pi
+    const uint32_t tgt = s_tgt[lane];
+    uint64_t exl = 0;
+    for (uint32_t b = wave; b < nb; b += W) {
+      const uint32_t off = s_boff[b], deg = s_bdeg[b];
+      bool fnd = false;
+      uint32_t x = deg;
+      if (tgt != kNoTarget) {
+        if (SORTED && deg > kWave) {
+          uint32_t lo = 0, hi2 = deg;
+          while (lo < hi2) {
+            const uint32_t mid = (lo + hi2) >> 1;
+            if (edge_at<false>(g, m.edge_start, m.eframe_base, (uint64_t)off + mid) < tgt) lo = mid + 1;
+            else hi2 = mid;
+          }
+          if (lo < deg && edge_at<false>(g, m.edge_start, m.eframe_base, (uint64_t)off + lo) == tgt) {
+            fnd = true;
+            x = lo + 1;
+          }
+        }
+      }
+      if (!SORTED || deg <= kWave) {  // wave-uniform: 64 entries at a time, broadcast lane by lane
+        for (uint32_t q0 = 0; q0 < deg; q0 += kWave) {
+          const uint32_t ev = q0 + lane < deg ? edge_at<false>(g, m.edge_start, m.eframe_base, (uint64_t)off + q0 + lane) : kNone;
+          const uint32_t qn = deg - q0 < (uint32_t)kWave ? deg - q0 : (uint32_t)kWave;
+          for (uint32_t q = 0; q < qn; q++) {
+            const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)ev, (int)q);
+            if (!fnd && tgt != kNoTarget && v == tgt) {
+              fnd = true;
+              x = q0 + q + 1;
+            }
+          }
+        }
+      }
+      const uint64_t mask = wave_ballot(fnd);
+      if (lane == 0) s_bmask[b] = mask;
+      exl += x;
+    }
+    s_bex[wave][lane] = exl;
+  }
+  uint64_t ex1_total;
+  (void)block_excl_scan(ex1, s_tmp, ex1_total);  // (its barriers also order the masks above)
+  uint64_t mk[kFBPer];
+#pragma unroll
+  for (int k = 0; k < kFBPer; k++) mk[k] = dg[k] >= 2 ? s_bmask[s_rowb[tid + k * kBlock]] : 0ull;
+  // every parent against every entry; a wave's compares are 64 rows' flags
+  for (uint32_t pi = 0; pi < npg; pi++) {
+    const uint32_t tgt = s_tgt[pi];
+    uint64_t bw[kFBPer];
+    uint64_t any = 0;
+#pragma unroll
+    for (int k = 0; k < kFBPer; k++) {
+      bw[k] = wave_ballot(c0[k] == tgt || ((mk[k] >> pi) & 1u));
+      any |= bw[k];
+    }
+    if (any) {  // wave-uniform, rare
+      uint32_t kept = 0;
+#pragma unroll
+      for (int k = 0; k < kFBPer; k++) {
+        kept += (uint32_t)__builtin_popcountll(bw[k]);
+        if (bw[k] && lane < 2)
+          f.bits[s_rank[pi] * (kFBRows / 32) + (k * kBlock + wave * kWave) / 32 + lane] = (uint32_t)(bw[k] >> (32 * lane));
+      }
+      if (lane == 0) s_kept[pi][wave] = kept;
+    }
+  }
+  __syncthreads();
+  if (tid < npg) {
+    uint64_t kt = 0, xt = ex1_total;
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+      kt += s_kept[tid][w];
+      xt += s_bex[w][tid];
+    }
+    const uint64_t rank = s_rank[tid];
+    p.tile_sum[rank] = kt;
+    p.tile_nz[rank] = kt;
+    p.tile_ex[rank] = xt;
+    u.row0[rank] = f.excl[s_par[tid]] + j0;
+  }
+}
+
 // Compaction of a fan-out filter's kept rows: one wave per tile reads the
 // tile's TR / 32 bit words and copies each kept row of T to its place in O
 // (the tile's prefix from the two-level scan, then bit order).
@@ -1556,6 +1736,41 @@ __global__ __launch_bounds__(kBlock) void k_compact_bits(StepParams p, const uin
   for (uint32_t x = w; x; x &= x - 1) {
     const uint64_t r = r_base + (uint64_t)lane * 32 + (uint32_t)__builtin_ctz(x);
     for (uint32_t col = 0; col < c; col++) p.O[o * c + col] = p.T[r * c + col];
+    o++;
+  }
+}
+
+// The same compaction for a fan-out that was never written: kept row r is
+// built from its lineage -- the parent row (the last compacted parent whose
+// first child row is <= r, a binary search) and that parent's list entry
+// r - its first child row, of the step's predicate (resident graph).
+template <int TR>
+__global__ __launch_bounds__(kBlock) void k_compact_lineage(GraphView g, StepParams p, const uint32_t *bits,
+                                                            const uint64_t *row0, FanoutFilter f,
+                                                            uint64_t edge_start) {
+  static_assert(TR / 32 <= kWave, "one word per lane");
+  const uint64_t lt = p.tile0 + blockIdx.x * (uint64_t)(kBlock / kWave) + threadIdx.x / kWave;
+  if (lt >= p.ntiles) return;
+  const uint64_t r_base = row0 ? row0[lt] : lt * TR;
+  const uint32_t lane = lane_id();
+  const uint32_t w = lane < TR / 32 ? bits[lt * (TR / 32) + lane] : 0u;
+  if (!wave_ballot(w != 0)) return;
+  const uint64_t pre = p.tile_nz[lt] + (p.chunk_nz ? p.chunk_nz[lt / kScanChunk] : 0);
+  const uint64_t inc = wave_incl_scan((uint64_t)__builtin_popcount(w));
+  uint64_t o = pre + inc - (uint64_t)__builtin_popcount(w);
+  const uint32_t c = p.c, pc = f.pc;
+  for (uint32_t x = w; x; x &= x - 1) {
+    const uint64_t r = r_base + (uint64_t)lane * 32 + (uint32_t)__builtin_ctz(x);
+    uint64_t lo = 0, hi = f.np - 1;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi + 1) >> 1;
+      if (f.excl[mid] <= r) lo = mid;
+      else hi = mid - 1;
+    }
+    const uint2 inf = f.info[lo];
+    const int32_t *prow = f.P + (uint64_t)inf.x * pc;
+    for (uint32_t col = 0; col < pc; col++) p.O[o * c + col] = prow[col];
+    p.O[o * c + pc] = (int32_t)g.edges[edge_start + inf.y + (r - f.excl[lo])];
     o++;
   }
 }

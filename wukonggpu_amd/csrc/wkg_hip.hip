@@ -861,6 +861,7 @@ int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir,
   // parent-aligned units of TRU rows when they average at least 256 rows
   // (long shared lists), else row tiles of TR rows
   constexpr int TR = 1024, TRU = 2048;
+  static_assert(TRU == kFBRows, "k_fanout_blocks fills the units' bit words");
   uint64_t nunits = 0;
   std::vector<uint64_t> ubase(np);
   for (uint64_t i = 0; i < np; i++) {
@@ -874,9 +875,10 @@ int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir,
   prof_add_bytes(units ? "fanout_units" : "fanout_row_tiles", 1.0);
   const int tr = units ? TRU : TR;
   const uint64_t ntiles = units ? nunits : (R + TR - 1) / TR, nchunks = (ntiles + kScanChunk - 1) / kScanChunk;
-  // uploads: pbase | lbase | ly (u32) [| ubase | xbase | poff | par (u32)]
+  // uploads: pbase | lbase | ly (u32) [| ubase | xbase | poff | par (u32) | ln | xb]
   const uint64_t w_ly = (nl + 1) / 2, w_par = (np + 1) / 2;
-  const uint64_t up_words = np + nl + w_ly + (units ? np + 2 * (nl + 1) + w_par : 0);
+  const uint64_t up_words = np + nl + w_ly + (units ? np + 2 * (nl + 1) + w_par + nl + (nl + 1) : 0);
+  uint64_t nwork = 0;  // units through k_fanout_blocks: (block of a list) x (group of its parents) items
   const uint64_t scan_words = 4 + 3 * ntiles + 3 * nchunks + (units ? ntiles : 0);  // (+ unit first rows)
   const uint64_t bit_words = (ntiles * (uint64_t)(tr / 32) + 1) / 2;
   uint64_t *buf = (uint64_t *)pool->alloc((up_words + scan_words + bit_words + total) * 8, s, err_slot());
@@ -906,12 +908,21 @@ int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir,
       }
       std::vector<uint64_t> fill(poff, poff + nl);
       for (uint64_t i = 0; i < np; i++) par[fill[plist[i]]++] = (uint32_t)i;
+      o += 2 * (nl + 1) + w_par;  // past xbase, poff and par
+      uint64_t *lnd = up.data() + o, *xb = lnd + nl;
+      xb[0] = 0;
+      for (uint64_t L = 0; L < nl; L++) {
+        lnd[L] = ln[L];
+        xb[L + 1] = xb[L] + ((ln[L] + kFBRows - 1) / kFBRows) * ((cnt[L + 1] + kFBGroup - 1) / kFBGroup);
+      }
+      nwork = xb[nl];
     }
   }
   uint64_t *d_pbase = buf, *d_lbase = buf + np;
   uint32_t *d_ly = (uint32_t *)(d_lbase + nl);
   uint64_t *d_ubase = buf + np + nl + w_ly, *d_xbase = d_ubase + np, *d_poff = d_xbase + nl + 1;
   uint32_t *d_par = (uint32_t *)(d_poff + nl + 1);
+  uint64_t *d_ln = (uint64_t *)d_par + w_par, *d_xb = d_ln + nl;
   uint64_t *scr = buf + up_words;
   uint32_t *bits = (uint32_t *)(scr + scan_words);
   uint2 *D = (uint2 *)(scr + scan_words + bit_words);
@@ -971,11 +982,25 @@ int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir,
   f.ec = mode == kModeK2K ? ec : -1;
   f.bits = bits;
   FanoutUnits u{d_xbase, d_poff, d_par, d_ubase, nl, row0};
+  FanoutBlocks fb{d_xb, d_poff, d_par, d_ubase, d_ln, nl, row0};
   {
     // per row: a D entry (8 B, mostly cached) and the list it names; the
     // input rows are not read
     ProfScope ps(this, s, "fanout_filter", 8.0 * (double)R);
     const uint64_t per = std::max<uint64_t>(thread_launch_rows() / (uint64_t)tr, 1);
+    if (units && fanout_blocks) {
+      // the bit words start zeroed (the kernel writes only words with a kept row)
+      k_zero_u64<<<(unsigned)std::min<uint64_t>((bit_words + kBlock - 1) / kBlock, 65535), kBlock, 0, s>>>(
+          (uint64_t *)bits, bit_words);
+      // work items of kFBRows rows x up to kFBGroup parents: per-launch rows as for units
+      const uint64_t wper = std::max<uint64_t>(per / kFBGroup, 1);
+      for (uint64_t w0 = 0; w0 < nwork; w0 += wper) {
+        p.tile0 = w0;
+        const unsigned grid = (unsigned)std::min<uint64_t>(wper, nwork - w0);
+        if (sorted) k_fanout_blocks<true><<<grid, kBlock, 0, s>>>(g, p, f, fb);
+        else k_fanout_blocks<false><<<grid, kBlock, 0, s>>>(g, p, f, fb);
+      }
+    } else
     for (uint64_t t0 = 0; t0 < ntiles; t0 += per) {
       p.tile0 = t0;
       const unsigned grid = (unsigned)std::min<uint64_t>(per, ntiles - t0);
@@ -1004,11 +1029,17 @@ int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir,
     p.O = t->d;
     ProfScope ps(this, s, "compact", 8.0 * (double)kept * c);
     const uint64_t per = std::max<uint64_t>(thread_launch_rows() / (uint64_t)tr, 4) & ~uint64_t(3);
+    const uint64_t estart = h_meta[in->lin_pid].edge_start;
     for (uint64_t t0 = 0; t0 < ntiles; t0 += per) {
       p.tile0 = t0;
       const unsigned grid = (unsigned)((std::min<uint64_t>(per, ntiles - t0) + kBlock / kWave - 1) / (kBlock / kWave));
-      if (units) k_compact_bits<TRU><<<grid, kBlock, 0, s>>>(p, bits, row0);
-      else k_compact_bits<TR><<<grid, kBlock, 0, s>>>(p, bits, nullptr);
+      if (in->d) {
+        if (units) k_compact_bits<TRU><<<grid, kBlock, 0, s>>>(p, bits, row0);
+        else k_compact_bits<TR><<<grid, kBlock, 0, s>>>(p, bits, nullptr);
+      } else {  // a fan-out that was never written: rows from the lineage
+        if (units) k_compact_lineage<TRU><<<grid, kBlock, 0, s>>>(g, p, bits, row0, f, estart);
+        else k_compact_lineage<TR><<<grid, kBlock, 0, s>>>(g, p, bits, nullptr, f, estart);
+      }
     }
     if ((le = hipGetLastError()) != hipSuccess)
       return done(fail(WKG_EDEVICE, "fan-out compaction launch failed: %s", hipGetErrorString(le)));
@@ -1017,6 +1048,84 @@ int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir,
   *out = t;
   t = nullptr;
   return done(WKG_OK);
+}
+
+// known_to_unknown on column sc of `in` followed by a filter (fmode) whose
+// start column is the new column: when the filter can run on the fan-out's
+// lineage (step_filter_fanout), the fan-out is never written -- a table
+// with lineage and no cells stands for it -- and *out is the filter's
+// output (*fused = 1); the rows and work counts are those of the two steps
+// run one after the other. Otherwise *out is the known_to_unknown output
+// (*fused = 0) and the caller runs the filter on it. The reference runs the
+// two handlers one after the other (core/rcache.hpp:232-414, then 334-466).
+int wkg_ctx::step_k2u_filter(const wkg_table *in, int sc, int pid, int dir, int fmode, int fpid, int fdir, int ec,
+                             int64_t cval, hipStream_t s, wkg_table **out, int *fused, uint64_t *ex_u,
+                             uint64_t *ex_f) {
+  *fused = 0;
+  *ex_f = 0;
+  const bool fits = lineage && !framed && in && in->rows > kSmallRows && in->cols >= 1 && sc >= 0 && sc < in->cols &&
+                    !(in->lin_parent && sc < in->lin_parent->cols) && in->cols + 1 <= kBlock &&
+                    in->rows < (1ull << 32) && (in->owned || in->slice_of);
+  if (!fits || check_pattern(fpid, fdir) != WKG_OK || (fmode == kModeK2K && (ec < 0 || ec >= in->cols)))
+    return step_known(kModeK2U, in, sc, pid, dir, -1, -1, s, out, ex_u);
+  int rc = check_pattern(pid, dir);
+  if (rc) return rc;
+  if ((rc = ensure_pred(pid, s))) return rc;
+  const int c = in->cols;
+  ProbePass pp;
+  if ((rc = probe_pass(kModeK2U, in, sc, pid, dir, -1, -1, nullptr, s, pp))) return rc;
+  const uint64_t E = pp.cnt[0], nz = pp.cnt[2];
+  *ex_u = E;
+  if (E >= group_filter_rows && E < (1ull << 32) && nz > 0 && E >= 4 * nz && in->bytes + in->rows * 16 <= pool->budget / 16) {
+    // the fan-out as lineage only
+    wkg_table *v = new (std::nothrow) wkg_table;
+    if (!v) {
+      release_pass(pp, s);
+      return fail(WKG_ENOMEM, "out of host memory");
+    }
+    v->d = nullptr;  // (no cells: not owned)
+    v->owned = false;
+    v->rows = E;
+    v->cols = c + 1;
+    v->bytes = 0;
+    v->stream = s;
+    wkg_table *par = const_cast<wkg_table *>(in);
+    par->refs++;
+    v->lin_parent = par;
+    v->lin_info = pp.cinfo;
+    v->lin_excl = pp.cexcl;
+    v->lin_n = nz;
+    v->lin_pid = pid;
+    v->lin_alloc = pp.lin;
+    pp.lin = nullptr;
+    release_pass(pp, s);
+    prof_add_bytes("fused_fanouts", 1.0);  // a count
+    rc = step_filter_fanout(fmode, v, fpid, fdir, ec, cval, s, out, ex_f);
+    if (rc != kNotTaken) {
+      drop_table(v);
+      if (rc == WKG_OK) *fused = 1;
+      return rc;
+    }
+    // the filter declined: write the fan-out after all
+    drop_table(v);
+    prof_add_bytes("fused_fanouts", -1.0);
+    return step_known(kModeK2U, in, sc, pid, dir, -1, -1, s, out, ex_u);
+  }
+  const int out_cols = c + 1;
+  if (table_limit && in->rows >= 2 && E * (uint64_t)out_cols * 4 > table_limit) {
+    release_pass(pp, s);
+    return fail(WKG_ENOMEM, "a %llu-row output is above the table limit", (unsigned long long)E);
+  }
+  wkg_table *t = nullptr;
+  if ((rc = new_table(E, out_cols, s, &t)) || (rc = expand_pass(in, pid, pp.cinfo, pp.cexcl, nz, E, s, t))) {
+    release_pass(pp, s);
+    drop_table(t);
+    return rc;
+  }
+  keep_lineage(in, pp, nz, t, pid);
+  release_pass(pp, s);
+  *out = t;
+  return WKG_OK;
 }
 
 int wkg_ctx::step_known(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval,
@@ -2010,6 +2119,24 @@ int wkg_set_small_query_server_times(wkg_ctx *ctx, double idle_s, double life_s)
   ctx->sq_idle_s = idle_s;
   ctx->sq_life_s = life_s;
   return WKG_OK;
+}
+
+int wkg_step_k2u_filter(wkg_ctx *ctx, const wkg_table *in, int start_col, int pid, int dir, int filter_kind,
+                        int filter_pid, int filter_dir, int end_col, int64_t const_end, void *stream, wkg_table **out,
+                        int *fused, uint64_t *expanded_k2u, uint64_t *expanded_filter) {
+  if (!ctx || !out || !fused) return WKG_EINVAL;
+  *out = nullptr;
+  *fused = 0;
+  uint64_t eu = 0, ef = 0;
+  if (filter_kind != WKG_STEP_K2K && filter_kind != WKG_STEP_K2C)
+    return ctx->fail(WKG_EINVAL, "the second step must be known_to_known or known_to_const (kind %d)", filter_kind);
+  if (!in) return ctx->fail(WKG_EINVAL, "null input table");
+  const int fmode = filter_kind == WKG_STEP_K2K ? kModeK2K : kModeK2C;
+  const int rc = ctx->step_k2u_filter(in, start_col, pid, dir, fmode, filter_pid, filter_dir, end_col, const_end,
+                                      ctx->pick(stream), out, fused, &eu, &ef);
+  if (expanded_k2u) *expanded_k2u = eu;
+  if (expanded_filter) *expanded_filter = ef;
+  return rc;
 }
 
 int wkg_step_k2k(wkg_ctx *ctx, const wkg_table *in, int start_col, int pid, int dir, int end_col,

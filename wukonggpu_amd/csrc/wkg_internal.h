@@ -142,6 +142,7 @@ struct wkg_ctx {
   int probe_tile = 0;        // rows per probe tile of the per-step calls (0: by input size)
   bool lineage = true;       // probe inherited columns once per parent row
   uint64_t group_filter_rows = 1ull << 22;  // smallest input a filter decides per lineage group
+  bool fanout_blocks = true;  // long shared lists: k_fanout_blocks (else k_fanout_units)
   bool small_steps = true;   // steps over at most kSmallRows rows in one launch
   uint64_t *noalloc_scratch = nullptr;  // this device's no-alloc step scratch (wkg_step_k2u_noalloc)
   SQServer *sqs = nullptr;   // the persistent small-query server (created on first use)
@@ -209,6 +210,8 @@ struct wkg_ctx {
   int step_single_list(uint64_t key, int pid, hipStream_t s, wkg_table **out, uint64_t *expanded);
   int step_known(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval, hipStream_t s,
                  wkg_table **out, uint64_t *expanded);
+  int step_k2u_filter(const wkg_table *in, int sc, int pid, int dir, int fmode, int fpid, int fdir, int ec,
+                      int64_t cval, hipStream_t s, wkg_table **out, int *fused, uint64_t *ex_u, uint64_t *ex_f);
   int probe_pass(int mode, const wkg_table *in, int sc, int pid, int dir, int ec, int64_t cval, int32_t *O,
                  hipStream_t s, ProbePass &pp, int wcol = -1, uint64_t *ext_scr = nullptr, void *ext_lin = nullptr,
                  int tile_rows = 0);

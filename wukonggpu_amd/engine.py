@@ -303,6 +303,15 @@ class Device:
             return "pending", None, 0
         return self._sq_result(rc, buf, rows, cols, ex)
 
+    def k2u_filter(self, t, start_col, pid, d, filter_kind, filter_pid, filter_dir, end_col=-1, const=-1):
+        """known_to_unknown then a filter on its new column (wkg_step_k2u_filter):
+        (Table, fused, expanded of the k2u, expanded of the filter)."""
+        out, fused, e1, e2 = C.c_void_p(), C.c_int(), C.c_uint64(), C.c_uint64()
+        self._check(N.hip_lib().wkg_step_k2u_filter(self._h, t._h, start_col, pid, d, filter_kind, filter_pid,
+                                                     filter_dir, end_col, const, None, C.byref(out),
+                                                     C.byref(fused), C.byref(e1), C.byref(e2)))
+        return Table(self, out.value), bool(fused.value), int(e1.value), int(e2.value)
+
     def k2k(self, t, start_col, pid, d, end_col):
         return self._step(N.hip_lib().wkg_step_k2k, t._h, start_col, pid, d, end_col)
 
@@ -481,6 +490,12 @@ class GPUEngine:
         rc = L.wke_set_heavy_mode(self._h, int(mode))
         if rc != 0:
             raise DeviceError(rc, "wke_set_heavy_mode failed")
+
+    def set_fused_filters(self, on):
+        """known_to_unknown + a filter on its new column in one call (default on)."""
+        rc = N.engine_lib().wke_set_fused_filters(self._h, 1 if on else 0)
+        if rc != 0:
+            raise DeviceError(rc, "wke_set_fused_filters failed")
 
     def set_serve_small(self, workgroups):
         """Small-query server workgroups for served light queries (0: light lanes only)."""
