@@ -156,6 +156,12 @@ int wkg_set_lineage(wkg_ctx *ctx, int on);
  * holding it; its output and work count are the per-row filter's. No
  * counterpart in the reference (core/gpu_hash.cu:320-466 tests every row). */
 int wkg_set_group_filter_rows(wkg_ctx *ctx, uint64_t rows);
+/* Such a filter over long lists shared by many parents runs a block of a
+ * list's entries against 64 of its parents per workgroup and lists the kept
+ * rows (at most kept_cap of them, 0: 2^22; with more it reruns with one
+ * parent per workgroup). On by default; 0 always runs one parent per
+ * workgroup. The output and work count are the same either way. */
+int wkg_set_fanout_blocks(wkg_ctx *ctx, int on, uint64_t kept_cap);
 /* A known-start step over at most 512 input rows runs its probe, scan,
  * compaction and (up to 2048 output rows) expand in one launch instead of
  * five; its output is the same, row for row. On by default; 0 turns it off.

@@ -403,18 +403,21 @@ def test_fanout_filters_match_oracle(graph_hubs):
            [hub, 2, 1, -1, -1, 2, 0, -2, -2, 3, 1, -3, -3, 2, 0, -2]]
     want = [oracle.run_query(gs, q)[:2] for q in qs]
     try:
-        for rows, lineage, launch, limit, fuse in [(0, True, 0, 0, True), (0, True, 4096, 0, True),
-                                                   (0, True, 0, 4 << 20, True), (1 << 22, True, 0, 0, True),
-                                                   (0, False, 0, 0, True), (0, True, 0, 0, False),
-                                                   (0, True, 0, 4 << 20, False)]:
+        for rows, lineage, launch, limit, fuse, blocks in [
+                (0, True, 0, 0, True, 1), (0, True, 4096, 0, True, 1), (0, True, 0, 4 << 20, True, 1),
+                (1 << 22, True, 0, 0, True, 1), (0, False, 0, 0, True, 1), (0, True, 0, 0, False, 1),
+                (0, True, 0, 4 << 20, False, 1), (0, True, 0, 0, True, 0), (0, True, 4096, 0, True, 2)]:
             dev.set_lineage(lineage)
             dev.set_group_filter_rows(rows)
             dev.set_launch_rows(launch)
             dev.set_table_limit(limit)
             eng.set_fused_filters(fuse)
+            # blocks: 1 the block filter, 0 one parent per workgroup, 2 the
+            # block filter with a 3-row kept list (it overflows and reruns)
+            dev.set_fanout_blocks(blocks != 0, 3 if blocks == 2 else 0)
             dev.profile(True)
             print("fan-out filters: group filter rows", rows, "lineage", lineage, "launch rows", launch,
-                  "table limit", limit, "fused with the fan-out", fuse, flush=True)
+                  "table limit", limit, "fused with the fan-out", fuse, "blocks", blocks, flush=True)
             for q, (wt, wex) in zip(qs, want):
                 res = eng.run_query(q)
                 assert res.status == 0, q
@@ -424,7 +427,9 @@ def test_fanout_filters_match_oracle(graph_hubs):
             taken = dev.profile_read("fanout_filters")[2]
             by_units, by_rows = dev.profile_read("fanout_units")[2], dev.profile_read("fanout_row_tiles")[2]
             fused = dev.profile_read("fused_fanouts")[2]
+            overflow = dev.profile_read("fanout_kept_overflow")[2]
             dev.profile(False)
+            assert (overflow > 0) if blocks == 2 else (overflow == 0), overflow
             if rows == 0 and lineage:
                 assert taken > 0
                 if limit == 0:
@@ -438,6 +443,7 @@ def test_fanout_filters_match_oracle(graph_hubs):
         dev.set_lineage(True)
         dev.set_table_limit(0)
         eng.set_fused_filters(True)
+        dev.set_fanout_blocks(True)
     eng.close()
     dev.close()
 

@@ -161,6 +161,7 @@ def hip_lib():
              i64, vp, P(vp), P(C.c_int), P(u64), P(u64))
         _sig(L, "wkg_query_small_collect", C.c_int, vp, u64, C.c_int, P(i32), u64, P(u64), P(C.c_int), P(u64))
         _sig(L, "wkg_set_small_query_server", C.c_int, vp, C.c_int)
+        _sig(L, "wkg_set_fanout_blocks", C.c_int, vp, C.c_int, u64)
         _sig(L, "wkg_set_small_query_server_times", C.c_int, vp, C.c_double, C.c_double)
         _sig(L, "wkg_step_batch", C.c_int, vp, C.c_int, P(StepDesc), vp, P(vp), P(u64))
         _sig(L, "wkg_profile_enable", C.c_int, vp, C.c_int)

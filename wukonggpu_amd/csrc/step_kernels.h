@@ -1549,13 +1549,12 @@ __global__ __launch_bounds__(kBlock) void k_fanout_units(GraphView g, StepParams
 //    broadcast lane by lane, or a binary search per lane for a long sorted
 //    list), leaving a 64-bit parent mask per entry;
 //  - per parent, each wave's compares are ballots: 64 rows' kept flags at
-//    once. Kept rows are rare: the bit words start zeroed and only words
-//    with a kept row are written.
-// The output is the units' layout of k_fanout_units (unit = parent x block
-// of kFBRows rows, ranked ubase[parent] + block: bit words, kept rows,
-// edges examined, first row), so the tile scan and the compaction are
-// unchanged. Work counts follow the reference's scan: first match + 1, or
-// the degree.
+//    once (kept rows are rare: a ballot with a kept row is kept in LDS).
+// The output is per unit (parent x block of kFBRows rows, ranked
+// ubase[parent] + block) its kept rows and edges examined, for the tile
+// scan, and a list of the kept rows with their unit and index in it, for
+// k_compact_kept. Work counts follow the reference's scan: first match + 1,
+// or the degree.
 constexpr int kFBRows = 2048;
 constexpr int kFBPer = kFBRows / kBlock;
 constexpr int kFBGroup = 64;
@@ -1567,6 +1566,11 @@ struct FanoutBlocks {
   const uint64_t *ln;     // per distinct list: its entries
   uint64_t nl;
   uint64_t *row0;         // per unit rank: its first row
+  // kept rows, unordered: (row, unit rank, index within the unit, 0); a row
+  // past the capacity is counted but not stored (the caller then reruns)
+  uint4 *kept;
+  uint64_t *nkept;
+  uint64_t cap;
 };
 
 template <bool SORTED>
@@ -1575,11 +1579,11 @@ __global__ __launch_bounds__(kBlock) void k_fanout_blocks(GraphView g, StepParam
   constexpr uint32_t kNone = 0xffffffffu, kNoTarget = 0xfffffffeu;  // (ids and valid targets are below 2^31)
   static_assert(kFBGroup == kWave, "lane pi tests parent pi");
   __shared__ uint32_t s_boff[kFBRows], s_bdeg[kFBRows];
-  __shared__ uint16_t s_rowb[kFBRows];               // a multi-entry row's index among them
-  __shared__ uint64_t s_bmask[kFBRows];              // per multi-entry row: the parents in its list
+  __shared__ uint64_t s_bmask[kFBRows];            // per longer list: the parents in it
+  __shared__ uint16_t s_pc[kFBGroup][kFBPer][W];   // per parent: kept rows per (k, wave) ballot
+  __shared__ uint32_t s_kcnt[kFBGroup];            // per parent: kept rows in the block
   __shared__ uint32_t s_tgt[kFBGroup], s_par[kFBGroup];
   __shared__ uint64_t s_rank[kFBGroup];
-  __shared__ uint32_t s_kept[kFBGroup][W];
   __shared__ uint64_t s_bex[W][kWave], s_tmp[W];
   __shared__ uint32_t s_nb;
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
@@ -1597,6 +1601,7 @@ __global__ __launch_bounds__(kBlock) void k_fanout_blocks(GraphView g, StepParam
   const uint32_t npg = (uint32_t)(cnt - grp * kFBGroup < (uint64_t)kFBGroup ? cnt - grp * kFBGroup : kFBGroup);
   const DevMeta m = g.meta[p.pid < g.num_metas ? p.pid : 0];
   if (tid == 0) s_nb = 0;
+  for (uint32_t z = tid; z < kFBGroup * kFBPer * W / 2; z += kBlock) ((uint32_t *)&s_pc[0][0][0])[z] = 0;
   if (tid < kFBGroup) {
     uint32_t tg = kNoTarget, i = 0;
     if (tid < npg) {
@@ -1611,23 +1616,23 @@ __global__ __launch_bounds__(kBlock) void k_fanout_blocks(GraphView g, StepParam
     }
     s_par[tid] = i;
     s_tgt[tid] = tg;
+    s_kcnt[tid] = 0;
   }
-  s_kept[tid / W][tid % W] = 0;  // (kFBGroup x W = kBlock entries)
   __syncthreads();
   // the block's D entries (the list's D entries start at any of its parents' pbase)
   const uint2 *Dp = f.D + f.pbase[s_par[0]] + j0;
-  uint32_t c0[kFBPer], dg[kFBPer];
+  uint32_t c0[kFBPer], bi[kFBPer];  // single entry (or kNone); a longer list's index (or kNone)
   uint64_t ex1 = 0;  // single-entry rows: one edge examined each, for every parent
 #pragma unroll
   for (int k = 0; k < kFBPer; k++) {
     const uint32_t row = tid + k * kBlock;
     const uint2 d = row < nrows ? Dp[row] : make_uint2(0u, 0u);
-    dg[k] = d.y;
     c0[k] = d.y == 1 ? d.x : kNone;
     ex1 += d.y == 1 ? 1u : 0u;
+    bi[k] = kNone;
     if (d.y >= 2) {
       const uint32_t b = atomicAdd(&s_nb, 1u);
-      s_rowb[row] = (uint16_t)b;
+      bi[k] = b;
       s_boff[b] = d.x;
       s_bdeg[b] = d.y;
     }
@@ -1635,15 +1640,15 @@ __global__ __launch_bounds__(kBlock) void k_fanout_blocks(GraphView g, StepParam
   __syncthreads();
   const uint32_t nb = s_nb;
   {
-    // each multi-entry list by one wave, lane pi testing parent pi
+    // each longer list by one wave, lane pi testing parent pi
     const uint32_t tgt = s_tgt[lane];
     uint64_t exl = 0;
     for (uint32_t b = wave; b < nb; b += W) {
       const uint32_t off = s_boff[b], deg = s_bdeg[b];
       bool fnd = false;
       uint32_t x = deg;
-      if (tgt != kNoTarget) {
-        if (SORTED && deg > kWave) {
+      if (SORTED && deg > kWave) {
+        if (tgt != kNoTarget) {
           uint32_t lo = 0, hi2 = deg;
           while (lo < hi2) {
             const uint32_t mid = (lo + hi2) >> 1;
@@ -1655,8 +1660,7 @@ __global__ __launch_bounds__(kBlock) void k_fanout_blocks(GraphView g, StepParam
             x = lo + 1;
           }
         }
-      }
-      if (!SORTED || deg <= kWave) {  // wave-uniform: 64 entries at a time, broadcast lane by lane
+      } else {  // wave-uniform: 64 entries at a time, broadcast lane by lane
         for (uint32_t q0 = 0; q0 < deg; q0 += kWave) {
           const uint32_t ev = q0 + lane < deg ? edge_at<false>(g, m.edge_start, m.eframe_base, (uint64_t)off + q0 + lane) : kNone;
           const uint32_t qn = deg - q0 < (uint32_t)kWave ? deg - q0 : (uint32_t)kWave;
@@ -1678,43 +1682,91 @@ __global__ __launch_bounds__(kBlock) void k_fanout_blocks(GraphView g, StepParam
   uint64_t ex1_total;
   (void)block_excl_scan(ex1, s_tmp, ex1_total);  // (its barriers also order the masks above)
   uint64_t mk[kFBPer];
+  uint32_t kmulti = 0;  // wave-uniform: bit k set when a row k of the wave has a longer list
 #pragma unroll
-  for (int k = 0; k < kFBPer; k++) mk[k] = dg[k] >= 2 ? s_bmask[s_rowb[tid + k * kBlock]] : 0ull;
-  // every parent against every entry; a wave's compares are 64 rows' flags
+  for (int k = 0; k < kFBPer; k++) {
+    mk[k] = bi[k] != kNone ? s_bmask[bi[k]] : 0ull;
+    kmulti |= wave_ballot(bi[k] != kNone) ? 1u << k : 0u;
+  }
+  const uint32_t tgt_l = s_tgt[lane];  // lane pi: parent pi's target
+  auto ballot_k = [&](uint32_t pi, uint32_t tgt, int k) -> uint64_t {
+    if (kmulti >> k & 1u) return wave_ballot(c0[k] == tgt || ((mk[k] >> pi) & 1u));
+    return wave_ballot(c0[k] == tgt);
+  };
+  // pass 1: every parent against every entry; a wave's compares are 64
+  // rows' flags; only ballots with a kept row (rare) are counted
   for (uint32_t pi = 0; pi < npg; pi++) {
-    const uint32_t tgt = s_tgt[pi];
-    uint64_t bw[kFBPer];
-    uint64_t any = 0;
+    const uint32_t tgt = (uint32_t)__builtin_amdgcn_readlane((int)tgt_l, (int)pi);
 #pragma unroll
     for (int k = 0; k < kFBPer; k++) {
-      bw[k] = wave_ballot(c0[k] == tgt || ((mk[k] >> pi) & 1u));
-      any |= bw[k];
-    }
-    if (any) {  // wave-uniform, rare
-      uint32_t kept = 0;
-#pragma unroll
-      for (int k = 0; k < kFBPer; k++) {
-        kept += (uint32_t)__builtin_popcountll(bw[k]);
-        if (bw[k] && lane < 2)
-          f.bits[s_rank[pi] * (kFBRows / 32) + (k * kBlock + wave * kWave) / 32 + lane] = (uint32_t)(bw[k] >> (32 * lane));
+      const uint64_t bw = ballot_k(pi, tgt, k);
+      if (bw && lane == 0) {
+        const uint32_t c = (uint32_t)__builtin_popcountll(bw);
+        s_pc[pi][k][wave] = (uint16_t)c;
+        atomicAdd(&s_kcnt[pi], c);
       }
-      if (lane == 0) s_kept[pi][wave] = kept;
     }
   }
   __syncthreads();
-  if (tid < npg) {
-    uint64_t kt = 0, xt = ex1_total;
-#pragma unroll
-    for (int w = 0; w < W; w++) {
-      kt += s_kept[tid][w];
-      xt += s_bex[w][tid];
+  // pass 2, parents with kept rows only: the ballots again, each kept row
+  // listed with its index among its parent's kept rows (row order: k, then
+  // wave, then lane)
+  for (uint32_t pi = 0; pi < npg; pi++) {
+    if (s_kcnt[pi] == 0) continue;  // block-uniform
+    const uint32_t tgt = (uint32_t)__builtin_amdgcn_readlane((int)tgt_l, (int)pi);
+    const uint64_t rank = s_rank[pi], first = f.excl[s_par[pi]] + j0;
+    uint32_t before = 0;  // kept rows of parent pi at smaller (k, wave)
+    for (int k = 0; k < kFBPer; k++) {
+      for (int w = 0; w < W; w++)
+        if (w < (int)wave) before += s_pc[pi][k][w];
+      const uint64_t bw = ballot_k(pi, tgt, k);
+      if (bw) {
+        uint64_t base = 0;
+        if (lane == 0) base = atomicAdd((unsigned long long *)u.nkept, (unsigned long long)__builtin_popcountll(bw));
+        base = shfl64(base, 0);
+        if ((bw >> lane) & 1u) {
+          const uint32_t lr = (uint32_t)__builtin_popcountll(bw & lanes_below());
+          const uint64_t slot = base + lr;
+          if (slot < u.cap)
+            u.kept[slot] = make_uint4((uint32_t)(first + (uint32_t)k * kBlock + wave * kWave + lane), (uint32_t)rank,
+                                      before + lr, 0u);
+        }
+      }
+      for (int w = (int)wave; w < W; w++) before += s_pc[pi][k][w];
     }
-    const uint64_t rank = s_rank[tid];
-    p.tile_sum[rank] = kt;
-    p.tile_nz[rank] = kt;
-    p.tile_ex[rank] = xt;
-    u.row0[rank] = f.excl[s_par[tid]] + j0;
   }
+  if (tid < npg) {
+    uint64_t xt = ex1_total;
+#pragma unroll
+    for (int w = 0; w < W; w++) xt += s_bex[w][tid];
+    const uint64_t rank = s_rank[tid];
+    p.tile_sum[rank] = s_kcnt[tid];
+    p.tile_nz[rank] = s_kcnt[tid];
+    p.tile_ex[rank] = xt;
+  }
+}
+
+// Compaction of the kept rows k_fanout_blocks listed: one thread per kept
+// row, its place = its unit's prefix from the two-level scan + its index in
+// the unit; the row is built from its lineage as in k_compact_lineage.
+__global__ __launch_bounds__(kBlock) void k_compact_kept(GraphView g, StepParams p, const uint4 *kept, uint64_t n,
+                                                         FanoutFilter f, uint64_t edge_start) {
+  const uint64_t k = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
+  if (k >= n) return;
+  const uint4 e = kept[k];
+  const uint64_t r = e.x, rank = e.y;
+  const uint64_t o = p.tile_nz[rank] + (p.chunk_nz ? p.chunk_nz[rank / kScanChunk] : 0) + e.z;
+  uint64_t lo = 0, hi = f.np - 1;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi + 1) >> 1;
+    if (f.excl[mid] <= r) lo = mid;
+    else hi = mid - 1;
+  }
+  const uint32_t c = p.c, pc = f.pc;
+  const uint2 inf = f.info[lo];
+  const int32_t *prow = f.P + (uint64_t)inf.x * pc;
+  for (uint32_t col = 0; col < pc; col++) p.O[o * c + col] = prow[col];
+  p.O[o * c + pc] = (int32_t)g.edges[edge_start + inf.y + (r - f.excl[lo])];
 }
 
 // Compaction of a fan-out filter's kept rows: one wave per tile reads the

@@ -982,67 +982,97 @@ int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir,
   f.ec = mode == kModeK2K ? ec : -1;
   f.bits = bits;
   FanoutUnits u{d_xbase, d_poff, d_par, d_ubase, nl, row0};
-  FanoutBlocks fb{d_xb, d_poff, d_par, d_ubase, d_ln, nl, row0};
-  {
-    // per row: a D entry (8 B, mostly cached) and the list it names; the
-    // input rows are not read
-    ProfScope ps(this, s, "fanout_filter", 8.0 * (double)R);
-    const uint64_t per = std::max<uint64_t>(thread_launch_rows() / (uint64_t)tr, 1);
-    if (units && fanout_blocks) {
-      // the bit words start zeroed (the kernel writes only words with a kept row)
-      k_zero_u64<<<(unsigned)std::min<uint64_t>((bit_words + kBlock - 1) / kBlock, 65535), kBlock, 0, s>>>(
-          (uint64_t *)bits, bit_words);
-      // work items of kFBRows rows x up to kFBGroup parents: per-launch rows as for units
-      const uint64_t wper = std::max<uint64_t>(per / kFBGroup, 1);
-      for (uint64_t w0 = 0; w0 < nwork; w0 += wper) {
-        p.tile0 = w0;
-        const unsigned grid = (unsigned)std::min<uint64_t>(wper, nwork - w0);
-        if (sorted) k_fanout_blocks<true><<<grid, kBlock, 0, s>>>(g, p, f, fb);
-        else k_fanout_blocks<false><<<grid, kBlock, 0, s>>>(g, p, f, fb);
-      }
-    } else
-    for (uint64_t t0 = 0; t0 < ntiles; t0 += per) {
-      p.tile0 = t0;
-      const unsigned grid = (unsigned)std::min<uint64_t>(per, ntiles - t0);
-      if (units) {
-        if (sorted) k_fanout_units<TRU, true><<<grid, kBlock, 0, s>>>(g, p, f, u);
-        else k_fanout_units<TRU, false><<<grid, kBlock, 0, s>>>(g, p, f, u);
+  // long shared lists: k_fanout_blocks, which lists its kept rows (up to
+  // kept_cap; more kept rows than that and the filter reruns on k_fanout_units)
+  bool blocks = units && fanout_blocks;
+  const uint64_t kept_cap = std::min<uint64_t>(R, fanout_kept_cap);
+  uint4 *kept_rows = nullptr;
+  if (blocks) {
+    kept_rows = (uint4 *)pool->alloc(kept_cap * 16 + 256, s, err_slot());
+    if (!kept_rows) return done(WKG_ENOMEM);
+  }
+  auto done2 = [&](int r) {
+    if (kept_rows) pool->release(kept_rows, s);
+    return done(r);
+  };
+  uint64_t *nkept = (uint64_t *)(kept_rows + kept_cap);
+  FanoutBlocks fb{d_xb, d_poff, d_par, d_ubase, d_ln, nl, row0, kept_rows, nkept, kept_cap};
+  hipError_t le;
+  uint64_t kept = 0, examined = 0;
+  for (;;) {
+    {
+      // per row: a D entry (8 B, mostly cached) and the list it names; the
+      // input rows are not read
+      ProfScope ps(this, s, "fanout_filter", 8.0 * (double)R);
+      const uint64_t per = std::max<uint64_t>(thread_launch_rows() / (uint64_t)tr, 1);
+      if (blocks) {
+        k_zero_u64<<<1, kBlock, 0, s>>>(nkept, 1);
+        // work items of kFBRows rows x up to kFBGroup parents: per-launch rows as
+        // for units, but at least 8 workgroups per CU (a launch fills the GPU)
+        const uint64_t wper = std::max<uint64_t>(per / kFBGroup, 8 * (uint64_t)ncu);
+        for (uint64_t w0 = 0; w0 < nwork; w0 += wper) {
+          p.tile0 = w0;
+          const unsigned grid = (unsigned)std::min<uint64_t>(wper, nwork - w0);
+          if (sorted) k_fanout_blocks<true><<<grid, kBlock, 0, s>>>(g, p, f, fb);
+          else k_fanout_blocks<false><<<grid, kBlock, 0, s>>>(g, p, f, fb);
+        }
       } else {
-        if (sorted) k_fanout_filter<TR, true><<<grid, kBlock, 0, s>>>(g, p, f);
-        else k_fanout_filter<TR, false><<<grid, kBlock, 0, s>>>(g, p, f);
+        for (uint64_t t0 = 0; t0 < ntiles; t0 += per) {
+          p.tile0 = t0;
+          const unsigned grid = (unsigned)std::min<uint64_t>(per, ntiles - t0);
+          if (units) {
+            if (sorted) k_fanout_units<TRU, true><<<grid, kBlock, 0, s>>>(g, p, f, u);
+            else k_fanout_units<TRU, false><<<grid, kBlock, 0, s>>>(g, p, f, u);
+          } else {
+            if (sorted) k_fanout_filter<TR, true><<<grid, kBlock, 0, s>>>(g, p, f);
+            else k_fanout_filter<TR, false><<<grid, kBlock, 0, s>>>(g, p, f);
+          }
+        }
       }
+      p.tile0 = 0;
     }
-    p.tile0 = 0;
+    p.sg = next_signal();
+    {
+      ProfScope ps(this, s, "scan_tiles", 48.0 * (double)ntiles);
+      k_scan_chunks<<<(unsigned)nchunks, kBlock, 0, s>>>(p);
+      if (nchunks > 1) k_scan_top<<<1, kBlock, 0, s>>>(p, nchunks);
+    }
+    le = hipGetLastError();
+    if (le != hipSuccess) return done2(fail(WKG_EDEVICE, "fan-out filter launch failed: %s", hipGetErrorString(le)));
+    if ((rc = wait_signal(p.sg, s))) return done2(rc);
+    kept = h_rb[0];
+    examined = h_rb[1];
+    if (!blocks || kept <= kept_cap) break;
+    blocks = false;  // more kept rows than the list holds: the bit-word path
+    prof_add_bytes("fanout_kept_overflow", 1.0);  // a count
   }
-  p.sg = next_signal();
-  {
-    ProfScope ps(this, s, "scan_tiles", 48.0 * (double)ntiles);
-    k_scan_chunks<<<(unsigned)nchunks, kBlock, 0, s>>>(p);
-    if (nchunks > 1) k_scan_top<<<1, kBlock, 0, s>>>(p, nchunks);
-  }
-  hipError_t le = hipGetLastError();
-  if (le != hipSuccess) return done(fail(WKG_EDEVICE, "fan-out filter launch failed: %s", hipGetErrorString(le)));
-  if ((rc = wait_signal(p.sg, s))) return done(rc);
-  const uint64_t kept = h_rb[0], examined = h_rb[1];
-  if ((rc = new_table(kept, c, s, &t))) return done(rc);
+  if ((rc = new_table(kept, c, s, &t))) return done2(rc);
   if (kept) {
     p.O = t->d;
     ProfScope ps(this, s, "compact", 8.0 * (double)kept * c);
-    const uint64_t per = std::max<uint64_t>(thread_launch_rows() / (uint64_t)tr, 4) & ~uint64_t(3);
     const uint64_t estart = h_meta[in->lin_pid].edge_start;
-    for (uint64_t t0 = 0; t0 < ntiles; t0 += per) {
-      p.tile0 = t0;
-      const unsigned grid = (unsigned)((std::min<uint64_t>(per, ntiles - t0) + kBlock / kWave - 1) / (kBlock / kWave));
-      if (in->d) {
-        if (units) k_compact_bits<TRU><<<grid, kBlock, 0, s>>>(p, bits, row0);
-        else k_compact_bits<TR><<<grid, kBlock, 0, s>>>(p, bits, nullptr);
-      } else {  // a fan-out that was never written: rows from the lineage
-        if (units) k_compact_lineage<TRU><<<grid, kBlock, 0, s>>>(g, p, bits, row0, f, estart);
-        else k_compact_lineage<TR><<<grid, kBlock, 0, s>>>(g, p, bits, nullptr, f, estart);
+    if (blocks) {
+      k_compact_kept<<<(unsigned)((kept + kBlock - 1) / kBlock), kBlock, 0, s>>>(g, p, kept_rows, kept, f, estart);
+    } else {
+      const uint64_t per = std::max<uint64_t>(thread_launch_rows() / (uint64_t)tr, 4) & ~uint64_t(3);
+      for (uint64_t t0 = 0; t0 < ntiles; t0 += per) {
+        p.tile0 = t0;
+        const unsigned grid = (unsigned)((std::min<uint64_t>(per, ntiles - t0) + kBlock / kWave - 1) / (kBlock / kWave));
+        if (in->d) {
+          if (units) k_compact_bits<TRU><<<grid, kBlock, 0, s>>>(p, bits, row0);
+          else k_compact_bits<TR><<<grid, kBlock, 0, s>>>(p, bits, nullptr);
+        } else {  // a fan-out that was never written: rows from the lineage
+          if (units) k_compact_lineage<TRU><<<grid, kBlock, 0, s>>>(g, p, bits, row0, f, estart);
+          else k_compact_lineage<TR><<<grid, kBlock, 0, s>>>(g, p, bits, nullptr, f, estart);
+        }
       }
     }
     if ((le = hipGetLastError()) != hipSuccess)
-      return done(fail(WKG_EDEVICE, "fan-out compaction launch failed: %s", hipGetErrorString(le)));
+      return done2(fail(WKG_EDEVICE, "fan-out compaction launch failed: %s", hipGetErrorString(le)));
+  }
+  if (kept_rows) {
+    pool->release(kept_rows, s);
+    kept_rows = nullptr;
   }
   if (expanded) *expanded = examined;
   *out = t;
@@ -1649,6 +1679,13 @@ int wkg_set_lineage(wkg_ctx *ctx, int on) {
 int wkg_set_group_filter_rows(wkg_ctx *ctx, uint64_t rows) {
   if (!ctx) return WKG_EINVAL;
   ctx->group_filter_rows = rows;
+  return WKG_OK;
+}
+
+int wkg_set_fanout_blocks(wkg_ctx *ctx, int on, uint64_t kept_cap) {
+  if (!ctx) return WKG_EINVAL;
+  ctx->fanout_blocks = on != 0;
+  ctx->fanout_kept_cap = kept_cap ? kept_cap : (1ull << 22);
   return WKG_OK;
 }
 

@@ -143,6 +143,7 @@ struct wkg_ctx {
   bool lineage = true;       // probe inherited columns once per parent row
   uint64_t group_filter_rows = 1ull << 22;  // smallest input a filter decides per lineage group
   bool fanout_blocks = true;  // long shared lists: k_fanout_blocks (else k_fanout_units)
+  uint64_t fanout_kept_cap = 1ull << 22;  // k_fanout_blocks' kept-row list (more: rerun on k_fanout_units)
   bool small_steps = true;   // steps over at most kSmallRows rows in one launch
   uint64_t *noalloc_scratch = nullptr;  // this device's no-alloc step scratch (wkg_step_k2u_noalloc)
   SQServer *sqs = nullptr;   // the persistent small-query server (created on first use)

@@ -151,6 +151,11 @@ class Device:
         """Probe a star's next arm once per input row of the fan-out before it (default on)."""
         self._check(N.hip_lib().wkg_set_lineage(self._h, 1 if on else 0))
 
+    def set_fanout_blocks(self, on, kept_cap=0):
+        """Long shared lists: 64 parents per workgroup with a kept-row list of
+        kept_cap rows (0: 2^22); off: one parent per workgroup."""
+        self._check(N.hip_lib().wkg_set_fanout_blocks(self._h, 1 if on else 0, int(kept_cap)))
+
     def set_group_filter_rows(self, rows):
         """Smallest table a filter from a star arm's new column decides once per
         lineage group (default 2^22 rows; 0: any size)."""
