@@ -271,7 +271,11 @@ def run_config5(args, quick=False, nested=False):
     prof_ms = (time.perf_counter() - t_prof) / n_prof * 1e3
     kernels = _kernel_table(dev, STEP_KERNELS, n_prof)
     roof = _expand_roofline(dev, n_prof, "expand_cycle_2b.traffic.json")
+    fanout = {k: dev.profile_read("fanout_" + k)[2] / n_prof
+              for k in ("filters", "parents", "lists", "list_entries", "work_items", "units", "row_tiles")}
+    fanout["fused"] = dev.profile_read("fused_fanouts")[2] / n_prof
     dev.profile(False)
+    log(f"fan-out filter per query: {fanout}")
     log(f"k_expand: {roof['launches_per_query']:.0f} launches/query, {roof['avg_launch_ms']:.3f} ms avg, "
         f"{roof['achieved']:.0f} GB/s algorithmic = {roof['frac']:.2f} of peak")
     config = {"workload": "6-pattern cyclic query (triangle closed by known_to_known, known_to_const filter) on a "
@@ -281,7 +285,8 @@ def run_config5(args, quick=False, nested=False):
               "expanded_edges_per_query": int(r.expanded), "row_half_splits_per_query": splits,
               "p50_ms": pct(lat, 0.5) * 1e3, "p99_ms": pct(lat, 0.99) * 1e3, "graph_build_s": build_s,
               "graph_load_s": load_s, "ms_per_query_with_kernel_events": prof_ms, "copy_kernel_gbs": copy_gbs,
-              "kernels_per_query": kernels, "gpu_kernel_ms_per_query": sum(v["ms"] for v in kernels.values())}
+              "kernels_per_query": kernels, "gpu_kernel_ms_per_query": sum(v["ms"] for v in kernels.values()),
+              "fanout_filter_per_query": fanout}
     if quick:
         out = _line(value, "expanded edges/s", args.steps, args.warmup, elapsed / args.steps * 1e3, config,
                     {"roofline": roof})

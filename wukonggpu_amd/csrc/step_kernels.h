@@ -1690,42 +1690,48 @@ __global__ __launch_bounds__(kBlock) void k_fanout_blocks(GraphView g, StepParam
   }
   const uint32_t tgt_l = s_tgt[lane];  // lane pi: parent pi's target
   auto ballot_k = [&](uint32_t pi, uint32_t tgt, int k) -> uint64_t {
-    if (kmulti >> k & 1u) return wave_ballot(c0[k] == tgt || ((mk[k] >> pi) & 1u));
-    return wave_ballot(c0[k] == tgt);
+    const uint64_t m1 = __builtin_amdgcn_uicmp(c0[k], tgt, 32);  // (32: equal) 64 rows' compares at once
+    if (kmulti >> k & 1u) return m1 | wave_ballot(((mk[k] >> pi) & 1u) != 0);
+    return m1;
   };
-  // pass 1: every parent against every entry; a wave's compares are 64
-  // rows' flags; only ballots with a kept row (rare) are counted
+  // pass 1: every parent against every single-entry row: one compare per
+  // 64 rows, only noting parents with a match (rare)
   for (uint32_t pi = 0; pi < npg; pi++) {
     const uint32_t tgt = (uint32_t)__builtin_amdgcn_readlane((int)tgt_l, (int)pi);
+    uint64_t any = 0;
 #pragma unroll
-    for (int k = 0; k < kFBPer; k++) {
-      const uint64_t bw = ballot_k(pi, tgt, k);
-      if (bw && lane == 0) {
-        const uint32_t c = (uint32_t)__builtin_popcountll(bw);
-        s_pc[pi][k][wave] = (uint16_t)c;
-        atomicAdd(&s_kcnt[pi], c);
-      }
-    }
+    for (int k = 0; k < kFBPer; k++) any |= __builtin_amdgcn_uicmp(c0[k], tgt, 32);
+    if (any && lane == 0) s_kcnt[pi] = 1;
   }
+  // ... and the parents a longer list keeps
+  for (uint32_t b = tid; b < nb; b += kBlock)
+    for (uint64_t msk = s_bmask[b]; msk; msk &= msk - 1) s_kcnt[__builtin_ctzll(msk)] = 1;
   __syncthreads();
-  // pass 2, parents with kept rows only: the ballots again, each kept row
-  // listed with its index among its parent's kept rows (row order: k, then
-  // wave, then lane)
+  // pass 2, parents with kept rows only: the ballots again, counted per
+  // (k, wave), then each kept row listed with its index among its parent's
+  // kept rows (row order: k, then wave, then lane)
   for (uint32_t pi = 0; pi < npg; pi++) {
     if (s_kcnt[pi] == 0) continue;  // block-uniform
     const uint32_t tgt = (uint32_t)__builtin_amdgcn_readlane((int)tgt_l, (int)pi);
+    uint64_t bw[kFBPer];
+#pragma unroll
+    for (int k = 0; k < kFBPer; k++) {
+      bw[k] = ballot_k(pi, tgt, k);
+      if (lane == 0) s_pc[pi][k][wave] = (uint16_t)__builtin_popcountll(bw[k]);
+    }
+    __syncthreads();
     const uint64_t rank = s_rank[pi], first = f.excl[s_par[pi]] + j0;
     uint32_t before = 0;  // kept rows of parent pi at smaller (k, wave)
+#pragma unroll
     for (int k = 0; k < kFBPer; k++) {
       for (int w = 0; w < W; w++)
         if (w < (int)wave) before += s_pc[pi][k][w];
-      const uint64_t bw = ballot_k(pi, tgt, k);
-      if (bw) {
+      if (bw[k]) {
         uint64_t base = 0;
-        if (lane == 0) base = atomicAdd((unsigned long long *)u.nkept, (unsigned long long)__builtin_popcountll(bw));
+        if (lane == 0) base = atomicAdd((unsigned long long *)u.nkept, (unsigned long long)__builtin_popcountll(bw[k]));
         base = shfl64(base, 0);
-        if ((bw >> lane) & 1u) {
-          const uint32_t lr = (uint32_t)__builtin_popcountll(bw & lanes_below());
+        if ((bw[k] >> lane) & 1u) {
+          const uint32_t lr = (uint32_t)__builtin_popcountll(bw[k] & lanes_below());
           const uint64_t slot = base + lr;
           if (slot < u.cap)
             u.kept[slot] = make_uint4((uint32_t)(first + (uint32_t)k * kBlock + wave * kWave + lane), (uint32_t)rank,
@@ -1734,7 +1740,9 @@ __global__ __launch_bounds__(kBlock) void k_fanout_blocks(GraphView g, StepParam
       }
       for (int w = (int)wave; w < W; w++) before += s_pc[pi][k][w];
     }
+    if (tid == 0) s_kcnt[pi] = before;  // (wave 0 counted every wave's rows)
   }
+  __syncthreads();
   if (tid < npg) {
     uint64_t xt = ex1_total;
 #pragma unroll

@@ -873,6 +873,9 @@ int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir,
   if (rc) return rc;
   prof_add_bytes("fanout_filters", 1.0);  // counts: taken, and by tile kind
   prof_add_bytes(units ? "fanout_units" : "fanout_row_tiles", 1.0);
+  prof_add_bytes("fanout_parents", (double)np);  // sizes, summed over the filters
+  prof_add_bytes("fanout_lists", (double)nl);
+  prof_add_bytes("fanout_list_entries", (double)total);
   const int tr = units ? TRU : TR;
   const uint64_t ntiles = units ? nunits : (R + TR - 1) / TR, nchunks = (ntiles + kScanChunk - 1) / kScanChunk;
   // uploads: pbase | lbase | ly (u32) [| ubase | xbase | poff | par (u32) | ln | xb]
@@ -916,6 +919,7 @@ int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir,
         xb[L + 1] = xb[L] + ((ln[L] + kFBRows - 1) / kFBRows) * ((cnt[L + 1] + kFBGroup - 1) / kFBGroup);
       }
       nwork = xb[nl];
+      prof_add_bytes("fanout_work_items", (double)nwork);
     }
   }
   uint64_t *d_pbase = buf, *d_lbase = buf + np;
