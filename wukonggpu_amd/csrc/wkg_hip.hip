@@ -882,6 +882,7 @@ int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir,
   const uint64_t w_ly = (nl + 1) / 2, w_par = (np + 1) / 2;
   const uint64_t up_words = np + nl + w_ly + (units ? np + 2 * (nl + 1) + w_par + nl + (nl + 1) : 0);
   uint64_t nwork = 0;  // units through k_fanout_blocks: (block of a list) x (group of its parents) items
+  double blocks_bytes = 0;  // k_fanout_blocks' reads and writes (see below)
   const uint64_t scan_words = 4 + 3 * ntiles + 3 * nchunks + (units ? ntiles : 0);  // (+ unit first rows)
   const uint64_t bit_words = (ntiles * (uint64_t)(tr / 32) + 1) / 2;
   uint64_t *buf = (uint64_t *)pool->alloc((up_words + scan_words + bit_words + total) * 8, s, err_slot());
@@ -920,6 +921,11 @@ int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir,
       }
       nwork = xb[nl];
       prof_add_bytes("fanout_work_items", (double)nwork);
+      // per work item its block's D entries (8 B each) and its parents'
+      // cells, ranks and first rows (24 B each); per unit its three counters
+      for (uint64_t L = 0; L < nl; L++)
+        blocks_bytes += 8.0 * (double)ln[L] * (double)((cnt[L + 1] + kFBGroup - 1) / kFBGroup) + 24.0 * (double)cnt[L + 1] * (double)((ln[L] + kFBRows - 1) / kFBRows);
+      blocks_bytes += 24.0 * (double)ntiles;
     }
   }
   uint64_t *d_pbase = buf, *d_lbase = buf + np;
@@ -1005,9 +1011,10 @@ int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir,
   uint64_t kept = 0, examined = 0;
   for (;;) {
     {
-      // per row: a D entry (8 B, mostly cached) and the list it names; the
-      // input rows are not read
-      ProfScope ps(this, s, "fanout_filter", 8.0 * (double)R);
+      // one parent per workgroup: per row a D entry (8 B, mostly cached) and
+      // the list it names; blocks: each D entry once per parent group (the
+      // input rows are never read)
+      ProfScope ps(this, s, "fanout_filter", blocks ? blocks_bytes : 8.0 * (double)R);
       const uint64_t per = std::max<uint64_t>(thread_launch_rows() / (uint64_t)tr, 1);
       if (blocks) {
         k_zero_u64<<<1, kBlock, 0, s>>>(nkept, 1);
