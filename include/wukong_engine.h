@@ -108,7 +108,9 @@ int wke_set_fused_filters(wke_engine *e, int on);
  * a constant and have at most 8 patterns go first to the small-query server
  * (wkg_query_small_submit), which admits each one as soon as one of its
  * `workgroups` workgroups is free, with no batch boundary; one that
- * outgrows its limits goes on to the light lanes. Default 16; 0 sends
+ * outgrows its limits goes on to the light lanes. Under load (more than 64
+ * light queries in flight) new light queries go to the light lanes, whose
+ * batches serve more queries per second. Default 16; 0 sends
  * every light query to the light lanes. The counters of the last serve:
  * queries the server answered and queries it declined. */
 int wke_set_serve_small(wke_engine *e, int workgroups);

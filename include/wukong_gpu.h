@@ -371,8 +371,15 @@ int wkg_set_small_query_server(wkg_ctx *ctx, int workgroups);
 /* A server workgroup exits after idle_s without a query (default 200 us)
  * and takes no query after life_s (default 20 ms, at most 10 s); the next
  * query launches it again. Work on a stream that shares the server's
- * hardware queue waits behind it up to that long. */
+ * hardware queue waits behind it up to that long: give the process enough
+ * hardware queues for its streams (environment GPU_MAX_HW_QUEUES, read when
+ * the HIP runtime starts; HIP's default is 4, and with fewer than 8 the
+ * server's first use lowers life_s to 2 ms). */
 int wkg_set_small_query_server_times(wkg_ctx *ctx, double idle_s, double life_s);
+int wkg_get_small_query_server_times(const wkg_ctx *ctx, double *idle_s, double *life_s);
+/* The process's hardware queues per device as HIP reads them
+ * (GPU_MAX_HW_QUEUES; 4 when unset). */
+int wkg_hw_queues(void);
 
 /* Launch and tile shapes (0 = default). The autotuner below picks them. */
 typedef struct wkg_batch_tuning {

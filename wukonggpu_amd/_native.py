@@ -9,6 +9,14 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(_HERE, "lib")
 
+# The small-query server is a persistent kernel on a stream of its own. HIP
+# maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default), and
+# work on a stream that shares the server's queue waits behind it. The
+# library's streams (context, executors, served lanes, server) fit in 16:
+# ask for that many unless the process set its own (the HIP runtime reads it
+# when it starts).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 # status codes (include/wukong_gpu.h)
 WKG_OK, WKG_EINVAL, WKG_ENOMEM, WKG_EDEVICE, WKG_ERANGE, WKG_ESTATE = 0, -1, -2, -3, -4, -5
 STEP_I2U, STEP_C2U, STEP_K2U, STEP_K2K, STEP_K2C = 0, 1, 2, 3, 4

@@ -837,6 +837,15 @@ int serve_loop(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq
   // goes on to the light lanes)
   const bool small = resident && e->serve_small > 0;
   if (small && wkg_set_small_query_server(e->ctx, e->serve_small) != WKG_OK) return e->fail_ctx(WKG_EINVAL);
+  // With few hardware queues the lanes' streams may share the server's: its
+  // workgroups then leave after 20 us without a query and within 1 ms, so a
+  // lane batch queued behind them waits at most that long.
+  double sq_idle = 0, sq_life = 0;
+  const bool sq_short = small && wkg_hw_queues() < 8;
+  if (sq_short) {
+    (void)wkg_get_small_query_server_times(e->ctx, &sq_idle, &sq_life);
+    (void)wkg_set_small_query_server_times(e->ctx, std::min(sq_idle, 20e-6), std::min(sq_life, 1e-3));
+  }
   const bool heavy_lane = any_heavy && !worker;
   const uint64_t heavy_arena = (heavy_lane || worker) ? table_budget_bytes / 2 : 0;
   const uint64_t light_arena = (table_budget_bytes - heavy_arena) / (uint64_t)light_lanes;
@@ -870,6 +879,7 @@ int serve_loop(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq
     }
     sq_run.clear();
     if (small) (void)wkg_set_small_query_server(e->ctx, sq_keep);
+    if (sq_short) (void)wkg_set_small_query_server_times(e->ctx, sq_idle, sq_life);
   };
   for (int i = 0; i < light_lanes + (heavy_lane ? 1 : 0); i++) {
     Lane l;
@@ -994,7 +1004,9 @@ int serve_loop(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq
       } else if ((int)plans[q].size() > e->cfg_patterns) {
         run_plain(q);
       } else if (small && !(any_heavy && heavy[q]) && plans[q].size() <= 8 &&
-                 (plans[q][0].kind == WKG_STEP_I2U || plans[q][0].kind == WKG_STEP_C2U) && sq_wait.size() < 64) {
+                 (plans[q][0].kind == WKG_STEP_I2U || plans[q][0].kind == WKG_STEP_C2U) && inflight[0] <= 64) {
+        // (under load -- more than 64 light queries in flight -- the light
+        // lanes' batches serve more queries per second than the server)
         sq_wait.push_back(q);
       } else if (worker && heavy[q]) {
         {

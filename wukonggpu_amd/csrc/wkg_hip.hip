@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -1848,6 +1849,7 @@ struct SQServer {
   // time of the last ticket and the life time of the launch a workgroup
   // surely runs (or is about to start), so no event query is needed
   std::chrono::steady_clock::time_point launch_t, submit_t;
+  std::atomic<uint64_t> polls{0};  // unanswered polls (collect without waiting)
   double tick_hz = 1e8;         // the device wall clock
   uint64_t *tail_h() { return (uint64_t *)h; }
   uint64_t *stop_h() { return (uint64_t *)(h + 64); }
@@ -1943,6 +1945,9 @@ int sq_init(wkg_ctx *ctx) {
   }
   memset(v->h, 0, bytes);
   v->busy.assign(SQServer::kSQSlots, 0);
+  // With HIP's default 4 hardware queues, other streams of the process are
+  // likely to share the server's: keep its workgroups' life short then.
+  if (wkg_hw_queues() < 8) ctx->sq_life_s = std::min(ctx->sq_life_s, 0.002);
   ctx->sqs = v;
   return WKG_OK;
 }
@@ -2013,6 +2018,9 @@ int sq_collect(wkg_ctx *ctx, uint64_t t, bool wait, int32_t *cells, uint64_t cap
                uint64_t *expanded) {
   SQServer *v = ctx->sqs;
   if (!v) return ctx->fail(WKG_EINVAL, "ticket %llu is not in flight", (unsigned long long)t);
+  if (!wait && __atomic_load_n(&v->slot_h(t)->resp.done, __ATOMIC_ACQUIRE) != t + 1 &&
+      (v->polls.fetch_add(1, std::memory_order_relaxed) & 63) != 63)
+    return WKG_EAGAIN;  // not answered (every 64th such poll also checks that a server runs, below)
   {
     std::lock_guard<std::mutex> lk(v->mu);
     if (t >= v->tail || t + SQServer::kSQSlots < v->tail || !v->busy[t & (SQServer::kSQSlots - 1)])
@@ -2159,6 +2167,19 @@ int wkg_set_small_query_server(wkg_ctx *ctx, int workgroups) {
   ctx->sq_stop();  // the next query launches the new shape
   ctx->sq_workgroups = workgroups;
   return WKG_OK;
+}
+
+int wkg_get_small_query_server_times(const wkg_ctx *ctx, double *idle_s, double *life_s) {
+  if (!ctx || !idle_s || !life_s) return WKG_EINVAL;
+  *idle_s = ctx->sq_idle_s;
+  *life_s = ctx->sq_life_s;
+  return WKG_OK;
+}
+
+int wkg_hw_queues(void) {
+  const char *hq = getenv("GPU_MAX_HW_QUEUES");
+  const int n = hq ? atoi(hq) : 0;
+  return n > 0 ? n : 4;
 }
 
 int wkg_set_small_query_server_times(wkg_ctx *ctx, double idle_s, double life_s) {
