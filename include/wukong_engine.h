@@ -97,6 +97,11 @@ int wke_set_tuning(wke_engine *e, const wkg_batch_tuning *t, int heavy);
  * per-step path when the graph is resident, else a heavy executor lane; 0 a
  * heavy executor lane; 1 the worker thread (resident graphs only). */
 int wke_set_heavy_mode(wke_engine *e, int mode);
+/* Worker threads of that heavy lane (default 2, at most 8): each runs heavy
+ * queries one at a time on its own stream and read-back block
+ * (wkg_thread_context_begin) with its share of the heavy arena, so heavy
+ * queries overlap on the device. */
+int wke_set_heavy_workers(wke_engine *e, int n);
 /* Plain path: a known_to_unknown step whose new column the next pattern
  * filters (known_to_known back to an earlier column, or known_to_const)
  * runs with that filter in one call (wkg_step_k2u_filter), which does not

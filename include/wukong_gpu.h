@@ -162,6 +162,13 @@ int wkg_set_group_filter_rows(wkg_ctx *ctx, uint64_t rows);
  * parent per workgroup). On by default; 0 always runs one parent per
  * workgroup. The output and work count are the same either way. */
 int wkg_set_fanout_blocks(wkg_ctx *ctx, int on, uint64_t kept_cap);
+/* The calling thread's per-step calls on ctx (stream NULL) get a stream and
+ * a pinned read-back block of their own until _end, so several threads can
+ * run per-step calls on one context at once (a graph that is resident;
+ * framed residency is not thread-safe). Tables it made stay valid after
+ * _end. No counterpart in the reference (one engine thread per GPU). */
+int wkg_thread_context_begin(wkg_ctx *ctx);
+int wkg_thread_context_end(wkg_ctx *ctx);
 /* A known-start step over at most 512 input rows runs its probe, scan,
  * compaction and (up to 2048 output rows) expand in one launch instead of
  * five; its output is the same, row for row. On by default; 0 turns it off.

@@ -490,6 +490,52 @@ def test_k2u_filter_call(graph_hubs):
     dev.close()
 
 
+def test_thread_contexts_run_queries_at_once(graph_hubs):
+    """Four host threads, each with a stream and read-back block of its own
+    (wkg_thread_context_begin), run queries on one context at the same time
+    (fan-outs, lineage filters, small queries); every answer equals the
+    oracle's. A thread cannot begin twice or end a context it has not
+    begun."""
+    import threading
+    _, gs = graph_hubs
+    dev = Device(0)
+    dev.load(gs)
+    eng = GPUEngine(dev)
+    rng = np.random.default_rng(96)
+    qs = random_queries(rng, 6000, 3, [5, 6, 7], 160, max_steps=5)
+    head = [7, 1, 0, -1, -1, 2, 1, -2, -2, 2, 0, -3, -3, 2, 1, -4]
+    qs += [head + [-4, 3, 1, -2], head + [-4, 3, 1, VB + 17]] * 4
+    want = [oracle.run_query(gs, q)[:2] for q in qs]
+    errors = []
+
+    def work(part):
+        try:
+            dev.thread_context_begin()
+            try:
+                with pytest.raises(DeviceError):
+                    dev.thread_context_begin()
+                for i in part:
+                    r = eng.run_query(qs[i])
+                    wt, wex = want[i]
+                    assert r.status == 0 and r.table.shape[0] == wt.shape[0], qs[i]
+                    assert np.array_equal(r.table.reshape(wt.shape), wt) and r.expanded == wex, qs[i]
+            finally:
+                dev.thread_context_end()
+        except BaseException as ex:  # reported below, on the test's thread
+            errors.append(ex)
+
+    ths = [threading.Thread(target=work, args=(range(k, len(qs), 4),)) for k in range(4)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    assert not errors, errors[0]
+    with pytest.raises(DeviceError):
+        dev.thread_context_end()
+    eng.close()
+    dev.close()
+
+
 def test_dataset_directory_and_persisted_image(tmp_path):
     """A graph read from a dataset directory in the reference's ID format
     (id_* files, str_index, str_normal), built, saved as an image file and

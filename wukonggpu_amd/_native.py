@@ -170,6 +170,8 @@ def hip_lib():
         _sig(L, "wkg_query_small_collect", C.c_int, vp, u64, C.c_int, P(i32), u64, P(u64), P(C.c_int), P(u64))
         _sig(L, "wkg_set_small_query_server", C.c_int, vp, C.c_int)
         _sig(L, "wkg_set_fanout_blocks", C.c_int, vp, C.c_int, u64)
+        _sig(L, "wkg_thread_context_begin", C.c_int, vp)
+        _sig(L, "wkg_thread_context_end", C.c_int, vp)
         _sig(L, "wkg_set_small_query_server_times", C.c_int, vp, C.c_double, C.c_double)
         _sig(L, "wkg_step_batch", C.c_int, vp, C.c_int, P(StepDesc), vp, P(vp), P(u64))
         _sig(L, "wkg_profile_enable", C.c_int, vp, C.c_int)
@@ -215,6 +217,7 @@ def engine_lib():
         _sig(L, "wke_serve_stats", C.c_int, vp, P(u64), P(u64), P(u64), P(u64))
         _sig(L, "wke_set_serve_small", C.c_int, vp, C.c_int)
         _sig(L, "wke_set_fused_filters", C.c_int, vp, C.c_int)
+        _sig(L, "wke_set_heavy_workers", C.c_int, vp, C.c_int)
         _sig(L, "wke_serve_small_stats", C.c_int, vp, P(u64), P(u64))
         _sig(L, "wke_serve", C.c_int, vp, P(i64), P(i64), C.c_int, P(dbl), P(C.c_uint8), u64, C.c_int,
              P(Result), P(dbl))

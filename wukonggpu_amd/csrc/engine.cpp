@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -163,6 +164,7 @@ struct wke_engine {
   // stream per context.
   std::mutex plain_mu;
   int heavy_mode = -1;  // wke_set_heavy_mode
+  int heavy_workers = 2;  // wke_set_heavy_workers: served heavy queries at once (worker threads)
   bool small_queries = true;  // wke_set_small_queries: try wkg_query_small first
   int serve_small = 16;       // wke_set_serve_small: small-query server workgroups while serving (0: none)
   bool fuse_filters = true;   // wke_set_fused_filters: known_to_unknown + filter on its new column in one call
@@ -308,7 +310,7 @@ struct wke_engine {
     ~HostRows() { free(data); }
   };
 
-  uint64_t splits = 0;  // row-half splits made by run_rest (wke_stats)
+  std::atomic<uint64_t> splits{0};  // row-half splits made by run_rest (wke_stats)
 
   // Runs r's remaining steps. A step whose tables do not fit the table
   // budget (WKG_ENOMEM) is retried on the two row halves of its input, each
@@ -700,6 +702,12 @@ int wke_set_heavy_mode(wke_engine *e, int mode) {
   return WKG_OK;
 }
 
+int wke_set_heavy_workers(wke_engine *e, int n) {
+  if (!e || n < 1 || n > 8) return WKG_EINVAL;
+  e->heavy_workers = n;
+  return WKG_OK;
+}
+
 int wke_set_fused_filters(wke_engine *e, int on) {
   if (!e) return WKG_EINVAL;
   e->fuse_filters = on != 0;
@@ -786,9 +794,10 @@ struct Lane {
   std::vector<int> run;
 };
 
-// The heavy lane on a resident graph: a host thread that runs heavy queries
-// one at a time on the per-step path (whose star arms use lineage), on the
-// context's own stream, while the light lanes' executors run on theirs.
+// The heavy lane on a resident graph: host threads that each run heavy
+// queries one at a time on the per-step path (whose star arms use lineage),
+// each on a stream and read-back block of its own (wkg_thread_context_*),
+// while the light lanes' executors run on theirs.
 struct PlainWorker {
   std::mutex mu;
   std::condition_variable cv;
@@ -798,9 +807,10 @@ struct PlainWorker {
     std::string why;
   };
   std::vector<Done> out;
-  bool stop = false, ready = false;
+  bool stop = false;
+  int ready = 0;
   int pending = 0;  // queued or running (main thread only)
-  std::thread th;
+  std::vector<std::thread> th;
 };
 
 }  // namespace
@@ -856,13 +866,14 @@ int serve_loop(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq
   std::vector<int32_t> sq_buf(WKG_SMALL_QUERY_CELLS);
   const int sq_keep = 4;                            // the context's server shape after serving
   auto release = [&] {
-    if (pw.th.joinable()) {
+    if (!pw.th.empty()) {
       {
         std::lock_guard<std::mutex> lk(pw.mu);
         pw.stop = true;
       }
-      pw.cv.notify_one();
-      pw.th.join();
+      pw.cv.notify_all();
+      for (std::thread &t : pw.th) t.join();
+      pw.th.clear();
     }
     for (Lane &l : lanes) {
       if (l.busy) {
@@ -899,38 +910,46 @@ int serve_loop(wke_engine *e, const int64_t *cmd, const int64_t *cmd_off, int nq
     // share of the budget), reserved before arrivals start and reset after
     // every query: no hipMalloc or hipFree (which wait on the device) while
     // the light lanes keep it busy.
-    pw.th = std::thread([&] {
-      const bool arena = wkg_thread_arena_begin(e->ctx, heavy_arena) == WKG_OK;
-      // short launches, so that light batches get the GPU between them
-      (void)wkg_thread_launch_rows(e->ctx, 1ull << 22);
-      {
-        std::lock_guard<std::mutex> lk(pw.mu);
-        pw.ready = true;
-      }
-      pw.cv.notify_all();
-      for (;;) {
-        int q;
+    // Each of heavy_workers threads has its own share of the arena, stream
+    // and read-back block, so heavy queries overlap on the device.
+    const int nw = std::max(1, e->heavy_workers);
+    for (int w = 0; w < nw; w++)
+      pw.th.emplace_back([&, nw] {
+        const bool own = wkg_thread_context_begin(e->ctx) == WKG_OK;
+        const bool arena = wkg_thread_arena_begin(e->ctx, heavy_arena / (uint64_t)nw) == WKG_OK;
+        // short launches, so that light batches get the GPU between them
+        (void)wkg_thread_launch_rows(e->ctx, 1ull << 22);
         {
-          std::unique_lock<std::mutex> lk(pw.mu);
-          pw.cv.wait(lk, [&] { return pw.stop || !pw.in.empty(); });
-          if (pw.stop) break;
-          q = pw.in.front();
-          pw.in.pop_front();
+          std::lock_guard<std::mutex> lk(pw.mu);
+          pw.ready++;
         }
-        int rc;
-        {
-          std::lock_guard<std::mutex> pl(e->plain_mu);
-          rc = wke_run_query(e, cmd + cmd_off[q], (int)(cmd_off[q + 1] - cmd_off[q]), &res[q]);
+        pw.cv.notify_all();
+        for (;;) {
+          int q;
+          {
+            std::unique_lock<std::mutex> lk(pw.mu);
+            pw.cv.wait(lk, [&] { return pw.stop || !pw.in.empty(); });
+            if (pw.stop) break;
+            q = pw.in.front();
+            pw.in.pop_front();
+          }
+          int rc;
+          if (own) {
+            rc = wke_run_query(e, cmd + cmd_off[q], (int)(cmd_off[q + 1] - cmd_off[q]), &res[q]);
+          } else {  // (no stream of its own: the context's, one query at a time)
+            std::lock_guard<std::mutex> pl(e->plain_mu);
+            rc = wke_run_query(e, cmd + cmd_off[q], (int)(cmd_off[q + 1] - cmd_off[q]), &res[q]);
+          }
           if (arena) (void)wkg_thread_arena_reset(e->ctx);
+          std::lock_guard<std::mutex> lk(pw.mu);
+          pw.out.push_back({q, rc, rc ? std::string(wke_last_error(e)) : std::string()});
         }
-        std::lock_guard<std::mutex> lk(pw.mu);
-        pw.out.push_back({q, rc, rc ? std::string(wke_last_error(e)) : std::string()});
-      }
-      if (arena) (void)wkg_thread_arena_end(e->ctx);
-      (void)wkg_thread_launch_rows(e->ctx, 0);
-    });
+        if (arena) (void)wkg_thread_arena_end(e->ctx);
+        (void)wkg_thread_launch_rows(e->ctx, 0);
+        if (own) (void)wkg_thread_context_end(e->ctx);
+      });
     std::unique_lock<std::mutex> lk(pw.mu);
-    pw.cv.wait(lk, [&] { return pw.ready; });
+    pw.cv.wait(lk, [&] { return pw.ready == nw; });
   }
   const auto t0 = Clock::now();
   auto now = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };

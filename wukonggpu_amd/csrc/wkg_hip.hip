@@ -184,7 +184,9 @@ GraphView wkg_ctx::view() const {
 // current on the calling thread (a context may be driven from several).
 hipStream_t wkg_ctx::pick(void *s) const {
   (void)hipSetDevice(device);
-  return s ? (hipStream_t)s : stream;
+  if (s) return (hipStream_t)s;
+  const ThreadCtx *t = tctx();
+  return t ? t->stream : stream;
 }
 
 int wkg_ctx::new_table(uint64_t rows, int cols, hipStream_t s, wkg_table **out) {
@@ -242,7 +244,7 @@ void wkg_ctx::unref_table(wkg_table *t) {
 // 20 ms of polling it waits on the stream instead (which also reports a
 // failed kernel whose signal never comes).
 int wkg_ctx::wait_signal(const Signal &sg, hipStream_t s) {
-  const uint64_t *flag = h_rb + kRbFlag;
+  const uint64_t *flag = rb_host() + kRbFlag;
   const auto t0 = std::chrono::steady_clock::now();
   for (uint32_t spins = 0; __atomic_load_n(flag, __ATOMIC_ACQUIRE) != sg.seq; spins++) {
     if ((spins & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) {
@@ -266,10 +268,10 @@ int wkg_ctx::read_words(const uint64_t *d, uint64_t n, hipStream_t s, uint64_t *
     return WKG_OK;
   }
   const Signal sg = next_signal();
-  k_stage<<<1, kBlock, 0, s>>>((const int32_t *)d, 2 * n, (int32_t *)((char *)h_rb_dev + kStageOff), sg);
+  k_stage<<<1, kBlock, 0, s>>>((const int32_t *)d, 2 * n, (int32_t *)((char *)rb_dev() + kStageOff), sg);
   LAUNCHCHK(this);
   if (int rc = wait_signal(sg, s)) return rc;
-  memcpy(host, (const char *)h_rb + kStageOff, n * 8);
+  memcpy(host, (const char *)rb_host() + kStageOff, n * 8);
   return WKG_OK;
 }
 
@@ -302,7 +304,7 @@ int wkg_ctx::step_single_list(uint64_t key, int pid, hipStream_t s, wkg_table **
     drop_table(t);
     return rc;
   }
-  const uint64_t deg = h_rb[0], off = h_rb[1];
+  const uint64_t deg = rb_host()[0], off = rb_host()[1];
   if (deg <= kListCap) {
     t->rows = deg;
     t->bytes = deg * 4;
@@ -495,7 +497,7 @@ int wkg_ctx::probe_pass(int mode, const wkg_table *in, int sc, int pid, int dir,
     release_pass(pp, s);
     return le != hipSuccess ? fail(WKG_EDEVICE, "probe launch failed: %s", hipGetErrorString(le)) : rc;
   }
-  memcpy(pp.cnt, h_rb, 3 * sizeof(uint64_t));
+  memcpy(pp.cnt, rb_host(), 3 * sizeof(uint64_t));
   if (mode == kModeK2U) {
     prof_add_bytes("finalize", 32.0 * pp.cnt[2]);
     prof_add_bytes("probe", 128.0 * (double)pp.cnt[1] + 16.0 * (double)pp.cnt[2]);
@@ -641,7 +643,7 @@ int wkg_ctx::step_k2u_lineage(const wkg_table *in, int sc, int pid, int dir, hip
     uint64_t cnt[3];
     rc = le == hipSuccess ? wait_signal(q.sg, s) : fail(WKG_EDEVICE, "lineage launch failed: %s",
                                                          hipGetErrorString(le));
-    if (!rc) memcpy(cnt, h_rb, sizeof cnt);
+    if (!rc) memcpy(cnt, rb_host(), sizeof cnt);
     if (rc) {
       pool->release(lift, s);
       release_pass(pp, s);
@@ -803,7 +805,7 @@ int wkg_ctx::step_filter_groups(int mode, const wkg_table *in, int pid, int dir,
   if (le != hipSuccess) return done(fail(WKG_EDEVICE, "filter launch failed: %s", hipGetErrorString(le)));
   if ((rc = wait_signal(p.sg, s))) return done(rc);
   uint64_t cnt[3];
-  memcpy(cnt, h_rb, sizeof cnt);
+  memcpy(cnt, rb_host(), sizeof cnt);
   prof_add_bytes("compact", 16.0 * cnt[2] + 8.0 * cnt[2] * c);
   t->rows = cnt[0];
   t->bytes = t->rows * (uint64_t)c * 4;
@@ -1052,8 +1054,8 @@ int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir,
     le = hipGetLastError();
     if (le != hipSuccess) return done2(fail(WKG_EDEVICE, "fan-out filter launch failed: %s", hipGetErrorString(le)));
     if ((rc = wait_signal(p.sg, s))) return done2(rc);
-    kept = h_rb[0];
-    examined = h_rb[1];
+    kept = rb_host()[0];
+    examined = rb_host()[1];
     if (!blocks || kept <= kept_cap) break;
     blocks = false;  // more kept rows than the list holds: the bit-word path
     prof_add_bytes("fanout_kept_overflow", 1.0);  // a count
@@ -1280,7 +1282,7 @@ int wkg_ctx::step_small(int mode, const wkg_table *in, int sc, int pid, int dir,
     drop_table(t);
     return rc;
   }
-  memcpy(pp.cnt, h_rb, 3 * sizeof(uint64_t));
+  memcpy(pp.cnt, rb_host(), 3 * sizeof(uint64_t));
   if (mode != kModeK2U) {  // filters: kept rows compacted in place
     t->rows = pp.cnt[0];
     t->bytes = t->rows * (uint64_t)c * 4;
@@ -1612,10 +1614,10 @@ int wkg_table_to_host(wkg_ctx *ctx, const wkg_table *t, int32_t *host, void *str
     return ctx->read_words((const uint64_t *)t->d, t->bytes / 8, s, (uint64_t *)host);
   if (t->bytes <= kStageBytes) {
     const Signal sg = ctx->next_signal();
-    k_stage<<<1, kBlock, 0, s>>>(t->d, t->bytes / 4, (int32_t *)((char *)ctx->h_rb_dev + kStageOff), sg);
+    k_stage<<<1, kBlock, 0, s>>>(t->d, t->bytes / 4, (int32_t *)((char *)ctx->rb_dev() + kStageOff), sg);
     LAUNCHCHK(ctx);
     if (int rc = ctx->wait_signal(sg, s)) return rc;
-    memcpy(host, (const char *)ctx->h_rb + kStageOff, t->bytes);
+    memcpy(host, (const char *)ctx->rb_host() + kStageOff, t->bytes);
     return WKG_OK;
   }
   HIPCHK(ctx, hipMemcpyAsync(host, t->d, t->bytes, hipMemcpyDeviceToHost, s));
@@ -1691,6 +1693,40 @@ int wkg_set_lineage(wkg_ctx *ctx, int on) {
 int wkg_set_group_filter_rows(wkg_ctx *ctx, uint64_t rows) {
   if (!ctx) return WKG_EINVAL;
   ctx->group_filter_rows = rows;
+  return WKG_OK;
+}
+
+int wkg_thread_context_begin(wkg_ctx *ctx) {
+  if (!ctx) return WKG_EINVAL;
+  ThreadCtx *&t = thread_ctx();
+  if (t) return ctx->fail(WKG_ESTATE, "the calling thread already has a thread context");
+  ThreadCtx *n = new (std::nothrow) ThreadCtx;
+  if (!n) return ctx->fail(WKG_ENOMEM, "out of host memory");
+  (void)hipSetDevice(ctx->device);
+  if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc((void **)&n->h_rb, kStageOff + kStageBytes, hipHostMallocCoherent | hipHostMallocMapped) !=
+          hipSuccess ||
+      hipHostGetDevicePointer((void **)&n->h_rb_dev, n->h_rb, 0) != hipSuccess) {
+    if (n->h_rb) (void)hipHostFree(n->h_rb);
+    if (n->stream) (void)hipStreamDestroy(n->stream);
+    delete n;
+    return ctx->fail(WKG_EDEVICE, "thread context set-up failed");
+  }
+  memset(n->h_rb, 0, kStageOff);  // the flag starts below every sequence number
+  n->ctx = ctx;
+  t = n;
+  return WKG_OK;
+}
+
+int wkg_thread_context_end(wkg_ctx *ctx) {
+  if (!ctx) return WKG_EINVAL;
+  ThreadCtx *&t = thread_ctx();
+  if (!t || t->ctx != ctx) return ctx->fail(WKG_ESTATE, "the calling thread has no context of its own here");
+  (void)hipStreamSynchronize(t->stream);  // (its tables' memory returns to the pool with this stream)
+  (void)hipStreamDestroy(t->stream);
+  (void)hipHostFree(t->h_rb);
+  delete t;
+  t = nullptr;
   return WKG_OK;
 }
 
@@ -2078,7 +2114,7 @@ int sq_one_launch(wkg_ctx *ctx, const wkg_plan_step *steps, int n, void *stream,
   hipStream_t s = ctx->pick(stream);
   SmallQuery q;
   sq_fill(ctx, steps, n, q);
-  int32_t *stage = (int32_t *)((char *)ctx->h_rb_dev + kStageOff);
+  int32_t *stage = (int32_t *)((char *)ctx->rb_dev() + kStageOff);
   const GraphView g = ctx->view();
   const Signal sg = ctx->next_signal();
   {
@@ -2089,9 +2125,10 @@ int sq_one_launch(wkg_ctx *ctx, const wkg_plan_step *steps, int n, void *stream,
   hipError_t le = hipGetLastError();
   if (le != hipSuccess) return ctx->fail(WKG_EDEVICE, "small query launch failed: %s", hipGetErrorString(le));
   if (int rc = ctx->wait_signal(sg, s)) return rc;
-  const uint64_t status = ctx->h_rb[0], r = ctx->h_rb[1], c = ctx->h_rb[2];
-  if (expanded) *expanded = ctx->h_rb[3];
-  if (status != 0) sq_note_decline(ctx, steps[0], ctx->h_rb[4]);
+  const uint64_t *rb = ctx->rb_host();
+  const uint64_t status = rb[0], r = rb[1], c = rb[2];
+  if (expanded) *expanded = rb[3];
+  if (status != 0) sq_note_decline(ctx, steps[0], rb[4]);
   ctx->prof_add_bytes(status ? "small_query_declined" : "small_query_taken", 1.0);  // counts
   if (status != 0)
     return ctx->fail(WKG_ERANGE, "a table outgrew the small-query limits (%u input rows, %u cells)", kSQRows, kSQCells);
@@ -2100,7 +2137,7 @@ int sq_one_launch(wkg_ctx *ctx, const wkg_plan_step *steps, int n, void *stream,
                      (unsigned long long)cap);
   *rows = r;
   *cols = (int)c;
-  if (r * c) memcpy(cells, (const char *)ctx->h_rb + kStageOff, r * c * 4);
+  if (r * c) memcpy(cells, (const char *)rb + kStageOff, r * c * 4);
   return WKG_OK;
 }
 

@@ -131,6 +131,21 @@ struct Residency {
 };
 
 struct SQServer;
+struct wkg_ctx;
+
+// A thread's own stream and pinned read-back block on one context
+// (wkg_thread_context_begin): that thread's per-step calls use them instead
+// of the context's, so several threads can run queries at once.
+struct ThreadCtx {
+  const wkg_ctx *ctx = nullptr;
+  hipStream_t stream = nullptr;
+  uint64_t *h_rb = nullptr, *h_rb_dev = nullptr;
+  uint64_t seq = 0;
+};
+inline ThreadCtx *&thread_ctx() {
+  static thread_local ThreadCtx *t = nullptr;
+  return t;
+}
 
 struct wkg_ctx {
   int device = 0;
@@ -205,7 +220,23 @@ struct wkg_ctx {
   void drop_table(wkg_table *t);
   void unref_table(wkg_table *t);
   int read_words(const uint64_t *d, uint64_t n, hipStream_t s, uint64_t *host);
-  wkg::Signal next_signal() { return wkg::Signal{h_rb_dev, ++rb_seq}; }
+  // the calling thread's read-back block (its own, or the context's)
+  ThreadCtx *tctx() const {
+    ThreadCtx *t = thread_ctx();
+    return t && t->ctx == this ? t : nullptr;
+  }
+  uint64_t *rb_host() const {
+    const ThreadCtx *t = tctx();
+    return t ? t->h_rb : h_rb;
+  }
+  uint64_t *rb_dev() const {
+    const ThreadCtx *t = tctx();
+    return t ? t->h_rb_dev : h_rb_dev;
+  }
+  wkg::Signal next_signal() {
+    ThreadCtx *t = tctx();
+    return t ? wkg::Signal{t->h_rb_dev, ++t->seq} : wkg::Signal{h_rb_dev, ++rb_seq};
+  }
   int wait_signal(const wkg::Signal &sg, hipStream_t s);
   int check_pattern(int pid, int dir);
   int step_single_list(uint64_t key, int pid, hipStream_t s, wkg_table **out, uint64_t *expanded);

@@ -151,6 +151,13 @@ class Device:
         """Probe a star's next arm once per input row of the fan-out before it (default on)."""
         self._check(N.hip_lib().wkg_set_lineage(self._h, 1 if on else 0))
 
+    def thread_context_begin(self):
+        """The calling thread's per-step calls get a stream and read-back block of their own."""
+        self._check(N.hip_lib().wkg_thread_context_begin(self._h))
+
+    def thread_context_end(self):
+        self._check(N.hip_lib().wkg_thread_context_end(self._h))
+
     def set_fanout_blocks(self, on, kept_cap=0):
         """Long shared lists: 64 parents per workgroup with a kept-row list of
         kept_cap rows (0: 2^22); off: one parent per workgroup."""
@@ -495,6 +502,12 @@ class GPUEngine:
         rc = L.wke_set_heavy_mode(self._h, int(mode))
         if rc != 0:
             raise DeviceError(rc, "wke_set_heavy_mode failed")
+
+    def set_heavy_workers(self, n):
+        """Served heavy queries at once (worker threads, each with its own stream)."""
+        rc = N.engine_lib().wke_set_heavy_workers(self._h, int(n))
+        if rc != 0:
+            raise DeviceError(rc, "wke_set_heavy_workers failed")
 
     def set_fused_filters(self, on):
         """known_to_unknown + a filter on its new column in one call (default on)."""
