@@ -153,6 +153,8 @@ def main():
     ap.add_argument("--served-heavy-only", action="store_true",
                     help="only the served mix's heavy queries, each alone on the plain path, with the filters "
                          "decided per lineage group and per row")
+    ap.add_argument("--heavy-workers", type=int, default=0,
+                    help="served heavy queries at once (worker threads; 0: the engine's default)")
     ap.add_argument("--group-filter-rows", type=int, default=1 << 22,
                     help="with --served-heavy-only: smallest table a filter decides per lineage group")
     ap.add_argument("--heavy-only", action="store_true",
@@ -423,6 +425,8 @@ def run_config23(args, nested=False):
     dev = Device(0)
     dev.load(w.gstore)
     eng = GPUEngine(dev)
+    if args.heavy_workers:
+        eng.set_heavy_workers(args.heavy_workers)
     P = w.info["predicates"]
     type_ids = list(range(2 + P + 1, 2 + P + w.info["types"]))
 
@@ -722,6 +726,8 @@ def run_config4(args, nested=False):
     dev = Device(0)
     dev.load(w.gstore)
     eng = GPUEngine(dev)
+    if args.heavy_workers:
+        eng.set_heavy_workers(args.heavy_workers)
     P = w.info["predicates"]
     sq, flags = workloads.served_mix(P, w.info["light_types"], w.info["t0"], args.served_n * 10, args.seed + 41)
     heavy_idx = [i for i in range(len(sq)) if flags[i]]
