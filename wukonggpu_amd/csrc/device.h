@@ -264,13 +264,30 @@ __device__ __forceinline__ uint64_t block_excl_scan(uint64_t v, uint64_t *tmp, u
   return pre + inc - v;
 }
 
+// LDS written by some lanes of a wave and read by others: order the
+// accesses at wave scope.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A workgroup barrier that orders LDS only: global memory operations still
+// in flight are not waited for.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Exclusive block scans of N values at once (one pair of barriers): v[k]
-// becomes its exclusive prefix over the block, total[k] the block sum. `tmp`
-// is N * kBlock/kWave u64 of LDS.
-template <int N>
+// becomes its exclusive prefix over the block (BS threads), total[k] the
+// block sum. `tmp` is N * BS/kWave u64 of LDS. LDS_ONLY: the barriers order
+// LDS only.
+template <int N, int BS = kBlock, bool LDS_ONLY = false>
 __device__ __forceinline__ void block_excl_scan_n(uint64_t (&v)[N], uint64_t *tmp, uint64_t (&total)[N]) {
   const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
-  constexpr int W = kBlock / kWave;
+  constexpr int W = BS / kWave;
   uint64_t inc[N];
 #pragma unroll
   for (int k = 0; k < N; k++) inc[k] = wave_incl_scan(v[k]);
@@ -278,20 +295,26 @@ __device__ __forceinline__ void block_excl_scan_n(uint64_t (&v)[N], uint64_t *tm
 #pragma unroll
     for (int k = 0; k < N; k++) tmp[k * W + wave] = inc[k];
   }
-  __syncthreads();
+  if constexpr (LDS_ONLY) lds_barrier(); else __syncthreads();
 #pragma unroll
   for (int k = 0; k < N; k++) {
     uint64_t pre = 0, tot = 0;
+    if constexpr (W <= 4) {  // every thread sums the few wave totals
 #pragma unroll
-    for (int w = 0; w < W; w++) {
-      const uint64_t t = tmp[k * W + w];
-      if ((uint32_t)w < wave) pre += t;
-      tot += t;
+      for (int w = 0; w < W; w++) {
+        const uint64_t t = tmp[k * W + w];
+        if ((uint32_t)w < wave) pre += t;
+        tot += t;
+      }
+    } else {  // lane j reads wave j's total; a wave scan of them
+      const uint64_t ti = wave_incl_scan(lane < (uint32_t)W ? tmp[k * W + lane] : 0ull);
+      pre = wave ? shfl64(ti, wave - 1) : 0ull;
+      tot = shfl64(ti, W - 1);
     }
     v[k] = pre + inc[k] - v[k];
     total[k] = tot;
   }
-  __syncthreads();
+  if constexpr (LDS_ONLY) lds_barrier(); else __syncthreads();
 }
 
 }  // namespace wkg

@@ -38,6 +38,7 @@
 // The launch sequence for a given level count is captured once as a hipGraph
 // (no memset nodes: the plan kernels reset the counters) and replayed.
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <string.h>
 
 #include <algorithm>
@@ -99,8 +100,12 @@ struct ExecArgs {
   GraphView g;
   ExecState *st;
   wkg_batch_result *res;  // [Q], right after st (one read-back copy)
-  QState *qs;             // [Q]
-  Item *items;            // [Q]
+  // query states as columns of 64-bit words (word k of query q at
+  // [k * Q + q]), so that one thread per query reads or writes them with
+  // consecutive addresses across a wave
+  uint64_t *qsw;
+  Item *items;            // [Q] the current passes' parameters (read per tile)
+  uint64_t *qtot;         // [Q][4] each query's step totals (StepParams::totals points here)
   uint64_t *tile_base;    // [Q+1] probe / finalize tiles
   uint64_t *chunk_base;   // [Q+1] tile-scan chunks
   uint64_t *part_base;    // [Q+1]
@@ -115,6 +120,31 @@ struct ExecArgs {
   uint64_t hcap;
   uint32_t Q, P;
 };
+
+template <typename T>
+__device__ __forceinline__ T soa_get(const uint64_t *w, uint32_t Q, uint32_t q) {
+  static_assert(sizeof(T) % 8 == 0, "whole words");
+  uint64_t t[sizeof(T) / 8];
+#pragma unroll
+  for (uint32_t k = 0; k < sizeof(T) / 8; k++) t[k] = w[(uint64_t)k * Q + q];
+  T v;
+  __builtin_memcpy(&v, t, sizeof(T));
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ void soa_put(uint64_t *w, uint32_t Q, uint32_t q, const T &v) {
+  static_assert(sizeof(T) % 8 == 0, "whole words");
+  uint64_t t[sizeof(T) / 8];
+  __builtin_memcpy(t, &v, sizeof(T));
+#pragma unroll
+  for (uint32_t k = 0; k < sizeof(T) / 8; k++) w[(uint64_t)k * Q + q] = t[k];
+}
+// A 32-bit field of query q's QState (its byte offset in the struct).
+__device__ __forceinline__ uint32_t *qs_field(const ExecArgs &a, uint32_t q, uint32_t off) {
+  return (uint32_t *)((char *)(a.qsw + (uint64_t)(off / 8) * a.Q + q) + off % 8);
+}
+__device__ __forceinline__ QState qs_get(const ExecArgs &a, uint32_t q) { return soa_get<QState>(a.qsw, a.Q, q); }
+__device__ __forceinline__ void qs_put(const ExecArgs &a, uint32_t q, const QState &s) { soa_put(a.qsw, a.Q, q, s); }
 
 // Arena placement. Level l writes its output tables into stack (l & 1) --
 // stack 0 grows up from 0, stack 1 down from cap -- after dropping that
@@ -259,6 +289,9 @@ struct BlockQueue {
 // The static deal: block b takes tiles b, b + G, b + 2G, ... of a pass's
 // flat tile space (G blocks). No atomics, and blocks differ by at most one
 // tile, which suits passes whose tiles cost about the same (large steps).
+// (Equal contiguous runs instead of this interleave measured slower on the
+// 512-query mix's first hop: tiles of one query cost alike, so a run of one
+// expensive query's tiles lands on one block.)
 struct StaticDeal {
   uint32_t &sq;  // LDS slot of the query search
   uint64_t n, t, qlo, qhi;
@@ -288,44 +321,94 @@ struct LevelNeed {
 };
 
 template <int TR>
-__device__ __forceinline__ LevelNeed level_need(const ExecArgs &a, uint32_t q, uint32_t level) {
+__device__ __forceinline__ LevelNeed level_need(const QState &s, const wkg_plan_step &d, uint32_t level) {
   LevelNeed n{false, 0, 0, 0, 0, 0};
-  const QState &s = a.qs[q];
   if (!level_active(s, level)) return n;
-  const wkg_plan_step &d = a.up_steps[(uint64_t)q * a.P + level];
   const bool key = d.kind == WKG_STEP_I2U || d.kind == WKG_STEP_C2U;
   n.active = true;
   n.R = key ? 1 : s.rows;
   n.nt = (n.R + TR - 1) / TR;
   n.nc = n.nt > kScanChunk ? (n.nt + kScanChunk - 1) / kScanChunk : 1;  // an empty step still writes totals
-  // totals[4] | tile_sum | tile_nz | tile_ex | chunk sums x 3 | hits (TR per tile, 16 B) | K2U: cinfo, cexcl
-  n.scratch = al(32 + 24 * n.nt + 24 * n.nc) + n.nt * TR * 16 + (d.kind == WKG_STEP_K2U || key ? 2 * al(n.R * 8) : 0);
+  // tile_sum | tile_nz | tile_ex | chunk sums x 3 | hits (TR per tile, 16 B) | K2U: cinfo, cexcl
+  n.scratch = al(24 * n.nt + 24 * n.nc) + n.nt * TR * 16 + (d.kind == WKG_STEP_K2U || key ? 2 * al(n.R * 8) : 0);
   n.table = (d.kind == WKG_STEP_K2K || d.kind == WKG_STEP_K2C) ? al(n.R * s.cols * 4) : 0;
   return n;
 }
 
-// Queries per thread per chunk in the one-block plan kernels: a chunk of
-// kBlock * kPlanK queries has its needs in registers and scans them once;
-// chunks follow one another with the running prefixes carried over.
-constexpr uint32_t kPlanK = 4;
+// The one-block plan kernels run kPlanBlock threads, one query each per
+// chunk of kPlanBlock queries; a chunk's needs are scanned once, and chunks
+// follow one another with the running prefixes carried over.
+constexpr int kPlanBlock = 512;
+
+// A record read or written by its own thread in an array of structs puts
+// up to 64 cache lines in one wave instruction, and the plan's one CU takes
+// those lines one after another. So the query states are stored as columns
+// (ExecArgs::qsw), and the parameter records, which the tile kernels read
+// whole, pass through the wave's own LDS area: the wave's 64 records are
+// copied as consecutive words (a few lines per instruction).
+constexpr uint32_t kStageWords = sizeof(StepParams) / 8;  // the largest staged record
+struct PlanLds {
+  uint64_t w[kPlanBlock / kWave][kWave * kStageWords + kWave / 2];  // + the put flags
+};
+
+// Lanes' records of queries q0 + lane (those below n) into the field at `off`
+// of an array of Items; only lanes with `put` are stored.
+template <typename T>
+__device__ __forceinline__ void wave_put(Item *items, uint32_t off, uint32_t q0, uint32_t n, const T &v, bool put,
+                                         uint64_t *lw) {
+  static_assert(sizeof(T) % 8 == 0 && sizeof(T) / 8 <= kStageWords, "record");
+  constexpr uint32_t W = sizeof(T) / 8;
+  uint32_t *flag = (uint32_t *)(lw + kWave * kStageWords);
+  const uint32_t lane = lane_id();
+  uint64_t t[W];
+  __builtin_memcpy(t, &v, sizeof(T));
+#pragma unroll
+  for (uint32_t k = 0; k < W; k++) lw[lane * W + k] = t[k];
+  flag[lane] = put && q0 + lane < n ? 1u : 0u;
+  wave_lds_sync();
+#pragma unroll
+  for (uint32_t u = 0; u < W; u++) {
+    const uint32_t m = u * kWave + lane, i = m / W, k = m - i * W;
+    if (flag[i]) *(uint64_t *)((char *)&items[q0 + i] + off + 8 * k) = lw[m];
+  }
+  wave_lds_sync();  // the area is free again
+}
+
+// The field at `off` of items[q0 + lane] for lanes with q0 + lane < n (the
+// others get T{}).
+template <typename T>
+__device__ __forceinline__ T wave_get(const Item *items, uint32_t off, uint32_t q0, uint32_t n, uint64_t *lw) {
+  static_assert(sizeof(T) % 8 == 0 && sizeof(T) / 8 <= kStageWords, "record");
+  constexpr uint32_t W = sizeof(T) / 8;
+  const uint32_t lane = lane_id();
+  uint64_t v[W];
+#pragma unroll
+  for (uint32_t u = 0; u < W; u++) {  // every load is issued before the first is used
+    const uint32_t m = u * kWave + lane, i = m / W, k = m - i * W;
+    v[u] = q0 + i < n ? *(const uint64_t *)((const char *)&items[q0 + i] + off + 8 * k) : 0ull;
+  }
+#pragma unroll
+  for (uint32_t u = 0; u < W; u++) lw[u * kWave + lane] = v[u];
+  wave_lds_sync();
+  uint64_t t[W];
+#pragma unroll
+  for (uint32_t k = 0; k < W; k++) t[k] = lw[lane * W + k];
+  wave_lds_sync();
+  T r;
+  __builtin_memcpy(&r, t, sizeof(T));
+  return r;
+}
 
 // Starts a batch: query states from the upload, the arena stacks empty.
-// Uses the plan phases' query-to-thread mapping (kPlanK consecutive queries
-// per thread), so each thread later reads back only states it wrote itself.
 __device__ __forceinline__ uint32_t begin_phase(const ExecArgs &a) {
   const uint64_t nq = *a.up_nq;
-  for (uint32_t c0 = 0; c0 < a.Q; c0 += kBlock * kPlanK) {
-#pragma unroll
-    for (uint32_t i = 0; i < kPlanK; i++) {
-      const uint32_t q = c0 + threadIdx.x * kPlanK + i;
-      if (q < a.Q) {
-        QState s{};
-        s.T = nullptr;
-        s.status = q < nq ? WKG_OK : kInactive;
-        s.nsteps = q < nq ? a.up_nsteps[q] : 0;
-        a.qs[q] = s;
-      }
-    }
+  for (uint32_t c0 = 0; c0 < a.Q; c0 += kPlanBlock) {
+    const uint32_t q = c0 + threadIdx.x;
+    QState s{};
+    s.T = nullptr;
+    s.status = q < nq ? WKG_OK : kInactive;
+    s.nsteps = q < nq ? a.up_nsteps[q] : 0;
+    if (q < a.Q) qs_put(a, q, s);
   }
   if (threadIdx.x == 0) {  // (the level plan writes w0, w1, out_used, scr_used, nt and nc)
     ExecState &st = *a.st;
@@ -342,8 +425,9 @@ __device__ __forceinline__ uint32_t begin_phase(const ExecArgs &a) {
 // odd levels (stack 1 is refilled), [0, hi_low) for even ones.
 template <int TR>
 __device__ __forceinline__ void level_phase(const ExecArgs &a, uint32_t level, uint32_t nq, uint64_t lo_top,
-                                            uint64_t hi_low) {
-  __shared__ uint64_t s_tmp[2 * (kBlock / kWave)];
+                                            uint64_t hi_low, PlanLds &pl) {
+  __shared__ uint64_t s_tmp[2 * (kPlanBlock / kWave)];
+  uint64_t *lw = pl.w[threadIdx.x / kWave];
   __shared__ unsigned long long s_ut, s_us;
   ExecState *st = a.st;
   const uint64_t w0 = (level & 1) ? lo_top : 0, w1 = (level & 1) ? a.cap : hi_low;
@@ -352,110 +436,82 @@ __device__ __forceinline__ void level_phase(const ExecArgs &a, uint32_t level, u
     s_ut = 0;
     s_us = 0;
   }
+  const uint32_t lv = level < a.P ? level : a.P - 1;
   uint64_t carry_s = 0, carry_t = 0, carry_nt = 0, carry_nc = 0;
-  for (uint32_t c0 = 0; c0 < nq; c0 += kBlock * kPlanK) {
-    const uint32_t qb = c0 + threadIdx.x * kPlanK;  // this thread's queries qb .. qb + kPlanK - 1
-    LevelNeed nd[kPlanK];
-    uint64_t sz[2] = {0, 0}, szt[2];
-#pragma unroll
-    for (uint32_t i = 0; i < kPlanK; i++) {
-      nd[i] = qb + i < nq ? level_need<TR>(a, qb + i, level) : LevelNeed{false, 0, 0, 0, 0, 0};
-      sz[0] += nd[i].scratch;
-      sz[1] += nd[i].table;
-    }
-    block_excl_scan_n<2>(sz, s_tmp, szt);
-    const uint64_t os0 = carry_s + sz[0], ot0 = carry_t + sz[1];
+  for (uint32_t c0 = 0; c0 < nq; c0 += kPlanBlock) {
+    const uint32_t q = c0 + threadIdx.x;
+    const uint32_t qc = q < nq ? q : nq - 1;  // (nq >= 1 here)
+    QState sv = qs_get(a, qc);
+    const wkg_plan_step d = a.up_steps[(uint64_t)qc * a.P + lv];
+    const LevelNeed n = q < nq ? level_need<TR>(sv, d, level) : LevelNeed{false, 0, 0, 0, 0, 0};
+    uint64_t sz[2] = {n.scratch, n.table}, szt[2];
+    block_excl_scan_n<2, kPlanBlock, true>(sz, s_tmp, szt);
+    const uint64_t os = carry_s + sz[0], ot = carry_t + sz[1];
     // fitting queries (in query order) and their tiles and scan chunks
-    bool fit[kPlanK];
-    uint64_t cnt[2] = {0, 0}, cntt[2];
-    {
-      uint64_t os = os0, ot = ot0;
-#pragma unroll
-      for (uint32_t i = 0; i < kPlanK; i++) {
-        fit[i] = nd[i].active && ot + nd[i].table + os + nd[i].scratch <= room;
-        if (fit[i]) {
-          cnt[0] += nd[i].nt;
-          cnt[1] += nd[i].nc;
-        }
-        os += nd[i].scratch;
-        ot += nd[i].table;
-      }
+    const bool fit = n.active && ot + n.table + os + n.scratch <= room;
+    uint64_t cnt[2] = {fit ? n.nt : 0, fit ? n.nc : 0}, cntt[2];
+    block_excl_scan_n<2, kPlanBlock, true>(cnt, s_tmp, cntt);
+    const uint64_t tbase = carry_nt + cnt[0], cbase = carry_nc + cnt[1];
+    if (q < nq) {
+      a.tile_base[q] = tbase;
+      a.chunk_base[q] = cbase;
     }
-    block_excl_scan_n<2>(cnt, s_tmp, cntt);
-    uint64_t tbase = carry_nt + cnt[0], cbase = carry_nc + cnt[1];
-    uint64_t os = os0, ot = ot0, used_t = 0, used_s = 0;
-#pragma unroll
-    for (uint32_t i = 0; i < kPlanK; i++) {
-      const uint32_t q = qb + i;
-      const LevelNeed &n = nd[i];
-      if (q < nq) {
-        a.tile_base[q] = tbase;
-        a.chunk_base[q] = cbase;
-      }
-      if (n.active) {
-        QState &s = a.qs[q];
-        if (!fit[i]) {
-          s.status = WKG_ENOMEM;
+    StepParams p{};
+    if (n.active) {
+      if (!fit) {
+        sv.status = WKG_ENOMEM;
+      } else {
+        p.pid = (uint32_t)d.pid;
+        p.dir = (uint32_t)d.dir;
+        p.R = n.R;
+        p.ntiles = n.nt;
+        if (d.kind == WKG_STEP_I2U || d.kind == WKG_STEP_C2U) {
+          p.key = d.kind == WKG_STEP_I2U ? make_key(0, (uint64_t)d.pid, (uint64_t)d.dir)
+                                         : make_key((uint64_t)d.vid, (uint64_t)d.pid, (uint64_t)d.dir);
+          p.mode = kModeK2U;
         } else {
-          const wkg_plan_step &d = a.up_steps[(uint64_t)q * a.P + level];
-          StepParams p{};
-          p.pid = (uint32_t)d.pid;
-          p.dir = (uint32_t)d.dir;
-          p.R = n.R;
-          p.ntiles = n.nt;
-          if (d.kind == WKG_STEP_I2U || d.kind == WKG_STEP_C2U) {
-            p.key = d.kind == WKG_STEP_I2U ? make_key(0, (uint64_t)d.pid, (uint64_t)d.dir)
-                                           : make_key((uint64_t)d.vid, (uint64_t)d.pid, (uint64_t)d.dir);
-            p.mode = kModeK2U;
-          } else {
-            p.T = s.T;
-            p.c = s.cols;
-            p.sc = (uint32_t)d.start_col;
-            p.mode = d.kind == WKG_STEP_K2U ? kModeK2U : d.kind == WKG_STEP_K2K ? kModeK2K : kModeK2C;
-            p.ec = d.kind == WKG_STEP_K2K ? (uint32_t)d.end_col : 0u;
-            p.target = (uint32_t)d.vid;
-            p.target_ok = (d.kind == WKG_STEP_K2C && d.vid >= 0 && d.vid <= 0xffffffffLL) ? 1u : 0u;
-          }
-          char *sb = scr_at(a, level, w0, w1, os, n.scratch);
-          p.totals = (uint64_t *)sb;
-          p.tile_sum = p.totals + 4;
-          p.tile_nz = p.tile_sum + n.nt;
-          p.tile_ex = p.tile_nz + n.nt;
-          p.chunk_sum = p.tile_ex + n.nt;
-          p.chunk_nz = p.chunk_sum + n.nc;
-          p.chunk_ex = p.chunk_nz + n.nc;
-          sb += al(32 + 24 * n.nt + 24 * n.nc);
-          p.hits = (uint4 *)sb;
-          sb += n.nt * TR * 16;
-          if (p.mode == kModeK2U) {
-            p.cinfo = (uint2 *)sb;
-            p.cexcl = (uint64_t *)(sb + al(n.R * 8));
-          } else {
-            p.O = (int32_t *)out_at(a, level, w0, w1, ot, n.table);
-          }
-          a.items[q].p = p;
-          s.nchunks = (uint32_t)n.nc;
-          s.chunks_done = 0;
-          tbase += n.nt;
-          cbase += n.nc;
-          used_t = ot + n.table;
-          used_s = os + n.scratch;
+          p.T = sv.T;
+          p.c = sv.cols;
+          p.sc = (uint32_t)d.start_col;
+          p.mode = d.kind == WKG_STEP_K2U ? kModeK2U : d.kind == WKG_STEP_K2K ? kModeK2K : kModeK2C;
+          p.ec = d.kind == WKG_STEP_K2K ? (uint32_t)d.end_col : 0u;
+          p.target = (uint32_t)d.vid;
+          p.target_ok = (d.kind == WKG_STEP_K2C && d.vid >= 0 && d.vid <= 0xffffffffLL) ? 1u : 0u;
         }
+        char *sb = scr_at(a, level, w0, w1, os, n.scratch);
+        p.totals = a.qtot + 4 * (uint64_t)q;
+        p.tile_sum = (uint64_t *)sb;
+        p.tile_nz = p.tile_sum + n.nt;
+        p.tile_ex = p.tile_nz + n.nt;
+        p.chunk_sum = p.tile_ex + n.nt;
+        p.chunk_nz = p.chunk_sum + n.nc;
+        p.chunk_ex = p.chunk_nz + n.nc;
+        sb += al(24 * n.nt + 24 * n.nc);
+        p.hits = (uint4 *)sb;
+        sb += n.nt * TR * 16;
+        if (p.mode == kModeK2U) {
+          p.cinfo = (uint2 *)sb;
+          p.cexcl = (uint64_t *)(sb + al(n.R * 8));
+        } else {
+          p.O = (int32_t *)out_at(a, level, w0, w1, ot, n.table);
+        }
+        sv.nchunks = (uint32_t)n.nc;
+        sv.chunks_done = 0;
+        // the fitting queries form a prefix (their cumulative needs are the
+        // smallest), so the highest ends in use are the last fitting query's
+        atomicMax(&s_ut, (unsigned long long)(ot + n.table));
+        atomicMax(&s_us, (unsigned long long)(os + n.scratch));
       }
-      os += n.scratch;
-      ot += n.table;
     }
-    // the fitting queries form a prefix (their cumulative needs are the
-    // smallest), so the highest ends in use are the last fitting query's
-    if (used_t) atomicMax(&s_ut, (unsigned long long)used_t);
-    if (used_s) atomicMax(&s_us, (unsigned long long)used_s);
+    wave_put(a.items, offsetof(Item, p), q - lane_id(), nq, p, n.active && fit, lw);
+    if (n.active) qs_put(a, q, sv);
     carry_s += szt[0];
     carry_t += szt[1];
     carry_nt += cntt[0];
     carry_nc += cntt[1];
   }
-  __syncthreads();
-  if (threadIdx.x == kBlock - 1) {
+  lds_barrier();
+  if (threadIdx.x == kPlanBlock - 1) {
     a.tile_base[nq] = carry_nt;
     a.chunk_base[nq] = carry_nc;
     st->w0 = w0;
@@ -470,8 +526,10 @@ __device__ __forceinline__ void level_phase(const ExecArgs &a, uint32_t level, u
 
 // ---- probe, tile scan, finalize ----------------------------------------------
 
+// Four waves per SIMD, like the per-step probe: the tile loop's state costs
+// a few spills, all outside the probe's round loop.
 template <int TR, bool SORTED, bool FRAMED, bool QUEUE>
-__global__ __launch_bounds__(kBlock, 3) void k_x_probe(ExecArgs a) {
+__global__ __launch_bounds__(kBlock, 4) void k_x_probe(ExecArgs a) {
   const uint32_t nq = (uint32_t)a.st->nq;
   uint64_t t, lt;
   uint32_t q;
@@ -493,8 +551,9 @@ __global__ __launch_bounds__(kBlock) void k_x_scan_chunks(ExecArgs a) {
   for (uint64_t u = blockIdx.x; u < NC; u += gridDim.x) {
     __syncthreads();
     const uint32_t q = block_find(a.chunk_base, nq, u, &s_q);
-    const uint32_t nch = a.qs[q].nchunks;
-    scan_chunk(a.items[q].p, u - a.chunk_base[q], nch);
+    const uint32_t nch = *qs_field(a, q, offsetof(QState, nchunks));
+    const StepParams &p = a.items[q].p;
+    scan_chunk(p, u - a.chunk_base[q], nch);
     if (nch > 1) {
       // thread 0 wrote this chunk's sums; the block that counts the query's
       // last chunk scans all of its chunk sums (release before the count,
@@ -502,12 +561,12 @@ __global__ __launch_bounds__(kBlock) void k_x_scan_chunks(ExecArgs a) {
       __syncthreads();
       if (threadIdx.x == 0) {
         __threadfence();
-        s_last = atomicAdd(&a.qs[q].chunks_done, 1u) + 1 == nch;
+        s_last = atomicAdd(qs_field(a, q, offsetof(QState, chunks_done)), 1u) + 1 == nch;
       }
       __syncthreads();
       if (s_last) {
         __threadfence();
-        scan_top(a.items[q].p, nch);
+        scan_top(p, nch);
       }
     }
   }
@@ -540,22 +599,21 @@ struct ExpandNeed {
 };
 
 template <int NV>
-__device__ __forceinline__ ExpandNeed expand_need(const ExecArgs &a, uint32_t q, uint32_t level) {
+__device__ __forceinline__ ExpandNeed expand_need(const QState &s, const StepParams &p, const uint64_t (&tot)[4],
+                                                  uint32_t level) {
   ExpandNeed n{false, false, false, false, 0, 0, 0, 0, 0};
-  const QState &s = a.qs[q];
   if (!level_active(s, level)) return n;
-  const StepParams &p = a.items[q].p;
   n.active = true;
   n.fin = level + 1 == s.nsteps;
   n.k2u = p.mode == kModeK2U;
-  n.rows = p.totals[0];
+  n.rows = tot[0];
   if (n.k2u) {
     n.cols = p.c + 1;
     if (n.rows >= kRowLimit) {
       n.bad = true;
       return n;
     }
-    const uint64_t nz = p.totals[2];
+    const uint64_t nz = tot[2];
     n.table = al(n.rows * n.cols * 4);
     n.nx = n.rows ? (nz + n.rows + NV - 1) / NV : 0;
     n.scratch = n.nx ? al((n.nx + 1) * 8) : 0;
@@ -568,133 +626,85 @@ __device__ __forceinline__ ExpandNeed expand_need(const ExecArgs &a, uint32_t q,
 // Commits level `level`'s results; gives the stack tops after it.
 template <int NV>
 __device__ __forceinline__ void expand_phase(const ExecArgs &a, uint32_t level, uint32_t nq, uint64_t &lo_top,
-                                             uint64_t &hi_low) {
-  __shared__ uint64_t s_tmp[3 * (kBlock / kWave)];
+                                             uint64_t &hi_low, PlanLds &pl) {
+  __shared__ uint64_t s_tmp[3 * (kPlanBlock / kWave)];
+  uint64_t *lw = pl.w[threadIdx.x / kWave];
   __shared__ unsigned long long s_ut, s_us, s_rb, s_over, s_lo, s_hi;
   ExecState *st = a.st;
   const uint64_t w0 = st->w0, w1 = st->w1, ou = st->out_used, su = st->scr_used;
   const uint64_t room = (w1 - w0) - ou - su;
   const uint64_t rb0 = st->res_bytes;
+  const uint64_t lo_prev = st->lo_top, hi_prev = st->hi_low, peak_prev = st->peak, over_prev = st->res_over;
   if (threadIdx.x == 0) {
     s_ut = s_us = s_rb = s_over = 0;
   }
   uint64_t carry_t = 0, carry_s = 0, carry_p = 0, carry_x = 0, carry_r = 0, carry_u = 0;
-  for (uint32_t c0 = 0; c0 < nq; c0 += kBlock * kPlanK) {
-    const uint32_t qb = c0 + threadIdx.x * kPlanK;
-    ExpandNeed nd[kPlanK];
-    uint64_t sz[2] = {0, 0}, szt[2];
-#pragma unroll
-    for (uint32_t i = 0; i < kPlanK; i++) {
-      nd[i] = qb + i < nq ? expand_need<NV>(a, qb + i, level) : ExpandNeed{false, false, false, false, 0, 0, 0, 0, 0};
-      sz[0] += nd[i].table;
-      sz[1] += nd[i].scratch;
-    }
-    block_excl_scan_n<2>(sz, s_tmp, szt);
-    const uint64_t ot0 = carry_t + sz[0], os0 = carry_s + sz[1];
+  for (uint32_t c0 = 0; c0 < nq; c0 += kPlanBlock) {
+    const uint32_t q = c0 + threadIdx.x;
+    const uint32_t qc = q < nq ? q : nq - 1;
+    QState sv = qs_get(a, qc);
+    const StepParams p = wave_get<StepParams>(a.items, offsetof(Item, p), q - lane_id(), nq, lw);
+    const uint64_t tot[4] = {a.qtot[4 * (uint64_t)qc], a.qtot[4 * (uint64_t)qc + 1], a.qtot[4 * (uint64_t)qc + 2], 0};
+    const ExpandNeed n = q < nq ? expand_need<NV>(sv, p, tot, level) : ExpandNeed{false, false, false, false, 0, 0, 0, 0, 0};
+    uint64_t sz[2] = {n.table, n.scratch}, szt[2];
+    block_excl_scan_n<2, kPlanBlock, true>(sz, s_tmp, szt);
+    const uint64_t ot = carry_t + sz[0], os = carry_s + sz[1];
     // arena fit; partition entries, expand tiles, result bytes of finishing queries
-    bool fit[kPlanK];
-    uint64_t cnt[3] = {0, 0, 0}, cntt[3];
-    {
-      uint64_t ot = ot0, os = os0;
-#pragma unroll
-      for (uint32_t i = 0; i < kPlanK; i++) {
-        const ExpandNeed &n = nd[i];
-        fit[i] = n.active && !n.bad && ot + n.table + os + n.scratch <= room;
-        if (fit[i]) {
-          if (n.nx) {
-            cnt[0] += n.nx + 1;
-            cnt[1] += n.nx;
-          }
-          if (n.fin) cnt[2] += al(n.rows * n.cols * 4);
-        }
-        ot += n.table;
-        os += n.scratch;
-      }
-    }
-    block_excl_scan_n<3>(cnt, s_tmp, cntt);
-    const uint64_t orb0 = carry_r + cnt[2];
+    const bool fit = n.active && !n.bad && ot + n.table + os + n.scratch <= room;
+    const uint64_t rb = fit && n.fin ? al(n.rows * n.cols * 4) : 0;
+    uint64_t cnt[3] = {fit && n.nx ? n.nx + 1 : 0, fit ? n.nx : 0, rb}, cntt[3];
+    block_excl_scan_n<3, kPlanBlock, true>(cnt, s_tmp, cntt);
+    const uint64_t orb = carry_r + cnt[2];
     // host-buffer fit (again a prefix) and copy units
-    uint64_t nu[1] = {0}, nut[1];
-    {
-      uint64_t orb = orb0;
-#pragma unroll
-      for (uint32_t i = 0; i < kPlanK; i++) {
-        const ExpandNeed &n = nd[i];
-        if (fit[i] && n.fin) {
-          const uint64_t b = al(n.rows * n.cols * 4);
-          if (rb0 + orb + b <= a.hcap) nu[0] += (b + kCopyUnit - 1) / kCopyUnit;
-          orb += b;
-        }
-      }
+    const bool host_fit = rb0 + orb + rb <= a.hcap;
+    uint64_t nu[1] = {fit && n.fin && host_fit ? (rb + kCopyUnit - 1) / kCopyUnit : 0}, nut[1];
+    block_excl_scan_n<1, kPlanBlock, true>(nu, s_tmp, nut);
+    const uint64_t pbase = carry_p + cnt[0], xbase = carry_x + cnt[1], ubase = carry_u + nu[0];
+    if (q < nq) {
+      a.part_base[q] = pbase;
+      a.xtile_base[q] = xbase;
+      a.unit_base[q] = ubase;
     }
-    block_excl_scan_n<1>(nu, s_tmp, nut);
-    uint64_t pbase = carry_p + cnt[0], xbase = carry_x + cnt[1], ubase = carry_u + nu[0];
-    uint64_t ot = ot0, os = os0, orb = orb0;
-    uint64_t used_t = 0, used_s = 0, used_r = 0, over = 0;
-#pragma unroll
-    for (uint32_t i = 0; i < kPlanK; i++) {
-      const uint32_t q = qb + i;
-      const ExpandNeed &n = nd[i];
-      if (q < nq) {
-        a.part_base[q] = pbase;
-        a.xtile_base[q] = xbase;
-        a.unit_base[q] = ubase;
-      }
-      if (n.active) {
-        QState &s = a.qs[q];
-        const StepParams &p = a.items[q].p;
-        if (n.bad) {
-          s.status = WKG_ERANGE;
-        } else if (!fit[i]) {
-          s.status = WKG_ENOMEM;
-        } else {
-          if (n.k2u) {
-            ExpandParams x{};
-            x.pid = p.pid;
-            x.T = p.T;
-            x.c = p.c;
-            x.R = p.totals[2];
-            x.cinfo = p.cinfo;
-            x.cexcl = p.cexcl;
-            x.part = (const uint64_t *)scr_at(a, level, w0, w1, su + os, n.scratch);
-            x.E = n.rows;
-            x.O = (int32_t *)out_at(a, level, w0, w1, ou + ot, n.table);
-            a.items[q].x = x;
-            s.T = x.O;
-            s.expanded += x.E;
-            if (n.nx) {
-              pbase += n.nx + 1;
-              xbase += n.nx;
-            }
-            used_t = ot + n.table;
-            used_s = os + n.scratch;
-          } else {  // filters: the kept rows were compacted into p.O
-            s.T = p.O;
-            s.expanded += p.totals[1];
-          }
-          s.rows = n.rows;
-          s.cols = (uint32_t)n.cols;
-          if (n.fin) {
-            const uint64_t b = al(n.rows * n.cols * 4);
-            if (rb0 + orb + b <= a.hcap) {
-              s.out_off = (rb0 + orb) / 4;
-              ubase += (b + kCopyUnit - 1) / kCopyUnit;
-              used_r = orb + b;
-            } else {
-              s.status = WKG_ENOMEM;
-              over += b;
-            }
-            orb += b;
+    ExpandParams x{};
+    if (n.active) {
+      if (n.bad) {
+        sv.status = WKG_ERANGE;
+      } else if (!fit) {
+        sv.status = WKG_ENOMEM;
+      } else {
+        if (n.k2u) {
+          x.pid = p.pid;
+          x.T = p.T;
+          x.c = p.c;
+          x.R = tot[2];
+          x.cinfo = p.cinfo;
+          x.cexcl = p.cexcl;
+          x.part = (const uint64_t *)scr_at(a, level, w0, w1, su + os, n.scratch);
+          x.E = n.rows;
+          x.O = (int32_t *)out_at(a, level, w0, w1, ou + ot, n.table);
+          sv.T = x.O;
+          sv.expanded += x.E;
+          atomicMax(&s_ut, (unsigned long long)(ot + n.table));
+          atomicMax(&s_us, (unsigned long long)(os + n.scratch));
+        } else {  // filters: the kept rows were compacted into p.O
+          sv.T = p.O;
+          sv.expanded += tot[1];
+        }
+        sv.rows = n.rows;
+        sv.cols = (uint32_t)n.cols;
+        if (n.fin) {
+          if (host_fit) {
+            sv.out_off = (rb0 + orb) / 4;
+            atomicMax(&s_rb, (unsigned long long)(orb + rb));
+          } else {
+            sv.status = WKG_ENOMEM;
+            atomicAdd(&s_over, (unsigned long long)rb);
           }
         }
       }
-      ot += n.table;
-      os += n.scratch;
     }
-    if (used_t) atomicMax(&s_ut, (unsigned long long)used_t);
-    if (used_s) atomicMax(&s_us, (unsigned long long)used_s);
-    if (used_r) atomicMax(&s_rb, (unsigned long long)used_r);
-    if (over) atomicAdd(&s_over, (unsigned long long)over);
+    wave_put(a.items, offsetof(Item, x), q - lane_id(), nq, x, n.active && fit && n.k2u, lw);
+    if (n.active) qs_put(a, q, sv);
     carry_t += szt[0];
     carry_s += szt[1];
     carry_p += cntt[0];
@@ -702,8 +712,8 @@ __device__ __forceinline__ void expand_phase(const ExecArgs &a, uint32_t level, 
     carry_r += cntt[2];
     carry_u += nut[0];
   }
-  __syncthreads();
-  if (threadIdx.x == kBlock - 1) {
+  lds_barrier();
+  if (threadIdx.x == kPlanBlock - 1) {
     a.part_base[nq] = carry_p;
     a.xtile_base[nq] = carry_x;
     a.unit_base[nq] = carry_u;
@@ -711,18 +721,18 @@ __device__ __forceinline__ void expand_phase(const ExecArgs &a, uint32_t level, 
     st->nx = carry_x;
     st->nu = carry_u;
     const uint64_t out = ou + s_ut, scr = su + s_us;
-    uint64_t lo = st->lo_top, hi = st->hi_low;
+    uint64_t lo = lo_prev, hi = hi_prev;
     if (level & 1) hi = w1 - out;
     else lo = w0 + out;
     st->lo_top = s_lo = lo;
     st->hi_low = s_hi = hi;
     const uint64_t in_use = a.cap - (w1 - w0) + out + scr;
-    if (in_use > st->peak) st->peak = in_use;
+    st->peak = in_use > peak_prev ? in_use : peak_prev;
     st->res_bytes = rb0 + s_rb;
-    st->res_over += s_over;
+    st->res_over = over_prev + s_over;
   }
   if (threadIdx.x < kSubQueues) st->queue[1][threadIdx.x][0] = 0;
-  __syncthreads();
+  lds_barrier();
   lo_top = s_lo;
   hi_low = s_hi;
 }
@@ -731,16 +741,18 @@ __device__ __forceinline__ void expand_phase(const ExecArgs &a, uint32_t level, 
 // for level -1, which starts the batch instead), then the plan of level+1
 // when it exists.
 template <int TR, int NV>
-__global__ __launch_bounds__(kBlock) void k_x_plan(ExecArgs a, int level, int levels) {
+__global__ __launch_bounds__(kPlanBlock) void k_x_plan(ExecArgs a, int level, int levels) {
+  __shared__ PlanLds s_pl;
   uint32_t nq;
   uint64_t lo_top = 0, hi_low = a.cap;
   if (level < 0) {
     nq = begin_phase(a);
   } else {
     nq = (uint32_t)a.st->nq;
-    expand_phase<NV>(a, (uint32_t)level, nq, lo_top, hi_low);
+    expand_phase<NV>(a, (uint32_t)level, nq, lo_top, hi_low, s_pl);
   }
-  if (level + 1 < levels) level_phase<TR>(a, (uint32_t)(level + 1), nq, lo_top, hi_low);
+  // (each thread reads back only the states it stored itself)
+  if (level + 1 < levels) level_phase<TR>(a, (uint32_t)(level + 1), nq, lo_top, hi_low, s_pl);
 }
 
 template <int NV>
@@ -781,7 +793,7 @@ __global__ __launch_bounds__(kBlock) void k_x_retire(ExecArgs a) {
   for (uint64_t u = blockIdx.x; u < NU; u += gridDim.x) {
     __syncthreads();
     const uint32_t q = block_find(a.unit_base, nq, u, &s_q);
-    const QState &s = a.qs[q];
+    const QState s = qs_get(a, q);
     const uint64_t bytes = s.rows * s.cols * 4;
     const uint64_t b0 = (u - a.unit_base[q]) * kCopyUnit;
     if (b0 >= bytes) continue;  // (units are sized from the same bytes)
@@ -796,7 +808,7 @@ __global__ __launch_bounds__(kBlock) void k_x_retire(ExecArgs a) {
 __global__ __launch_bounds__(kBlock) void k_x_records(ExecArgs a) {
   const uint32_t nq = (uint32_t)a.st->nq;
   for (uint32_t q = threadIdx.x; q < nq; q += kBlock) {
-    const QState &s = a.qs[q];
+    const QState s = qs_get(a, q);
     wkg_batch_result r{};
     r.status = s.status;
     r.expanded = s.expanded;
@@ -866,12 +878,12 @@ template <int TR, int VT, bool SORTED, bool FRAMED, bool QUEUE>
 void enqueue_levels(const ExecArgs &a, hipStream_t s, int levels, unsigned gp, unsigned gx, unsigned ga,
                     unsigned gf) {
   constexpr int NV = kBlock * VT;
-  k_x_plan<TR, NV><<<1, kBlock, 0, s>>>(a, -1, levels);
+  k_x_plan<TR, NV><<<1, kPlanBlock, 0, s>>>(a, -1, levels);
   for (int l = 0; l < levels; l++) {
     k_x_probe<TR, SORTED, FRAMED, QUEUE><<<gp, kBlock, 0, s>>>(a);
     k_x_scan_chunks<<<ga, kBlock, 0, s>>>(a);
     k_x_finalize<TR><<<gf, kBlock, 0, s>>>(a);
-    k_x_plan<TR, NV><<<1, kBlock, 0, s>>>(a, l, levels);
+    k_x_plan<TR, NV><<<1, kPlanBlock, 0, s>>>(a, l, levels);
     k_x_partition<NV><<<ga, kBlock, 0, s>>>(a);
     k_x_expand<VT, FRAMED, QUEUE><<<gx, kBlock, 0, s>>>(a);
     k_x_retire<<<ga, kBlock, 0, s>>>(a);
@@ -997,11 +1009,12 @@ int wkg_batch_create(wkg_ctx *ctx, int max_queries, int max_patterns, uint64_t a
   if (e == hipSuccess) e = hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
   b->ncu = dev_cu > 0 ? dev_cu : 256;
   const uint64_t Q = (uint64_t)max_queries, P = (uint64_t)max_patterns;
-  // control block: ExecState | results[Q] | qs[Q] | items[Q] | 5 x base[Q+1] | upload
+  // control block: ExecState | results[Q] | query states | items[Q] | totals[Q] | 5 x base[Q+1] | upload
   const uint64_t o_res = al(sizeof(ExecState));
   const uint64_t o_qs = al(o_res + Q * sizeof(wkg_batch_result));
   const uint64_t o_items = al(o_qs + Q * sizeof(QState));
-  const uint64_t o_base = al(o_items + Q * sizeof(Item));
+  const uint64_t o_tot = al(o_items + Q * sizeof(Item));
+  const uint64_t o_base = al(o_tot + Q * 32);
   const uint64_t base_bytes = al((Q + 1) * 8);
   const uint64_t o_up = o_base + 5 * base_bytes;
   b->up_bytes = al(8 + Q * 4) + Q * P * sizeof(wkg_plan_step);
@@ -1028,8 +1041,9 @@ int wkg_batch_create(wkg_ctx *ctx, int max_queries, int max_patterns, uint64_t a
   a.g = ctx->view();
   a.st = (ExecState *)b->dctl;
   a.res = (wkg_batch_result *)(b->dctl + o_res);
-  a.qs = (QState *)(b->dctl + o_qs);
+  a.qsw = (uint64_t *)(b->dctl + o_qs);
   a.items = (Item *)(b->dctl + o_items);
+  a.qtot = (uint64_t *)(b->dctl + o_tot);
   a.tile_base = (uint64_t *)(b->dctl + o_base);
   a.chunk_base = (uint64_t *)(b->dctl + o_base + base_bytes);
   a.part_base = (uint64_t *)(b->dctl + o_base + 2 * base_bytes);
@@ -1216,12 +1230,12 @@ int wkg_batch_autotune(wkg_batch *b, int nq, const wkg_plan_step *steps, const i
     for (int xi : {512, 1024, 2048})
       for (int aw : {1, 2}) {
         // work queue: more workgroups than fit at once are fine (late ones
-        // find the queue drained); static deal: at most what fits (three
+        // find the queue drained); static deal: at most what fits (four
         // probe workgroups per CU), or the late ones would run their share
         // after the others are done
         for (int pw : {1, 2, 4})
           for (int xw : {2, 4, 8}) cand.push_back(wkg_batch_tuning{pw, xw, aw, tr, xi, 0});
-        for (int pw : {2, 3})
+        for (int pw : {2, 3, 4})
           for (int xw : {4, 8}) cand.push_back(wkg_batch_tuning{pw, xw, aw, tr, xi, 1});
       }
   double best_t = 1e30;

@@ -61,14 +61,6 @@ struct SplitArgs {
   const uint64_t *base;  // exclusive scan of cnt, same layout
 };
 
-// LDS written by some lanes of a wave and read by others: order the
-// accesses at wave scope.
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // floor(x / d) with m = fastmod_magic(d): the estimate is at most two low.
 __device__ __forceinline__ uint32_t fastdiv32(uint32_t x, uint32_t d, uint64_t m) {
   uint64_t q = __umul64hi((uint64_t)x, m);

@@ -238,7 +238,7 @@ __device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams 
   }
 }
 
-// At most 168 VGPRs, so three waves per SIMD hide the probe's load chains.
+// At most 128 VGPRs, so four waves per SIMD hide the probe's load chains.
 template <int TR, bool SORTED, bool FRAMED>
 __global__ __launch_bounds__(kBlock, 4) void k_probe(GraphView g, StepParams p) {
   probe_tile<TR, SORTED, FRAMED>(g, p, p.tile0 + blockIdx.x);
