@@ -165,6 +165,10 @@ def main():
                     help="run only the 512-query mix through the executor, 5 batches (for profiler passes)")
     ap.add_argument("--cycle-only", action="store_true",
                     help="run only the config-5 cyclic query, timed and with HIP events (for profiler passes)")
+    ap.add_argument("--framed-only", action="store_true",
+                    help="run only the config-2/3 graph under a graph budget (framed residency)")
+    ap.add_argument("--framed-budget", type=int, default=1 << 30,
+                    help="graph budget of the framed-residency measurement, bytes (0: skip it)")
     ap.add_argument("--shape", default="",
                     help="with --mix-only: the executor shape as key=value pairs (probe_wg_per_cu=3,...) "
                          "instead of autotuning")
@@ -176,7 +180,7 @@ def main():
     if args.served_heavy_only:
         args.batch_only = True
     profiler_pass = (args.heavy_only or args.probe_only or args.mix_only or args.batch_only or args.exchange_only
-                     or args.cycle_only)
+                     or args.cycle_only or args.framed_only)
     if profiler_pass:
         args.no_cpu = True
     if args.exchange_only:
@@ -416,7 +420,7 @@ def run_config23(args, nested=False):
     w = workloads.config2_3(seed=args.seed)
     log(f"config 2/3 graph: {w.info['triples']} triples, {w.info['keys']} keys, image "
         f"{w.info['image_bytes'] / 1e9:.2f} GB built in {time.time() - t0:.1f}s")
-    profiler_pass = args.heavy_only or args.probe_only or args.mix_only or args.batch_only
+    profiler_pass = args.heavy_only or args.probe_only or args.mix_only or args.batch_only or args.framed_only
     device_build = None
     if not profiler_pass and not nested:
         device_build = compare_device_build(w, Device)
@@ -459,6 +463,12 @@ def run_config23(args, nested=False):
     if args.batch_only:
         eng.configure(max_queries=1024, max_patterns=8, arena_bytes=16 << 30)
         return done({"batch_mode": batch_mode(eng, w.gstore, P, type_ids, w.info["t0"], args)})
+    if args.framed_only:
+        heavy, mix = w.queries["heavy"], w.queries["batched"]
+        ref_heavy = eng.run_query(heavy).table
+        ref_mix = eng.run_plain(mix)
+        return done({"framed": framed_mode(Device, GPUEngine, w.gstore, heavy, mix, ref_heavy, ref_mix,
+                                           args.framed_budget or (1 << 30))})
     if args.mix_only:
         mix = w.queries["batched"]
         eng.configure(max_queries=1024, max_patterns=8, arena_bytes=32 << 30)
@@ -592,6 +602,10 @@ def run_config23(args, nested=False):
 
     # ---- fork-join table exchange: the split a sharded deployment adds per step ---------------
     exchange = table_exchange(dev, args.seed)
+    # ---- framed residency: the same graph under a graph budget below its image -----------------
+    framed = None
+    if args.framed_budget and not nested:
+        framed = framed_mode(Device, GPUEngine, w.gstore, heavy, mix, r.table, plain, args.framed_budget)
     # the 151M-row hub fan-out table (the heavy query's first four patterns), for the oracle below
     rp = eng.run_query(heavy[:16]) if not args.no_cpu else None
     eng.close()
@@ -649,7 +663,7 @@ def run_config23(args, nested=False):
             "plain_p99_ms": pct([x.latency_s for x in plain], 0.99) * 1e3,
             "batched_ms_per_batch": t_batched * 1e3, "expanded_edges": int(mix_edges)},
         "heavy_via_executor_ms": pct(hx_lat, 0.5) * 1e3, "heavy_tuning": heavy_tuning,
-        "served": served, "batch_mode": bm, "table_exchange": exchange, "parity": parity})
+        "served": served, "batch_mode": bm, "table_exchange": exchange, "framed": framed, "parity": parity})
 
 
 def served_sweep(eng, dev, sq, sflags, alone, rates, seed, budget, span_s=0.0):
@@ -1044,6 +1058,55 @@ def random_key_probe(dev, w, seed, line_peak):
             "segment_bytes": seg_bytes, "random_lines_same_working_set": seg_lines,
             "frac_of_random_lines_same_working_set": rand_rate / seg_lines,
             "workload": "known_to_unknown over 64M uniformly random vertex ids, predicate 2 OUT"}
+
+
+def framed_mode(Device, GPUEngine, gstore, heavy, mix, heavy_table, mix_ref, budget):
+    """The config-2/3 graph under a graph budget below its image: predicate
+    segments in frames loaded on demand, least recently used evicted
+    (core/shard_manager.hpp), with the pipelined load of a later pattern's
+    predicate (core/rcache.hpp:273-281) off and on. Per setting, on a fresh
+    context: the heavy query cold (its segments loaded) and warm, then the
+    512-query mix through the plain path; every answer is compared with the
+    resident graph's."""
+    import numpy as np
+    out = {"budget_bytes": budget, "image_bytes": gstore.image_nbytes()}
+    for on in (False, True):
+        dev = Device(0)
+        t0 = time.perf_counter()
+        dev.load(gstore, graph_budget_bytes=budget)
+        t_load = time.perf_counter() - t0
+        eng = GPUEngine(dev)
+        eng.set_pipeline(on)
+        q0 = time.perf_counter()
+        r1 = eng.run_query(heavy)
+        cold = time.perf_counter() - q0
+        st1 = dev.stats()
+        q0 = time.perf_counter()
+        r2 = eng.run_query(heavy)
+        warm = time.perf_counter() - q0
+        st2 = dev.stats()
+        q0 = time.perf_counter()
+        res = eng.run_plain(mix)
+        t_mix = time.perf_counter() - q0
+        st3 = dev.stats()
+        same = bool(np.array_equal(r1.table, heavy_table) and np.array_equal(r2.table, heavy_table)
+                    and _same(res, mix_ref))
+        key = "pipeline" if on else "no_pipeline"
+        out[key] = {
+            "load_s": t_load, "resident_bytes": st1["resident_bytes"],
+            "heavy_cold_ms": cold * 1e3, "heavy_cold_segment_loads": st1["segment_loads"],
+            "heavy_warm_ms": warm * 1e3, "heavy_warm_segment_loads": st2["segment_loads"] - st1["segment_loads"],
+            "mix_s": t_mix, "mix_q_per_s": len(mix) / t_mix,
+            "mix_segment_loads": st3["segment_loads"] - st2["segment_loads"],
+            "mix_evictions": st3["segment_evictions"] - st2["segment_evictions"],
+            "prefetches": st3["prefetches"], "identical": same}
+        log(f"framed ({budget / 2**30:.1f} GiB of {gstore.image_nbytes() / 2**30:.1f} GiB), pipeline {on}: "
+            f"load {t_load:.1f} s, heavy cold {cold * 1e3:.1f} ms / warm {warm * 1e3:.2f} ms, mix "
+            f"{len(mix) / t_mix:.0f} q/s ({out[key]['mix_segment_loads']} frame loads, "
+            f"{out[key]['mix_evictions']} evictions, {st3['prefetches']} prefetches), identical={same}")
+        eng.close()
+        dev.close()
+    return out
 
 
 def table_exchange(dev, seed, rows=1 << 25, cols=4, parts=(8, 64, 1024)):

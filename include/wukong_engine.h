@@ -108,6 +108,13 @@ int wke_set_heavy_workers(wke_engine *e, int n);
  * write the fan-out when the filter can run on its lineage. Same tables and
  * work counts; on by default, 0 runs the two steps one after the other. */
 int wke_set_fused_filters(wke_engine *e, int on);
+/* Plain path on a framed graph: before each step, the first later pattern's
+ * predicate that is not resident starts loading on the copy stream
+ * (wkg_prefetch_predicate, the current step's predicate kept), as the
+ * reference's global_gpu_enable_pipeline does (core/rcache.hpp:273-281). On
+ * by default (the reference's shipped config, scripts/config:33); no effect
+ * on a resident graph. */
+int wke_set_pipeline(wke_engine *e, int on);
 
 /* Light queries of wke_serve / wke_serve_closed that start from an index or
  * a constant and have at most 8 patterns go first to the small-query server

@@ -73,7 +73,8 @@ const char *wkg_last_error(const wkg_ctx *ctx);
  * in pinned host memory and loads per-predicate segments into fixed-size
  * frames on demand, evicting least recently used predicates not needed by the
  * current step (ShardManager, core/shard_manager.hpp:44-557). The host arrays
- * must stay valid until wkg_destroy in the framed case. Without
+ * must stay valid until wkg_destroy in the framed case (the segments are
+ * copied from a pinned copy of them when one can be allocated). Without
  * WKG_LOAD_SORTED_LISTS the lists are checked once on the device; unsorted
  * lists make the known_to_known/known_to_const filters use a linear scan. */
 int wkg_load_graph(wkg_ctx *ctx, const uint64_t *slots, uint64_t num_slots,
@@ -83,8 +84,35 @@ int wkg_load_graph(wkg_ctx *ctx, const uint64_t *slots, uint64_t num_slots,
 
 /* Makes predicate/type `pid` resident (ShardManager::load_predicate,
  * core/shard_manager.hpp:473-555; check_pred_exist :351-359). A no-op when the
- * whole image is resident. */
+ * whole image is resident. When a prefetch of pid is under way, later work on
+ * the context's streams waits for it on the device instead. */
 int wkg_ensure_predicate(wkg_ctx *ctx, int pid);
+
+/* 1 when pid's whole segment is in frames (or the image is resident), 0 when
+ * not (ShardManager::check_pred_exist, core/shard_manager.hpp:351-359). */
+int wkg_predicate_resident(const wkg_ctx *ctx, int pid);
+
+/* Starts loading pid's segment on the context's copy stream and returns
+ * (the pipelined load of a later pattern's predicate while predicate
+ * `in_use` runs, ShardManager::load_predicate(pred, current, ..., async),
+ * core/rcache.hpp:273-281, global_gpu_enable_pipeline). Frames are taken
+ * now; neither `in_use` (< 0: none) nor the most recently used predicate is
+ * evicted for them. The next wkg_ensure_predicate or step on pid waits for
+ * the copies on the device. Copies run from a pinned copy of the host image,
+ * made at wkg_load_graph when host memory allows (else they read the
+ * caller's arrays and the host waits for them). A no-op for a resident image
+ * or a predicate already resident or under way. */
+int wkg_prefetch_predicate(wkg_ctx *ctx, int pid, int in_use);
+
+/* Loads the listed predicates' segments now, in order (n < 0: every
+ * predicate id from 1 up), as the reference does at start-up
+ * (global_gpu_preload_predicts, core/rcache.hpp:191-217); later ones may
+ * evict earlier ones when they do not all fit. Returns when the copies are
+ * done. */
+int wkg_preload_predicates(wkg_ctx *ctx, const int *pids, int n);
+
+/* Prefetches started (wkg_prefetch_predicate that copied something). */
+int wkg_graph_prefetches(const wkg_ctx *ctx, uint64_t *n);
 
 /* Builds the gstore image on the device from n ID triples (host arrays, as
  * read by wks_triples_read_file) and makes it the context's resident graph:

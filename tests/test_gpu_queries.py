@@ -351,6 +351,61 @@ def test_framed_residency_matches_oracle(graph_mid):
     dev.close()
 
 
+def test_framed_prefetch_and_preload(graph_mid):
+    """Pipelined segment loads (a later pattern's predicate loads on the copy
+    stream while a step runs, core/rcache.hpp:273-281) and start-up preloads
+    (core/rcache.hpp:191-217) on a framed graph: the answers stay the
+    oracle's with the pipeline on and off; a prefetched predicate is resident
+    at once and the one in use is never its victim; a preloaded predicate is
+    resident; on a resident image all three are no-ops."""
+    _, gs = graph_mid
+    dev = Device(0)
+    dev.load(gs, graph_budget_bytes=4 << 20)
+    eng = GPUEngine(dev)
+    rng = np.random.default_rng(43)
+    qs = random_queries(rng, 20000, 12, [14, 15, 16], 60, max_steps=4)
+    for on in (True, False):
+        print(f"framed: plain path, pipeline {on}", flush=True)
+        eng.set_pipeline(on)
+        before = dev.stats()["prefetches"]
+        for q in qs:
+            _check(gs, q, eng.run_query(q))
+        made = dev.stats()["prefetches"] - before
+        assert (made > 0) if on else (made == 0), (on, made)
+    print("framed: explicit prefetches", flush=True)
+    preds = list(range(2, 14))
+    fetched = 0
+    for in_use, p in zip(preds, preds[1:] + preds[:1]):
+        dev.ensure(in_use)
+        try:
+            dev.prefetch(p, in_use=in_use)
+        except DeviceError as e:  # the two segments do not fit the budget together
+            assert e.rc == -2, e
+            assert dev.resident(in_use) and not dev.resident(p), (p, in_use)
+            continue
+        fetched += 1
+        assert dev.resident(p) and dev.resident(in_use), (p, in_use)
+        dev.ensure(p)  # waits for the copies on the device
+    assert fetched > 0
+    for q in qs[:20]:
+        _check(gs, q, eng.run_query(q))
+    print("framed: preloads", flush=True)
+    dev.preload()  # every predicate, later ones evicting earlier ones
+    for p in preds:
+        dev.preload([p])
+        assert dev.resident(p)
+    for q in qs[:20]:
+        _check(gs, q, eng.run_query(q))
+    eng.close()
+    dev.close()
+    dev = Device(0)
+    dev.load(gs)
+    dev.prefetch(3, in_use=2)
+    dev.preload([2, 3])
+    assert dev.resident(3) and dev.stats()["prefetches"] == 0
+    dev.close()
+
+
 def test_long_index_lists_resident(graph_mid):
     """Predicate-index lists longer than one probe-and-copy launch (8192
     entries) on a resident graph: each list and each query that starts from

@@ -136,6 +136,10 @@ def hip_lib():
         _sig(L, "wkg_last_error", C.c_char_p, vp)
         _sig(L, "wkg_load_graph", C.c_int, vp, P(u64), u64, P(u32), u64, P(PredMeta), C.c_int, u64, u32)
         _sig(L, "wkg_ensure_predicate", C.c_int, vp, C.c_int)
+        _sig(L, "wkg_predicate_resident", C.c_int, vp, C.c_int)
+        _sig(L, "wkg_prefetch_predicate", C.c_int, vp, C.c_int, C.c_int)
+        _sig(L, "wkg_preload_predicates", C.c_int, vp, P(i32), C.c_int)
+        _sig(L, "wkg_graph_prefetches", C.c_int, vp, P(u64))
         _sig(L, "wkg_graph_stats", C.c_int, vp, P(u64), P(u64), P(u64))
         _sig(L, "wkg_table_from_host", C.c_int, vp, P(i32), u64, C.c_int, vp, P(vp))
         _sig(L, "wkg_table_to_host", C.c_int, vp, vp, P(i32), vp)
@@ -217,6 +221,7 @@ def engine_lib():
         _sig(L, "wke_serve_stats", C.c_int, vp, P(u64), P(u64), P(u64), P(u64))
         _sig(L, "wke_set_serve_small", C.c_int, vp, C.c_int)
         _sig(L, "wke_set_fused_filters", C.c_int, vp, C.c_int)
+        _sig(L, "wke_set_pipeline", C.c_int, vp, C.c_int)
         _sig(L, "wke_set_heavy_workers", C.c_int, vp, C.c_int)
         _sig(L, "wke_serve_small_stats", C.c_int, vp, P(u64), P(u64))
         _sig(L, "wke_serve", C.c_int, vp, P(i64), P(i64), C.c_int, P(dbl), P(C.c_uint8), u64, C.c_int,

@@ -168,6 +168,7 @@ struct wke_engine {
   bool small_queries = true;  // wke_set_small_queries: try wkg_query_small first
   int serve_small = 16;       // wke_set_serve_small: small-query server workgroups while serving (0: none)
   bool fuse_filters = true;   // wke_set_fused_filters: known_to_unknown + filter on its new column in one call
+  bool pipeline = true;       // wke_set_pipeline: framed graphs load a later pattern's predicate meanwhile
   uint64_t sv_fallbacks = 0, sv_solo = 0, sv_worker = 0, sv_backlog = 0;  // last wke_serve
   uint64_t sv_small = 0, sv_small_declined = 0;
   // device-resident executor shape (wke_configure) and tuning (wke_autotune)
@@ -217,11 +218,31 @@ struct wke_engine {
     return rc;
   }
 
+  // Framed graphs: the first later pattern's predicate that is not resident
+  // starts loading on the copy stream while this step (on predicate
+  // `in_use`) runs (global_gpu_enable_pipeline, core/rcache.hpp:273-281). A
+  // prefetch that fails leaves the load to that step.
+  void prefetch_next(const Request &r, int cols, int in_use) {
+    if (wkg_graph_framed(ctx) != 1) return;
+    std::string why;
+    for (int st = r.step + 1; st * 4 < r.ncmd; st++) {
+      wkg_step_desc d;
+      int nc;
+      if (dispatch(r.cmd, st, cols, &d, &nc, why) != WKG_OK) return;
+      cols = nc;
+      const int res = wkg_predicate_resident(ctx, d.pid);
+      if (res == 1) continue;
+      if (res == 0) (void)wkg_prefetch_predicate(ctx, d.pid, in_use);
+      return;
+    }
+  }
+
   int run_one_step(Request &r, void *stream) {
     wkg_step_desc d;
     int nc;
     int rc = plan(r, &d, &nc);
     if (rc) return rc;
+    if (pipeline) prefetch_next(r, nc, d.pid);
     wkg_table *o = nullptr;
     uint64_t ex = 0;
     // known_to_unknown whose new column the next pattern filters: one call,
@@ -711,6 +732,12 @@ int wke_set_heavy_workers(wke_engine *e, int n) {
 int wke_set_fused_filters(wke_engine *e, int on) {
   if (!e) return WKG_EINVAL;
   e->fuse_filters = on != 0;
+  return WKG_OK;
+}
+
+int wke_set_pipeline(wke_engine *e, int on) {
+  if (!e) return WKG_EINVAL;
+  e->pipeline = on != 0;
   return WKG_OK;
 }
 

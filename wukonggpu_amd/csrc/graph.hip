@@ -81,7 +81,13 @@ void wkg_ctx::free_graph() {
   if (d_vframes) (void)hipFree(d_vframes);
   if (d_eframes) (void)hipFree(d_eframes);
   if (res) {
-    if (res->copy_stream) (void)hipStreamDestroy(res->copy_stream);
+    if (res->copy_stream) {
+      (void)hipStreamSynchronize(res->copy_stream);
+      (void)hipStreamDestroy(res->copy_stream);
+    }
+    for (hipEvent_t e : res->ready)
+      if (e) (void)hipEventDestroy(e);
+    if (res->pinned) (void)hipHostFree(res->pinned);
     delete res;
   }
   d_slots = nullptr;
@@ -95,11 +101,25 @@ void wkg_ctx::free_graph() {
   resident_bytes = 0;
 }
 
-// Loads missing frames of `pid`, evicting least recently used predicates that
-// are neither `pid` nor listed in `keep`.
+// Makes `pid` resident for work queued on stream s: waits (on s) for its
+// prefetch if one is under way, else loads its missing frames on s.
 int wkg_ctx::ensure_pred(int pid, hipStream_t s, const std::vector<int> *keep) {
   if (!framed) return WKG_OK;
   if (pid < 0 || (uint32_t)pid >= num_metas) return fail(WKG_EINVAL, "predicate id %d out of range", pid);
+  Residency &r = *res;
+  if (r.pending[pid]) {
+    r.pending[pid] = 0;
+    if (hipStreamWaitEvent(s, r.ready[pid], 0) != hipSuccess)
+      return fail(WKG_EDEVICE, "waiting for predicate %d's prefetch failed", pid);
+  }
+  return load_pred(pid, s, keep, false);
+}
+
+// Loads missing frames of `pid` with copies queued on stream s, evicting
+// least recently used predicates that are neither `pid` nor listed in `keep`.
+// A prefetch (on the copy stream) records pid's ready event after its copies
+// and leaves the predicate pending; it does not count as a use.
+int wkg_ctx::load_pred(int pid, hipStream_t s, const std::vector<int> *keep, bool prefetch) {
   Residency &r = *res;
   auto touch = [&](int p) {
     if (r.in_lru[p]) r.lru.erase(r.lru_pos[p]);
@@ -108,7 +128,7 @@ int wkg_ctx::ensure_pred(int pid, hipStream_t s, const std::vector<int> *keep) {
     r.in_lru[p] = 1;
   };
   if (r.used_v[pid] == r.vlen[pid] && r.used_e[pid] == r.elen[pid]) {
-    touch(pid);
+    if (!prefetch) touch(pid);
     return WKG_OK;
   }
   if (r.vlen[pid] > r.n_vframes || r.elen[pid] > r.n_eframes)
@@ -133,9 +153,10 @@ int wkg_ctx::ensure_pred(int pid, hipStream_t s, const std::vector<int> *keep) {
         break;
       }
       if (victim < 0) return false;
-      if (!synced) {  // frames may still be read by queued work
+      if (!synced) {  // frames may still be read (or prefetched into) by queued work
         (void)hipDeviceSynchronize();
         synced = true;
+        std::fill(r.pending.begin(), r.pending.end(), 0);  // every prefetch has landed
       }
       std::vector<uint64_t> &tab = vert ? r.h_vframes : r.h_eframes;
       const uint64_t base = vert ? h_meta[victim].vframe_base : h_meta[victim].eframe_base;
@@ -217,6 +238,16 @@ int wkg_ctx::ensure_pred(int pid, hipStream_t s, const std::vector<int> *keep) {
         hipMemcpy(d_eframes + m.eframe_base, r.h_eframes.data() + m.eframe_base, r.elen[pid] * 8,
                   hipMemcpyHostToDevice) != hipSuccess)
       return fail(WKG_EDEVICE, "frame table update failed");
+  }
+  if (prefetch) {
+    if (!r.ready[pid] && hipEventCreateWithFlags(&r.ready[pid], hipEventDisableTiming) != hipSuccess) {
+      r.ready[pid] = nullptr;
+      return fail(WKG_EDEVICE, "event creation failed");
+    }
+    if (hipEventRecord(r.ready[pid], s) != hipSuccess) return fail(WKG_EDEVICE, "event record failed");
+    r.pending[pid] = 1;
+    r.stat_prefetches++;
+    // newest in the LRU order, so the next prefetch does not evict it first
   }
   touch(pid);
   return WKG_OK;
@@ -397,6 +428,27 @@ int wkg_load_graph(wkg_ctx *ctx, const uint64_t *slots, uint64_t num_slots, cons
     ctx->res = r;
     r->h_slots = slots;
     r->h_edges = edges;
+    // Segment copies read the image from pinned memory, so that a prefetch
+    // on the copy stream runs while the host goes on (the reference pins the
+    // host image in place with cudaHostRegister, core/rcache.hpp:187-189;
+    // here the image is copied once into memory allocated pinned). Without
+    // room for the copy, copies read the caller's arrays.
+    if (hipHostMalloc((void **)&r->pinned, slot_bytes + edge_bytes, hipHostMallocDefault) == hipSuccess) {
+      memcpy(r->pinned, slots, slot_bytes);
+      if (num_edges) memcpy(r->pinned + slot_bytes, edges, num_edges * 4);
+      r->h_slots = (const uint64_t *)r->pinned;
+      r->h_edges = (const uint32_t *)(r->pinned + slot_bytes);
+    } else {
+      (void)hipGetLastError();
+      r->pinned = nullptr;
+    }
+    if (hipStreamCreateWithFlags(&r->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+      r->copy_stream = nullptr;
+      ctx->free_graph();
+      return ctx->fail(WKG_EDEVICE, "copy stream creation failed");
+    }
+    r->ready.assign(num_metas, nullptr);
+    r->pending.assign(num_metas, 0);
     // split the budget between bucket and edge frames in proportion to the
     // segments, frames of at most 16 MiB and at least 64 per pool
     uint64_t seg_b = 0, seg_e = 0, max_b = 1, max_e = 1;
@@ -472,6 +524,61 @@ int wkg_ensure_predicate(wkg_ctx *ctx, int pid) {
   if (!ctx) return WKG_EINVAL;
   if (!ctx->loaded) return ctx->fail(WKG_ESTATE, "no graph loaded");
   return ctx->ensure_pred(pid, ctx->stream);
+}
+
+int wkg_prefetch_predicate(wkg_ctx *ctx, int pid, int in_use) {
+  if (!ctx) return WKG_EINVAL;
+  if (!ctx->loaded) return ctx->fail(WKG_ESTATE, "no graph loaded");
+  if (!ctx->framed) return WKG_OK;
+  if (pid < 0 || (uint32_t)pid >= ctx->num_metas) return ctx->fail(WKG_EINVAL, "predicate id %d out of range", pid);
+  Residency &r = *ctx->res;
+  if (r.pending[pid]) return WKG_OK;
+  // neither the predicate in use nor the most recently used one is a victim
+  std::vector<int> keep;
+  if (in_use >= 0) keep.push_back(in_use);
+  if (!r.lru.empty()) keep.push_back(r.lru.front());
+  const int rc = ctx->load_pred(pid, r.copy_stream, &keep, true);
+  if (rc != WKG_OK && r.ready[pid]) {
+    // a part of the segment may have been queued before the failure: the
+    // next use of pid waits for it too
+    if (hipEventRecord(r.ready[pid], r.copy_stream) == hipSuccess) r.pending[pid] = 1;
+  } else if (rc != WKG_OK) {
+    (void)hipStreamSynchronize(r.copy_stream);
+  }
+  return rc;
+}
+
+int wkg_predicate_resident(const wkg_ctx *ctx, int pid) {
+  if (!ctx) return WKG_EINVAL;
+  if (!ctx->loaded) return WKG_ESTATE;
+  if (pid < 0 || (uint32_t)pid >= ctx->num_metas) return WKG_EINVAL;
+  if (!ctx->framed) return 1;
+  const Residency &r = *ctx->res;
+  return r.used_v[pid] == r.vlen[pid] && r.used_e[pid] == r.elen[pid] ? 1 : 0;
+}
+
+int wkg_graph_prefetches(const wkg_ctx *ctx, uint64_t *n) {
+  if (!ctx || !n) return WKG_EINVAL;
+  *n = ctx->res ? ctx->res->stat_prefetches : 0;
+  return WKG_OK;
+}
+
+int wkg_preload_predicates(wkg_ctx *ctx, const int *pids, int n) {
+  if (!ctx) return WKG_EINVAL;
+  if (!ctx->loaded) return ctx->fail(WKG_ESTATE, "no graph loaded");
+  if (n > 0 && !pids) return ctx->fail(WKG_EINVAL, "null predicate list");
+  if (!ctx->framed) return WKG_OK;
+  std::vector<int> list;
+  if (n < 0)
+    for (uint32_t p = 1; p < ctx->num_metas; p++) list.push_back((int)p);
+  else
+    list.assign(pids, pids + n);
+  for (int p : list) {
+    if (p < 0 || (uint32_t)p >= ctx->num_metas) return ctx->fail(WKG_EINVAL, "predicate id %d out of range", p);
+    if (ctx->h_meta[p].psz == 0 && ctx->h_meta[p].edge_len == 0) continue;  // nothing stored
+    if (int rc = ctx->ensure_pred(p, ctx->stream)) return rc;
+  }
+  return hipStreamSynchronize(ctx->stream) == hipSuccess ? WKG_OK : ctx->fail(WKG_EDEVICE, "preload failed");
 }
 
 int wkg_graph_framed(const wkg_ctx *ctx) {

@@ -125,9 +125,13 @@ struct Residency {
   std::list<int> lru;                              // resident predicates, most recent first
   std::vector<std::list<int>::iterator> lru_pos;
   std::vector<char> in_lru;
-  const uint64_t *h_slots = nullptr;               // caller's image (host)
+  const uint64_t *h_slots = nullptr;               // the image on the host: a pinned copy, or the caller's
   const uint32_t *h_edges = nullptr;
-  hipStream_t copy_stream = nullptr;
+  char *pinned = nullptr;                          // the pinned copy (owned), when it could be made
+  hipStream_t copy_stream = nullptr;               // prefetches
+  std::vector<hipEvent_t> ready;                   // per predicate: its prefetch's copies are done
+  std::vector<char> pending;                       // per predicate: prefetched, not yet waited for
+  uint64_t stat_prefetches = 0;
 };
 
 struct SQServer;
@@ -263,6 +267,7 @@ struct wkg_ctx {
   int set_image_metas(uint64_t nslots, uint64_t nedges, const wkg_pred_meta_abi *m_in, int nmetas);
   int finish_resident(uint32_t flags, hipStream_t s);
   int ensure_pred(int pid, hipStream_t s, const std::vector<int> *keep = nullptr);
+  int load_pred(int pid, hipStream_t s, const std::vector<int> *keep, bool prefetch);
   void free_graph();
 };
 

@@ -175,9 +175,34 @@ class Device:
         return float(g.value)
 
     def stats(self):
-        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        a, b, c, d = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
         self._check(N.hip_lib().wkg_graph_stats(self._h, C.byref(a), C.byref(b), C.byref(c)))
-        return {"resident_bytes": a.value, "segment_loads": b.value, "segment_evictions": c.value}
+        self._check(N.hip_lib().wkg_graph_prefetches(self._h, C.byref(d)))
+        return {"resident_bytes": a.value, "segment_loads": b.value, "segment_evictions": c.value,
+                "prefetches": d.value}
+
+    def ensure(self, pid):
+        """Makes predicate pid's segment resident (framed graphs)."""
+        self._check(N.hip_lib().wkg_ensure_predicate(self._h, int(pid)))
+
+    def resident(self, pid):
+        """True when predicate pid's whole segment is in frames (always, for a resident image)."""
+        rc = N.hip_lib().wkg_predicate_resident(self._h, int(pid))
+        if rc < 0:
+            self._check(rc)
+        return rc == 1
+
+    def prefetch(self, pid, in_use=-1):
+        """Starts loading pid's segment on the copy stream (framed graphs)."""
+        self._check(N.hip_lib().wkg_prefetch_predicate(self._h, int(pid), int(in_use)))
+
+    def preload(self, pids=None):
+        """Loads the listed predicates' segments now (None: every predicate id from 1 up)."""
+        if pids is None:
+            self._check(N.hip_lib().wkg_preload_predicates(self._h, None, -1))
+        else:
+            a = (C.c_int32 * len(pids))(*[int(p) for p in pids])
+            self._check(N.hip_lib().wkg_preload_predicates(self._h, a, len(pids)))
 
     def table(self, rows):
         """Host rows (2-D int array) -> device Table."""
@@ -514,6 +539,12 @@ class GPUEngine:
         rc = N.engine_lib().wke_set_fused_filters(self._h, 1 if on else 0)
         if rc != 0:
             raise DeviceError(rc, "wke_set_fused_filters failed")
+
+    def set_pipeline(self, on):
+        """Framed graphs: load a later pattern's predicate while a step runs (default on)."""
+        rc = N.engine_lib().wke_set_pipeline(self._h, 1 if on else 0)
+        if rc != 0:
+            raise DeviceError(rc, "wke_set_pipeline failed")
 
     def set_serve_small(self, workgroups):
         """Small-query server workgroups for served light queries (0: light lanes only)."""
