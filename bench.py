@@ -604,7 +604,7 @@ def run_config23(args, nested=False):
     exchange = table_exchange(dev, args.seed)
     # ---- framed residency: the same graph under a graph budget below its image -----------------
     framed = None
-    if args.framed_budget and not nested:
+    if args.framed_budget and (not nested or time_left(args) > 120):
         framed = framed_mode(Device, GPUEngine, w.gstore, heavy, mix, r.table, plain, args.framed_budget)
     # the 151M-row hub fan-out table (the heavy query's first four patterns), for the oracle below
     rp = eng.run_query(heavy[:16]) if not args.no_cpu else None
