@@ -108,11 +108,17 @@ int wkg_ctx::ensure_pred(int pid, hipStream_t s, const std::vector<int> *keep) {
   if (pid < 0 || (uint32_t)pid >= num_metas) return fail(WKG_EINVAL, "predicate id %d out of range", pid);
   Residency &r = *res;
   if (r.pending[pid]) {
-    r.pending[pid] = 0;
-    if (hipStreamWaitEvent(s, r.ready[pid], 0) != hipSuccess)
-      return fail(WKG_EDEVICE, "waiting for predicate %d's prefetch failed", pid);
+    // its copies were queued on some stream (a prefetch, or a load for
+    // another stream): work on s waits for them until they are known done
+    if (hipEventQuery(r.ready[pid]) == hipSuccess) {
+      r.pending[pid] = 0;
+    } else if (hipStreamWaitEvent(s, r.ready[pid], 0) != hipSuccess) {
+      return fail(WKG_EDEVICE, "waiting for predicate %d's segment copies failed", pid);
+    }
   }
-  return load_pred(pid, s, keep, false);
+  const int rc = load_pred(pid, s, keep, false);
+  if (rc != WKG_OK) (void)hipStreamSynchronize(s);  // copies queued before the failure are not tracked
+  return rc;
 }
 
 // Loads missing frames of `pid` with copies queued on stream s, evicting
@@ -239,17 +245,20 @@ int wkg_ctx::load_pred(int pid, hipStream_t s, const std::vector<int> *keep, boo
                   hipMemcpyHostToDevice) != hipSuccess)
       return fail(WKG_EDEVICE, "frame table update failed");
   }
-  if (prefetch) {
-    if (!r.ready[pid] && hipEventCreateWithFlags(&r.ready[pid], hipEventDisableTiming) != hipSuccess) {
-      r.ready[pid] = nullptr;
-      return fail(WKG_EDEVICE, "event creation failed");
-    }
-    if (hipEventRecord(r.ready[pid], s) != hipSuccess) return fail(WKG_EDEVICE, "event record failed");
-    r.pending[pid] = 1;
-    r.stat_prefetches++;
-    // newest in the LRU order, so the next prefetch does not evict it first
+  // The copies read pinned memory, so they may still run when this returns:
+  // the ready event lets work on other streams wait for them (ensure_pred).
+  if (!r.ready[pid] && hipEventCreateWithFlags(&r.ready[pid], hipEventDisableTiming) != hipSuccess) {
+    r.ready[pid] = nullptr;
+    (void)hipStreamSynchronize(s);
+    return fail(WKG_EDEVICE, "event creation failed");
   }
-  touch(pid);
+  if (hipEventRecord(r.ready[pid], s) != hipSuccess) {
+    (void)hipStreamSynchronize(s);
+    return fail(WKG_EDEVICE, "event record failed");
+  }
+  r.pending[pid] = 1;
+  if (prefetch) r.stat_prefetches++;
+  touch(pid);  // (a prefetch too: newest in the LRU order, so the next prefetch does not evict it first)
   return WKG_OK;
 }
 
