@@ -17,7 +17,8 @@ def _table(rng, rows, cols, lo=1 << 17, span=1 << 20):
 @pytest.mark.parametrize("rows,cols,col,nparts", [
     (0, 3, 1, 8), (1, 1, 0, 1), (63, 2, 0, 2), (64, 3, 2, 3), (65, 1, 0, 8), (1023, 4, 1, 8), (1024, 5, 4, 7),
     (1025, 2, 1, 64), (5000, 17, 9, 65), (20000, 3, 0, 1000), (70000, 6, 5, 1024), (300000, 4, 2, 8),
-    (2000, 8, 6, 8), (9000, 8, 1, 100), (4000, 1, 0, 300), (3000, 12, 11, 5)])
+    (2000, 8, 6, 8), (9000, 8, 1, 100), (4000, 1, 0, 300), (3000, 12, 11, 5),
+    (100000, 4, 3, 64), (50000, 2, 0, 33), (9000, 1, 0, 64), (4097, 4, 0, 64)])
 def test_split_matches_oracle(dev_small, rows, cols, col, nparts):
     rng = np.random.default_rng(rows + 31 * nparts)
     t = _table(rng, rows, cols)

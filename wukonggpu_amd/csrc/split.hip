@@ -21,7 +21,9 @@
 //            so base[part][slab] is where the slab's rows of that part go;
 //   scatter: each 64-row chunk ranks its rows inside their parts, then the
 //            rows are copied to their places (rows of one part are
-//            contiguous there too).
+//            contiguous there too); with 9..64 parts and rows of 1, 2 or 4
+//            cells, a workgroup's four slabs are first sorted by part in LDS
+//            and leave as one run per part (k_split_scatter_staged).
 // A chunk's rows are grouped by part with one ballot per bit of the part
 // number (ceil(log2 parts) ballots, whatever the parts in the chunk); with
 // at most 64 parts the per-part counts and offsets live in one register per
@@ -274,6 +276,79 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_scatter(SplitArgs a) {
   }
 }
 
+// Scatter with the workgroup's rows sorted by part in LDS first (at most 64
+// parts, rows of C = 1, 2 or 4 cells): workgroup b takes the kWaves slabs
+// kWaves*b .. kWaves*b + kWaves - 1 (kStageRows rows). Each wave ranks its
+// slab's rows as the direct scatter does, but puts each row at its place in
+// the workgroup's part-sorted block in LDS; then the workgroup copies the
+// block out, each part's rows to their contiguous destination, so a part's
+// rows from the whole workgroup leave as one run of whole cache lines
+// rather than a few rows per chunk (partly written lines otherwise make
+// the writes grow with the part count).
+constexpr uint32_t kStageRows = kWaves * 1024;
+template <int C>
+__global__ __launch_bounds__(kSplitBlock) void k_split_scatter_staged(SplitArgs a) {
+  using T = typename VecOf<C>::T;
+  __shared__ T s_rows[kStageRows];
+  __shared__ uint8_t s_part[kStageRows];
+  __shared__ uint32_t s_pstart[kWave];
+  __shared__ uint64_t s_gbase[kWave];
+  const uint32_t w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const uint64_t s0 = (uint64_t)blockIdx.x * kWaves;  // the workgroup's first slab
+  const uint64_t row0 = s0 * a.slab;
+  const uint32_t nrows = (uint32_t)min<uint64_t>(kStageRows, a.rows - row0);
+  const uint64_t nsl = min<uint64_t>(kWaves, a.nslabs - s0);
+  if (w == 0) {  // lane q: part q's rows in the workgroup, its first place in LDS and in the output
+    uint64_t c = 0;
+    if (lane < a.parts)
+      for (uint64_t k = 0; k < nsl; k++) c += a.cnt[(uint64_t)lane * a.nslabs + s0 + k];
+    const uint64_t incl = wave_incl_scan(c);
+    s_pstart[lane] = (uint32_t)(incl - c);
+    s_gbase[lane] = lane < a.parts ? a.base[(uint64_t)lane * a.nslabs + s0] : 0;
+  }
+  __syncthreads();
+  const uint64_t slab = s0 + w;
+  if (slab < a.nslabs) {
+    const uint64_t below = lanes_below();
+    // lane q: where the wave's next row of part q goes in LDS
+    uint32_t run = 0;
+    if (lane < a.parts)
+      run = s_pstart[lane] + (uint32_t)(a.base[(uint64_t)lane * a.nslabs + slab] - s_gbase[lane]);
+    const uint64_t r0 = slab * a.slab, r1 = min(r0 + a.slab, a.rows);
+    for (uint64_t cb = r0; cb < r1; cb += kU * kWave) {
+      T rowv[kU];
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        const uint64_t r = cb + (uint64_t)u * kWave + lane;
+        rowv[u] = T(0);
+        if (r < r1) rowv[u] = ((const T *)a.in)[r];
+      }
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        const uint64_t r = cb + (uint64_t)u * kWave + lane;
+        const bool valid = r < r1;
+        uint32_t key = 0;
+#pragma unroll
+        for (int c = 0; c < C; c++)
+          if ((uint32_t)c == a.col) key = (uint32_t)rowv[u][c];
+        const uint32_t p = valid ? (uint32_t)fastmod(key, a.parts, a.pmagic) : 0u;
+        const Groups g = group_parts(p, valid, lane, a.nbits);
+        const uint32_t at = shfl32(run, p) + (uint32_t)__builtin_popcountll(g.peers & below);
+        run += (uint32_t)__builtin_popcountll(g.own);
+        if (valid) {
+          s_rows[at] = rowv[u];
+          s_part[at] = (uint8_t)p;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < nrows; j += kSplitBlock) {
+    const uint32_t p = s_part[j];
+    ((T *)a.out)[s_gbase[p] + (j - s_pstart[p])] = s_rows[j];
+  }
+}
+
 // Key loads as V-cell vectors (the row's vector that holds the key cell):
 // rows of 2 or 4k cells are read as whole 8- or 16-byte lane accesses.
 template <bool SMALL>
@@ -370,8 +445,17 @@ int wkg_table_split(wkg_ctx *ctx, const wkg_table *in, int col, int nparts, void
     }
     if (e == hipSuccess) {
       ProfScope ps(ctx, s, "split_scatter", 8.0 * (double)in->rows * (double)in->cols);
-      if (small) launch_scatter<true>(a, (unsigned)blocks, 0, s);
-      else launch_scatter<false>(a, (unsigned)blocks, (size_t)kWaves * nparts * 8, s);
+      // (with at most 8 parts the direct scatter's runs are long already,
+      // and it measured faster there)
+      if (small && nparts > 8 && a.slab == 1024 && (a.cols == 1 || a.cols == 2 || a.cols == 4)) {
+        if (a.cols == 1) k_split_scatter_staged<1><<<(unsigned)blocks, kSplitBlock, 0, s>>>(a);
+        else if (a.cols == 2) k_split_scatter_staged<2><<<(unsigned)blocks, kSplitBlock, 0, s>>>(a);
+        else k_split_scatter_staged<4><<<(unsigned)blocks, kSplitBlock, 0, s>>>(a);
+      } else if (small) {
+        launch_scatter<true>(a, (unsigned)blocks, 0, s);
+      } else {
+        launch_scatter<false>(a, (unsigned)blocks, (size_t)kWaves * nparts * 8, s);
+      }
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(heads.data(), d_heads, (size_t)nparts * 8, hipMemcpyDeviceToHost, s);
