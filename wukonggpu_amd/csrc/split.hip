@@ -292,19 +292,31 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_scatter_staged(SplitArgs 
   __shared__ T s_rows[kStageRows];
   __shared__ uint8_t s_part[kStageRows];
   __shared__ uint32_t s_pstart[kWave];
+  __shared__ uint32_t s_woff[kWaves][kWave];
   __shared__ uint64_t s_gbase[kWave];
   const uint32_t w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const uint64_t s0 = (uint64_t)blockIdx.x * kWaves;  // the workgroup's first slab
   const uint64_t row0 = s0 * a.slab;
   const uint32_t nrows = (uint32_t)min<uint64_t>(kStageRows, a.rows - row0);
-  const uint64_t nsl = min<uint64_t>(kWaves, a.nslabs - s0);
-  if (w == 0) {  // lane q: part q's rows in the workgroup, its first place in LDS and in the output
-    uint64_t c = 0;
-    if (lane < a.parts)
-      for (uint64_t k = 0; k < nsl; k++) c += a.cnt[(uint64_t)lane * a.nslabs + s0 + k];
+  const uint32_t nsl = (uint32_t)min<uint64_t>(kWaves, a.nslabs - s0);
+  if (w == 0) {
+    // lane q: part q's places for the workgroup's slabs, read as one short
+    // run of the part-major scan (its end: the next slab's place, or the
+    // next part's first place after the last slab)
+    uint64_t b[kWaves + 1] = {0, 0, 0, 0, 0};
+    if (lane < a.parts) {
+      const uint64_t *bp = a.base + (uint64_t)lane * a.nslabs + s0;
+#pragma unroll
+      for (uint32_t k = 0; k < kWaves; k++)
+        if (k < nsl) b[k] = bp[k];
+      b[kWaves] = s0 + nsl < a.nslabs ? bp[nsl] : lane + 1 < a.parts ? a.base[(uint64_t)(lane + 1) * a.nslabs] : a.rows;
+    }
+    const uint64_t c = b[kWaves] - b[0];  // part q's rows in the workgroup
     const uint64_t incl = wave_incl_scan(c);
     s_pstart[lane] = (uint32_t)(incl - c);
-    s_gbase[lane] = lane < a.parts ? a.base[(uint64_t)lane * a.nslabs + s0] : 0;
+    s_gbase[lane] = b[0];
+#pragma unroll
+    for (uint32_t k = 0; k < kWaves; k++) s_woff[k][lane] = (uint32_t)(b[k] - b[0]);
   }
   __syncthreads();
   const uint64_t slab = s0 + w;
@@ -312,8 +324,7 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_scatter_staged(SplitArgs 
     const uint64_t below = lanes_below();
     // lane q: where the wave's next row of part q goes in LDS
     uint32_t run = 0;
-    if (lane < a.parts)
-      run = s_pstart[lane] + (uint32_t)(a.base[(uint64_t)lane * a.nslabs + slab] - s_gbase[lane]);
+    if (lane < a.parts) run = s_pstart[lane] + s_woff[w][lane];
     const uint64_t r0 = slab * a.slab, r1 = min(r0 + a.slab, a.rows);
     for (uint64_t cb = r0; cb < r1; cb += kU * kWave) {
       T rowv[kU];
