@@ -526,10 +526,14 @@ __device__ __forceinline__ void level_phase(const ExecArgs &a, uint32_t level, u
 
 // ---- probe, tile scan, finalize ----------------------------------------------
 
-// Four waves per SIMD, like the per-step probe: the tile loop's state costs
-// a few spills, all outside the probe's round loop.
+// 1024-row tiles: four waves per SIMD, like the per-step probe (the tile
+// loop's state costs a few spills, all outside the probe's round loop; the
+// 512-query mix's first hop went from 186 to 175 us). 2048-row tiles hold
+// twice the rows' cells in registers: at four waves they spill inside the
+// round loop (the heavy query's 145M-row probe took 2.8 ms instead of 1.5),
+// so they keep three waves and no spills.
 template <int TR, bool SORTED, bool FRAMED, bool QUEUE>
-__global__ __launch_bounds__(kBlock, 4) void k_x_probe(ExecArgs a) {
+__global__ __launch_bounds__(kBlock, TR == 1024 ? 4 : 3) void k_x_probe(ExecArgs a) {
   const uint32_t nq = (uint32_t)a.st->nq;
   uint64_t t, lt;
   uint32_t q;
