@@ -142,6 +142,32 @@ struct ProbeCarry {
   uint32_t deg = 0, off = 0;
 };
 
+// A pointer read from memory (a parameter record in HBM) is generic to the
+// compiler, which then accesses it with flat instructions. Those count
+// against the LDS wait counter as well, so every later LDS wait or workgroup
+// barrier also waits for them. The same pointer in the global address space
+// keeps the accesses vector-memory instructions. (Kernel-argument pointers
+// are global already; the cast costs nothing there.)
+#define WKG_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ WKG_GLOBAL T *gmem(T *p) {
+  return (WKG_GLOBAL T *)p;
+}
+// HIP's uint2/uint4 cannot be copied in another address space (their
+// operators are members); these move them as plain clang vectors.
+typedef uint32_t gu32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t gu32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint2 gload(const uint2 *p) {
+  const gu32x2 v = *(const WKG_GLOBAL gu32x2 *)p;
+  return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ uint4 gload(const uint4 *p) {
+  const gu32x4 v = *(const WKG_GLOBAL gu32x4 *)p;
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gstore(uint2 *p, uint2 v) { *(WKG_GLOBAL gu32x2 *)p = gu32x2{v.x, v.y}; }
+__device__ __forceinline__ void gstore(uint4 *p, uint4 v) { *(WKG_GLOBAL gu32x4 *)p = gu32x4{v.x, v.y, v.z, v.w}; }
+
 // Bit mask of the lanes below this one.
 __device__ __forceinline__ uint64_t lanes_below() {
   const uint32_t lane = lane_id();

@@ -532,19 +532,41 @@ __device__ __forceinline__ void level_phase(const ExecArgs &a, uint32_t level, u
 // twice the rows' cells in registers: at four waves they spill inside the
 // round loop (the heavy query's 145M-row probe took 2.8 ms instead of 1.5),
 // so they keep three waves and no spills.
+//
+// The tile's step parameters come from the query's record in HBM. Read
+// there by reference, every field is read again after each wave fence of
+// the probe (a fence orders all memory), a memory round trip per round. So
+// 2048-row tiles copy the record into registers once per tile (three waves
+// leave room); 1024-row tiles, with no registers to spare at four waves,
+// copy it into LDS and re-read it there.
+template <int TR, bool SORTED, bool FRAMED>
+__device__ __forceinline__ void x_probe_tile(const ExecArgs &a, uint32_t q, uint64_t lt, uint64_t *s_pw) {
+  if constexpr (TR == 1024) {
+    constexpr uint32_t kWords = sizeof(StepParams) / 8;
+    static_assert(sizeof(StepParams) % 8 == 0, "the record is copied in 8-byte words");
+    if (threadIdx.x < kWords) s_pw[threadIdx.x] = gmem((const uint64_t *)&a.items[q].p)[threadIdx.x];
+    __syncthreads();  // (the next tile's copy comes after the deal's barrier)
+    probe_tile<TR, SORTED, FRAMED>(a.g, *(const StepParams *)s_pw, lt);
+  } else {
+    const StepParams p = a.items[q].p;
+    probe_tile<TR, SORTED, FRAMED>(a.g, p, lt);
+  }
+}
+
 template <int TR, bool SORTED, bool FRAMED, bool QUEUE>
 __global__ __launch_bounds__(kBlock, TR == 1024 ? 4 : 3) void k_x_probe(ExecArgs a) {
+  __shared__ uint64_t s_pw[sizeof(StepParams) / 8];
   const uint32_t nq = (uint32_t)a.st->nq;
   uint64_t t, lt;
   uint32_t q;
   if constexpr (QUEUE) {
     __shared__ QueueLds s_wq;
     BlockQueue wq(s_wq, a.st->queue[0], a.st->nt);
-    while (wq.next(a.tile_base, nq, t, q, lt)) probe_tile<TR, SORTED, FRAMED>(a.g, a.items[q].p, lt);
+    while (wq.next(a.tile_base, nq, t, q, lt)) x_probe_tile<TR, SORTED, FRAMED>(a, q, lt, s_pw);
   } else {
     __shared__ uint32_t s_q;
     StaticDeal wd(s_q, a.st->nt);
-    while (wd.next(a.tile_base, nq, t, q, lt)) probe_tile<TR, SORTED, FRAMED>(a.g, a.items[q].p, lt);
+    while (wd.next(a.tile_base, nq, t, q, lt)) x_probe_tile<TR, SORTED, FRAMED>(a, q, lt, s_pw);
   }
 }
 
@@ -590,7 +612,8 @@ __global__ __launch_bounds__(kBlock) void k_x_finalize(ExecArgs a) {
       lo = a.tile_base[q];
       hi = a.tile_base[q + 1];
     }
-    finalize_tile<TR>(a.items[q].p, t - lo);
+    const StepParams p = a.items[q].p;
+    finalize_tile<TR>(p, t - lo);
   }
 }
 
@@ -779,11 +802,17 @@ __global__ __launch_bounds__(kBlock) void k_x_expand(ExecArgs a) {
   if constexpr (QUEUE) {
     __shared__ QueueLds s_wq;
     BlockQueue wq(s_wq, a.st->queue[1], a.st->nx);
-    while (wq.next(a.xtile_base, nq, t, q, lt)) expand_tile<VT, FRAMED>(a.g, a.items[q].x, lt);
+    while (wq.next(a.xtile_base, nq, t, q, lt)) {
+      const ExpandParams x = a.items[q].x;
+      expand_tile<VT, FRAMED>(a.g, x, lt);
+    }
   } else {
     __shared__ uint32_t s_q;
     StaticDeal wd(s_q, a.st->nx);
-    while (wd.next(a.xtile_base, nq, t, q, lt)) expand_tile<VT, FRAMED>(a.g, a.items[q].x, lt);
+    while (wd.next(a.xtile_base, nq, t, q, lt)) {
+      const ExpandParams x = a.items[q].x;
+      expand_tile<VT, FRAMED>(a.g, x, lt);
+    }
   }
 }
 

@@ -150,13 +150,14 @@ __device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams 
 
   // all rounds' table reads are issued up front
   constexpr uint32_t kWaveRows = TR / (kBlock / kWave);
+  const auto T = gmem(p.T);
   int32_t cells[kRounds], ends[kRounds];
 #pragma unroll
   for (int rd = 0; rd < kRounds; rd++) {
     const uint64_t row = base + wave * kWaveRows + rd * kWave + lane;
     const bool in = row < p.R && !p.key;
-    cells[rd] = in ? p.T[row * p.c + p.sc] : -1;
-    ends[rd] = in && p.mode == kModeK2K ? p.T[row * p.c + p.ec] : 0;
+    cells[rd] = in ? T[row * p.c + p.sc] : -1;
+    ends[rd] = in && p.mode == kModeK2K ? T[row * p.c + p.ec] : 0;
   }
   ProbeCarry carry;
   uint32_t probed = 0;
@@ -213,7 +214,7 @@ __device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams 
     uint64_t w = 1;
     if (p.weighted) {
       const uint64_t row = base + tid * kRowsPerThread + k;
-      w = row < p.R ? (uint32_t)p.T[row * p.c + p.wc] : 0u;
+      w = row < p.R ? (uint32_t)T[row * p.c + p.wc] : 0u;
     }
     ex += (uint64_t)s_aux[tid * kRowsPerThread + k] * w;
   }
@@ -226,15 +227,15 @@ __device__ __forceinline__ void probe_tile(const GraphView &g, const StepParams 
   for (int k = 0; k < kRowsPerThread; k++) {
     const uint32_t local = tid * kRowsPerThread + k;
     const uint32_t c = s_cnt[local];
-    if (c) hits[pos++] = make_uint4((uint32_t)(base + local), p.mode == kModeK2U ? s_aux[local] : 0u, c, 0u);
+    if (c) gstore(hits + pos++, make_uint4((uint32_t)(base + local), p.mode == kModeK2U ? s_aux[local] : 0u, c, 0u));
   }
   if (tid == 0) {
-    p.tile_sum[lt] = sum_total;
-    p.tile_nz[lt] = nz_total;
+    gmem(p.tile_sum)[lt] = sum_total;
+    gmem(p.tile_nz)[lt] = nz_total;
     uint32_t keys = 0;
 #pragma unroll
     for (int w = 0; w < kBlock / kWave; w++) keys += s_probed[w];
-    p.tile_ex[lt] = p.mode == kModeK2U ? (uint64_t)keys : ex_total;
+    gmem(p.tile_ex)[lt] = p.mode == kModeK2U ? (uint64_t)keys : ex_total;
   }
 }
 
@@ -371,23 +372,25 @@ template <int TR>
 __device__ __forceinline__ void finalize_tile(const StepParams &p, uint64_t lt) {
   const uint32_t lane = lane_id();
   auto nz_prefix = [&](uint64_t t) -> uint64_t {
-    if (t >= p.ntiles) return p.totals[2];
-    return p.tile_nz[t] + (p.chunk_nz ? p.chunk_nz[t / kScanChunk] : 0);
+    if (t >= p.ntiles) return gmem(p.totals)[2];
+    return gmem(p.tile_nz)[t] + (p.chunk_nz ? gmem(p.chunk_nz)[t / kScanChunk] : 0);
   };
   const uint64_t nz0 = nz_prefix(lt);
   const uint32_t nnz = (uint32_t)(nz_prefix(lt + 1) - nz0);
   if (nnz == 0) return;
-  const uint64_t out0 = p.tile_sum[lt] + (p.chunk_sum ? p.chunk_sum[lt / kScanChunk] : 0);
+  const uint64_t out0 = gmem(p.tile_sum)[lt] + (p.chunk_sum ? gmem(p.chunk_sum)[lt / kScanChunk] : 0);
   const uint4 *hits = p.hits + lt * TR;
+  const auto T = gmem(p.T);
+  const auto O = gmem(p.O);
   if (p.mode == kModeK2U) {
     uint64_t carry = 0;
     for (uint32_t k0 = 0; k0 < nnz; k0 += kWave) {
       const uint32_t j = k0 + lane;
-      const uint4 h = j < nnz ? hits[j] : make_uint4(0, 0, 0, 0);
+      const uint4 h = j < nnz ? gload(hits + j) : make_uint4(0, 0, 0, 0);
       const uint64_t inc = wave_incl_scan(h.z);
       if (j < nnz) {
-        p.cinfo[nz0 + j] = make_uint2(h.x, h.y);
-        p.cexcl[nz0 + j] = out0 + carry + inc - h.z;
+        gstore(p.cinfo + nz0 + j, make_uint2(h.x, h.y));
+        gmem(p.cexcl)[nz0 + j] = out0 + carry + inc - h.z;
       }
       carry += shfl64(inc, kWave - 1);
     }
@@ -396,18 +399,18 @@ __device__ __forceinline__ void finalize_tile(const StepParams &p, uint64_t lt) 
     const uint32_t c = p.c;
     for (uint32_t k0 = 0; k0 < nnz; k0 += kWave) {
       const uint32_t nsel = nnz - k0 < (uint32_t)kWave ? nnz - k0 : (uint32_t)kWave;
-      const uint32_t myrow = lane < nsel ? hits[k0 + lane].x : 0u;
+      const uint32_t myrow = lane < nsel ? gload(hits + k0 + lane).x : 0u;
       if (c <= (uint32_t)kWave) {  // 64 / c rows per pass, one column per lane
         const uint32_t rpp = kWave / c, col = lane % c, r0 = lane / c;
         for (uint32_t it = 0; it * rpp < nsel; it++) {
           const uint32_t r = it * rpp + r0;
           const uint32_t src = shfl32(myrow, r < (uint32_t)kWave ? r : 0u);
-          if (r0 < rpp && r < nsel) p.O[(out0 + k0 + r) * c + col] = p.T[(uint64_t)src * c + col];
+          if (r0 < rpp && r < nsel) O[(out0 + k0 + r) * c + col] = T[(uint64_t)src * c + col];
         }
       } else {
         for (uint32_t r = 0; r < nsel; r++) {
           const uint32_t src = shfl32(myrow, r);
-          for (uint32_t col = lane; col < c; col += kWave) p.O[(out0 + k0 + r) * c + col] = p.T[(uint64_t)src * c + col];
+          for (uint32_t col = lane; col < c; col += kWave) O[(out0 + k0 + r) * c + col] = T[(uint64_t)src * c + col];
         }
       }
     }
@@ -475,7 +478,7 @@ __device__ __forceinline__ void expand_tile(const GraphView &g, const ExpandPara
   const uint64_t d0 = tile * (uint64_t)NV;
   const uint64_t dend = a.R + a.E;
   const uint64_t d1 = d0 + NV < dend ? d0 + NV : dend;
-  const uint64_t i0 = a.part[tile], i1 = a.part[tile + 1];
+  const uint64_t i0 = gmem(a.part)[tile], i1 = gmem(a.part)[tile + 1];
   const uint64_t j0 = d0 - i0, j1 = d1 - i1;
   const uint32_t nr = (uint32_t)(i1 - i0 + 1);  // local row r <-> compacted row i0 - 1 + r
   for (uint32_t r = tid; r < nr; r += kBlock) {
@@ -485,8 +488,8 @@ __device__ __forceinline__ void expand_tile(const GraphView &g, const ExpandPara
       s_row[0] = 0;
     } else {
       const uint64_t i = i0 + r - 1;
-      const uint2 ci = a.cinfo[i];
-      s_ex[r] = (int32_t)((int64_t)a.cexcl[i] - (int64_t)j0);
+      const uint2 ci = gload(a.cinfo + i);
+      s_ex[r] = (int32_t)((int64_t)gmem(a.cexcl)[i] - (int64_t)j0);
       s_row[r] = ci.x;
       s_off[r] = ci.y;
     }
@@ -536,14 +539,15 @@ __device__ __forceinline__ void expand_tile(const GraphView &g, const ExpandPara
   const uint32_t nint = nout * nc;
   const uint64_t g0 = j0 * nc;
   auto cell = [&](uint32_t r, uint32_t col) -> int32_t {
-    return col < c ? a.T[(uint64_t)s_row[s_orow[r]] * c + col] : (int32_t)s_edge[r];
+    return col < c ? gmem(a.T)[(uint64_t)s_row[s_orow[r]] * c + col] : (int32_t)s_edge[r];
   };
   // streaming (nontemporal) stores: the table is far larger than the caches
   typedef int32_t v4i __attribute__((ext_vector_type(4)));
   const uint32_t head = (uint32_t)((4 - (g0 & 3)) & 3) < nint ? (uint32_t)((4 - (g0 & 3)) & 3) : nint;
-  if (tid < head) __builtin_nontemporal_store(cell(tid / nc, tid % nc), &a.O[g0 + tid]);
+  const auto O = gmem(a.O);
+  if (tid < head) __builtin_nontemporal_store(cell(tid / nc, tid % nc), &O[g0 + tid]);
   const uint32_t nquad = (nint - head) / 4;
-  v4i *oq = (v4i *)(a.O + g0 + head);
+  WKG_GLOBAL v4i *oq = (WKG_GLOBAL v4i *)(O + g0 + head);
   for (uint32_t q = tid; q < nquad; q += kBlock) {
     const uint32_t e = head + 4 * q;
     uint32_t r = e / nc, col = e - r * nc;
@@ -561,7 +565,7 @@ __device__ __forceinline__ void expand_tile(const GraphView &g, const ExpandPara
   const uint32_t tail0 = head + 4 * nquad;
   if (tid < nint - tail0) {
     const uint32_t e = tail0 + tid;
-    __builtin_nontemporal_store(cell(e / nc, e % nc), &a.O[g0 + e]);
+    __builtin_nontemporal_store(cell(e / nc, e % nc), &O[g0 + e]);
   }
   __syncthreads();  // LDS is reused by the caller's next tile
 }
