@@ -578,7 +578,7 @@ __global__ __launch_bounds__(kBlock) void k_x_scan_chunks(ExecArgs a) {
     __syncthreads();
     const uint32_t q = block_find(a.chunk_base, nq, u, &s_q);
     const uint32_t nch = *qs_field(a, q, offsetof(QState, nchunks));
-    const StepParams &p = a.items[q].p;
+    const StepParams p = a.items[q].p;
     scan_chunk(p, u - a.chunk_base[q], nch);
     if (nch > 1) {
       // thread 0 wrote this chunk's sums; the block that counts the query's

@@ -246,7 +246,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_probe(GraphView g, StepParams p) 
 }
 
 // In-place exclusive scan of a[0, n) by one block; returns the total.
-__device__ __forceinline__ uint64_t block_scan_array(uint64_t *a, uint64_t n, uint64_t *s_tmp) {
+__device__ __forceinline__ uint64_t block_scan_array(uint64_t *a_, uint64_t n, uint64_t *s_tmp) {
+  const auto a = gmem(a_);
   const uint64_t per = (n + kBlock - 1) / kBlock;
   const uint64_t lo = threadIdx.x * per < n ? threadIdx.x * per : n;
   const uint64_t hi = lo + per < n ? lo + per : n;
@@ -299,13 +300,14 @@ __device__ __forceinline__ void scan_chunk(const StepParams &p, uint64_t chunk, 
   __shared__ uint64_t s_tmp[kBlock / kWave];
   constexpr int PT = kScanChunk / kBlock;
   const uint64_t lo = chunk * (uint64_t)kScanChunk + threadIdx.x * PT;
+  const auto tile_sum = gmem(p.tile_sum), tile_nz = gmem(p.tile_nz);
   uint64_t vs[PT], vn[PT], s = 0, nzs = 0, e = 0;
 #pragma unroll
   for (int k = 0; k < PT; k++) {
     const bool in = lo + k < p.ntiles;
-    vs[k] = in ? p.tile_sum[lo + k] : 0;
-    vn[k] = in ? p.tile_nz[lo + k] : 0;
-    if (in) e += p.tile_ex[lo + k];
+    vs[k] = in ? tile_sum[lo + k] : 0;
+    vn[k] = in ? tile_nz[lo + k] : 0;
+    if (in) e += gmem(p.tile_ex)[lo + k];
     s += vs[k];
     nzs += vn[k];
   }
@@ -316,26 +318,26 @@ __device__ __forceinline__ void scan_chunk(const StepParams &p, uint64_t chunk, 
 #pragma unroll
   for (int k = 0; k < PT; k++)
     if (lo + k < p.ntiles) {
-      p.tile_sum[lo + k] = rs;
-      p.tile_nz[lo + k] = rn;
+      tile_sum[lo + k] = rs;
+      tile_nz[lo + k] = rn;
       rs += vs[k];
       rn += vn[k];
     }
   if (threadIdx.x == 0) {
     if (nchunks == 1) {
-      p.totals[0] = ts;
-      p.totals[1] = te;
-      p.totals[2] = tn;
-      p.chunk_sum[0] = 0;
-      p.chunk_nz[0] = 0;
+      gmem(p.totals)[0] = ts;
+      gmem(p.totals)[1] = te;
+      gmem(p.totals)[2] = tn;
+      gmem(p.chunk_sum)[0] = 0;
+      gmem(p.chunk_nz)[0] = 0;
       if (p.sg.h) {
         const uint64_t v[3] = {ts, te, tn};
         publish(p.sg, v, 3);
       }
     } else {
-      p.chunk_sum[chunk] = ts;
-      p.chunk_nz[chunk] = tn;
-      p.chunk_ex[chunk] = te;
+      gmem(p.chunk_sum)[chunk] = ts;
+      gmem(p.chunk_nz)[chunk] = tn;
+      gmem(p.chunk_ex)[chunk] = te;
     }
   }
 }
@@ -348,12 +350,12 @@ __device__ __forceinline__ void scan_top(const StepParams &p, uint64_t nchunks) 
   const uint64_t rows = block_scan_array(p.chunk_sum, nchunks, s_tmp);
   const uint64_t nz = block_scan_array(p.chunk_nz, nchunks, s_tmp);
   uint64_t e = 0, ex = 0;
-  for (uint64_t i = threadIdx.x; i < nchunks; i += kBlock) e += p.chunk_ex[i];
+  for (uint64_t i = threadIdx.x; i < nchunks; i += kBlock) e += gmem(p.chunk_ex)[i];
   (void)block_excl_scan(e, s_tmp, ex);
   if (threadIdx.x == 0) {
-    p.totals[0] = rows;
-    p.totals[1] = ex;
-    p.totals[2] = nz;
+    gmem(p.totals)[0] = rows;
+    gmem(p.totals)[1] = ex;
+    gmem(p.totals)[2] = nz;
     if (p.sg.h) {
       const uint64_t v[3] = {rows, ex, nz};
       publish(p.sg, v, 3);
