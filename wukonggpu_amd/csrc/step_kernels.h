@@ -1570,6 +1570,7 @@ struct FanoutBlocks {
   const uint32_t *par;    // compacted parents, grouped by list, row order within a list
   const uint64_t *ubase;  // per compacted parent: rank of its first unit of kFBRows rows
   const uint64_t *ln;     // per distinct list: its entries
+  const uint64_t *lbase;  // per distinct list: its first entry in D
   uint64_t nl;
   uint64_t *row0;         // per unit rank: its first row
   // kept rows, unordered: (row, unit rank, index within the unit, 0); a row
@@ -1592,6 +1593,7 @@ __global__ __launch_bounds__(kBlock) void k_fanout_blocks(GraphView g, StepParam
   __shared__ uint64_t s_rank[kFBGroup];
   __shared__ uint64_t s_bex[W][kWave], s_tmp[W];
   __shared__ uint32_t s_nb;
+  __shared__ unsigned long long s_hit;            // parents with a kept row in the block
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
   const uint64_t e = p.tile0 + blockIdx.x;
   uint64_t L = 0, hi = u.nl - 1;  // the list: the last with xb <= e
@@ -1605,8 +1607,20 @@ __global__ __launch_bounds__(kBlock) void k_fanout_blocks(GraphView g, StepParam
   const uint64_t n = u.ln[L], j0 = jb * kFBRows;
   const uint32_t nrows = (uint32_t)(n - j0 < (uint64_t)kFBRows ? n - j0 : (uint64_t)kFBRows);
   const uint32_t npg = (uint32_t)(cnt - grp * kFBGroup < (uint64_t)kFBGroup ? cnt - grp * kFBGroup : kFBGroup);
+  // the block's D entries, loaded now and used once the parents' targets
+  // (a chain of dependent loads) are in
+  const uint2 *Dp = f.D + u.lbase[L] + j0;
+  uint2 dv[kFBPer];
+#pragma unroll
+  for (int k = 0; k < kFBPer; k++) {
+    const uint32_t row = tid + k * kBlock;
+    dv[k] = row < nrows ? Dp[row] : make_uint2(0u, 0u);
+  }
   const DevMeta m = g.meta[p.pid < g.num_metas ? p.pid : 0];
-  if (tid == 0) s_nb = 0;
+  if (tid == 0) {
+    s_nb = 0;
+    s_hit = 0;
+  }
   for (uint32_t z = tid; z < kFBGroup * kFBPer * W / 2; z += kBlock) ((uint32_t *)&s_pc[0][0][0])[z] = 0;
   if (tid < kFBGroup) {
     uint32_t tg = kNoTarget, i = 0;
@@ -1625,14 +1639,11 @@ __global__ __launch_bounds__(kBlock) void k_fanout_blocks(GraphView g, StepParam
     s_kcnt[tid] = 0;
   }
   __syncthreads();
-  // the block's D entries (the list's D entries start at any of its parents' pbase)
-  const uint2 *Dp = f.D + f.pbase[s_par[0]] + j0;
   uint32_t c0[kFBPer], bi[kFBPer];  // single entry (or kNone); a longer list's index (or kNone)
   uint64_t ex1 = 0;  // single-entry rows: one edge examined each, for every parent
 #pragma unroll
   for (int k = 0; k < kFBPer; k++) {
-    const uint32_t row = tid + k * kBlock;
-    const uint2 d = row < nrows ? Dp[row] : make_uint2(0u, 0u);
+    const uint2 d = dv[k];
     c0[k] = d.y == 1 ? d.x : kNone;
     ex1 += d.y == 1 ? 1u : 0u;
     bi[k] = kNone;
@@ -1701,23 +1712,25 @@ __global__ __launch_bounds__(kBlock) void k_fanout_blocks(GraphView g, StepParam
     return m1;
   };
   // pass 1: every parent against every single-entry row: one compare per
-  // 64 rows, only noting parents with a match (rare)
+  // 64 rows, only noting parents with a match (rare) in a wave-uniform mask
+  uint64_t hit = 0;
   for (uint32_t pi = 0; pi < npg; pi++) {
     const uint32_t tgt = (uint32_t)__builtin_amdgcn_readlane((int)tgt_l, (int)pi);
     uint64_t any = 0;
 #pragma unroll
     for (int k = 0; k < kFBPer; k++) any |= __builtin_amdgcn_uicmp(c0[k], tgt, 32);
-    if (any && lane == 0) s_kcnt[pi] = 1;
+    hit |= (any ? 1ull : 0ull) << pi;
   }
+  if (hit && lane == 0) atomicOr(&s_hit, (unsigned long long)hit);
   // ... and the parents a longer list keeps
   for (uint32_t b = tid; b < nb; b += kBlock)
-    for (uint64_t msk = s_bmask[b]; msk; msk &= msk - 1) s_kcnt[__builtin_ctzll(msk)] = 1;
+    if (s_bmask[b]) atomicOr(&s_hit, (unsigned long long)s_bmask[b]);
   __syncthreads();
   // pass 2, parents with kept rows only: the ballots again, counted per
   // (k, wave), then each kept row listed with its index among its parent's
   // kept rows (row order: k, then wave, then lane)
-  for (uint32_t pi = 0; pi < npg; pi++) {
-    if (s_kcnt[pi] == 0) continue;  // block-uniform
+  for (uint64_t hm = s_hit; hm; hm &= hm - 1) {  // block-uniform
+    const uint32_t pi = (uint32_t)__builtin_ctzll(hm);
     const uint32_t tgt = (uint32_t)__builtin_amdgcn_readlane((int)tgt_l, (int)pi);
     uint64_t bw[kFBPer];
 #pragma unroll

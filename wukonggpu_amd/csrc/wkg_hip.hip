@@ -1009,7 +1009,7 @@ int wkg_ctx::step_filter_fanout(int mode, const wkg_table *in, int pid, int dir,
     return done(r);
   };
   uint64_t *nkept = (uint64_t *)(kept_rows + kept_cap);
-  FanoutBlocks fb{d_xb, d_poff, d_par, d_ubase, d_ln, nl, row0, kept_rows, nkept, kept_cap};
+  FanoutBlocks fb{d_xb, d_poff, d_par, d_ubase, d_ln, d_lbase, nl, row0, kept_rows, nkept, kept_cap};
   hipError_t le;
   uint64_t kept = 0, examined = 0;
   for (;;) {
