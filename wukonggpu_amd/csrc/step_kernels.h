@@ -1555,7 +1555,11 @@ __global__ __launch_bounds__(kBlock) void k_fanout_units(GraphView g, StepParams
 //    broadcast lane by lane, or a binary search per lane for a long sorted
 //    list), leaving a 64-bit parent mask per entry;
 //  - per parent, each wave's compares are ballots: 64 rows' kept flags at
-//    once (kept rows are rare: a ballot with a kept row is kept in LDS).
+//    once (kept rows are rare: pass 1 only notes the parents with one in a
+//    mask, and pass 2 redoes those parents' ballots);
+//  - a workgroup's time is its chain of dependent loads (list, parent slot,
+//    parent row, end cell), so the block's D entries are loaded first, from
+//    the list's base, and arrive while the parents' targets are fetched.
 // The output is per unit (parent x block of kFBRows rows, ranked
 // ubase[parent] + block) its kept rows and edges examined, for the tile
 // scan, and a list of the kept rows with their unit and index in it, for
